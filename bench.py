@@ -1,0 +1,175 @@
+"""Benchmark: path-contributions/s of the NEE speckle tracer on one MI355X.
+
+Workload (default): the 1024 x 1024 transmission-matrix render -- 128^3 voxel grid with four
+Perlin-thresholded materials, HG g = 0.9, 1024 plane-wave illuminations x 1024 plane-wave views,
+one frame, lambda = 1. A "step" is one batch of ``--batch`` paths (default 2^16) traced end to end:
+free flight, NEE to every illumination and view, and the complex contraction into the 1024 x 1024
+field. The timed region is ``--steps`` batches on the device with the medium and tables already
+resident in HBM; the value is paths * L * V * frames / wall time.
+
+The JSON line also carries:
+  roofline      the dominant kernel's achieved rate against its MI355X peak (HIP events on the
+                tracer's stream); the contraction's MFMA rate is reported beside it;
+  accuracy      relative Frobenius error of the mixed-precision field against the FP64 terms +
+                FP64 contraction on the same paths (one batch);
+  cpu_baseline  the FP64 CPU restatement (OpenMP) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def workload(name: str, batch: int, steps: int):
+    from temporalspecklescode_amd import workloads as W
+    if name == "tm1024":
+        cfg = W.c4(paths=batch * steps, batch=batch)
+        return cfg, 3
+    if name == "hetero64":
+        cfg = W.c2(paths=batch * steps, batch=batch)
+        return cfg, 3
+    if name == "slab2d":
+        cfg = W.c1(paths=batch * steps, batch=batch)
+        return cfg, 2
+    if name == "corr128":
+        cfg = W.c3(paths=batch * steps, batch=batch)
+        return cfg, 3
+    raise SystemExit("unknown workload %s" % name)
+
+
+def cpu_baseline(name: str, seconds: float, threads: int):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from temporalspecklescode_amd.config import sizes
+    n = 8
+    cfg, dims = workload(name, n, 1)
+    nl, nv, nt, _ = sizes(cfg, dims)
+    elapsed = 0.0
+    while True:
+        cfg, dims = workload(name, n, 1)
+        t0 = time.perf_counter()
+        O.render(cfg, dims, nthreads=threads)
+        elapsed = time.perf_counter() - t0
+        if elapsed > seconds / 4 or n >= (1 << 20):
+            break
+        n *= 4
+    rate = n / elapsed
+    return {"value": rate * nl * nv * nt, "unit": "path-contributions/s", "cores": threads, "kind": "port",
+            "sample": "%d paths of the same workload (seed and batch as the GPU run's first paths), %.1f s, "
+                      "FP64 scalar restatement, OpenMP" % (n, elapsed),
+            "paths_per_s": rate}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=1 << 16)
+    ap.add_argument("--workload", default="tm1024")
+    ap.add_argument("--contraction", default="bf16x3")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU sharding by path range is a design "
+              "only (DESIGN.md), so --gpus %d is not measured" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from temporalspecklescode_amd import Tracer
+    from temporalspecklescode_amd.config import sizes
+
+    cfg, dims = workload(args.workload, args.batch, args.steps)
+    cfg["precision"] = {"terms": "fp32", "contraction": args.contraction}
+    nl, nv, nt, _ = sizes(cfg, dims)
+    print("bench: workload %s, batch %d, steps %d, L %d V %d T %d" % (args.workload, args.batch, args.steps,
+                                                                     nl, nv, nt), file=sys.stderr, flush=True)
+    if args.warmup > 0:
+        wcfg = dict(cfg)
+        wcfg["simulation"] = dict(cfg["simulation"], fullIteration=args.warmup, seed=cfg["simulation"]["seed"] + 1000)
+        with Tracer(wcfg, dims) as tw:
+            tw.trace()
+        print("bench: warmup done", file=sys.stderr, flush=True)
+    with Tracer(cfg, dims) as tr:
+        t0 = time.perf_counter()
+        u, it = tr.trace()          # synchronous: returns after the device finished
+        wall = time.perf_counter() - t0
+        tm = tr.timing()
+    paths = args.batch * args.steps
+    contributions = paths * nl * nv * nt
+    value = contributions / wall
+    print("bench: timed %.3f s, stages %s" % (wall, json.dumps(tm)), file=sys.stderr, flush=True)
+
+    # roofline: the stage with the largest device time is the dominant kernel
+    nb = max(1, tm["batches"])
+    nee_ms = tm["nee_ms"] / nb
+    con_ms = (tm["contract_ms"] + tm["single_ms"]) / nb
+    vtx_per_path = tm["vertices"] / max(1, paths)
+    # NEE algorithmic bytes per path: four complex64 term rows written once (A, B, a, b) plus the
+    # vertex records read (20 B each, + 12 B per frame when scatterers move)
+    nee_bytes = args.batch * (16.0 * (nl + nv) * nt + 20.0 * vtx_per_path)
+    nee_gbs = nee_bytes / (nee_ms * 1e-3) / 1e9
+    flops = 2 * 8.0 * args.batch * nl * nv * nt   # multiple + single contraction, 8 flops each
+    con_tflops = flops / (con_ms * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("nee_bytes_per_launch")
+    roof_nee = {"kernel": "k_nee", "bound": "hbm", "achieved": round(nee_gbs, 3), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(nee_gbs / HBM_PEAK_GBS, 6), "traffic": traffic,
+                "avg_launch_ms": round(nee_ms, 4), "algorithmic_bytes_per_launch": nee_bytes,
+                "voxel_steps_per_s": tm["dda_steps"] / max(1e-9, tm["nee_ms"] * 1e-3)}
+    roof_con = {"kernel": "k_contract_bf16x3" if args.contraction == "bf16x3" else "k_contract_fma",
+                "bound": "mfma", "achieved": round(con_tflops, 3), "peak": MFMA_BF16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(con_tflops / MFMA_BF16_PEAK_TFLOPS, 6), "traffic": None,
+                "avg_launch_ms": round(con_ms, 4)}
+    dominant = roof_nee if nee_ms >= con_ms else roof_con
+
+    accuracy = None
+    if not args.no_accuracy:
+        acfg = dict(cfg)
+        acfg["simulation"] = dict(cfg["simulation"], fullIteration=1)
+        n = min(args.batch, 4096)
+        acfg["simulation"]["batchSize"] = n
+        with Tracer(acfg, dims) as tm32:
+            d32 = tm32.debug_batch(0, n)
+        acfg["precision"] = {"terms": "fp64", "contraction": "fp64"}
+        with Tracer(acfg, dims) as tm64:
+            d64 = tm64.debug_batch(0, n)
+        same = np.array_equal(d32["k"], d64["k"])
+        err = float(np.linalg.norm(d32["u"] - d64["u"]) / max(1e-300, np.linalg.norm(d64["u"])))
+        accuracy = {"rel_fro_err_vs_fp64": err, "paths": n, "same_vertices": bool(same)}
+
+    out = {
+        "metric": "path-contributions/sec (L×V complex) at err≤1e-3 vs FP64; p50/p99 render ms",
+        "value": value, "unit": "path-contributions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 transport+terms, fp64 phase, bf16x3 contraction" if args.contraction == "bf16x3" else "fp32",
+        "data": "synthetic (seeded Perlin medium, counter-based Philox paths)",
+        "config": {"workload": args.workload, "batch_paths": args.batch, "L": nl, "V": nv, "frames": nt,
+                   "paths": paths, "contraction": args.contraction},
+        "paths_per_s": paths / wall, "vertices_per_s": tm["vertices"] / wall,
+        "roofline": dominant, "roofline_contraction": roof_con, "roofline_nee": roof_nee,
+        "accuracy": accuracy, "stage_ms": {k: tm[k] for k in ("transport_ms", "nee_ms", "contract_ms", "single_ms")},
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        threads = max(1, min(threads, 16))
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

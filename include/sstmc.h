@@ -1,0 +1,259 @@
+/*
+ * sstmc.h -- C ABI of the MI355X-native next-event-estimation speckle tracer.
+ *
+ * This is the drop-in boundary for the reference's `sstmc2D(config)` / `sstmc3D(config)`
+ * MEX entry (MatlabInterface.cu:1565-1825) and for the C++ plugin split below it
+ * (Tracer / Sampler / Source / Medium / Scattering, TracerInterface.cuh:9-50,
+ * SamplerInterface.cuh:11-176, SourceInterface.cuh:10-179, MediumInterface.cuh:39-135,
+ * Scattering.cuh:32-125).  Every argument is a plain pointer or size; nothing here names a
+ * torch or HIP type, so a MEX shim, ctypes, cgo or JNI binding can call it directly.
+ *
+ * Conventions kept from the reference:
+ *   - the config mirrors `config.simulation/medium/scattering/illumination/view/sampler/tracer`
+ *     field by field (README.md:59-349, parsed at MatlabInterface.cu:274-1563);
+ *   - material 0 is "outside the grid"; grid ids are 1..materialNum-1 (MatlabInterface.cu:975-992);
+ *   - a single scattering cell is duplicated into material 1 (MatlabInterface.cu:750-760);
+ *   - output u is complex, laid out u(l, v, t) with l fastest (MatlabInterface.cu:1731-1753),
+ *     written here as interleaved (re, im) doubles; normalisation by sqrt(iter) (field) or
+ *     iter (correlation) stays with the caller (README.md:365-373);
+ *   - iter = batchSize * fullIteration + missed first-vertex samples (MatlabInterface.cu:1796);
+ *   - error codes mirror ErrorType (Simulation.cuh:111-162) and are returned, never thrown.
+ *
+ * Deliberate differences (each also stated where it is implemented):
+ *   - random numbers come from a counter-based Philox4x32-10 stream keyed by (seed, path id)
+ *     instead of per-thread XORWOW states (TracerInterface.cuh:56-63), so every path is
+ *     reproducible alone, in any batch and in any request mix;
+ *   - no process-global device state: any number of handles may be live at once.
+ */
+#ifndef SSTMC_H
+#define SSTMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- error codes */
+/* Mirrors ErrorType (Simulation.cuh:111-162); extra codes are appended after MISC_ERROR. */
+enum {
+    SSTMC_NO_ERROR = 0,
+    SSTMC_DEVICE_ERROR = 1,
+    SSTMC_ALLOCATION_ERROR = 2,
+    SSTMC_KERNEL_ERROR = 3,
+    SSTMC_BLAS_ERROR = 4,
+    SSTMC_NOT_SUPPORTED = 5,
+    SSTMC_MISC_ERROR = 6,
+    SSTMC_INVALID_CONFIG = 7,      /* config validation (MatlabInterface.cu mexErrMsgIdAndTxt sites) */
+    SSTMC_NO_HIT = 8,              /* first-vertex sampling exceeded its retry budget */
+    SSTMC_BUDGET_EXCEEDED = 9,     /* serving: request larger than the admission budget */
+    SSTMC_NOT_READY = 10           /* serving: request still in flight */
+};
+
+/* ---------------------------------------------------------------- enums */
+enum { SSTMC_MEDIUM_HETEROGENEOUS = 1 };                 /* config.medium.type */
+enum { SSTMC_SCATTERING_ISOTROPIC = 1,                    /* config.scattering.type{m} */
+       SSTMC_SCATTERING_HG = 3 };                         /* values follow ScatteringNS::ScatteringType */
+enum { SSTMC_SOURCE_NONE = 0,                             /* config.illumination/view.type */
+       SSTMC_SOURCE_POINT = 1,
+       SSTMC_SOURCE_FARFIELD = 2 };
+enum { SSTMC_SAMPLER_TEMPORAL = 1 };                      /* config.sampler.type */
+enum { SSTMC_TRACER_NEE = 1 };                            /* config.tracer.type */
+enum { SSTMC_OUTPUT_FIELD = 0,                            /* u(l,v,t): the field per frame */
+       SSTMC_OUTPUT_CORRELATION = 1 };                    /* C(l,v,d) = sum u(t=0) conj(u(t=d)), per-sub-path estimator */
+enum { SSTMC_TERMS_FP32 = 0,                              /* path terms (NEE attenuation, g, f) in FP32; */
+       SSTMC_TERMS_FP64 = 1 };                            /* path length and phase are FP64 in both */
+enum { SSTMC_CONTRACT_BF16X3 = 0,                         /* hi+lo split BF16 MFMA, FP32 accumulate */
+       SSTMC_CONTRACT_FP32 = 1,                           /* plain FP32 FMA tiles */
+       SSTMC_CONTRACT_FP64 = 2 };                         /* FP64 FMA tiles (accuracy reference) */
+
+/* ---------------------------------------------------------------- config */
+/* config.simulation (MatlabInterface.cu:1459-1563, 1590-1671). Zero means "reference default". */
+typedef struct sstmc_simulation {
+    uint32_t batchSize;        /* P, default 1024 (MatlabInterface.cu:1467) */
+    uint32_t fullIteration;    /* default 1 */
+    uint32_t renderingsNum;    /* default 1; rendering i uses seed + i (MatlabInterface.cu:1799) */
+    uint32_t deviceNumber;     /* HIP device ordinal */
+    uint64_t seed;             /* config.simulation.seed (MatlabInterface.cu:1649) */
+    double lambda;             /* default 1.0; one wavelength (Nw = 1) */
+} sstmc_simulation;
+
+/* config.medium, heterogeneous rectilinear grid (HeterogeneousMedium.cuh:107-233). */
+typedef struct sstmc_medium {
+    int32_t type;                  /* SSTMC_MEDIUM_HETEROGENEOUS */
+    uint32_t xAxisSize, yAxisSize, zAxisSize;   /* number of bin edges; zAxisSize ignored in 2D */
+    const double* xAxis;
+    const double* yAxis;
+    const double* zAxis;
+    const uint32_t* materialGrid;  /* (nx-1)(ny-1)(nz-1) ids, x fastest */
+    uint32_t materialNum;          /* entries of sigs/siga, including material 0 */
+    const double* sigs;
+    const double* siga;            /* may be NULL: all zero (MatlabInterface.cu:1019-1022) */
+} sstmc_medium;
+
+/* config.scattering (MatlabInterface.cu:712-888): one entry per material. */
+typedef struct sstmc_scattering {
+    uint32_t materialNum;
+    const int32_t* type;           /* SSTMC_SCATTERING_ISOTROPIC or SSTMC_SCATTERING_HG */
+    const double* g;               /* HG anisotropy per material (ignored for isotropic) */
+} sstmc_scattering;
+
+/* config.illumination / config.view (MatlabInterface.cu:1062-1215). */
+typedef struct sstmc_source {
+    int32_t type;                  /* SSTMC_SOURCE_FARFIELD or SSTMC_SOURCE_POINT */
+    uint32_t num;                  /* number of directions / locations */
+    const double* x;               /* direction.x or location.x */
+    const double* y;
+    const double* z;               /* ignored in 2D */
+    int32_t isAreaSource;          /* PointSource only (PointSource.cuh:140-147) */
+} sstmc_source;
+
+/* config.sampler, TemporalSampler (MatlabInterface.cu:1217-1312). */
+typedef struct sstmc_sampler {
+    int32_t type;                  /* SSTMC_SAMPLER_TEMPORAL */
+    uint32_t tNum;                 /* frames Nt */
+    const double* t;               /* frame times */
+    const double* D;               /* Brownian coefficient per material */
+    const double* Ux;              /* linear velocity per material */
+    const double* Uy;
+    const double* Uz;
+    int32_t isTemporalAmplitude;   /* default 1: g/f use displaced positions */
+} sstmc_sampler;
+
+/* config.tracer (MatlabInterface.cu:1374-1457). */
+typedef struct sstmc_tracer {
+    int32_t type;                  /* SSTMC_TRACER_NEE */
+    int32_t isCBS;                 /* must be 0 (coherent backscattering not built) */
+    int32_t output;                /* SSTMC_OUTPUT_FIELD or SSTMC_OUTPUT_CORRELATION */
+} sstmc_tracer;
+
+/* Precision / performance policy of this implementation (no counterpart in the reference,
+ * which is FP64 throughout, Simulation.cuh:22-24). Transport is always FP32. */
+typedef struct sstmc_precision {
+    int32_t terms;                 /* SSTMC_TERMS_FP32 (default) or SSTMC_TERMS_FP64 */
+    int32_t contraction;           /* SSTMC_CONTRACT_BF16X3 (default), _FP32 or _FP64 */
+} sstmc_precision;
+
+typedef struct sstmc_config {
+    uint32_t dims;                 /* 2 -> sstmc2D, 3 -> sstmc3D */
+    sstmc_simulation simulation;
+    sstmc_medium medium;
+    sstmc_scattering scattering;
+    sstmc_source illumination;
+    sstmc_source view;
+    sstmc_sampler sampler;
+    sstmc_tracer tracer;
+    sstmc_precision precision;
+} sstmc_config;
+
+typedef struct sstmc_handle sstmc_handle;
+
+/* ---------------------------------------------------------------- tracer handle */
+/* Builds Simulation + Scattering + Medium + Sources + Sampler + Tracer, as mexFunction does
+ * (MatlabInterface.cu:1671-1700). Replaces the constructors' `ErrorType* err` out-parameters. */
+int sstmc_create(const sstmc_config* cfg, sstmc_handle** out);
+
+/* Tracer::trace (TracerInterface.cuh:21, NextEventEstimation.cuh:1461-1637): one rendering of
+ * fullIteration batches. u: 2*Nl*Nv*Nt doubles (host), iter: one double (may be NULL). */
+int sstmc_trace(sstmc_handle* h, double* u, double* iter);
+
+/* Tracer::updateSeed (TracerInterface.cuh:136-152). */
+int sstmc_update_seed(sstmc_handle* h, uint64_t seed);
+
+/* Tracer::getZeroContributionPaths / getPhotonsBudget (TracerInterface.cuh:23-24) of the
+ * last trace: missed first-vertex samples, and the number of scattering vertices traced. */
+uint64_t sstmc_zero_contribution_paths(const sstmc_handle* h);
+uint64_t sstmc_photons_budget(const sstmc_handle* h);
+
+/* Sizes of the handle's problem: Nl, Nv, Nt (sampler->getIllumination/View/SamplerSize). */
+int sstmc_sizes(const sstmc_handle* h, uint32_t* nl, uint32_t* nv, uint32_t* nt);
+
+void sstmc_destroy(sstmc_handle* h);
+
+/* getErrorMsg (MatlabInterface.cu:24-272). Static string, never NULL. */
+const char* sstmc_error_string(int code);
+
+/* The whole MEX call: `[u, iter] = sstmc{2,3}D(config)` (MatlabInterface.cu:1565-1825).
+ * u: 2*Nl*Nv*Nt*renderingsNum doubles, iter: renderingsNum doubles. */
+int sstmc_render(const sstmc_config* cfg, double* u, double* iter);
+
+/* ---------------------------------------------------------------- stage timing */
+/* Device time of each stage of the last trace, from HIP events on the tracer's stream, in ms.
+ * Mirrors the TIME_REC TimeTracer report (NextEventEstimation.cuh:107-156). */
+typedef struct sstmc_timing {
+    double transport_ms;           /* free flight + direction sampling + frames (a) */
+    double nee_ms;                 /* NEE attenuation, g, f, probabilities, phase (b) */
+    double contract_ms;            /* complex contraction (c) */
+    double single_ms;              /* single-scattering contraction / fused kernel */
+    double total_ms;               /* first event to last event */
+    uint64_t vertices;             /* scattering vertices traced */
+    uint64_t dda_steps;            /* voxel steps walked by the NEE attenuation */
+    uint32_t batches;
+} sstmc_timing;
+int sstmc_last_timing(const sstmc_handle* h, sstmc_timing* out);
+
+/* ---------------------------------------------------------------- parity / inspection */
+/* Traces one batch of P = npaths paths with global path ids [path_begin, path_begin+npaths)
+ * and returns what the contraction consumes, for comparison with a CPU transcription.
+ * Sizes are returned first by calling with all output pointers NULL (nvtx_out filled).
+ *  path_k[P]        scattering vertices per path (x1..xK)
+ *  path_misses[P]   missed first-vertex attempts
+ *  path_off[P+1]    vertex offsets
+ *  vtx_pos[nvtx*3]  central positions (FP32 transport)
+ *  vtx_mat[nvtx]    material ids
+ *  vtx_rho[nvtx]    random phase of the vertex, in [0,1)
+ *  vtx_dxt[nvtx*Nt*3] displacement of the scatterer per frame (frame position = pos + dxt,
+ *                     exact in FP64)
+ *  vtx_len[nvtx*Nt]   path length from x1 per frame, FP64 (as accumulated by the NEE kernel)
+ *  term_a[P*Nt*Nl*2]  multiple-scattering illumination terms A_p[l,t] (interleaved complex)
+ *  term_b[P*Nt*Nv*2]  multiple-scattering view terms B_p[v,t]
+ *  term_sa[P*Nt*Nl*2] single-scattering illumination factor g_l(x1) w1 exp(i 2 pi rho_1)
+ *  term_sb[P*Nt*Nv*2] single-scattering view factor g_v(x1)
+ *  u_batch[2*Nl*Nv*Nt] the batch's contribution to u (single + multiple, this handle's contraction)
+ * Term layouts are [p][t][l] and [p][t][v]; any output pointer may be NULL. */
+int sstmc_debug_batch(sstmc_handle* h, uint64_t path_begin, uint32_t npaths,
+                      uint64_t* nvtx_out,
+                      uint32_t* path_k, uint32_t* path_misses, uint64_t* path_off,
+                      float* vtx_pos, uint32_t* vtx_mat, float* vtx_rho,
+                      float* vtx_dxt, double* vtx_len,
+                      double* term_a, double* term_b, double* term_sa, double* term_sb,
+                      double* u_batch);
+
+/* Contracts caller-supplied terms on the device with the handle's contraction precision:
+ * u[t][v][l] += sum_p a[p][t][l] * b[p][t][v] (complex, interleaved doubles in and out).
+ * Exposes kernel (c) alone for accuracy tests against an FP64 GEMM. */
+int sstmc_debug_contract(sstmc_handle* h, uint32_t npaths, const double* a, const double* b,
+                         double* u);
+
+/* ---------------------------------------------------------------- serving */
+/* Device-resident request loop with continuous batching (no counterpart in the reference,
+ * whose `__constant__` globals allow one live simulation per process, Simulation.cuh:168-173).
+ * Requests are admitted by HBM footprint; results stay on the device until fetched. */
+typedef struct sstmc_server sstmc_server;
+typedef struct sstmc_server_stats {
+    uint64_t completed;
+    double p50_ms, p99_ms, mean_ms;     /* submit -> completion latency */
+    double paths_per_s;                 /* completed paths / busy wall time */
+    double contributions_per_s;         /* completed paths*L*V*T / busy wall time */
+    uint64_t hbm_in_use, hbm_budget;
+} sstmc_server_stats;
+
+int sstmc_server_create(uint32_t device, uint64_t hbm_budget_bytes, uint32_t max_inflight,
+                        sstmc_server** out);
+/* Queues one rendering of cfg (fullIteration batches); returns a request id. */
+int sstmc_submit(sstmc_server* s, const sstmc_config* cfg, uint64_t* request_id);
+/* Advances the loop without blocking; returns SSTMC_NO_ERROR when the request is done,
+ * SSTMC_NOT_READY otherwise. */
+int sstmc_poll(sstmc_server* s, uint64_t request_id);
+/* Blocks until the request is done. */
+int sstmc_wait(sstmc_server* s, uint64_t request_id);
+/* Copies the finished result (2*Nl*Nv*Nt doubles, iter) and releases its device memory. */
+int sstmc_fetch(sstmc_server* s, uint64_t request_id, double* u, double* iter);
+int sstmc_server_stats_get(sstmc_server* s, sstmc_server_stats* out);
+void sstmc_server_destroy(sstmc_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SSTMC_H */

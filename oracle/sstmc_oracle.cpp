@@ -1,0 +1,1027 @@
+// sstmc_oracle.cpp -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+//
+// Scalar FP64 restatement of the reference's next-event-estimation field tracer
+// (TemporalSampler + NextEventEstimation, non-CBS, one wavelength), used only by tests/,
+// __graft_entry__.smoke() and the cpu_baseline leg of bench.py as the checker and the CPU
+// baseline. It shares nothing with the HIP product except the config struct declared in
+// include/sstmc.h (the boundary format).
+//
+// Random numbers: the reference draws from per-thread curand XORWOW states whose streams
+// depend on compaction order (TracerInterface.cuh:56-63, quirk Q2), so its per-path outputs
+// are not reproducible. This restatement and the GPU path both use the counter-based stream
+// spec below instead; everything else follows the reference files cited per function.
+//
+//   Philox4x32-10 (Salmon et al., SC'11): key = (seed lo, seed hi),
+//   counter = (path lo, path hi, event, purpose<<24 | block), 4 words per block, words used
+//   in order.  event = a for the a-th attempt at the first vertex, 0x80000000|k for the
+//   sampling of vertex k >= 2.  purpose 0 = transport, 1 = frame displacements (slot s reads
+//   block s: normal u1,u2 then direction words), 2 = random phase (block 0 word 0).
+//   uniform(w) = ((w>>8)+0.5)*2^-24 in (0,1);  index(w,N) = ((2*(w>>8)+1)*N) >> 25
+//   (= ceil(u*N)-1 of Simulation.cuh:306-309 for that u).
+//
+// Parity status of each part is recorded in the test files (tests/test_oracle_*.py).
+
+#include "../include/sstmc.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+typedef std::complex<double> cplx;
+const double PI = 3.14159265358979323846;
+const double EPS = 1e-12;  // EPSILON for PRECISION==DOUBLE (Simulation.cuh:36-40)
+
+// ------------------------------------------------------------------ Philox4x32-10
+struct Philox {
+    static void block(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+        uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+        uint32_t k0 = key_in[0], k1 = key_in[1];
+        for (int r = 0; r < 10; ++r) {
+            uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+            uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+            uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+            uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+            uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+            c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+            k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+        }
+        out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+    }
+};
+
+struct Stream {
+    uint32_t key[2];
+    uint32_t ctr[4];
+    uint32_t buf[4];
+    int pos;
+    Stream(uint64_t seed, uint64_t path, uint32_t event, uint32_t purpose) {
+        key[0] = (uint32_t)seed; key[1] = (uint32_t)(seed >> 32);
+        ctr[0] = (uint32_t)path; ctr[1] = (uint32_t)(path >> 32); ctr[2] = event;
+        ctr[3] = purpose << 24;
+        pos = 4;
+    }
+    void seek_block(uint32_t b) { ctr[3] = (ctr[3] & 0xff000000u) | b; pos = 4; }
+    uint32_t word() {
+        if (pos == 4) { Philox::block(ctr, key, buf); ctr[3]++; pos = 0; }
+        return buf[pos++];
+    }
+    double uniform() { return ((double)(word() >> 8) + 0.5) * (1.0 / 16777216.0); }
+    uint32_t index(uint32_t n) { return (uint32_t)(((2ull * (word() >> 8) + 1ull) * n) >> 25); }
+};
+
+// ------------------------------------------------------------------ small vectors
+struct V3 {
+    double c[3];
+    V3() { c[0] = c[1] = c[2] = 0; }
+    V3(double x, double y, double z) { c[0] = x; c[1] = y; c[2] = z; }
+    double& operator[](int i) { return c[i]; }
+    double operator[](int i) const { return c[i]; }
+};
+inline V3 add(const V3& a, const V3& b) { return V3(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
+inline V3 sub(const V3& a, const V3& b) { return V3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+inline V3 scale(double s, const V3& a) { return V3(s * a[0], s * a[1], s * a[2]); }
+
+struct Problem;
+
+// VectorType.cuh:95-165 device semantics: dot, norm, normalize (zero vector -> +z / +y).
+struct Geo {
+    int dims;
+    double dot(const V3& a, const V3& b) const {
+        return dims == 3 ? a[0] * b[0] + a[1] * b[1] + a[2] * b[2] : a[0] * b[0] + a[1] * b[1];
+    }
+    double norm(const V3& a) const { return std::sqrt(dot(a, a)); }
+    V3 normalize(const V3& a) const {
+        double n = norm(a);
+        if (n == 0) return dims == 3 ? V3(0, 0, 1) : V3(0, 1, 0);
+        return scale(1.0 / n, a);
+    }
+};
+
+// ------------------------------------------------------------------ medium
+// HeterogeneousMedium.cuh + MediumInterface.cuh.
+struct Medium {
+    int dims;
+    std::vector<double> ax[3];
+    std::vector<uint32_t> grid;
+    std::vector<double> sigs, siga;
+    V3 bmin, bmax;
+    double maxSigt;
+    uint32_t nbin(int a) const { return (uint32_t)ax[a].size() - 1; }
+
+    // MediumNS::isInsideKernel (MediumInterface.cuh:204-213): closed box.
+    bool inside(const V3& p) const {
+        for (int a = 0; a < dims; ++a)
+            if (!(p[a] >= bmin[a] && p[a] <= bmax[a])) return false;
+        return true;
+    }
+    // binarySearchKernel (Simulation.cuh:325-367), clamped to a valid cell: the reference
+    // returns the edge count minus one for a point exactly on the max face (quirk Q13).
+    uint32_t bin(int a, double x) const {
+        const std::vector<double>& e = ax[a];
+        uint32_t n = (uint32_t)e.size();
+        if (e[0] > x) return 0;
+        if (e[n - 1] <= x) return n - 2;
+        uint32_t lo = 0, hi = n - 1;  // e[lo] <= x < e[hi]
+        while (hi - lo > 1) {
+            uint32_t m = (lo + hi) / 2;
+            if (e[m] <= x) lo = m; else hi = m;
+        }
+        return lo;
+    }
+    uint32_t cell(const uint32_t* b) const {
+        uint32_t idx = b[0] + b[1] * nbin(0);
+        if (dims == 3) idx += b[2] * nbin(0) * nbin(1);
+        return idx;
+    }
+    uint32_t material(const V3& p) const {
+        uint32_t b[3] = {bin(0, p[0]), bin(1, p[1]), dims == 3 ? bin(2, p[2]) : 0u};
+        return grid[cell(b)];
+    }
+    double sigt(uint32_t m) const { return sigs[m] + siga[m]; }
+
+    // bdDistance (HeterogeneousMedium.cuh:269-383): does the ray from o along d hit the box.
+    bool pointsToward(const V3& o_in, const V3& d) const {
+        V3 r, o;
+        for (int a = 0; a < 3; ++a) {
+            r[a] = 0.5 * (bmax[a] - bmin[a]);
+            o[a] = o_in[a] - 0.5 * (bmax[a] + bmin[a]);
+        }
+        double invR[3], invD[3];
+        for (int a = 0; a < dims; ++a) {
+            invR[a] = std::fabs(r[a]) < EPS ? 1e12 : 1.0 / r[a];
+            invD[a] = std::fabs(d[a]) < EPS ? 1e12 : 1.0 / d[a];
+        }
+        double mx = -1e300;
+        for (int a = 0; a < dims; ++a) mx = std::max(mx, std::fabs(o[a]) * invR[a]);
+        int winding = mx <= 1.0 ? -1 : 1;
+        for (int a = 0; a < dims; ++a) {
+            int sgn = d[a] < 0 ? 1 : -1;
+            double dist = invD[a] * (r[a] * (winding * sgn) - o[a]);
+            bool ok = dist >= 0.0;
+            for (int b = 0; b < dims && ok; ++b)
+                if (b != a) ok = std::fabs(std::fma(d[b], dist, o[b])) < r[b];
+            if (ok) return true;
+        }
+        return false;
+    }
+
+    // heterogeneousSampleKernel (HeterogeneousMedium.cuh:576-678): Woodcock tracking with the
+    // global majorant (setMaterial, :820-860), absorption as termination. Returns material (0 =
+    // escaped or absorbed) and writes the sampled point.
+    int woodcock(const V3& o, const V3& d, Stream& s, V3* out) const {
+        double dist = 0.0;
+        V3 x;
+        uint32_t m = 0;
+        for (;;) {
+            dist += -std::log(s.uniform()) / maxSigt;
+            x = add(o, scale(dist, d));
+            if (!inside(x)) {
+                if (!pointsToward(x, d)) break;
+            } else {
+                m = material(x);
+                if (s.uniform() <= sigt(m) / maxSigt) break;
+            }
+        }
+        *out = x;
+        if (!inside(x)) return 0;
+        if (s.uniform() > sigs[m] / sigt(m)) return 0;
+        return (int)m;
+    }
+
+    // heterogeneousAttenuationKernel (HeterogeneousMedium.cuh:386-573): exact voxel walk from
+    // p along d to the box boundary; returns the optical depth. Outside -> 0.
+    double opticalDepth(const V3& p_in, const V3& d, uint64_t* steps) const {
+        if (!inside(p_in)) return 0.0;
+        V3 p = p_in;
+        double invD[3];
+        int dirsgn[3];
+        for (int a = 0; a < dims; ++a) {
+            invD[a] = std::fabs(d[a]) < EPS ? (d[a] >= 0 ? 1e12 : -1e12) : 1.0 / d[a];
+            dirsgn[a] = d[a] < 0 ? -1 : 1;  // winding * sgn with winding = -1
+        }
+        uint32_t b[3] = {bin(0, p[0]), bin(1, p[1]), dims == 3 ? bin(2, p[2]) : 0u};
+        double st = sigt(grid[cell(b)]);
+        double tau = 0.0;
+        for (;;) {
+            if (steps) ++*steps;
+            double dist[3];
+            for (int a = 0; a < dims; ++a) {
+                double lo = ax[a][b[a]], hi = ax[a][b[a] + 1];
+                double r = 0.5 * (hi - lo), o = p[a] - 0.5 * (hi + lo);
+                dist[a] = invD[a] * (r * dirsgn[a] - o);
+            }
+            int ax_hit;
+            if (dims == 2) ax_hit = dist[0] < dist[1] ? 0 : 1;
+            else if (dist[0] < dist[1] && dist[0] < dist[2]) ax_hit = 0;
+            else if (dist[1] < dist[2]) ax_hit = 1;
+            else ax_hit = 2;
+            double bd = dist[ax_hit];
+            tau += bd * st;
+            int hit = dirsgn[ax_hit];
+            if ((b[ax_hit] == 0 && hit == -1) || (b[ax_hit] == nbin(ax_hit) - 1 && hit == 1)) break;
+            p = add(p, scale(bd, d));
+            b[ax_hit] = (uint32_t)((int)b[ax_hit] + hit);
+            st = sigt(grid[cell(b)]);
+        }
+        return tau;
+    }
+};
+
+// ------------------------------------------------------------------ scattering
+// Scattering.cuh: isotropic and Henyey-Greenstein (setup :500-572, eval :895-1110).
+struct Scatter {
+    int dims;
+    std::vector<int> type;  // per material
+    std::vector<double> g;
+    bool allIsotropic;      // Scattering::isIsotropicScattering() (Scattering.cuh:1505-1527)
+
+    double isoAmp() const { return dims == 3 ? 0.282094791773878143 : 0.398942280401432678; }
+    double isoPdf() const { return dims == 3 ? 0.079577471545947668 : 0.159154943091895336; }
+    double pdf(uint32_t m, double c) const {
+        if (type[m] == SSTMC_SCATTERING_ISOTROPIC) return isoPdf();
+        double gg = g[m];
+        double up = dims == 3 ? (1.0 - gg * gg) / (4.0 * PI) : 2.0 * ((1.0 - gg * gg) / (4.0 * PI));
+        double den = std::fma(-2.0 * gg, c, 1.0 + gg * gg);
+        return dims == 3 ? up / (den * std::sqrt(den)) : up / den;
+    }
+    double amp(uint32_t m, double c) const {
+        if (type[m] == SSTMC_SCATTERING_ISOTROPIC) return isoAmp();
+        return std::sqrt(pdf(m, c));
+    }
+    // randomDirection (VectorType.cuh:316-362).
+    V3 randomDirection(Stream& s) const {
+        if (dims == 2) {
+            double th = 2.0 * s.uniform();
+            return V3(std::sin(PI * th), std::cos(PI * th), 0);
+        }
+        double ph = 2.0 * s.uniform();
+        double sp = std::sin(PI * ph), cp = std::cos(PI * ph);
+        double ct = std::fma(s.uniform(), 2.0, -1.0);
+        double st = std::sqrt(1.0 - ct * ct);
+        return V3(st * cp, st * sp, ct);
+    }
+    // sampleDeviceImplementation (Scattering.cuh:1091-1109) with sampleCosine (:1009-1063)
+    // and vector_rotation (VectorType.cuh:261-282).
+    V3 sample(uint32_t m, const V3& prev, Stream& s) const {
+        if (type[m] == SSTMC_SCATTERING_ISOTROPIC) return randomDirection(s);
+        double gg = g[m], c;
+        if (std::fabs(gg) < 0.001) {
+            c = s.uniform() * 2.0 - 1.0;
+        } else {
+            double u = s.uniform();
+            if (dims == 2) {
+                c = std::cos(2.0 * std::atan(((1.0 - gg) / (1.0 + gg)) * std::tan((PI / 2.0) * (1 - 2.0 * u))));
+            } else {
+                double sv = (1.0 - gg * gg) / (1.0 - gg + 2.0 * gg * u);
+                c = (1.0 + gg * gg - sv * sv) / (2.0 * gg);
+            }
+        }
+        double sa = (s.uniform() < 0.5 ? -1.0 : 1.0) * std::sqrt(1.0 - c * c);
+        if (dims == 2)
+            return V3(prev[0] * c - prev[1] * sa, prev[0] * sa + prev[1] * c, 0);
+        double ph = 2.0 * s.uniform();
+        double sp = std::sin(PI * ph), cp = std::cos(PI * ph);
+        if (std::fabs(prev[2]) > 1.0 - EPS)
+            return V3(sa * cp, sa * sp, c * (std::signbit(prev[2]) ? -1.0 : 1.0));
+        double stw = std::sqrt(1.0 - prev[2] * prev[2]);
+        return V3(sa * (prev[0] * prev[2] * cp - prev[1] * sp) / stw + prev[0] * c,
+                  sa * (prev[1] * prev[2] * cp + prev[0] * sp) / stw + prev[1] * c,
+                  -sa * cp * stw + prev[2] * c);
+    }
+};
+
+// ------------------------------------------------------------------ sources
+// FarField.cuh (:71-263, :439-599) and PointSource.cuh (:72-172, :236-446).
+struct Source {
+    int type = SSTMC_SOURCE_NONE;
+    bool illum = true;
+    bool area = false;
+    std::vector<V3> pt;        // FarField: normalised directions d; PointSource: locations
+    // FarField sampling tables (fillBuffers, FarField.cuh:71-127)
+    std::vector<V3> invDir, tangent, perpT, perpS;
+    std::vector<double> minT, maxT, minS, maxS, prob;
+    uint32_t n() const { return (uint32_t)pt.size(); }
+};
+
+struct Frames {
+    uint32_t nt = 1;
+    uint32_t neg = 0;
+    std::vector<double> dt;    // per chain slot (TemporalSampler.cuh:1093-1132)
+    std::vector<double> D;     // per material
+    std::vector<V3> U;
+    bool temporalAmp = true;
+};
+
+struct Vertex {
+    V3 x;                      // central position
+    V3 dirIn;                  // free-flight direction that reached x
+    uint32_t m;
+    double rho;
+    std::vector<V3> xt;        // per output frame
+    std::vector<double> len;   // path length from x1 per frame
+};
+
+struct Problem {
+    int dims;
+    Geo geo;
+    Medium med;
+    Scatter sc;
+    Source il, vw;
+    Frames fr;
+    double lambda;
+    uint32_t P, fullIter;
+    uint64_t seed;
+    int output;                // SSTMC_OUTPUT_FIELD / _CORRELATION
+    uint32_t maxAttempts = 1u << 20;
+
+    uint32_t L() const { return il.n(); }
+    uint32_t V() const { return vw.n(); }
+    uint32_t T() const { return fr.nt; }
+
+    // projectBoxToLine (MediumInterface.cuh:156-199) with pointToLineProjectionDistance (:137-153).
+    void projectBox(const V3& p0, const V3& v, double* tmin, double* tmax) const {
+        int nc = dims == 3 ? 8 : 4;
+        double lo = 0, hi = 0;
+        for (int c = 0; c < nc; ++c) {
+            V3 p((c & 1) ? med.bmax[0] : med.bmin[0], (c & 2) ? med.bmax[1] : med.bmin[1],
+                 (c & 4) ? med.bmax[2] : med.bmin[2]);
+            V3 q = sub(p, p0);
+            V3 o;
+            for (int a = 0; a < dims; ++a) {
+                double acc = 0;
+                for (int b = 0; b < dims; ++b) acc += v[a] * v[b] * q[b];
+                o[a] = acc;
+            }
+            double t = geo.dot(o, v);
+            if (c == 0 || t < lo) lo = t;
+            if (c == 0 || t > hi) hi = t;
+        }
+        *tmin = lo; *tmax = hi;
+    }
+
+    // FarFieldNS::fillBuffers (FarField.cuh:71-127).
+    void buildFarField(Source& s) {
+        V3 r, ctr;
+        double rmax = -1e300;
+        for (int a = 0; a < dims; ++a) {
+            r[a] = 0.5 * (med.bmax[a] - med.bmin[a]);
+            ctr[a] = 0.5 * (med.bmax[a] + med.bmin[a]);
+            rmax = std::max(rmax, r[a]);
+        }
+        double rad = rmax + EPS;
+        uint32_t n = s.n();
+        s.invDir.resize(n); s.tangent.resize(n); s.perpT.resize(n); s.perpS.resize(n);
+        s.minT.resize(n); s.maxT.resize(n); s.minS.resize(n); s.maxS.resize(n); s.prob.resize(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            V3 d = s.pt[i];
+            V3 opp = scale(-1.0, d);
+            for (int a = 0; a < dims; ++a) s.invDir[i][a] = std::fabs(d[a]) < EPS ? 1e12 : 1.0 / d[a];
+            s.tangent[i] = add(ctr, scale(rad, opp));
+            if (dims == 3) {
+                V3 b0, b1;
+                if (std::fabs(1.0 - std::fabs(opp[2])) <= EPS) {
+                    b0 = V3(1, 0, 0); b1 = V3(0, 1, 0);
+                } else {
+                    b0 = geo.normalize(V3(-opp[1], opp[0], 0));
+                    b1 = V3(b0[1] * opp[2] - b0[2] * opp[1], b0[2] * opp[0] - b0[0] * opp[2],
+                            b0[0] * opp[1] - b0[1] * opp[0]);
+                }
+                s.perpT[i] = b0; s.perpS[i] = b1;
+                projectBox(s.tangent[i], b0, &s.minT[i], &s.maxT[i]);
+                projectBox(s.tangent[i], b1, &s.minS[i], &s.maxS[i]);
+                s.prob[i] = 1.0 / ((s.maxT[i] - s.minT[i]) * (s.maxS[i] - s.minS[i]));
+            } else {
+                s.perpT[i] = V3(-opp[1], opp[0], 0);
+                projectBox(s.tangent[i], s.perpT[i], &s.minT[i], &s.maxT[i]);
+                s.prob[i] = 1.0 / (s.maxT[i] - s.minT[i]);
+            }
+        }
+    }
+
+    // sampleFarFieldKernel (FarField.cuh:129-205) / PointSourceNS::sampleFirstPointKernel
+    // (PointSource.cuh:72-90): source point x0 and direction of the first flight.
+    void sampleSource(Stream& s, V3* x0, V3* dir) const {
+        uint32_t l = s.index(L());
+        if (il.type == SSTMC_SOURCE_POINT) {
+            *x0 = il.pt[l];
+            *dir = sc.randomDirection(s);
+            return;
+        }
+        V3 d = il.pt[l];
+        V3 p = add(il.tangent[l], scale(il.minT[l] + s.uniform() * (il.maxT[l] - il.minT[l]), il.perpT[l]));
+        if (dims == 3)
+            p = add(p, scale(il.minS[l] + s.uniform() * (il.maxS[l] - il.minS[l]), il.perpS[l]));
+        V3 r, sh;
+        for (int a = 0; a < 3; ++a) {
+            r[a] = 0.5 * (med.bmax[a] - med.bmin[a]);
+            sh[a] = p[a] - 0.5 * (med.bmax[a] + med.bmin[a]);
+        }
+        double bd = 0;
+        for (int a = 0; a < dims; ++a) {
+            int sgn = d[a] < 0 ? 1 : -1;
+            double dist = il.invDir[l][a] * (r[a] * sgn - sh[a]);
+            bool ok = dist >= 0.0;
+            for (int b = 0; b < dims && ok; ++b)
+                if (b != a) ok = std::fabs(std::fma(d[b], dist, sh[b])) < r[b];
+            if (ok) { bd = dist; break; }
+        }
+        *x0 = add(p, scale(bd + EPS, d));
+        *dir = d;
+    }
+
+    // TemporalSampler dtshift_host / dtShiftKernel / copyPointsFromBuffers*Temporal
+    // (TemporalSampler.cuh:376-436, 540-647, 778-937): displaced position per output frame.
+    void frames(uint64_t path, uint32_t event, Vertex& v) const {
+        uint32_t nt = fr.nt;
+        std::vector<V3> slot(nt);
+        Stream s(seed, path, event, 1);
+        for (uint32_t k = 0; k < nt; ++k) {
+            double dt = fr.dt[k];
+            double Dm = fr.D[v.m];
+            V3 Um = fr.U[v.m];
+            s.seek_block(k);
+            double u1 = s.uniform(), u2 = s.uniform();
+            double nrm = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * PI * u2);
+            V3 w = sc.randomDirection(s);
+            double dr = nrm * std::sqrt(2.0 * dims * Dm * std::fabs(dt));
+            for (int a = 0; a < dims; ++a) slot[k][a] = std::fma(dr, w[a], dt * Um[a]);
+        }
+        // cumulative sums along the negative chain [0, neg) and positive chain [neg, nt)
+        for (uint32_t k = 0; k < nt; ++k) {
+            bool first = (k == 0) || (k == fr.neg);
+            V3 base = first ? v.x : slot[k - 1];
+            slot[k] = add(base, slot[k]);
+        }
+        v.xt.assign(nt, V3());
+        for (uint32_t t = 0; t < nt; ++t) v.xt[t] = slot[t < fr.neg ? fr.neg - 1 - t : t];
+    }
+
+    // One path slot: NextEventEstimation::sapmleFirstPathPoints retry loop
+    // (NextEventEstimation.cuh:643-679) then sapmleNextPathPoints until the path dies (:681-759).
+    int transport(uint64_t path, std::vector<Vertex>& vs, uint32_t* misses) const {
+        vs.clear();
+        *misses = 0;
+        Vertex v1;
+        uint32_t event = 0;
+        for (;; ++event) {
+            if (event >= maxAttempts) return SSTMC_NO_HIT;
+            Stream s(seed, path, event, 0);
+            V3 x0, d;
+            sampleSource(s, &x0, &d);
+            V3 x;
+            int m = med.woodcock(x0, d, s, &x);
+            if (m > 0) { v1.x = x; v1.dirIn = d; v1.m = (uint32_t)m; break; }
+            ++*misses;
+        }
+        v1.rho = Stream(seed, path, event, 2).uniform();
+        frames(path, event, v1);
+        v1.len.assign(fr.nt, 0.0);
+        vs.push_back(v1);
+        for (uint32_t k = 2;; ++k) {
+            const Vertex& prev = vs.back();
+            uint32_t ev = 0x80000000u | k;
+            Stream s(seed, path, ev, 0);
+            V3 d = sc.sample(prev.m, prev.dirIn, s);
+            V3 x;
+            int m = med.woodcock(prev.x, d, s, &x);
+            if (m <= 0) break;
+            Vertex v;
+            v.x = x; v.dirIn = d; v.m = (uint32_t)m;
+            v.rho = Stream(seed, path, ev, 2).uniform();
+            frames(path, ev, v);
+            v.len.resize(fr.nt);
+            for (uint32_t t = 0; t < fr.nt; ++t)
+                v.len[t] = prev.len[t] + geo.norm(sub(v.xt[t], prev.xt[t]));
+            vs.push_back(v);
+        }
+        return SSTMC_NO_ERROR;
+    }
+
+    // ---------------------------------------------------------------- NEE terms
+    // g: FarField gDirectionKernel (FarField.cuh:223-263) / gPointKernel (PointSource.cuh:117-172)
+    cplx g(const Source& s, uint32_t i, const V3& x, uint64_t* steps) const {
+        if (s.type == SSTMC_SOURCE_FARFIELD) {
+            const V3& d = s.pt[i];
+            V3 wd = s.illum ? scale(-1.0, d) : d;
+            double T = std::exp(-med.opticalDepth(x, wd, steps));
+            double a = std::sqrt(T);
+            double ph = (s.illum ? 1.0 : -1.0) * 2.0 * geo.dot(d, x) / lambda;
+            return cplx(a * std::cos(PI * ph), a * std::sin(PI * ph));
+        }
+        V3 q = sub(s.pt[i], x);
+        double r = geo.norm(q);
+        double rr = 1.0 / r;
+        double T = std::exp(-med.opticalDepth(x, geo.normalize(q), steps));
+        double at = T * rr;
+        if (dims == 3) at *= rr * sc.isoAmp();
+        if (s.area) at *= rr * std::fabs(dims == 3 ? x[2] - s.pt[i][2] : x[1] - s.pt[i][1]);
+        double a = std::sqrt(at);
+        double ph = 2.0 * r / lambda;
+        return cplx(a * std::cos(PI * ph), a * std::sin(PI * ph));
+    }
+    // firstPointProbability (FarField.cuh:469-499, PointSource.cuh:263-294, quirk Q7 in 2D).
+    double pr(uint32_t l, const V3& x1, uint64_t* steps) const {
+        if (il.type == SSTMC_SOURCE_FARFIELD)
+            return std::exp(-med.opticalDepth(x1, scale(-1.0, il.pt[l]), steps)) * il.prob[l];
+        V3 q = sub(il.pt[l], x1);
+        double rr = 1.0 / geo.norm(q);
+        double T = std::exp(-med.opticalDepth(x1, geo.normalize(q), steps));
+        return dims == 3 ? T * (rr * rr) / (4.0 * PI) : T * (rr * (8.0 * PI));
+    }
+    // incoming direction at x1 from illumination l (FarField d_l; PointSource n(x1 - s_l)).
+    V3 inDir(uint32_t l, const V3& x1) const {
+        if (il.type == SSTMC_SOURCE_FARFIELD) return il.pt[l];
+        return geo.normalize(sub(x1, il.pt[l]));
+    }
+    // outgoing direction from x toward view v (FarField d_v; PointSource n(s_v - x)).
+    V3 outDir(uint32_t v, const V3& x) const {
+        if (vw.type == SSTMC_SOURCE_FARFIELD) return vw.pt[v];
+        return geo.normalize(sub(vw.pt[v], x));
+    }
+    // threePointFunctionSingle (FarField.cuh:546-599, PointSource.cuh:363-446)
+    double fsingle(uint32_t l, uint32_t v, uint32_t m, const V3& x1) const {
+        if (il.type == SSTMC_SOURCE_POINT && sc.allIsotropic) return sc.isoAmp();
+        return sc.amp(m, geo.dot(inDir(l, x1), outDir(v, x1)));
+    }
+    // illumination f at pL=2 (FarField.cuh:546-556; PointSource.cuh:347-361)
+    double fil(uint32_t l, uint32_t m, const V3& x1, const V3& x2) const {
+        if (il.type == SSTMC_SOURCE_POINT && sc.allIsotropic) return sc.isoAmp();
+        return sc.amp(m, geo.dot(inDir(l, x1), geo.normalize(sub(x2, x1))));
+    }
+    // view f at pL>=2: amplitude(-d_v | s_v, x_k, x_{k-1}) (NextEventEstimation.cuh:1560-1563)
+    double fvw(uint32_t v, uint32_t m, const V3& xprev, const V3& xk) const {
+        if (vw.type == SSTMC_SOURCE_POINT && sc.allIsotropic) return sc.isoAmp();
+        V3 a = vw.type == SSTMC_SOURCE_FARFIELD ? scale(-1.0, vw.pt[v])
+                                                  : geo.normalize(sub(xk, vw.pt[v]));
+        return sc.amp(m, geo.dot(a, geo.normalize(sub(xprev, xk))));
+    }
+
+    struct Terms {
+        std::vector<cplx> A;   // [t][l]
+        std::vector<cplx> B;   // [t][v]   (field) or sum_k G_k[v,0] conj G_k[v,t] (correlation)
+        std::vector<cplx> S;   // [t][v][l]
+        bool multiple = false;
+    };
+
+    // Per-path terms of Appendix A (NextEventEstimation.cuh:1016-1136,1138-1458;
+    // TemporalSampler.cuh:668-719).
+    void terms(const std::vector<Vertex>& vs, Terms& out, uint64_t* steps) const {
+        uint32_t nl = L(), nv = V(), nt = T();
+        const Vertex& v1 = vs[0];
+        out.A.assign((size_t)nt * nl, cplx(0));
+        out.B.assign((size_t)nt * nv, cplx(0));
+        out.S.assign((size_t)nt * nv * nl, cplx(0));
+        out.multiple = vs.size() >= 2;
+        std::vector<double> prl(nl);
+        double p1 = 0;
+        for (uint32_t l = 0; l < nl; ++l) { prl[l] = pr(l, v1.x, steps); p1 += prl[l]; }
+        p1 /= nl;
+        double w1 = 1.0 / std::sqrt(p1);
+        std::vector<cplx> gl((size_t)nt * nl), gv1((size_t)nt * nv);
+        for (uint32_t t = 0; t < nt; ++t) {
+            for (uint32_t l = 0; l < nl; ++l) gl[t * nl + l] = g(il, l, v1.xt[t], steps);
+            for (uint32_t v = 0; v < nv; ++v) gv1[t * nv + v] = g(vw, v, v1.xt[t], steps);
+        }
+        cplx e1(std::cos(2.0 * PI * v1.rho), std::sin(2.0 * PI * v1.rho));
+        for (uint32_t t = 0; t < nt; ++t) {
+            const V3& xf = fr.temporalAmp ? v1.xt[t] : v1.x;
+            for (uint32_t v = 0; v < nv; ++v)
+                for (uint32_t l = 0; l < nl; ++l)
+                    out.S[((size_t)t * nv + v) * nl + l] =
+                        gl[t * nl + l] * w1 * e1 * gv1[t * nv + v] * fsingle(l, v, v1.m, xf);
+        }
+        if (!out.multiple) return;
+        const Vertex& v2 = vs[1];
+        double p2 = 0;
+        for (uint32_t l = 0; l < nl; ++l)
+            p2 += prl[l] * sc.pdf(v1.m, geo.dot(inDir(l, v1.x), geo.normalize(sub(v2.x, v1.x))));
+        p2 /= nl;
+        double w2 = 1.0 / std::sqrt(p2);
+        for (uint32_t t = 0; t < nt; ++t) {
+            const V3& a1 = fr.temporalAmp ? v1.xt[t] : v1.x;
+            const V3& a2 = fr.temporalAmp ? v2.xt[t] : v2.x;
+            for (uint32_t l = 0; l < nl; ++l)
+                out.A[t * nl + l] = gl[t * nl + l] * fil(l, v1.m, a1, a2) * w2;
+        }
+        std::vector<cplx> G((size_t)nt * nv);
+        for (size_t k = 1; k < vs.size(); ++k) {
+            const Vertex& xk = vs[k];
+            const Vertex& xp = vs[k - 1];
+            for (uint32_t t = 0; t < nt; ++t) {
+                const V3& pk = fr.temporalAmp ? xk.xt[t] : xk.x;
+                const V3& pp = fr.temporalAmp ? xp.xt[t] : xp.x;
+                double ph = xk.rho + xk.len[t] / lambda;
+                cplx e(std::cos(2.0 * PI * ph), std::sin(2.0 * PI * ph));
+                for (uint32_t v = 0; v < nv; ++v)
+                    G[t * nv + v] = g(vw, v, xk.xt[t], steps) * fvw(v, xk.m, pp, pk) * e;
+            }
+            for (uint32_t t = 0; t < nt; ++t)
+                for (uint32_t v = 0; v < nv; ++v) {
+                    if (output == SSTMC_OUTPUT_FIELD) out.B[t * nv + v] += G[t * nv + v];
+                    else out.B[t * nv + v] += G[v] * std::conj(G[t * nv + v]);
+                }
+        }
+        if (output == SSTMC_OUTPUT_CORRELATION) {
+            for (uint32_t t = nt; t-- > 0;)  // frame 0 last: it is the reference of every lag
+                for (uint32_t l = 0; l < nl; ++l)
+                    out.A[t * nl + l] = out.A[l] * std::conj(out.A[t * nl + l]);
+        }
+    }
+    // Adds the path's contribution to U[t][v][l].
+    void accumulate(const Terms& tm, std::vector<cplx>& U) const {
+        uint32_t nl = L(), nv = V(), nt = T();
+        for (uint32_t t = 0; t < nt; ++t)
+            for (uint32_t v = 0; v < nv; ++v)
+                for (uint32_t l = 0; l < nl; ++l) {
+                    size_t i = ((size_t)t * nv + v) * nl + l;
+                    cplx s = tm.S[i];
+                    if (output == SSTMC_OUTPUT_CORRELATION) s = tm.S[(size_t)v * nl + l] * std::conj(s);
+                    U[i] += s;
+                    if (tm.multiple) U[i] += tm.A[t * nl + l] * tm.B[t * nv + v];
+                }
+    }
+};
+
+// ------------------------------------------------------------------ config parsing
+// Mirrors buildScattering/buildMedium/buildSource/buildSampler (MatlabInterface.cu:712-1372).
+int build(const sstmc_config* c, Problem& pb) {
+    if (!c || (c->dims != 2 && c->dims != 3)) return SSTMC_INVALID_CONFIG;
+    int dims = (int)c->dims;
+    pb.dims = dims;
+    pb.geo.dims = dims;
+    pb.lambda = c->simulation.lambda > 0 ? c->simulation.lambda : 1.0;
+    pb.P = c->simulation.batchSize ? c->simulation.batchSize : 1024;
+    pb.fullIter = c->simulation.fullIteration ? c->simulation.fullIteration : 1;
+    pb.seed = c->simulation.seed;
+    pb.output = c->tracer.output;
+    if (c->tracer.isCBS) return SSTMC_NOT_SUPPORTED;
+    // scattering: one cell -> materials 0 and 1 (MatlabInterface.cu:750-760, 878-881)
+    uint32_t nm = c->scattering.materialNum;
+    if (nm == 0 || nm > 10 || !c->scattering.type) return SSTMC_INVALID_CONFIG;
+    uint32_t nmat = nm == 1 ? 2 : nm;
+    pb.sc.dims = dims;
+    pb.sc.type.assign(nmat, 0);
+    pb.sc.g.assign(nmat, 0.0);
+    for (uint32_t m = 0; m < nmat; ++m) {
+        uint32_t src = nm == 1 ? 0 : m;
+        int ty = c->scattering.type[src];
+        if (ty != SSTMC_SCATTERING_ISOTROPIC && ty != SSTMC_SCATTERING_HG) return SSTMC_NOT_SUPPORTED;
+        pb.sc.type[m] = ty;
+        pb.sc.g[m] = (ty == SSTMC_SCATTERING_HG && c->scattering.g) ? c->scattering.g[src] : 0.0;
+    }
+    pb.sc.allIsotropic = true;
+    for (uint32_t m = 0; m < nmat; ++m)
+        if (!(pb.sc.type[m] == SSTMC_SCATTERING_ISOTROPIC || std::fabs(pb.sc.g[m]) < 0.0001))
+            pb.sc.allIsotropic = false;
+    // medium
+    const sstmc_medium& md = c->medium;
+    Medium& med = pb.med;
+    med.dims = dims;
+    uint32_t sz[3] = {md.xAxisSize, md.yAxisSize, dims == 3 ? md.zAxisSize : 2u};
+    const double* axp[3] = {md.xAxis, md.yAxis, md.zAxis};
+    for (int a = 0; a < 3; ++a) {
+        if (a < dims) {
+            if (sz[a] < 2 || !axp[a]) return SSTMC_INVALID_CONFIG;
+            med.ax[a].assign(axp[a], axp[a] + sz[a]);
+            for (uint32_t i = 1; i < sz[a]; ++i)
+                if (!(med.ax[a][i] > med.ax[a][i - 1])) return SSTMC_INVALID_CONFIG;
+        } else {
+            med.ax[a] = {0.0, 0.0};
+        }
+        med.bmin[a] = med.ax[a].front();
+        med.bmax[a] = med.ax[a].back();
+    }
+    size_t ncell = (size_t)(sz[0] - 1) * (sz[1] - 1) * (dims == 3 ? sz[2] - 1 : 1);
+    if (!md.materialGrid || !md.sigs || md.materialNum != nmat) return SSTMC_INVALID_CONFIG;
+    med.grid.assign(md.materialGrid, md.materialGrid + ncell);
+    for (uint32_t v : med.grid)
+        if (v == 0 || v >= nmat) return SSTMC_INVALID_CONFIG;
+    med.sigs.assign(md.sigs, md.sigs + nmat);
+    med.siga.assign(nmat, 0.0);
+    if (md.siga) med.siga.assign(md.siga, md.siga + nmat);
+    med.maxSigt = 0;
+    for (uint32_t m = 0; m < nmat; ++m) med.maxSigt = std::max(med.maxSigt, med.sigt(m));
+    if (!(med.maxSigt > 0)) return SSTMC_INVALID_CONFIG;
+    // sources
+    const sstmc_source* srcs[2] = {&c->illumination, &c->view};
+    Source* dst[2] = {&pb.il, &pb.vw};
+    for (int i = 0; i < 2; ++i) {
+        const sstmc_source& s = *srcs[i];
+        Source& d = *dst[i];
+        d.type = s.type;
+        d.illum = i == 0;
+        d.area = s.isAreaSource != 0;
+        if ((s.type != SSTMC_SOURCE_FARFIELD && s.type != SSTMC_SOURCE_POINT) || s.num == 0 ||
+            !s.x || !s.y || (dims == 3 && !s.z))
+            return SSTMC_INVALID_CONFIG;
+        d.pt.resize(s.num);
+        for (uint32_t k = 0; k < s.num; ++k) {
+            V3 p(s.x[k], s.y[k], dims == 3 ? s.z[k] : 0.0);
+            d.pt[k] = s.type == SSTMC_SOURCE_FARFIELD ? pb.geo.normalize(p) : p;
+        }
+        if (s.type == SSTMC_SOURCE_FARFIELD && i == 0) pb.buildFarField(d);
+    }
+    // sampler: time table (TemporalSampler.cuh:941-1132)
+    const sstmc_sampler& sp = c->sampler;
+    if (sp.tNum == 0 || !sp.t || !sp.D) return SSTMC_INVALID_CONFIG;
+    Frames& fr = pb.fr;
+    fr.nt = sp.tNum;
+    fr.temporalAmp = sp.isTemporalAmplitude != 0;
+    fr.D.assign(sp.D, sp.D + nmat);
+    for (double dv : fr.D)
+        if (dv < 0) return SSTMC_INVALID_CONFIG;
+    fr.U.assign(nmat, V3());
+    for (uint32_t m = 0; m < nmat; ++m)
+        fr.U[m] = V3(sp.Ux ? sp.Ux[m] : 0.0, sp.Uy ? sp.Uy[m] : 0.0, (dims == 3 && sp.Uz) ? sp.Uz[m] : 0.0);
+    double mean = 0;
+    for (uint32_t t = 0; t < fr.nt; ++t) mean += sp.t[t];
+    mean /= fr.nt;
+    std::vector<uint32_t> neg, pos;
+    std::vector<int> used(fr.nt, 0);
+    for (uint32_t t = 0; t < fr.nt; ++t) (sp.t[t] - mean < 0 ? neg : pos).push_back(t);
+    auto order = [&](std::vector<uint32_t>& lst, bool negative) {
+        std::vector<uint32_t> out;
+        for (size_t i = 0; i < lst.size(); ++i) {
+            bool first = true;
+            double best = 0;
+            uint32_t bi = 0;
+            for (uint32_t t = 0; t < fr.nt; ++t) {
+                if (used[t] || ((sp.t[t] - mean < 0) != negative)) continue;
+                double dd = negative ? mean - sp.t[t] : sp.t[t] - mean;
+                if (first || dd < best) { first = false; best = dd; bi = t; }
+            }
+            used[bi] = 1;
+            out.push_back(bi);
+        }
+        lst = out;
+    };
+    order(neg, true);
+    order(pos, false);
+    fr.neg = (uint32_t)neg.size();
+    fr.dt.assign(fr.nt, 0.0);
+    int prev = -1;
+    for (uint32_t k = 0; k < fr.nt; ++k) {
+        int tick = (int)(k < fr.neg ? neg[k] : pos[k - fr.neg]);
+        if (k == 0 || k == fr.neg) prev = -1;
+        fr.dt[k] = prev < 0 ? sp.t[tick] - mean : sp.t[tick] - sp.t[prev];
+        prev = tick;
+    }
+    return SSTMC_NO_ERROR;
+}
+
+}  // namespace
+
+// ====================================================================== C API (tests only)
+extern "C" {
+
+// One rendering: fullIteration batches of batchSize paths, path ids 0.. in order
+// (MatlabInterface.cu:1783-1801 for a single rendering). u: 2*L*V*T doubles, layout u[t][v][l].
+int oracle_render(const sstmc_config* cfg, double* u, double* iter, uint64_t* vertices, int nthreads) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    uint64_t npaths = (uint64_t)pb.P * pb.fullIter;
+    size_t usz = (size_t)pb.L() * pb.V() * pb.T();
+#ifdef _OPENMP
+    int nth = nthreads > 0 ? nthreads : omp_get_max_threads();
+#else
+    int nth = 1;
+    (void)nthreads;
+#endif
+    std::vector<std::vector<cplx>> Ut(nth, std::vector<cplx>(usz, cplx(0)));
+    std::vector<uint64_t> miss(nth, 0), nvtx(nth, 0);
+    std::vector<int> errs(nth, 0);
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nth)
+#endif
+    {
+#ifdef _OPENMP
+        int tid = omp_get_thread_num();
+#else
+        int tid = 0;
+#endif
+        std::vector<Vertex> vs;
+        Problem::Terms tm;
+        uint64_t chunk = (npaths + nth - 1) / nth;
+        uint64_t b = chunk * tid, e = std::min<uint64_t>(npaths, b + chunk);
+        for (uint64_t p = b; p < e; ++p) {
+            uint32_t ms = 0;
+            int rc = pb.transport(p, vs, &ms);
+            if (rc) { errs[tid] = rc; break; }
+            miss[tid] += ms;
+            nvtx[tid] += vs.size();
+            pb.terms(vs, tm, nullptr);
+            pb.accumulate(tm, Ut[tid]);
+        }
+    }
+    for (int t = 0; t < nth; ++t)
+        if (errs[t]) return errs[t];
+    uint64_t mtot = 0, vtot = 0;
+    for (size_t i = 0; i < usz; ++i) {
+        cplx s(0);
+        for (int t = 0; t < nth; ++t) s += Ut[t][i];
+        u[2 * i] = s.real();
+        u[2 * i + 1] = s.imag();
+    }
+    for (int t = 0; t < nth; ++t) { mtot += miss[t]; vtot += nvtx[t]; }
+    if (iter) *iter = (double)npaths + (double)mtot;
+    if (vertices) *vertices = vtot;
+    return SSTMC_NO_ERROR;
+}
+
+// Per-path transport + terms for path ids [begin, begin+n). Arrays as sstmc_debug_batch in
+// include/sstmc.h; call first with vtx_pos == NULL to get the vertex count.
+int oracle_paths(const sstmc_config* cfg, uint64_t begin, uint32_t n, uint64_t* nvtx_out,
+                 uint32_t* path_k, uint32_t* path_misses, uint64_t* path_off,
+                 double* vtx_pos, uint32_t* vtx_mat, double* vtx_rho, double* vtx_xt, double* vtx_len,
+                 double* term_a, double* term_b, double* term_s) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    uint32_t nl = pb.L(), nv = pb.V(), nt = pb.T();
+    std::vector<std::vector<Vertex>> all(n);
+    std::vector<uint32_t> ms(n);
+    std::vector<int> errs(n, 0);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (int64_t i = 0; i < (int64_t)n; ++i) errs[i] = pb.transport(begin + i, all[i], &ms[i]);
+    for (uint32_t i = 0; i < n; ++i)
+        if (errs[i]) return errs[i];
+    uint64_t tot = 0;
+    for (uint32_t i = 0; i < n; ++i) tot += all[i].size();
+    if (nvtx_out) *nvtx_out = tot;
+    if (!vtx_pos) return SSTMC_NO_ERROR;
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        path_k[i] = (uint32_t)all[i].size();
+        path_misses[i] = ms[i];
+        path_off[i] = off;
+        for (const Vertex& v : all[i]) {
+            for (int a = 0; a < 3; ++a) vtx_pos[3 * off + a] = v.x[a];
+            vtx_mat[off] = v.m;
+            vtx_rho[off] = v.rho;
+            for (uint32_t t = 0; t < nt; ++t) {
+                for (int a = 0; a < 3; ++a) vtx_xt[(off * nt + t) * 3 + a] = v.xt[t][a];
+                vtx_len[off * nt + t] = v.len[t];
+            }
+            ++off;
+        }
+    }
+    path_off[n] = off;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        Problem::Terms tm;
+        pb.terms(all[i], tm, nullptr);
+        for (size_t j = 0; j < tm.A.size(); ++j) {
+            term_a[2 * ((size_t)i * nt * nl + j)] = tm.multiple ? tm.A[j].real() : 0.0;
+            term_a[2 * ((size_t)i * nt * nl + j) + 1] = tm.multiple ? tm.A[j].imag() : 0.0;
+        }
+        for (size_t j = 0; j < tm.B.size(); ++j) {
+            term_b[2 * ((size_t)i * nt * nv + j)] = tm.B[j].real();
+            term_b[2 * ((size_t)i * nt * nv + j) + 1] = tm.B[j].imag();
+        }
+        if (term_s)
+            for (size_t j = 0; j < tm.S.size(); ++j) {
+                term_s[2 * ((size_t)i * nt * nv * nl + j)] = tm.S[j].real();
+                term_s[2 * ((size_t)i * nt * nv * nl + j) + 1] = tm.S[j].imag();
+            }
+    }
+    return SSTMC_NO_ERROR;
+}
+
+// NEE terms in FP64 for caller-supplied vertices (for example the GPU's FP32 transport), so
+// that term parity can be checked on identical paths. Inputs as produced by oracle_paths /
+// sstmc_debug_batch (positions as doubles).
+int oracle_terms_from_vertices(const sstmc_config* cfg, uint32_t n, const uint32_t* path_k,
+                               const uint64_t* path_off, const double* vtx_pos, const uint32_t* vtx_mat,
+                               const double* vtx_rho, const double* vtx_xt, const double* vtx_len,
+                               double* term_a, double* term_b, double* term_s) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    uint32_t nl = pb.L(), nv = pb.V(), nt = pb.T();
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        std::vector<Vertex> vs(path_k[i]);
+        for (uint32_t k = 0; k < path_k[i]; ++k) {
+            uint64_t o = path_off[i] + k;
+            Vertex& v = vs[k];
+            v.x = V3(vtx_pos[3 * o], vtx_pos[3 * o + 1], vtx_pos[3 * o + 2]);
+            v.m = vtx_mat[o];
+            v.rho = vtx_rho[o];
+            v.xt.resize(nt);
+            v.len.resize(nt);
+            for (uint32_t t = 0; t < nt; ++t) {
+                v.xt[t] = V3(vtx_xt[(o * nt + t) * 3], vtx_xt[(o * nt + t) * 3 + 1], vtx_xt[(o * nt + t) * 3 + 2]);
+                v.len[t] = vtx_len[o * nt + t];
+            }
+        }
+        Problem::Terms tm;
+        pb.terms(vs, tm, nullptr);
+        for (size_t j = 0; j < tm.A.size(); ++j) {
+            term_a[2 * ((size_t)i * nt * nl + j)] = tm.multiple ? tm.A[j].real() : 0.0;
+            term_a[2 * ((size_t)i * nt * nl + j) + 1] = tm.multiple ? tm.A[j].imag() : 0.0;
+        }
+        for (size_t j = 0; j < tm.B.size(); ++j) {
+            term_b[2 * ((size_t)i * nt * nv + j)] = tm.B[j].real();
+            term_b[2 * ((size_t)i * nt * nv + j) + 1] = tm.B[j].imag();
+        }
+        if (term_s)
+            for (size_t j = 0; j < tm.S.size(); ++j) {
+                term_s[2 * ((size_t)i * nt * nv * nl + j)] = tm.S[j].real();
+                term_s[2 * ((size_t)i * nt * nv * nl + j) + 1] = tm.S[j].imag();
+            }
+    }
+    return SSTMC_NO_ERROR;
+}
+
+// --- unit-level entry points, for closed-form and invariant checks -------------------------
+void oracle_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    Philox::block(ctr, key, out);
+}
+
+// Woodcock / DDA / scattering primitives on a built problem.
+double oracle_optical_depth(const sstmc_config* cfg, const double* p, const double* d, uint64_t* steps) {
+    Problem pb;
+    if (build(cfg, pb)) return -1.0;
+    return pb.med.opticalDepth(V3(p[0], p[1], pb.dims == 3 ? p[2] : 0), V3(d[0], d[1], pb.dims == 3 ? d[2] : 0), steps);
+}
+
+// Samples n free flights from origin o along d with path ids [0,n) (event 0); writes the
+// distance to the sampled point and the returned material (0 = escaped/absorbed).
+int oracle_free_flights(const sstmc_config* cfg, const double* o, const double* d, uint32_t n,
+                        double* dist, int32_t* mat) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    V3 O(o[0], o[1], pb.dims == 3 ? o[2] : 0), D(d[0], d[1], pb.dims == 3 ? d[2] : 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        Stream s(pb.seed, i, 0, 0);
+        V3 x;
+        mat[i] = pb.med.woodcock(O, D, s, &x);
+        dist[i] = pb.geo.norm(sub(x, O));
+    }
+    return SSTMC_NO_ERROR;
+}
+
+// Samples n scattering directions for material m about prev (path ids [0,n), event 0).
+int oracle_sample_directions(const sstmc_config* cfg, uint32_t m, const double* prev, uint32_t n, double* out) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    V3 pv(prev[0], prev[1], pb.dims == 3 ? prev[2] : 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        Stream s(pb.seed, i, 0, 0);
+        V3 d = pb.sc.sample(m, pv, s);
+        for (int a = 0; a < 3; ++a) out[3 * i + a] = d[a];
+    }
+    return SSTMC_NO_ERROR;
+}
+
+double oracle_phase_amplitude(const sstmc_config* cfg, uint32_t m, double cosang, int pdf) {
+    Problem pb;
+    if (build(cfg, pb)) return -1.0;
+    return pdf ? pb.sc.pdf(m, cosang) : pb.sc.amp(m, cosang);
+}
+
+// FarField sampling table of illumination l: tangent, perpT, perpS, minT, maxT, minS, maxS, prob.
+int oracle_farfield_table(const sstmc_config* cfg, uint32_t l, double* out16) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    if (pb.il.type != SSTMC_SOURCE_FARFIELD || l >= pb.L()) return SSTMC_INVALID_CONFIG;
+    for (int a = 0; a < 3; ++a) {
+        out16[a] = pb.il.tangent[l][a];
+        out16[3 + a] = pb.il.perpT[l][a];
+        out16[6 + a] = pb.il.perpS[l][a];
+    }
+    out16[9] = pb.il.minT[l]; out16[10] = pb.il.maxT[l];
+    out16[11] = pb.il.minS[l]; out16[12] = pb.il.maxS[l];
+    out16[13] = pb.il.prob[l];
+    return SSTMC_NO_ERROR;
+}
+
+// Frame chain table: dt per slot and the number of negative-chain slots.
+int oracle_frame_table(const sstmc_config* cfg, double* dt, uint32_t* neg) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    for (uint32_t k = 0; k < pb.fr.nt; ++k) dt[k] = pb.fr.dt[k];
+    *neg = pb.fr.neg;
+    return SSTMC_NO_ERROR;
+}
+
+}  // extern "C"

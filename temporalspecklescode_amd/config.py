@@ -1,0 +1,256 @@
+"""MATLAB-style config dict -> ``sstmc_config`` C struct (include/sstmc.h).
+
+The dict mirrors the reference's ``config`` struct field for field (README.md:59-349, parsed by
+MatlabInterface.cu:274-1563): ``simulation``, ``medium``, ``scattering``, ``illumination``,
+``view``, ``sampler``, ``tracer``.  Numeric arrays may be lists or numpy arrays;
+``medium.materialGrid`` may be an (nx, ny[, nz]) array indexed like MATLAB (x fastest in memory
+after Fortran-order flattening) or an already-flat vector.  Extra, MI355X-specific keys live
+under ``config['precision']`` (``terms``: 'fp32'|'fp64', ``contraction``: 'bf16x3'|'fp32'|'fp64')
+and ``config['tracer']['output']`` ('field'|'correlation').
+
+Validation follows the reference's MEX checks where they exist (material ids in 1..M-1,
+MatlabInterface.cu:975-992; D >= 0, :1293-1297; batchSize > 0, :1497-1509) and raises
+``ValueError`` with the reference's message text.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+# enum values, kept equal to include/sstmc.h
+MEDIUM_HETEROGENEOUS = 1
+SCATTERING_ISOTROPIC = 1
+SCATTERING_HG = 3
+SOURCE_NONE, SOURCE_POINT, SOURCE_FARFIELD = 0, 1, 2
+SAMPLER_TEMPORAL = 1
+TRACER_NEE = 1
+OUTPUT_FIELD, OUTPUT_CORRELATION = 0, 1
+TERMS_FP32, TERMS_FP64 = 0, 1
+CONTRACT_BF16X3, CONTRACT_FP32, CONTRACT_FP64 = 0, 1, 2
+
+_DP = C.POINTER(C.c_double)
+_U32P = C.POINTER(C.c_uint32)
+_I32P = C.POINTER(C.c_int32)
+
+
+class Simulation(C.Structure):
+    _fields_ = [("batchSize", C.c_uint32), ("fullIteration", C.c_uint32),
+                ("renderingsNum", C.c_uint32), ("deviceNumber", C.c_uint32),
+                ("seed", C.c_uint64), ("lambda_", C.c_double)]
+
+
+class Medium(C.Structure):
+    _fields_ = [("type", C.c_int32), ("xAxisSize", C.c_uint32), ("yAxisSize", C.c_uint32),
+                ("zAxisSize", C.c_uint32), ("xAxis", _DP), ("yAxis", _DP), ("zAxis", _DP),
+                ("materialGrid", _U32P), ("materialNum", C.c_uint32), ("sigs", _DP), ("siga", _DP)]
+
+
+class Scattering(C.Structure):
+    _fields_ = [("materialNum", C.c_uint32), ("type", _I32P), ("g", _DP)]
+
+
+class Source(C.Structure):
+    _fields_ = [("type", C.c_int32), ("num", C.c_uint32), ("x", _DP), ("y", _DP), ("z", _DP),
+                ("isAreaSource", C.c_int32)]
+
+
+class Sampler(C.Structure):
+    _fields_ = [("type", C.c_int32), ("tNum", C.c_uint32), ("t", _DP), ("D", _DP),
+                ("Ux", _DP), ("Uy", _DP), ("Uz", _DP), ("isTemporalAmplitude", C.c_int32)]
+
+
+class Tracer(C.Structure):
+    _fields_ = [("type", C.c_int32), ("isCBS", C.c_int32), ("output", C.c_int32)]
+
+
+class Precision(C.Structure):
+    _fields_ = [("terms", C.c_int32), ("contraction", C.c_int32)]
+
+
+class Config(C.Structure):
+    _fields_ = [("dims", C.c_uint32), ("simulation", Simulation), ("medium", Medium),
+                ("scattering", Scattering), ("illumination", Source), ("view", Source),
+                ("sampler", Sampler), ("tracer", Tracer), ("precision", Precision)]
+
+
+class CConfig:
+    """Owns an ``sstmc_config`` plus the numpy buffers its pointers reference."""
+
+    def __init__(self, cfg: dict, dims: int):
+        self._keep = []
+        self.struct = Config()
+        self.dims = dims
+        self._fill(cfg, dims)
+
+    # -- helpers
+    def _arr(self, a, dtype):
+        arr = np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel(order="F"))
+        self._keep.append(arr)
+        return arr
+
+    def _dptr(self, a):
+        arr = self._arr(a, np.float64)
+        return arr.ctypes.data_as(_DP), arr.size
+
+    def ptr(self):
+        return C.byref(self.struct)
+
+    def _fill(self, cfg, dims):
+        s = self.struct
+        s.dims = dims
+        sim = cfg.get("simulation", {}) or {}
+        bs = sim.get("batchSize", 1024)
+        if bs <= 0:
+            raise ValueError("Batch size must be a positive integer.")
+        s.simulation.batchSize = int(bs)
+        fi = sim.get("fullIteration", 1)
+        if fi <= 0:
+            raise ValueError("Full iteraion type must be non-complex positive double.")
+        s.simulation.fullIteration = int(fi)
+        rn = sim.get("renderingsNum", 1)
+        if rn <= 0:
+            raise ValueError("Renderings num type must be non-complex positive double.")
+        s.simulation.renderingsNum = int(rn)
+        s.simulation.deviceNumber = int(sim.get("deviceNumber", 0))
+        s.simulation.seed = int(abs(sim.get("seed", 0)))
+        lam = np.atleast_1d(np.asarray(sim.get("lambda", 1.0), dtype=np.float64))
+        if lam.size != 1:
+            raise ValueError("only one wavelength is supported (Nw = 1)")
+        s.simulation.lambda_ = float(lam[0])
+
+        # scattering (MatlabInterface.cu:712-888)
+        sc = cfg["scattering"]
+        types = sc["type"]
+        if isinstance(types, str):
+            types = [types]
+        amps = sc.get("amplitudeFunction", [0.0] * len(types))
+        if not isinstance(amps, (list, tuple)):
+            amps = [amps]
+        tcode, gvals = [], []
+        for i, t in enumerate(types):
+            if t == "Isotropic":
+                tcode.append(SCATTERING_ISOTROPIC)
+                gvals.append(0.0)
+            elif t in ("HenyeyGreenstein", "HG"):
+                tcode.append(SCATTERING_HG)
+                gvals.append(float(np.asarray(amps[i]).ravel()[0]))
+            else:
+                raise ValueError(
+                    "Unsupported scattering type in material %d. Type entered: %s. Available "
+                    "scattering types are Isotropic and HenyeyGreenstein (HG)" % (i, t))
+        n_sc = len(tcode)
+        material_num = 2 if n_sc == 1 else n_sc
+        ta = self._arr(tcode, np.int32)
+        s.scattering.materialNum = n_sc
+        s.scattering.type = ta.ctypes.data_as(_I32P)
+        s.scattering.g, _ = self._dptr(gvals)
+
+        # medium (MatlabInterface.cu:890-1060)
+        md = cfg["medium"]
+        if md.get("type") != "Heterogeneous":
+            raise ValueError("Unsupported medium type. Available medium type is Heterogeneous")
+        s.medium.type = MEDIUM_HETEROGENEOUS
+        axes = [np.asarray(md["xAxis"], np.float64).ravel(), np.asarray(md["yAxis"], np.float64).ravel()]
+        if dims == 3:
+            axes.append(np.asarray(md["zAxis"], np.float64).ravel())
+        for a in axes:
+            if a.size < 2 or np.any(np.diff(a) <= 0):
+                raise ValueError("medium axes must hold at least two increasing values")
+        s.medium.xAxis, s.medium.xAxisSize = self._dptr(axes[0])
+        s.medium.yAxis, s.medium.yAxisSize = self._dptr(axes[1])
+        if dims == 3:
+            s.medium.zAxis, s.medium.zAxisSize = self._dptr(axes[2])
+        ncell = int(np.prod([a.size - 1 for a in axes]))
+        grid = np.asarray(md["materialGrid"])
+        grid = grid.ravel(order="F") if grid.ndim > 1 else grid.ravel()
+        if grid.size != ncell:
+            raise ValueError("materialGrid must hold %d elements" % ncell)
+        if np.any(grid == 0):
+            raise ValueError("Material 0 is only allowed outside the grid. Allowed material numbers "
+                             "are from 1 to %d." % (material_num - 1))
+        if np.any(grid >= material_num):
+            raise ValueError("Material %d is not defined. Allowed material numbers are from 1 to %d."
+                             % (int(grid.max()), material_num - 1))
+        ga = self._arr(grid, np.uint32)
+        s.medium.materialGrid = ga.ctypes.data_as(_U32P)
+        sigs = np.asarray(md["sigs"], np.float64).ravel()
+        if sigs.size != material_num:
+            raise ValueError("sigs must hold %d elements" % material_num)
+        s.medium.sigs, _ = self._dptr(sigs)
+        s.medium.materialNum = material_num
+        if md.get("siga") is not None:
+            siga = np.asarray(md["siga"], np.float64).ravel()
+            if siga.size != material_num:
+                raise ValueError("siga must hold %d elements" % material_num)
+            s.medium.siga, _ = self._dptr(siga)
+
+        # sources (MatlabInterface.cu:1062-1215)
+        for name in ("illumination", "view"):
+            self._source(getattr(s, name), cfg[name], dims, name)
+
+        # sampler (MatlabInterface.cu:1217-1312)
+        sp = cfg["sampler"]
+        if sp.get("type", "TemporalSampler") != "TemporalSampler":
+            raise ValueError("Unsupported sampler type. Available sampler type is TemporalSampler")
+        s.sampler.type = SAMPLER_TEMPORAL
+        t = np.asarray(sp.get("t", [0.0]), np.float64).ravel()
+        s.sampler.t, s.sampler.tNum = self._dptr(t)
+        D = np.asarray(sp.get("D", np.zeros(material_num)), np.float64).ravel()
+        if D.size != material_num:
+            raise ValueError("D must hold %d elements" % material_num)
+        if np.any(D < 0):
+            raise ValueError("D values in temporal sampler must be non-negative")
+        s.sampler.D, _ = self._dptr(D)
+        U = sp.get("U", {}) or {}
+        s.sampler.Ux, _ = self._dptr(np.broadcast_to(np.asarray(U.get("x", 0.0), np.float64), (material_num,)))
+        s.sampler.Uy, _ = self._dptr(np.broadcast_to(np.asarray(U.get("y", 0.0), np.float64), (material_num,)))
+        s.sampler.Uz, _ = self._dptr(np.broadcast_to(np.asarray(U.get("z", 0.0), np.float64), (material_num,)))
+        s.sampler.isTemporalAmplitude = 1 if sp.get("isTemporalAmplitude", True) else 0
+
+        tr = cfg.get("tracer", {"type": "nee"})
+        if tr.get("type", "nee") != "nee":
+            raise ValueError("Unsupported tracer type. Available tracer type is nee")
+        s.tracer.type = TRACER_NEE
+        s.tracer.isCBS = 1 if tr.get("isCBS", False) else 0
+        out = tr.get("output", "field")
+        s.tracer.output = {"field": OUTPUT_FIELD, "correlation": OUTPUT_CORRELATION}[out]
+
+        pr = cfg.get("precision", {}) or {}
+        s.precision.terms = {"fp32": TERMS_FP32, "fp64": TERMS_FP64}[pr.get("terms", "fp32")]
+        s.precision.contraction = {"bf16x3": CONTRACT_BF16X3, "fp32": CONTRACT_FP32,
+                                   "fp64": CONTRACT_FP64}[pr.get("contraction", "bf16x3")]
+
+    def _source(self, dst, src, dims, name):
+        ty = src["type"]
+        if ty == "PointSource":
+            dst.type = SOURCE_POINT
+            loc = src["location"]
+        elif ty == "FarField":
+            dst.type = SOURCE_FARFIELD
+            loc = src["direction"]
+        else:
+            raise ValueError("Unsupported %s source type %s. Available types are PointSource and "
+                             "FarField" % (name, ty))
+        xs = np.atleast_1d(np.asarray(loc["x"], np.float64)).ravel()
+        ys = np.atleast_1d(np.asarray(loc["y"], np.float64)).ravel()
+        if xs.size != ys.size:
+            raise ValueError("%s x and y must have the same size" % name)
+        dst.x, dst.num = self._dptr(xs)
+        dst.y, _ = self._dptr(ys)
+        if dims == 3:
+            zs = np.atleast_1d(np.asarray(loc["z"], np.float64)).ravel()
+            if zs.size != xs.size:
+                raise ValueError("%s x and z must have the same size" % name)
+            dst.z, _ = self._dptr(zs)
+        dst.isAreaSource = 1 if src.get("isAreaSource", False) else 0
+
+
+def sizes(cfg: dict, dims: int):
+    """(Nl, Nv, Nt, renderingsNum) of a config dict."""
+    def n(src):
+        loc = src["location"] if src["type"] == "PointSource" else src["direction"]
+        return int(np.atleast_1d(np.asarray(loc["x"])).size)
+    nt = int(np.asarray(cfg["sampler"].get("t", [0.0])).size)
+    rn = int((cfg.get("simulation") or {}).get("renderingsNum", 1))
+    return n(cfg["illumination"]), n(cfg["view"]), nt, rn

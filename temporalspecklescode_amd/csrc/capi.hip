@@ -1,0 +1,276 @@
+// capi.hip -- the extern "C" boundary declared in include/sstmc.h.
+//
+// sstmc_create follows mexFunction's construction order (MatlabInterface.cu:1671-1700):
+// scattering -> medium -> illumination/view -> sampler -> tracer, with the reference's config
+// validation (MatlabInterface.cu:712-1563) turned into SSTMC_INVALID_CONFIG returns.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+
+#include "../../include/sstmc.h"
+#include "tracer.h"
+
+using namespace sstmc;
+
+struct sstmc_handle {
+    int device = 0;
+    uint32_t renderings = 1;
+    uint64_t seed = 0;
+    std::unique_ptr<Scattering> scattering;
+    std::unique_ptr<Medium> medium;
+    std::unique_ptr<Source> illumination, view;
+    std::unique_ptr<TemporalSampler> sampler;
+    std::unique_ptr<NextEventEstimation> tracer;
+};
+
+static int buildSource(const sstmc_source& s, const Medium& med, int dims, bool illum, std::unique_ptr<Source>* out) {
+    if (s.num == 0 || !s.x || !s.y || (dims == 3 && !s.z)) return SSTMC_INVALID_CONFIG;
+    int err = SSTMC_NO_ERROR;
+    if (s.type == SSTMC_SOURCE_FARFIELD)
+        out->reset(new FarField(&err, med, dims, s.x, s.y, s.z, s.num, illum));
+    else if (s.type == SSTMC_SOURCE_POINT)
+        out->reset(new PointSource(&err, dims, s.x, s.y, s.z, s.num, s.isAreaSource != 0, illum));
+    else
+        return SSTMC_INVALID_CONFIG;
+    return err;
+}
+
+// Host-only validation, before anything touches the device (MatlabInterface.cu:712-1563).
+static int validate(const sstmc_config* c) {
+    if (!c) return SSTMC_INVALID_CONFIG;
+    int dims = (int)c->dims;
+    if (dims != 2 && dims != 3) return SSTMC_INVALID_CONFIG;
+    if (c->tracer.type != SSTMC_TRACER_NEE && c->tracer.type != 0) return SSTMC_INVALID_CONFIG;
+    if (c->tracer.isCBS) return SSTMC_NOT_SUPPORTED;
+    if (c->tracer.output != SSTMC_OUTPUT_FIELD && c->tracer.output != SSTMC_OUTPUT_CORRELATION)
+        return SSTMC_INVALID_CONFIG;
+    const sstmc_scattering& sc = c->scattering;
+    if (sc.materialNum == 0 || sc.materialNum > (uint32_t)kMaxMaterials || !sc.type) return SSTMC_INVALID_CONFIG;
+    uint32_t nmat = sc.materialNum == 1 ? 2 : sc.materialNum;
+    for (uint32_t m = 0; m < sc.materialNum; ++m) {
+        if (sc.type[m] != SSTMC_SCATTERING_ISOTROPIC && sc.type[m] != SSTMC_SCATTERING_HG) return SSTMC_NOT_SUPPORTED;
+        if (sc.type[m] == SSTMC_SCATTERING_HG && sc.g && !(std::fabs(sc.g[m]) < 1.0)) return SSTMC_INVALID_CONFIG;
+    }
+    const sstmc_medium& md = c->medium;
+    if (md.type != SSTMC_MEDIUM_HETEROGENEOUS) return SSTMC_NOT_SUPPORTED;
+    const double* ax[3] = {md.xAxis, md.yAxis, md.zAxis};
+    uint32_t sz[3] = {md.xAxisSize, md.yAxisSize, md.zAxisSize};
+    size_t ncell = 1;
+    for (int a = 0; a < dims; ++a) {
+        if (!ax[a] || sz[a] < 2) return SSTMC_INVALID_CONFIG;
+        for (uint32_t i = 1; i < sz[a]; ++i)
+            if (!(ax[a][i] > ax[a][i - 1])) return SSTMC_INVALID_CONFIG;
+        ncell *= sz[a] - 1;
+    }
+    if (!md.materialGrid || !md.sigs || md.materialNum != nmat) return SSTMC_INVALID_CONFIG;
+    for (size_t i = 0; i < ncell; ++i)   // material 0 only outside the grid (MatlabInterface.cu:975-992)
+        if (md.materialGrid[i] == 0 || md.materialGrid[i] >= nmat) return SSTMC_INVALID_CONFIG;
+    double maj = 0;
+    for (uint32_t m = 0; m < nmat; ++m) {
+        if (!(md.sigs[m] >= 0) || (md.siga && !(md.siga[m] >= 0))) return SSTMC_INVALID_CONFIG;
+        maj = std::max(maj, md.sigs[m] + (md.siga ? md.siga[m] : 0.0));
+    }
+    if (!(maj > 0)) return SSTMC_INVALID_CONFIG;
+    const sstmc_source* srcs[2] = {&c->illumination, &c->view};
+    for (const sstmc_source* s : srcs) {
+        if (s->type != SSTMC_SOURCE_FARFIELD && s->type != SSTMC_SOURCE_POINT) return SSTMC_INVALID_CONFIG;
+        if (s->num == 0 || !s->x || !s->y || (dims == 3 && !s->z)) return SSTMC_INVALID_CONFIG;
+    }
+    const sstmc_sampler& sp = c->sampler;
+    if (sp.type != SSTMC_SAMPLER_TEMPORAL && sp.type != 0) return SSTMC_NOT_SUPPORTED;
+    if (sp.tNum == 0 || !sp.t || !sp.D) return SSTMC_INVALID_CONFIG;
+    for (uint32_t m = 0; m < nmat; ++m)
+        if (!(sp.D[m] >= 0)) return SSTMC_INVALID_CONFIG;   // MatlabInterface.cu:1293-1297
+    if (c->simulation.lambda < 0 || !std::isfinite(c->simulation.lambda)) return SSTMC_INVALID_CONFIG;
+    if (c->precision.terms != SSTMC_TERMS_FP32 && c->precision.terms != SSTMC_TERMS_FP64) return SSTMC_INVALID_CONFIG;
+    if (c->precision.contraction < SSTMC_CONTRACT_BF16X3 || c->precision.contraction > SSTMC_CONTRACT_FP64)
+        return SSTMC_INVALID_CONFIG;
+    return SSTMC_NO_ERROR;
+}
+
+static int create(const sstmc_config* c, sstmc_handle** out) {
+    if (!c || !out) return SSTMC_INVALID_CONFIG;
+    *out = nullptr;
+    int vrc = validate(c);
+    if (vrc) return vrc;
+    int dims = (int)c->dims;
+    std::unique_ptr<sstmc_handle> h(new sstmc_handle());
+    h->device = (int)c->simulation.deviceNumber;
+    if (hipSetDevice(h->device) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    h->renderings = c->simulation.renderingsNum ? c->simulation.renderingsNum : 1;
+    h->seed = c->simulation.seed;
+
+    // scattering (MatlabInterface.cu:712-888): a single cell also defines material 1
+    const sstmc_scattering& sc = c->scattering;
+    if (sc.materialNum == 0 || sc.materialNum > (uint32_t)kMaxMaterials || !sc.type) return SSTMC_INVALID_CONFIG;
+    uint32_t nmat = sc.materialNum == 1 ? 2 : sc.materialNum;
+    h->scattering.reset(new Scattering(dims));
+    for (uint32_t m = 0; m < nmat; ++m) {
+        uint32_t src = sc.materialNum == 1 ? 0 : m;
+        int rc;
+        if (sc.type[src] == SSTMC_SCATTERING_ISOTROPIC) rc = h->scattering->setIsotropicScattering(m);
+        else if (sc.type[src] == SSTMC_SCATTERING_HG) rc = h->scattering->setHenyeyGreensteinScattering(sc.g ? sc.g[src] : 0.0, m);
+        else return SSTMC_NOT_SUPPORTED;
+        if (rc) return rc;
+    }
+
+    // medium (MatlabInterface.cu:951-1056)
+    const sstmc_medium& md = c->medium;
+    if (md.type != SSTMC_MEDIUM_HETEROGENEOUS) return SSTMC_NOT_SUPPORTED;
+    const double* ax[3] = {md.xAxis, md.yAxis, md.zAxis};
+    uint32_t sz[3] = {md.xAxisSize, md.yAxisSize, md.zAxisSize};
+    size_t ncell = 1;
+    for (int a = 0; a < dims; ++a) {
+        if (!ax[a] || sz[a] < 2) return SSTMC_INVALID_CONFIG;
+        for (uint32_t i = 1; i < sz[a]; ++i)
+            if (!(ax[a][i] > ax[a][i - 1])) return SSTMC_INVALID_CONFIG;
+        ncell *= sz[a] - 1;
+    }
+    if (!md.materialGrid || !md.sigs || md.materialNum != nmat) return SSTMC_INVALID_CONFIG;
+    for (size_t i = 0; i < ncell; ++i)
+        if (md.materialGrid[i] == 0 || md.materialGrid[i] >= nmat) return SSTMC_INVALID_CONFIG;
+    for (uint32_t m = 0; m < nmat; ++m)
+        if (md.sigs[m] < 0 || (md.siga && md.siga[m] < 0)) return SSTMC_INVALID_CONFIG;
+    int err = SSTMC_NO_ERROR;
+    h->medium.reset(new HeterogeneousMedium(&err, dims, md.materialGrid, md.xAxis, md.xAxisSize, md.yAxis,
+                                            md.yAxisSize, md.zAxis, md.zAxisSize));
+    if (err) return err;
+    if ((err = h->medium->setMaterial(md.sigs, md.siga, nmat))) return err;
+
+    // sources (MatlabInterface.cu:1062-1215)
+    if ((err = buildSource(c->illumination, *h->medium, dims, true, &h->illumination))) return err;
+    if ((err = buildSource(c->view, *h->medium, dims, false, &h->view))) return err;
+
+    // sampler (MatlabInterface.cu:1217-1312)
+    const sstmc_sampler& sp = c->sampler;
+    if (sp.type != SSTMC_SAMPLER_TEMPORAL && sp.type != 0) return SSTMC_NOT_SUPPORTED;
+    if (sp.tNum == 0 || !sp.t || !sp.D) return SSTMC_INVALID_CONFIG;
+    h->sampler.reset(new TemporalSampler(&err, sp.t, sp.tNum, sp.isTemporalAmplitude != 0));
+    if (err) return err;
+    for (uint32_t m = 0; m < nmat; ++m) {
+        if (sp.D[m] < 0) return SSTMC_INVALID_CONFIG;   // MatlabInterface.cu:1293-1297
+        err = h->sampler->setMaterial(sp.D[m], sp.Ux ? sp.Ux[m] : 0.0, sp.Uy ? sp.Uy[m] : 0.0,
+                                      (dims == 3 && sp.Uz) ? sp.Uz[m] : 0.0, m);
+        if (err) return err;
+    }
+    h->sampler->finalize(nmat);
+
+    // tracer (MatlabInterface.cu:1374-1457, defaults :1467, :1590-1594)
+    TracerOptions opt;
+    opt.batchSize = c->simulation.batchSize ? c->simulation.batchSize : 1024;
+    opt.fullIteration = c->simulation.fullIteration ? c->simulation.fullIteration : 1;
+    opt.seed = c->simulation.seed;
+    opt.lambda = c->simulation.lambda > 0 ? c->simulation.lambda : 1.0;
+    if (!(std::isfinite(opt.lambda))) return SSTMC_INVALID_CONFIG;
+    opt.output = c->tracer.output;
+    opt.terms = c->precision.terms;
+    opt.contraction = c->precision.contraction;
+    h->tracer.reset(new NextEventEstimation(&err, *h->medium, *h->scattering, *h->illumination, *h->view,
+                                            *h->sampler, opt));
+    if (err) return err;
+    *out = h.release();
+    return SSTMC_NO_ERROR;
+}
+
+extern "C" {
+
+int sstmc_create(const sstmc_config* cfg, sstmc_handle** out) {
+    try {
+        return create(cfg, out);
+    } catch (...) {
+        return SSTMC_ALLOCATION_ERROR;
+    }
+}
+
+int sstmc_trace(sstmc_handle* h, double* u, double* iter) {
+    if (!h || !u) return SSTMC_INVALID_CONFIG;
+    hipSetDevice(h->device);
+    int rc = h->tracer->trace(u);
+    if (rc) return rc;
+    // iter = batchSize * fullIteration + zeroContributionPaths (MatlabInterface.cu:1796)
+    if (iter) *iter = (double)h->tracer->paths() + (double)h->tracer->getZeroContributionPaths();
+    return SSTMC_NO_ERROR;
+}
+
+int sstmc_update_seed(sstmc_handle* h, uint64_t seed) {
+    if (!h) return SSTMC_INVALID_CONFIG;
+    h->seed = seed;
+    h->tracer->updateSeed(seed);
+    return SSTMC_NO_ERROR;
+}
+
+uint64_t sstmc_zero_contribution_paths(const sstmc_handle* h) { return h ? h->tracer->getZeroContributionPaths() : 0; }
+uint64_t sstmc_photons_budget(const sstmc_handle* h) { return h ? h->tracer->getPhotonsBudget() : 0; }
+
+int sstmc_sizes(const sstmc_handle* h, uint32_t* nl, uint32_t* nv, uint32_t* nt) {
+    if (!h) return SSTMC_INVALID_CONFIG;
+    if (nl) *nl = h->tracer->L();
+    if (nv) *nv = h->tracer->V();
+    if (nt) *nt = h->tracer->T();
+    return SSTMC_NO_ERROR;
+}
+
+void sstmc_destroy(sstmc_handle* h) {
+    if (!h) return;
+    hipSetDevice(h->device);
+    delete h;
+}
+
+const char* sstmc_error_string(int code) {
+    switch (code) {
+        case SSTMC_NO_ERROR: return "No error";
+        case SSTMC_DEVICE_ERROR: return "Device error: the HIP device could not be used";
+        case SSTMC_ALLOCATION_ERROR: return "Allocation error: not enough device memory, decrease batchSize";
+        case SSTMC_KERNEL_ERROR: return "Kernel error: a HIP kernel failed";
+        case SSTMC_BLAS_ERROR: return "BLAS error";
+        case SSTMC_NOT_SUPPORTED: return "Not supported by this build";
+        case SSTMC_MISC_ERROR: return "Miscellaneous error";
+        case SSTMC_INVALID_CONFIG: return "Invalid config";
+        case SSTMC_NO_HIT: return "No path reached the medium within the retry budget";
+        case SSTMC_BUDGET_EXCEEDED: return "Request larger than the admission budget";
+        case SSTMC_NOT_READY: return "Request still in flight";
+        default: return "Unknown error";
+    }
+}
+
+int sstmc_render(const sstmc_config* cfg, double* u, double* iter) {
+    sstmc_handle* h = nullptr;
+    int rc = sstmc_create(cfg, &h);
+    if (rc) return rc;
+    uint32_t nl, nv, nt;
+    sstmc_sizes(h, &nl, &nv, &nt);
+    size_t stride = 2 * (size_t)nl * nv * nt;
+    for (uint32_t i = 0; i < h->renderings; ++i) {
+        sstmc_update_seed(h, cfg->simulation.seed + i);   // seed++ per rendering (MatlabInterface.cu:1799)
+        rc = sstmc_trace(h, u + stride * i, iter ? iter + i : nullptr);
+        if (rc) break;
+    }
+    sstmc_destroy(h);
+    return rc;
+}
+
+int sstmc_last_timing(const sstmc_handle* h, sstmc_timing* out) {
+    if (!h || !out) return SSTMC_INVALID_CONFIG;
+    h->tracer->timing(out);
+    return SSTMC_NO_ERROR;
+}
+
+int sstmc_debug_batch(sstmc_handle* h, uint64_t path_begin, uint32_t npaths, uint64_t* nvtx_out, uint32_t* path_k,
+                      uint32_t* path_misses, uint64_t* path_off, float* vtx_pos, uint32_t* vtx_mat, float* vtx_rho,
+                      float* vtx_dxt, double* vtx_len, double* term_a, double* term_b, double* term_sa,
+                      double* term_sb, double* u_batch) {
+    if (!h) return SSTMC_INVALID_CONFIG;
+    hipSetDevice(h->device);
+    return h->tracer->debugBatch(path_begin, npaths, nvtx_out, path_k, path_misses, path_off, vtx_pos, vtx_mat, vtx_rho,
+                                 vtx_dxt, vtx_len, term_a, term_b, term_sa, term_sb, u_batch);
+}
+
+int sstmc_debug_contract(sstmc_handle* h, uint32_t npaths, const double* a, const double* b, double* u) {
+    if (!h || !a || !b || !u) return SSTMC_INVALID_CONFIG;
+    hipSetDevice(h->device);
+    return h->tracer->debugContract(npaths, a, b, u);
+}
+
+}  // extern "C"

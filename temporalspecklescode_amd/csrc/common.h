@@ -1,0 +1,148 @@
+// common.h -- device-side types shared by the transport, NEE and contraction kernels.
+//
+// Layout decisions (MI355X-first, not a translation of the reference's AoS `TemporalPathPoint`,
+// TemporalSampler.cuh:43-50):
+//   * every per-request table is passed by value or by pointer in kernel arguments; there is no
+//     `__constant__` global state (the reference keeps medium, scattering and sampler tables in
+//     process-global `__constant__` symbols, MediumInterface.cuh:30-35, Scattering.cuh:15-29,
+//     TemporalSampler.cuh:14-38), so any number of requests can be in flight on one GPU;
+//   * the material grid is one byte per voxel (ids <= 9), 2 MiB for 128^3, resident in L2;
+//   * vertices are written once by the transport kernel as SoA records and read by the NEE kernel
+//     as wave-uniform broadcasts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sstmc {
+
+constexpr int kMaxMaterials = 10;   // MATERIAL_NUM (Simulation.cuh:32)
+constexpr uint32_t kMaxAttempts = 1u << 20;  // first-vertex retry budget per path slot
+
+// ---------------------------------------------------------------- counter-based RNG
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11). Stream spec shared with the CPU
+// restatement in oracle/: key = (seed lo, seed hi); counter = (path lo, path hi, event,
+// purpose << 24 | block). Events: a = first-vertex attempt a, 0x80000000|k = sampling of vertex
+// k >= 2. Purposes: 0 transport, 1 frame displacement (slot s = block s), 2 random phase.
+enum : uint32_t { kPurposeTransport = 0, kPurposeFrames = 1, kPurposePhase = 2 };
+constexpr uint32_t kEventVertex = 0x80000000u;
+
+__host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+struct Rng {
+    uint32_t k0, k1, c0, c1, c2, c3;
+    uint32_t buf[4];
+    int pos;
+    __host__ __device__ Rng(uint64_t seed, uint64_t path, uint32_t event, uint32_t purpose)
+        : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)), c0((uint32_t)path), c1((uint32_t)(path >> 32)),
+          c2(event), c3(purpose << 24), pos(4) {}
+    __host__ __device__ void seek(uint32_t block) { c3 = (c3 & 0xff000000u) | block; pos = 4; }
+    __host__ __device__ uint32_t word() {
+        if (pos == 4) {
+            philox4x32_10(c0, c1, c2, c3, k0, k1, buf);
+            ++c3;
+            pos = 0;
+        }
+        return buf[pos++];
+    }
+    // ((w >> 8) + 0.5) * 2^-24: exactly representable in FP32, strictly inside (0, 1).
+    __host__ __device__ float uniform() { return ((float)(word() >> 8) + 0.5f) * (1.0f / 16777216.0f); }
+    // ceil(u * n) - 1 for that u (Simulation.cuh:306-309), in exact integer arithmetic.
+    __host__ __device__ uint32_t index(uint32_t n) {
+        return (uint32_t)(((2ull * (word() >> 8) + 1ull) * n) >> 25);
+    }
+};
+
+// ---------------------------------------------------------------- small vectors
+struct f3 { float x, y, z; };
+struct d3 { double x, y, z; };
+__host__ __device__ inline f3 mk(float x, float y, float z) { f3 r = {x, y, z}; return r; }
+__host__ __device__ inline d3 mkd(double x, double y, double z) { d3 r = {x, y, z}; return r; }
+__host__ __device__ inline float comp(const f3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+__host__ __device__ inline double comp(const d3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+// ---------------------------------------------------------------- problem tables (device view)
+// Medium: HeterogeneousMedium.cuh:17-49 + MediumInterface.cuh:8-35. Edges are kept in FP32 for
+// transport and FP32 NEE, and in FP64 for the FP64 NEE variant.
+struct MediumView {
+    int dims;
+    uint32_t n[3];                 // cells per axis (1 for the unused z in 2D)
+    const float* ef[3];            // edges, n[a] + 1 each
+    const double* ed[3];
+    float bminf[3], bmaxf[3];
+    double bmind[3], bmaxd[3];
+    const uint8_t* grid;           // material id per cell, x fastest
+    float sigtf[kMaxMaterials];
+    double sigtd[kMaxMaterials];
+    float accept[kMaxMaterials];   // sigt(m) / maxSigt  (Woodcock acceptance)
+    float albedo[kMaxMaterials];   // sigs(m) / sigt(m)  (absorption test)
+    float invMaj;                  // 1 / maxSigt
+    int uniformAxis[3];            // 1 when the edges are equally spaced
+    float e0[3], invH[3];          // first edge and inverse spacing for the fast locate
+};
+
+// Scattering: Scattering.cuh:15-29, HG table :500-572.
+struct ScatterView {
+    int dims;
+    int allIsotropic;
+    int type[kMaxMaterials];       // 1 isotropic, 3 HG
+    float g[kMaxMaterials];
+    double gd[kMaxMaterials];
+};
+
+// Sources: FarField (FarField.cuh:71-127) / PointSource (PointSource.cuh:176-216).
+// FarField illumination sampling table per source, 20 floats: tangent[3] perpT[3] perpS[3]
+// minT maxT minS maxS invDir[3] dir[3] -- precomputed on the host in FP64.
+constexpr int kFarFieldTable = 20;
+struct SourceView {
+    int type;                      // 1 point, 2 far field
+    int illum;                     // 1 illumination, 0 view
+    int area;                      // PointSource isAreaSource
+    uint32_t n;
+    const float* pf;               // directions (normalised) or locations, n x 3
+    const double* pd;
+    const float* table;            // FarField illumination sampling table (n x 20), else null
+    const float* probf;            // FarField 1 / projected area, n
+    const double* probd;
+};
+
+// Temporal frames: TemporalSampler.cuh:14-38 + the time table built in its ctor (:941-1132).
+struct FramesView {
+    uint32_t nt;                   // output frames
+    uint32_t neg;                  // slots of the negative chain
+    const float* dt;               // per chain slot
+    int trivial;                   // every displacement is exactly zero (dt*U == 0 and D*dt == 0)
+    int temporalAmp;               // isTemporalAmplitude
+    float D[kMaxMaterials];
+    float U[kMaxMaterials][3];
+};
+
+// One batch of vertices (SoA), written by the transport kernel.
+struct VertexBatch {
+    uint32_t* k;                   // [P] scattering vertices per path
+    uint32_t* misses;              // [P]
+    uint32_t* off;                 // [P + 1] exclusive scan of k
+    float4* xm;                    // [cap] position + material id (as float bits)
+    float* rho;                    // [cap] random phase
+    float* delta;                  // [cap * nt * 3] displacement per output frame (if non-trivial)
+    uint64_t cap;
+    uint32_t* status;              // [4]: 0 overflow flag, 1 no-hit flag
+};
+
+}  // namespace sstmc

@@ -1,0 +1,458 @@
+// contract.hip -- (c) the complex contraction U[t][v][l] += sum_p A[t][p][l] * B[t][p][v].
+//
+// The reference materialises the Nl x Nv x P outer products and reduces them with cublasZgemv
+// (pointwiseMultipication + matrixMultipication, MatrixOperations.cuh:44-149, 444-550, called from
+// NextEventEstimation.cuh:1134,1388,1451): ~48 bytes of HBM traffic per path-contribution. Here it
+// is one dense GEMM per frame with K = paths:
+//   * bf16x3: each FP32 operand is split into bf16 hi + lo (hi = rn(x), lo = rn(x - hi)) and the
+//     product is hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_bf16, FP32 accumulate. bf16 keeps
+//     FP32's exponent range, so no per-path rescaling is needed. The complex product is the real
+//     GEMM [Are Aim] x [[Bre Bim] [-Bim Bre]] with K' = 2P; the second operand is formed from the
+//     (re, im) bf16 pairs in registers by a sign flip and a half-word rotate.
+//   * fp32 / fp64: LDS-tiled FMA kernels, the accuracy references.
+// Split-K partial sums go to a workspace and are reduced in a fixed order, so a result does not
+// depend on scheduling: a request gives the same bits alone or in a mix.
+#include "common.h"
+#include "kernels.h"
+
+namespace sstmc {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
+
+// round-to-nearest-even FP32 -> bf16 bits
+__device__ inline uint32_t bf16bits(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0xffff) ? 0x40u : 0u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ inline float bf16val(uint32_t h) { return __uint_as_float(h << 16); }
+
+// ---------------------------------------------------------------- bf16x3 MFMA kernel
+constexpr int kTileL = 128, kTileV = 128, kChunk = 32;         // paths per LDS chunk
+constexpr int kRowBytes = kChunk * 2 * 2 + 16;                  // k' = 2*paths bf16 + 16 B pad
+constexpr int kRowWords = kRowBytes / 4;
+
+__global__ void __launch_bounds__(256) k_contract_bf16x3(const float2* __restrict__ A, const float2* __restrict__ B,
+                                                         uint32_t P, uint32_t L, uint32_t V, uint32_t pathsPerSplit,
+                                                         float2* __restrict__ part) {
+    __shared__ uint32_t sAh[kTileL * kRowWords], sAl[kTileL * kRowWords];
+    __shared__ uint32_t sBh[kTileV * kRowWords], sBl[kTileV * kRowWords];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t l0 = blockIdx.x * kTileL, v0 = blockIdx.y * kTileV;
+    const uint32_t split = blockIdx.z % gridDim.z;
+    const uint32_t t = 0;  // one frame per launch (the host offsets A, B and part)
+    (void)t;
+    const uint32_t pBegin = split * pathsPerSplit;
+    const uint32_t pEnd = min(P, pBegin + pathsPerSplit);
+    const uint32_t wl = (wave & 1) * 64, wv = (wave >> 1) * 64;   // wave sub-tile origin
+    f32x16 accRe[2][2], accIm[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) { accRe[i][j][r] = 0.f; accIm[i][j][r] = 0.f; }
+
+    const uint32_t ld_row = tid & 127;        // l (or v) row this thread converts
+    const uint32_t ld_grp = tid >> 7;         // path group parity
+    for (uint32_t pc = pBegin; pc < pEnd; pc += kChunk) {
+        // global -> registers -> bf16 hi/lo -> LDS; each thread covers 4 groups of 4 paths
+#pragma unroll
+        for (int gi = 0; gi < 4; ++gi) {
+            const uint32_t g = ld_grp + 2 * gi;          // 0..7
+            uint32_t ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t p = pc + 4 * g + q;
+                float2 av = make_float2(0.f, 0.f), bv = make_float2(0.f, 0.f);
+                if (p < pEnd) {
+                    if (l0 + ld_row < L) av = A[(size_t)p * L + l0 + ld_row];
+                    if (v0 + ld_row < V) bv = B[(size_t)p * V + v0 + ld_row];
+                }
+                uint32_t hr = bf16bits(av.x), hi = bf16bits(av.y);
+                uint32_t lr = bf16bits(av.x - bf16val(hr)), li = bf16bits(av.y - bf16val(hi));
+                ah[q] = hr | (hi << 16);
+                al[q] = lr | (li << 16);
+                hr = bf16bits(bv.x); hi = bf16bits(bv.y);
+                lr = bf16bits(bv.x - bf16val(hr)); li = bf16bits(bv.y - bf16val(hi));
+                bh[q] = hr | (hi << 16);
+                bl[q] = lr | (li << 16);
+            }
+            const uint32_t w = ld_row * kRowWords + 4 * g;
+            *(u32x4*)&sAh[w] = u32x4{ah[0], ah[1], ah[2], ah[3]};
+            *(u32x4*)&sAl[w] = u32x4{al[0], al[1], al[2], al[3]};
+            *(u32x4*)&sBh[w] = u32x4{bh[0], bh[1], bh[2], bh[3]};
+            *(u32x4*)&sBl[w] = u32x4{bl[0], bl[1], bl[2], bl[3]};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {      // 4 x (k' = 16) per chunk of 32 paths
+            const uint32_t kw = ks * 8 + (lane >> 5) * 4;   // word offset of this lane's 8 k' values
+            bf16x8 aH[2], aL[2], bReH[2], bReL[2], bImH[2], bImL[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const uint32_t row = wl + 32 * i + (lane & 31);
+                u32x4 h = *(const u32x4*)&sAh[row * kRowWords + kw];
+                u32x4 lo = *(const u32x4*)&sAl[row * kRowWords + kw];
+                aH[i] = __builtin_bit_cast(bf16x8, h);
+                aL[i] = __builtin_bit_cast(bf16x8, lo);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const uint32_t row = wv + 32 * j + (lane & 31);
+                u32x4 h = *(const u32x4*)&sBh[row * kRowWords + kw];
+                u32x4 lo = *(const u32x4*)&sBl[row * kRowWords + kw];
+                u32x4 hre, lre, him, lim;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    hre[q] = h[q] ^ 0x80000000u;                 // (re, -im)
+                    lre[q] = lo[q] ^ 0x80000000u;
+                    him[q] = (h[q] >> 16) | (h[q] << 16);        // (im, re)
+                    lim[q] = (lo[q] >> 16) | (lo[q] << 16);
+                }
+                bReH[j] = __builtin_bit_cast(bf16x8, hre);
+                bReL[j] = __builtin_bit_cast(bf16x8, lre);
+                bImH[j] = __builtin_bit_cast(bf16x8, him);
+                bImL[j] = __builtin_bit_cast(bf16x8, lim);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    accRe[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aL[i], bReH[j], accRe[i][j], 0, 0, 0);
+                    accRe[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aH[i], bReL[j], accRe[i][j], 0, 0, 0);
+                    accRe[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aH[i], bReH[j], accRe[i][j], 0, 0, 0);
+                    accIm[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aL[i], bImH[j], accIm[i][j], 0, 0, 0);
+                    accIm[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aH[i], bImL[j], accIm[i][j], 0, 0, 0);
+                    accIm[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aH[i], bImH[j], accIm[i][j], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    // C layout of a 32x32 f32 MFMA tile: lane holds column (lane % 32), rows
+    // 8*(r/4) + 4*(lane/32) + r%4 for accumulator r.
+    float2* out = part + (size_t)blockIdx.z * L * V;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const uint32_t v = v0 + wv + 32 * j + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t l = l0 + wl + 32 * i + 8 * (r / 4) + 4 * (lane >> 5) + (r & 3);
+                if (l < L && v < V) out[(size_t)v * L + l] = make_float2(accRe[i][j][r], accIm[i][j][r]);
+            }
+        }
+}
+
+// ---------------------------------------------------------------- FMA reference kernels
+template <typename T> struct C2T;
+template <> struct C2T<float> { typedef float2 T; };
+template <> struct C2T<double> { typedef double2 T; };
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_contract_fma(const typename C2T<T>::T* __restrict__ A,
+                                                      const typename C2T<T>::T* __restrict__ B, uint32_t P,
+                                                      uint32_t L, uint32_t V, uint32_t pathsPerSplit,
+                                                      typename C2T<T>::T* __restrict__ part) {
+    typedef typename C2T<T>::T C;
+    __shared__ C sA[16][64 + 1], sB[16][64 + 1];
+    const uint32_t tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const uint32_t l0 = blockIdx.x * 64, v0 = blockIdx.y * 64;
+    const uint32_t pBegin = blockIdx.z * pathsPerSplit, pEnd = min(P, pBegin + pathsPerSplit);
+    T re[4][4], im[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { re[i][j] = 0; im[i][j] = 0; }
+    for (uint32_t pc = pBegin; pc < pEnd; pc += 16) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t e = tid + 256 * q, kk = e >> 6, c = e & 63;
+            const uint32_t p = pc + kk;
+            C av, bv;
+            av.x = av.y = 0; bv.x = bv.y = 0;
+            if (p < pEnd && l0 + c < L) av = A[(size_t)p * L + l0 + c];
+            if (p < pEnd && v0 + c < V) bv = B[(size_t)p * V + v0 + c];
+            sA[kk][c] = av;
+            sB[kk][c] = bv;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < 16; ++kk) {
+            C a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) { a[i] = sA[kk][tx + 16 * i]; b[i] = sB[kk][ty + 16 * i]; }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    re[i][j] = fma(a[i].x, b[j].x, fma(-a[i].y, b[j].y, re[i][j]));
+                    im[i][j] = fma(a[i].x, b[j].y, fma(a[i].y, b[j].x, im[i][j]));
+                }
+        }
+        __syncthreads();
+    }
+    C* out = part + (size_t)blockIdx.z * L * V;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t l = l0 + tx + 16 * i, v = v0 + ty + 16 * j;
+            if (l < L && v < V) {
+                C c;
+                c.x = re[i][j];
+                c.y = im[i][j];
+                out[(size_t)v * L + l] = c;
+            }
+        }
+}
+
+// Fixed-order reduction of split partials into the FP64 accumulator, with the elementwise
+// single-scattering factor F (or F^2 for correlations) and a scalar.
+template <typename C>
+__global__ void __launch_bounds__(256) k_reduce(const C* __restrict__ part, uint32_t splits, uint32_t LV,
+                                                const float* __restrict__ F, const double* __restrict__ Fd, int fsq,
+                                                double fscalar, double* __restrict__ U) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= LV) return;
+    double re = 0, im = 0;
+    for (uint32_t s = 0; s < splits; ++s) {
+        C c = part[(size_t)s * LV + i];
+        re += (double)c.x;
+        im += (double)c.y;
+    }
+    double f = fscalar;
+    if (Fd) f *= fsq ? Fd[i] * Fd[i] : Fd[i];
+    else if (F) f *= fsq ? (double)F[i] * F[i] : (double)F[i];
+    U[2 * (size_t)i] += f * re;
+    U[2 * (size_t)i + 1] += f * im;
+}
+
+static uint32_t chooseSplits(uint32_t tiles, uint32_t P, uint32_t minPaths) {
+    uint32_t want = (1024 + tiles - 1) / tiles;
+    uint32_t maxByP = (P + minPaths - 1) / minPaths;
+    uint32_t s = want < maxByP ? want : maxByP;
+    return s < 1 ? 1 : s;
+}
+
+// The split-K workspace belongs to the caller (one per tracer), grown here when too small.
+hipError_t launchContract(int kind, const void* A, const void* B, uint32_t P, uint32_t L, uint32_t V, uint32_t nt,
+                          const float* F, const double* Fd, int fsq, double fscalar, double* U, Workspace* ws,
+                          hipStream_t st) {
+    uint32_t tl = kind == kContractBf16x3 ? kTileL : 64, tv = kind == kContractBf16x3 ? kTileV : 64;
+    uint32_t tiles = ((L + tl - 1) / tl) * ((V + tv - 1) / tv);
+    uint32_t splits = chooseSplits(tiles, P, kind == kContractBf16x3 ? 512 : 256);
+    uint32_t pps = (P + splits - 1) / splits;
+    if (kind == kContractBf16x3) pps = (pps + kChunk - 1) / kChunk * kChunk;
+    else pps = (pps + 15) / 16 * 16;
+    splits = (P + pps - 1) / pps;
+    if (splits < 1) splits = 1;
+    size_t elem = kind == kContractFp64 ? 16 : 8;
+    size_t need = (size_t)splits * L * V * elem;
+    if (ws->bytes < need) {
+        if (ws->ptr) {
+            hipStreamSynchronize(st);
+            hipFree(ws->ptr);
+        }
+        ws->ptr = nullptr;
+        ws->bytes = 0;
+        hipError_t e = hipMalloc(&ws->ptr, need);
+        if (e != hipSuccess) return e;
+        ws->bytes = need;
+    }
+    const uint32_t LV = L * V;
+    for (uint32_t t = 0; t < nt; ++t) {
+        dim3 grid((L + tl - 1) / tl, (V + tv - 1) / tv, splits);
+        if (kind == kContractBf16x3) {
+            const float2* a = (const float2*)A + (size_t)t * P * L;
+            const float2* b = (const float2*)B + (size_t)t * P * V;
+            hipLaunchKernelGGL(k_contract_bf16x3, grid, dim3(256), 0, st, a, b, P, L, V, pps, (float2*)ws->ptr);
+            hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st, (const float2*)ws->ptr,
+                               splits, LV, F, Fd, fsq, fscalar, U + 2 * (size_t)t * LV);
+        } else if (kind == kContractFp32) {
+            const float2* a = (const float2*)A + (size_t)t * P * L;
+            const float2* b = (const float2*)B + (size_t)t * P * V;
+            hipLaunchKernelGGL(k_contract_fma<float>, grid, dim3(256), 0, st, a, b, P, L, V, pps, (float2*)ws->ptr);
+            hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st, (const float2*)ws->ptr,
+                               splits, LV, F, Fd, fsq, fscalar, U + 2 * (size_t)t * LV);
+        } else {
+            const double2* a = (const double2*)A + (size_t)t * P * L;
+            const double2* b = (const double2*)B + (size_t)t * P * V;
+            hipLaunchKernelGGL(k_contract_fma<double>, grid, dim3(256), 0, st, a, b, P, L, V, pps, (double2*)ws->ptr);
+            hipLaunchKernelGGL(k_reduce<double2>, dim3((LV + 255) / 256), dim3(256), 0, st, (const double2*)ws->ptr,
+                               splits, LV, F, Fd, fsq, fscalar, U + 2 * (size_t)t * LV);
+        }
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- generic single scattering
+// S_p[l,v,t] = a_p[l,t] b_p[v,t] f(l -> x1^t -> v) when f depends on the first vertex (point
+// sources with anisotropic scattering, or per-material phase functions): threePointFunctionSingle
+// (FarField.cuh:546-599, PointSource.cuh:363-446) fused with the sum over paths. Block = 16 x 16
+// (l, v) outputs; per chunk of 16 paths the unit directions and terms are staged in LDS.
+template <typename R> struct SingleT;
+template <> struct SingleT<float> {
+    typedef float2 C;
+    __device__ static float sq(float x) { return sqrtf(x); }
+};
+template <> struct SingleT<double> {
+    typedef double2 C;
+    __device__ static double sq(double x) { return sqrt(x); }
+};
+
+template <int DIMS, typename R>
+__device__ inline R singleAmp(const ScatterView& s, uint32_t m, R c) {
+    if (s.type[m] == 1) return DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678;
+    R g = (R)s.gd[m];
+    R up = DIMS == 3 ? ((R)1 - g * g) / (R)12.566370614359172954 : ((R)1 - g * g) / (R)6.283185307179586477;
+    R den = fma((R)-2 * g, c, (R)1 + g * g);
+    return SingleT<R>::sq(DIMS == 3 ? up / (den * SingleT<R>::sq(den)) : up / den);
+}
+
+template <int DIMS, typename R>
+__global__ void __launch_bounds__(256) k_single_generic(ScatterView sc, SourceView il, SourceView vw, FramesView fr,
+                                                        VertexBatch vb, const typename SingleT<R>::C* __restrict__ a,
+                                                        const typename SingleT<R>::C* __restrict__ b, uint32_t P,
+                                                        uint32_t pathsPerSplit, uint32_t splits,
+                                                        typename SingleT<R>::C* __restrict__ part) {
+    typedef typename SingleT<R>::C C;
+    __shared__ R su[16][16][3], sw[16][16][3];
+    __shared__ C sa[16][16], sb[16][16];
+    __shared__ uint32_t sm[16];
+    const uint32_t L = il.n, V = vw.n;
+    const uint32_t tl = threadIdx.x & 15, tq = threadIdx.x >> 4;
+    const uint32_t split = blockIdx.z % splits, t = blockIdx.z / splits;
+    const uint32_t l = blockIdx.x * 16 + tl, v = blockIdx.y * 16 + tq;
+    const uint32_t pBegin = split * pathsPerSplit, pEnd = min(P, pBegin + pathsPerSplit);
+    const bool iso = il.type == 1 && sc.allIsotropic;
+    R re = 0, im = 0;
+    for (uint32_t pc = pBegin; pc < pEnd; pc += 16) {
+        const uint32_t p = pc + tq;
+        const uint32_t ls = blockIdx.x * 16 + tl, vs = blockIdx.y * 16 + tl;
+        C av, bv;
+        av.x = av.y = 0; bv.x = bv.y = 0;
+        double x1[3] = {0, 0, 0};
+        uint32_t m1 = 0;
+        bool ok = p < pEnd && vb.k[p] > 0 && (uint64_t)vb.off[P] <= vb.cap;
+        if (ok) {
+            const uint64_t o = vb.off[p];
+            const float4 xm = vb.xm[o];
+            m1 = __float_as_uint(xm.w);
+            x1[0] = xm.x; x1[1] = xm.y; x1[2] = xm.z;
+            if (fr.temporalAmp && !fr.trivial) {
+                const float* d = vb.delta + (o * fr.nt + t) * 3;
+                x1[0] += (double)d[0]; x1[1] += (double)d[1]; x1[2] += (double)d[2];
+            }
+            if (ls < L) av = a[((size_t)t * P + p) * L + ls];
+            if (vs < V) bv = b[((size_t)t * P + p) * V + vs];
+        }
+        // incoming direction of illumination ls at x1, outgoing direction toward view vs
+        double u[3] = {0, 0, 1}, w[3] = {0, 0, 1};
+        if (ls < L) {
+            if (il.type == 2) { u[0] = il.pd[3 * ls]; u[1] = il.pd[3 * ls + 1]; u[2] = il.pd[3 * ls + 2]; }
+            else {
+                double q[3] = {x1[0] - il.pd[3 * ls], x1[1] - il.pd[3 * ls + 1], DIMS == 3 ? x1[2] - il.pd[3 * ls + 2] : 0.0};
+                double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+                if (n > 0) { u[0] = q[0] / n; u[1] = q[1] / n; u[2] = q[2] / n; }
+            }
+        }
+        if (vs < V) {
+            if (vw.type == 2) { w[0] = vw.pd[3 * vs]; w[1] = vw.pd[3 * vs + 1]; w[2] = vw.pd[3 * vs + 2]; }
+            else {
+                double q[3] = {vw.pd[3 * vs] - x1[0], vw.pd[3 * vs + 1] - x1[1], DIMS == 3 ? vw.pd[3 * vs + 2] - x1[2] : 0.0};
+                double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+                if (n > 0) { w[0] = q[0] / n; w[1] = q[1] / n; w[2] = q[2] / n; }
+            }
+        }
+        if (DIMS == 2) { u[2] = 0; w[2] = 0; }
+        su[tq][tl][0] = (R)u[0]; su[tq][tl][1] = (R)u[1]; su[tq][tl][2] = (R)u[2];
+        sw[tq][tl][0] = (R)w[0]; sw[tq][tl][1] = (R)w[1]; sw[tq][tl][2] = (R)w[2];
+        sa[tq][tl] = av;
+        sb[tq][tl] = bv;
+        if (tl == 0) sm[tq] = m1;
+        __syncthreads();
+        for (int q = 0; q < 16; ++q) {
+            C x = sa[q][tl], y = sb[q][tq];
+            R c = su[q][tl][0] * sw[q][tq][0] + su[q][tl][1] * sw[q][tq][1] + su[q][tl][2] * sw[q][tq][2];
+            R f = iso ? (DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678) : singleAmp<DIMS, R>(sc, sm[q], c);
+            R pr = x.x * y.x - x.y * y.y, pi = x.x * y.y + x.y * y.x;
+            re = fma(f, pr, re);
+            im = fma(f, pi, im);
+        }
+        __syncthreads();
+    }
+    if (l < L && v < V) {
+        C c;
+        c.x = re;
+        c.y = im;
+        part[((size_t)blockIdx.z) * L * V + (size_t)v * L + l] = c;
+    }
+}
+
+hipError_t launchSingleGeneric(int dims, int fp64, const ScatterView& sc, const SourceView& il,
+                               const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
+                               const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st) {
+    const uint32_t L = il.n, V = vw.n, NT = fr.nt;
+    uint32_t tiles = ((L + 15) / 16) * ((V + 15) / 16);
+    uint32_t splits = chooseSplits(tiles, P, 256);
+    uint32_t pps = ((P + splits - 1) / splits + 15) / 16 * 16;
+    splits = (P + pps - 1) / pps;
+    if (splits < 1) splits = 1;
+    size_t elem = fp64 ? 16 : 8;
+    size_t need = (size_t)splits * NT * L * V * elem;
+    if (ws->bytes < need) {
+        if (ws->ptr) {
+            hipStreamSynchronize(st);
+            hipFree(ws->ptr);
+        }
+        ws->ptr = nullptr;
+        ws->bytes = 0;
+        hipError_t e = hipMalloc(&ws->ptr, need);
+        if (e != hipSuccess) return e;
+        ws->bytes = need;
+    }
+    dim3 grid((L + 15) / 16, (V + 15) / 16, splits * NT);
+    const uint32_t LV = L * V;
+    if (fp64) {
+        if (dims == 2)
+            hipLaunchKernelGGL((k_single_generic<2, double>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
+                               (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr);
+        else
+            hipLaunchKernelGGL((k_single_generic<3, double>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
+                               (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr);
+        for (uint32_t t = 0; t < NT; ++t)
+            hipLaunchKernelGGL(k_reduce<double2>, dim3((LV + 255) / 256), dim3(256), 0, st,
+                               (const double2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float*)nullptr,
+                               (const double*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
+    } else {
+        if (dims == 2)
+            hipLaunchKernelGGL((k_single_generic<2, float>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
+                               (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr);
+        else
+            hipLaunchKernelGGL((k_single_generic<3, float>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
+                               (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr);
+        for (uint32_t t = 0; t < NT; ++t)
+            hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st,
+                               (const float2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float*)nullptr,
+                               (const double*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
+    }
+    return hipGetLastError();
+}
+
+__global__ void k_zero(double* U, size_t n) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) U[i] = 0.0;
+}
+
+hipError_t launchZeroU(double* U, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, U, n);
+    return hipGetLastError();
+}
+
+}  // namespace sstmc

@@ -1,0 +1,231 @@
+// geometry.h -- medium lookups, free flight, scattering sampling and the NEE voxel walk.
+// Device functions templated on the real type R (float for transport and the FP32 NEE
+// variant, double for the FP64 NEE variant) and on DIMS (2 or 3).
+#pragma once
+
+#include "common.h"
+
+namespace sstmc {
+
+template <typename R> struct Real;
+template <> struct Real<float> {
+    typedef f3 V;
+    __device__ static const float* edges(const MediumView& m, int a) { return m.ef[a]; }
+    __device__ static float sigt(const MediumView& m, int i) { return m.sigtf[i]; }
+    __device__ static float bmin(const MediumView& m, int a) { return m.bminf[a]; }
+    __device__ static float bmax(const MediumView& m, int a) { return m.bmaxf[a]; }
+};
+template <> struct Real<double> {
+    typedef d3 V;
+    __device__ static const double* edges(const MediumView& m, int a) { return m.ed[a]; }
+    __device__ static double sigt(const MediumView& m, int i) { return m.sigtd[i]; }
+    __device__ static double bmin(const MediumView& m, int a) { return m.bmind[a]; }
+    __device__ static double bmax(const MediumView& m, int a) { return m.bmaxd[a]; }
+};
+
+__device__ inline float vget(const f3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+__device__ inline double vget(const d3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+// MediumNS::isInsideKernel (MediumInterface.cuh:204-213): closed box.
+template <int DIMS, typename R, typename V>
+__device__ inline bool inside(const MediumView& m, const V& p) {
+    bool in = p.x >= Real<R>::bmin(m, 0) && p.x <= Real<R>::bmax(m, 0) &&
+              p.y >= Real<R>::bmin(m, 1) && p.y <= Real<R>::bmax(m, 1);
+    if (DIMS == 3) in = in && p.z >= Real<R>::bmin(m, 2) && p.z <= Real<R>::bmax(m, 2);
+    return in;
+}
+
+// Cell index along one axis with the semantics of binarySearchKernel (Simulation.cuh:325-367):
+// e[b] <= x < e[b+1], clamped to a valid cell. Equally spaced axes take a multiply and at most a
+// one-cell correction against the stored edges, so both paths return the same cell.
+template <typename R>
+__device__ inline int locate(const MediumView& m, int a, R x) {
+    const R* e = Real<R>::edges(m, a);
+    int n = (int)m.n[a];
+    int b;
+    if (m.uniformAxis[a]) {
+        float f = ((float)x - m.e0[a]) * m.invH[a];
+        b = f <= 0.f ? 0 : (f >= (float)n ? n - 1 : (int)f);
+        if (b > 0 && x < e[b]) --b;
+        if (b < n - 1 && x >= e[b + 1]) ++b;
+        return b;
+    }
+    if (x < e[0]) return 0;
+    if (x >= e[n]) return n - 1;
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (e[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline uint32_t cellIndex(const MediumView& m, int bx, int by, int bz) {
+    return (uint32_t)bx + m.n[0] * ((uint32_t)by + m.n[1] * (uint32_t)bz);
+}
+
+// bdDistance (HeterogeneousMedium.cuh:269-383) reduced to its use in Woodcock tracking: is the
+// ray from a point outside the box heading into it.
+template <int DIMS>
+__device__ inline bool pointsToward(const MediumView& m, const f3& p, const f3& d) {
+    float r[3], o[3], dd[3] = {d.x, d.y, d.z}, pp[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        r[a] = 0.5f * (m.bmaxf[a] - m.bminf[a]);
+        o[a] = pp[a] - 0.5f * (m.bmaxf[a] + m.bminf[a]);
+    }
+    float mx = -3.0e38f;
+#pragma unroll
+    for (int a = 0; a < DIMS; ++a) {
+        float ir = fabsf(r[a]) < 1e-12f ? 1e12f : 1.0f / r[a];
+        mx = fmaxf(mx, fabsf(o[a]) * ir);
+    }
+    int winding = mx <= 1.0f ? -1 : 1;
+#pragma unroll
+    for (int a = 0; a < DIMS; ++a) {
+        float inv = fabsf(dd[a]) < 1e-12f ? 1e12f : 1.0f / dd[a];
+        int sgn = dd[a] < 0.f ? 1 : -1;
+        float dist = inv * (r[a] * (float)(winding * sgn) - o[a]);
+        bool ok = dist >= 0.0f;
+#pragma unroll
+        for (int b = 0; b < DIMS; ++b)
+            if (b != a) ok = ok && fabsf(fmaf(dd[b], dist, o[b])) < r[b];
+        if (ok) return true;
+    }
+    return false;
+}
+
+template <int DIMS>
+__device__ inline uint32_t materialAt(const MediumView& m, const f3& x) {
+    int bx = locate<float>(m, 0, x.x);
+    int by = locate<float>(m, 1, x.y);
+    int bz = DIMS == 3 ? locate<float>(m, 2, x.z) : 0;
+    return m.grid[cellIndex(m, bx, by, bz)];
+}
+
+// heterogeneousSampleKernel (HeterogeneousMedium.cuh:576-678): Woodcock tracking against the
+// global majorant, absorption as termination. Returns the material of the sampled scatterer or 0
+// (escaped / absorbed). Distances use -ln(u) with u uniform on (0,1) (the reference writes
+// -ln(1-u); same law).
+template <int DIMS>
+__device__ inline uint32_t woodcock(const MediumView& m, const f3& o, const f3& d, Rng& rng, f3* out) {
+    float dist = 0.f;
+    f3 x = o;
+    uint32_t mat = 0;
+    for (;;) {
+        dist += -__logf(rng.uniform()) * m.invMaj;
+        x = mk(fmaf(dist, d.x, o.x), fmaf(dist, d.y, o.y), DIMS == 3 ? fmaf(dist, d.z, o.z) : 0.f);
+        if (!inside<DIMS, float>(m, x)) {
+            if (!pointsToward<DIMS>(m, x, d)) break;
+        } else {
+            mat = materialAt<DIMS>(m, x);
+            if (rng.uniform() <= m.accept[mat]) break;
+        }
+    }
+    *out = x;
+    if (!inside<DIMS, float>(m, x)) return 0;
+    if (rng.uniform() > m.albedo[mat]) return 0;
+    return mat;
+}
+
+// randomDirection (VectorType.cuh:316-362).
+template <int DIMS>
+__device__ inline f3 randomDirection(Rng& rng) {
+    if (DIMS == 2) {
+        float s, c;
+        sincospif(2.0f * rng.uniform(), &s, &c);
+        return mk(s, c, 0.f);
+    }
+    float sp, cp;
+    sincospif(2.0f * rng.uniform(), &sp, &cp);
+    float ct = fmaf(rng.uniform(), 2.0f, -1.0f);
+    float st = sqrtf(fmaxf(0.f, 1.0f - ct * ct));
+    return mk(st * cp, st * sp, ct);
+}
+
+// sampleDeviceImplementation (Scattering.cuh:1091-1109): HG inverse CDF (:1039-1063) and
+// vector_rotation (VectorType.cuh:261-282) about the incoming direction.
+template <int DIMS>
+__device__ inline f3 sampleDirection(const ScatterView& s, uint32_t mat, const f3& prev, Rng& rng) {
+    if (s.type[mat] == 1) return randomDirection<DIMS>(rng);
+    float g = s.g[mat], c;
+    if (fabsf(g) < 0.001f) {
+        c = rng.uniform() * 2.0f - 1.0f;
+    } else {
+        float u = rng.uniform();
+        if (DIMS == 2) {
+            c = cosf(2.0f * atanf(((1.0f - g) / (1.0f + g)) * tanf(1.57079632679489662f * (1.0f - 2.0f * u))));
+        } else {
+            float sv = (1.0f - g * g) / (1.0f - g + 2.0f * g * u);
+            c = (1.0f + g * g - sv * sv) / (2.0f * g);
+        }
+    }
+    c = fminf(1.0f, fmaxf(-1.0f, c));
+    float sa = (rng.uniform() < 0.5f ? -1.0f : 1.0f) * sqrtf(fmaxf(0.f, 1.0f - c * c));
+    if (DIMS == 2) return mk(prev.x * c - prev.y * sa, prev.x * sa + prev.y * c, 0.f);
+    float sp, cp;
+    sincospif(2.0f * rng.uniform(), &sp, &cp);
+    float az = fabsf(prev.z);
+    if (az >= 1.0f) return mk(sa * cp, sa * sp, prev.z < 0.f ? -c : c);
+    float stw = sqrtf((1.0f - az) * (1.0f + az));
+    float inv = 1.0f / stw;
+    return mk(sa * (prev.x * prev.z * cp - prev.y * sp) * inv + prev.x * c,
+              sa * (prev.y * prev.z * cp + prev.x * sp) * inv + prev.y * c,
+              -sa * cp * stw + prev.z * c);
+}
+
+// ---------------------------------------------------------------- NEE voxel walk
+// heterogeneousAttenuationKernel (HeterogeneousMedium.cuh:386-573): exact walk from p along d to
+// the box boundary, tau = sum(segment * sigt). The crossing parameter of each face is computed
+// from the fixed origin p (one rounding per crossing instead of an accumulated position) and the
+// axis choice keeps the reference's strict-< tie order (x, then y, else z). The walk stops early
+// once tau > tauCut; callers pass a cut beyond which the result they form is exactly zero in R.
+template <int DIMS, typename R, typename V>
+__device__ inline R opticalDepth(const MediumView& m, const V& p, const V& d, R tauCut, uint32_t* steps) {
+    if (!inside<DIMS, R>(m, p)) return (R)0;
+    const R* e0 = Real<R>::edges(m, 0);
+    const R* e1 = Real<R>::edges(m, 1);
+    const R* e2 = Real<R>::edges(m, 2);
+    int b0 = locate<R>(m, 0, p.x), b1 = locate<R>(m, 1, p.y), b2 = DIMS == 3 ? locate<R>(m, 2, p.z) : 0;
+    const R big = (R)1e12;
+    R i0 = fabs(d.x) < (R)1e-12 ? (d.x >= 0 ? big : -big) : (R)1 / d.x;
+    R i1 = fabs(d.y) < (R)1e-12 ? (d.y >= 0 ? big : -big) : (R)1 / d.y;
+    R i2 = DIMS == 3 ? (fabs(d.z) < (R)1e-12 ? (d.z >= 0 ? big : -big) : (R)1 / d.z) : big;
+    int s0 = d.x < 0 ? -1 : 1, s1 = d.y < 0 ? -1 : 1, s2 = (DIMS == 3 && d.z < 0) ? -1 : 1;
+    int u0 = s0 > 0, u1 = s1 > 0, u2 = s2 > 0;
+    int n0 = (int)m.n[0], n1 = (int)m.n[1], n2 = (int)m.n[2];
+    R t0 = (e0[b0 + u0] - p.x) * i0;
+    R t1 = (e1[b1 + u1] - p.y) * i1;
+    R t2 = DIMS == 3 ? (e2[b2 + u2] - p.z) * i2 : (R)3e38;
+    R tcur = 0, tau = 0;
+    R st = Real<R>::sigt(m, m.grid[cellIndex(m, b0, b1, b2)]);
+    uint32_t cnt = 0;
+    for (;;) {
+        ++cnt;
+        int ax;
+        if (DIMS == 2) ax = t0 < t1 ? 0 : 1;
+        else ax = (t0 < t1 && t0 < t2) ? 0 : (t1 < t2 ? 1 : 2);
+        R tn = ax == 0 ? t0 : (ax == 1 ? t1 : t2);
+        tau += (tn - tcur) * st;
+        tcur = tn;
+        if (ax == 0) {
+            b0 += s0;
+            if (b0 < 0 || b0 >= n0) break;
+            t0 = (e0[b0 + u0] - p.x) * i0;
+        } else if (ax == 1) {
+            b1 += s1;
+            if (b1 < 0 || b1 >= n1) break;
+            t1 = (e1[b1 + u1] - p.y) * i1;
+        } else {
+            b2 += s2;
+            if (b2 < 0 || b2 >= n2) break;
+            t2 = (e2[b2 + u2] - p.z) * i2;
+        }
+        if (tau > tauCut) break;
+        st = Real<R>::sigt(m, m.grid[cellIndex(m, b0, b1, b2)]);
+    }
+    if (steps) *steps += cnt;
+    return tau;
+}
+
+}  // namespace sstmc

@@ -1,0 +1,65 @@
+// kernels.h -- launch entry points of the HIP kernels (one translation unit per stage).
+#pragma once
+
+#include "common.h"
+
+namespace sstmc {
+
+// NEE outputs of one batch (element type float2 for FP32 terms, double2 for FP64 terms).
+// Layouts are frame-major so each frame is a (P x L) / (P x V) row-major operand of the
+// contraction: A[t][p][l], B[t][p][v], a[t][p][l], b[t][p][v].
+struct NeeOut {
+    void* A;          // multiple-scattering illumination terms (or A(0) conj A(t) for correlation)
+    void* B;          // multiple-scattering view terms (or sum_k G_k(0) conj G_k(t))
+    void* a;          // single-scattering illumination terms g_l * w1 * exp(i 2 pi rho_1)
+    void* b;          // single-scattering view terms g_v
+    unsigned long long* steps;   // voxel steps of the NEE walks (accumulated)
+};
+
+// How single scattering is contracted (decided on the host from the source/scattering types).
+enum SingleMode : int {
+    kSingleSeparable = 0,  // S = (a^T b) o F with F a fixed L x V matrix or scalar
+    kSingleGeneric = 1     // S_p[l,v] needs f(l -> x1 -> v) per path (fused kernel)
+};
+
+struct NeeParams {
+    double lambda;
+    int output;            // 0 field, 1 correlation
+    int singleMode;
+    float tauCutF;         // early stop of far-field walks: exp(-tau/2) is exactly 0 in FP32 beyond it
+    double tauCutD;
+    uint32_t npaths;
+};
+
+hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc, const SourceView& il,
+                           const FramesView& fr, const VertexBatch& vb, uint64_t seed, uint64_t pathBegin,
+                           uint32_t npaths, int write, hipStream_t st);
+hipError_t launchScan(const uint32_t* k, uint32_t* off, uint32_t* blockSums, uint32_t n, hipStream_t st);
+hipError_t launchBatchTotals(const uint32_t* misses, const uint32_t* off, uint32_t n, unsigned long long* acc,
+                             hipStream_t st);
+
+hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
+                     const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
+                     const NeeOut& out, hipStream_t st);
+
+// Contraction U[t][v][l] += scale(l,v) * sum_p A[t][p][l] * B[t][p][v] for t < nt.
+// U is complex FP64 (interleaved). F (optional, real, [v][l]) multiplies the sum elementwise,
+// squared when fsq != 0; fscalar multiplies everything.
+enum ContractKind : int { kContractBf16x3 = 0, kContractFp32 = 1, kContractFp64 = 2 };
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+hipError_t launchContract(int kind, const void* A, const void* B, uint32_t P, uint32_t L, uint32_t V, uint32_t nt,
+                          const float* F, const double* Fd, int fsq, double fscalar, double* U, Workspace* ws,
+                          hipStream_t st);
+
+// Generic single scattering (field output): U[t][v][l] += sum_p a[t][p][l] b[t][p][v] f_p(l, v, t).
+hipError_t launchSingleGeneric(int dims, int fp64, const ScatterView& sc, const SourceView& il,
+                               const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
+                               const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st);
+
+// U += 0 helpers
+hipError_t launchZeroU(double* U, size_t n, hipStream_t st);
+
+}  // namespace sstmc

@@ -1,0 +1,382 @@
+// nee.hip -- (b) next-event estimation terms of every path of a batch.
+//
+// One wavefront per path. Lanes stride over illuminations / views (x frames), so the 64 voxel
+// walks issued together start at the same vertex and fan out over neighbouring directions: their
+// material-grid reads stay in the same cache lines. The vertex list of the path is read as
+// wave-uniform broadcasts. The per-path outputs are exactly what the reference materialises per
+// bounce before its cuBLAS reductions (NextEventEstimation.cuh:1016-1458):
+//   a[t][p][l] = g_l(x1^t) w1 exp(i 2 pi rho_1)          (single scattering, illumination side)
+//   b[t][p][v] = g_v(x1^t)                                (single scattering, view side)
+//   A[t][p][l] = g_l(x1^t) f_l(x1^t -> x2^t) w2           (multiple scattering, illumination side)
+//   B[t][p][v] = sum_{k>=2} g_v(x_k^t) f_v(x_{k-1}^t -> x_k^t) exp(i 2 pi (rho_k + L_k^t / lambda))
+// with w1 = mean_l(pr_l)^-1/2, w2 = mean_l(pr_l pdf_l)^-1/2 (pathProbabilitySingle/Multiple,
+// NextEventEstimation.cuh:853-1009). For the correlation output the same slots hold
+// a(0) conj a(t), b(0) conj b(t), A(0) conj A(t) and sum_k G_k(0) conj G_k(t).
+//
+// Precision: voxel walks, exponentials and phase functions run in R (FP32 or FP64); every
+// phase (source phase, rho, path length / lambda) is summed and reduced mod 1 in FP64, and path
+// lengths are accumulated in FP64 from the FP32 transport positions.
+#include "common.h"
+#include "geometry.h"
+#include "kernels.h"
+
+namespace sstmc {
+
+template <typename R> struct Cx;
+template <> struct Cx<float> {
+    typedef float2 T;
+    __device__ static float2 make(float a, float b) { return make_float2(a, b); }
+};
+template <> struct Cx<double> {
+    typedef double2 T;
+    __device__ static double2 make(double a, double b) { return make_double2(a, b); }
+};
+
+template <typename R> __device__ inline R sqrtR(R x);
+template <> __device__ inline float sqrtR<float>(float x) { return sqrtf(x); }
+template <> __device__ inline double sqrtR<double>(double x) { return sqrt(x); }
+template <typename R> __device__ inline R expR(R x);
+template <> __device__ inline float expR<float>(float x) { return __expf(x); }
+template <> __device__ inline double expR<double>(double x) { return exp(x); }
+template <typename R> __device__ inline void sincospiR(R x, R* s, R* c);
+template <> __device__ inline void sincospiR<float>(float x, float* s, float* c) { sincospif(x, s, c); }
+template <> __device__ inline void sincospiR<double>(double x, double* s, double* c) { sincospi(x, s, c); }
+
+// Position / direction in the walk's real type.
+template <typename R> struct Pos;
+template <> struct Pos<float> {
+    typedef f3 V;
+    __device__ static f3 pick(const f3& pf, const d3& pd) { return pf; }
+    __device__ static f3 fromD(const d3& d) { return mk((float)d.x, (float)d.y, (float)d.z); }
+};
+template <> struct Pos<double> {
+    typedef d3 V;
+    __device__ static d3 pick(const f3& pf, const d3& pd) { return pd; }
+    __device__ static d3 fromD(const d3& d) { return d; }
+};
+
+__device__ inline d3 dsub(const d3& a, const d3& b) { return mkd(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <int DIMS> __device__ inline double ddot(const d3& a, const d3& b) {
+    return DIMS == 3 ? a.x * b.x + a.y * b.y + a.z * b.z : a.x * b.x + a.y * b.y;
+}
+template <int DIMS> __device__ inline d3 dnormalize(const d3& a) {
+    double n2 = ddot<DIMS>(a, a);
+    if (n2 == 0.0) return DIMS == 3 ? mkd(0, 0, 1) : mkd(0, 1, 0);
+    double inv = 1.0 / sqrt(n2);
+    return mkd(a.x * inv, a.y * inv, a.z * inv);
+}
+
+// Scattering amplitude / pdf (Scattering.cuh:919-931, 983-995; isotropic constants
+// Simulation.cuh:266-272).
+template <int DIMS, typename R>
+__device__ inline R phasePdf(const ScatterView& s, uint32_t m, R c) {
+    if (s.type[m] == 1) return DIMS == 3 ? (R)0.079577471545947668 : (R)0.159154943091895336;
+    R g = (R)s.gd[m];
+    R up = DIMS == 3 ? ((R)1 - g * g) / (R)12.566370614359172954 : ((R)1 - g * g) / (R)6.283185307179586477;
+    R den = fma((R)-2 * g, c, (R)1 + g * g);
+    return DIMS == 3 ? up / (den * sqrtR<R>(den)) : up / den;
+}
+template <int DIMS, typename R>
+__device__ inline R phaseAmp(const ScatterView& s, uint32_t m, R c) {
+    if (s.type[m] == 1) return DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678;
+    return sqrtR<R>(phasePdf<DIMS, R>(s, m, c));
+}
+template <int DIMS, typename R> __device__ inline R isoAmp() {
+    return DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678;
+}
+
+struct Geo {
+    f3 pf;    // position in FP32 (x + delta rounded)
+    d3 pd;    // position in FP64 (x + delta exact)
+};
+
+__device__ inline Geo vertexFrame(const VertexBatch& vb, const FramesView& fr, uint64_t o, uint32_t t,
+                                  const float4& xm) {
+    Geo g;
+    if (fr.trivial) {
+        g.pf = mk(xm.x, xm.y, xm.z);
+        g.pd = mkd(xm.x, xm.y, xm.z);
+    } else {
+        const float* d = vb.delta + (o * fr.nt + t) * 3;
+        float dx = d[0], dy = d[1], dz = d[2];
+        g.pf = mk(xm.x + dx, xm.y + dy, xm.z + dz);
+        g.pd = mkd((double)xm.x + (double)dx, (double)xm.y + (double)dy, (double)xm.z + (double)dz);
+    }
+    return g;
+}
+
+// g of one source at one position: amplitude (with sqrt of the transmittance) and phase in cycles.
+// FarField: gDirectionKernel (FarField.cuh:223-263); PointSource: gPointKernel (PointSource.cuh:117-172).
+template <int DIMS, typename R>
+__device__ inline void gTerm(const MediumView& m, const SourceView& s, uint32_t i, const Geo& x, double invLambda,
+                             R tauCut, uint32_t* steps, R* amp, double* ph) {
+    typedef typename Pos<R>::V V;
+    V p = Pos<R>::pick(x.pf, x.pd);
+    if (s.type == 2) {
+        d3 dd = mkd(s.pd[3 * i], s.pd[3 * i + 1], s.pd[3 * i + 2]);
+        double phase = ddot<DIMS>(dd, x.pd) * invLambda;
+        *ph = s.illum ? phase : -phase;
+        d3 wd = s.illum ? mkd(-dd.x, -dd.y, -dd.z) : dd;
+        V w = Pos<R>::fromD(wd);
+        if (sizeof(R) == 4) w = Pos<R>::fromD(mkd(s.illum ? -(double)s.pf[3 * i] : (double)s.pf[3 * i],
+                                                  s.illum ? -(double)s.pf[3 * i + 1] : (double)s.pf[3 * i + 1],
+                                                  s.illum ? -(double)s.pf[3 * i + 2] : (double)s.pf[3 * i + 2]));
+        R tau = opticalDepth<DIMS, R>(m, p, w, tauCut, steps);
+        *amp = expR<R>(-(R)0.5 * tau);
+    } else {
+        d3 sp = mkd(s.pd[3 * i], s.pd[3 * i + 1], DIMS == 3 ? s.pd[3 * i + 2] : 0.0);
+        d3 q = dsub(sp, x.pd);
+        double r = sqrt(ddot<DIMS>(q, q));
+        double rr = 1.0 / r;
+        *ph = r * invLambda;
+        V w = Pos<R>::fromD(mkd(q.x * rr, q.y * rr, q.z * rr));
+        R tau = opticalDepth<DIMS, R>(m, p, w, (R)3e38, steps);
+        double fac = DIMS == 3 ? rr * 0.531125966013598 /* sqrt(ISOTROPIC_AMPLITUDE) */ : sqrt(rr);
+        if (s.area) fac *= sqrt(rr * fabs(DIMS == 3 ? x.pd.z - sp.z : x.pd.y - sp.y));
+        *amp = expR<R>(-(R)0.5 * tau) * (R)fac;
+    }
+}
+
+// Incoming unit direction at x1 from illumination l (FarField d_l; PointSource n(x1 - s_l)).
+template <int DIMS>
+__device__ inline d3 illumIn(const SourceView& il, uint32_t l, const d3& x1) {
+    if (il.type == 2) return mkd(il.pd[3 * l], il.pd[3 * l + 1], il.pd[3 * l + 2]);
+    return dnormalize<DIMS>(dsub(x1, mkd(il.pd[3 * l], il.pd[3 * l + 1], DIMS == 3 ? il.pd[3 * l + 2] : 0.0)));
+}
+
+template <typename T> __device__ inline T waveSum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline double frac1(double x) { return x - floor(x); }
+
+template <typename R>
+__device__ inline typename Cx<R>::T cis(R amp, double phaseCycles) {
+    R s, c;
+    sincospiR<R>((R)(2.0 * frac1(phaseCycles)), &s, &c);
+    return Cx<R>::make(amp * c, amp * s);
+}
+
+template <int DIMS, typename R, int IT>
+__global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, SourceView il, SourceView vw, FramesView fr,
+                                             VertexBatch vb, NeeParams prm, NeeOut out) {
+    typedef typename Cx<R>::T C;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (p >= prm.npaths) return;
+    const uint32_t P = prm.npaths, L = il.n, V = vw.n, NT = fr.nt;
+    const bool corr = prm.output == 1;
+    const double invLambda = 1.0 / prm.lambda;
+    const R tauCutFF = sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD;
+    C* A = (C*)out.A;
+    C* B = (C*)out.B;
+    C* a = (C*)out.a;
+    C* b = (C*)out.b;
+    // a pool overflow leaves the vertex offsets unusable; the batch is then re-run by the host
+    const bool poolOk = (uint64_t)vb.off[prm.npaths] <= vb.cap;
+    const uint32_t K = poolOk ? vb.k[p] : 0u;
+    const uint64_t off = vb.off[p];
+    const C zero = Cx<R>::make(0, 0);
+    if (K == 0) {
+        for (uint32_t i = lane; i < L * NT; i += 64) {
+            uint32_t l = i % L, t = i / L;
+            A[((size_t)t * P + p) * L + l] = zero;
+            a[((size_t)t * P + p) * L + l] = zero;
+        }
+        for (uint32_t i = lane; i < V * NT; i += 64) {
+            uint32_t v = i % V, t = i / V;
+            B[((size_t)t * P + p) * V + v] = zero;
+            b[((size_t)t * P + p) * V + v] = zero;
+        }
+        return;
+    }
+    uint32_t steps = 0;
+    const float4 x1m = vb.xm[off];
+    const uint32_t m1 = __float_as_uint(x1m.w);
+    const double rho1 = (double)vb.rho[off];
+    const bool multi = K >= 2;
+    const float4 x2m = multi ? vb.xm[off + 1] : x1m;
+    const Geo x1c = {mk(x1m.x, x1m.y, x1m.z), mkd(x1m.x, x1m.y, x1m.z)};
+    const d3 x2cd = mkd(x2m.x, x2m.y, x2m.z);
+    const d3 seg12 = dnormalize<DIMS>(dsub(x2cd, x1c.pd));
+    const R tauInf = (R)3e38;
+
+    // ---- path probabilities from the central first vertex (pathProbabilitySingle/Multiple)
+    R s1 = 0, s2 = 0;
+    for (uint32_t l = lane; l < L; l += 64) {
+        R pr;
+        if (il.type == 2) {
+            typename Pos<R>::V p1 = Pos<R>::pick(x1c.pf, x1c.pd);
+            typename Pos<R>::V w;
+            if (sizeof(R) == 4) w = Pos<R>::fromD(mkd(-(double)il.pf[3 * l], -(double)il.pf[3 * l + 1], -(double)il.pf[3 * l + 2]));
+            else w = Pos<R>::fromD(mkd(-il.pd[3 * l], -il.pd[3 * l + 1], -il.pd[3 * l + 2]));
+            R tau = opticalDepth<DIMS, R>(m, p1, w, sizeof(R) == 4 ? (R)prm.tauCutF : tauInf, &steps);
+            pr = expR<R>(-tau) * (sizeof(R) == 4 ? (R)il.probf[l] : (R)il.probd[l]);
+        } else {
+            d3 sp = mkd(il.pd[3 * l], il.pd[3 * l + 1], DIMS == 3 ? il.pd[3 * l + 2] : 0.0);
+            d3 q = dsub(sp, x1c.pd);
+            double rr = 1.0 / sqrt(ddot<DIMS>(q, q));
+            typename Pos<R>::V w = Pos<R>::fromD(mkd(q.x * rr, q.y * rr, q.z * rr));
+            R tau = opticalDepth<DIMS, R>(m, Pos<R>::pick(x1c.pf, x1c.pd), w, tauInf, &steps);
+            // PointSource.cuh:92-115 (2D factor rr * 8 pi kept as in the reference, quirk Q7)
+            double fac = DIMS == 3 ? rr * rr / 12.566370614359172954 : rr * 25.132741228718345908;
+            pr = expR<R>(-tau) * (R)fac;
+        }
+        s1 += pr;
+        if (multi) {
+            d3 in = illumIn<DIMS>(il, l, x1c.pd);
+            s2 += pr * phasePdf<DIMS, R>(sc, m1, (R)ddot<DIMS>(in, seg12));
+        }
+    }
+    s1 = waveSum(s1);
+    s2 = waveSum(s2);
+    const R p1 = s1 / (R)L, p2 = s2 / (R)L;
+    const R w1 = p1 > 0 ? (R)1 / sqrtR<R>(p1) : (R)0;
+    const R w2 = p2 > 0 ? (R)1 / sqrtR<R>(p2) : (R)0;
+    const bool isoPointIl = il.type == 1 && sc.allIsotropic;
+
+    // ---- illumination terms a (single) and A (multiple)
+    for (uint32_t i = lane; i < L * NT; i += 64) {
+        const uint32_t l = i % L, t = i / L;
+        R ga[2], fa[2] = {0, 0};
+        double gp[2];
+        const int nf = corr ? 2 : 1;
+        for (int f = 0; f < nf; ++f) {
+            uint32_t tt = (corr && f == 0) ? 0 : t;
+            Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
+            gTerm<DIMS, R>(m, il, l, x1t, invLambda, il.type == 2 ? tauCutFF : tauInf, &steps, &ga[f], &gp[f]);
+            if (multi) {
+                Geo x2t = vertexFrame(vb, fr, off + 1, tt, x2m);
+                const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
+                const d3 pb = fr.temporalAmp ? x2t.pd : x2cd;
+                if (isoPointIl) fa[f] = isoAmp<DIMS, R>();
+                else fa[f] = phaseAmp<DIMS, R>(sc, m1, (R)ddot<DIMS>(illumIn<DIMS>(il, l, pa), dnormalize<DIMS>(dsub(pb, pa))));
+            }
+        }
+        const size_t o = ((size_t)t * P + p) * L + l;
+        if (!corr) {
+            a[o] = cis<R>(ga[0] * w1, gp[0] + rho1);
+            A[o] = multi ? cis<R>(ga[0] * fa[0] * w2, gp[0]) : zero;
+        } else {
+            a[o] = cis<R>(ga[0] * ga[1] * w1 * w1, gp[0] - gp[1]);
+            A[o] = multi ? cis<R>(ga[0] * fa[0] * ga[1] * fa[1] * w2 * w2, gp[0] - gp[1]) : zero;
+        }
+    }
+
+    // ---- single-scattering view terms b = g_v(x1^t)
+    for (uint32_t i = lane; i < V * NT; i += 64) {
+        const uint32_t v = i % V, t = i / V;
+        R ga[2];
+        double gp[2];
+        const int nf = corr ? 2 : 1;
+        for (int f = 0; f < nf; ++f) {
+            uint32_t tt = (corr && f == 0) ? 0 : t;
+            Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
+            gTerm<DIMS, R>(m, vw, v, x1t, invLambda, vw.type == 2 ? tauCutFF : tauInf, &steps, &ga[f], &gp[f]);
+        }
+        const size_t o = ((size_t)t * P + p) * V + v;
+        b[o] = corr ? cis<R>(ga[0] * ga[1], gp[0] - gp[1]) : cis<R>(ga[0], gp[0]);
+    }
+
+    // ---- multiple-scattering view terms B, IT items per lane per chunk
+    const bool isoPointVw = vw.type == 1 && sc.allIsotropic;
+    const uint32_t items = V * NT;
+    for (uint32_t base = 0; base < items; base += 64 * IT) {
+        R accRe[IT], accIm[IT];
+        double len0[IT], lent[IT];
+#pragma unroll
+        for (int j = 0; j < IT; ++j) { accRe[j] = 0; accIm[j] = 0; len0[j] = 0; lent[j] = 0; }
+        for (uint32_t k = 1; k < K; ++k) {
+            const uint64_t ok = off + k, op = off + k - 1;
+            const float4 xkm = vb.xm[ok], xpm = vb.xm[op];
+            const uint32_t mk_ = __float_as_uint(xkm.w);
+            const double rhok = (double)vb.rho[ok];
+#pragma unroll
+            for (int j = 0; j < IT; ++j) {
+                const uint32_t i = base + (uint32_t)j * 64 + lane;
+                if (i >= items) continue;
+                const uint32_t v = i % V, t = i / V;
+                const int nf = corr ? 2 : 1;
+                R gr[2], gi[2];
+                for (int f = 0; f < nf; ++f) {
+                    const uint32_t tt = (corr && f == 0) ? 0 : t;
+                    Geo xk = vertexFrame(vb, fr, ok, tt, xkm);
+                    Geo xp = vertexFrame(vb, fr, op, tt, xpm);
+                    d3 sg = dsub(xk.pd, xp.pd);
+                    double ln = sqrt(ddot<DIMS>(sg, sg));
+                    double lacc;
+                    if (corr && f == 0) { len0[j] += ln; lacc = len0[j]; }
+                    else { lent[j] += ln; lacc = lent[j]; }
+                    R ga;
+                    double gp;
+                    gTerm<DIMS, R>(m, vw, v, xk, invLambda, vw.type == 2 ? tauCutFF : tauInf, &steps, &ga, &gp);
+                    R fv;
+                    if (isoPointVw) {
+                        fv = isoAmp<DIMS, R>();
+                    } else {
+                        const d3 pk = fr.temporalAmp ? xk.pd : mkd(xkm.x, xkm.y, xkm.z);
+                        const d3 pp = fr.temporalAmp ? xp.pd : mkd(xpm.x, xpm.y, xpm.z);
+                        d3 av;
+                        if (vw.type == 2) av = mkd(-vw.pd[3 * v], -vw.pd[3 * v + 1], -vw.pd[3 * v + 2]);
+                        else av = dnormalize<DIMS>(dsub(pk, mkd(vw.pd[3 * v], vw.pd[3 * v + 1], DIMS == 3 ? vw.pd[3 * v + 2] : 0.0)));
+                        fv = phaseAmp<DIMS, R>(sc, mk_, (R)ddot<DIMS>(av, dnormalize<DIMS>(dsub(pp, pk))));
+                    }
+                    typename Cx<R>::T g = cis<R>(ga * fv, gp + rhok + lacc * invLambda);
+                    gr[f] = g.x;
+                    gi[f] = g.y;
+                }
+                if (!corr) {
+                    accRe[j] += gr[0];
+                    accIm[j] += gi[0];
+                } else {  // G(0) * conj(G(t))
+                    accRe[j] += gr[0] * gr[1] + gi[0] * gi[1];
+                    accIm[j] += gi[0] * gr[1] - gr[0] * gi[1];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < IT; ++j) {
+            const uint32_t i = base + (uint32_t)j * 64 + lane;
+            if (i >= items) continue;
+            const uint32_t v = i % V, t = i / V;
+            B[((size_t)t * P + p) * V + v] = Cx<R>::make(accRe[j], accIm[j]);
+        }
+    }
+    uint32_t tot = waveSum(steps);
+    if (lane == 0 && out.steps) atomicAdd(out.steps, (unsigned long long)tot);
+}
+
+template <int DIMS, typename R>
+static void launchNeeT(int it, const MediumView& m, const ScatterView& sc, const SourceView& il, const SourceView& vw,
+                       const FramesView& fr, const VertexBatch& vb, const NeeParams& p, const NeeOut& out,
+                       hipStream_t st) {
+    dim3 grid((p.npaths + 3) / 4), block(256);
+    if (it >= 16)
+        hipLaunchKernelGGL((k_nee<DIMS, R, 16>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    else if (it >= 8)
+        hipLaunchKernelGGL((k_nee<DIMS, R, 8>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    else if (it >= 4)
+        hipLaunchKernelGGL((k_nee<DIMS, R, 4>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    else
+        hipLaunchKernelGGL((k_nee<DIMS, R, 1>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+}
+
+hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
+                     const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
+                     const NeeOut& out, hipStream_t st) {
+    // one view item per lane per pass over the path's vertices: the vertex records are re-read
+    // (wave-uniform, L1-resident) instead of holding many accumulators in registers
+    int it = 1;
+    if (dims == 2) {
+        if (fp64) launchNeeT<2, double>(it, m, sc, il, vw, fr, vb, p, out, st);
+        else launchNeeT<2, float>(it, m, sc, il, vw, fr, vb, p, out, st);
+    } else {
+        if (fp64) launchNeeT<3, double>(it, m, sc, il, vw, fr, vb, p, out, st);
+        else launchNeeT<3, float>(it, m, sc, il, vw, fr, vb, p, out, st);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace sstmc

@@ -1,0 +1,224 @@
+// tracer.h -- host-side plugin split of the reference, MI355X-native underneath.
+//
+// The classes keep the reference's Medium / Scattering / Source / Sampler / Tracer interface
+// split (MediumInterface.cuh:39-135, Scattering.cuh:32-125, SourceInterface.cuh:10-179,
+// SamplerInterface.cuh:11-176, TracerInterface.cuh:9-50): constructors take host arrays and
+// report through an `int* err` out-parameter, objects own their device memory. Instead of each
+// class launching its own small kernels with a device sync after each (about 30 launches per
+// bounce in NextEventEstimation::trace), each class exposes a device view (common.h) and the
+// tracer runs a fixed sequence of fused kernels per batch of paths on its own HIP stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/sstmc.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace sstmc {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    int alloc(size_t count);
+    int upload(const T* host, size_t count);
+    void release();
+};
+
+// ---------------------------------------------------------------- Medium
+class Medium {
+public:
+    virtual ~Medium() = default;
+    const MediumView& view() const { return v_; }
+    int dims() const { return v_.dims; }
+    double boxMin(int a) const { return v_.bmind[a]; }
+    double boxMax(int a) const { return v_.bmaxd[a]; }
+    // Medium::setMaterial (MediumInterface.cuh:386-397) for all materials at once.
+    virtual int setMaterial(const double* sigs, const double* siga, uint32_t materialNum) = 0;
+    uint32_t materialNum() const { return nmat_; }
+    double sigs(uint32_t m) const { return sigs_[m]; }
+
+protected:
+    MediumView v_{};
+    uint32_t nmat_ = 0;
+    std::vector<double> sigs_, siga_;
+};
+
+// HeterogeneousMedium (HeterogeneousMedium.cuh:52-233): rectilinear grid of material ids.
+class HeterogeneousMedium : public Medium {
+public:
+    HeterogeneousMedium(int* err, int dims, const uint32_t* materialGrid, const double* xAxis, uint32_t xAxisSize,
+                        const double* yAxis, uint32_t yAxisSize, const double* zAxis, uint32_t zAxisSize);
+    int setMaterial(const double* sigs, const double* siga, uint32_t materialNum) override;
+
+private:
+    DevBuf<float> ef_[3];
+    DevBuf<double> ed_[3];
+    DevBuf<uint8_t> grid_;
+};
+
+// ---------------------------------------------------------------- Scattering
+class Scattering {
+public:
+    explicit Scattering(int dims);
+    int setIsotropicScattering(uint32_t materialNum);
+    int setHenyeyGreensteinScattering(double g, uint32_t materialNum);
+    // Scattering::isIsotropicScattering (Scattering.cuh:1505-1527): HG with |g| < 1e-4 counts.
+    bool isIsotropicScattering() const;
+    // true when every material in [1, n) has the same phase function
+    bool isUniform(uint32_t n) const;
+    bool isExactlyIsotropic(uint32_t n) const;
+    double amplitude(uint32_t m, double cosang) const;
+    const ScatterView& view() const { return v_; }
+
+private:
+    ScatterView v_{};
+    bool defined_[kMaxMaterials] = {};
+};
+
+// ---------------------------------------------------------------- Sources
+class Source {
+public:
+    virtual ~Source() = default;
+    const SourceView& view() const { return v_; }
+    uint32_t size() const { return v_.n; }
+    int type() const { return v_.type; }
+    const std::vector<double>& hostPoints() const { return pts_; }
+
+protected:
+    SourceView v_{};
+    std::vector<double> pts_;   // normalised directions or locations, n x 3 (host)
+    DevBuf<float> pf_;
+    DevBuf<double> pd_;
+};
+
+// FarField (FarField.cuh:8-396): plane waves. For an illumination the sampling table of
+// fillBuffers (FarField.cuh:71-127) is built here on the host in FP64.
+class FarField : public Source {
+public:
+    FarField(int* err, const Medium& medium, int dims, const double* x, const double* y, const double* z,
+             uint32_t n, bool illumination);
+
+private:
+    DevBuf<float> table_, probf_;
+    DevBuf<double> probd_;
+};
+
+// PointSource (PointSource.cuh:8-216).
+class PointSource : public Source {
+public:
+    PointSource(int* err, int dims, const double* x, const double* y, const double* z, uint32_t n, bool areaSource,
+                bool illumination);
+};
+
+// ---------------------------------------------------------------- Sampler
+// TemporalSampler (TemporalSampler.cuh:55-137, time table :941-1132).
+class TemporalSampler {
+public:
+    TemporalSampler(int* err, const double* ticks, uint32_t ticksNum, bool isTemporalAmplitude);
+    int setMaterial(double D, double ux, double uy, double uz, uint32_t materialNum);
+    void finalize(uint32_t materialNum);
+    const FramesView& view() const { return v_; }
+    uint32_t frames() const { return v_.nt; }
+
+private:
+    FramesView v_{};
+    std::vector<double> dt_;
+    DevBuf<float> dtDev_;
+    std::vector<double> D_, U_;
+};
+
+// ---------------------------------------------------------------- Tracer
+struct TracerOptions {
+    uint32_t batchSize;
+    uint32_t fullIteration;
+    uint64_t seed;
+    double lambda;
+    int output;        // SSTMC_OUTPUT_*
+    int terms;         // SSTMC_TERMS_*
+    int contraction;   // SSTMC_CONTRACT_*
+};
+
+class Tracer {
+public:
+    virtual ~Tracer() = default;
+    // Tracer::trace (TracerInterface.cuh:21): u is host memory, 2*Nl*Nv*Nt doubles.
+    virtual int trace(double* u) = 0;
+    void updateSeed(uint64_t seed) { seed_ = seed; }
+    uint64_t getZeroContributionPaths() const { return zeroContributionPaths_; }
+    uint64_t getPhotonsBudget() const { return photonsBudget_; }
+
+protected:
+    uint64_t seed_ = 0;
+    uint64_t zeroContributionPaths_ = 0;
+    uint64_t photonsBudget_ = 0;
+};
+
+// NextEventEstimation (NextEventEstimation.cuh:163-1637), batch driver.
+class NextEventEstimation : public Tracer {
+public:
+    NextEventEstimation(int* err, const Medium& medium, const Scattering& scattering, const Source& illumination,
+                        const Source& view, const TemporalSampler& sampler, const TracerOptions& opt);
+    ~NextEventEstimation() override;
+    int trace(double* u) override;
+    // Asynchronous form used by the serving loop: enqueue every batch on the stream; finish()
+    // checks the device status, retries on pool overflow and copies the result out.
+    int enqueue();
+    int finish(double* u);
+    int debugBatch(uint64_t pathBegin, uint32_t npaths, uint64_t* nvtxOut, uint32_t* pathK, uint32_t* pathMisses,
+                   uint64_t* pathOff, float* vtxPos, uint32_t* vtxMat, float* vtxRho, float* vtxXt, double* vtxLen,
+                   double* termA, double* termB, double* termSa, double* termSb, double* uBatch);
+    int debugContract(uint32_t npaths, const double* a, const double* b, double* u);
+    void timing(sstmc_timing* out) const;
+    hipStream_t stream() const { return stream_; }
+    uint32_t L() const { return L_; }
+    uint32_t V() const { return V_; }
+    uint32_t T() const { return T_; }
+    uint64_t deviceBytes() const { return bytes_; }
+    uint64_t paths() const { return (uint64_t)P_ * opt_.fullIteration; }
+
+private:
+    int allocate();
+    int growPool(uint64_t cap);
+    int runBatch(uint64_t pathBegin, uint32_t npaths, double* U, bool timed, uint32_t b);
+    int contractSingle(uint32_t npaths, double* U);
+
+    const Medium& medium_;
+    const Scattering& sc_;
+    const Source& il_;
+    const Source& vw_;
+    const TemporalSampler& sampler_;
+    TracerOptions opt_;
+    int dims_;
+    uint32_t L_, V_, T_, P_;
+    int singleMode_;
+    double fscalar_ = 1.0;
+    bool fp64_;
+    int contractKind_;
+    hipStream_t stream_ = nullptr;
+    uint64_t bytes_ = 0;
+
+    DevBuf<uint32_t> k_, misses_, off_, blockSums_, status_;
+    DevBuf<unsigned long long> totals_;   // [0] misses, [1] vertices, [2] NEE voxel steps
+    DevBuf<float4> xm_;
+    DevBuf<float> rho_, delta_;
+    uint64_t cap_ = 0;
+    DevBuf<unsigned char> A_, B_, a_, b_;
+    DevBuf<double> U_;
+    DevBuf<float> Ff_;
+    DevBuf<double> Fd_;
+    Workspace wsMulti_, wsSingle_;
+    std::vector<hipEvent_t> ev_;
+    uint32_t evBatches_ = 0;
+    sstmc_timing last_{};
+};
+
+}  // namespace sstmc

@@ -1,0 +1,679 @@
+// tracer.hip -- host implementation of the plugin classes and the per-batch driver.
+#include "tracer.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace sstmc {
+
+static const double kPi = 3.14159265358979323846;
+static const double kEps = 1e-12;   // EPSILON of the FP64 reference (Simulation.cuh:36-40)
+
+template <typename T>
+int DevBuf<T>::alloc(size_t count) {
+    release();
+    if (count == 0) return SSTMC_NO_ERROR;
+    if (hipMalloc((void**)&p, count * sizeof(T)) != hipSuccess) {
+        p = nullptr;
+        return SSTMC_ALLOCATION_ERROR;
+    }
+    n = count;
+    return SSTMC_NO_ERROR;
+}
+template <typename T>
+int DevBuf<T>::upload(const T* host, size_t count) {
+    int e = alloc(count);
+    if (e) return e;
+    if (count && hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return SSTMC_ALLOCATION_ERROR;
+    return SSTMC_NO_ERROR;
+}
+template <typename T>
+void DevBuf<T>::release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+}
+template struct DevBuf<float>;
+template struct DevBuf<double>;
+template struct DevBuf<uint8_t>;
+template struct DevBuf<uint32_t>;
+template struct DevBuf<float4>;
+template struct DevBuf<unsigned long long>;
+
+// ====================================================================== HeterogeneousMedium
+HeterogeneousMedium::HeterogeneousMedium(int* err, int dims, const uint32_t* grid, const double* xAxis,
+                                         uint32_t xAxisSize, const double* yAxis, uint32_t yAxisSize,
+                                         const double* zAxis, uint32_t zAxisSize) {
+    *err = SSTMC_NO_ERROR;
+    v_.dims = dims;
+    const double* ax[3] = {xAxis, yAxis, zAxis};
+    uint32_t sz[3] = {xAxisSize, yAxisSize, dims == 3 ? zAxisSize : 2u};
+    static const double flat[2] = {0.0, 0.0};
+    for (int a = 0; a < 3; ++a) {
+        const double* e = a < dims ? ax[a] : flat;
+        uint32_t n = sz[a];
+        v_.n[a] = n - 1;
+        std::vector<float> f(n);
+        for (uint32_t i = 0; i < n; ++i) f[i] = (float)e[i];
+        if ((*err = ef_[a].upload(f.data(), n))) return;
+        if ((*err = ed_[a].upload(e, n))) return;
+        v_.ef[a] = ef_[a].p;
+        v_.ed[a] = ed_[a].p;
+        v_.bmind[a] = e[0];
+        v_.bmaxd[a] = e[n - 1];
+        v_.bminf[a] = (float)e[0];
+        v_.bmaxf[a] = (float)e[n - 1];
+        // equal spacing: the fast locate path computes the cell and corrects it against the edges
+        double h = (e[n - 1] - e[0]) / (double)(n - 1);
+        bool uni = n >= 2 && h > 0;
+        for (uint32_t i = 0; i < n && uni; ++i)
+            if (std::fabs(e[i] - (e[0] + h * i)) > 1e-9 * std::max(1.0, std::fabs(e[i]))) uni = false;
+        v_.uniformAxis[a] = uni ? 1 : 0;
+        v_.e0[a] = (float)e[0];
+        v_.invH[a] = uni ? (float)(1.0 / h) : 0.f;
+    }
+    size_t ncell = (size_t)v_.n[0] * v_.n[1] * v_.n[2];
+    std::vector<uint8_t> g(ncell);
+    for (size_t i = 0; i < ncell; ++i) g[i] = (uint8_t)grid[i];
+    if ((*err = grid_.upload(g.data(), ncell))) return;
+    v_.grid = grid_.p;
+}
+
+int HeterogeneousMedium::setMaterial(const double* sigs, const double* siga, uint32_t materialNum) {
+    nmat_ = materialNum;
+    sigs_.assign(sigs, sigs + materialNum);
+    siga_.assign(materialNum, 0.0);
+    if (siga) siga_.assign(siga, siga + materialNum);
+    // majorant over every material (HeterogeneousMedium::setMaterial, :820-860)
+    double maj = 0;
+    for (uint32_t m = 0; m < materialNum; ++m) maj = std::max(maj, sigs_[m] + siga_[m]);
+    if (!(maj > 0)) return SSTMC_INVALID_CONFIG;
+    for (uint32_t m = 0; m < (uint32_t)kMaxMaterials; ++m) {
+        double st = m < materialNum ? sigs_[m] + siga_[m] : 0.0;
+        double ss = m < materialNum ? sigs_[m] : 0.0;
+        v_.sigtf[m] = (float)st;
+        v_.sigtd[m] = st;
+        v_.accept[m] = (float)(st / maj);
+        v_.albedo[m] = st > 0 ? (float)(ss / st) : 0.f;
+    }
+    v_.invMaj = (float)(1.0 / maj);
+    return SSTMC_NO_ERROR;
+}
+
+// ====================================================================== Scattering
+Scattering::Scattering(int dims) { v_.dims = dims; }
+
+int Scattering::setIsotropicScattering(uint32_t m) {
+    if (m >= (uint32_t)kMaxMaterials) return SSTMC_INVALID_CONFIG;
+    v_.type[m] = SSTMC_SCATTERING_ISOTROPIC;
+    v_.g[m] = 0.f;
+    v_.gd[m] = 0.0;
+    defined_[m] = true;
+    v_.allIsotropic = isIsotropicScattering() ? 1 : 0;
+    return SSTMC_NO_ERROR;
+}
+
+int Scattering::setHenyeyGreensteinScattering(double g, uint32_t m) {
+    if (m >= (uint32_t)kMaxMaterials || !(std::fabs(g) < 1.0)) return SSTMC_INVALID_CONFIG;
+    v_.type[m] = SSTMC_SCATTERING_HG;
+    v_.g[m] = (float)g;
+    v_.gd[m] = g;
+    defined_[m] = true;
+    v_.allIsotropic = isIsotropicScattering() ? 1 : 0;
+    return SSTMC_NO_ERROR;
+}
+
+bool Scattering::isIsotropicScattering() const {
+    for (int m = 0; m < kMaxMaterials; ++m) {
+        if (!defined_[m]) continue;
+        if (v_.type[m] == SSTMC_SCATTERING_HG && !(std::fabs(v_.gd[m]) < 0.0001)) return false;
+    }
+    return true;
+}
+
+bool Scattering::isUniform(uint32_t n) const {
+    for (uint32_t m = 2; m < n; ++m)
+        if (v_.type[m] != v_.type[1] || (v_.type[m] == SSTMC_SCATTERING_HG && v_.gd[m] != v_.gd[1])) return false;
+    return true;
+}
+
+bool Scattering::isExactlyIsotropic(uint32_t n) const {
+    for (uint32_t m = 1; m < n; ++m)
+        if (!(v_.type[m] == SSTMC_SCATTERING_ISOTROPIC || v_.gd[m] == 0.0)) return false;
+    return true;
+}
+
+double Scattering::amplitude(uint32_t m, double c) const {
+    double iso = v_.dims == 3 ? 0.282094791773878143 : 0.398942280401432678;
+    if (v_.type[m] == SSTMC_SCATTERING_ISOTROPIC) return iso;
+    double g = v_.gd[m];
+    double up = v_.dims == 3 ? (1.0 - g * g) / (4.0 * kPi) : (1.0 - g * g) / (2.0 * kPi);
+    double den = std::fma(-2.0 * g, c, 1.0 + g * g);
+    return std::sqrt(v_.dims == 3 ? up / (den * std::sqrt(den)) : up / den);
+}
+
+// ====================================================================== Sources
+static void normalize(int dims, double* v) {
+    double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + (dims == 3 ? v[2] * v[2] : 0.0));
+    if (n == 0) {
+        v[0] = 0; v[1] = dims == 3 ? 0 : 1; v[2] = dims == 3 ? 1 : 0;
+        return;
+    }
+    v[0] /= n; v[1] /= n; v[2] = dims == 3 ? v[2] / n : 0.0;
+}
+
+// projectBoxToLine (MediumInterface.cuh:156-199): extent of the box corners projected on a line.
+static void projectBox(const Medium& m, int dims, const double* p0, const double* v, double* tmin, double* tmax) {
+    int nc = dims == 3 ? 8 : 4;
+    for (int c = 0; c < nc; ++c) {
+        double p[3] = {(c & 1) ? m.boxMax(0) : m.boxMin(0), (c & 2) ? m.boxMax(1) : m.boxMin(1),
+                       (c & 4) ? m.boxMax(2) : m.boxMin(2)};
+        double q[3] = {p[0] - p0[0], p[1] - p0[1], p[2] - p0[2]};
+        double o[3] = {0, 0, 0};
+        for (int a = 0; a < dims; ++a)
+            for (int b = 0; b < dims; ++b) o[a] += v[a] * v[b] * q[b];
+        double t = 0;
+        for (int a = 0; a < dims; ++a) t += o[a] * v[a];
+        if (c == 0 || t < *tmin) *tmin = t;
+        if (c == 0 || t > *tmax) *tmax = t;
+    }
+}
+
+FarField::FarField(int* err, const Medium& med, int dims, const double* x, const double* y, const double* z,
+                   uint32_t n, bool illumination) {
+    *err = SSTMC_NO_ERROR;
+    v_.type = SSTMC_SOURCE_FARFIELD;
+    v_.illum = illumination ? 1 : 0;
+    v_.n = n;
+    pts_.assign(3 * (size_t)n, 0.0);
+    std::vector<float> pf(3 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        double d[3] = {x[i], y[i], dims == 3 ? z[i] : 0.0};
+        normalize(dims, d);
+        for (int a = 0; a < 3; ++a) { pts_[3 * i + a] = d[a]; pf[3 * i + a] = (float)d[a]; }
+    }
+    if ((*err = pd_.upload(pts_.data(), pts_.size()))) return;
+    if ((*err = pf_.upload(pf.data(), pf.size()))) return;
+    v_.pf = pf_.p;
+    v_.pd = pd_.p;
+    if (!illumination) return;
+    // fillBuffers (FarField.cuh:71-127)
+    double r[3], ctr[3], rmax = -1e300;
+    for (int a = 0; a < 3; ++a) {
+        r[a] = 0.5 * (med.boxMax(a) - med.boxMin(a));
+        ctr[a] = 0.5 * (med.boxMax(a) + med.boxMin(a));
+        if (a < dims) rmax = std::max(rmax, r[a]);
+    }
+    double rad = rmax + kEps;
+    std::vector<float> tab((size_t)n * kFarFieldTable, 0.f);
+    std::vector<float> probf(n);
+    std::vector<double> probd(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* d = &pts_[3 * i];
+        double opp[3] = {-d[0], -d[1], -d[2]};
+        double tang[3] = {ctr[0] + rad * opp[0], ctr[1] + rad * opp[1], dims == 3 ? ctr[2] + rad * opp[2] : 0.0};
+        double T[3] = {0, 0, 0}, S[3] = {0, 0, 0};
+        double minT = 0, maxT = 0, minS = 0, maxS = 0, prob;
+        if (dims == 3) {
+            if (std::fabs(1.0 - std::fabs(opp[2])) <= kEps) {
+                T[0] = 1; S[1] = 1;
+            } else {
+                T[0] = -opp[1]; T[1] = opp[0]; T[2] = 0;
+                normalize(3, T);
+                S[0] = T[1] * opp[2] - T[2] * opp[1];
+                S[1] = T[2] * opp[0] - T[0] * opp[2];
+                S[2] = T[0] * opp[1] - T[1] * opp[0];
+            }
+            projectBox(med, dims, tang, T, &minT, &maxT);
+            projectBox(med, dims, tang, S, &minS, &maxS);
+            prob = 1.0 / ((maxT - minT) * (maxS - minS));
+        } else {
+            T[0] = -opp[1]; T[1] = opp[0];
+            projectBox(med, dims, tang, T, &minT, &maxT);
+            prob = 1.0 / (maxT - minT);
+        }
+        float* t = &tab[(size_t)i * kFarFieldTable];
+        for (int a = 0; a < 3; ++a) {
+            t[a] = (float)tang[a];
+            t[3 + a] = (float)T[a];
+            t[6 + a] = (float)S[a];
+            t[14 + a] = (float)(std::fabs(d[a]) < kEps ? 1e12 : 1.0 / d[a]);
+            t[17 + a] = (float)d[a];
+        }
+        t[9] = (float)minT; t[10] = (float)maxT; t[11] = (float)minS; t[12] = (float)maxS;
+        probf[i] = (float)prob;
+        probd[i] = prob;
+    }
+    if ((*err = table_.upload(tab.data(), tab.size()))) return;
+    if ((*err = probf_.upload(probf.data(), n))) return;
+    if ((*err = probd_.upload(probd.data(), n))) return;
+    v_.table = table_.p;
+    v_.probf = probf_.p;
+    v_.probd = probd_.p;
+}
+
+PointSource::PointSource(int* err, int dims, const double* x, const double* y, const double* z, uint32_t n,
+                         bool areaSource, bool illumination) {
+    *err = SSTMC_NO_ERROR;
+    v_.type = SSTMC_SOURCE_POINT;
+    v_.illum = illumination ? 1 : 0;
+    v_.area = areaSource ? 1 : 0;
+    v_.n = n;
+    pts_.assign(3 * (size_t)n, 0.0);
+    std::vector<float> pf(3 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        pts_[3 * i] = x[i];
+        pts_[3 * i + 1] = y[i];
+        pts_[3 * i + 2] = dims == 3 ? z[i] : 0.0;
+        for (int a = 0; a < 3; ++a) pf[3 * i + a] = (float)pts_[3 * i + a];
+    }
+    if ((*err = pd_.upload(pts_.data(), pts_.size()))) return;
+    if ((*err = pf_.upload(pf.data(), pf.size()))) return;
+    v_.pf = pf_.p;
+    v_.pd = pd_.p;
+}
+
+// ====================================================================== TemporalSampler
+TemporalSampler::TemporalSampler(int* err, const double* t, uint32_t nt, bool isTemporalAmplitude) {
+    *err = SSTMC_NO_ERROR;
+    v_.nt = nt;
+    v_.temporalAmp = isTemporalAmplitude ? 1 : 0;
+    // time table of the TemporalSampler constructor (TemporalSampler.cuh:941-1132): frames below
+    // the mean form the negative chain, the rest the positive chain, each ordered by distance to
+    // the mean; a slot's step is its time minus the previous slot's (or minus the mean).
+    double mean = 0;
+    for (uint32_t i = 0; i < nt; ++i) mean += t[i];
+    mean /= nt;
+    std::vector<int> used(nt, 0);
+    std::vector<uint32_t> order;
+    uint32_t neg = 0;
+    for (uint32_t i = 0; i < nt; ++i) neg += (t[i] - mean < 0) ? 1u : 0u;
+    for (int pass = 0; pass < 2; ++pass) {
+        bool negative = pass == 0;
+        uint32_t cnt = negative ? neg : nt - neg;
+        for (uint32_t c = 0; c < cnt; ++c) {
+            bool first = true;
+            double best = 0;
+            uint32_t bi = 0;
+            for (uint32_t i = 0; i < nt; ++i) {
+                if (used[i] || ((t[i] - mean < 0) != negative)) continue;
+                double dd = negative ? mean - t[i] : t[i] - mean;
+                if (first || dd < best) { first = false; best = dd; bi = i; }
+            }
+            used[bi] = 1;
+            order.push_back(bi);
+        }
+    }
+    v_.neg = neg;
+    dt_.assign(nt, 0.0);
+    int prev = -1;
+    for (uint32_t k = 0; k < nt; ++k) {
+        if (k == 0 || k == neg) prev = -1;
+        int tick = (int)order[k];
+        dt_[k] = prev < 0 ? t[tick] - mean : t[tick] - t[prev];
+        prev = tick;
+    }
+    std::vector<float> f(nt);
+    for (uint32_t k = 0; k < nt; ++k) f[k] = (float)dt_[k];
+    if ((*err = dtDev_.upload(f.data(), nt))) return;
+    v_.dt = dtDev_.p;
+    D_.assign(kMaxMaterials, 0.0);
+    U_.assign(3 * kMaxMaterials, 0.0);
+}
+
+int TemporalSampler::setMaterial(double D, double ux, double uy, double uz, uint32_t m) {
+    if (m >= (uint32_t)kMaxMaterials || D < 0) return SSTMC_INVALID_CONFIG;
+    D_[m] = D;
+    U_[3 * m] = ux; U_[3 * m + 1] = uy; U_[3 * m + 2] = uz;
+    v_.D[m] = (float)D;
+    v_.U[m][0] = (float)ux; v_.U[m][1] = (float)uy; v_.U[m][2] = (float)uz;
+    return SSTMC_NO_ERROR;
+}
+
+void TemporalSampler::finalize(uint32_t materialNum) {
+    bool trivial = true;
+    for (uint32_t k = 0; k < v_.nt; ++k)
+        for (uint32_t m = 1; m < materialNum; ++m) {
+            if (dt_[k] != 0.0 && (D_[m] != 0.0 || U_[3 * m] != 0.0 || U_[3 * m + 1] != 0.0 || U_[3 * m + 2] != 0.0))
+                trivial = false;
+        }
+    v_.trivial = trivial ? 1 : 0;
+}
+
+// ====================================================================== NextEventEstimation
+NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const Scattering& scattering,
+                                         const Source& illumination, const Source& view,
+                                         const TemporalSampler& sampler, const TracerOptions& opt)
+    : medium_(medium), sc_(scattering), il_(illumination), vw_(view), sampler_(sampler), opt_(opt) {
+    seed_ = opt.seed;
+    dims_ = medium.dims();
+    L_ = il_.size();
+    V_ = vw_.size();
+    T_ = sampler_.frames();
+    P_ = opt.batchSize;
+    fp64_ = opt.terms == SSTMC_TERMS_FP64;
+    contractKind_ = fp64_ ? kContractFp64
+                          : (opt.contraction == SSTMC_CONTRACT_FP32 || opt.contraction == SSTMC_CONTRACT_FP64
+                                 ? kContractFp32 : kContractBf16x3);
+    // single scattering: separable when f does not depend on the first vertex
+    // (threePointFunctionSingle, FarField.cuh:546-599 / PointSource.cuh:363-446)
+    uint32_t nm = medium.materialNum();
+    double iso = dims_ == 3 ? 0.282094791773878143 : 0.398942280401432678;
+    singleMode_ = kSingleGeneric;
+    bool needF = false;
+    if (il_.type() == SSTMC_SOURCE_POINT && sc_.isIsotropicScattering()) {
+        singleMode_ = kSingleSeparable;
+        fscalar_ = iso;
+    } else if (il_.type() == SSTMC_SOURCE_FARFIELD && vw_.type() == SSTMC_SOURCE_FARFIELD && sc_.isUniform(nm)) {
+        singleMode_ = kSingleSeparable;
+        needF = true;
+    } else if (sc_.isExactlyIsotropic(nm)) {
+        singleMode_ = kSingleSeparable;
+        fscalar_ = iso;
+    }
+    if (opt_.output == SSTMC_OUTPUT_CORRELATION) {
+        if (singleMode_ != kSingleSeparable) { *err = SSTMC_NOT_SUPPORTED; return; }
+        fscalar_ *= fscalar_;
+    }
+    if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) { *err = SSTMC_DEVICE_ERROR; return; }
+    if ((*err = allocate())) return;
+    if (needF) {
+        std::vector<float> ff((size_t)L_ * V_);
+        std::vector<double> fd((size_t)L_ * V_);
+        const std::vector<double>& dl = il_.hostPoints();
+        const std::vector<double>& dv = vw_.hostPoints();
+        for (uint32_t v = 0; v < V_; ++v)
+            for (uint32_t l = 0; l < L_; ++l) {
+                double c = dl[3 * l] * dv[3 * v] + dl[3 * l + 1] * dv[3 * v + 1] + dl[3 * l + 2] * dv[3 * v + 2];
+                double f = sc_.amplitude(1, c);
+                fd[(size_t)v * L_ + l] = f;
+                ff[(size_t)v * L_ + l] = (float)f;
+            }
+        if ((*err = Ff_.upload(ff.data(), ff.size()))) return;
+        if ((*err = Fd_.upload(fd.data(), fd.size()))) return;
+    }
+    *err = SSTMC_NO_ERROR;
+}
+
+NextEventEstimation::~NextEventEstimation() {
+    if (stream_) hipStreamSynchronize(stream_);
+    for (hipEvent_t e : ev_) hipEventDestroy(e);
+    if (wsMulti_.ptr) hipFree(wsMulti_.ptr);
+    if (wsSingle_.ptr) hipFree(wsSingle_.ptr);
+    if (stream_) hipStreamDestroy(stream_);
+}
+
+int NextEventEstimation::allocate() {
+    int e;
+    if ((e = k_.alloc(P_)) || (e = misses_.alloc(P_)) || (e = off_.alloc(P_ + 1)) ||
+        (e = blockSums_.alloc((P_ + 1023) / 1024 + 1)) || (e = status_.alloc(4)) || (e = totals_.alloc(4)))
+        return e;
+    size_t elem = fp64_ ? 16 : 8;
+    size_t na = (size_t)T_ * P_ * L_ * elem, nb = (size_t)T_ * P_ * V_ * elem;
+    if ((e = A_.alloc(na)) || (e = a_.alloc(na)) || (e = B_.alloc(nb)) || (e = b_.alloc(nb))) return e;
+    if ((e = U_.alloc(2 * (size_t)T_ * L_ * V_))) return e;
+    bytes_ = 2 * (na + nb) + 16 * (size_t)T_ * L_ * V_ + 20 * (size_t)P_;
+    return growPool((uint64_t)P_ * 32);
+}
+
+int NextEventEstimation::growPool(uint64_t cap) {
+    int e;
+    if ((e = xm_.alloc(cap)) || (e = rho_.alloc(cap))) return e;
+    if (!sampler_.view().trivial) {
+        if ((e = delta_.alloc(cap * T_ * 3))) return e;
+    } else {
+        delta_.release();
+    }
+    bytes_ += cap * (20 + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_)) - cap_ * 20;
+    cap_ = cap;
+    return SSTMC_NO_ERROR;
+}
+
+int NextEventEstimation::contractSingle(uint32_t npaths, double* U) {
+    FramesView fr = sampler_.view();
+    VertexBatch vb{k_.p, misses_.p, off_.p, xm_.p, rho_.p, delta_.p, cap_, status_.p};
+    hipError_t he;
+    if (singleMode_ == kSingleSeparable) {
+        he = launchContract(contractKind_, a_.p, b_.p, npaths, L_, V_, T_, Ff_.p, Fd_.p,
+                            opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0, fscalar_, U, &wsSingle_, stream_);
+    } else {
+        he = launchSingleGeneric(dims_, fp64_ ? 1 : 0, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p, npaths, U,
+                                 &wsSingle_, stream_);
+    }
+    return he == hipSuccess ? SSTMC_NO_ERROR : SSTMC_KERNEL_ERROR;
+}
+
+int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U, bool timed, uint32_t b) {
+    const MediumView& m = medium_.view();
+    const ScatterView& sc = sc_.view();
+    const SourceView& il = il_.view();
+    const SourceView& vw = vw_.view();
+    const FramesView& fr = sampler_.view();
+    VertexBatch vb{k_.p, misses_.p, off_.p, xm_.p, rho_.p, delta_.p, cap_, status_.p};
+    hipEvent_t* e = timed ? &ev_[(size_t)b * 5] : nullptr;
+    if (e) hipEventRecord(e[0], stream_);
+    if (launchTransport(dims_, m, sc, il, fr, vb, seed_, pathBegin, npaths, 0, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    if (launchScan(k_.p, off_.p, blockSums_.p, npaths, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    if (launchTransport(dims_, m, sc, il, fr, vb, seed_, pathBegin, npaths, 1, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    if (launchBatchTotals(misses_.p, off_.p, npaths, totals_.p, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    if (e) hipEventRecord(e[1], stream_);
+    NeeParams prm;
+    prm.lambda = opt_.lambda;
+    prm.output = opt_.output;
+    prm.singleMode = singleMode_;
+    prm.tauCutF = 210.0f;     // exp(-105) rounds to zero in FP32: far-field walks stop there
+    prm.tauCutD = 1.0e300;
+    prm.npaths = npaths;
+    NeeOut out{A_.p, B_.p, a_.p, b_.p, totals_.p + 2};
+    if (launchNee(dims_, fp64_ ? 1 : 0, m, sc, il, vw, fr, vb, prm, out, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    if (e) hipEventRecord(e[2], stream_);
+    if (launchContract(contractKind_, A_.p, B_.p, npaths, L_, V_, T_, nullptr, nullptr, 0, 1.0, U, &wsMulti_, stream_) !=
+        hipSuccess)
+        return SSTMC_KERNEL_ERROR;
+    if (e) hipEventRecord(e[3], stream_);
+    int rc = contractSingle(npaths, U);
+    if (rc) return rc;
+    if (e) hipEventRecord(e[4], stream_);
+    return SSTMC_NO_ERROR;
+}
+
+int NextEventEstimation::enqueue() {
+    uint32_t nb = opt_.fullIteration;
+    if (evBatches_ < nb) {
+        for (hipEvent_t ev : ev_) hipEventDestroy(ev);
+        ev_.assign((size_t)nb * 5, nullptr);
+        for (auto& ev : ev_)
+            if (hipEventCreate(&ev) != hipSuccess) return SSTMC_DEVICE_ERROR;
+        evBatches_ = nb;
+    }
+    if (hipMemsetAsync(status_.p, 0, 4 * sizeof(uint32_t), stream_) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    if (hipMemsetAsync(totals_.p, 0, 4 * sizeof(unsigned long long), stream_) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    if (launchZeroU(U_.p, U_.n, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    for (uint32_t b = 0; b < nb; ++b) {
+        int rc = runBatch((uint64_t)b * P_, P_, U_.p, true, b);
+        if (rc) return rc;
+    }
+    return SSTMC_NO_ERROR;
+}
+
+int NextEventEstimation::finish(double* u) {
+    if (hipStreamSynchronize(stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    uint32_t st[4];
+    unsigned long long tot[4];
+    if (hipMemcpy(st, status_.p, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    if (hipMemcpy(tot, totals_.p, sizeof(tot), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    if (st[1]) return SSTMC_NO_HIT;
+    if (st[0] & 2u) return SSTMC_KERNEL_ERROR;   // count and write passes disagreed
+    if (st[0] & 1u) return SSTMC_NOT_READY;      // pool overflow: caller grows and retries
+    zeroContributionPaths_ = tot[0];
+    photonsBudget_ = tot[1];
+    if (u && hipMemcpy(u, U_.p, U_.n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    // stage timing from the per-batch events
+    sstmc_timing t{};
+    for (uint32_t b = 0; b < opt_.fullIteration; ++b) {
+        float ms;
+        hipEvent_t* e = &ev_[(size_t)b * 5];
+        if (hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess) t.transport_ms += ms;
+        if (hipEventElapsedTime(&ms, e[1], e[2]) == hipSuccess) t.nee_ms += ms;
+        if (hipEventElapsedTime(&ms, e[2], e[3]) == hipSuccess) t.contract_ms += ms;
+        if (hipEventElapsedTime(&ms, e[3], e[4]) == hipSuccess) t.single_ms += ms;
+    }
+    float ms;
+    if (opt_.fullIteration && hipEventElapsedTime(&ms, ev_[0], ev_[(size_t)(opt_.fullIteration - 1) * 5 + 4]) == hipSuccess)
+        t.total_ms = ms;
+    t.vertices = tot[1];
+    t.dda_steps = tot[2];
+    t.batches = opt_.fullIteration;
+    last_ = t;
+    return SSTMC_NO_ERROR;
+}
+
+int NextEventEstimation::trace(double* u) {
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        int rc = enqueue();
+        if (rc) return rc;
+        rc = finish(u);
+        if (rc != SSTMC_NOT_READY) return rc;
+        // the vertex pool overflowed in some batch: size it from the largest batch and redo
+        unsigned long long tot[4];
+        if (hipMemcpy(tot, totals_.p, sizeof(tot), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
+        uint64_t want = std::max<uint64_t>(cap_ * 2, (uint64_t)tot[3] + tot[3] / 4 + 1024);
+        if ((rc = growPool(want))) return rc;
+    }
+    return SSTMC_ALLOCATION_ERROR;
+}
+
+void NextEventEstimation::timing(sstmc_timing* out) const { *out = last_; }
+
+int NextEventEstimation::debugBatch(uint64_t pathBegin, uint32_t n, uint64_t* nvtxOut, uint32_t* pathK,
+                                    uint32_t* pathMisses, uint64_t* pathOff, float* vtxPos, uint32_t* vtxMat,
+                                    float* vtxRho, float* vtxXt, double* vtxLen, double* termA, double* termB,
+                                    double* termSa, double* termSb, double* uBatch) {
+    if (n == 0 || n > P_) return SSTMC_INVALID_CONFIG;
+    DevBuf<double> Ub;
+    int rc;
+    if ((rc = Ub.alloc(U_.n))) return rc;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        hipMemsetAsync(status_.p, 0, 4 * sizeof(uint32_t), stream_);
+        hipMemsetAsync(totals_.p, 0, 4 * sizeof(unsigned long long), stream_);
+        launchZeroU(Ub.p, Ub.n, stream_);
+        if ((rc = runBatch(pathBegin, n, Ub.p, false, 0))) return rc;
+        if (hipStreamSynchronize(stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+        uint32_t st[4];
+        unsigned long long tot[4];
+        hipMemcpy(st, status_.p, sizeof(st), hipMemcpyDeviceToHost);
+        hipMemcpy(tot, totals_.p, sizeof(tot), hipMemcpyDeviceToHost);
+        if (st[1]) return SSTMC_NO_HIT;
+        if (st[0] & 2u) return SSTMC_KERNEL_ERROR;
+        if (st[0] & 1u) {
+            if ((rc = growPool(std::max<uint64_t>(cap_ * 2, tot[3] + tot[3] / 4 + 1024)))) return rc;
+            continue;
+        }
+        std::vector<uint32_t> k(n), ms(n), off(n + 1);
+        hipMemcpy(k.data(), k_.p, n * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(ms.data(), misses_.p, n * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(off.data(), off_.p, (n + 1) * 4, hipMemcpyDeviceToHost);
+        uint64_t nv = off[n];
+        if (nvtxOut) *nvtxOut = nv;
+        if (!pathK) return SSTMC_NO_ERROR;
+        for (uint32_t i = 0; i < n; ++i) { pathK[i] = k[i]; pathMisses[i] = ms[i]; pathOff[i] = off[i]; }
+        pathOff[n] = nv;
+        std::vector<float4> xm(nv);
+        std::vector<float> rho(nv), dl;
+        hipMemcpy(xm.data(), xm_.p, nv * sizeof(float4), hipMemcpyDeviceToHost);
+        hipMemcpy(rho.data(), rho_.p, nv * 4, hipMemcpyDeviceToHost);
+        const FramesView& fr = sampler_.view();
+        if (!fr.trivial) {
+            dl.resize(nv * T_ * 3);
+            hipMemcpy(dl.data(), delta_.p, dl.size() * 4, hipMemcpyDeviceToHost);
+        }
+        for (uint64_t o = 0; o < nv; ++o) {
+            vtxPos[3 * o] = xm[o].x; vtxPos[3 * o + 1] = xm[o].y; vtxPos[3 * o + 2] = xm[o].z;
+            uint32_t mb;
+            std::memcpy(&mb, &xm[o].w, 4);
+            vtxMat[o] = mb;
+            vtxRho[o] = rho[o];
+            for (uint32_t t = 0; t < T_; ++t)
+                for (int a = 0; a < 3; ++a)
+                    vtxXt[(o * T_ + t) * 3 + a] = fr.trivial ? 0.f : dl[(o * T_ + t) * 3 + a];
+        }
+        // path length from x1 per frame, accumulated in FP64 from the FP32 transport positions
+        // exactly as the NEE kernel does (vtxXt holds the per-frame displacement)
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t t = 0; t < T_; ++t) {
+                double acc = 0;
+                for (uint32_t j = 0; j < k[i]; ++j) {
+                    uint64_t o = off[i] + j;
+                    if (j > 0) {
+                        double s2 = 0;
+                        for (int a = 0; a < dims_; ++a) {
+                            double xa = (double)vtxPos[3 * o + a] + (double)vtxXt[(o * T_ + t) * 3 + a];
+                            double xb = (double)vtxPos[3 * (o - 1) + a] + (double)vtxXt[((o - 1) * T_ + t) * 3 + a];
+                            s2 += (xa - xb) * (xa - xb);
+                        }
+                        acc += std::sqrt(s2);
+                    }
+                    vtxLen[o * T_ + t] = acc;
+                }
+            }
+        // per-path terms, from [t][p][x] device layout to [p][t][x]
+        size_t elem = fp64_ ? 16 : 8;
+        auto fetch = [&](const DevBuf<unsigned char>& src, uint32_t width, double* dst) {
+            std::vector<unsigned char> h((size_t)T_ * n * width * elem);
+            hipMemcpy(h.data(), src.p, h.size(), hipMemcpyDeviceToHost);
+            for (uint32_t t = 0; t < T_; ++t)
+                for (uint32_t p = 0; p < n; ++p)
+                    for (uint32_t x = 0; x < width; ++x) {
+                        size_t si = ((size_t)t * n + p) * width + x;
+                        size_t di = (((size_t)p * T_ + t) * width + x) * 2;
+                        if (fp64_) {
+                            const double* s = (const double*)h.data();
+                            dst[di] = s[2 * si]; dst[di + 1] = s[2 * si + 1];
+                        } else {
+                            const float* s = (const float*)h.data();
+                            dst[di] = s[2 * si]; dst[di + 1] = s[2 * si + 1];
+                        }
+                    }
+        };
+        if (termA) fetch(A_, L_, termA);
+        if (termB) fetch(B_, V_, termB);
+        if (termSa) fetch(a_, L_, termSa);
+        if (termSb) fetch(b_, V_, termSb);
+        if (uBatch) hipMemcpy(uBatch, Ub.p, Ub.n * sizeof(double), hipMemcpyDeviceToHost);
+        return SSTMC_NO_ERROR;
+    }
+    return SSTMC_ALLOCATION_ERROR;
+}
+
+int NextEventEstimation::debugContract(uint32_t n, const double* a, const double* b, double* u) {
+    if (n == 0) return SSTMC_INVALID_CONFIG;
+    size_t na = (size_t)T_ * n * L_, nb = (size_t)T_ * n * V_;
+    DevBuf<unsigned char> da, db;
+    DevBuf<double> du;
+    int rc;
+    if ((rc = du.alloc(U_.n))) return rc;
+    if (contractKind_ == kContractFp64) {
+        std::vector<double> ha(2 * na), hb(2 * nb);
+        for (size_t i = 0; i < 2 * na; ++i) ha[i] = a[i];
+        for (size_t i = 0; i < 2 * nb; ++i) hb[i] = b[i];
+        if ((rc = da.upload((const unsigned char*)ha.data(), ha.size() * 8))) return rc;
+        if ((rc = db.upload((const unsigned char*)hb.data(), hb.size() * 8))) return rc;
+    } else {
+        std::vector<float> ha(2 * na), hb(2 * nb);
+        for (size_t i = 0; i < 2 * na; ++i) ha[i] = (float)a[i];
+        for (size_t i = 0; i < 2 * nb; ++i) hb[i] = (float)b[i];
+        if ((rc = da.upload((const unsigned char*)ha.data(), ha.size() * 4))) return rc;
+        if ((rc = db.upload((const unsigned char*)hb.data(), hb.size() * 4))) return rc;
+    }
+    launchZeroU(du.p, du.n, stream_);
+    if (launchContract(contractKind_, da.p, db.p, n, L_, V_, T_, nullptr, nullptr, 0, 1.0, du.p, &wsMulti_, stream_) !=
+        hipSuccess)
+        return SSTMC_KERNEL_ERROR;
+    if (hipStreamSynchronize(stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    if (hipMemcpy(u, du.p, du.n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    return SSTMC_NO_ERROR;
+}
+
+}  // namespace sstmc

@@ -1,0 +1,157 @@
+"""ctypes binding of the FP64 CPU restatement in oracle/ (test infrastructure only)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from temporalspecklescode_amd.config import CConfig, sizes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "_build", "libsstmc_oracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        _lib = C.CDLL(LIB_PATH)
+        _lib.oracle_render.restype = C.c_int
+        _lib.oracle_paths.restype = C.c_int
+        _lib.oracle_terms_from_vertices.restype = C.c_int
+        _lib.oracle_optical_depth.restype = C.c_double
+        _lib.oracle_phase_amplitude.restype = C.c_double
+        _lib.oracle_phase_amplitude.argtypes = [C.c_void_p, C.c_uint32, C.c_double, C.c_int]
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def render(cfg: dict, dims: int, nthreads: int = 0):
+    """One rendering: returns (u[l, v, t] complex128, iter, vertices)."""
+    cc = CConfig(cfg, dims)
+    nl, nv, nt, _ = sizes(cfg, dims)
+    u = np.zeros(2 * nl * nv * nt)
+    it = C.c_double()
+    nvtx = C.c_uint64()
+    rc = lib().oracle_render(cc.ptr(), _p(u), C.byref(it), C.byref(nvtx), C.c_int(nthreads))
+    if rc:
+        raise RuntimeError("oracle_render failed with code %d" % rc)
+    uc = (u[0::2] + 1j * u[1::2]).reshape(nt, nv, nl).transpose(2, 1, 0)
+    return uc, it.value, nvtx.value
+
+
+def paths(cfg: dict, dims: int, begin: int, n: int):
+    """Per-path vertices and terms for global path ids [begin, begin+n)."""
+    cc = CConfig(cfg, dims)
+    nl, nv, nt, _ = sizes(cfg, dims)
+    nvtx = C.c_uint64()
+    L = lib()
+    rc = L.oracle_paths(cc.ptr(), C.c_uint64(begin), C.c_uint32(n), C.byref(nvtx), *([None] * 13))
+    if rc:
+        raise RuntimeError("oracle_paths failed with code %d" % rc)
+    m = nvtx.value
+    out = dict(
+        k=np.zeros(n, np.uint32), misses=np.zeros(n, np.uint32), off=np.zeros(n + 1, np.uint64),
+        pos=np.zeros((m, 3)), mat=np.zeros(m, np.uint32), rho=np.zeros(m), xt=np.zeros((m, nt, 3)),
+        len=np.zeros((m, nt)), a=np.zeros((n, nt, nl, 2)), b=np.zeros((n, nt, nv, 2)),
+        s=np.zeros((n, nt, nv, nl, 2)))
+    rc = L.oracle_paths(cc.ptr(), C.c_uint64(begin), C.c_uint32(n), C.byref(nvtx),
+                        _p(out["k"]), _p(out["misses"]), _p(out["off"]), _p(out["pos"]), _p(out["mat"]),
+                        _p(out["rho"]), _p(out["xt"]), _p(out["len"]), _p(out["a"]), _p(out["b"]), _p(out["s"]))
+    if rc:
+        raise RuntimeError("oracle_paths failed with code %d" % rc)
+    for key in ("a", "b", "s"):
+        out[key] = out[key][..., 0] + 1j * out[key][..., 1]
+    return out
+
+
+def terms_from_vertices(cfg: dict, dims: int, k, off, pos, mat, rho, xt, ln):
+    """FP64 NEE terms for given vertices (positions as float64)."""
+    cc = CConfig(cfg, dims)
+    nl, nv, nt, _ = sizes(cfg, dims)
+    n = len(k)
+    k = np.ascontiguousarray(k, np.uint32)
+    off = np.ascontiguousarray(off, np.uint64)
+    pos = np.ascontiguousarray(pos, np.float64)
+    mat = np.ascontiguousarray(mat, np.uint32)
+    rho = np.ascontiguousarray(rho, np.float64)
+    xt = np.ascontiguousarray(xt, np.float64)
+    ln = np.ascontiguousarray(ln, np.float64)
+    a = np.zeros((n, nt, nl, 2)); b = np.zeros((n, nt, nv, 2)); s = np.zeros((n, nt, nv, nl, 2))
+    rc = lib().oracle_terms_from_vertices(cc.ptr(), C.c_uint32(n), _p(k), _p(off), _p(pos), _p(mat), _p(rho),
+                                          _p(xt), _p(ln), _p(a), _p(b), _p(s))
+    if rc:
+        raise RuntimeError("oracle_terms_from_vertices failed with code %d" % rc)
+    return a[..., 0] + 1j * a[..., 1], b[..., 0] + 1j * b[..., 1], s[..., 0] + 1j * s[..., 1]
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().oracle_philox(c, k, o)
+    return list(o)
+
+
+def optical_depth(cfg, dims, p, d):
+    cc = CConfig(cfg, dims)
+    pa = np.ascontiguousarray(p, np.float64)
+    da = np.ascontiguousarray(d, np.float64)
+    steps = C.c_uint64()
+    tau = lib().oracle_optical_depth(cc.ptr(), _p(pa), _p(da), C.byref(steps))
+    return tau, steps.value
+
+
+def free_flights(cfg, dims, o, d, n):
+    cc = CConfig(cfg, dims)
+    oa = np.ascontiguousarray(o, np.float64)
+    da = np.ascontiguousarray(d, np.float64)
+    dist = np.zeros(n)
+    mat = np.zeros(n, np.int32)
+    rc = lib().oracle_free_flights(cc.ptr(), _p(oa), _p(da), C.c_uint32(n), _p(dist), _p(mat))
+    if rc:
+        raise RuntimeError("oracle_free_flights failed with code %d" % rc)
+    return dist, mat
+
+
+def sample_directions(cfg, dims, m, prev, n):
+    cc = CConfig(cfg, dims)
+    pa = np.ascontiguousarray(prev, np.float64)
+    out = np.zeros((n, 3))
+    rc = lib().oracle_sample_directions(cc.ptr(), C.c_uint32(m), _p(pa), C.c_uint32(n), _p(out))
+    if rc:
+        raise RuntimeError("oracle_sample_directions failed with code %d" % rc)
+    return out
+
+
+def phase(cfg, dims, m, cosang, pdf=False):
+    cc = CConfig(cfg, dims)
+    return lib().oracle_phase_amplitude(cc.ptr(), m, float(cosang), 1 if pdf else 0)
+
+
+def farfield_table(cfg, dims, l):
+    cc = CConfig(cfg, dims)
+    out = np.zeros(16)
+    rc = lib().oracle_farfield_table(cc.ptr(), C.c_uint32(l), _p(out))
+    if rc:
+        raise RuntimeError("oracle_farfield_table failed with code %d" % rc)
+    return dict(tangent=out[0:3], perpT=out[3:6], perpS=out[6:9], minT=out[9], maxT=out[10],
+                minS=out[11], maxS=out[12], prob=out[13])
+
+
+def frame_table(cfg, dims):
+    cc = CConfig(cfg, dims)
+    nt = sizes(cfg, dims)[2]
+    dt = np.zeros(nt)
+    neg = C.c_uint32()
+    rc = lib().oracle_frame_table(cc.ptr(), _p(dt), C.byref(neg))
+    if rc:
+        raise RuntimeError("oracle_frame_table failed with code %d" % rc)
+    return dt, neg.value
