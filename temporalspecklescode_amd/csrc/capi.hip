@@ -174,6 +174,11 @@ static int create(const sstmc_config* c, sstmc_handle** out) {
     return SSTMC_NO_ERROR;
 }
 
+namespace sstmc {
+NextEventEstimation* handleTracer(sstmc_handle* h) { return h ? h->tracer.get() : nullptr; }
+int configValidate(const sstmc_config* c) { return validate(c); }
+}  // namespace sstmc
+
 extern "C" {
 
 int sstmc_create(const sstmc_config* cfg, sstmc_handle** out) {

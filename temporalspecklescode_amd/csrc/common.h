@@ -95,6 +95,9 @@ struct MediumView {
     float invMaj;                  // 1 / maxSigt
     int uniformAxis[3];            // 1 when the edges are equally spaced
     float e0[3], invH[3];          // first edge and inverse spacing for the fast locate
+    float h[3];                    // spacing of an equally spaced axis
+    int allUniform;                // every axis equally spaced: the branch-free FP32 walk applies
+    const float* sigtGrid;         // sigt per cell (FP32), x fastest: the walk's only memory read
 };
 
 // Scattering: Scattering.cuh:15-29, HG table :500-572.

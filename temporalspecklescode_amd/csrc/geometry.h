@@ -10,17 +10,17 @@ namespace sstmc {
 template <typename R> struct Real;
 template <> struct Real<float> {
     typedef f3 V;
-    __device__ static const float* edges(const MediumView& m, int a) { return m.ef[a]; }
-    __device__ static float sigt(const MediumView& m, int i) { return m.sigtf[i]; }
-    __device__ static float bmin(const MediumView& m, int a) { return m.bminf[a]; }
-    __device__ static float bmax(const MediumView& m, int a) { return m.bmaxf[a]; }
+    __host__ __device__ static const float* edges(const MediumView& m, int a) { return m.ef[a]; }
+    __host__ __device__ static float sigt(const MediumView& m, int i) { return m.sigtf[i]; }
+    __host__ __device__ static float bmin(const MediumView& m, int a) { return m.bminf[a]; }
+    __host__ __device__ static float bmax(const MediumView& m, int a) { return m.bmaxf[a]; }
 };
 template <> struct Real<double> {
     typedef d3 V;
-    __device__ static const double* edges(const MediumView& m, int a) { return m.ed[a]; }
-    __device__ static double sigt(const MediumView& m, int i) { return m.sigtd[i]; }
-    __device__ static double bmin(const MediumView& m, int a) { return m.bmind[a]; }
-    __device__ static double bmax(const MediumView& m, int a) { return m.bmaxd[a]; }
+    __host__ __device__ static const double* edges(const MediumView& m, int a) { return m.ed[a]; }
+    __host__ __device__ static double sigt(const MediumView& m, int i) { return m.sigtd[i]; }
+    __host__ __device__ static double bmin(const MediumView& m, int a) { return m.bmind[a]; }
+    __host__ __device__ static double bmax(const MediumView& m, int a) { return m.bmaxd[a]; }
 };
 
 __device__ inline float vget(const f3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
@@ -28,7 +28,7 @@ __device__ inline double vget(const d3& v, int a) { return a == 0 ? v.x : (a == 
 
 // MediumNS::isInsideKernel (MediumInterface.cuh:204-213): closed box.
 template <int DIMS, typename R, typename V>
-__device__ inline bool inside(const MediumView& m, const V& p) {
+__host__ __device__ inline bool inside(const MediumView& m, const V& p) {
     bool in = p.x >= Real<R>::bmin(m, 0) && p.x <= Real<R>::bmax(m, 0) &&
               p.y >= Real<R>::bmin(m, 1) && p.y <= Real<R>::bmax(m, 1);
     if (DIMS == 3) in = in && p.z >= Real<R>::bmin(m, 2) && p.z <= Real<R>::bmax(m, 2);
@@ -39,7 +39,7 @@ __device__ inline bool inside(const MediumView& m, const V& p) {
 // e[b] <= x < e[b+1], clamped to a valid cell. Equally spaced axes take a multiply and at most a
 // one-cell correction against the stored edges, so both paths return the same cell.
 template <typename R>
-__device__ inline int locate(const MediumView& m, int a, R x) {
+__host__ __device__ inline int locate(const MediumView& m, int a, R x) {
     const R* e = Real<R>::edges(m, a);
     int n = (int)m.n[a];
     int b;
@@ -60,7 +60,7 @@ __device__ inline int locate(const MediumView& m, int a, R x) {
     return lo;
 }
 
-__device__ inline uint32_t cellIndex(const MediumView& m, int bx, int by, int bz) {
+__host__ __device__ inline uint32_t cellIndex(const MediumView& m, int bx, int by, int bz) {
     return (uint32_t)bx + m.n[0] * ((uint32_t)by + m.n[1] * (uint32_t)bz);
 }
 
@@ -181,7 +181,7 @@ __device__ inline f3 sampleDirection(const ScatterView& s, uint32_t mat, const f
 // axis choice keeps the reference's strict-< tie order (x, then y, else z). The walk stops early
 // once tau > tauCut; callers pass a cut beyond which the result they form is exactly zero in R.
 template <int DIMS, typename R, typename V>
-__device__ inline R opticalDepth(const MediumView& m, const V& p, const V& d, R tauCut, uint32_t* steps) {
+__host__ __device__ inline R opticalDepth(const MediumView& m, const V& p, const V& d, R tauCut, uint32_t* steps) {
     if (!inside<DIMS, R>(m, p)) return (R)0;
     const R* e0 = Real<R>::edges(m, 0);
     const R* e1 = Real<R>::edges(m, 1);
@@ -223,6 +223,65 @@ __device__ inline R opticalDepth(const MediumView& m, const V& p, const V& d, R 
         }
         if (tau > tauCut) break;
         st = Real<R>::sigt(m, m.grid[cellIndex(m, b0, b1, b2)]);
+    }
+    if (steps) *steps += cnt;
+    return tau;
+}
+
+// Branch-free FP32 walk for equally spaced grids (the common case). Per step: pick the axis of
+// the nearest face with the reference's tie order, add segment * sigt, advance the cell index by
+// that axis' stride. The crossing parameter of the k-th face along an axis is
+// t_first + k * t_delta (one rounding, no accumulation). The only memory read per step is the
+// next cell's sigt, issued one step ahead of its use.
+template <int DIMS>
+__host__ __device__ inline float opticalDepthFast(const MediumView& m, const f3& p, const f3& d, float tauCut, uint32_t* steps) {
+    if (!inside<DIMS, float>(m, p)) return 0.f;
+    const float pp[3] = {p.x, p.y, DIMS == 3 ? p.z : 0.f};
+    const float dd[3] = {d.x, d.y, DIMS == 3 ? d.z : 0.f};
+    float tf[3], td[3];
+    int kmax[3], stride[3];
+    int cell = 0, mul = 1;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (a >= DIMS) {
+            tf[a] = 3.0e38f; td[a] = 0.f; kmax[a] = 0x7fffffff; stride[a] = 0;
+            continue;
+        }
+        const int n = (int)m.n[a];
+        const int b = locate<float>(m, a, pp[a]);
+        const float inv = fabsf(dd[a]) < 1e-12f ? (dd[a] >= 0.f ? 1e12f : -1e12f) : 1.0f / dd[a];
+        const int up = dd[a] >= 0.f;
+        tf[a] = (m.ef[a][b + up] - pp[a]) * inv;
+        td[a] = m.h[a] * fabsf(inv);
+        kmax[a] = up ? n - 1 - b : b;
+        stride[a] = up ? mul : -mul;
+        cell += b * mul;
+        mul *= n;
+    }
+    float t0 = tf[0], t1 = tf[1], t2 = tf[2];
+    int k0 = 0, k1 = 0, k2 = 0;
+    float tcur = 0.f, tau = 0.f;
+    float st = m.sigtGrid[cell];
+    uint32_t cnt = 0;
+    for (;;) {
+        ++cnt;
+        const bool m0 = (t0 < t1) && (t0 < t2);
+        const bool m1 = !m0 && (t1 < t2);
+        const float tn = m0 ? t0 : (m1 ? t1 : t2);
+        k0 += m0 ? 1 : 0;
+        k1 += m1 ? 1 : 0;
+        k2 += (!m0 && !m1) ? 1 : 0;
+        const bool out = (k0 > kmax[0]) | (k1 > kmax[1]) | (k2 > kmax[2]);
+        const int next = out ? cell : cell + (m0 ? stride[0] : (m1 ? stride[1] : stride[2]));
+        const float stNext = m.sigtGrid[next];
+        tau = fmaf(tn - tcur, st, tau);
+        tcur = tn;
+        if (out || tau > tauCut) break;
+        cell = next;
+        st = stNext;
+        t0 = m0 ? fmaf((float)k0, td[0], tf[0]) : t0;
+        t1 = m1 ? fmaf((float)k1, td[1], tf[1]) : t1;
+        t2 = (!m0 && !m1) ? fmaf((float)k2, td[2], tf[2]) : t2;
     }
     if (steps) *steps += cnt;
     return tau;

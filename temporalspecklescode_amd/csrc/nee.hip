@@ -68,18 +68,42 @@ template <int DIMS> __device__ inline d3 dnormalize(const d3& a) {
 
 // Scattering amplitude / pdf (Scattering.cuh:919-931, 983-995; isotropic constants
 // Simulation.cuh:266-272).
+// The cosine arrives in FP64 (it is formed from FP64 unit vectors) and 1 + g^2 - 2 g c is
+// evaluated in FP64: for forward-peaked HG it cancels to (1 - g)^2 near c = 1, which FP32 would
+// resolve to ~1e-5 relative only.
 template <int DIMS, typename R>
-__device__ inline R phasePdf(const ScatterView& s, uint32_t m, R c) {
+__device__ inline R phasePdf(const ScatterView& s, uint32_t m, double c) {
     if (s.type[m] == 1) return DIMS == 3 ? (R)0.079577471545947668 : (R)0.159154943091895336;
-    R g = (R)s.gd[m];
-    R up = DIMS == 3 ? ((R)1 - g * g) / (R)12.566370614359172954 : ((R)1 - g * g) / (R)6.283185307179586477;
-    R den = fma((R)-2 * g, c, (R)1 + g * g);
+    const double g = s.gd[m];
+    const R up = (R)(DIMS == 3 ? (1.0 - g * g) / 12.566370614359172954 : (1.0 - g * g) / 6.283185307179586477);
+    const R den = (R)fma(-2.0 * g, c, 1.0 + g * g);
     return DIMS == 3 ? up / (den * sqrtR<R>(den)) : up / den;
 }
 template <int DIMS, typename R>
-__device__ inline R phaseAmp(const ScatterView& s, uint32_t m, R c) {
+__device__ inline R phaseAmp(const ScatterView& s, uint32_t m, double c) {
     if (s.type[m] == 1) return DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678;
     return sqrtR<R>(phasePdf<DIMS, R>(s, m, c));
+}
+
+// Voxel walk dispatch: the branch-free FP32 walk on equally spaced grids, the general walk
+// (edge tables, FP64 when R is double) otherwise.
+template <int DIMS, typename R, typename V>
+__device__ inline R walk(const MediumView& m, const V& p, const V& d, R tauCut, uint32_t* steps);
+template <>
+__device__ inline float walk<2, float, f3>(const MediumView& m, const f3& p, const f3& d, float tauCut, uint32_t* steps) {
+    return m.allUniform ? opticalDepthFast<2>(m, p, d, tauCut, steps) : opticalDepth<2, float>(m, p, d, tauCut, steps);
+}
+template <>
+__device__ inline float walk<3, float, f3>(const MediumView& m, const f3& p, const f3& d, float tauCut, uint32_t* steps) {
+    return m.allUniform ? opticalDepthFast<3>(m, p, d, tauCut, steps) : opticalDepth<3, float>(m, p, d, tauCut, steps);
+}
+template <>
+__device__ inline double walk<2, double, d3>(const MediumView& m, const d3& p, const d3& d, double tauCut, uint32_t* steps) {
+    return opticalDepth<2, double>(m, p, d, tauCut, steps);
+}
+template <>
+__device__ inline double walk<3, double, d3>(const MediumView& m, const d3& p, const d3& d, double tauCut, uint32_t* steps) {
+    return opticalDepth<3, double>(m, p, d, tauCut, steps);
 }
 template <int DIMS, typename R> __device__ inline R isoAmp() {
     return DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678;
@@ -121,7 +145,7 @@ __device__ inline void gTerm(const MediumView& m, const SourceView& s, uint32_t 
         if (sizeof(R) == 4) w = Pos<R>::fromD(mkd(s.illum ? -(double)s.pf[3 * i] : (double)s.pf[3 * i],
                                                   s.illum ? -(double)s.pf[3 * i + 1] : (double)s.pf[3 * i + 1],
                                                   s.illum ? -(double)s.pf[3 * i + 2] : (double)s.pf[3 * i + 2]));
-        R tau = opticalDepth<DIMS, R>(m, p, w, tauCut, steps);
+        R tau = walk<DIMS, R>(m, p, w, tauCut, steps);
         *amp = expR<R>(-(R)0.5 * tau);
     } else {
         d3 sp = mkd(s.pd[3 * i], s.pd[3 * i + 1], DIMS == 3 ? s.pd[3 * i + 2] : 0.0);
@@ -130,7 +154,7 @@ __device__ inline void gTerm(const MediumView& m, const SourceView& s, uint32_t 
         double rr = 1.0 / r;
         *ph = r * invLambda;
         V w = Pos<R>::fromD(mkd(q.x * rr, q.y * rr, q.z * rr));
-        R tau = opticalDepth<DIMS, R>(m, p, w, (R)3e38, steps);
+        R tau = walk<DIMS, R>(m, p, w, (R)3e38, steps);
         double fac = DIMS == 3 ? rr * 0.531125966013598 /* sqrt(ISOTROPIC_AMPLITUDE) */ : sqrt(rr);
         if (s.area) fac *= sqrt(rr * fabs(DIMS == 3 ? x.pd.z - sp.z : x.pd.y - sp.y));
         *amp = expR<R>(-(R)0.5 * tau) * (R)fac;
@@ -212,14 +236,14 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
             typename Pos<R>::V w;
             if (sizeof(R) == 4) w = Pos<R>::fromD(mkd(-(double)il.pf[3 * l], -(double)il.pf[3 * l + 1], -(double)il.pf[3 * l + 2]));
             else w = Pos<R>::fromD(mkd(-il.pd[3 * l], -il.pd[3 * l + 1], -il.pd[3 * l + 2]));
-            R tau = opticalDepth<DIMS, R>(m, p1, w, sizeof(R) == 4 ? (R)prm.tauCutF : tauInf, &steps);
+            R tau = walk<DIMS, R>(m, p1, w, sizeof(R) == 4 ? (R)prm.tauCutF : tauInf, &steps);
             pr = expR<R>(-tau) * (sizeof(R) == 4 ? (R)il.probf[l] : (R)il.probd[l]);
         } else {
             d3 sp = mkd(il.pd[3 * l], il.pd[3 * l + 1], DIMS == 3 ? il.pd[3 * l + 2] : 0.0);
             d3 q = dsub(sp, x1c.pd);
             double rr = 1.0 / sqrt(ddot<DIMS>(q, q));
             typename Pos<R>::V w = Pos<R>::fromD(mkd(q.x * rr, q.y * rr, q.z * rr));
-            R tau = opticalDepth<DIMS, R>(m, Pos<R>::pick(x1c.pf, x1c.pd), w, tauInf, &steps);
+            R tau = walk<DIMS, R>(m, Pos<R>::pick(x1c.pf, x1c.pd), w, tauInf, &steps);
             // PointSource.cuh:92-115 (2D factor rr * 8 pi kept as in the reference, quirk Q7)
             double fac = DIMS == 3 ? rr * rr / 12.566370614359172954 : rr * 25.132741228718345908;
             pr = expR<R>(-tau) * (R)fac;
@@ -227,7 +251,7 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
         s1 += pr;
         if (multi) {
             d3 in = illumIn<DIMS>(il, l, x1c.pd);
-            s2 += pr * phasePdf<DIMS, R>(sc, m1, (R)ddot<DIMS>(in, seg12));
+            s2 += pr * phasePdf<DIMS, R>(sc, m1, ddot<DIMS>(in, seg12));
         }
     }
     s1 = waveSum(s1);
@@ -252,7 +276,7 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
                 const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
                 const d3 pb = fr.temporalAmp ? x2t.pd : x2cd;
                 if (isoPointIl) fa[f] = isoAmp<DIMS, R>();
-                else fa[f] = phaseAmp<DIMS, R>(sc, m1, (R)ddot<DIMS>(illumIn<DIMS>(il, l, pa), dnormalize<DIMS>(dsub(pb, pa))));
+                else fa[f] = phaseAmp<DIMS, R>(sc, m1, ddot<DIMS>(illumIn<DIMS>(il, l, pa), dnormalize<DIMS>(dsub(pb, pa))));
             }
         }
         const size_t o = ((size_t)t * P + p) * L + l;
@@ -264,6 +288,12 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
             A[o] = multi ? cis<R>(ga[0] * fa[0] * ga[1] * fa[1] * w2 * w2, gp[0] - gp[1]) : zero;
         }
     }
+
+    if (!multi)
+        for (uint32_t i = lane; i < V * NT; i += 64) {
+            const uint32_t v = i % V, t = i / V;
+            B[((size_t)t * P + p) * V + v] = zero;
+        }
 
     // ---- single-scattering view terms b = g_v(x1^t)
     for (uint32_t i = lane; i < V * NT; i += 64) {
@@ -280,68 +310,82 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
         b[o] = corr ? cis<R>(ga[0] * ga[1], gp[0] - gp[1]) : cis<R>(ga[0], gp[0]);
     }
 
-    // ---- multiple-scattering view terms B, IT items per lane per chunk
-    const bool isoPointVw = vw.type == 1 && sc.allIsotropic;
+    uint32_t tot = waveSum(steps);
+    if (lane == 0 && out.steps) atomicAdd(out.steps, (unsigned long long)tot);
+}
+
+
+// Multiple-scattering view terms B (the hot loop): one wavefront per path, one (view, frame)
+// item per lane per pass over the path's vertices. Templated on the view source type and the
+// output mode so the loop carries only what it needs.
+template <int DIMS, typename R, int VWFF, int CORR>
+__global__ void __launch_bounds__(256) k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr,
+                                                  VertexBatch vb, NeeParams prm, NeeOut out) {
+    typedef typename Cx<R>::T C;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (p >= prm.npaths) return;
+    const bool poolOk = (uint64_t)vb.off[prm.npaths] <= vb.cap;
+    const uint32_t K = poolOk ? vb.k[p] : 0u;
+    if (K < 2) return;                      // k_nee wrote the zero rows
+    const uint64_t off = vb.off[p];
+    const uint32_t P = prm.npaths, V = vw.n, NT = fr.nt;
+    const double invLambda = 1.0 / prm.lambda;
+    const R tauCut = VWFF ? (sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD) : (R)3e38;
+    C* B = (C*)out.B;
+    const bool isoPoint = !VWFF && sc.allIsotropic;
+    uint32_t steps = 0;
     const uint32_t items = V * NT;
-    for (uint32_t base = 0; base < items; base += 64 * IT) {
-        R accRe[IT], accIm[IT];
-        double len0[IT], lent[IT];
-#pragma unroll
-        for (int j = 0; j < IT; ++j) { accRe[j] = 0; accIm[j] = 0; len0[j] = 0; lent[j] = 0; }
-        for (uint32_t k = 1; k < K; ++k) {
-            const uint64_t ok = off + k, op = off + k - 1;
-            const float4 xkm = vb.xm[ok], xpm = vb.xm[op];
-            const uint32_t mk_ = __float_as_uint(xkm.w);
-            const double rhok = (double)vb.rho[ok];
-#pragma unroll
-            for (int j = 0; j < IT; ++j) {
-                const uint32_t i = base + (uint32_t)j * 64 + lane;
-                if (i >= items) continue;
-                const uint32_t v = i % V, t = i / V;
-                const int nf = corr ? 2 : 1;
+    for (uint32_t base = 0; base < items; base += 64) {
+        const uint32_t i = base + lane;
+        const bool active = i < items;
+        const uint32_t v = active ? i % V : 0, t = active ? i / V : 0;
+        R accRe = 0, accIm = 0;
+        double len0 = 0, lent = 0;
+        // view direction (far field) or location (point) of this lane's item
+        const d3 vd = mkd(vw.pd[3 * v], vw.pd[3 * v + 1], DIMS == 3 ? vw.pd[3 * v + 2] : 0.0);
+        if (active) {
+            for (uint32_t k = 1; k < K; ++k) {
+                const uint64_t ok = off + k, op = off + k - 1;
+                const float4 xkm = vb.xm[ok], xpm = vb.xm[op];
+                const uint32_t mk_ = __float_as_uint(xkm.w);
+                const double rhok = (double)vb.rho[ok];
                 R gr[2], gi[2];
-                for (int f = 0; f < nf; ++f) {
-                    const uint32_t tt = (corr && f == 0) ? 0 : t;
-                    Geo xk = vertexFrame(vb, fr, ok, tt, xkm);
-                    Geo xp = vertexFrame(vb, fr, op, tt, xpm);
-                    d3 sg = dsub(xk.pd, xp.pd);
-                    double ln = sqrt(ddot<DIMS>(sg, sg));
+#pragma unroll
+                for (int f = 0; f < (CORR ? 2 : 1); ++f) {
+                    const uint32_t tt = (CORR && f == 0) ? 0 : t;
+                    const Geo xk = vertexFrame(vb, fr, ok, tt, xkm);
+                    const Geo xp = vertexFrame(vb, fr, op, tt, xpm);
+                    const d3 sg = dsub(xk.pd, xp.pd);
+                    const double ln = sqrt(ddot<DIMS>(sg, sg));
                     double lacc;
-                    if (corr && f == 0) { len0[j] += ln; lacc = len0[j]; }
-                    else { lent[j] += ln; lacc = lent[j]; }
+                    if (CORR && f == 0) { len0 += ln; lacc = len0; }
+                    else { lent += ln; lacc = lent; }
                     R ga;
                     double gp;
-                    gTerm<DIMS, R>(m, vw, v, xk, invLambda, vw.type == 2 ? tauCutFF : tauInf, &steps, &ga, &gp);
+                    gTerm<DIMS, R>(m, vw, v, xk, invLambda, tauCut, &steps, &ga, &gp);
                     R fv;
-                    if (isoPointVw) {
+                    if (isoPoint) {
                         fv = isoAmp<DIMS, R>();
                     } else {
                         const d3 pk = fr.temporalAmp ? xk.pd : mkd(xkm.x, xkm.y, xkm.z);
                         const d3 pp = fr.temporalAmp ? xp.pd : mkd(xpm.x, xpm.y, xpm.z);
-                        d3 av;
-                        if (vw.type == 2) av = mkd(-vw.pd[3 * v], -vw.pd[3 * v + 1], -vw.pd[3 * v + 2]);
-                        else av = dnormalize<DIMS>(dsub(pk, mkd(vw.pd[3 * v], vw.pd[3 * v + 1], DIMS == 3 ? vw.pd[3 * v + 2] : 0.0)));
-                        fv = phaseAmp<DIMS, R>(sc, mk_, (R)ddot<DIMS>(av, dnormalize<DIMS>(dsub(pp, pk))));
+                        const d3 av = VWFF ? mkd(-vd.x, -vd.y, -vd.z) : dnormalize<DIMS>(dsub(pk, vd));
+                        fv = phaseAmp<DIMS, R>(sc, mk_, ddot<DIMS>(av, dnormalize<DIMS>(dsub(pp, pk))));
                     }
                     typename Cx<R>::T g = cis<R>(ga * fv, gp + rhok + lacc * invLambda);
                     gr[f] = g.x;
                     gi[f] = g.y;
                 }
-                if (!corr) {
-                    accRe[j] += gr[0];
-                    accIm[j] += gi[0];
+                if (!CORR) {
+                    accRe += gr[0];
+                    accIm += gi[0];
                 } else {  // G(0) * conj(G(t))
-                    accRe[j] += gr[0] * gr[1] + gi[0] * gi[1];
-                    accIm[j] += gi[0] * gr[1] - gr[0] * gi[1];
+                    accRe += gr[0] * gr[1] + gi[0] * gi[1];
+                    accIm += gi[0] * gr[1] - gr[0] * gi[1];
                 }
             }
-        }
-#pragma unroll
-        for (int j = 0; j < IT; ++j) {
-            const uint32_t i = base + (uint32_t)j * 64 + lane;
-            if (i >= items) continue;
-            const uint32_t v = i % V, t = i / V;
-            B[((size_t)t * P + p) * V + v] = Cx<R>::make(accRe[j], accIm[j]);
+            B[((size_t)t * P + p) * V + v] = Cx<R>::make(accRe, accIm);
         }
     }
     uint32_t tot = waveSum(steps);
@@ -353,14 +397,13 @@ static void launchNeeT(int it, const MediumView& m, const ScatterView& sc, const
                        const FramesView& fr, const VertexBatch& vb, const NeeParams& p, const NeeOut& out,
                        hipStream_t st) {
     dim3 grid((p.npaths + 3) / 4), block(256);
-    if (it >= 16)
-        hipLaunchKernelGGL((k_nee<DIMS, R, 16>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
-    else if (it >= 8)
-        hipLaunchKernelGGL((k_nee<DIMS, R, 8>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
-    else if (it >= 4)
-        hipLaunchKernelGGL((k_nee<DIMS, R, 4>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
-    else
-        hipLaunchKernelGGL((k_nee<DIMS, R, 1>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    (void)it;
+    hipLaunchKernelGGL((k_nee<DIMS, R, 1>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    const bool ff = vw.type == 2, corr = p.output == 1;
+    if (ff && !corr) hipLaunchKernelGGL((k_nee_view<DIMS, R, 1, 0>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
+    else if (ff && corr) hipLaunchKernelGGL((k_nee_view<DIMS, R, 1, 1>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
+    else if (!ff && !corr) hipLaunchKernelGGL((k_nee_view<DIMS, R, 0, 0>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
+    else hipLaunchKernelGGL((k_nee_view<DIMS, R, 0, 1>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
 }
 
 hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,

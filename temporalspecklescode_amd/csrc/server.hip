@@ -1,16 +1,387 @@
-// server.hip -- serving entry points (device-resident request loop).
-// Placeholder bodies until the continuous-batching loop lands; they fail loudly.
+// server.hip -- device-resident request loop with continuous batching.
+//
+// The reference can hold one simulation per process: its tables live in process-global
+// `__constant__` symbols (Simulation.cuh:168-173 and every Interface/*.cuh). Here every request
+// owns its tracer (medium, sources, sampler, buffers, HIP stream), so several requests are
+// resident at once. A worker thread:
+//   * admits queued requests in arrival order while the HBM footprint of the admitted set stays
+//     within the budget and fewer than max_inflight are active;
+//   * keeps up to two batches in flight per active request, round-robin over requests, so work
+//     that arrives while others run is interleaved with it on the device;
+//   * on completion copies the field into a small device buffer owned by the request and frees
+//     the tracer, so finished results stay on the device until fetched.
+// A request's result depends only on its config: batches, random numbers and reductions are
+// per request and in a fixed order, so it is bit-identical alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
 #include "../../include/sstmc.h"
+#include "tracer.h"
+
+using namespace sstmc;
+typedef std::chrono::steady_clock Clock;
+
+extern "C" int sstmc_create(const sstmc_config* cfg, sstmc_handle** out);
+extern "C" void sstmc_destroy(sstmc_handle* h);
+namespace sstmc {
+NextEventEstimation* handleTracer(sstmc_handle* h);
+int configValidate(const sstmc_config* c);
+}
+
+namespace {
+
+// Deep copy of a config: the caller may free its arrays after sstmc_submit returns.
+struct ConfigCopy {
+    sstmc_config c{};
+    std::vector<double> ax[3], sigs, siga, g, src[2][3], t, D, U[3];
+    std::vector<uint32_t> grid;
+    std::vector<int32_t> stype;
+
+    template <typename T>
+    static const T* keep(std::vector<T>& dst, const T* p, size_t n) {
+        if (!p) return nullptr;
+        dst.assign(p, p + n);
+        return dst.data();
+    }
+    explicit ConfigCopy(const sstmc_config& in) : c(in) {
+        int dims = (int)in.dims;
+        const sstmc_medium& m = in.medium;
+        c.medium.xAxis = keep(ax[0], m.xAxis, m.xAxisSize);
+        c.medium.yAxis = keep(ax[1], m.yAxis, m.yAxisSize);
+        c.medium.zAxis = dims == 3 ? keep(ax[2], m.zAxis, m.zAxisSize) : nullptr;
+        size_t ncell = (size_t)(m.xAxisSize - 1) * (m.yAxisSize - 1) * (dims == 3 ? m.zAxisSize - 1 : 1);
+        c.medium.materialGrid = keep(grid, m.materialGrid, ncell);
+        c.medium.sigs = keep(sigs, m.sigs, m.materialNum);
+        c.medium.siga = keep(siga, m.siga, m.materialNum);
+        c.scattering.type = keep(stype, in.scattering.type, in.scattering.materialNum);
+        c.scattering.g = keep(g, in.scattering.g, in.scattering.materialNum);
+        const sstmc_source* s[2] = {&in.illumination, &in.view};
+        sstmc_source* d[2] = {&c.illumination, &c.view};
+        for (int i = 0; i < 2; ++i) {
+            d[i]->x = keep(src[i][0], s[i]->x, s[i]->num);
+            d[i]->y = keep(src[i][1], s[i]->y, s[i]->num);
+            d[i]->z = dims == 3 ? keep(src[i][2], s[i]->z, s[i]->num) : nullptr;
+        }
+        uint32_t nmat = m.materialNum;
+        c.sampler.t = keep(t, in.sampler.t, in.sampler.tNum);
+        c.sampler.D = keep(D, in.sampler.D, nmat);
+        c.sampler.Ux = keep(U[0], in.sampler.Ux, nmat);
+        c.sampler.Uy = keep(U[1], in.sampler.Uy, nmat);
+        c.sampler.Uz = dims == 3 ? keep(U[2], in.sampler.Uz, nmat) : nullptr;
+    }
+};
+
+enum State { kQueued, kActive, kDone, kFailed, kFetched };
+
+struct Request {
+    uint64_t id;
+    std::unique_ptr<ConfigCopy> cfg;
+    State state = kQueued;
+    int error = SSTMC_NO_ERROR;
+    uint64_t footprint = 0;
+    sstmc_handle* handle = nullptr;
+    NextEventEstimation* tracer = nullptr;
+    uint32_t nextBatch = 0, batches = 0;
+    std::deque<hipEvent_t> inflight;
+    double* result = nullptr;          // device copy of the field after completion
+    size_t resultDoubles = 0;
+    double iter = 0;
+    uint64_t paths = 0, contributions = 0;
+    Clock::time_point submitted, done;
+};
+
+}  // namespace
+
+struct sstmc_server {
+    int device = 0;
+    uint64_t budget = 0, inUse = 0;
+    uint32_t maxInflight = 4;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::map<uint64_t, std::unique_ptr<Request>> reqs;
+    std::deque<uint64_t> queue;
+    std::vector<uint64_t> active;
+    uint64_t nextId = 1;
+    bool stop = false;
+    std::thread worker;
+    std::vector<double> latMs;
+    uint64_t donePaths = 0, doneContrib = 0;
+    Clock::time_point firstAdmit, lastDone;
+    bool anyAdmitted = false;
+
+    void run();
+    void admit();
+    int pump(Request& r);
+    void complete(Request& r);
+};
+
+static uint64_t estimate(const sstmc_config& c) {
+    uint32_t P = c.simulation.batchSize ? c.simulation.batchSize : 1024;
+    bool fp64 = c.precision.terms == SSTMC_TERMS_FP64;
+    bool frames = false;
+    for (uint32_t m = 1; m < c.medium.materialNum; ++m) {
+        if (c.sampler.D[m] != 0 || (c.sampler.Ux && c.sampler.Ux[m] != 0) || (c.sampler.Uy && c.sampler.Uy[m] != 0) ||
+            (c.sampler.Uz && c.sampler.Uz[m] != 0))
+            frames = c.sampler.tNum > 1;
+    }
+    return NextEventEstimation::footprint(P, c.illumination.num, c.view.num, c.sampler.tNum, fp64, frames);
+}
+
+void sstmc_server::admit() {
+    // caller holds mu
+    while (!queue.empty() && active.size() < maxInflight) {
+        Request& r = *reqs[queue.front()];
+        if (inUse + r.footprint > budget && !active.empty()) break;   // wait for memory to free up
+        queue.pop_front();
+        sstmc_handle* h = nullptr;
+        int rc = sstmc_create(&r.cfg->c, &h);
+        if (rc) {
+            r.state = kFailed;
+            r.error = rc;
+            r.done = Clock::now();
+            cv.notify_all();
+            continue;
+        }
+        r.handle = h;
+        r.tracer = handleTracer(h);
+        r.batches = r.tracer->batches();
+        r.nextBatch = 0;
+        rc = r.tracer->begin();
+        if (rc) {
+            r.state = kFailed;
+            r.error = rc;
+            sstmc_destroy(h);
+            r.handle = nullptr;
+            cv.notify_all();
+            continue;
+        }
+        r.state = kActive;
+        inUse += r.footprint;
+        active.push_back(r.id);
+        if (!anyAdmitted) { firstAdmit = Clock::now(); anyAdmitted = true; }
+    }
+}
+
+// Queue batches (at most two in flight) and retire finished ones. Returns 1 when the request's
+// last batch has completed.
+int sstmc_server::pump(Request& r) {
+    while (!r.inflight.empty() && hipEventQuery(r.inflight.front()) == hipSuccess) {
+        hipEventDestroy(r.inflight.front());
+        r.inflight.pop_front();
+    }
+    while (r.nextBatch < r.batches && r.inflight.size() < 2) {
+        int rc = r.tracer->enqueueBatch(r.nextBatch++);
+        if (rc) { r.error = rc; return -1; }
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { r.error = SSTMC_DEVICE_ERROR; return -1; }
+        hipEventRecord(e, r.tracer->stream());
+        r.inflight.push_back(e);
+    }
+    return (r.nextBatch == r.batches && r.inflight.empty()) ? 1 : 0;
+}
+
+void sstmc_server::complete(Request& r) {
+    size_t n = 2ull * r.tracer->L() * r.tracer->V() * r.tracer->T();
+    int rc = hipMalloc((void**)&r.result, n * sizeof(double)) == hipSuccess ? SSTMC_NO_ERROR : SSTMC_ALLOCATION_ERROR;
+    if (!rc) rc = r.tracer->finish(r.result, true);
+    if (rc == SSTMC_NOT_READY) {       // vertex pool overflow: the pool was grown, re-run the request
+        hipFree(r.result);
+        r.result = nullptr;
+        r.tracer->begin();
+        r.nextBatch = 0;
+        return;
+    }
+    r.resultDoubles = n;
+    r.iter = (double)r.tracer->paths() + (double)r.tracer->getZeroContributionPaths();
+    r.paths = r.tracer->paths();
+    r.contributions = r.paths * (uint64_t)r.tracer->L() * r.tracer->V() * r.tracer->T();
+    sstmc_destroy(r.handle);
+    r.handle = nullptr;
+    r.tracer = nullptr;
+    inUse -= r.footprint;
+    r.done = Clock::now();
+    if (rc) {
+        r.state = kFailed;
+        r.error = rc;
+    } else {
+        r.state = kDone;
+        latMs.push_back(std::chrono::duration<double, std::milli>(r.done - r.submitted).count());
+        donePaths += r.paths;
+        doneContrib += r.contributions;
+        lastDone = r.done;
+    }
+    active.erase(std::find(active.begin(), active.end(), r.id));
+    cv.notify_all();
+}
+
+void sstmc_server::run() {
+    hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    while (!stop) {
+        admit();
+        if (active.empty()) {
+            cv.wait_for(lk, std::chrono::milliseconds(2));
+            continue;
+        }
+        std::vector<uint64_t> ids = active;
+        for (uint64_t id : ids) {
+            Request& r = *reqs[id];
+            if (r.state != kActive) continue;
+            int st = pump(r);
+            if (st < 0) {
+                sstmc_destroy(r.handle);
+                r.handle = nullptr;
+                inUse -= r.footprint;
+                r.state = kFailed;
+                r.done = Clock::now();
+                active.erase(std::find(active.begin(), active.end(), id));
+                cv.notify_all();
+            } else if (st == 1) {
+                complete(r);
+            }
+        }
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
+        lk.lock();
+    }
+    for (uint64_t id : active) {
+        Request& r = *reqs[id];
+        for (hipEvent_t e : r.inflight) { hipEventSynchronize(e); hipEventDestroy(e); }
+        if (r.handle) sstmc_destroy(r.handle);
+        r.handle = nullptr;
+    }
+}
 
 extern "C" {
-int sstmc_server_create(uint32_t, uint64_t, uint32_t, sstmc_server** out) {
-    if (out) *out = nullptr;
-    return SSTMC_NOT_SUPPORTED;
+
+int sstmc_server_create(uint32_t device, uint64_t hbm_budget_bytes, uint32_t max_inflight, sstmc_server** out) {
+    if (!out) return SSTMC_INVALID_CONFIG;
+    *out = nullptr;
+    if (hipSetDevice((int)device) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    sstmc_server* s = new sstmc_server();
+    s->device = (int)device;
+    if (hbm_budget_bytes == 0) {
+        size_t freeB = 0, totalB = 0;
+        hipMemGetInfo(&freeB, &totalB);
+        hbm_budget_bytes = (uint64_t)(freeB * 0.8);
+    }
+    s->budget = hbm_budget_bytes;
+    s->maxInflight = max_inflight ? max_inflight : 4;
+    s->worker = std::thread([s] { s->run(); });
+    *out = s;
+    return SSTMC_NO_ERROR;
 }
-int sstmc_submit(sstmc_server*, const sstmc_config*, uint64_t*) { return SSTMC_NOT_SUPPORTED; }
-int sstmc_poll(sstmc_server*, uint64_t) { return SSTMC_NOT_SUPPORTED; }
-int sstmc_wait(sstmc_server*, uint64_t) { return SSTMC_NOT_SUPPORTED; }
-int sstmc_fetch(sstmc_server*, uint64_t, double*, double*) { return SSTMC_NOT_SUPPORTED; }
-int sstmc_server_stats_get(sstmc_server*, sstmc_server_stats*) { return SSTMC_NOT_SUPPORTED; }
-void sstmc_server_destroy(sstmc_server*) {}
+
+int sstmc_submit(sstmc_server* s, const sstmc_config* cfg, uint64_t* request_id) {
+    if (!s || !cfg || !request_id) return SSTMC_INVALID_CONFIG;
+    int rc = configValidate(cfg);
+    if (rc) return rc;
+    std::unique_ptr<Request> r(new Request());
+    r->cfg.reset(new ConfigCopy(*cfg));
+    r->footprint = estimate(r->cfg->c);
+    if (r->footprint > s->budget) return SSTMC_BUDGET_EXCEEDED;
+    r->submitted = Clock::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->id = s->nextId++;
+    *request_id = r->id;
+    s->queue.push_back(r->id);
+    s->reqs[r->id] = std::move(r);
+    s->cv.notify_all();
+    return SSTMC_NO_ERROR;
 }
+
+int sstmc_poll(sstmc_server* s, uint64_t id) {
+    if (!s) return SSTMC_INVALID_CONFIG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return SSTMC_INVALID_CONFIG;
+    const Request& r = *it->second;
+    if (r.state == kDone || r.state == kFetched) return SSTMC_NO_ERROR;
+    if (r.state == kFailed) return r.error ? r.error : SSTMC_MISC_ERROR;
+    return SSTMC_NOT_READY;
+}
+
+int sstmc_wait(sstmc_server* s, uint64_t id) {
+    if (!s) return SSTMC_INVALID_CONFIG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return SSTMC_INVALID_CONFIG;
+    Request& r = *it->second;
+    s->cv.wait(lk, [&] { return r.state == kDone || r.state == kFailed || r.state == kFetched; });
+    return r.state == kFailed ? (r.error ? r.error : SSTMC_MISC_ERROR) : SSTMC_NO_ERROR;
+}
+
+int sstmc_fetch(sstmc_server* s, uint64_t id, double* u, double* iter) {
+    if (!s || !u) return SSTMC_INVALID_CONFIG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return SSTMC_INVALID_CONFIG;
+    Request& r = *it->second;
+    if (r.state == kFailed) return r.error ? r.error : SSTMC_MISC_ERROR;
+    if (r.state != kDone) return SSTMC_NOT_READY;
+    hipSetDevice(s->device);
+    if (hipMemcpy(u, r.result, r.resultDoubles * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        return SSTMC_DEVICE_ERROR;
+    if (iter) *iter = r.iter;
+    hipFree(r.result);
+    r.result = nullptr;
+    r.state = kFetched;
+    r.cfg.reset();
+    return SSTMC_NO_ERROR;
+}
+
+int sstmc_server_stats_get(sstmc_server* s, sstmc_server_stats* out) {
+    if (!s || !out) return SSTMC_INVALID_CONFIG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    sstmc_server_stats st{};
+    st.completed = s->latMs.size();
+    if (!s->latMs.empty()) {
+        std::vector<double> v = s->latMs;
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double q) {
+            double pos = q * (v.size() - 1);
+            size_t i = (size_t)pos;
+            double f = pos - i;
+            return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+        };
+        st.p50_ms = pct(0.50);
+        st.p99_ms = pct(0.99);
+        double sum = 0;
+        for (double x : v) sum += x;
+        st.mean_ms = sum / v.size();
+        double busy = std::chrono::duration<double>(s->lastDone - s->firstAdmit).count();
+        if (busy > 0) {
+            st.paths_per_s = s->donePaths / busy;
+            st.contributions_per_s = s->doneContrib / busy;
+        }
+    }
+    st.hbm_in_use = s->inUse;
+    st.hbm_budget = s->budget;
+    *out = st;
+    return SSTMC_NO_ERROR;
+}
+
+void sstmc_server_destroy(sstmc_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    hipSetDevice(s->device);
+    for (auto& kv : s->reqs)
+        if (kv.second->result) hipFree(kv.second->result);
+    delete s;
+}
+
+}  // extern "C"

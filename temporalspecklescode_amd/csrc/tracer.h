@@ -63,6 +63,8 @@ private:
     DevBuf<float> ef_[3];
     DevBuf<double> ed_[3];
     DevBuf<uint8_t> grid_;
+    DevBuf<float> sigtGrid_;
+    std::vector<uint8_t> hostGrid_;
 };
 
 // ---------------------------------------------------------------- Scattering
@@ -169,10 +171,17 @@ public:
                         const Source& view, const TemporalSampler& sampler, const TracerOptions& opt);
     ~NextEventEstimation() override;
     int trace(double* u) override;
-    // Asynchronous form used by the serving loop: enqueue every batch on the stream; finish()
-    // checks the device status, retries on pool overflow and copies the result out.
+    // Asynchronous form used by the serving loop: begin() clears the accumulators, enqueueBatch()
+    // queues one batch on the tracer's stream, finish() checks the device status (returns
+    // SSTMC_NOT_READY after a vertex-pool overflow, after which the pool has been grown and the
+    // trace must be re-run) and copies the field to host or device memory.
+    int begin();
+    int enqueueBatch(uint32_t b);
     int enqueue();
-    int finish(double* u);
+    int finish(double* u, bool deviceDst = false);
+    uint32_t batches() const { return opt_.fullIteration; }
+    // device bytes a tracer with these sizes allocates (admission control)
+    static uint64_t footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames);
     int debugBatch(uint64_t pathBegin, uint32_t npaths, uint64_t* nvtxOut, uint32_t* pathK, uint32_t* pathMisses,
                    uint64_t* pathOff, float* vtxPos, uint32_t* vtxMat, float* vtxRho, float* vtxXt, double* vtxLen,
                    double* termA, double* termB, double* termSa, double* termSb, double* uBatch);

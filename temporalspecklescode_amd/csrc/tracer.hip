@@ -73,11 +73,13 @@ HeterogeneousMedium::HeterogeneousMedium(int* err, int dims, const uint32_t* gri
         v_.uniformAxis[a] = uni ? 1 : 0;
         v_.e0[a] = (float)e[0];
         v_.invH[a] = uni ? (float)(1.0 / h) : 0.f;
+        v_.h[a] = (float)h;
     }
+    v_.allUniform = v_.uniformAxis[0] && v_.uniformAxis[1] && (dims == 2 || v_.uniformAxis[2]) ? 1 : 0;
     size_t ncell = (size_t)v_.n[0] * v_.n[1] * v_.n[2];
-    std::vector<uint8_t> g(ncell);
-    for (size_t i = 0; i < ncell; ++i) g[i] = (uint8_t)grid[i];
-    if ((*err = grid_.upload(g.data(), ncell))) return;
+    hostGrid_.resize(ncell);
+    for (size_t i = 0; i < ncell; ++i) hostGrid_[i] = (uint8_t)grid[i];
+    if ((*err = grid_.upload(hostGrid_.data(), ncell))) return;
     v_.grid = grid_.p;
 }
 
@@ -99,6 +101,12 @@ int HeterogeneousMedium::setMaterial(const double* sigs, const double* siga, uin
         v_.albedo[m] = st > 0 ? (float)(ss / st) : 0.f;
     }
     v_.invMaj = (float)(1.0 / maj);
+    // per-cell sigt for the FP32 voxel walk (one 4-byte read per step, no table indirection)
+    std::vector<float> sg(hostGrid_.size());
+    for (size_t i = 0; i < sg.size(); ++i) sg[i] = v_.sigtf[hostGrid_[i]];
+    int e = sigtGrid_.upload(sg.data(), sg.size());
+    if (e) return e;
+    v_.sigtGrid = sigtGrid_.p;
     return SSTMC_NO_ERROR;
 }
 
@@ -479,7 +487,16 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     return SSTMC_NO_ERROR;
 }
 
-int NextEventEstimation::enqueue() {
+uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames) {
+    uint64_t elem = fp64 ? 16 : 8;
+    uint64_t terms = 2ull * T * P * (uint64_t)(L + V) * elem;
+    uint64_t field = 16ull * T * L * V;
+    uint64_t pool = (uint64_t)P * 32 * (20 + (frames ? 12ull * T : 0));
+    uint64_t ws = 2ull * 8 * T * L * V * elem;   // split-K partials, two workspaces
+    return terms + field + pool + ws + 24ull * P + (1ull << 20);
+}
+
+int NextEventEstimation::begin() {
     uint32_t nb = opt_.fullIteration;
     if (evBatches_ < nb) {
         for (hipEvent_t ev : ev_) hipEventDestroy(ev);
@@ -491,14 +508,23 @@ int NextEventEstimation::enqueue() {
     if (hipMemsetAsync(status_.p, 0, 4 * sizeof(uint32_t), stream_) != hipSuccess) return SSTMC_DEVICE_ERROR;
     if (hipMemsetAsync(totals_.p, 0, 4 * sizeof(unsigned long long), stream_) != hipSuccess) return SSTMC_DEVICE_ERROR;
     if (launchZeroU(U_.p, U_.n, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
-    for (uint32_t b = 0; b < nb; ++b) {
-        int rc = runBatch((uint64_t)b * P_, P_, U_.p, true, b);
-        if (rc) return rc;
-    }
     return SSTMC_NO_ERROR;
 }
 
-int NextEventEstimation::finish(double* u) {
+int NextEventEstimation::enqueueBatch(uint32_t b) {
+    if (b >= opt_.fullIteration) return SSTMC_INVALID_CONFIG;
+    return runBatch((uint64_t)b * P_, P_, U_.p, true, b);
+}
+
+int NextEventEstimation::enqueue() {
+    int rc = begin();
+    if (rc) return rc;
+    for (uint32_t b = 0; b < opt_.fullIteration; ++b)
+        if ((rc = enqueueBatch(b))) return rc;
+    return SSTMC_NO_ERROR;
+}
+
+int NextEventEstimation::finish(double* u, bool deviceDst) {
     if (hipStreamSynchronize(stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
     uint32_t st[4];
     unsigned long long tot[4];
@@ -506,10 +532,16 @@ int NextEventEstimation::finish(double* u) {
     if (hipMemcpy(tot, totals_.p, sizeof(tot), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
     if (st[1]) return SSTMC_NO_HIT;
     if (st[0] & 2u) return SSTMC_KERNEL_ERROR;   // count and write passes disagreed
-    if (st[0] & 1u) return SSTMC_NOT_READY;      // pool overflow: caller grows and retries
+    if (st[0] & 1u) {                            // pool overflow: grow from the largest batch, re-run
+        uint64_t want = std::max<uint64_t>(cap_ * 2, (uint64_t)tot[3] + tot[3] / 4 + 1024);
+        int rc = growPool(want);
+        return rc ? rc : SSTMC_NOT_READY;
+    }
     zeroContributionPaths_ = tot[0];
     photonsBudget_ = tot[1];
-    if (u && hipMemcpy(u, U_.p, U_.n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    if (u && hipMemcpy(u, U_.p, U_.n * sizeof(double), deviceDst ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost) !=
+                 hipSuccess)
+        return SSTMC_DEVICE_ERROR;
     // stage timing from the per-batch events
     sstmc_timing t{};
     for (uint32_t b = 0; b < opt_.fullIteration; ++b) {
@@ -535,12 +567,7 @@ int NextEventEstimation::trace(double* u) {
         int rc = enqueue();
         if (rc) return rc;
         rc = finish(u);
-        if (rc != SSTMC_NOT_READY) return rc;
-        // the vertex pool overflowed in some batch: size it from the largest batch and redo
-        unsigned long long tot[4];
-        if (hipMemcpy(tot, totals_.p, sizeof(tot), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
-        uint64_t want = std::max<uint64_t>(cap_ * 2, (uint64_t)tot[3] + tot[3] / 4 + 1024);
-        if ((rc = growPool(want))) return rc;
+        if (rc != SSTMC_NOT_READY) return rc;   // NOT_READY: the pool was grown, run again
     }
     return SSTMC_ALLOCATION_ERROR;
 }

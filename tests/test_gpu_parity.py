@@ -24,9 +24,13 @@ def _rel(a, b):
 
 
 def _oracle_u(cfg, dims, d, a, b, s_unused=None):
-    """Batch field from oracle terms on the GPU's vertices: sum_p S_p + A_p (x) B_p."""
+    """Batch field from oracle terms on the GPU's vertices: sum_p S_p + A_p (x) B_p. For the
+    correlation output A and B already hold the per-path lag products; the single-scattering
+    field S_p[t] is turned into S_p[0] conj(S_p[t]) here."""
     A, B, S = O.terms_from_vertices(cfg, dims, d["k"], d["off"], d["pos"].astype(np.float64), d["mat"],
                                     d["rho"].astype(np.float64), d["xt"], d["len"])
+    if cfg.get("tracer", {}).get("output") == "correlation":
+        S = S[:, :1] * np.conj(S)
     U = S.sum(axis=0).transpose(2, 1, 0) + np.einsum("ptl,ptv->lvt", A, B)
     return A, B, S, U
 
