@@ -71,6 +71,37 @@ def cpu_baseline(name: str, seconds: float, threads: int):
             "paths_per_s": rate}
 
 
+def serving(n_requests: int, max_paths_log2: int):
+    """Replays the mixed request stream (Poisson arrivals, 2D/3D, three media, L and V in
+    [16, 512], 1-8 lags of linear motion) through the continuous-batching server; returns
+    latency percentiles and throughput."""
+    from temporalspecklescode_amd import workloads as W
+    from temporalspecklescode_amd.serving import Server
+    from temporalspecklescode_amd.config import sizes
+    stream = W.serving_stream(n_requests, max_paths_log2=max_paths_log2)
+    t0 = time.perf_counter()
+    ids = []
+    contrib = 0
+    with Server(max_inflight=8) as srv:
+        for (arrival, dims, cfg) in stream:
+            now = time.perf_counter() - t0
+            if arrival > now:
+                time.sleep(arrival - now)
+            ids.append(srv.submit(cfg, dims))
+            nl, nv, nt, _ = sizes(cfg, dims)
+            contrib += cfg["simulation"]["batchSize"] * cfg["simulation"]["fullIteration"] * nl * nv * nt
+        for rid in ids:
+            srv.wait(rid)
+            srv.fetch(rid)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    print("bench: served %d requests in %.2f s, p50 %.1f ms p99 %.1f ms" % (n_requests, wall, st["p50_ms"],
+                                                                         st["p99_ms"]), file=sys.stderr, flush=True)
+    return {"requests": n_requests, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "paths_per_s": st["paths_per_s"], "contributions_per_s": st["contributions_per_s"],
+            "wall_s": wall, "max_paths": 1 << max_paths_log2, "arrival_rate_per_s": 20.0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +113,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--serving", type=int, default=0, help="requests of the mixed stream to serve after the "
+                    "throughput measurement (0: skip)")
+    ap.add_argument("--serving-max-paths-log2", type=int, default=18)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding by path range is a design "
@@ -164,6 +198,8 @@ def main():
         "roofline": dominant, "roofline_contraction": roof_con, "roofline_nee": roof_nee,
         "accuracy": accuracy, "stage_ms": {k: tm[k] for k in ("transport_ms", "nee_ms", "contract_ms", "single_ms")},
     }
+    if args.serving > 0:
+        out["serving"] = serving(args.serving, args.serving_max_paths_log2)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         threads = max(1, min(threads, 16))

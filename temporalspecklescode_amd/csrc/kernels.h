@@ -29,6 +29,7 @@ struct NeeParams {
     float tauCutF;         // early stop of far-field walks: exp(-tau/2) is exactly 0 in FP32 beyond it
     double tauCutD;
     uint32_t npaths;
+    uint32_t lpp;          // lanes per path (power of two, 8..64), set by launchNee
 };
 
 hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc, const SourceView& il,

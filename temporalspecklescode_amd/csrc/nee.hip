@@ -173,6 +173,12 @@ template <typename T> __device__ inline T waveSum(T v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// Sum over an aligned group of lpp lanes (lpp a power of two <= 64): xor partners stay inside
+// the group, so groups working on different paths never mix.
+template <typename T> __device__ inline T groupSum(T v, uint32_t lpp) {
+    for (uint32_t o = lpp >> 1; o > 0; o >>= 1) v += __shfl_xor(v, (int)o, 64);
+    return v;
+}
 
 __device__ inline double frac1(double x) { return x - floor(x); }
 
@@ -183,13 +189,13 @@ __device__ inline typename Cx<R>::T cis(R amp, double phaseCycles) {
     return Cx<R>::make(amp * c, amp * s);
 }
 
-template <int DIMS, typename R, int IT>
-__global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, SourceView il, SourceView vw, FramesView fr,
-                                             VertexBatch vb, NeeParams prm, NeeOut out) {
+// First-vertex work of one path, on a group of lpp lanes: path probabilities, illumination terms
+// a and A, single-scattering view terms b (and zero B rows when the path has one vertex).
+template <int DIMS, typename R>
+__device__ inline void nee_path(const MediumView& m, const ScatterView& sc, const SourceView& il, const SourceView& vw,
+                                const FramesView& fr, const VertexBatch& vb, const NeeParams& prm, const NeeOut& out,
+                                uint32_t p, uint32_t sub, uint32_t lpp, uint32_t* stepsOut) {
     typedef typename Cx<R>::T C;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (p >= prm.npaths) return;
     const uint32_t P = prm.npaths, L = il.n, V = vw.n, NT = fr.nt;
     const bool corr = prm.output == 1;
     const double invLambda = 1.0 / prm.lambda;
@@ -204,12 +210,12 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
     const uint64_t off = vb.off[p];
     const C zero = Cx<R>::make(0, 0);
     if (K == 0) {
-        for (uint32_t i = lane; i < L * NT; i += 64) {
+        for (uint32_t i = sub; i < L * NT; i += lpp) {
             uint32_t l = i % L, t = i / L;
             A[((size_t)t * P + p) * L + l] = zero;
             a[((size_t)t * P + p) * L + l] = zero;
         }
-        for (uint32_t i = lane; i < V * NT; i += 64) {
+        for (uint32_t i = sub; i < V * NT; i += lpp) {
             uint32_t v = i % V, t = i / V;
             B[((size_t)t * P + p) * V + v] = zero;
             b[((size_t)t * P + p) * V + v] = zero;
@@ -229,7 +235,7 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
 
     // ---- path probabilities from the central first vertex (pathProbabilitySingle/Multiple)
     R s1 = 0, s2 = 0;
-    for (uint32_t l = lane; l < L; l += 64) {
+    for (uint32_t l = sub; l < L; l += lpp) {
         R pr;
         if (il.type == 2) {
             typename Pos<R>::V p1 = Pos<R>::pick(x1c.pf, x1c.pd);
@@ -254,15 +260,15 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
             s2 += pr * phasePdf<DIMS, R>(sc, m1, ddot<DIMS>(in, seg12));
         }
     }
-    s1 = waveSum(s1);
-    s2 = waveSum(s2);
+    s1 = groupSum(s1, lpp);
+    s2 = groupSum(s2, lpp);
     const R p1 = s1 / (R)L, p2 = s2 / (R)L;
     const R w1 = p1 > 0 ? (R)1 / sqrtR<R>(p1) : (R)0;
     const R w2 = p2 > 0 ? (R)1 / sqrtR<R>(p2) : (R)0;
     const bool isoPointIl = il.type == 1 && sc.allIsotropic;
 
     // ---- illumination terms a (single) and A (multiple)
-    for (uint32_t i = lane; i < L * NT; i += 64) {
+    for (uint32_t i = sub; i < L * NT; i += lpp) {
         const uint32_t l = i % L, t = i / L;
         R ga[2], fa[2] = {0, 0};
         double gp[2];
@@ -290,13 +296,13 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
     }
 
     if (!multi)
-        for (uint32_t i = lane; i < V * NT; i += 64) {
+        for (uint32_t i = sub; i < V * NT; i += lpp) {
             const uint32_t v = i % V, t = i / V;
             B[((size_t)t * P + p) * V + v] = zero;
         }
 
     // ---- single-scattering view terms b = g_v(x1^t)
-    for (uint32_t i = lane; i < V * NT; i += 64) {
+    for (uint32_t i = sub; i < V * NT; i += lpp) {
         const uint32_t v = i % V, t = i / V;
         R ga[2];
         double gp[2];
@@ -309,10 +315,23 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
         const size_t o = ((size_t)t * P + p) * V + v;
         b[o] = corr ? cis<R>(ga[0] * ga[1], gp[0] - gp[1]) : cis<R>(ga[0], gp[0]);
     }
+    *stepsOut += steps;
+}
 
+template <int DIMS, typename R, int IT>
+__global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, SourceView il, SourceView vw, FramesView fr,
+                                             VertexBatch vb, NeeParams prm, NeeOut out) {
+    typedef typename Cx<R>::T C;
+    // lanes are split into groups of lpp lanes, one path per group (several paths per wavefront
+    // when L and V are small)
+    const uint32_t lane = threadIdx.x & 63, lpp = prm.lpp, sub = lane & (lpp - 1);
+    const uint32_t p = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / lpp) + lane / lpp;
+    uint32_t steps = 0;
+    if (p < prm.npaths) nee_path<DIMS, R>(m, sc, il, vw, fr, vb, prm, out, p, sub, lpp, &steps);
     uint32_t tot = waveSum(steps);
     if (lane == 0 && out.steps) atomicAdd(out.steps, (unsigned long long)tot);
 }
+
 
 
 // Multiple-scattering view terms B (the hot loop): one wavefront per path, one (view, frame)
@@ -322,13 +341,11 @@ template <int DIMS, typename R, int VWFF, int CORR>
 __global__ void __launch_bounds__(256) k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr,
                                                   VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (p >= prm.npaths) return;
+    const uint32_t lane = threadIdx.x & 63, lpp = prm.lpp, sub = lane & (lpp - 1);
+    const uint32_t p = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / lpp) + lane / lpp;
     const bool poolOk = (uint64_t)vb.off[prm.npaths] <= vb.cap;
-    const uint32_t K = poolOk ? vb.k[p] : 0u;
-    if (K < 2) return;                      // k_nee wrote the zero rows
-    const uint64_t off = vb.off[p];
+    const uint32_t K = (p < prm.npaths && poolOk) ? vb.k[p] : 0u;   // K < 2: k_nee wrote the zero rows
+    const uint64_t off = p < prm.npaths ? vb.off[p] : 0;
     const uint32_t P = prm.npaths, V = vw.n, NT = fr.nt;
     const double invLambda = 1.0 / prm.lambda;
     const R tauCut = VWFF ? (sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD) : (R)3e38;
@@ -336,8 +353,8 @@ __global__ void __launch_bounds__(256) k_nee_view(MediumView m, ScatterView sc, 
     const bool isoPoint = !VWFF && sc.allIsotropic;
     uint32_t steps = 0;
     const uint32_t items = V * NT;
-    for (uint32_t base = 0; base < items; base += 64) {
-        const uint32_t i = base + lane;
+    for (uint32_t base = 0; base < items && K >= 2; base += lpp) {
+        const uint32_t i = base + sub;
         const bool active = i < items;
         const uint32_t v = active ? i % V : 0, t = active ? i / V : 0;
         R accRe = 0, accIm = 0;
@@ -396,7 +413,8 @@ template <int DIMS, typename R>
 static void launchNeeT(int it, const MediumView& m, const ScatterView& sc, const SourceView& il, const SourceView& vw,
                        const FramesView& fr, const VertexBatch& vb, const NeeParams& p, const NeeOut& out,
                        hipStream_t st) {
-    dim3 grid((p.npaths + 3) / 4), block(256);
+    const uint32_t ppb = 4 * (64 / p.lpp);   // paths per 256-thread block
+    dim3 grid((p.npaths + ppb - 1) / ppb), block(256);
     (void)it;
     hipLaunchKernelGGL((k_nee<DIMS, R, 1>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
     const bool ff = vw.type == 2, corr = p.output == 1;
@@ -410,14 +428,18 @@ hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView&
                      const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
                      const NeeOut& out, hipStream_t st) {
     // one view item per lane per pass over the path's vertices: the vertex records are re-read
-    // (wave-uniform, L1-resident) instead of holding many accumulators in registers
+    // (group-uniform, L1-resident) instead of holding many accumulators in registers
     int it = 1;
+    NeeParams q = p;
+    uint32_t want = il.n > vw.n * fr.nt ? il.n : vw.n * fr.nt;
+    q.lpp = 64;
+    while (q.lpp > 8 && q.lpp / 2 >= want) q.lpp /= 2;
     if (dims == 2) {
-        if (fp64) launchNeeT<2, double>(it, m, sc, il, vw, fr, vb, p, out, st);
-        else launchNeeT<2, float>(it, m, sc, il, vw, fr, vb, p, out, st);
+        if (fp64) launchNeeT<2, double>(it, m, sc, il, vw, fr, vb, q, out, st);
+        else launchNeeT<2, float>(it, m, sc, il, vw, fr, vb, q, out, st);
     } else {
-        if (fp64) launchNeeT<3, double>(it, m, sc, il, vw, fr, vb, p, out, st);
-        else launchNeeT<3, float>(it, m, sc, il, vw, fr, vb, p, out, st);
+        if (fp64) launchNeeT<3, double>(it, m, sc, il, vw, fr, vb, q, out, st);
+        else launchNeeT<3, float>(it, m, sc, il, vw, fr, vb, q, out, st);
     }
     return hipGetLastError();
 }

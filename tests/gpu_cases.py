@@ -33,3 +33,33 @@ def slab_frames(paths=1024, nl=6, nv=5, nt=5, output="field", U=0.0, seed=4):
     cfg["sampler"]["U"] = {"x": np.array([0.0, U]), "y": np.zeros(2), "z": np.zeros(2)}
     cfg["tracer"] = {"type": "nee", "output": output}
     return cfg
+
+
+def hetero2_points(paths=256, n=12, seed=5, area=False):
+    """2D heterogeneous grid with non-uniform edges and point sources outside the box."""
+    edges = np.concatenate([[-6.0], np.sort(np.random.default_rng(seed).uniform(-5.5, 5.5, n - 1)), [6.0]])
+    grid = W.random_materials((n, n), seed, 3)
+    return {
+        "simulation": {"batchSize": paths, "seed": seed, "lambda": 0.7},
+        "medium": {"type": "Heterogeneous", "xAxis": edges, "yAxis": edges, "materialGrid": grid,
+                   "sigs": [0.0, 0.4, 0.9, 1.6], "siga": [0.0, 0.01, 0.03, 0.02]},
+        "scattering": {"type": ["HG"] * 4, "amplitudeFunction": [0.7, 0.7, 0.5, 0.3]},
+        "illumination": {"type": "PointSource", "location": {"x": [-2.0, 1.0, 3.0], "y": [-7.0, -7.0, -6.5]},
+                         "isAreaSource": area},
+        "view": {"type": "PointSource", "location": {"x": [-3.0, 0.0, 2.5, 4.0], "y": [7.0, 6.5, 7.0, 7.5]},
+                 "isAreaSource": area},
+        "sampler": {"type": "TemporalSampler", "t": [0.0], "D": np.zeros(4),
+                    "U": {"x": np.zeros(4), "y": np.zeros(4)}},
+        "tracer": {"type": "nee"},
+    }
+
+
+def moving_ff_points(paths=256, seed=6, temporal_amp=True):
+    """3D grid, far-field illumination and point views, linear motion + Brownian frames."""
+    cfg = hetero3_small(paths, n=12, seed=seed)
+    cfg["view"] = {"type": "PointSource", "location": {"x": [-3.0, 0.0, 3.0], "y": [1.0, -2.0, 0.5],
+                                                        "z": [7.0, 7.5, 8.0]}}
+    cfg["sampler"] = {"type": "TemporalSampler", "t": np.array([0.0, 1.0, 2.5]), "D": np.array([0.0, 1e-3, 0.0, 4e-3]),
+                      "U": {"x": np.array([0.0, 0.02, -0.01, 0.0]), "y": np.zeros(4), "z": np.array([0.0, 0.0, 0.03, 0.01])},
+                      "isTemporalAmplitude": temporal_amp}
+    return cfg
