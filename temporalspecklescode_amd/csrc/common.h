@@ -6,7 +6,9 @@
 //     `__constant__` global state (the reference keeps medium, scattering and sampler tables in
 //     process-global `__constant__` symbols, MediumInterface.cuh:30-35, Scattering.cuh:15-29,
 //     TemporalSampler.cuh:14-38), so any number of requests can be in flight on one GPU;
-//   * the material grid is one byte per voxel (ids <= 9), 2 MiB for 128^3, resident in L2;
+//   * the material grid is one byte per voxel (ids <= 9): 2 MiB for 128^3 stays resident in one
+//     XCD's 4 MB L2, where a per-voxel FP32 sigt grid (8 MiB) would not; the NEE walks look sigt
+//     up in a per-block LDS copy of the material table;
 //   * vertices are written once by the transport kernel as SoA records and read by the NEE kernel
 //     as wave-uniform broadcasts.
 #pragma once
@@ -97,7 +99,6 @@ struct MediumView {
     float e0[3], invH[3];          // first edge and inverse spacing for the fast locate
     float h[3];                    // spacing of an equally spaced axis
     int allUniform;                // every axis equally spaced: the branch-free FP32 walk applies
-    const float* sigtGrid;         // sigt per cell (FP32), x fastest: the walk's only memory read
 };
 
 // Scattering: Scattering.cuh:15-29, HG table :500-572.

@@ -180,8 +180,10 @@ __device__ inline f3 sampleDirection(const ScatterView& s, uint32_t mat, const f
 // from the fixed origin p (one rounding per crossing instead of an accumulated position) and the
 // axis choice keeps the reference's strict-< tie order (x, then y, else z). The walk stops early
 // once tau > tauCut; callers pass a cut beyond which the result they form is exactly zero in R.
+// `sig` is the per-material sigt table (an LDS copy in the kernels).
 template <int DIMS, typename R, typename V>
-__host__ __device__ inline R opticalDepth(const MediumView& m, const V& p, const V& d, R tauCut, uint32_t* steps) {
+__host__ __device__ inline R opticalDepth(const MediumView& m, const R* sig, const V& p, const V& d, R tauCut,
+                                          uint32_t* steps) {
     if (!inside<DIMS, R>(m, p)) return (R)0;
     const R* e0 = Real<R>::edges(m, 0);
     const R* e1 = Real<R>::edges(m, 1);
@@ -198,7 +200,7 @@ __host__ __device__ inline R opticalDepth(const MediumView& m, const V& p, const
     R t1 = (e1[b1 + u1] - p.y) * i1;
     R t2 = DIMS == 3 ? (e2[b2 + u2] - p.z) * i2 : (R)3e38;
     R tcur = 0, tau = 0;
-    R st = Real<R>::sigt(m, m.grid[cellIndex(m, b0, b1, b2)]);
+    R st = sig[m.grid[cellIndex(m, b0, b1, b2)]];
     uint32_t cnt = 0;
     for (;;) {
         ++cnt;
@@ -222,7 +224,7 @@ __host__ __device__ inline R opticalDepth(const MediumView& m, const V& p, const
             t2 = (e2[b2 + u2] - p.z) * i2;
         }
         if (tau > tauCut) break;
-        st = Real<R>::sigt(m, m.grid[cellIndex(m, b0, b1, b2)]);
+        st = sig[m.grid[cellIndex(m, b0, b1, b2)]];
     }
     if (steps) *steps += cnt;
     return tau;
@@ -231,59 +233,83 @@ __host__ __device__ inline R opticalDepth(const MediumView& m, const V& p, const
 // Branch-free FP32 walk for equally spaced grids (the common case). Per step: pick the axis of
 // the nearest face with the reference's tie order, add segment * sigt, advance the cell index by
 // that axis' stride. The crossing parameter of the k-th face along an axis is
-// t_first + k * t_delta (one rounding, no accumulation). The only memory read per step is the
-// next cell's sigt, issued one step ahead of its use.
+// t_first + k * t_delta (one rounding, no accumulation), recomputed for every axis from its
+// crossing count. The walk ends at the first crossing at or beyond tExit, the parameter of the
+// box face the ray leaves through, computed by the same expression as the crossings, so the exit
+// test needs no per-axis counters and does not wait for memory.
+//
+// Memory per step: the material byte of the next cell (global, L2-resident) and its sigt from
+// the table `sig` (LDS in the kernels). The three stages are software-pipelined one step apart:
+// step j reads the byte of cell j+1, looks up sigt of cell j (byte read in step j-1) and adds
+// segment(j-1) * sigt(j-1) (looked up in step j-1), so each read has a whole step of independent
+// work to hide behind. The sum is formed in the same order as a sequential walk and the tauCut
+// test sees the same partial sums, so the result is bit-identical to one.
 template <int DIMS>
-__host__ __device__ inline float opticalDepthFast(const MediumView& m, const f3& p, const f3& d, float tauCut, uint32_t* steps) {
+__host__ __device__ inline float opticalDepthFast(const MediumView& m, const float* sig, const f3& p, const f3& d,
+                                                  float tauCut, uint32_t* steps) {
     if (!inside<DIMS, float>(m, p)) return 0.f;
     const float pp[3] = {p.x, p.y, DIMS == 3 ? p.z : 0.f};
     const float dd[3] = {d.x, d.y, DIMS == 3 ? d.z : 0.f};
-    float tf[3], td[3];
-    int kmax[3], stride[3];
-    int cell = 0, mul = 1;
+    float tf[3] = {3.0e38f, 3.0e38f, 3.0e38f}, td[3] = {0.f, 0.f, 0.f};
+    uint32_t stride[3] = {0u, 0u, 0u};   // signed strides in two's complement: cell + stride wraps
+    uint32_t cell = 0u, mul = 1u;
+    float tExit = 3.0e38f;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        if (a >= DIMS) {
-            tf[a] = 3.0e38f; td[a] = 0.f; kmax[a] = 0x7fffffff; stride[a] = 0;
-            continue;
-        }
+    for (int a = 0; a < DIMS; ++a) {
         const int n = (int)m.n[a];
         const int b = locate<float>(m, a, pp[a]);
         const float inv = fabsf(dd[a]) < 1e-12f ? (dd[a] >= 0.f ? 1e12f : -1e12f) : 1.0f / dd[a];
         const int up = dd[a] >= 0.f;
         tf[a] = (m.ef[a][b + up] - pp[a]) * inv;
         td[a] = m.h[a] * fabsf(inv);
-        kmax[a] = up ? n - 1 - b : b;
-        stride[a] = up ? mul : -mul;
-        cell += b * mul;
-        mul *= n;
+        // faces crossed before the one that leaves the box along this axis
+        tExit = fminf(tExit, fmaf((float)(up ? n - 1 - b : b), td[a], tf[a]));
+        stride[a] = up ? mul : 0u - mul;
+        cell += (uint32_t)b * mul;
+        mul *= (uint32_t)n;
     }
     float t0 = tf[0], t1 = tf[1], t2 = tf[2];
-    int k0 = 0, k1 = 0, k2 = 0;
+    float k0 = 0.f, k1 = 0.f, k2 = 0.f;
     float tcur = 0.f, tau = 0.f;
-    float st = m.sigtGrid[cell];
-    uint32_t cnt = 0;
-    for (;;) {
-        ++cnt;
-        const bool m0 = (t0 < t1) && (t0 < t2);
-        const bool m1 = !m0 && (t1 < t2);
-        const float tn = m0 ? t0 : (m1 ? t1 : t2);
-        k0 += m0 ? 1 : 0;
-        k1 += m1 ? 1 : 0;
-        k2 += (!m0 && !m1) ? 1 : 0;
-        const bool out = (k0 > kmax[0]) | (k1 > kmax[1]) | (k2 > kmax[2]);
-        const int next = out ? cell : cell + (m0 ? stride[0] : (m1 ? stride[1] : stride[2]));
-        const float stNext = m.sigtGrid[next];
-        tau = fmaf(tn - tcur, st, tau);
-        tcur = tn;
-        if (out || tau > tauCut) break;
-        cell = next;
-        st = stNext;
-        t0 = m0 ? fmaf((float)k0, td[0], tf[0]) : t0;
-        t1 = m1 ? fmaf((float)k1, td[1], tf[1]) : t1;
-        t2 = (!m0 && !m1) ? fmaf((float)k2, td[2], tf[2]) : t2;
+    // One step. In: MAT = byte of the current cell (read last step), SEGI/SIGI = segment and sigt
+    // of the previous cell. Out: MATN (read issued), SEGO/SIGO for this cell, CODE = 0 to go on,
+    // 1 when tau passed the cut, 2 when this cell is the last one (its segment is added after the
+    // loop, so no step waits for its own reads).
+#define SSTMC_WALK_STEP(MAT, SEGI, SIGI, MATN, SEGO, SIGO, CODE)                                   \
+    {                                                                                               \
+        const bool m0 = DIMS == 3 ? (t0 < t1) && (t0 < t2) : t0 < t1;                               \
+        const bool m1 = !m0 && (DIMS == 3 ? t1 < t2 : true);                                        \
+        const bool m2 = !m0 && !m1;                                                                 \
+        const float tn = m0 ? t0 : (m1 ? t1 : t2);                                                  \
+        const bool out = tn >= tExit;                                                               \
+        cell += out ? 0u : (m0 ? stride[0] : (m1 ? stride[1] : stride[2]));                         \
+        MATN = m.grid[cell];                                                                        \
+        SIGO = sig[MAT];                                                                            \
+        SEGO = tn - tcur;                                                                           \
+        tcur = tn;                                                                                  \
+        k0 += m0 ? 1.f : 0.f;                                                                       \
+        k1 += m1 ? 1.f : 0.f;                                                                       \
+        if (DIMS == 3) k2 += m2 ? 1.f : 0.f;                                                        \
+        t0 = fmaf(k0, td[0], tf[0]);                                                                \
+        t1 = fmaf(k1, td[1], tf[1]);                                                                \
+        if (DIMS == 3) t2 = fmaf(k2, td[2], tf[2]);                                                 \
+        tau = fmaf(SEGI, SIGI, tau);  /* previous cell */                                           \
+        CODE = tau > tauCut ? 1 : (out ? 2 : 0);                                                    \
     }
-    if (steps) *steps += cnt;
+    // two steps per iteration with the pipeline registers swapping roles (no register copies)
+    uint32_t ma = m.grid[cell], mb;
+    float sga = 0.f, sia = 0.f, sgb, sib;       // nothing before the first cell: adds exactly 0
+    float lastSeg = 0.f, lastSig = 0.f;
+    int code;
+    for (;;) {
+        SSTMC_WALK_STEP(ma, sga, sia, mb, sgb, sib, code)
+        if (code) { lastSeg = sgb; lastSig = sib; break; }
+        SSTMC_WALK_STEP(mb, sgb, sib, ma, sga, sia, code)
+        if (code) { lastSeg = sga; lastSig = sia; break; }
+    }
+#undef SSTMC_WALK_STEP
+    if (code == 2) tau = fmaf(lastSeg, lastSig, tau);   // last cell
+    if (steps) *steps += (uint32_t)(k0 + k1 + k2);
     return tau;
 }
 

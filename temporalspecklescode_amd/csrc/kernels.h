@@ -29,7 +29,9 @@ struct NeeParams {
     float tauCutF;         // early stop of far-field walks: exp(-tau/2) is exactly 0 in FP32 beyond it
     double tauCutD;
     uint32_t npaths;
-    uint32_t lpp;          // lanes per path (power of two, 8..64), set by launchNee
+    uint32_t lpp;          // k_nee: lanes per path (power of two, 8..64), set by launchNee
+    uint32_t lppView;      // k_nee_view: lanes per work unit (power of two, 8..64), set by launchNee
+    uint32_t viewBlocks;   // k_nee_view: work units per path (blocks of lppView view items)
 };
 
 hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc, const SourceView& il,

@@ -86,24 +86,25 @@ __device__ inline R phaseAmp(const ScatterView& s, uint32_t m, double c) {
 }
 
 // Voxel walk dispatch: the branch-free FP32 walk on equally spaced grids, the general walk
-// (edge tables, FP64 when R is double) otherwise.
+// (edge tables, FP64 when R is double) otherwise. `sig` is the block's LDS copy of the
+// per-material sigt table.
 template <int DIMS, typename R, typename V>
-__device__ inline R walk(const MediumView& m, const V& p, const V& d, R tauCut, uint32_t* steps);
-template <>
-__device__ inline float walk<2, float, f3>(const MediumView& m, const f3& p, const f3& d, float tauCut, uint32_t* steps) {
-    return m.allUniform ? opticalDepthFast<2>(m, p, d, tauCut, steps) : opticalDepth<2, float>(m, p, d, tauCut, steps);
+__device__ inline R walk(const MediumView& m, const R* sig, const V& p, const V& d, R tauCut, uint32_t* steps) {
+    if constexpr (sizeof(R) == 4) {
+        if (m.allUniform) return opticalDepthFast<DIMS>(m, sig, p, d, tauCut, steps);
+    }
+    return opticalDepth<DIMS, R>(m, sig, p, d, tauCut, steps);
 }
-template <>
-__device__ inline float walk<3, float, f3>(const MediumView& m, const f3& p, const f3& d, float tauCut, uint32_t* steps) {
-    return m.allUniform ? opticalDepthFast<3>(m, p, d, tauCut, steps) : opticalDepth<3, float>(m, p, d, tauCut, steps);
-}
-template <>
-__device__ inline double walk<2, double, d3>(const MediumView& m, const d3& p, const d3& d, double tauCut, uint32_t* steps) {
-    return opticalDepth<2, double>(m, p, d, tauCut, steps);
-}
-template <>
-__device__ inline double walk<3, double, d3>(const MediumView& m, const d3& p, const d3& d, double tauCut, uint32_t* steps) {
-    return opticalDepth<3, double>(m, p, d, tauCut, steps);
+
+// Per-block LDS copy of the per-material sigt table in R. Every thread of the block must call it.
+template <typename R>
+__device__ inline const R* loadSigTable(const MediumView& m, R* lds) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < kMaxMaterials; ++i) lds[i] = Real<R>::sigt(m, i);
+    }
+    __syncthreads();
+    return lds;
 }
 template <int DIMS, typename R> __device__ inline R isoAmp() {
     return DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678;
@@ -132,8 +133,8 @@ __device__ inline Geo vertexFrame(const VertexBatch& vb, const FramesView& fr, u
 // g of one source at one position: amplitude (with sqrt of the transmittance) and phase in cycles.
 // FarField: gDirectionKernel (FarField.cuh:223-263); PointSource: gPointKernel (PointSource.cuh:117-172).
 template <int DIMS, typename R>
-__device__ inline void gTerm(const MediumView& m, const SourceView& s, uint32_t i, const Geo& x, double invLambda,
-                             R tauCut, uint32_t* steps, R* amp, double* ph) {
+__device__ inline void gTerm(const MediumView& m, const R* sig, const SourceView& s, uint32_t i, const Geo& x,
+                             double invLambda, R tauCut, uint32_t* steps, R* amp, double* ph) {
     typedef typename Pos<R>::V V;
     V p = Pos<R>::pick(x.pf, x.pd);
     if (s.type == 2) {
@@ -145,7 +146,7 @@ __device__ inline void gTerm(const MediumView& m, const SourceView& s, uint32_t 
         if (sizeof(R) == 4) w = Pos<R>::fromD(mkd(s.illum ? -(double)s.pf[3 * i] : (double)s.pf[3 * i],
                                                   s.illum ? -(double)s.pf[3 * i + 1] : (double)s.pf[3 * i + 1],
                                                   s.illum ? -(double)s.pf[3 * i + 2] : (double)s.pf[3 * i + 2]));
-        R tau = walk<DIMS, R>(m, p, w, tauCut, steps);
+        R tau = walk<DIMS, R>(m, sig, p, w, tauCut, steps);
         *amp = expR<R>(-(R)0.5 * tau);
     } else {
         d3 sp = mkd(s.pd[3 * i], s.pd[3 * i + 1], DIMS == 3 ? s.pd[3 * i + 2] : 0.0);
@@ -154,7 +155,7 @@ __device__ inline void gTerm(const MediumView& m, const SourceView& s, uint32_t 
         double rr = 1.0 / r;
         *ph = r * invLambda;
         V w = Pos<R>::fromD(mkd(q.x * rr, q.y * rr, q.z * rr));
-        R tau = walk<DIMS, R>(m, p, w, (R)3e38, steps);
+        R tau = walk<DIMS, R>(m, sig, p, w, (R)3e38, steps);
         double fac = DIMS == 3 ? rr * 0.531125966013598 /* sqrt(ISOTROPIC_AMPLITUDE) */ : sqrt(rr);
         if (s.area) fac *= sqrt(rr * fabs(DIMS == 3 ? x.pd.z - sp.z : x.pd.y - sp.y));
         *amp = expR<R>(-(R)0.5 * tau) * (R)fac;
@@ -192,7 +193,8 @@ __device__ inline typename Cx<R>::T cis(R amp, double phaseCycles) {
 // First-vertex work of one path, on a group of lpp lanes: path probabilities, illumination terms
 // a and A, single-scattering view terms b (and zero B rows when the path has one vertex).
 template <int DIMS, typename R>
-__device__ inline void nee_path(const MediumView& m, const ScatterView& sc, const SourceView& il, const SourceView& vw,
+__device__ inline void nee_path(const MediumView& m, const R* sig, const ScatterView& sc, const SourceView& il,
+                                const SourceView& vw,
                                 const FramesView& fr, const VertexBatch& vb, const NeeParams& prm, const NeeOut& out,
                                 uint32_t p, uint32_t sub, uint32_t lpp, uint32_t* stepsOut) {
     typedef typename Cx<R>::T C;
@@ -242,14 +244,14 @@ __device__ inline void nee_path(const MediumView& m, const ScatterView& sc, cons
             typename Pos<R>::V w;
             if (sizeof(R) == 4) w = Pos<R>::fromD(mkd(-(double)il.pf[3 * l], -(double)il.pf[3 * l + 1], -(double)il.pf[3 * l + 2]));
             else w = Pos<R>::fromD(mkd(-il.pd[3 * l], -il.pd[3 * l + 1], -il.pd[3 * l + 2]));
-            R tau = walk<DIMS, R>(m, p1, w, sizeof(R) == 4 ? (R)prm.tauCutF : tauInf, &steps);
+            R tau = walk<DIMS, R>(m, sig, p1, w, sizeof(R) == 4 ? (R)prm.tauCutF : tauInf, &steps);
             pr = expR<R>(-tau) * (sizeof(R) == 4 ? (R)il.probf[l] : (R)il.probd[l]);
         } else {
             d3 sp = mkd(il.pd[3 * l], il.pd[3 * l + 1], DIMS == 3 ? il.pd[3 * l + 2] : 0.0);
             d3 q = dsub(sp, x1c.pd);
             double rr = 1.0 / sqrt(ddot<DIMS>(q, q));
             typename Pos<R>::V w = Pos<R>::fromD(mkd(q.x * rr, q.y * rr, q.z * rr));
-            R tau = walk<DIMS, R>(m, Pos<R>::pick(x1c.pf, x1c.pd), w, tauInf, &steps);
+            R tau = walk<DIMS, R>(m, sig, Pos<R>::pick(x1c.pf, x1c.pd), w, tauInf, &steps);
             // PointSource.cuh:92-115 (2D factor rr * 8 pi kept as in the reference, quirk Q7)
             double fac = DIMS == 3 ? rr * rr / 12.566370614359172954 : rr * 25.132741228718345908;
             pr = expR<R>(-tau) * (R)fac;
@@ -276,7 +278,7 @@ __device__ inline void nee_path(const MediumView& m, const ScatterView& sc, cons
         for (int f = 0; f < nf; ++f) {
             uint32_t tt = (corr && f == 0) ? 0 : t;
             Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
-            gTerm<DIMS, R>(m, il, l, x1t, invLambda, il.type == 2 ? tauCutFF : tauInf, &steps, &ga[f], &gp[f]);
+            gTerm<DIMS, R>(m, sig, il, l, x1t, invLambda, il.type == 2 ? tauCutFF : tauInf, &steps, &ga[f], &gp[f]);
             if (multi) {
                 Geo x2t = vertexFrame(vb, fr, off + 1, tt, x2m);
                 const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
@@ -310,7 +312,7 @@ __device__ inline void nee_path(const MediumView& m, const ScatterView& sc, cons
         for (int f = 0; f < nf; ++f) {
             uint32_t tt = (corr && f == 0) ? 0 : t;
             Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
-            gTerm<DIMS, R>(m, vw, v, x1t, invLambda, vw.type == 2 ? tauCutFF : tauInf, &steps, &ga[f], &gp[f]);
+            gTerm<DIMS, R>(m, sig, vw, v, x1t, invLambda, vw.type == 2 ? tauCutFF : tauInf, &steps, &ga[f], &gp[f]);
         }
         const size_t o = ((size_t)t * P + p) * V + v;
         b[o] = corr ? cis<R>(ga[0] * ga[1], gp[0] - gp[1]) : cis<R>(ga[0], gp[0]);
@@ -326,23 +328,31 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
     // when L and V are small)
     const uint32_t lane = threadIdx.x & 63, lpp = prm.lpp, sub = lane & (lpp - 1);
     const uint32_t p = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / lpp) + lane / lpp;
+    __shared__ R sigLds[kMaxMaterials];
+    const R* sig = loadSigTable<R>(m, sigLds);
     uint32_t steps = 0;
-    if (p < prm.npaths) nee_path<DIMS, R>(m, sc, il, vw, fr, vb, prm, out, p, sub, lpp, &steps);
+    if (p < prm.npaths) nee_path<DIMS, R>(m, sig, sc, il, vw, fr, vb, prm, out, p, sub, lpp, &steps);
     uint32_t tot = waveSum(steps);
     if (lane == 0 && out.steps) atomicAdd(out.steps, (unsigned long long)tot);
 }
 
 
 
-// Multiple-scattering view terms B (the hot loop): one wavefront per path, one (view, frame)
-// item per lane per pass over the path's vertices. Templated on the view source type and the
-// output mode so the loop carries only what it needs.
+// Multiple-scattering view terms B (the hot loop). Work unit = (path, block of lppView view
+// items): one (view, frame) item per lane, looping over the path's vertices k >= 2 (vertex
+// records are group-uniform broadcasts). Splitting a path over its item blocks keeps the longest
+// unit short: vertex counts are heavy-tailed (the longest path of a batch can hold ~15x the mean),
+// and a unit spanning all items of such a path would outlast a whole SIMD's share of the batch.
+// Templated on the view source type and the output mode so the loop carries only what it needs.
 template <int DIMS, typename R, int VWFF, int CORR>
 __global__ void __launch_bounds__(256) k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr,
                                                   VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
-    const uint32_t lane = threadIdx.x & 63, lpp = prm.lpp, sub = lane & (lpp - 1);
-    const uint32_t p = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / lpp) + lane / lpp;
+    __shared__ R sigLds[kMaxMaterials];
+    const R* sig = loadSigTable<R>(m, sigLds);
+    const uint32_t lane = threadIdx.x & 63, lpp = prm.lppView, sub = lane & (lpp - 1);
+    const uint64_t unit = (uint64_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / lpp) + lane / lpp;
+    const uint32_t p = (uint32_t)(unit / prm.viewBlocks), blk = (uint32_t)(unit % prm.viewBlocks);
     const bool poolOk = (uint64_t)vb.off[prm.npaths] <= vb.cap;
     const uint32_t K = (p < prm.npaths && poolOk) ? vb.k[p] : 0u;   // K < 2: k_nee wrote the zero rows
     const uint64_t off = p < prm.npaths ? vb.off[p] : 0;
@@ -353,9 +363,9 @@ __global__ void __launch_bounds__(256) k_nee_view(MediumView m, ScatterView sc, 
     const bool isoPoint = !VWFF && sc.allIsotropic;
     uint32_t steps = 0;
     const uint32_t items = V * NT;
-    for (uint32_t base = 0; base < items && K >= 2; base += lpp) {
-        const uint32_t i = base + sub;
-        const bool active = i < items;
+    {
+        const uint32_t i = blk * lpp + sub;
+        const bool active = i < items && K >= 2;
         const uint32_t v = active ? i % V : 0, t = active ? i / V : 0;
         R accRe = 0, accIm = 0;
         double len0 = 0, lent = 0;
@@ -380,7 +390,7 @@ __global__ void __launch_bounds__(256) k_nee_view(MediumView m, ScatterView sc, 
                     else { lent += ln; lacc = lent; }
                     R ga;
                     double gp;
-                    gTerm<DIMS, R>(m, vw, v, xk, invLambda, tauCut, &steps, &ga, &gp);
+                    gTerm<DIMS, R>(m, sig, vw, v, xk, invLambda, tauCut, &steps, &ga, &gp);
                     R fv;
                     if (isoPoint) {
                         fv = isoAmp<DIMS, R>();
@@ -417,6 +427,8 @@ static void launchNeeT(int it, const MediumView& m, const ScatterView& sc, const
     dim3 grid((p.npaths + ppb - 1) / ppb), block(256);
     (void)it;
     hipLaunchKernelGGL((k_nee<DIMS, R, 1>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    const uint64_t units = (uint64_t)p.npaths * p.viewBlocks, upb = 4 * (64 / p.lppView);
+    grid = dim3((uint32_t)((units + upb - 1) / upb));
     const bool ff = vw.type == 2, corr = p.output == 1;
     if (ff && !corr) hipLaunchKernelGGL((k_nee_view<DIMS, R, 1, 0>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
     else if (ff && corr) hipLaunchKernelGGL((k_nee_view<DIMS, R, 1, 1>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
@@ -434,6 +446,10 @@ hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView&
     uint32_t want = il.n > vw.n * fr.nt ? il.n : vw.n * fr.nt;
     q.lpp = 64;
     while (q.lpp > 8 && q.lpp / 2 >= want) q.lpp /= 2;
+    const uint32_t items = vw.n * fr.nt;
+    q.lppView = 64;
+    while (q.lppView > 8 && q.lppView / 2 >= items) q.lppView /= 2;
+    q.viewBlocks = (items + q.lppView - 1) / q.lppView;
     if (dims == 2) {
         if (fp64) launchNeeT<2, double>(it, m, sc, il, vw, fr, vb, q, out, st);
         else launchNeeT<2, float>(it, m, sc, il, vw, fr, vb, q, out, st);

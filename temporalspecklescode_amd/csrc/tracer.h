@@ -63,7 +63,6 @@ private:
     DevBuf<float> ef_[3];
     DevBuf<double> ed_[3];
     DevBuf<uint8_t> grid_;
-    DevBuf<float> sigtGrid_;
     std::vector<uint8_t> hostGrid_;
 };
 
@@ -197,6 +196,7 @@ public:
 private:
     int allocate();
     int growPool(uint64_t cap);
+    int sizePool();
     int runBatch(uint64_t pathBegin, uint32_t npaths, double* U, bool timed, uint32_t b);
     int contractSingle(uint32_t npaths, double* U);
 
@@ -220,6 +220,7 @@ private:
     DevBuf<float4> xm_;
     DevBuf<float> rho_, delta_;
     uint64_t cap_ = 0;
+    bool poolSized_ = false;   // sizePool ran: the pool holds a batch of this request with margin
     DevBuf<unsigned char> A_, B_, a_, b_;
     DevBuf<double> U_;
     DevBuf<float> Ff_;

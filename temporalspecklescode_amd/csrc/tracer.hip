@@ -101,12 +101,6 @@ int HeterogeneousMedium::setMaterial(const double* sigs, const double* siga, uin
         v_.albedo[m] = st > 0 ? (float)(ss / st) : 0.f;
     }
     v_.invMaj = (float)(1.0 / maj);
-    // per-cell sigt for the FP32 voxel walk (one 4-byte read per step, no table indirection)
-    std::vector<float> sg(hostGrid_.size());
-    for (size_t i = 0; i < sg.size(); ++i) sg[i] = v_.sigtf[hostGrid_[i]];
-    int e = sigtGrid_.upload(sg.data(), sg.size());
-    if (e) return e;
-    v_.sigtGrid = sigtGrid_.p;
     return SSTMC_NO_ERROR;
 }
 
@@ -496,7 +490,33 @@ uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint
     return terms + field + pool + ws + 24ull * P + (1ull << 20);
 }
 
+// Sizes the vertex pool from the exact vertex count of the first batch (its count pass and scan,
+// read back once), so that the batches queued after it do not overflow a pool sized by guess and
+// have to run again. The margin covers batch-to-batch variation; a batch that still overflows is
+// re-run by finish().
+int NextEventEstimation::sizePool() {
+    if (poolSized_) return SSTMC_NO_ERROR;
+    const FramesView& fr = sampler_.view();
+    VertexBatch vb{k_.p, misses_.p, off_.p, xm_.p, rho_.p, delta_.p, cap_, status_.p};
+    if (launchTransport(dims_, medium_.view(), sc_.view(), il_.view(), fr, vb, seed_, 0, P_, 0, stream_) != hipSuccess ||
+        launchScan(k_.p, off_.p, blockSums_.p, P_, stream_) != hipSuccess)
+        return SSTMC_KERNEL_ERROR;
+    uint32_t need = 0;
+    if (hipMemcpyAsync(&need, off_.p + P_, sizeof(need), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess)
+        return SSTMC_DEVICE_ERROR;
+    const uint64_t want = (uint64_t)need + need / 4 + 1024;
+    if (want > cap_) {
+        int rc = growPool(want);
+        if (rc) return rc;
+    }
+    poolSized_ = true;
+    return SSTMC_NO_ERROR;
+}
+
 int NextEventEstimation::begin() {
+    int rc = sizePool();
+    if (rc) return rc;
     uint32_t nb = opt_.fullIteration;
     if (evBatches_ < nb) {
         for (hipEvent_t ev : ev_) hipEventDestroy(ev);

@@ -91,7 +91,7 @@ def perlin_materials(n: int, seed: int, nmat: int = 4, base_freq: float = 1.0 / 
 
 
 def random_materials(shape, seed: int, nmat: int) -> np.ndarray:
-    """i.i.d. uniform ids 1..nmat-1... from Philox(seed, voxel index), voxel index x fastest."""
+    """i.i.d. uniform material ids 1..nmat from Philox(seed, voxel index), voxel index x fastest."""
     n = int(np.prod(shape))
     u = _uniform(seed, n)
     ids = 1 + np.minimum((u * nmat).astype(np.int64), nmat - 1)
@@ -267,7 +267,7 @@ def serving_stream(n_requests: int, seed: int = 7, max_paths_log2: int = 22, max
         if dims == 3:
             key = n
             if key not in media3:
-                media3[key] = random_materials((n, n, n), 100 + n, nmat)
+                media3[key] = random_materials((n, n, n), 100 + n, nmat - 1)
             grid = media3[key]
             medium = {"type": "Heterogeneous", "xAxis": edges, "yAxis": edges, "zAxis": edges,
                       "materialGrid": grid, "sigs": sigs, "siga": siga}
@@ -278,7 +278,7 @@ def serving_stream(n_requests: int, seed: int = 7, max_paths_log2: int = 22, max
                 xs = (u[i, 11] * 2 - 1) * (n / 2 - 1) * np.linspace(-1, 1, V)
                 vw = {"type": "PointSource", "location": {"x": xs, "y": np.zeros(V), "z": np.full(V, n / 2 + 1.0)}}
         else:
-            grid = random_materials((n, n), 200 + n, nmat)
+            grid = random_materials((n, n), 200 + n, nmat - 1)
             medium = {"type": "Heterogeneous", "xAxis": edges, "yAxis": edges, "materialGrid": grid,
                       "sigs": sigs, "siga": siga}
             thl = np.deg2rad(np.linspace(-8, 8, L))

@@ -98,9 +98,11 @@ def test_fp64_terms_match_oracle_on_gpu_vertices(case):
 
 @pytest.mark.parametrize("case,dims", [("c1", 2), ("hetero_ff", 3), ("hetero_ps", 3), ("frames", 3)])
 def test_fp32_terms_amplitude_and_phase(case, dims):
-    """FP32 NEE (walks, exponentials, phase functions) with FP64 phase: per-path amplitude
-    relative error <= 1e-5 (Frobenius over the path's L and V terms) and phase error <= 1e-4 rad
-    on every term within 1e-3 of the path's largest."""
+    """FP32 NEE (walks, exponentials, phase functions) with FP64 phase. Single terms (A, a, b):
+    per-path amplitude relative error <= 1e-5 (Frobenius over the path's L or V terms) and phase
+    error <= 1e-4 rad on every term within 1e-3 of the path's largest. B_v = sum_k G_k(v) adds
+    K-1 such terms with random phases; their errors add while the sum partly cancels
+    (E|sum| ~ sqrt(K-1) |G|), so B is held to a complex relative error <= 5e-5 per path."""
     cfg = {"c1": G.c1_small(256), "hetero_ff": G.hetero3_small(128), "hetero_ps": G.hetero3_small(128, point=True),
            "frames": G.slab_frames(128)}[case]
     cfg["precision"] = {"terms": "fp32", "contraction": "fp32"}
@@ -117,6 +119,10 @@ def test_fp32_terms_amplitude_and_phase(case, dims):
             nr = np.linalg.norm(ref)
             if nr == 0:
                 assert np.linalg.norm(got) == 0
+                continue
+            if name == "B":
+                err = np.linalg.norm(got - ref) / nr
+                assert err < 5e-5, (name, p, err)
                 continue
             amp = np.linalg.norm(np.abs(got) - np.abs(ref)) / nr
             big = np.abs(ref) > 1e-3 * np.abs(ref).max()
