@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+VALU_PEAK_LANE_INSTR = 256 * 64 * 2.4e9   # 256 CUs x 64 lanes/clk (4 SIMD16) x 2.4 GHz, non-packed
 
 
 def workload(name: str, batch: int, steps: int):
@@ -146,30 +147,41 @@ def main():
     value = contributions / wall
     print("bench: timed %.3f s, stages %s" % (wall, json.dumps(tm)), file=sys.stderr, flush=True)
 
-    # roofline: the stage with the largest device time is the dominant kernel
+    # roofline of the dominant kernel (k_nee_view, the multiple-scattering view terms: >90% of the
+    # device time), timed by HIP events around its launches on the tracer's stream
     nb = max(1, tm["batches"])
-    nee_ms = tm["nee_ms"] / nb
+    view_ms = tm["nee_view_ms"] / nb
     con_ms = (tm["contract_ms"] + tm["single_ms"]) / nb
-    vtx_per_path = tm["vertices"] / max(1, paths)
-    # NEE algorithmic bytes per path: four complex64 term rows written once (A, B, a, b) plus the
-    # vertex records read (20 B each, + 12 B per frame when scatterers move)
-    nee_bytes = args.batch * (16.0 * (nl + nv) * nt + 20.0 * vtx_per_path)
-    nee_gbs = nee_bytes / (nee_ms * 1e-3) / 1e9
+    vtx_launch = tm["vertices"] / nb
+    # algorithmic bytes of the tracing stage: 8 (L+V) T per path (complex64 term slabs written
+    # once for the contraction) + (64 + 16 T) per vertex (vertex state in and out)
+    alg_bytes = args.batch * 8.0 * (nl + nv) * nt + vtx_launch * (64.0 + 16.0 * nt)
+    gbs = alg_bytes / (view_ms * 1e-3) / 1e9
+    steps_s = tm["dda_steps_view"] / max(1e-9, tm["nee_view_ms"] * 1e-3)   # k_nee_view's own walks
+    prof = {}
+    tpath = os.path.join(ROOT, "profiles", "counters_%s.json" % args.workload)
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            prof = json.load(f)
+    traffic = None
+    if prof.get("batch") == args.batch and "k_nee_view_hbm_bytes" in prof:
+        traffic = prof["k_nee_view_hbm_bytes"]
+    roof = {"kernel": "k_nee_view", "bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 6), "traffic": traffic, "avg_launch_ms": round(view_ms, 3),
+            "algorithmic_bytes_per_launch": alg_bytes, "voxel_steps_per_s": steps_s}
+    # the walk is issue-bound, not HBM-bound: VALU lane-instructions per voxel step (from the
+    # SQ_INSTS_VALU counter over the same kernel) against the non-packed FP32 VALU issue peak
+    if "k_nee_view_valu_lane_instr_per_step" in prof:
+        ips = prof["k_nee_view_valu_lane_instr_per_step"]
+        ach = ips * steps_s
+        roof["valu"] = {"lane_instr_per_step": ips, "achieved": ach, "peak": VALU_PEAK_LANE_INSTR,
+                        "unit": "lane-instr/s", "frac": round(ach / VALU_PEAK_LANE_INSTR, 4)}
     flops = 2 * 8.0 * args.batch * nl * nv * nt   # multiple + single contraction, 8 flops each
     con_tflops = flops / (con_ms * 1e-3) / 1e12
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("nee_bytes_per_launch")
-    roof_nee = {"kernel": "k_nee", "bound": "hbm", "achieved": round(nee_gbs, 3), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(nee_gbs / HBM_PEAK_GBS, 6), "traffic": traffic,
-                "avg_launch_ms": round(nee_ms, 4), "algorithmic_bytes_per_launch": nee_bytes,
-                "voxel_steps_per_s": tm["dda_steps"] / max(1e-9, tm["nee_ms"] * 1e-3)}
     roof_con = {"kernel": "k_contract_bf16x3" if args.contraction == "bf16x3" else "k_contract_fma",
                 "bound": "mfma", "achieved": round(con_tflops, 3), "peak": MFMA_BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(con_tflops / MFMA_BF16_PEAK_TFLOPS, 6), "traffic": None,
-                "avg_launch_ms": round(con_ms, 4)}
-    dominant = roof_nee if nee_ms >= con_ms else roof_con
+                "avg_launch_ms": round(con_ms / 2, 4)}
 
     accuracy = None
     if not args.no_accuracy:
@@ -195,8 +207,9 @@ def main():
         "config": {"workload": args.workload, "batch_paths": args.batch, "L": nl, "V": nv, "frames": nt,
                    "paths": paths, "contraction": args.contraction},
         "paths_per_s": paths / wall, "vertices_per_s": tm["vertices"] / wall,
-        "roofline": dominant, "roofline_contraction": roof_con, "roofline_nee": roof_nee,
-        "accuracy": accuracy, "stage_ms": {k: tm[k] for k in ("transport_ms", "nee_ms", "contract_ms", "single_ms")},
+        "roofline": roof, "roofline_contraction": roof_con,
+        "accuracy": accuracy,
+        "stage_ms": {k: tm[k] for k in ("transport_ms", "nee_ms", "nee_view_ms", "contract_ms", "single_ms")},
     }
     if args.serving > 0:
         out["serving"] = serving(args.serving, args.serving_max_paths_log2)

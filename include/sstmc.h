@@ -190,6 +190,9 @@ typedef struct sstmc_timing {
     uint64_t vertices;             /* scattering vertices traced */
     uint64_t dda_steps;            /* voxel steps walked by the NEE attenuation */
     uint32_t batches;
+    double nee_view_ms;            /* part of nee_ms: multiple-scattering view terms (the voxel-walk
+                                      kernel that dominates nee_ms) */
+    uint64_t dda_steps_view;       /* part of dda_steps walked by that kernel */
 } sstmc_timing;
 int sstmc_last_timing(const sstmc_handle* h, sstmc_timing* out);
 

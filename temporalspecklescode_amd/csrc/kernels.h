@@ -13,7 +13,8 @@ struct NeeOut {
     void* B;          // multiple-scattering view terms (or sum_k G_k(0) conj G_k(t))
     void* a;          // single-scattering illumination terms g_l * w1 * exp(i 2 pi rho_1)
     void* b;          // single-scattering view terms g_v
-    unsigned long long* steps;   // voxel steps of the NEE walks (accumulated)
+    unsigned long long* steps;   // voxel steps of the NEE walks (accumulated): k_nee adds to
+                                 // steps[0], k_nee_view to steps[2]
 };
 
 // How single scattering is contracted (decided on the host from the source/scattering types).
@@ -41,9 +42,11 @@ hipError_t launchScan(const uint32_t* k, uint32_t* off, uint32_t* blockSums, uin
 hipError_t launchBatchTotals(const uint32_t* misses, const uint32_t* off, uint32_t n, unsigned long long* acc,
                              hipStream_t st);
 
+// k_nee (first vertex) then k_nee_view (vertices k >= 2); `between`, when not null, is recorded
+// on `st` between the two launches.
 hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
                      const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
-                     const NeeOut& out, hipStream_t st);
+                     const NeeOut& out, hipStream_t st, hipEvent_t between = nullptr);
 
 // Contraction U[t][v][l] += scale(l,v) * sum_p A[t][p][l] * B[t][p][v] for t < nt.
 // U is complex FP64 (interleaved). F (optional, real, [v][l]) multiplies the sum elementwise,

@@ -416,17 +416,17 @@ __global__ void __launch_bounds__(256) k_nee_view(MediumView m, ScatterView sc, 
         }
     }
     uint32_t tot = waveSum(steps);
-    if (lane == 0 && out.steps) atomicAdd(out.steps, (unsigned long long)tot);
+    if (lane == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
 }
 
 template <int DIMS, typename R>
-static void launchNeeT(int it, const MediumView& m, const ScatterView& sc, const SourceView& il, const SourceView& vw,
+static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceView& il, const SourceView& vw,
                        const FramesView& fr, const VertexBatch& vb, const NeeParams& p, const NeeOut& out,
-                       hipStream_t st) {
+                       hipStream_t st, hipEvent_t between) {
     const uint32_t ppb = 4 * (64 / p.lpp);   // paths per 256-thread block
     dim3 grid((p.npaths + ppb - 1) / ppb), block(256);
-    (void)it;
     hipLaunchKernelGGL((k_nee<DIMS, R, 1>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    if (between) hipEventRecord(between, st);
     const uint64_t units = (uint64_t)p.npaths * p.viewBlocks, upb = 4 * (64 / p.lppView);
     grid = dim3((uint32_t)((units + upb - 1) / upb));
     const bool ff = vw.type == 2, corr = p.output == 1;
@@ -438,10 +438,10 @@ static void launchNeeT(int it, const MediumView& m, const ScatterView& sc, const
 
 hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
                      const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
-                     const NeeOut& out, hipStream_t st) {
-    // one view item per lane per pass over the path's vertices: the vertex records are re-read
-    // (group-uniform, L1-resident) instead of holding many accumulators in registers
-    int it = 1;
+                     const NeeOut& out, hipStream_t st, hipEvent_t between) {
+    // lane groups: k_nee gives each path lpp lanes (its L and V items), k_nee_view gives each
+    // (path, block of lppView view items) unit lppView lanes; both shrink below a wavefront when
+    // there are few items so that several paths share one
     NeeParams q = p;
     uint32_t want = il.n > vw.n * fr.nt ? il.n : vw.n * fr.nt;
     q.lpp = 64;
@@ -451,11 +451,11 @@ hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView&
     while (q.lppView > 8 && q.lppView / 2 >= items) q.lppView /= 2;
     q.viewBlocks = (items + q.lppView - 1) / q.lppView;
     if (dims == 2) {
-        if (fp64) launchNeeT<2, double>(it, m, sc, il, vw, fr, vb, q, out, st);
-        else launchNeeT<2, float>(it, m, sc, il, vw, fr, vb, q, out, st);
+        if (fp64) launchNeeT<2, double>(m, sc, il, vw, fr, vb, q, out, st, between);
+        else launchNeeT<2, float>(m, sc, il, vw, fr, vb, q, out, st, between);
     } else {
-        if (fp64) launchNeeT<3, double>(it, m, sc, il, vw, fr, vb, q, out, st);
-        else launchNeeT<3, float>(it, m, sc, il, vw, fr, vb, q, out, st);
+        if (fp64) launchNeeT<3, double>(m, sc, il, vw, fr, vb, q, out, st, between);
+        else launchNeeT<3, float>(m, sc, il, vw, fr, vb, q, out, st, between);
     }
     return hipGetLastError();
 }

@@ -226,6 +226,12 @@ private:
     DevBuf<float> Ff_;
     DevBuf<double> Fd_;
     Workspace wsMulti_, wsSingle_;
+    // per batch: 0 start, 1 after transport, 5 between k_nee and k_nee_view, 2 after NEE,
+    // 3 after the multiple-scattering contraction, 4 after the single-scattering one
+    static constexpr size_t kEventsPerBatch = 6;
+    // device totals: 0 missed attempts, 1 vertices, 2 k_nee voxel steps, 3 largest batch (vertices),
+    // 4 k_nee_view voxel steps
+    static constexpr size_t kTotals = 5;
     std::vector<hipEvent_t> ev_;
     uint32_t evBatches_ = 0;
     sstmc_timing last_{};

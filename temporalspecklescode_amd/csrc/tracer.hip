@@ -410,7 +410,7 @@ NextEventEstimation::~NextEventEstimation() {
 int NextEventEstimation::allocate() {
     int e;
     if ((e = k_.alloc(P_)) || (e = misses_.alloc(P_)) || (e = off_.alloc(P_ + 1)) ||
-        (e = blockSums_.alloc((P_ + 1023) / 1024 + 1)) || (e = status_.alloc(4)) || (e = totals_.alloc(4)))
+        (e = blockSums_.alloc((P_ + 1023) / 1024 + 1)) || (e = status_.alloc(4)) || (e = totals_.alloc(kTotals)))
         return e;
     size_t elem = fp64_ ? 16 : 8;
     size_t na = (size_t)T_ * P_ * L_ * elem, nb = (size_t)T_ * P_ * V_ * elem;
@@ -454,7 +454,7 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     const SourceView& vw = vw_.view();
     const FramesView& fr = sampler_.view();
     VertexBatch vb{k_.p, misses_.p, off_.p, xm_.p, rho_.p, delta_.p, cap_, status_.p};
-    hipEvent_t* e = timed ? &ev_[(size_t)b * 5] : nullptr;
+    hipEvent_t* e = timed ? &ev_[(size_t)b * kEventsPerBatch] : nullptr;
     if (e) hipEventRecord(e[0], stream_);
     if (launchTransport(dims_, m, sc, il, fr, vb, seed_, pathBegin, npaths, 0, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
     if (launchScan(k_.p, off_.p, blockSums_.p, npaths, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
@@ -469,7 +469,8 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     prm.tauCutD = 1.0e300;
     prm.npaths = npaths;
     NeeOut out{A_.p, B_.p, a_.p, b_.p, totals_.p + 2};
-    if (launchNee(dims_, fp64_ ? 1 : 0, m, sc, il, vw, fr, vb, prm, out, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    if (launchNee(dims_, fp64_ ? 1 : 0, m, sc, il, vw, fr, vb, prm, out, stream_, e ? e[5] : nullptr) != hipSuccess)
+        return SSTMC_KERNEL_ERROR;
     if (e) hipEventRecord(e[2], stream_);
     if (launchContract(contractKind_, A_.p, B_.p, npaths, L_, V_, T_, nullptr, nullptr, 0, 1.0, U, &wsMulti_, stream_) !=
         hipSuccess)
@@ -520,13 +521,14 @@ int NextEventEstimation::begin() {
     uint32_t nb = opt_.fullIteration;
     if (evBatches_ < nb) {
         for (hipEvent_t ev : ev_) hipEventDestroy(ev);
-        ev_.assign((size_t)nb * 5, nullptr);
+        ev_.assign((size_t)nb * kEventsPerBatch, nullptr);
         for (auto& ev : ev_)
             if (hipEventCreate(&ev) != hipSuccess) return SSTMC_DEVICE_ERROR;
         evBatches_ = nb;
     }
     if (hipMemsetAsync(status_.p, 0, 4 * sizeof(uint32_t), stream_) != hipSuccess) return SSTMC_DEVICE_ERROR;
-    if (hipMemsetAsync(totals_.p, 0, 4 * sizeof(unsigned long long), stream_) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    if (hipMemsetAsync(totals_.p, 0, kTotals * sizeof(unsigned long long), stream_) != hipSuccess)
+        return SSTMC_DEVICE_ERROR;
     if (launchZeroU(U_.p, U_.n, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
     return SSTMC_NO_ERROR;
 }
@@ -547,7 +549,7 @@ int NextEventEstimation::enqueue() {
 int NextEventEstimation::finish(double* u, bool deviceDst) {
     if (hipStreamSynchronize(stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
     uint32_t st[4];
-    unsigned long long tot[4];
+    unsigned long long tot[kTotals];
     if (hipMemcpy(st, status_.p, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
     if (hipMemcpy(tot, totals_.p, sizeof(tot), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
     if (st[1]) return SSTMC_NO_HIT;
@@ -566,17 +568,19 @@ int NextEventEstimation::finish(double* u, bool deviceDst) {
     sstmc_timing t{};
     for (uint32_t b = 0; b < opt_.fullIteration; ++b) {
         float ms;
-        hipEvent_t* e = &ev_[(size_t)b * 5];
+        hipEvent_t* e = &ev_[(size_t)b * kEventsPerBatch];
         if (hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess) t.transport_ms += ms;
         if (hipEventElapsedTime(&ms, e[1], e[2]) == hipSuccess) t.nee_ms += ms;
+        if (hipEventElapsedTime(&ms, e[5], e[2]) == hipSuccess) t.nee_view_ms += ms;
         if (hipEventElapsedTime(&ms, e[2], e[3]) == hipSuccess) t.contract_ms += ms;
         if (hipEventElapsedTime(&ms, e[3], e[4]) == hipSuccess) t.single_ms += ms;
     }
     float ms;
-    if (opt_.fullIteration && hipEventElapsedTime(&ms, ev_[0], ev_[(size_t)(opt_.fullIteration - 1) * 5 + 4]) == hipSuccess)
+    if (opt_.fullIteration && hipEventElapsedTime(&ms, ev_[0], ev_[(size_t)(opt_.fullIteration - 1) * kEventsPerBatch + 4]) == hipSuccess)
         t.total_ms = ms;
     t.vertices = tot[1];
-    t.dda_steps = tot[2];
+    t.dda_steps = tot[2] + tot[4];
+    t.dda_steps_view = tot[4];
     t.batches = opt_.fullIteration;
     last_ = t;
     return SSTMC_NO_ERROR;
@@ -604,12 +608,12 @@ int NextEventEstimation::debugBatch(uint64_t pathBegin, uint32_t n, uint64_t* nv
     if ((rc = Ub.alloc(U_.n))) return rc;
     for (int attempt = 0; attempt < 8; ++attempt) {
         hipMemsetAsync(status_.p, 0, 4 * sizeof(uint32_t), stream_);
-        hipMemsetAsync(totals_.p, 0, 4 * sizeof(unsigned long long), stream_);
+        hipMemsetAsync(totals_.p, 0, kTotals * sizeof(unsigned long long), stream_);
         launchZeroU(Ub.p, Ub.n, stream_);
         if ((rc = runBatch(pathBegin, n, Ub.p, false, 0))) return rc;
         if (hipStreamSynchronize(stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
         uint32_t st[4];
-        unsigned long long tot[4];
+        unsigned long long tot[kTotals];
         hipMemcpy(st, status_.p, sizeof(st), hipMemcpyDeviceToHost);
         hipMemcpy(tot, totals_.p, sizeof(tot), hipMemcpyDeviceToHost);
         if (st[1]) return SSTMC_NO_HIT;
