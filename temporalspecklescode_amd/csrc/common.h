@@ -99,6 +99,14 @@ struct MediumView {
     float e0[3], invH[3];          // first edge and inverse spacing for the fast locate
     float h[3];                    // spacing of an equally spaced axis
     int allUniform;                // every axis equally spaced: the branch-free FP32 walk applies
+    // NEE walk grid (built per tracer): 4 x material id per cell (byte offset into a float sigt
+    // table), axes in the order that puts the view walks' dominant axis fastest; cell (i0, i1, i2)
+    // is walkGrid[walkPad + i0*walkStride[0] + i1*walkStride[1] + i2*walkStride[2]]. walkPad
+    // (a multiple of 4, >= one layer of the slowest axis) padding bytes precede and follow the
+    // cells, and the allocation is a whole number of 4-byte words.
+    const uint8_t* walkGrid;
+    uint32_t walkPad;
+    uint32_t walkStride[3];
 };
 
 // Scattering: Scattering.cuh:15-29, HG table :500-572.

@@ -5,6 +5,12 @@
 
 #include "common.h"
 
+// Hook on every walk-grid cell index whose word the fast walk reads; empty in the library. A
+// host-side test defines it to check that the word lies inside the padded grid.
+#ifndef SSTMC_WALK_INDEX_CHECK
+#define SSTMC_WALK_INDEX_CHECK(m, cell)
+#endif
+
 namespace sstmc {
 
 template <typename R> struct Real;
@@ -230,20 +236,30 @@ __host__ __device__ inline R opticalDepth(const MediumView& m, const R* sig, con
     return tau;
 }
 
-// Branch-free FP32 walk for equally spaced grids (the common case). Per step: pick the axis of
-// the nearest face with the reference's tie order, add segment * sigt, advance the cell index by
-// that axis' stride. The crossing parameter of the k-th face along an axis is
-// t_first + k * t_delta (one rounding, no accumulation), recomputed for every axis from its
-// crossing count. The walk ends at the first crossing at or beyond tExit, the parameter of the
-// box face the ray leaves through, computed by the same expression as the crossings, so the exit
-// test needs no per-axis counters and does not wait for memory.
+// Branch-free FP32 walk for equally spaced grids (the common case). Per step: the nearest face
+// crossing is min3 of the three pending crossings; the axis is the highest one whose crossing
+// equals it, which is the reference's strict-< tie order (x only when strictly nearest, then y,
+// else z). Add segment * sigt, advance the cell index by that axis' stride. The crossing
+// parameter of the k-th face along an axis is t_first + k * t_delta (one rounding, no
+// accumulation), recomputed for every axis from its crossing count. The walk ends at the first
+// crossing at or beyond tExit, the parameter of the box face the ray leaves through, computed by
+// the same expression as the crossings, so the exit test needs no per-axis counters and does not
+// wait for memory.
 //
-// Memory per step: the material byte of the next cell (global, L2-resident) and its sigt from
-// the table `sig` (LDS in the kernels). The three stages are software-pipelined one step apart:
-// step j reads the byte of cell j+1, looks up sigt of cell j (byte read in step j-1) and adds
-// segment(j-1) * sigt(j-1) (looked up in step j-1), so each read has a whole step of independent
-// work to hide behind. The sum is formed in the same order as a sequential walk and the tauCut
-// test sees the same partial sums, so the result is bit-identical to one.
+// Memory: the walk grid (m.walkGrid: one byte per cell, 4 x material id = the byte offset of the
+// material's sigt in the table `sig`, which is in LDS in the kernels; L2-resident; axes ordered
+// with the dominant walk direction fastest) is read one aligned 4-byte word at a time. Each walk
+// keeps the word holding its current cell and reads a new one only when its next cell lies in
+// another word, i.e. after a step along a slow axis or every fourth step along the fast one, so
+// the 64 walks of a wavefront issue few reads and touch few cache lines per step. The cell index
+// includes the grid's leading padding, so the cell one step past any face is still inside the
+// allocation and never below 0 (that word is read but not used).
+//
+// The stages are software-pipelined one step apart: step j reads the word of cell j+1 (when it
+// changes), takes cell j's byte from its word and looks up its sigt, and adds segment(j-1) *
+// sigt(j-1); each read has a whole step of independent work to hide behind. The sum is formed in
+// the same order as a sequential walk and the tauCut test sees the same partial sums, so the
+// result is bit-identical to one.
 template <int DIMS>
 __host__ __device__ inline float opticalDepthFast(const MediumView& m, const float* sig, const f3& p, const f3& d,
                                                   float tauCut, uint32_t* steps) {
@@ -251,8 +267,8 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
     const float pp[3] = {p.x, p.y, DIMS == 3 ? p.z : 0.f};
     const float dd[3] = {d.x, d.y, DIMS == 3 ? d.z : 0.f};
     float tf[3] = {3.0e38f, 3.0e38f, 3.0e38f}, td[3] = {0.f, 0.f, 0.f};
-    uint32_t stride[3] = {0u, 0u, 0u};   // signed strides in two's complement: cell + stride wraps
-    uint32_t cell = 0u, mul = 1u;
+    uint32_t stride[3] = {0u, 0u, 0u};   // walk-grid strides in two's complement
+    uint32_t cell = m.walkPad;
     float tExit = 3.0e38f;
 #pragma unroll
     for (int a = 0; a < DIMS; ++a) {
@@ -264,27 +280,32 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
         td[a] = m.h[a] * fabsf(inv);
         // faces crossed before the one that leaves the box along this axis
         tExit = fminf(tExit, fmaf((float)(up ? n - 1 - b : b), td[a], tf[a]));
-        stride[a] = up ? mul : 0u - mul;
-        cell += (uint32_t)b * mul;
-        mul *= (uint32_t)n;
+        stride[a] = up ? m.walkStride[a] : 0u - m.walkStride[a];
+        cell += (uint32_t)b * m.walkStride[a];
     }
+    const uint32_t* w32 = (const uint32_t*)m.walkGrid;
+    const char* sigb = (const char*)sig;
     float t0 = tf[0], t1 = tf[1], t2 = tf[2];
     float k0 = 0.f, k1 = 0.f, k2 = 0.f;
     float tcur = 0.f, tau = 0.f;
-    // One step. In: MAT = byte of the current cell (read last step), SEGI/SIGI = segment and sigt
-    // of the previous cell. Out: MATN (read issued), SEGO/SIGO for this cell, CODE = 0 to go on,
-    // 1 when tau passed the cut, 2 when this cell is the last one (its segment is added after the
-    // loop, so no step waits for its own reads).
-#define SSTMC_WALK_STEP(MAT, SEGI, SIGI, MATN, SEGO, SIGO, CODE)                                   \
+    // One step. In: CELL and WORD (the word holding CELL, read last step or kept), SEGI/SIGI =
+    // segment and sigt of the previous cell. Out: CELLN, WORDN (read issued when the word
+    // changes), SEGO/SIGO for this cell, DONE when tau passed the cut or this cell is the last one
+    // (tau is not changed after the cut test, so after the loop "tau <= tauCut" means the last
+    // cell's segment is still to be added; it is added there, so no step waits for its own reads).
+#define SSTMC_WALK_STEP(CELL, WORD, SEGI, SIGI, CELLN, WORDN, SEGO, SIGO, DONE)                    \
     {                                                                                               \
-        const bool m0 = DIMS == 3 ? (t0 < t1) && (t0 < t2) : t0 < t1;                               \
-        const bool m1 = !m0 && (DIMS == 3 ? t1 < t2 : true);                                        \
-        const bool m2 = !m0 && !m1;                                                                 \
-        const float tn = m0 ? t0 : (m1 ? t1 : t2);                                                  \
-        const bool out = tn >= tExit;                                                               \
-        cell += out ? 0u : (m0 ? stride[0] : (m1 ? stride[1] : stride[2]));                         \
-        MATN = m.grid[cell];                                                                        \
-        SIGO = sig[MAT];                                                                            \
+        SIGO = *(const float*)(sigb + ((WORD >> ((CELL & 3u) << 3)) & 0xffu));                      \
+        const float tn = DIMS == 3 ? fminf(fminf(t0, t1), t2) : fminf(t0, t1);                      \
+        const bool m2 = DIMS == 3 && tn == t2;                                                      \
+        const bool m1 = !m2 && tn == t1;                                                            \
+        const bool m0 = !m2 && !m1;                                                                 \
+        CELLN = CELL + (m2 ? stride[2] : (m1 ? stride[1] : stride[0]));                             \
+        WORDN = WORD;                                                                               \
+        if ((CELLN ^ CELL) > 3u) {                                                                  \
+            SSTMC_WALK_INDEX_CHECK(m, CELLN);                                                       \
+            WORDN = w32[CELLN >> 2];                                                                \
+        }                                                                                           \
         SEGO = tn - tcur;                                                                           \
         tcur = tn;                                                                                  \
         k0 += m0 ? 1.f : 0.f;                                                                       \
@@ -294,21 +315,22 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
         t1 = fmaf(k1, td[1], tf[1]);                                                                \
         if (DIMS == 3) t2 = fmaf(k2, td[2], tf[2]);                                                 \
         tau = fmaf(SEGI, SIGI, tau);  /* previous cell */                                           \
-        CODE = tau > tauCut ? 1 : (out ? 2 : 0);                                                    \
+        DONE = (tau > tauCut) | (tn >= tExit);                                                      \
     }
     // two steps per iteration with the pipeline registers swapping roles (no register copies)
-    uint32_t ma = m.grid[cell], mb;
+    SSTMC_WALK_INDEX_CHECK(m, cell);
+    uint32_t ca = cell, cb, wa = w32[cell >> 2], wb;
     float sga = 0.f, sia = 0.f, sgb, sib;       // nothing before the first cell: adds exactly 0
     float lastSeg = 0.f, lastSig = 0.f;
-    int code;
+    bool done;
     for (;;) {
-        SSTMC_WALK_STEP(ma, sga, sia, mb, sgb, sib, code)
-        if (code) { lastSeg = sgb; lastSig = sib; break; }
-        SSTMC_WALK_STEP(mb, sgb, sib, ma, sga, sia, code)
-        if (code) { lastSeg = sga; lastSig = sia; break; }
+        SSTMC_WALK_STEP(ca, wa, sga, sia, cb, wb, sgb, sib, done)
+        if (done) { lastSeg = sgb; lastSig = sib; break; }
+        SSTMC_WALK_STEP(cb, wb, sgb, sib, ca, wa, sga, sia, done)
+        if (done) { lastSeg = sga; lastSig = sia; break; }
     }
 #undef SSTMC_WALK_STEP
-    if (code == 2) tau = fmaf(lastSeg, lastSig, tau);   // last cell
+    if (!(tau > tauCut)) tau = fmaf(lastSeg, lastSig, tau);   // left the box: last cell
     if (steps) *steps += (uint32_t)(k0 + k1 + k2);
     return tau;
 }

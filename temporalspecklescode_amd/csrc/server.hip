@@ -133,7 +133,9 @@ static uint64_t estimate(const sstmc_config& c) {
             (c.sampler.Uz && c.sampler.Uz[m] != 0))
             frames = c.sampler.tNum > 1;
     }
-    return NextEventEstimation::footprint(P, c.illumination.num, c.view.num, c.sampler.tNum, fp64, frames);
+    const uint64_t cells = (uint64_t)(c.medium.xAxisSize - 1) * (c.medium.yAxisSize - 1) *
+                           (c.dims == 3 ? (uint64_t)(c.medium.zAxisSize - 1) : 1ull);
+    return NextEventEstimation::footprint(P, c.illumination.num, c.view.num, c.sampler.tNum, fp64, frames, cells);
 }
 
 void sstmc_server::admit() {

@@ -45,6 +45,8 @@ public:
     virtual int setMaterial(const double* sigs, const double* siga, uint32_t materialNum) = 0;
     uint32_t materialNum() const { return nmat_; }
     double sigs(uint32_t m) const { return sigs_[m]; }
+    // material id per cell (x fastest) on the host
+    virtual const std::vector<uint8_t>& hostGrid() const = 0;
 
 protected:
     MediumView v_{};
@@ -58,6 +60,7 @@ public:
     HeterogeneousMedium(int* err, int dims, const uint32_t* materialGrid, const double* xAxis, uint32_t xAxisSize,
                         const double* yAxis, uint32_t yAxisSize, const double* zAxis, uint32_t zAxisSize);
     int setMaterial(const double* sigs, const double* siga, uint32_t materialNum) override;
+    const std::vector<uint8_t>& hostGrid() const override { return hostGrid_; }
 
 private:
     DevBuf<float> ef_[3];
@@ -180,7 +183,7 @@ public:
     int finish(double* u, bool deviceDst = false);
     uint32_t batches() const { return opt_.fullIteration; }
     // device bytes a tracer with these sizes allocates (admission control)
-    static uint64_t footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames);
+    static uint64_t footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames, uint64_t cells);
     int debugBatch(uint64_t pathBegin, uint32_t npaths, uint64_t* nvtxOut, uint32_t* pathK, uint32_t* pathMisses,
                    uint64_t* pathOff, float* vtxPos, uint32_t* vtxMat, float* vtxRho, float* vtxXt, double* vtxLen,
                    double* termA, double* termB, double* termSa, double* termSb, double* uBatch);
@@ -197,10 +200,13 @@ private:
     int allocate();
     int growPool(uint64_t cap);
     int sizePool();
+    int buildWalkGrid();
     int runBatch(uint64_t pathBegin, uint32_t npaths, double* U, bool timed, uint32_t b);
     int contractSingle(uint32_t npaths, double* U);
 
     const Medium& medium_;
+    MediumView mview_{};            // the medium's view plus this tracer's walk grid
+    DevBuf<uint8_t> walkGrid_;
     const Scattering& sc_;
     const Source& il_;
     const Source& vw_;

@@ -380,6 +380,7 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
         fscalar_ *= fscalar_;
     }
     if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) { *err = SSTMC_DEVICE_ERROR; return; }
+    if ((*err = buildWalkGrid())) return;
     if ((*err = allocate())) return;
     if (needF) {
         std::vector<float> ff((size_t)L_ * V_);
@@ -416,7 +417,7 @@ int NextEventEstimation::allocate() {
     size_t na = (size_t)T_ * P_ * L_ * elem, nb = (size_t)T_ * P_ * V_ * elem;
     if ((e = A_.alloc(na)) || (e = a_.alloc(na)) || (e = B_.alloc(nb)) || (e = b_.alloc(nb))) return e;
     if ((e = U_.alloc(2 * (size_t)T_ * L_ * V_))) return e;
-    bytes_ = 2 * (na + nb) + 16 * (size_t)T_ * L_ * V_ + 20 * (size_t)P_;
+    bytes_ += 2 * (na + nb) + 16 * (size_t)T_ * L_ * V_ + 20 * (size_t)P_;
     return growPool((uint64_t)P_ * 32);
 }
 
@@ -428,7 +429,8 @@ int NextEventEstimation::growPool(uint64_t cap) {
     } else {
         delta_.release();
     }
-    bytes_ += cap * (20 + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_)) - cap_ * 20;
+    const uint64_t perVertex = 20 + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_);
+    bytes_ += (cap - cap_) * perVertex;
     cap_ = cap;
     return SSTMC_NO_ERROR;
 }
@@ -448,7 +450,7 @@ int NextEventEstimation::contractSingle(uint32_t npaths, double* U) {
 }
 
 int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U, bool timed, uint32_t b) {
-    const MediumView& m = medium_.view();
+    const MediumView& m = mview_;
     const ScatterView& sc = sc_.view();
     const SourceView& il = il_.view();
     const SourceView& vw = vw_.view();
@@ -482,13 +484,70 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     return SSTMC_NO_ERROR;
 }
 
-uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames) {
+uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames,
+                                        uint64_t cells) {
     uint64_t elem = fp64 ? 16 : 8;
     uint64_t terms = 2ull * T * P * (uint64_t)(L + V) * elem;
     uint64_t field = 16ull * T * L * V;
     uint64_t pool = (uint64_t)P * 32 * (20 + (frames ? 12ull * T : 0));
     uint64_t ws = 2ull * 8 * T * L * V * elem;   // split-K partials, two workspaces
-    return terms + field + pool + ws + 24ull * P + (1ull << 20);
+    uint64_t grids = 3 * cells;                  // material grid + padded walk grid
+    return terms + field + pool + ws + grids + 24ull * P + (1ull << 20);
+}
+
+// The NEE walk grid: 4 x material id per cell (the byte offset of the material's sigt in the
+// kernels' float table), with the axes ordered so that the axis the view walks mostly run along
+// is fastest. A walk then meets its next cell in the same 4-byte word for runs of cells along
+// that axis and re-reads memory only when it leaves the word; the 64 walks of a wavefront, which
+// fan out from one vertex, then issue few reads and touch few cache lines per step. Padding of
+// one slowest-axis layer (rounded up to a word) on both sides keeps the cell one step past any
+// face inside the allocation; the total is a whole number of words.
+int NextEventEstimation::buildWalkGrid() {
+    mview_ = medium_.view();
+    const uint32_t* n = mview_.n;
+    // dominant axis of the view walks: far field, mean |direction component|; points, mean
+    // |component| of the unit vector from the box centre to the sensor
+    double w[3] = {0, 0, 0}, c[3];
+    for (int a = 0; a < 3; ++a) c[a] = 0.5 * (medium_.boxMin(a) + medium_.boxMax(a));
+    const std::vector<double>& pv = vw_.hostPoints();
+    for (size_t i = 0; i + 2 < pv.size(); i += 3) {
+        double q[3] = {pv[i], pv[i + 1], dims_ == 3 ? pv[i + 2] : 0.0};
+        if (vw_.type() != SSTMC_SOURCE_FARFIELD)
+            for (int a = 0; a < 3; ++a) q[a] -= a < dims_ ? c[a] : q[a];
+        const double r = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+        if (r > 0)
+            for (int a = 0; a < dims_; ++a) w[a] += std::fabs(q[a]) / r;
+    }
+    int fast = 0;
+    for (int a = 1; a < dims_; ++a)
+        if (w[a] > w[fast]) fast = a;
+    int order[3] = {fast, -1, -1}, k = 1;
+    for (int a = 0; a < 3; ++a)
+        if (a != fast) order[k++] = a;
+    uint32_t S[3] = {0, 0, 0}, mul = 1;
+    for (int j = 0; j < 3; ++j) {
+        S[order[j]] = order[j] < dims_ ? mul : 0u;
+        if (order[j] < dims_) mul *= n[order[j]];
+    }
+    const size_t ncell = (size_t)n[0] * n[1] * n[2];
+    const int slow = dims_ == 3 ? order[2] : order[1];
+    const size_t pad = ((size_t)S[slow] + 3) & ~(size_t)3;
+    const size_t total = (pad + ncell + pad + 3) & ~(size_t)3;
+    if (total >= 0xffffffffull) return SSTMC_INVALID_CONFIG;
+    std::vector<uint8_t> wg(total, 0);
+    const std::vector<uint8_t>& g = medium_.hostGrid();
+    for (uint32_t i2 = 0; i2 < n[2]; ++i2)
+        for (uint32_t i1 = 0; i1 < n[1]; ++i1)
+            for (uint32_t i0 = 0; i0 < n[0]; ++i0)
+                wg[pad + (size_t)i0 * S[0] + (size_t)i1 * S[1] + (size_t)i2 * S[2]] =
+                    (uint8_t)(4 * g[i0 + (size_t)n[0] * (i1 + (size_t)n[1] * i2)]);
+    int e = walkGrid_.upload(wg.data(), wg.size());
+    if (e) return e;
+    mview_.walkGrid = walkGrid_.p;
+    mview_.walkPad = (uint32_t)pad;
+    for (int a = 0; a < 3; ++a) mview_.walkStride[a] = S[a];
+    bytes_ += total;
+    return SSTMC_NO_ERROR;
 }
 
 // Sizes the vertex pool from the exact vertex count of the first batch (its count pass and scan,
@@ -499,7 +558,7 @@ int NextEventEstimation::sizePool() {
     if (poolSized_) return SSTMC_NO_ERROR;
     const FramesView& fr = sampler_.view();
     VertexBatch vb{k_.p, misses_.p, off_.p, xm_.p, rho_.p, delta_.p, cap_, status_.p};
-    if (launchTransport(dims_, medium_.view(), sc_.view(), il_.view(), fr, vb, seed_, 0, P_, 0, stream_) != hipSuccess ||
+    if (launchTransport(dims_, mview_, sc_.view(), il_.view(), fr, vb, seed_, 0, P_, 0, stream_) != hipSuccess ||
         launchScan(k_.p, off_.p, blockSums_.p, P_, stream_) != hipSuccess)
         return SSTMC_KERNEL_ERROR;
     uint32_t need = 0;

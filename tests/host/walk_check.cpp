@@ -1,0 +1,129 @@
+// Host-side check of the FP32 fast voxel walk (opticalDepthFast) on random grids and rays:
+//   * every 4-byte walk-grid word it reads lies inside the padded grid (SSTMC_WALK_INDEX_CHECK),
+//     for walk grids with each axis as the fastest one;
+//   * tau agrees with the general FP32 walk (opticalDepth<3, float>, per-crossing parameters from
+//     the fixed origin) to 1e-5 relative, and with the FP64 general walk to 2e-3 absolute;
+//   * the tau cut: either both walks stop past the cut or both agree as above.
+// Rays start inside the box, on cell faces and corners, and leave through all six faces.
+// Built and run by tests/test_walk_host.py (hipcc, host code only; AddressSanitizer when usable).
+#include <cstdio>
+#include <cstdlib>
+#include <cstdint>
+#include <cmath>
+#include <random>
+#include <vector>
+
+static size_t g_walk_bytes = 0;   // bytes of the padded walk grid under test
+static long g_bad_index = 0;
+#define SSTMC_WALK_INDEX_CHECK(m, cell)                                                         \
+    do {                                                                                        \
+        if (((size_t)(cell) & ~(size_t)3) + 4 > g_walk_bytes) ++g_bad_index;                    \
+    } while (0)
+
+#include "../../temporalspecklescode_amd/csrc/common.h"
+#include "../../temporalspecklescode_amd/csrc/geometry.h"
+
+using namespace sstmc;
+
+// Walk grid with axis `fast` fastest, the other axes in increasing order, one slowest-axis layer
+// of padding (rounded up to a word) on both sides, whole words in total.
+static std::vector<uint8_t> walkGrid(const std::vector<uint8_t>& grid, const uint32_t n[3], int dims, int fast,
+                                     uint32_t S[3], uint32_t* pad) {
+    int order[3] = {fast, -1, -1}, k = 1;
+    for (int a = 0; a < 3; ++a)
+        if (a != fast) order[k++] = a;
+    uint32_t mul = 1;
+    for (int j = 0; j < 3; ++j) {
+        S[order[j]] = order[j] < dims ? mul : 0u;
+        if (order[j] < dims) mul *= n[order[j]];
+    }
+    const size_t ncell = (size_t)n[0] * n[1] * n[2];
+    const size_t p = ((size_t)S[dims == 3 ? order[2] : order[1]] + 3) & ~(size_t)3;
+    std::vector<uint8_t> wg((p + ncell + p + 3) & ~(size_t)3, 0);
+    for (uint32_t i2 = 0; i2 < n[2]; ++i2)
+        for (uint32_t i1 = 0; i1 < n[1]; ++i1)
+            for (uint32_t i0 = 0; i0 < n[0]; ++i0)
+                wg[p + (size_t)i0 * S[0] + (size_t)i1 * S[1] + (size_t)i2 * S[2]] =
+                    (uint8_t)(4 * grid[i0 + (size_t)n[0] * (i1 + (size_t)n[1] * i2)]);
+    *pad = (uint32_t)p;
+    return wg;
+}
+
+template <int DIMS>
+static int run(int n, double h, int trials, unsigned seed, int fast) {
+    std::mt19937 rng(seed);
+    std::vector<float> ef(n + 1);
+    std::vector<double> ed(n + 1);
+    for (int i = 0; i <= n; ++i) { ed[i] = -0.5 * n * h + i * h; ef[i] = (float)ed[i]; }
+    const int nz = DIMS == 3 ? n : 1;
+    const size_t nc = (size_t)n * n * nz;
+    std::vector<uint8_t> grid(nc);
+    for (auto& g : grid) g = (uint8_t)(1 + rng() % 4);
+    const uint32_t nn[3] = {(uint32_t)n, (uint32_t)n, (uint32_t)nz};
+    uint32_t S[3], pad;
+    std::vector<uint8_t> wg = walkGrid(grid, nn, DIMS, fast, S, &pad);
+    g_walk_bytes = wg.size();
+    const double sig[kMaxMaterials] = {0, 0.21, 1.01, 3.02, 5.05};
+    MediumView m{};
+    m.dims = DIMS;
+    for (int a = 0; a < 3; ++a) {
+        m.n[a] = a < DIMS ? n : 1;
+        m.ef[a] = ef.data(); m.ed[a] = ed.data();
+        m.bminf[a] = ef[0]; m.bmaxf[a] = ef[n]; m.bmind[a] = ed[0]; m.bmaxd[a] = ed[n];
+        m.uniformAxis[a] = 1; m.e0[a] = ef[0]; m.invH[a] = (float)(1.0 / h); m.h[a] = (float)h;
+    }
+    m.allUniform = 1; m.grid = grid.data(); m.walkGrid = wg.data(); m.walkPad = pad;
+    for (int a = 0; a < 3; ++a) m.walkStride[a] = S[a];
+    float sigf[kMaxMaterials];
+    for (int i = 0; i < kMaxMaterials; ++i) { m.sigtf[i] = (float)sig[i]; m.sigtd[i] = sig[i]; sigf[i] = (float)sig[i]; }
+    std::uniform_real_distribution<double> U(ed[0], ed[n]), N(-1, 1);
+    long bad = 0;
+    double worstRel = 0, worstAbs64 = 0;
+    for (int t = 0; t < trials; ++t) {
+        double p[3] = {U(rng), U(rng), DIMS == 3 ? U(rng) : 0.0};
+        if (t % 4 == 1) p[t % 3 % DIMS] = ed[rng() % (n + 1)];           // on a face
+        if (t % 9 == 2) { p[0] = ed[rng() % (n + 1)]; p[1] = ed[rng() % (n + 1)]; }   // on an edge
+        double d[3] = {N(rng), N(rng), DIMS == 3 ? N(rng) : 0.0};
+        if (t % 6 == 3) { d[0] = d[1] = d[2] = 0; d[t % DIMS] = (t & 8) ? 1 : -1; }   // axis aligned
+        if (t % 10 == 5) { d[0] = (t & 16) ? 1 : -1; d[1] = (t & 32) ? 1 : -1; d[2] = DIMS == 3 ? ((t & 64) ? 1 : -1) : 0; }
+        double nn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        for (double& x : d) x /= nn;
+        const f3 pf = mk((float)p[0], (float)p[1], (float)p[2]), df = mk((float)d[0], (float)d[1], (float)d[2]);
+        const d3 pd = mkd(pf.x, pf.y, pf.z), dd = mkd(df.x, df.y, df.z);
+        const float cut = (t % 5 == 0) ? 7.5f : 3.0e38f;
+        uint32_t s0 = 0, s1 = 0, s2 = 0;
+        const float fast = opticalDepthFast<DIMS>(m, sigf, pf, df, cut, &s0);
+        const float slow = opticalDepth<DIMS, float>(m, m.sigtf, pf, df, cut, &s1);
+        const double ref = opticalDepth<DIMS, double>(m, m.sigtd, pd, dd, 1e300, &s2);
+        if (cut < 1e30f && (fast > cut || slow > cut)) {
+            if (!(fast > cut && slow > cut)) ++bad;
+            continue;
+        }
+        const double rel = std::fabs((double)fast - slow) / std::max(1.0, (double)std::fabs(slow));
+        const double abs64 = std::fabs((double)fast - ref);
+        worstRel = std::max(worstRel, rel);
+        worstAbs64 = std::max(worstAbs64, abs64);
+        if (rel > 1e-5 || abs64 > 2e-3) {
+            if (bad < 5) printf("  ray %d: fast %.9g slow %.9g fp64 %.12g\n", t, fast, slow, ref);
+            ++bad;
+        }
+    }
+    printf("DIMS %d n %d h %g fastest axis %d: %d rays, %ld mismatches, %ld out-of-range words, max rel vs FP32 "
+           "general %.2e, max abs vs FP64 %.2e\n", DIMS, n, h, fast, trials, bad, g_bad_index, worstRel, worstAbs64);
+    return bad != 0 || g_bad_index != 0;
+}
+
+int main() {
+    int fail = 0;
+    for (int fast = 0; fast < 3; ++fast) {
+        fail |= run<3>(32, 1.0, 40000, 1 + fast, fast);
+        fail |= run<3>(7, 0.37, 40000, 4 + fast, fast);
+        fail |= run<3>(9, 0.1, 20000, 7 + fast, fast);
+    }
+    for (int fast = 0; fast < 2; ++fast) {
+        fail |= run<2>(64, 1.0, 40000, 10 + fast, fast);
+        fail |= run<2>(5, 0.25, 40000, 12 + fast, fast);
+    }
+    printf(fail ? "FAIL\n" : "OK\n");
+    return fail;
+}
