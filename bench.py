@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0
-VALU_PEAK_LANE_INSTR = 256 * 64 * 2.4e9   # 256 CUs x 64 lanes/clk (4 SIMD16) x 2.4 GHz, non-packed
+VALU_PEAK_LANE_INSTR = 157.3e12 / 2       # published FP32 vector peak in lane-FMAs (2 flops each) per s
 
 
 def workload(name: str, batch: int, steps: int):
@@ -169,8 +169,9 @@ def main():
     roof = {"kernel": "k_nee_view", "bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 6), "traffic": traffic, "avg_launch_ms": round(view_ms, 3),
             "algorithmic_bytes_per_launch": alg_bytes, "voxel_steps_per_s": steps_s}
-    # the walk is issue-bound, not HBM-bound: VALU lane-instructions per voxel step (from the
-    # SQ_INSTS_VALU counter over the same kernel) against the non-packed FP32 VALU issue peak
+    # the walk is VALU-issue-bound, not HBM-bound: VALU lane-instructions per voxel step (from the
+    # SQ_INSTS_VALU counter over the same kernel, inactive lanes included) against MI355X's
+    # published FP32 vector peak counted in lane-instructions (157.3 TFLOP/s / 2 flops per FMA)
     if "k_nee_view_valu_lane_instr_per_step" in prof:
         ips = prof["k_nee_view_valu_lane_instr_per_step"]
         ach = ips * steps_s

@@ -295,7 +295,6 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
     // cell's segment is still to be added; it is added there, so no step waits for its own reads).
 #define SSTMC_WALK_STEP(CELL, WORD, SEGI, SIGI, CELLN, WORDN, SEGO, SIGO, DONE)                    \
     {                                                                                               \
-        SIGO = *(const float*)(sigb + ((WORD >> ((CELL & 3u) << 3)) & 0xffu));                      \
         const float tn = DIMS == 3 ? fminf(fminf(t0, t1), t2) : fminf(t0, t1);                      \
         const bool m2 = DIMS == 3 && tn == t2;                                                      \
         const bool m1 = !m2 && tn == t1;                                                            \
@@ -314,6 +313,7 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
         t0 = fmaf(k0, td[0], tf[0]);                                                                \
         t1 = fmaf(k1, td[1], tf[1]);                                                                \
         if (DIMS == 3) t2 = fmaf(k2, td[2], tf[2]);                                                 \
+        SIGO = *(const float*)(sigb + ((WORD >> ((CELL & 3u) << 3)) & 0xffu));                      \
         tau = fmaf(SEGI, SIGI, tau);  /* previous cell */                                           \
         DONE = (tau > tauCut) | (tn >= tExit);                                                      \
     }
