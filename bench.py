@@ -12,7 +12,9 @@ The JSON line also carries:
                 tracer's stream); the contraction's MFMA rate is reported beside it;
   accuracy      relative Frobenius error of the mixed-precision field against the FP64 terms +
                 FP64 contraction on the same paths (one batch);
-  cpu_baseline  the FP64 CPU restatement (OpenMP) on a bounded sample of the same workload.
+  cpu_baseline  the FP64 CPU restatement (OpenMP) on a bounded sample of the same workload;
+  plain_gpu     the same tracer without split precision (FP32 terms + FP32 contraction, and FP64
+                terms + FP64 contraction) on the same workload, with mixed/plain ratios.
 """
 from __future__ import annotations
 
@@ -72,6 +74,43 @@ def cpu_baseline(name: str, seconds: float, threads: int):
             "paths_per_s": rate}
 
 
+def plain_gpu(cfg: dict, dims: int, batches: int, mixed_value: float):
+    """The same tracer without split precision (the plain GPU tracer): FP32 transport, FP64
+    path length and phase, and (i) FP32 terms with an FP32 FMA contraction, (ii) FP64 terms with
+    an FP64 contraction. Each is timed over `batches` batches (the kernels are already loaded by the
+    mixed run) and its field error is taken against FP64 terms + FP64 contraction on the same
+    4096 paths."""
+    from temporalspecklescode_amd import Tracer
+    from temporalspecklescode_amd.config import sizes
+    nl, nv, nt, _ = sizes(cfg, dims)
+    batch = cfg["simulation"]["batchSize"]
+    out = {}
+    ref = None
+    for name, prec in (("fp64", {"terms": "fp64", "contraction": "fp64"}), ("fp32", {"terms": "fp32", "contraction": "fp32"})):
+        c = dict(cfg, precision=prec)
+        c["simulation"] = dict(cfg["simulation"], fullIteration=batches)
+        with Tracer(c, dims) as tr:
+            t0 = time.perf_counter()
+            tr.trace()
+            wall = time.perf_counter() - t0
+        a = dict(c)
+        n = min(batch, 4096)
+        a["simulation"] = dict(cfg["simulation"], fullIteration=1, batchSize=n)
+        with Tracer(a, dims) as td:
+            u = td.debug_batch(0, n)["u"]
+        if ref is None:
+            ref = u
+        err = float(np.linalg.norm(u - ref) / max(1e-300, np.linalg.norm(ref)))
+        value = batch * batches * nl * nv * nt / wall
+        out[name] = {"value": value, "ms_per_step": wall * 1e3 / batches, "steps": batches,
+                     "rel_fro_err_vs_fp64": err, "precision": prec}
+        print("bench: plain %s %.3e path-contributions/s (%.1f ms/batch)" % (name, value, wall * 1e3 / batches),
+              file=sys.stderr, flush=True)
+    out["mixed_over_plain_fp32"] = mixed_value / out["fp32"]["value"]
+    out["mixed_over_plain_fp64"] = mixed_value / out["fp64"]["value"]
+    return out
+
+
 def serving(n_requests: int, max_paths_log2: int):
     """Replays the mixed request stream (Poisson arrivals, 2D/3D, three media, L and V in
     [16, 512], 1-8 lags of linear motion) through the continuous-batching server; returns
@@ -117,10 +156,12 @@ def main():
     ap.add_argument("--serving", type=int, default=0, help="requests of the mixed stream to serve after the "
                     "throughput measurement (0: skip)")
     ap.add_argument("--serving-max-paths-log2", type=int, default=18)
+    ap.add_argument("--plain-steps", type=int, default=2, help="batches timed for each plain (no split "
+                    "precision) variant of the same workload (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU sharding by path range is a design "
-              "only (DESIGN.md), so --gpus %d is not measured" % args.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU; sharding paths over GPUs by path-id range is not implemented, "
+              "so --gpus %d is not measured" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from temporalspecklescode_amd import Tracer
@@ -212,6 +253,8 @@ def main():
         "accuracy": accuracy,
         "stage_ms": {k: tm[k] for k in ("transport_ms", "nee_ms", "nee_view_ms", "contract_ms", "single_ms")},
     }
+    if args.plain_steps > 0:
+        out["plain_gpu"] = plain_gpu(cfg, dims, args.plain_steps, value)
     if args.serving > 0:
         out["serving"] = serving(args.serving, args.serving_max_paths_log2)
     if not args.no_cpu:

@@ -146,6 +146,15 @@ struct FramesView {
 };
 
 // One batch of vertices (SoA), written by the transport kernel.
+// Per (vertex k >= 2, frame t) quantities the view terms need, derived once from the vertex list
+// by k_vertex_prep (FP64): the unit vector from x_k^t back to x_{k-1}^t (positions displaced only
+// when the amplitude follows the scatterer motion), and the phase base rho_k + L_k^t / lambda in
+// cycles, L_k^t being the path length from x_1^t to x_k^t.
+struct __align__(32) VertexRec {
+    double bx, by, bz;
+    double phase;
+};
+
 struct VertexBatch {
     uint32_t* k;                   // [P] scattering vertices per path
     uint32_t* misses;              // [P]
@@ -155,6 +164,7 @@ struct VertexBatch {
     float* delta;                  // [cap * nt * 3] displacement per output frame (if non-trivial)
     uint64_t cap;
     uint32_t* status;              // [4]: 0 overflow flag, 1 no-hit flag
+    VertexRec* rec;                // [cap * nt], record of vertex o, frame t at o * nt + t
 };
 
 }  // namespace sstmc

@@ -236,6 +236,16 @@ __host__ __device__ inline R opticalDepth(const MediumView& m, const R* sig, con
     return tau;
 }
 
+// Byte (cell & 3) of a little-endian 4-byte word. On the device one bit-field extract whose
+// offset operand is taken mod 32, so (cell << 3) needs no mask.
+__host__ __device__ inline uint32_t byteOf(uint32_t word, uint32_t cell) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ubfe(word, cell << 3, 8);
+#else
+    return (word >> ((cell & 3u) << 3)) & 0xffu;
+#endif
+}
+
 // Branch-free FP32 walk for equally spaced grids (the common case). Per step: the nearest face
 // crossing is min3 of the three pending crossings; the axis is the highest one whose crossing
 // equals it, which is the reference's strict-< tie order (x only when strictly nearest, then y,
@@ -313,7 +323,7 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
         t0 = fmaf(k0, td[0], tf[0]);                                                                \
         t1 = fmaf(k1, td[1], tf[1]);                                                                \
         if (DIMS == 3) t2 = fmaf(k2, td[2], tf[2]);                                                 \
-        SIGO = *(const float*)(sigb + ((WORD >> ((CELL & 3u) << 3)) & 0xffu));                      \
+        SIGO = *(const float*)(sigb + byteOf(WORD, CELL));                                          \
         tau = fmaf(SEGI, SIGI, tau);  /* previous cell */                                           \
         DONE = (tau > tauCut) | (tn >= tExit);                                                      \
     }

@@ -32,7 +32,8 @@ struct NeeParams {
     uint32_t npaths;
     uint32_t lpp;          // k_nee: lanes per path (power of two, 8..64), set by launchNee
     uint32_t lppView;      // k_nee_view: lanes per work unit (power of two, 8..64), set by launchNee
-    uint32_t viewBlocks;   // k_nee_view: work units per path (blocks of lppView view items)
+    uint32_t viewBlocks;   // view kernels: blocks of lppView items (views, or view x frame items) per path
+    uint32_t lagChunks;    // correlation view kernel: frame chunks per path (1 otherwise)
 };
 
 hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc, const SourceView& il,

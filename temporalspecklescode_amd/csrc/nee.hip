@@ -338,13 +338,73 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
 
 
 
+// Vertex records for the view terms: one thread per path walks its vertices once per frame and
+// writes, for every vertex k >= 2, the unit vector back to the previous vertex (the direction the
+// view-side HG amplitude is evaluated against, FarField.cuh:546-599 / PointSource.cuh:347-446)
+// and the phase base rho_k + L_k^t / lambda with L_k^t the FP64 path length from x_1^t
+// (pathContriobutionKernel, TemporalSampler.cuh:668-719). Done once per (vertex, frame) here
+// instead of once per (vertex, view, frame) in k_nee_view.
+template <int DIMS>
+__global__ void __launch_bounds__(256) k_vertex_prep(VertexBatch vb, FramesView fr, uint32_t npaths, double invLambda) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npaths) return;
+    if ((uint64_t)vb.off[npaths] > vb.cap) return;   // pool overflow: the batch is re-run
+    const uint32_t K = vb.k[p], NT = fr.nt;
+    if (K < 2) return;                                // records exist for vertices k >= 2 only
+    const uint64_t off = vb.off[p];
+    for (uint32_t t = 0; t < NT; ++t) {
+        float4 xpm = vb.xm[off];
+        Geo xp = vertexFrame(vb, fr, off, t, xpm);
+        double len = 0;
+        for (uint32_t k = 1; k < K; ++k) {
+            const uint64_t ok = off + k;
+            const float4 xkm = vb.xm[ok];
+            const Geo xk = vertexFrame(vb, fr, ok, t, xkm);
+            const d3 sg = dsub(xk.pd, xp.pd);
+            len += sqrt(ddot<DIMS>(sg, sg));
+            const d3 pk = fr.temporalAmp ? xk.pd : mkd(xkm.x, xkm.y, xkm.z);
+            const d3 pp = fr.temporalAmp ? xp.pd : mkd(xpm.x, xpm.y, xpm.z);
+            const d3 b = dnormalize<DIMS>(dsub(pp, pk));
+            VertexRec r;
+            r.bx = b.x; r.by = b.y; r.bz = b.z;
+            r.phase = (double)vb.rho[ok] + len * invLambda;
+            vb.rec[ok * NT + t] = r;
+            xpm = xkm;
+            xp = xk;
+        }
+    }
+}
+
+// G_k of one view at one frame: attenuated source term g_v(x_k^t) times the HG amplitude against
+// the record's back direction, with the record's phase base added to the source phase.
+template <int DIMS, typename R, int VWFF>
+__device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, const ScatterView& sc,
+                                            const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
+                                            uint32_t v, const d3& vd, uint64_t ok, uint32_t t, const float4& xkm,
+                                            double invLambda, R tauCut, bool isoPoint, uint32_t* steps) {
+    const Geo xk = vertexFrame(vb, fr, ok, t, xkm);
+    const VertexRec rec = vb.rec[ok * fr.nt + t];
+    R ga;
+    double gp;
+    gTerm<DIMS, R>(m, sig, vw, v, xk, invLambda, tauCut, steps, &ga, &gp);
+    R fv;
+    if (isoPoint) {
+        fv = isoAmp<DIMS, R>();
+    } else {
+        const d3 pk = fr.temporalAmp ? xk.pd : mkd(xkm.x, xkm.y, xkm.z);
+        const d3 av = VWFF ? mkd(-vd.x, -vd.y, -vd.z) : dnormalize<DIMS>(dsub(pk, vd));
+        fv = phaseAmp<DIMS, R>(sc, __float_as_uint(xkm.w), ddot<DIMS>(av, mkd(rec.bx, rec.by, rec.bz)));
+    }
+    return cis<R>(ga * fv, gp + rec.phase);
+}
+
 // Multiple-scattering view terms B (the hot loop). Work unit = (path, block of lppView view
 // items): one (view, frame) item per lane, looping over the path's vertices k >= 2 (vertex
 // records are group-uniform broadcasts). Splitting a path over its item blocks keeps the longest
 // unit short: vertex counts are heavy-tailed (the longest path of a batch can hold ~15x the mean),
 // and a unit spanning all items of such a path would outlast a whole SIMD's share of the batch.
-// Templated on the view source type and the output mode so the loop carries only what it needs.
-template <int DIMS, typename R, int VWFF, int CORR>
+// Templated on the view source type so the loop carries only what it needs.
+template <int DIMS, typename R, int VWFF>
 __global__ void __launch_bounds__(256) k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr,
                                                   VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
@@ -362,58 +422,76 @@ __global__ void __launch_bounds__(256) k_nee_view(MediumView m, ScatterView sc, 
     C* B = (C*)out.B;
     const bool isoPoint = !VWFF && sc.allIsotropic;
     uint32_t steps = 0;
-    const uint32_t items = V * NT;
-    {
-        const uint32_t i = blk * lpp + sub;
-        const bool active = i < items && K >= 2;
-        const uint32_t v = active ? i % V : 0, t = active ? i / V : 0;
+    const uint32_t i = blk * lpp + sub;
+    const bool active = i < V * NT && K >= 2;
+    const uint32_t v = active ? i % V : 0, t = active ? i / V : 0;
+    // view direction (far field) or location (point) of this lane's item
+    const d3 vd = mkd(vw.pd[3 * v], vw.pd[3 * v + 1], DIMS == 3 ? vw.pd[3 * v + 2] : 0.0);
+    if (active) {
         R accRe = 0, accIm = 0;
-        double len0 = 0, lent = 0;
-        // view direction (far field) or location (point) of this lane's item
-        const d3 vd = mkd(vw.pd[3 * v], vw.pd[3 * v + 1], DIMS == 3 ? vw.pd[3 * v + 2] : 0.0);
-        if (active) {
-            for (uint32_t k = 1; k < K; ++k) {
-                const uint64_t ok = off + k, op = off + k - 1;
-                const float4 xkm = vb.xm[ok], xpm = vb.xm[op];
-                const uint32_t mk_ = __float_as_uint(xkm.w);
-                const double rhok = (double)vb.rho[ok];
-                R gr[2], gi[2];
-#pragma unroll
-                for (int f = 0; f < (CORR ? 2 : 1); ++f) {
-                    const uint32_t tt = (CORR && f == 0) ? 0 : t;
-                    const Geo xk = vertexFrame(vb, fr, ok, tt, xkm);
-                    const Geo xp = vertexFrame(vb, fr, op, tt, xpm);
-                    const d3 sg = dsub(xk.pd, xp.pd);
-                    const double ln = sqrt(ddot<DIMS>(sg, sg));
-                    double lacc;
-                    if (CORR && f == 0) { len0 += ln; lacc = len0; }
-                    else { lent += ln; lacc = lent; }
-                    R ga;
-                    double gp;
-                    gTerm<DIMS, R>(m, sig, vw, v, xk, invLambda, tauCut, &steps, &ga, &gp);
-                    R fv;
-                    if (isoPoint) {
-                        fv = isoAmp<DIMS, R>();
-                    } else {
-                        const d3 pk = fr.temporalAmp ? xk.pd : mkd(xkm.x, xkm.y, xkm.z);
-                        const d3 pp = fr.temporalAmp ? xp.pd : mkd(xpm.x, xpm.y, xpm.z);
-                        const d3 av = VWFF ? mkd(-vd.x, -vd.y, -vd.z) : dnormalize<DIMS>(dsub(pk, vd));
-                        fv = phaseAmp<DIMS, R>(sc, mk_, ddot<DIMS>(av, dnormalize<DIMS>(dsub(pp, pk))));
-                    }
-                    typename Cx<R>::T g = cis<R>(ga * fv, gp + rhok + lacc * invLambda);
-                    gr[f] = g.x;
-                    gi[f] = g.y;
-                }
-                if (!CORR) {
-                    accRe += gr[0];
-                    accIm += gi[0];
-                } else {  // G(0) * conj(G(t))
-                    accRe += gr[0] * gr[1] + gi[0] * gi[1];
-                    accIm += gi[0] * gr[1] - gr[0] * gi[1];
-                }
-            }
-            B[((size_t)t * P + p) * V + v] = Cx<R>::make(accRe, accIm);
+        for (uint32_t k = 1; k < K; ++k) {
+            const uint64_t ok = off + k;
+            const C g = viewTerm<DIMS, R, VWFF>(m, sig, sc, vw, fr, vb, v, vd, ok, t, vb.xm[ok], invLambda, tauCut,
+                                               isoPoint, &steps);
+            accRe += g.x;
+            accIm += g.y;
         }
+        B[((size_t)t * P + p) * V + v] = Cx<R>::make(accRe, accIm);
+    }
+    uint32_t tot = waveSum(steps);
+    if (lane == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
+}
+
+// Correlation view terms sum_k G_k(0) conj G_k(t) (the per-sub-path estimator). Work unit =
+// (path, block of lppView views, chunk of up to kCorrLags frames): one view per lane; per vertex,
+// G_k(0) is formed once and reused for every frame of the chunk, whose running sums live in LDS
+// (one slot per thread and frame).
+constexpr uint32_t kCorrLags = 16;
+template <int DIMS, typename R, int VWFF>
+__global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView sc, SourceView vw, FramesView fr,
+                                                       VertexBatch vb, NeeParams prm, NeeOut out) {
+    typedef typename Cx<R>::T C;
+    __shared__ R sigLds[kMaxMaterials];
+    __shared__ C acc[kCorrLags][256];
+    const R* sig = loadSigTable<R>(m, sigLds);
+    const uint32_t lane = threadIdx.x & 63, lpp = prm.lppView, sub = lane & (lpp - 1);
+    const uint64_t unit = (uint64_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / lpp) + lane / lpp;
+    const uint32_t perPath = prm.viewBlocks * prm.lagChunks;
+    const uint32_t p = (uint32_t)(unit / perPath), rem = (uint32_t)(unit % perPath);
+    const uint32_t blk = rem % prm.viewBlocks, chunk = rem / prm.viewBlocks;
+    const bool poolOk = (uint64_t)vb.off[prm.npaths] <= vb.cap;
+    const uint32_t K = (p < prm.npaths && poolOk) ? vb.k[p] : 0u;
+    const uint64_t off = p < prm.npaths ? vb.off[p] : 0;
+    const uint32_t P = prm.npaths, V = vw.n, NT = fr.nt;
+    const uint32_t t0 = chunk * kCorrLags, nt = NT - t0 < kCorrLags ? NT - t0 : kCorrLags;
+    const double invLambda = 1.0 / prm.lambda;
+    const R tauCut = VWFF ? (sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD) : (R)3e38;
+    C* B = (C*)out.B;
+    const bool isoPoint = !VWFF && sc.allIsotropic;
+    uint32_t steps = 0;
+    const uint32_t v = blk * lpp + sub;
+    const bool active = v < V && K >= 2 && t0 < NT;
+    const uint32_t vv = active ? v : 0;
+    const d3 vd = mkd(vw.pd[3 * vv], vw.pd[3 * vv + 1], DIMS == 3 ? vw.pd[3 * vv + 2] : 0.0);
+    if (active) {
+        for (uint32_t j = 0; j < nt; ++j) acc[j][threadIdx.x] = Cx<R>::make(0, 0);
+        for (uint32_t k = 1; k < K; ++k) {
+            const uint64_t ok = off + k;
+            const float4 xkm = vb.xm[ok];
+            const C g0 = viewTerm<DIMS, R, VWFF>(m, sig, sc, vw, fr, vb, vv, vd, ok, 0, xkm, invLambda, tauCut,
+                                                isoPoint, &steps);
+            for (uint32_t j = 0; j < nt; ++j) {
+                const uint32_t t = t0 + j;
+                const C gt = t == 0 ? g0
+                                    : viewTerm<DIMS, R, VWFF>(m, sig, sc, vw, fr, vb, vv, vd, ok, t, xkm, invLambda,
+                                                              tauCut, isoPoint, &steps);
+                C a = acc[j][threadIdx.x];
+                a.x += g0.x * gt.x + g0.y * gt.y;      // G(0) * conj(G(t))
+                a.y += g0.y * gt.x - g0.x * gt.y;
+                acc[j][threadIdx.x] = a;
+            }
+        }
+        for (uint32_t j = 0; j < nt; ++j) B[((size_t)(t0 + j) * P + p) * V + v] = acc[j][threadIdx.x];
     }
     uint32_t tot = waveSum(steps);
     if (lane == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
@@ -426,30 +504,35 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
     const uint32_t ppb = 4 * (64 / p.lpp);   // paths per 256-thread block
     dim3 grid((p.npaths + ppb - 1) / ppb), block(256);
     hipLaunchKernelGGL((k_nee<DIMS, R, 1>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, vb, fr, p.npaths,
+                       1.0 / p.lambda);
     if (between) hipEventRecord(between, st);
-    const uint64_t units = (uint64_t)p.npaths * p.viewBlocks, upb = 4 * (64 / p.lppView);
+    const uint64_t units = (uint64_t)p.npaths * p.viewBlocks * p.lagChunks, upb = 4 * (64 / p.lppView);
     grid = dim3((uint32_t)((units + upb - 1) / upb));
     const bool ff = vw.type == 2, corr = p.output == 1;
-    if (ff && !corr) hipLaunchKernelGGL((k_nee_view<DIMS, R, 1, 0>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
-    else if (ff && corr) hipLaunchKernelGGL((k_nee_view<DIMS, R, 1, 1>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
-    else if (!ff && !corr) hipLaunchKernelGGL((k_nee_view<DIMS, R, 0, 0>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
-    else hipLaunchKernelGGL((k_nee_view<DIMS, R, 0, 1>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
+    if (ff && !corr) hipLaunchKernelGGL((k_nee_view<DIMS, R, 1>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
+    else if (ff && corr) hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, 1>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
+    else if (!ff && !corr) hipLaunchKernelGGL((k_nee_view<DIMS, R, 0>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
+    else hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, 0>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
 }
 
 hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
                      const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
                      const NeeOut& out, hipStream_t st, hipEvent_t between) {
-    // lane groups: k_nee gives each path lpp lanes (its L and V items), k_nee_view gives each
-    // (path, block of lppView view items) unit lppView lanes; both shrink below a wavefront when
-    // there are few items so that several paths share one
+    // lane groups: k_nee gives each path lpp lanes (its L and V items), the view kernels give each
+    // work unit lppView lanes; both shrink below a wavefront when there are few items so that
+    // several paths share one. Field output: a unit is (path, block of view x frame items).
+    // Correlation: a unit is (path, block of views, chunk of up to kCorrLags frames).
     NeeParams q = p;
     uint32_t want = il.n > vw.n * fr.nt ? il.n : vw.n * fr.nt;
     q.lpp = 64;
     while (q.lpp > 8 && q.lpp / 2 >= want) q.lpp /= 2;
-    const uint32_t items = vw.n * fr.nt;
+    const bool corr = p.output == 1;
+    const uint32_t items = corr ? vw.n : vw.n * fr.nt;
     q.lppView = 64;
     while (q.lppView > 8 && q.lppView / 2 >= items) q.lppView /= 2;
     q.viewBlocks = (items + q.lppView - 1) / q.lppView;
+    q.lagChunks = corr ? (fr.nt + kCorrLags - 1) / kCorrLags : 1;
     if (dims == 2) {
         if (fp64) launchNeeT<2, double>(m, sc, il, vw, fr, vb, q, out, st, between);
         else launchNeeT<2, float>(m, sc, il, vw, fr, vb, q, out, st, between);

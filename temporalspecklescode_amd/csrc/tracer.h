@@ -200,6 +200,7 @@ private:
     int allocate();
     int growPool(uint64_t cap);
     int sizePool();
+    VertexBatch vertexBatch() const;
     int buildWalkGrid();
     int runBatch(uint64_t pathBegin, uint32_t npaths, double* U, bool timed, uint32_t b);
     int contractSingle(uint32_t npaths, double* U);
@@ -225,6 +226,7 @@ private:
     DevBuf<unsigned long long> totals_;   // [0] misses, [1] vertices, [2] NEE voxel steps
     DevBuf<float4> xm_;
     DevBuf<float> rho_, delta_;
+    DevBuf<VertexRec> rec_;
     uint64_t cap_ = 0;
     bool poolSized_ = false;   // sizePool ran: the pool holds a batch of this request with margin
     DevBuf<unsigned char> A_, B_, a_, b_;

@@ -41,6 +41,7 @@ template struct DevBuf<uint8_t>;
 template struct DevBuf<uint32_t>;
 template struct DevBuf<float4>;
 template struct DevBuf<unsigned long long>;
+template struct DevBuf<VertexRec>;
 
 // ====================================================================== HeterogeneousMedium
 HeterogeneousMedium::HeterogeneousMedium(int* err, int dims, const uint32_t* grid, const double* xAxis,
@@ -421,15 +422,19 @@ int NextEventEstimation::allocate() {
     return growPool((uint64_t)P_ * 32);
 }
 
+VertexBatch NextEventEstimation::vertexBatch() const {
+    return VertexBatch{k_.p, misses_.p, off_.p, xm_.p, rho_.p, delta_.p, cap_, status_.p, rec_.p};
+}
+
 int NextEventEstimation::growPool(uint64_t cap) {
     int e;
-    if ((e = xm_.alloc(cap)) || (e = rho_.alloc(cap))) return e;
+    if ((e = xm_.alloc(cap)) || (e = rho_.alloc(cap)) || (e = rec_.alloc(cap * T_))) return e;
     if (!sampler_.view().trivial) {
         if ((e = delta_.alloc(cap * T_ * 3))) return e;
     } else {
         delta_.release();
     }
-    const uint64_t perVertex = 20 + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_);
+    const uint64_t perVertex = 20 + 32 * (uint64_t)T_ + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_);
     bytes_ += (cap - cap_) * perVertex;
     cap_ = cap;
     return SSTMC_NO_ERROR;
@@ -437,7 +442,7 @@ int NextEventEstimation::growPool(uint64_t cap) {
 
 int NextEventEstimation::contractSingle(uint32_t npaths, double* U) {
     FramesView fr = sampler_.view();
-    VertexBatch vb{k_.p, misses_.p, off_.p, xm_.p, rho_.p, delta_.p, cap_, status_.p};
+    VertexBatch vb = vertexBatch();
     hipError_t he;
     if (singleMode_ == kSingleSeparable) {
         he = launchContract(contractKind_, a_.p, b_.p, npaths, L_, V_, T_, Ff_.p, Fd_.p,
@@ -455,7 +460,7 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     const SourceView& il = il_.view();
     const SourceView& vw = vw_.view();
     const FramesView& fr = sampler_.view();
-    VertexBatch vb{k_.p, misses_.p, off_.p, xm_.p, rho_.p, delta_.p, cap_, status_.p};
+    VertexBatch vb = vertexBatch();
     hipEvent_t* e = timed ? &ev_[(size_t)b * kEventsPerBatch] : nullptr;
     if (e) hipEventRecord(e[0], stream_);
     if (launchTransport(dims_, m, sc, il, fr, vb, seed_, pathBegin, npaths, 0, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
@@ -489,7 +494,7 @@ uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint
     uint64_t elem = fp64 ? 16 : 8;
     uint64_t terms = 2ull * T * P * (uint64_t)(L + V) * elem;
     uint64_t field = 16ull * T * L * V;
-    uint64_t pool = (uint64_t)P * 32 * (20 + (frames ? 12ull * T : 0));
+    uint64_t pool = (uint64_t)P * 32 * (20 + 32ull * T + (frames ? 12ull * T : 0));
     uint64_t ws = 2ull * 8 * T * L * V * elem;   // split-K partials, two workspaces
     uint64_t grids = 3 * cells;                  // material grid + padded walk grid
     return terms + field + pool + ws + grids + 24ull * P + (1ull << 20);
@@ -557,7 +562,7 @@ int NextEventEstimation::buildWalkGrid() {
 int NextEventEstimation::sizePool() {
     if (poolSized_) return SSTMC_NO_ERROR;
     const FramesView& fr = sampler_.view();
-    VertexBatch vb{k_.p, misses_.p, off_.p, xm_.p, rho_.p, delta_.p, cap_, status_.p};
+    VertexBatch vb = vertexBatch();
     if (launchTransport(dims_, mview_, sc_.view(), il_.view(), fr, vb, seed_, 0, P_, 0, stream_) != hipSuccess ||
         launchScan(k_.p, off_.p, blockSums_.p, P_, stream_) != hipSuccess)
         return SSTMC_KERNEL_ERROR;

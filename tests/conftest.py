@@ -14,6 +14,6 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def native():
-    """The HIP library; on a GPU box it must load, there is no CPU fallback."""
+    """The HIP library; where a GPU is present it must load, there is no CPU fallback."""
     from temporalspecklescode_amd import _native
     return _native.lib()
