@@ -405,7 +405,7 @@ __device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, 
 // and a unit spanning all items of such a path would outlast a whole SIMD's share of the batch.
 // Templated on the view source type so the loop carries only what it needs.
 template <int DIMS, typename R, int VWFF>
-__global__ void __launch_bounds__(256) k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr,
                                                   VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
     __shared__ R sigLds[kMaxMaterials];
