@@ -174,6 +174,11 @@ void sstmc_destroy(sstmc_handle* h);
 /* getErrorMsg (MatlabInterface.cu:24-272). Static string, never NULL. */
 const char* sstmc_error_string(int code);
 
+/* The config checks of sstmc_create alone (the MEX argument validation, MatlabInterface.cu:
+ * 712-1563): returns the code sstmc_create would return for a config error, SSTMC_NO_ERROR for a
+ * config it would accept. Touches no device; host entries use it to report errors early. */
+int sstmc_validate(const sstmc_config* cfg);
+
 /* The whole MEX call: `[u, iter] = sstmc{2,3}D(config)` (MatlabInterface.cu:1565-1825).
  * u: 2*Nl*Nv*Nt*renderingsNum doubles, iter: renderingsNum doubles. */
 int sstmc_render(const sstmc_config* cfg, double* u, double* iter);

@@ -2,7 +2,8 @@
 
 Every .hip file under csrc/ is compiled to an object in parallel, then linked into
 ``temporalspecklescode_amd/libsstmc.so``. The library has no torch dependency: its only
-interface is the C ABI of include/sstmc.h.
+interface is the C ABI of include/sstmc.h. The command-line host ``sstmc_cli`` (cli/) is a plain
+C++ program over that ABI, built next to the library and found through its run path.
 """
 from __future__ import annotations
 
@@ -18,6 +19,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libsstmc.so")
+CLI = os.path.join(PKG, "sstmc_cli")
 ARCH = os.environ.get("SSTMC_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -50,9 +52,22 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    build_cli()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", CLI)
     return LIB
+
+
+def build_cli() -> str:
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(PKG, "cli", "sstmc_cli.cpp"), "-L" + PKG, "-lsstmc", "-Wl,-rpath,$ORIGIN",
+           "-o", CLI + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("sstmc_cli build failed:\n%s\n%s" % (r.stdout, r.stderr))
+    os.replace(CLI + ".tmp", CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -181,6 +181,8 @@ int configValidate(const sstmc_config* c) { return validate(c); }
 
 extern "C" {
 
+int sstmc_validate(const sstmc_config* cfg) { return validate(cfg); }
+
 int sstmc_create(const sstmc_config* cfg, sstmc_handle** out) {
     try {
         return create(cfg, out);
