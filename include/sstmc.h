@@ -228,6 +228,14 @@ int sstmc_debug_batch(sstmc_handle* h, uint64_t path_begin, uint32_t npaths,
                       double* term_a, double* term_b, double* term_sa, double* term_sb,
                       double* u_batch);
 
+/* Coherent-backscattering terms of the last sstmc_debug_batch call (same npaths), for a handle
+ * created with tracer.isCBS: the reversed path v -> x1 -> ... -> xK -> l
+ * (NextEventEstimation.cuh:1212-1235, 1432-1443).
+ *  term_ar[P*Nt*Nv*2]  view connected at x1: g_v(x1) f_v(x1 -> x2) w2 (zero when K < 2)
+ *  term_br[P*Nt*Nl*2]  sum_{k>=2} g_l(x_k) f_l(x_k -> x_{k-1}) exp(i 2 pi (rho_k + L_k / lambda))
+ * The multiple-scattering field is (sum_p A^T B + sum_p Br^T Ar) / sqrt(2). */
+int sstmc_debug_cbs(sstmc_handle* h, uint32_t npaths, double* term_ar, double* term_br);
+
 /* Contracts caller-supplied terms on the device with the handle's contraction precision:
  * u[t][v][l] += sum_p a[p][t][l] * b[p][t][v] (complex, interleaved doubles in and out).
  * Exposes kernel (c) alone for accuracy tests against an FP64 GEMM. */

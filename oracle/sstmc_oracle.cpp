@@ -212,7 +212,7 @@ struct Medium {
         double tau = 0.0;
         for (;;) {
             if (steps) ++*steps;
-            double dist[3];
+            double dist[3] = {0.0, 0.0, 0.0};
             for (int a = 0; a < dims; ++a) {
                 double lo = ax[a][b[a]], hi = ax[a][b[a] + 1];
                 double r = 0.5 * (hi - lo), o = p[a] - 0.5 * (hi + lo);
@@ -340,6 +340,7 @@ struct Problem {
     uint32_t P, fullIter;
     uint64_t seed;
     int output;                // SSTMC_OUTPUT_FIELD / _CORRELATION
+    bool cbs = false;          // coherent backscattering: add the reversed path of every path
     uint32_t maxAttempts = 1u << 20;
 
     uint32_t L() const { return il.n(); }
@@ -564,10 +565,30 @@ struct Problem {
         return sc.amp(m, geo.dot(a, geo.normalize(sub(xprev, xk))));
     }
 
+    // Incoming unit direction at x from source i of s, as the three-point functions use it
+    // (FarField: illumination d, view -d, FarField.cuh:546-556; PointSource: n(x - s),
+    // PointSource.cuh:347-361).
+    V3 incoming(const Source& s, uint32_t i, const V3& x) const {
+        if (s.type == SSTMC_SOURCE_FARFIELD) return s.illum ? s.pt[i] : scale(-1.0, s.pt[i]);
+        return geo.normalize(sub(x, s.pt[i]));
+    }
+    // Three-point function of source i of s at vertex p1 with the path continuing to p2
+    // (threePointsThroughput, NextEventEstimation.cuh:773-800).
+    double f3(const Source& s, uint32_t i, uint32_t m, const V3& p1, const V3& p2) const {
+        if (s.type == SSTMC_SOURCE_POINT && sc.allIsotropic) return sc.isoAmp();
+        return sc.amp(m, geo.dot(incoming(s, i, p1), geo.normalize(sub(p2, p1))));
+    }
+
     struct Terms {
         std::vector<cplx> A;   // [t][l]
         std::vector<cplx> B;   // [t][v]   (field) or sum_k G_k[v,0] conj G_k[v,t] (correlation)
         std::vector<cplx> S;   // [t][v][l]
+        // coherent backscattering, reversed path v -> x1 -> ... -> xK -> l
+        // (NextEventEstimation.cuh:1212-1235, 1432-1443): Ar = view connected at x1 (Gv0 with f
+        // toward x2 and the multiple-scattering weight), Br = illumination connected at every
+        // later vertex (Gl with f toward the previous vertex and the path phase).
+        std::vector<cplx> Ar;  // [t][v]
+        std::vector<cplx> Br;  // [t][l]
         bool multiple = false;
     };
 
@@ -611,6 +632,28 @@ struct Problem {
             for (uint32_t l = 0; l < nl; ++l)
                 out.A[t * nl + l] = gl[t * nl + l] * fil(l, v1.m, a1, a2) * w2;
         }
+        if (cbs) {
+            out.Ar.assign((size_t)nt * nv, cplx(0));
+            out.Br.assign((size_t)nt * nl, cplx(0));
+            for (uint32_t t = 0; t < nt; ++t) {
+                const V3& a1 = fr.temporalAmp ? v1.xt[t] : v1.x;
+                const V3& a2 = fr.temporalAmp ? v2.xt[t] : v2.x;
+                for (uint32_t v = 0; v < nv; ++v)
+                    out.Ar[t * nv + v] = gv1[t * nv + v] * f3(vw, v, v1.m, a1, a2) * w2;
+            }
+            for (size_t k = 1; k < vs.size(); ++k) {
+                const Vertex& xk = vs[k];
+                const Vertex& xp = vs[k - 1];
+                for (uint32_t t = 0; t < nt; ++t) {
+                    const V3& pk = fr.temporalAmp ? xk.xt[t] : xk.x;
+                    const V3& pp = fr.temporalAmp ? xp.xt[t] : xp.x;
+                    double ph = xk.rho + xk.len[t] / lambda;
+                    cplx e(std::cos(2.0 * PI * ph), std::sin(2.0 * PI * ph));
+                    for (uint32_t l = 0; l < nl; ++l)
+                        out.Br[t * nl + l] += g(il, l, xk.xt[t], steps) * f3(il, l, xk.m, pk, pp) * e;
+                }
+            }
+        }
         std::vector<cplx> G((size_t)nt * nv);
         for (size_t k = 1; k < vs.size(); ++k) {
             const Vertex& xk = vs[k];
@@ -645,7 +688,12 @@ struct Problem {
                     cplx s = tm.S[i];
                     if (output == SSTMC_OUTPUT_CORRELATION) s = tm.S[(size_t)v * nl + l] * std::conj(s);
                     U[i] += s;
-                    if (tm.multiple) U[i] += tm.A[t * nl + l] * tm.B[t * nv + v];
+                    if (tm.multiple && !cbs) U[i] += tm.A[t * nl + l] * tm.B[t * nv + v];
+                    // field CBS: v = 1/sqrt(2) * (direct + reversed) for multiple scattering
+                    // (NextEventEstimation.cuh:1385-1391, 1448-1454)
+                    if (tm.multiple && cbs)
+                        U[i] += (tm.A[t * nl + l] * tm.B[t * nv + v] + tm.Br[t * nl + l] * tm.Ar[t * nv + v]) *
+                                0.70710678118654752440;
                 }
     }
 };
@@ -662,7 +710,8 @@ int build(const sstmc_config* c, Problem& pb) {
     pb.fullIter = c->simulation.fullIteration ? c->simulation.fullIteration : 1;
     pb.seed = c->simulation.seed;
     pb.output = c->tracer.output;
-    if (c->tracer.isCBS) return SSTMC_NOT_SUPPORTED;
+    pb.cbs = c->tracer.isCBS != 0;
+    if (pb.cbs && pb.output == SSTMC_OUTPUT_CORRELATION) return SSTMC_NOT_SUPPORTED;
     // scattering: one cell -> materials 0 and 1 (MatlabInterface.cu:750-760, 878-881)
     uint32_t nm = c->scattering.materialNum;
     if (nm == 0 || nm > 10 || !c->scattering.type) return SSTMC_INVALID_CONFIG;
@@ -944,6 +993,52 @@ int oracle_terms_from_vertices(const sstmc_config* cfg, uint32_t n, const uint32
                 term_s[2 * ((size_t)i * nt * nv * nl + j)] = tm.S[j].real();
                 term_s[2 * ((size_t)i * nt * nv * nl + j) + 1] = tm.S[j].imag();
             }
+    }
+    return SSTMC_NO_ERROR;
+}
+
+// Coherent-backscattering terms (Ar [n][t][v], Br [n][t][l], complex interleaved) for
+// caller-supplied vertices, with the same inputs as oracle_terms_from_vertices. Zero rows for
+// paths with fewer than two vertices.
+int oracle_cbs_terms_from_vertices(const sstmc_config* cfg, uint32_t n, const uint32_t* path_k,
+                                   const uint64_t* path_off, const double* vtx_pos, const uint32_t* vtx_mat,
+                                   const double* vtx_rho, const double* vtx_xt, const double* vtx_len,
+                                   double* term_ar, double* term_br) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    if (!pb.cbs) return SSTMC_INVALID_CONFIG;
+    uint32_t nl = pb.L(), nv = pb.V(), nt = pb.T();
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        std::vector<Vertex> vs(path_k[i]);
+        for (uint32_t k = 0; k < path_k[i]; ++k) {
+            uint64_t o = path_off[i] + k;
+            Vertex& v = vs[k];
+            v.x = V3(vtx_pos[3 * o], vtx_pos[3 * o + 1], vtx_pos[3 * o + 2]);
+            v.m = vtx_mat[o];
+            v.rho = vtx_rho[o];
+            v.xt.resize(nt);
+            v.len.resize(nt);
+            for (uint32_t t = 0; t < nt; ++t) {
+                v.xt[t] = V3(vtx_xt[(o * nt + t) * 3], vtx_xt[(o * nt + t) * 3 + 1], vtx_xt[(o * nt + t) * 3 + 2]);
+                v.len[t] = vtx_len[o * nt + t];
+            }
+        }
+        Problem::Terms tm;
+        pb.terms(vs, tm, nullptr);
+        for (size_t j = 0; j < (size_t)nt * nv; ++j) {
+            cplx x = tm.multiple ? tm.Ar[j] : cplx(0);
+            term_ar[2 * ((size_t)i * nt * nv + j)] = x.real();
+            term_ar[2 * ((size_t)i * nt * nv + j) + 1] = x.imag();
+        }
+        for (size_t j = 0; j < (size_t)nt * nl; ++j) {
+            cplx x = tm.multiple ? tm.Br[j] : cplx(0);
+            term_br[2 * ((size_t)i * nt * nl + j)] = x.real();
+            term_br[2 * ((size_t)i * nt * nl + j) + 1] = x.imag();
+        }
     }
     return SSTMC_NO_ERROR;
 }

@@ -121,6 +121,14 @@ class Tracer:
         out["xt"] = out["pos"].astype(np.float64)[:, None, :] + out["dxt"].astype(np.float64)
         return out
 
+    def debug_cbs(self, n: int):
+        """Coherent-backscattering terms of the last debug_batch(begin, n): Ar [n, Nt, Nv] (view
+        connected at x1) and Br [n, Nt, Nl] (illumination connected at the later vertices)."""
+        ar = np.zeros((n, self.T, self.V, 2))
+        br = np.zeros((n, self.T, self.L, 2))
+        _native.check(_native.lib().sstmc_debug_cbs(self._h, C.c_uint32(n), _p(ar), _p(br)), "sstmc_debug_cbs")
+        return ar[..., 0] + 1j * ar[..., 1], br[..., 0] + 1j * br[..., 1]
+
     def debug_contract(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
         """u[l, v, t] = sum_p a[p, t, l] b[p, t, v] with this tracer's contraction kernel."""
         n = a.shape[0]

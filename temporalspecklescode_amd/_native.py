@@ -16,7 +16,7 @@ EXPORTS = [
     "sstmc_photons_budget", "sstmc_sizes", "sstmc_destroy", "sstmc_error_string", "sstmc_render",
     "sstmc_last_timing", "sstmc_debug_batch", "sstmc_debug_contract", "sstmc_server_create",
     "sstmc_submit", "sstmc_poll", "sstmc_wait", "sstmc_fetch", "sstmc_server_stats_get",
-    "sstmc_server_destroy", "sstmc_validate",
+    "sstmc_server_destroy", "sstmc_validate", "sstmc_debug_cbs",
 ]
 
 

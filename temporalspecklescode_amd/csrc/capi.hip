@@ -43,7 +43,9 @@ static int validate(const sstmc_config* c) {
     int dims = (int)c->dims;
     if (dims != 2 && dims != 3) return SSTMC_INVALID_CONFIG;
     if (c->tracer.type != SSTMC_TRACER_NEE && c->tracer.type != 0) return SSTMC_INVALID_CONFIG;
-    if (c->tracer.isCBS) return SSTMC_NOT_SUPPORTED;
+    // coherent backscattering is implemented for the field output (NextEventEstimation.cuh:
+    // 1212-1235, 1385-1391, 1432-1454); its correlation form (CBS02-CBS04, :1237-1383) is not
+    if (c->tracer.isCBS && c->tracer.output == SSTMC_OUTPUT_CORRELATION) return SSTMC_NOT_SUPPORTED;
     if (c->tracer.output != SSTMC_OUTPUT_FIELD && c->tracer.output != SSTMC_OUTPUT_CORRELATION)
         return SSTMC_INVALID_CONFIG;
     const sstmc_scattering& sc = c->scattering;
@@ -165,6 +167,7 @@ static int create(const sstmc_config* c, sstmc_handle** out) {
     opt.lambda = c->simulation.lambda > 0 ? c->simulation.lambda : 1.0;
     if (!(std::isfinite(opt.lambda))) return SSTMC_INVALID_CONFIG;
     opt.output = c->tracer.output;
+    opt.cbs = c->tracer.isCBS ? 1 : 0;
     opt.terms = c->precision.terms;
     opt.contraction = c->precision.contraction;
     h->tracer.reset(new NextEventEstimation(&err, *h->medium, *h->scattering, *h->illumination, *h->view,
@@ -272,6 +275,12 @@ int sstmc_debug_batch(sstmc_handle* h, uint64_t path_begin, uint32_t npaths, uin
     hipSetDevice(h->device);
     return h->tracer->debugBatch(path_begin, npaths, nvtx_out, path_k, path_misses, path_off, vtx_pos, vtx_mat, vtx_rho,
                                  vtx_dxt, vtx_len, term_a, term_b, term_sa, term_sb, u_batch);
+}
+
+int sstmc_debug_cbs(sstmc_handle* h, uint32_t npaths, double* term_ar, double* term_br) {
+    if (!h) return SSTMC_INVALID_CONFIG;
+    hipSetDevice(h->device);
+    return h->tracer->debugCbsTerms(npaths, term_ar, term_br);
 }
 
 int sstmc_debug_contract(sstmc_handle* h, uint32_t npaths, const double* a, const double* b, double* u) {

@@ -13,6 +13,9 @@ struct NeeOut {
     void* B;          // multiple-scattering view terms (or sum_k G_k(0) conj G_k(t))
     void* a;          // single-scattering illumination terms g_l * w1 * exp(i 2 pi rho_1)
     void* b;          // single-scattering view terms g_v
+    // coherent backscattering (null when off): the reversed path v -> x1 -> ... -> xK -> l
+    void* Ar;         // [t][p][v] view connected at x1: g_v(x1) f_v(x1 -> x2) w2
+    void* Br;         // [t][p][l] illumination connected at x_k, k >= 2, summed with the path phase
     unsigned long long* steps;   // voxel steps of the NEE walks (accumulated): k_nee adds to
                                  // steps[0], k_nee_view to steps[2]
 };

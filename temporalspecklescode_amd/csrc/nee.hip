@@ -162,6 +162,18 @@ __device__ inline void gTerm(const MediumView& m, const R* sig, const SourceView
     }
 }
 
+// Incoming unit direction at x from source i as the three-point functions use it: FarField
+// illumination d, FarField view -d (FarField.cuh:546-556), PointSource n(x - s_i)
+// (PointSource.cuh:347-361).
+template <int DIMS>
+__device__ inline d3 incoming(const SourceView& s, uint32_t i, const d3& x) {
+    if (s.type == 2) {
+        const d3 d = mkd(s.pd[3 * i], s.pd[3 * i + 1], s.pd[3 * i + 2]);
+        return s.illum ? d : mkd(-d.x, -d.y, -d.z);
+    }
+    return dnormalize<DIMS>(dsub(x, mkd(s.pd[3 * i], s.pd[3 * i + 1], DIMS == 3 ? s.pd[3 * i + 2] : 0.0)));
+}
+
 // Incoming unit direction at x1 from illumination l (FarField d_l; PointSource n(x1 - s_l)).
 template <int DIMS>
 __device__ inline d3 illumIn(const SourceView& il, uint32_t l, const d3& x1) {
@@ -206,6 +218,8 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
     C* B = (C*)out.B;
     C* a = (C*)out.a;
     C* b = (C*)out.b;
+    C* Ar = (C*)out.Ar;
+    C* Br = (C*)out.Br;
     // a pool overflow leaves the vertex offsets unusable; the batch is then re-run by the host
     const bool poolOk = (uint64_t)vb.off[prm.npaths] <= vb.cap;
     const uint32_t K = poolOk ? vb.k[p] : 0u;
@@ -221,7 +235,10 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
             uint32_t v = i % V, t = i / V;
             B[((size_t)t * P + p) * V + v] = zero;
             b[((size_t)t * P + p) * V + v] = zero;
+            if (Ar) Ar[((size_t)t * P + p) * V + v] = zero;
         }
+        if (Br)
+            for (uint32_t i = sub; i < L * NT; i += lpp) Br[((size_t)(i / L) * P + p) * L + i % L] = zero;
         return;
     }
     uint32_t steps = 0;
@@ -297,11 +314,15 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         }
     }
 
-    if (!multi)
+    if (!multi) {
         for (uint32_t i = sub; i < V * NT; i += lpp) {
             const uint32_t v = i % V, t = i / V;
             B[((size_t)t * P + p) * V + v] = zero;
         }
+        if (Br)
+            for (uint32_t i = sub; i < L * NT; i += lpp) Br[((size_t)(i / L) * P + p) * L + i % L] = zero;
+    }
+    const bool isoPointVw = vw.type == 1 && sc.allIsotropic;
 
     // ---- single-scattering view terms b = g_v(x1^t)
     for (uint32_t i = sub; i < V * NT; i += lpp) {
@@ -316,6 +337,18 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         }
         const size_t o = ((size_t)t * P + p) * V + v;
         b[o] = corr ? cis<R>(ga[0] * ga[1], gp[0] - gp[1]) : cis<R>(ga[0], gp[0]);
+        if (Ar) {   // reversed path: the view connected at x1 with f toward x2 (field output only)
+            R fv = 0;
+            if (multi) {
+                const Geo x1t = vertexFrame(vb, fr, off, t, x1m);
+                const Geo x2t = vertexFrame(vb, fr, off + 1, t, x2m);
+                const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
+                const d3 pb = fr.temporalAmp ? x2t.pd : x2cd;
+                fv = isoPointVw ? isoAmp<DIMS, R>()
+                                : phaseAmp<DIMS, R>(sc, m1, ddot<DIMS>(incoming<DIMS>(vw, v, pa), dnormalize<DIMS>(dsub(pb, pa))));
+            }
+            Ar[o] = multi ? cis<R>(ga[0] * fv * w2, gp[0]) : zero;
+        }
     }
     *stepsOut += steps;
 }
@@ -392,7 +425,8 @@ __device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, 
         fv = isoAmp<DIMS, R>();
     } else {
         const d3 pk = fr.temporalAmp ? xk.pd : mkd(xkm.x, xkm.y, xkm.z);
-        const d3 av = VWFF ? mkd(-vd.x, -vd.y, -vd.z) : dnormalize<DIMS>(dsub(pk, vd));
+        // incoming direction at x_k (FarField: illumination d, view -d; points: from the source)
+        const d3 av = VWFF ? (vw.illum ? vd : mkd(-vd.x, -vd.y, -vd.z)) : dnormalize<DIMS>(dsub(pk, vd));
         fv = phaseAmp<DIMS, R>(sc, __float_as_uint(xkm.w), ddot<DIMS>(av, mkd(rec.bx, rec.by, rec.bz)));
     }
     return cis<R>(ga * fv, gp + rec.phase);
@@ -514,6 +548,21 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
     else if (ff && corr) hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, 1>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
     else if (!ff && !corr) hipLaunchKernelGGL((k_nee_view<DIMS, R, 0>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
     else hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, 0>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
+    if (out.Br) {
+        // coherent backscattering: the same per-vertex sums with the illumination connected at
+        // x_k (the reversed path), into Br
+        NeeParams q = p;
+        q.lppView = 64;
+        while (q.lppView > 8 && q.lppView / 2 >= il.n * fr.nt) q.lppView /= 2;
+        q.viewBlocks = (il.n * fr.nt + q.lppView - 1) / q.lppView;
+        q.lagChunks = 1;
+        NeeOut o = out;
+        o.B = out.Br;
+        const uint64_t u2 = (uint64_t)q.npaths * q.viewBlocks, upb2 = 4 * (64 / q.lppView);
+        const dim3 g2((uint32_t)((u2 + upb2 - 1) / upb2));
+        if (il.type == 2) hipLaunchKernelGGL((k_nee_view<DIMS, R, 1>), g2, block, 0, st, m, sc, il, fr, vb, q, o);
+        else hipLaunchKernelGGL((k_nee_view<DIMS, R, 0>), g2, block, 0, st, m, sc, il, fr, vb, q, o);
+    }
 }
 
 hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,

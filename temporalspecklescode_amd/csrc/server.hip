@@ -135,7 +135,8 @@ static uint64_t estimate(const sstmc_config& c) {
     }
     const uint64_t cells = (uint64_t)(c.medium.xAxisSize - 1) * (c.medium.yAxisSize - 1) *
                            (c.dims == 3 ? (uint64_t)(c.medium.zAxisSize - 1) : 1ull);
-    return NextEventEstimation::footprint(P, c.illumination.num, c.view.num, c.sampler.tNum, fp64, frames, cells);
+    return NextEventEstimation::footprint(P, c.illumination.num, c.view.num, c.sampler.tNum, fp64, frames, cells,
+                                          c.tracer.isCBS != 0);
 }
 
 void sstmc_server::admit() {

@@ -147,6 +147,7 @@ struct TracerOptions {
     uint64_t seed;
     double lambda;
     int output;        // SSTMC_OUTPUT_*
+    int cbs;           // coherent backscattering (field output only)
     int terms;         // SSTMC_TERMS_*
     int contraction;   // SSTMC_CONTRACT_*
 };
@@ -183,11 +184,14 @@ public:
     int finish(double* u, bool deviceDst = false);
     uint32_t batches() const { return opt_.fullIteration; }
     // device bytes a tracer with these sizes allocates (admission control)
-    static uint64_t footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames, uint64_t cells);
+    static uint64_t footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames, uint64_t cells,
+                              bool cbs);
     int debugBatch(uint64_t pathBegin, uint32_t npaths, uint64_t* nvtxOut, uint32_t* pathK, uint32_t* pathMisses,
                    uint64_t* pathOff, float* vtxPos, uint32_t* vtxMat, float* vtxRho, float* vtxXt, double* vtxLen,
                    double* termA, double* termB, double* termSa, double* termSb, double* uBatch);
     int debugContract(uint32_t npaths, const double* a, const double* b, double* u);
+    // coherent-backscattering terms (Ar [p][t][v], Br [p][t][l]) of the last debugBatch of n paths
+    int debugCbsTerms(uint32_t n, double* termAr, double* termBr) const;
     void timing(sstmc_timing* out) const;
     hipStream_t stream() const { return stream_; }
     uint32_t L() const { return L_; }
@@ -202,6 +206,8 @@ private:
     int sizePool();
     VertexBatch vertexBatch() const;
     int buildWalkGrid();
+    void fetchTerms(const DevBuf<unsigned char>& src, uint32_t n, uint32_t width, double* dst) const;
+    uint32_t lastDebugPaths_ = 0;
     int runBatch(uint64_t pathBegin, uint32_t npaths, double* U, bool timed, uint32_t b);
     int contractSingle(uint32_t npaths, double* U);
 
@@ -230,6 +236,7 @@ private:
     uint64_t cap_ = 0;
     bool poolSized_ = false;   // sizePool ran: the pool holds a batch of this request with margin
     DevBuf<unsigned char> A_, B_, a_, b_;
+    DevBuf<unsigned char> Ar_, Br_;   // coherent backscattering: reversed-path terms
     DevBuf<double> U_;
     DevBuf<float> Ff_;
     DevBuf<double> Fd_;

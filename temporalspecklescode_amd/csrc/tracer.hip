@@ -417,8 +417,9 @@ int NextEventEstimation::allocate() {
     size_t elem = fp64_ ? 16 : 8;
     size_t na = (size_t)T_ * P_ * L_ * elem, nb = (size_t)T_ * P_ * V_ * elem;
     if ((e = A_.alloc(na)) || (e = a_.alloc(na)) || (e = B_.alloc(nb)) || (e = b_.alloc(nb))) return e;
+    if (opt_.cbs && ((e = Ar_.alloc(nb)) || (e = Br_.alloc(na)))) return e;
     if ((e = U_.alloc(2 * (size_t)T_ * L_ * V_))) return e;
-    bytes_ += 2 * (na + nb) + 16 * (size_t)T_ * L_ * V_ + 20 * (size_t)P_;
+    bytes_ += (opt_.cbs ? 3 : 2) * (na + nb) + 16 * (size_t)T_ * L_ * V_ + 20 * (size_t)P_;
     return growPool((uint64_t)P_ * 32);
 }
 
@@ -475,12 +476,18 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     prm.tauCutF = 210.0f;     // exp(-105) rounds to zero in FP32: far-field walks stop there
     prm.tauCutD = 1.0e300;
     prm.npaths = npaths;
-    NeeOut out{A_.p, B_.p, a_.p, b_.p, totals_.p + 2};
+    NeeOut out{A_.p, B_.p, a_.p, b_.p, Ar_.p, Br_.p, totals_.p + 2};
     if (launchNee(dims_, fp64_ ? 1 : 0, m, sc, il, vw, fr, vb, prm, out, stream_, e ? e[5] : nullptr) != hipSuccess)
         return SSTMC_KERNEL_ERROR;
     if (e) hipEventRecord(e[2], stream_);
-    if (launchContract(contractKind_, A_.p, B_.p, npaths, L_, V_, T_, nullptr, nullptr, 0, 1.0, U, &wsMulti_, stream_) !=
-        hipSuccess)
+    // multiple scattering: U += sum_p A^T B; with coherent backscattering the reversed paths add
+    // sum_p Br^T Ar and both are scaled by 1/sqrt(2) (NextEventEstimation.cuh:1385-1391,1448-1454)
+    const double msScale = opt_.cbs ? 0.70710678118654752440 : 1.0;
+    if (launchContract(contractKind_, A_.p, B_.p, npaths, L_, V_, T_, nullptr, nullptr, 0, msScale, U, &wsMulti_,
+                       stream_) != hipSuccess)
+        return SSTMC_KERNEL_ERROR;
+    if (opt_.cbs && launchContract(contractKind_, Br_.p, Ar_.p, npaths, L_, V_, T_, nullptr, nullptr, 0, msScale, U,
+                                   &wsMulti_, stream_) != hipSuccess)
         return SSTMC_KERNEL_ERROR;
     if (e) hipEventRecord(e[3], stream_);
     int rc = contractSingle(npaths, U);
@@ -490,9 +497,9 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
 }
 
 uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames,
-                                        uint64_t cells) {
+                                        uint64_t cells, bool cbs) {
     uint64_t elem = fp64 ? 16 : 8;
-    uint64_t terms = 2ull * T * P * (uint64_t)(L + V) * elem;
+    uint64_t terms = (cbs ? 3ull : 2ull) * T * P * (uint64_t)(L + V) * elem;
     uint64_t field = 16ull * T * L * V;
     uint64_t pool = (uint64_t)P * 32 * (20 + 32ull * T + (frames ? 12ull * T : 0));
     uint64_t ws = 2ull * 8 * T * L * V * elem;   // split-K partials, two workspaces
@@ -662,6 +669,33 @@ int NextEventEstimation::trace(double* u) {
 
 void NextEventEstimation::timing(sstmc_timing* out) const { *out = last_; }
 
+// Per-path terms of the last batch, from the [t][p][x] device layout to [p][t][x] complex doubles.
+void NextEventEstimation::fetchTerms(const DevBuf<unsigned char>& src, uint32_t n, uint32_t width, double* dst) const {
+    const size_t elem = fp64_ ? 16 : 8;
+    std::vector<unsigned char> h((size_t)T_ * n * width * elem);
+    hipMemcpy(h.data(), src.p, h.size(), hipMemcpyDeviceToHost);
+    for (uint32_t t = 0; t < T_; ++t)
+        for (uint32_t p = 0; p < n; ++p)
+            for (uint32_t x = 0; x < width; ++x) {
+                size_t si = ((size_t)t * n + p) * width + x;
+                size_t di = (((size_t)p * T_ + t) * width + x) * 2;
+                if (fp64_) {
+                    const double* s = (const double*)h.data();
+                    dst[di] = s[2 * si]; dst[di + 1] = s[2 * si + 1];
+                } else {
+                    const float* s = (const float*)h.data();
+                    dst[di] = s[2 * si]; dst[di + 1] = s[2 * si + 1];
+                }
+            }
+}
+
+int NextEventEstimation::debugCbsTerms(uint32_t n, double* termAr, double* termBr) const {
+    if (!opt_.cbs || n != lastDebugPaths_ || n == 0) return SSTMC_INVALID_CONFIG;
+    if (termAr) fetchTerms(Ar_, n, V_, termAr);
+    if (termBr) fetchTerms(Br_, n, L_, termBr);
+    return SSTMC_NO_ERROR;
+}
+
 int NextEventEstimation::debugBatch(uint64_t pathBegin, uint32_t n, uint64_t* nvtxOut, uint32_t* pathK,
                                     uint32_t* pathMisses, uint64_t* pathOff, float* vtxPos, uint32_t* vtxMat,
                                     float* vtxRho, float* vtxXt, double* vtxLen, double* termA, double* termB,
@@ -733,25 +767,10 @@ int NextEventEstimation::debugBatch(uint64_t pathBegin, uint32_t n, uint64_t* nv
                     vtxLen[o * T_ + t] = acc;
                 }
             }
-        // per-path terms, from [t][p][x] device layout to [p][t][x]
-        size_t elem = fp64_ ? 16 : 8;
         auto fetch = [&](const DevBuf<unsigned char>& src, uint32_t width, double* dst) {
-            std::vector<unsigned char> h((size_t)T_ * n * width * elem);
-            hipMemcpy(h.data(), src.p, h.size(), hipMemcpyDeviceToHost);
-            for (uint32_t t = 0; t < T_; ++t)
-                for (uint32_t p = 0; p < n; ++p)
-                    for (uint32_t x = 0; x < width; ++x) {
-                        size_t si = ((size_t)t * n + p) * width + x;
-                        size_t di = (((size_t)p * T_ + t) * width + x) * 2;
-                        if (fp64_) {
-                            const double* s = (const double*)h.data();
-                            dst[di] = s[2 * si]; dst[di + 1] = s[2 * si + 1];
-                        } else {
-                            const float* s = (const float*)h.data();
-                            dst[di] = s[2 * si]; dst[di + 1] = s[2 * si + 1];
-                        }
-                    }
+            fetchTerms(src, n, width, dst);
         };
+        lastDebugPaths_ = n;
         if (termA) fetch(A_, L_, termA);
         if (termB) fetch(B_, V_, termB);
         if (termSa) fetch(a_, L_, termSa);

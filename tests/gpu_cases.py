@@ -63,3 +63,23 @@ def moving_ff_points(paths=256, seed=6, temporal_amp=True):
                       "U": {"x": np.array([0.0, 0.02, -0.01, 0.0]), "y": np.zeros(4), "z": np.array([0.0, 0.0, 0.03, 0.01])},
                       "isTemporalAmplitude": temporal_amp}
     return cfg
+
+
+def cbs_backscatter(paths=256, n=12, nd=6, g=0.8, seed=9, point=False):
+    """Coherent backscattering, reflection geometry: plane waves entering through z = -n/2 and
+    views that are their exact negations (view v = -illumination v), heterogeneous grid; or point
+    sources and sensors on the same side of the box."""
+    cfg = W.c2(paths=paths, batch=paths, seed=seed, n=n)
+    if not point:
+        d = W._dirs_random(nd, 10.0, seed, 21)
+        cfg["illumination"] = {"type": "FarField", "direction": d}
+        cfg["view"] = {"type": "FarField", "direction": {k: -np.asarray(v) for k, v in d.items()}}
+    else:
+        xy = np.linspace(-n / 2 + 2, n / 2 - 2, 2)
+        gx, gy = np.meshgrid(xy, xy, indexing="ij")
+        loc = {"x": gx.ravel(), "y": gy.ravel(), "z": np.full(4, -n / 2 - 1.0)}
+        cfg["illumination"] = {"type": "PointSource", "location": loc}
+        cfg["view"] = {"type": "PointSource", "location": {k: v + 0.5 for k, v in loc.items()}}
+    cfg["scattering"]["amplitudeFunction"] = [g] * 4
+    cfg["tracer"] = {"type": "nee", "isCBS": True}
+    return cfg

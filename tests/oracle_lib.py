@@ -92,6 +92,22 @@ def terms_from_vertices(cfg: dict, dims: int, k, off, pos, mat, rho, xt, ln):
     return a[..., 0] + 1j * a[..., 1], b[..., 0] + 1j * b[..., 1], s[..., 0] + 1j * s[..., 1]
 
 
+def cbs_terms_from_vertices(cfg: dict, dims: int, k, off, pos, mat, rho, xt, ln):
+    """FP64 coherent-backscattering terms (Ar [n, Nt, Nv], Br [n, Nt, Nl]) for given vertices."""
+    cc = CConfig(cfg, dims)
+    nl, nv, nt, _ = sizes(cfg, dims)
+    n = len(k)
+    args = [np.ascontiguousarray(x, dt) for x, dt in ((k, np.uint32), (off, np.uint64), (pos, np.float64),
+                                                      (mat, np.uint32), (rho, np.float64), (xt, np.float64),
+                                                      (ln, np.float64))]
+    ar = np.zeros((n, nt, nv, 2))
+    br = np.zeros((n, nt, nl, 2))
+    rc = lib().oracle_cbs_terms_from_vertices(cc.ptr(), C.c_uint32(n), *[_p(a) for a in args], _p(ar), _p(br))
+    if rc:
+        raise RuntimeError("oracle_cbs_terms_from_vertices failed with code %d" % rc)
+    return ar[..., 0] + 1j * ar[..., 1], br[..., 0] + 1j * br[..., 1]
+
+
 def philox(ctr, key):
     c = (C.c_uint32 * 4)(*ctr)
     k = (C.c_uint32 * 2)(*key)

@@ -58,9 +58,11 @@ def test_invalid_configs_rejected_before_device_use(native):
     negd = np.array([0.0, -1.0])
     cc.struct.sampler.D = negd.ctypes.data_as(C.POINTER(C.c_double))
     assert native.sstmc_create(cc.ptr(), C.byref(h)) == 7
-    # coherent backscattering is not built: NOT_SUPPORTED
-    cb = dict(base); cb["tracer"] = {"type": "nee", "isCBS": True}
+    # coherent backscattering with the correlation output is not built: NOT_SUPPORTED
+    cb = dict(base); cb["tracer"] = {"type": "nee", "isCBS": True, "output": "correlation"}
     assert _create(cb, 2) == 5
+    cb["tracer"] = {"type": "nee", "isCBS": True}
+    assert native.sstmc_validate(config.CConfig(cb, 2).ptr()) == 0
     # non-increasing axis
     cc = config.CConfig(base, 2)
     ax = np.array([1.0, 0.0])

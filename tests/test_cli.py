@@ -88,9 +88,12 @@ def test_invalid_configs_match_python_messages(tmp_path, mutate):
 
 def test_validate_codes_from_the_abi(tmp_path):
     c = W.c1(paths=1024, batch=1024)
-    c = dict(c, tracer={"type": "nee", "isCBS": True})
+    c = dict(c, tracer={"type": "nee", "isCBS": True, "output": "correlation"})
     r = U.run(c, tmp_path, "--check", dims=2)
     assert r.returncode == 5 and "not supported" in r.stderr.lower()
+    c["tracer"] = {"type": "nee", "isCBS": True}
+    r = U.run(c, tmp_path, "--check", dims=2)
+    assert r.returncode == 0, r.stderr
     r = U.run(W.c1(paths=1024, batch=1024), tmp_path, "--check", dims=4)
     assert r.returncode == 7
 
