@@ -408,9 +408,15 @@ __global__ void __launch_bounds__(256) k_vertex_prep(VertexBatch vb, FramesView 
     }
 }
 
-// G_k of one view at one frame: attenuated source term g_v(x_k^t) times the HG amplitude against
+// Source kinds of the per-vertex sums, fixed at compile time so that the FarField role (which
+// flips the walk direction, the phase sign and the incoming direction) is no runtime branch in the
+// hot loop: point sources; FarField views (the NEE view terms); FarField illuminations (the
+// reversed paths of coherent backscattering).
+enum : int { kSrcPoint = 0, kSrcFarView = 1, kSrcFarIllum = 2 };
+
+// G_k of one source at one frame: attenuated source term g(x_k^t) times the HG amplitude against
 // the record's back direction, with the record's phase base added to the source phase.
-template <int DIMS, typename R, int VWFF>
+template <int DIMS, typename R, int SRC>
 __device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, const ScatterView& sc,
                                             const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
                                             uint32_t v, const d3& vd, uint64_t ok, uint32_t t, const float4& xkm,
@@ -426,7 +432,9 @@ __device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, 
     } else {
         const d3 pk = fr.temporalAmp ? xk.pd : mkd(xkm.x, xkm.y, xkm.z);
         // incoming direction at x_k (FarField: illumination d, view -d; points: from the source)
-        const d3 av = VWFF ? (vw.illum ? vd : mkd(-vd.x, -vd.y, -vd.z)) : dnormalize<DIMS>(dsub(pk, vd));
+        const d3 av = SRC == kSrcFarIllum  ? vd
+                      : SRC == kSrcFarView ? mkd(-vd.x, -vd.y, -vd.z)
+                                           : dnormalize<DIMS>(dsub(pk, vd));
         fv = phaseAmp<DIMS, R>(sc, __float_as_uint(xkm.w), ddot<DIMS>(av, mkd(rec.bx, rec.by, rec.bz)));
     }
     return cis<R>(ga * fv, gp + rec.phase);
@@ -437,11 +445,13 @@ __device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, 
 // records are group-uniform broadcasts). Splitting a path over its item blocks keeps the longest
 // unit short: vertex counts are heavy-tailed (the longest path of a batch can hold ~15x the mean),
 // and a unit spanning all items of such a path would outlast a whole SIMD's share of the batch.
-// Templated on the view source type so the loop carries only what it needs.
-template <int DIMS, typename R, int VWFF>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr,
-                                                  VertexBatch vb, NeeParams prm, NeeOut out) {
+// Templated on the source kind so the loop carries only what it needs.
+template <int DIMS, typename R, int SRC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+k_nee_view(MediumView m, ScatterView sc, SourceView src, FramesView fr, VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
+    SourceView vw = src;
+    if (SRC != kSrcPoint) vw.illum = SRC == kSrcFarIllum;
     __shared__ R sigLds[kMaxMaterials];
     const R* sig = loadSigTable<R>(m, sigLds);
     const uint32_t lane = threadIdx.x & 63, lpp = prm.lppView, sub = lane & (lpp - 1);
@@ -452,9 +462,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
     const uint64_t off = p < prm.npaths ? vb.off[p] : 0;
     const uint32_t P = prm.npaths, V = vw.n, NT = fr.nt;
     const double invLambda = 1.0 / prm.lambda;
-    const R tauCut = VWFF ? (sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD) : (R)3e38;
+    const R tauCut = SRC != kSrcPoint ? (sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD) : (R)3e38;
     C* B = (C*)out.B;
-    const bool isoPoint = !VWFF && sc.allIsotropic;
+    const bool isoPoint = SRC == kSrcPoint && sc.allIsotropic;
     uint32_t steps = 0;
     const uint32_t i = blk * lpp + sub;
     const bool active = i < V * NT && K >= 2;
@@ -465,7 +475,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
         R accRe = 0, accIm = 0;
         for (uint32_t k = 1; k < K; ++k) {
             const uint64_t ok = off + k;
-            const C g = viewTerm<DIMS, R, VWFF>(m, sig, sc, vw, fr, vb, v, vd, ok, t, vb.xm[ok], invLambda, tauCut,
+            const C g = viewTerm<DIMS, R, SRC>(m, sig, sc, vw, fr, vb, v, vd, ok, t, vb.xm[ok], invLambda, tauCut,
                                                isoPoint, &steps);
             accRe += g.x;
             accIm += g.y;
@@ -481,10 +491,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
 // G_k(0) is formed once and reused for every frame of the chunk, whose running sums live in LDS
 // (one slot per thread and frame).
 constexpr uint32_t kCorrLags = 16;
-template <int DIMS, typename R, int VWFF>
-__global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView sc, SourceView vw, FramesView fr,
+template <int DIMS, typename R, int SRC>
+__global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView sc, SourceView src, FramesView fr,
                                                        VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
+    SourceView vw = src;
+    if (SRC != kSrcPoint) vw.illum = SRC == kSrcFarIllum;
     __shared__ R sigLds[kMaxMaterials];
     __shared__ C acc[kCorrLags][256];
     const R* sig = loadSigTable<R>(m, sigLds);
@@ -499,9 +511,9 @@ __global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView
     const uint32_t P = prm.npaths, V = vw.n, NT = fr.nt;
     const uint32_t t0 = chunk * kCorrLags, nt = NT - t0 < kCorrLags ? NT - t0 : kCorrLags;
     const double invLambda = 1.0 / prm.lambda;
-    const R tauCut = VWFF ? (sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD) : (R)3e38;
+    const R tauCut = SRC != kSrcPoint ? (sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD) : (R)3e38;
     C* B = (C*)out.B;
-    const bool isoPoint = !VWFF && sc.allIsotropic;
+    const bool isoPoint = SRC == kSrcPoint && sc.allIsotropic;
     uint32_t steps = 0;
     const uint32_t v = blk * lpp + sub;
     const bool active = v < V && K >= 2 && t0 < NT;
@@ -512,12 +524,12 @@ __global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView
         for (uint32_t k = 1; k < K; ++k) {
             const uint64_t ok = off + k;
             const float4 xkm = vb.xm[ok];
-            const C g0 = viewTerm<DIMS, R, VWFF>(m, sig, sc, vw, fr, vb, vv, vd, ok, 0, xkm, invLambda, tauCut,
+            const C g0 = viewTerm<DIMS, R, SRC>(m, sig, sc, vw, fr, vb, vv, vd, ok, 0, xkm, invLambda, tauCut,
                                                 isoPoint, &steps);
             for (uint32_t j = 0; j < nt; ++j) {
                 const uint32_t t = t0 + j;
                 const C gt = t == 0 ? g0
-                                    : viewTerm<DIMS, R, VWFF>(m, sig, sc, vw, fr, vb, vv, vd, ok, t, xkm, invLambda,
+                                    : viewTerm<DIMS, R, SRC>(m, sig, sc, vw, fr, vb, vv, vd, ok, t, xkm, invLambda,
                                                               tauCut, isoPoint, &steps);
                 C a = acc[j][threadIdx.x];
                 a.x += g0.x * gt.x + g0.y * gt.y;      // G(0) * conj(G(t))
@@ -544,10 +556,13 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
     const uint64_t units = (uint64_t)p.npaths * p.viewBlocks * p.lagChunks, upb = 4 * (64 / p.lppView);
     grid = dim3((uint32_t)((units + upb - 1) / upb));
     const bool ff = vw.type == 2, corr = p.output == 1;
-    if (ff && !corr) hipLaunchKernelGGL((k_nee_view<DIMS, R, 1>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
-    else if (ff && corr) hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, 1>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
-    else if (!ff && !corr) hipLaunchKernelGGL((k_nee_view<DIMS, R, 0>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
-    else hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, 0>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
+    if (ff && !corr)
+        hipLaunchKernelGGL((k_nee_view<DIMS, R, kSrcFarView>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
+    else if (ff && corr)
+        hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, kSrcFarView>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
+    else if (!ff && !corr)
+        hipLaunchKernelGGL((k_nee_view<DIMS, R, kSrcPoint>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
+    else hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, kSrcPoint>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
     if (out.Br) {
         // coherent backscattering: the same per-vertex sums with the illumination connected at
         // x_k (the reversed path), into Br
@@ -560,8 +575,9 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
         o.B = out.Br;
         const uint64_t u2 = (uint64_t)q.npaths * q.viewBlocks, upb2 = 4 * (64 / q.lppView);
         const dim3 g2((uint32_t)((u2 + upb2 - 1) / upb2));
-        if (il.type == 2) hipLaunchKernelGGL((k_nee_view<DIMS, R, 1>), g2, block, 0, st, m, sc, il, fr, vb, q, o);
-        else hipLaunchKernelGGL((k_nee_view<DIMS, R, 0>), g2, block, 0, st, m, sc, il, fr, vb, q, o);
+        if (il.type == 2)
+            hipLaunchKernelGGL((k_nee_view<DIMS, R, kSrcFarIllum>), g2, block, 0, st, m, sc, il, fr, vb, q, o);
+        else hipLaunchKernelGGL((k_nee_view<DIMS, R, kSrcPoint>), g2, block, 0, st, m, sc, il, fr, vb, q, o);
     }
 }
 
