@@ -85,6 +85,7 @@ __host__ __device__ inline double comp(const d3& v, int a) { return a == 0 ? v.x
 struct MediumView {
     int dims;
     uint32_t n[3];                 // cells per axis (1 for the unused z in 2D)
+    float nf[3];                   // n as float (the fast locate's clamp, read as a scalar operand)
     const float* ef[3];            // edges, n[a] + 1 each
     const double* ed[3];
     float bminf[3], bmaxf[3];

@@ -51,7 +51,7 @@ __host__ __device__ inline int locate(const MediumView& m, int a, R x) {
     int b;
     if (m.uniformAxis[a]) {
         float f = ((float)x - m.e0[a]) * m.invH[a];
-        b = f <= 0.f ? 0 : (f >= (float)n ? n - 1 : (int)f);
+        b = f <= 0.f ? 0 : (f >= m.nf[a] ? n - 1 : (int)f);
         if (b > 0 && x < e[b]) --b;
         if (b < n - 1 && x >= e[b + 1]) ++b;
         return b;

@@ -131,35 +131,55 @@ __device__ inline Geo vertexFrame(const VertexBatch& vb, const FramesView& fr, u
 }
 
 // g of one source at one position: amplitude (with sqrt of the transmittance) and phase in cycles.
-// FarField: gDirectionKernel (FarField.cuh:223-263); PointSource: gPointKernel (PointSource.cuh:117-172).
+// FarField: gDirectionKernel (FarField.cuh:223-263), from the source's direction dd (FP64) and df
+// (FP32, the walk direction's rounding); the illumination walks against the direction and takes
+// the positive phase, the view walks along it with the negative phase.
+template <int DIMS, typename R>
+__device__ inline void gFar(const MediumView& m, const R* sig, const d3& dd, const f3& df, bool illum, const Geo& x,
+                            double invLambda, R tauCut, uint32_t* steps, R* amp, double* ph) {
+    typedef typename Pos<R>::V V;
+    const V p = Pos<R>::pick(x.pf, x.pd);
+    const double phase = ddot<DIMS>(dd, x.pd) * invLambda;
+    *ph = illum ? phase : -phase;
+    V w;
+    if constexpr (sizeof(R) == 4) w = illum ? mk(-df.x, -df.y, -df.z) : df;
+    else w = illum ? mkd(-dd.x, -dd.y, -dd.z) : dd;
+    const R tau = walk<DIMS, R>(m, sig, p, w, tauCut, steps);
+    *amp = expR<R>(-(R)0.5 * tau);
+}
+// PointSource: gPointKernel (PointSource.cuh:117-172), from the source location sp (z = 0 in 2D).
+template <int DIMS, typename R>
+__device__ inline void gPoint(const MediumView& m, const R* sig, const d3& sp, bool area, const Geo& x,
+                              double invLambda, uint32_t* steps, R* amp, double* ph) {
+    typedef typename Pos<R>::V V;
+    const V p = Pos<R>::pick(x.pf, x.pd);
+    const d3 q = dsub(sp, x.pd);
+    const double r = sqrt(ddot<DIMS>(q, q));
+    const double rr = 1.0 / r;
+    *ph = r * invLambda;
+    const V w = Pos<R>::fromD(mkd(q.x * rr, q.y * rr, q.z * rr));
+    const R tau = walk<DIMS, R>(m, sig, p, w, (R)3e38, steps);
+    double fac = DIMS == 3 ? rr * 0.531125966013598 /* sqrt(ISOTROPIC_AMPLITUDE) */ : sqrt(rr);
+    if (area) fac *= sqrt(rr * fabs(DIMS == 3 ? x.pd.z - sp.z : x.pd.y - sp.y));
+    *amp = expR<R>(-(R)0.5 * tau) * (R)fac;
+}
+// FP64 direction or location of source i (z = 0 for a 2D location) and its FP32 direction.
+template <int DIMS>
+__device__ inline d3 sourceD(const SourceView& s, uint32_t i) {
+    const double* q = s.pd + 3 * (size_t)i;
+    return mkd(q[0], q[1], DIMS == 3 || s.type == 2 ? q[2] : 0.0);
+}
+__device__ inline f3 sourceF(const SourceView& s, uint32_t i) {
+    const float* q = s.pf + 3 * (size_t)i;
+    return mk(q[0], q[1], q[2]);
+}
 template <int DIMS, typename R>
 __device__ inline void gTerm(const MediumView& m, const R* sig, const SourceView& s, uint32_t i, const Geo& x,
                              double invLambda, R tauCut, uint32_t* steps, R* amp, double* ph) {
-    typedef typename Pos<R>::V V;
-    V p = Pos<R>::pick(x.pf, x.pd);
-    if (s.type == 2) {
-        d3 dd = mkd(s.pd[3 * i], s.pd[3 * i + 1], s.pd[3 * i + 2]);
-        double phase = ddot<DIMS>(dd, x.pd) * invLambda;
-        *ph = s.illum ? phase : -phase;
-        d3 wd = s.illum ? mkd(-dd.x, -dd.y, -dd.z) : dd;
-        V w = Pos<R>::fromD(wd);
-        if (sizeof(R) == 4) w = Pos<R>::fromD(mkd(s.illum ? -(double)s.pf[3 * i] : (double)s.pf[3 * i],
-                                                  s.illum ? -(double)s.pf[3 * i + 1] : (double)s.pf[3 * i + 1],
-                                                  s.illum ? -(double)s.pf[3 * i + 2] : (double)s.pf[3 * i + 2]));
-        R tau = walk<DIMS, R>(m, sig, p, w, tauCut, steps);
-        *amp = expR<R>(-(R)0.5 * tau);
-    } else {
-        d3 sp = mkd(s.pd[3 * i], s.pd[3 * i + 1], DIMS == 3 ? s.pd[3 * i + 2] : 0.0);
-        d3 q = dsub(sp, x.pd);
-        double r = sqrt(ddot<DIMS>(q, q));
-        double rr = 1.0 / r;
-        *ph = r * invLambda;
-        V w = Pos<R>::fromD(mkd(q.x * rr, q.y * rr, q.z * rr));
-        R tau = walk<DIMS, R>(m, sig, p, w, (R)3e38, steps);
-        double fac = DIMS == 3 ? rr * 0.531125966013598 /* sqrt(ISOTROPIC_AMPLITUDE) */ : sqrt(rr);
-        if (s.area) fac *= sqrt(rr * fabs(DIMS == 3 ? x.pd.z - sp.z : x.pd.y - sp.y));
-        *amp = expR<R>(-(R)0.5 * tau) * (R)fac;
-    }
+    if (s.type == 2)
+        gFar<DIMS, R>(m, sig, sourceD<DIMS>(s, i), sizeof(R) == 4 ? sourceF(s, i) : mk(0.f, 0.f, 0.f), s.illum != 0,
+                      x, invLambda, tauCut, steps, amp, ph);
+    else gPoint<DIMS, R>(m, sig, sourceD<DIMS>(s, i), s.area != 0, x, invLambda, steps, amp, ph);
 }
 
 // Incoming unit direction at x from source i as the three-point functions use it: FarField
@@ -415,17 +435,20 @@ __global__ void __launch_bounds__(256) k_vertex_prep(VertexBatch vb, FramesView 
 enum : int { kSrcPoint = 0, kSrcFarView = 1, kSrcFarIllum = 2 };
 
 // G_k of one source at one frame: attenuated source term g(x_k^t) times the HG amplitude against
-// the record's back direction, with the record's phase base added to the source phase.
+// the record's back direction, with the record's phase base added to the source phase. The
+// source's direction or location arrives in registers (vd FP64, vf FP32 direction), loaded once
+// per lane before the vertex loop.
 template <int DIMS, typename R, int SRC>
 __device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, const ScatterView& sc,
                                             const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
-                                            uint32_t v, const d3& vd, uint64_t ok, uint32_t t, const float4& xkm,
+                                            const d3& vd, const f3& vf, uint64_t ok, uint32_t t, const float4& xkm,
                                             double invLambda, R tauCut, bool isoPoint, uint32_t* steps) {
     const Geo xk = vertexFrame(vb, fr, ok, t, xkm);
     const VertexRec rec = vb.rec[ok * fr.nt + t];
     R ga;
     double gp;
-    gTerm<DIMS, R>(m, sig, vw, v, xk, invLambda, tauCut, steps, &ga, &gp);
+    if constexpr (SRC == kSrcPoint) gPoint<DIMS, R>(m, sig, vd, vw.area != 0, xk, invLambda, steps, &ga, &gp);
+    else gFar<DIMS, R>(m, sig, vd, vf, SRC == kSrcFarIllum, xk, invLambda, tauCut, steps, &ga, &gp);
     R fv;
     if (isoPoint) {
         fv = isoAmp<DIMS, R>();
@@ -448,10 +471,8 @@ __device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, 
 // Templated on the source kind so the loop carries only what it needs.
 template <int DIMS, typename R, int SRC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
-k_nee_view(MediumView m, ScatterView sc, SourceView src, FramesView fr, VertexBatch vb, NeeParams prm, NeeOut out) {
+k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
-    SourceView vw = src;
-    if (SRC != kSrcPoint) vw.illum = SRC == kSrcFarIllum;
     __shared__ R sigLds[kMaxMaterials];
     const R* sig = loadSigTable<R>(m, sigLds);
     const uint32_t lane = threadIdx.x & 63, lpp = prm.lppView, sub = lane & (lpp - 1);
@@ -470,12 +491,13 @@ k_nee_view(MediumView m, ScatterView sc, SourceView src, FramesView fr, VertexBa
     const bool active = i < V * NT && K >= 2;
     const uint32_t v = active ? i % V : 0, t = active ? i / V : 0;
     // view direction (far field) or location (point) of this lane's item
-    const d3 vd = mkd(vw.pd[3 * v], vw.pd[3 * v + 1], DIMS == 3 ? vw.pd[3 * v + 2] : 0.0);
+    const d3 vd = sourceD<DIMS>(vw, v);
+    const f3 vf = SRC != kSrcPoint && sizeof(R) == 4 ? sourceF(vw, v) : mk(0.f, 0.f, 0.f);
     if (active) {
         R accRe = 0, accIm = 0;
         for (uint32_t k = 1; k < K; ++k) {
             const uint64_t ok = off + k;
-            const C g = viewTerm<DIMS, R, SRC>(m, sig, sc, vw, fr, vb, v, vd, ok, t, vb.xm[ok], invLambda, tauCut,
+            const C g = viewTerm<DIMS, R, SRC>(m, sig, sc, vw, fr, vb, vd, vf, ok, t, vb.xm[ok], invLambda, tauCut,
                                                isoPoint, &steps);
             accRe += g.x;
             accIm += g.y;
@@ -492,11 +514,9 @@ k_nee_view(MediumView m, ScatterView sc, SourceView src, FramesView fr, VertexBa
 // (one slot per thread and frame).
 constexpr uint32_t kCorrLags = 16;
 template <int DIMS, typename R, int SRC>
-__global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView sc, SourceView src, FramesView fr,
+__global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView sc, SourceView vw, FramesView fr,
                                                        VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
-    SourceView vw = src;
-    if (SRC != kSrcPoint) vw.illum = SRC == kSrcFarIllum;
     __shared__ R sigLds[kMaxMaterials];
     __shared__ C acc[kCorrLags][256];
     const R* sig = loadSigTable<R>(m, sigLds);
@@ -518,18 +538,19 @@ __global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView
     const uint32_t v = blk * lpp + sub;
     const bool active = v < V && K >= 2 && t0 < NT;
     const uint32_t vv = active ? v : 0;
-    const d3 vd = mkd(vw.pd[3 * vv], vw.pd[3 * vv + 1], DIMS == 3 ? vw.pd[3 * vv + 2] : 0.0);
+    const d3 vd = sourceD<DIMS>(vw, vv);
+    const f3 vf = SRC != kSrcPoint && sizeof(R) == 4 ? sourceF(vw, vv) : mk(0.f, 0.f, 0.f);
     if (active) {
         for (uint32_t j = 0; j < nt; ++j) acc[j][threadIdx.x] = Cx<R>::make(0, 0);
         for (uint32_t k = 1; k < K; ++k) {
             const uint64_t ok = off + k;
             const float4 xkm = vb.xm[ok];
-            const C g0 = viewTerm<DIMS, R, SRC>(m, sig, sc, vw, fr, vb, vv, vd, ok, 0, xkm, invLambda, tauCut,
+            const C g0 = viewTerm<DIMS, R, SRC>(m, sig, sc, vw, fr, vb, vd, vf, ok, 0, xkm, invLambda, tauCut,
                                                 isoPoint, &steps);
             for (uint32_t j = 0; j < nt; ++j) {
                 const uint32_t t = t0 + j;
                 const C gt = t == 0 ? g0
-                                    : viewTerm<DIMS, R, SRC>(m, sig, sc, vw, fr, vb, vv, vd, ok, t, xkm, invLambda,
+                                    : viewTerm<DIMS, R, SRC>(m, sig, sc, vw, fr, vb, vd, vf, ok, t, xkm, invLambda,
                                                               tauCut, isoPoint, &steps);
                 C a = acc[j][threadIdx.x];
                 a.x += g0.x * gt.x + g0.y * gt.y;      // G(0) * conj(G(t))

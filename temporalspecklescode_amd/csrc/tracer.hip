@@ -56,6 +56,7 @@ HeterogeneousMedium::HeterogeneousMedium(int* err, int dims, const uint32_t* gri
         const double* e = a < dims ? ax[a] : flat;
         uint32_t n = sz[a];
         v_.n[a] = n - 1;
+        v_.nf[a] = (float)(n - 1);
         std::vector<float> f(n);
         for (uint32_t i = 0; i < n; ++i) f[i] = (float)e[i];
         if ((*err = ef_[a].upload(f.data(), n))) return;

@@ -68,6 +68,7 @@ static int run(int n, double h, int trials, unsigned seed, int fast) {
     m.dims = DIMS;
     for (int a = 0; a < 3; ++a) {
         m.n[a] = a < DIMS ? n : 1;
+        m.nf[a] = (float)m.n[a];
         m.ef[a] = ef.data(); m.ed[a] = ed.data();
         m.bminf[a] = ef[0]; m.bmaxf[a] = ef[n]; m.bmind[a] = ed[0]; m.bmaxd[a] = ed[n];
         m.uniformAxis[a] = 1; m.e0[a] = ef[0]; m.invH[a] = (float)(1.0 / h); m.h[a] = (float)h;
