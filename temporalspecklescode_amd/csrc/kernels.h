@@ -35,8 +35,9 @@ struct NeeParams {
     uint32_t npaths;
     uint32_t lpp;          // k_nee: lanes per path (power of two, 8..64), set by launchNee
     uint32_t lppView;      // k_nee_view: lanes per work unit (power of two, 8..64), set by launchNee
-    uint32_t viewBlocks;   // view kernels: blocks of lppView items (views, or view x frame items) per path
-    uint32_t lagChunks;    // correlation view kernel: frame chunks per path (1 otherwise)
+    uint32_t viewBlocks;   // view kernels: blocks of lppView sources per path and frame chunk
+    uint32_t frameChunks;  // view kernels: frame chunks per path (field output: one frame each,
+                           // correlation: up to 16 frames each)
 };
 
 hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc, const SourceView& il,

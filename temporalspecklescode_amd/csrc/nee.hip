@@ -463,36 +463,68 @@ __device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, 
     return cis<R>(ga * fv, gp + rec.phase);
 }
 
-// Multiple-scattering view terms B (the hot loop). Work unit = (path, block of lppView view
-// items): one (view, frame) item per lane, looping over the path's vertices k >= 2 (vertex
-// records are group-uniform broadcasts). Splitting a path over its item blocks keeps the longest
-// unit short: vertex counts are heavy-tailed (the longest path of a batch can hold ~15x the mean),
-// and a unit spanning all items of such a path would outlast a whole SIMD's share of the batch.
-// Templated on the source kind so the loop carries only what it needs.
-template <int DIMS, typename R, int SRC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+// Work unit of the view kernels: (path, frame chunk, block of lppView sources), one source per
+// lane. With WAVE (lppView == 64) a unit fills the wavefront and its index goes through
+// readfirstlane, so the path, its vertex count and offset, the frame and every per-vertex read
+// (position, record, displacement) are wave-uniform: scalar registers and scalar loads, leaving
+// the vector memory path to the walks' scattered word reads.
+constexpr uint32_t kCorrLags = 16;   // frames per correlation work unit
+struct ViewUnit {
+    uint32_t p, chunk, blk, sub;
+};
+template <bool WAVE>
+__device__ inline ViewUnit viewUnit(const NeeParams& prm) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    uint64_t unit;
+    ViewUnit u;
+    if constexpr (WAVE) {
+        unit = (uint32_t)__builtin_amdgcn_readfirstlane((int)wave);
+        u.sub = lane;
+    } else {
+        const uint32_t lpp = prm.lppView;
+        unit = (uint64_t)wave * (64 / lpp) + lane / lpp;
+        u.sub = lane & (lpp - 1);
+    }
+    const uint32_t perPath = prm.viewBlocks * prm.frameChunks;
+    u.p = (uint32_t)(unit / perPath);
+    const uint32_t rem = (uint32_t)(unit % perPath);
+    u.blk = rem % prm.viewBlocks;
+    u.chunk = rem / prm.viewBlocks;
+    return u;
+}
+
+// Multiple-scattering view terms B (the hot loop). Work unit = (path, frame t, block of lppView
+// sources): one source per lane, looping over the path's vertices k >= 2. Splitting a path over
+// its blocks keeps the longest unit short: vertex counts are heavy-tailed (the longest path of a
+// batch can hold ~15x the mean), and a unit spanning all sources of such a path would outlast a
+// whole SIMD's share of the batch. Templated on the source kind so the loop carries only what it
+// needs. Waves per SIMD (measured on 1024 x 1024 far-field and 64 x 64 point-source workloads): 8
+// for FP32 far-field sources (64 VGPRs; the latency of the walk's scattered word reads wants the
+// occupancy), 6 for point sources (80 VGPRs; their longer per-vertex set-up spills in the vertex
+// loop at 64) and for FP64.
+template <int DIMS, typename R, int SRC, bool WAVE>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(SRC != kSrcPoint && sizeof(R) == 4 ? 8 : 6)))
 k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
     __shared__ R sigLds[kMaxMaterials];
     const R* sig = loadSigTable<R>(m, sigLds);
-    const uint32_t lane = threadIdx.x & 63, lpp = prm.lppView, sub = lane & (lpp - 1);
-    const uint64_t unit = (uint64_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / lpp) + lane / lpp;
-    const uint32_t p = (uint32_t)(unit / prm.viewBlocks), blk = (uint32_t)(unit % prm.viewBlocks);
-    const bool poolOk = (uint64_t)vb.off[prm.npaths] <= vb.cap;
-    const uint32_t K = (p < prm.npaths && poolOk) ? vb.k[p] : 0u;   // K < 2: k_nee wrote the zero rows
-    const uint64_t off = p < prm.npaths ? vb.off[p] : 0;
-    const uint32_t P = prm.npaths, V = vw.n, NT = fr.nt;
+    const ViewUnit u = viewUnit<WAVE>(prm);
+    const uint32_t p = u.p, t = u.chunk, P = prm.npaths, V = vw.n;
+    const bool poolOk = (uint64_t)vb.off[P] <= vb.cap;
+    const uint32_t K = (p < P && poolOk) ? vb.k[p] : 0u;   // K < 2: k_nee wrote the zero rows
+    const uint64_t off = p < P ? vb.off[p] : 0;
     const double invLambda = 1.0 / prm.lambda;
     const R tauCut = SRC != kSrcPoint ? (sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD) : (R)3e38;
     C* B = (C*)out.B;
     const bool isoPoint = SRC == kSrcPoint && sc.allIsotropic;
     uint32_t steps = 0;
-    const uint32_t i = blk * lpp + sub;
-    const bool active = i < V * NT && K >= 2;
-    const uint32_t v = active ? i % V : 0, t = active ? i / V : 0;
-    // view direction (far field) or location (point) of this lane's item
-    const d3 vd = sourceD<DIMS>(vw, v);
-    const f3 vf = SRC != kSrcPoint && sizeof(R) == 4 ? sourceF(vw, v) : mk(0.f, 0.f, 0.f);
+    const uint32_t v = u.blk * (WAVE ? 64u : prm.lppView) + u.sub;
+    const bool active = v < V && K >= 2;
+    // direction (far field) or location (point) of this lane's source
+    const d3 vd = sourceD<DIMS>(vw, active ? v : 0);
+    const f3 vf = SRC != kSrcPoint && sizeof(R) == 4 ? sourceF(vw, active ? v : 0) : mk(0.f, 0.f, 0.f);
     if (active) {
         R accRe = 0, accIm = 0;
         for (uint32_t k = 1; k < K; ++k) {
@@ -505,37 +537,32 @@ k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBat
         B[((size_t)t * P + p) * V + v] = Cx<R>::make(accRe, accIm);
     }
     uint32_t tot = waveSum(steps);
-    if (lane == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
+    if ((threadIdx.x & 63) == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
 }
 
 // Correlation view terms sum_k G_k(0) conj G_k(t) (the per-sub-path estimator). Work unit =
-// (path, block of lppView views, chunk of up to kCorrLags frames): one view per lane; per vertex,
+// (path, chunk of up to kCorrLags frames, block of lppView views): one view per lane; per vertex,
 // G_k(0) is formed once and reused for every frame of the chunk, whose running sums live in LDS
 // (one slot per thread and frame).
-constexpr uint32_t kCorrLags = 16;
-template <int DIMS, typename R, int SRC>
+template <int DIMS, typename R, int SRC, bool WAVE>
 __global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView sc, SourceView vw, FramesView fr,
                                                        VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
     __shared__ R sigLds[kMaxMaterials];
     __shared__ C acc[kCorrLags][256];
     const R* sig = loadSigTable<R>(m, sigLds);
-    const uint32_t lane = threadIdx.x & 63, lpp = prm.lppView, sub = lane & (lpp - 1);
-    const uint64_t unit = (uint64_t)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / lpp) + lane / lpp;
-    const uint32_t perPath = prm.viewBlocks * prm.lagChunks;
-    const uint32_t p = (uint32_t)(unit / perPath), rem = (uint32_t)(unit % perPath);
-    const uint32_t blk = rem % prm.viewBlocks, chunk = rem / prm.viewBlocks;
-    const bool poolOk = (uint64_t)vb.off[prm.npaths] <= vb.cap;
-    const uint32_t K = (p < prm.npaths && poolOk) ? vb.k[p] : 0u;
-    const uint64_t off = p < prm.npaths ? vb.off[p] : 0;
-    const uint32_t P = prm.npaths, V = vw.n, NT = fr.nt;
-    const uint32_t t0 = chunk * kCorrLags, nt = NT - t0 < kCorrLags ? NT - t0 : kCorrLags;
+    const ViewUnit u = viewUnit<WAVE>(prm);
+    const uint32_t p = u.p, P = prm.npaths, V = vw.n, NT = fr.nt;
+    const bool poolOk = (uint64_t)vb.off[P] <= vb.cap;
+    const uint32_t K = (p < P && poolOk) ? vb.k[p] : 0u;
+    const uint64_t off = p < P ? vb.off[p] : 0;
+    const uint32_t t0 = u.chunk * kCorrLags, nt = NT - t0 < kCorrLags ? NT - t0 : kCorrLags;
     const double invLambda = 1.0 / prm.lambda;
     const R tauCut = SRC != kSrcPoint ? (sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD) : (R)3e38;
     C* B = (C*)out.B;
     const bool isoPoint = SRC == kSrcPoint && sc.allIsotropic;
     uint32_t steps = 0;
-    const uint32_t v = blk * lpp + sub;
+    const uint32_t v = u.blk * (WAVE ? 64u : prm.lppView) + u.sub;
     const bool active = v < V && K >= 2 && t0 < NT;
     const uint32_t vv = active ? v : 0;
     const d3 vd = sourceD<DIMS>(vw, vv);
@@ -561,7 +588,34 @@ __global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView
         for (uint32_t j = 0; j < nt; ++j) B[((size_t)(t0 + j) * P + p) * V + v] = acc[j][threadIdx.x];
     }
     uint32_t tot = waveSum(steps);
-    if (lane == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
+    if ((threadIdx.x & 63) == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
+}
+
+// Lanes per view-kernel work unit for n sources: a wavefront, halved (down to 8) while half still
+// covers them, so that several units share a wavefront when there are few sources.
+static inline void viewLanes(uint32_t n, NeeParams* q) {
+    q->lppView = 64;
+    while (q->lppView > 8 && q->lppView / 2 >= n) q->lppView /= 2;
+    q->viewBlocks = (n + q->lppView - 1) / q->lppView;
+}
+
+template <int DIMS, typename R, int SRC>
+static void launchView(bool corr, const MediumView& m, const ScatterView& sc, const SourceView& src,
+                       const FramesView& fr, const VertexBatch& vb, const NeeParams& q, const NeeOut& out,
+                       hipStream_t st) {
+    const uint64_t units = (uint64_t)q.npaths * q.viewBlocks * q.frameChunks, upb = 4 * (64 / q.lppView);
+    const dim3 grid((uint32_t)((units + upb - 1) / upb)), block(256);
+    const bool wave = q.lppView == 64;
+    if (corr) {
+        if constexpr (SRC != kSrcFarIllum) {
+            if (wave) hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, SRC, true>), grid, block, 0, st, m, sc, src, fr, vb, q, out);
+            else hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, SRC, false>), grid, block, 0, st, m, sc, src, fr, vb, q, out);
+        }
+    } else if (wave) {
+        hipLaunchKernelGGL((k_nee_view<DIMS, R, SRC, true>), grid, block, 0, st, m, sc, src, fr, vb, q, out);
+    } else {
+        hipLaunchKernelGGL((k_nee_view<DIMS, R, SRC, false>), grid, block, 0, st, m, sc, src, fr, vb, q, out);
+    }
 }
 
 template <int DIMS, typename R>
@@ -574,31 +628,19 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
     hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, vb, fr, p.npaths,
                        1.0 / p.lambda);
     if (between) hipEventRecord(between, st);
-    const uint64_t units = (uint64_t)p.npaths * p.viewBlocks * p.lagChunks, upb = 4 * (64 / p.lppView);
-    grid = dim3((uint32_t)((units + upb - 1) / upb));
-    const bool ff = vw.type == 2, corr = p.output == 1;
-    if (ff && !corr)
-        hipLaunchKernelGGL((k_nee_view<DIMS, R, kSrcFarView>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
-    else if (ff && corr)
-        hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, kSrcFarView>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
-    else if (!ff && !corr)
-        hipLaunchKernelGGL((k_nee_view<DIMS, R, kSrcPoint>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
-    else hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, kSrcPoint>), grid, block, 0, st, m, sc, vw, fr, vb, p, out);
+    const bool corr = p.output == 1;
+    if (vw.type == 2) launchView<DIMS, R, kSrcFarView>(corr, m, sc, vw, fr, vb, p, out, st);
+    else launchView<DIMS, R, kSrcPoint>(corr, m, sc, vw, fr, vb, p, out, st);
     if (out.Br) {
         // coherent backscattering: the same per-vertex sums with the illumination connected at
         // x_k (the reversed path), into Br
         NeeParams q = p;
-        q.lppView = 64;
-        while (q.lppView > 8 && q.lppView / 2 >= il.n * fr.nt) q.lppView /= 2;
-        q.viewBlocks = (il.n * fr.nt + q.lppView - 1) / q.lppView;
-        q.lagChunks = 1;
+        viewLanes(il.n, &q);
+        q.frameChunks = fr.nt;
         NeeOut o = out;
         o.B = out.Br;
-        const uint64_t u2 = (uint64_t)q.npaths * q.viewBlocks, upb2 = 4 * (64 / q.lppView);
-        const dim3 g2((uint32_t)((u2 + upb2 - 1) / upb2));
-        if (il.type == 2)
-            hipLaunchKernelGGL((k_nee_view<DIMS, R, kSrcFarIllum>), g2, block, 0, st, m, sc, il, fr, vb, q, o);
-        else hipLaunchKernelGGL((k_nee_view<DIMS, R, kSrcPoint>), g2, block, 0, st, m, sc, il, fr, vb, q, o);
+        if (il.type == 2) launchView<DIMS, R, kSrcFarIllum>(false, m, sc, il, fr, vb, q, o, st);
+        else launchView<DIMS, R, kSrcPoint>(false, m, sc, il, fr, vb, q, o, st);
     }
 }
 
@@ -606,19 +648,16 @@ hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView&
                      const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
                      const NeeOut& out, hipStream_t st, hipEvent_t between) {
     // lane groups: k_nee gives each path lpp lanes (its L and V items), the view kernels give each
-    // work unit lppView lanes; both shrink below a wavefront when there are few items so that
-    // several paths share one. Field output: a unit is (path, block of view x frame items).
-    // Correlation: a unit is (path, block of views, chunk of up to kCorrLags frames).
+    // work unit lppView lanes (one source each); both shrink below a wavefront when there are few
+    // items so that several paths share one. A view unit is (path, frame, block of views) for
+    // field output and (path, chunk of up to kCorrLags frames, block of views) for correlation.
     NeeParams q = p;
     uint32_t want = il.n > vw.n * fr.nt ? il.n : vw.n * fr.nt;
     q.lpp = 64;
     while (q.lpp > 8 && q.lpp / 2 >= want) q.lpp /= 2;
     const bool corr = p.output == 1;
-    const uint32_t items = corr ? vw.n : vw.n * fr.nt;
-    q.lppView = 64;
-    while (q.lppView > 8 && q.lppView / 2 >= items) q.lppView /= 2;
-    q.viewBlocks = (items + q.lppView - 1) / q.lppView;
-    q.lagChunks = corr ? (fr.nt + kCorrLags - 1) / kCorrLags : 1;
+    viewLanes(vw.n, &q);
+    q.frameChunks = corr ? (fr.nt + kCorrLags - 1) / kCorrLags : fr.nt;
     if (dims == 2) {
         if (fp64) launchNeeT<2, double>(m, sc, il, vw, fr, vb, q, out, st, between);
         else launchNeeT<2, float>(m, sc, il, vw, fr, vb, q, out, st, between);
