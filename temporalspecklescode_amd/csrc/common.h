@@ -19,6 +19,7 @@
 namespace sstmc {
 
 constexpr int kMaxMaterials = 10;   // MATERIAL_NUM (Simulation.cuh:32)
+constexpr int kWalkPalette = 16;    // palette entries of the packed walk grid (4-bit cells)
 constexpr uint32_t kMaxAttempts = 1u << 20;  // first-vertex retry budget per path slot
 
 // ---------------------------------------------------------------- counter-based RNG
@@ -99,15 +100,20 @@ struct MediumView {
     int uniformAxis[3];            // 1 when the edges are equally spaced
     float e0[3], invH[3];          // first edge and inverse spacing for the fast locate
     float h[3];                    // spacing of an equally spaced axis
-    int allUniform;                // every axis equally spaced: the branch-free FP32 walk applies
-    // NEE walk grid (built per tracer): 4 x material id per cell (byte offset into a float sigt
-    // table), axes in the order that puts the view walks' dominant axis fastest; cell (i0, i1, i2)
-    // is walkGrid[walkPad + i0*walkStride[0] + i1*walkStride[1] + i2*walkStride[2]]. walkPad
-    // (a multiple of 4, >= one layer of the slowest axis) padding bytes precede and follow the
-    // cells, and the allocation is a whole number of 4-byte words.
-    const uint8_t* walkGrid;
-    uint32_t walkPad;
+    int allUniform;                // every axis equally spaced
+    // NEE walk grid (built per tracer), the material grid packed for the branch-free FP32 walk:
+    // each cell holds its index in a palette of the materials present (walkBits = 1, 2 or 4 bits),
+    // cells in the order that puts the view walks' dominant axis fastest. Cell (i0, i1, i2) starts
+    // at bit walkOrigin + i0*walkStride[0] + i1*walkStride[1] + i2*walkStride[2] (strides in bits)
+    // of the little-endian word array walkGrid; walkOrigin padding bits (whole words, at least one
+    // layer of the slowest axis) precede the cells and as many follow them.
+    const uint32_t* walkGrid;
+    uint32_t walkWords;            // 32-bit words of walkGrid
+    uint32_t walkOrigin;
     uint32_t walkStride[3];
+    uint32_t walkBits;
+    int walkFast;                  // the fast walk applies (equal spacing; bit offsets fit 32 bits)
+    float walkSig[kWalkPalette];   // sigt per palette index
 };
 
 // Scattering: Scattering.cuh:15-29, HG table :500-572.

@@ -236,15 +236,43 @@ __host__ __device__ inline R opticalDepth(const MediumView& m, const R* sig, con
     return tau;
 }
 
-// Byte (cell & 3) of a little-endian 4-byte word. On the device one bit-field extract whose
-// offset operand is taken mod 32, so (cell << 3) needs no mask.
-__host__ __device__ inline uint32_t byteOf(uint32_t word, uint32_t cell) {
+// The `bits`-bit field at bit (cell mod 32) of a little-endian word. On the device one bit-field
+// extract, whose offset operand is taken mod 32, so the walk's bit offset needs no mask.
+__host__ __device__ inline uint32_t cellField(uint32_t word, uint32_t cell, uint32_t bits) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __builtin_amdgcn_ubfe(word, cell << 3, 8);
+    return __builtin_amdgcn_ubfe(word, cell, bits);
 #else
-    return (word >> ((cell & 3u) << 3)) & 0xffu;
+    return (word >> (cell & 31u)) & ((1u << bits) - 1u);
 #endif
 }
+
+// Reads of the walk grid's 32-bit words by word index. On the device a buffer load with index
+// addressing (descriptor stride 4 bytes, range = the grid's words), so the word index (bit offset
+// >> 5) is the whole address computation; on the host a plain array read.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef int32_t sstmc_i32x4 __attribute__((ext_vector_type(4)));
+__device__ int32_t sstmc_struct_buffer_load_i32(sstmc_i32x4 rsrc, int32_t vindex, int32_t voffset, int32_t soffset,
+                                                int32_t aux) __asm("llvm.amdgcn.struct.buffer.load.i32");
+struct WalkWords {
+    sstmc_i32x4 rsrc;
+    __device__ explicit WalkWords(const MediumView& m) {
+        const uint64_t a = (uint64_t)m.walkGrid;
+        rsrc.x = (int32_t)(uint32_t)a;
+        rsrc.y = (int32_t)(((uint32_t)(a >> 32) & 0xffffu) | (4u << 16));   // base[47:32], stride 4
+        rsrc.z = (int32_t)m.walkWords;                                      // records
+        rsrc.w = 0x00020000;                                                // 32-bit data format
+    }
+    __device__ uint32_t operator()(uint32_t bit) const {
+        return (uint32_t)sstmc_struct_buffer_load_i32(rsrc, (int32_t)(bit >> 5), 0, 0, 0);
+    }
+};
+#else
+struct WalkWords {
+    const uint32_t* w;
+    explicit WalkWords(const MediumView& m) : w(m.walkGrid) {}
+    uint32_t operator()(uint32_t bit) const { return w[bit >> 5]; }
+};
+#endif
 
 // Branch-free FP32 walk for equally spaced grids (the common case). Per step: the nearest face
 // crossing is min3 of the three pending crossings; the axis is the highest one whose crossing
@@ -256,17 +284,18 @@ __host__ __device__ inline uint32_t byteOf(uint32_t word, uint32_t cell) {
 // the same expression as the crossings, so the exit test needs no per-axis counters and does not
 // wait for memory.
 //
-// Memory: the walk grid (m.walkGrid: one byte per cell, 4 x material id = the byte offset of the
-// material's sigt in the table `sig`, which is in LDS in the kernels; L2-resident; axes ordered
-// with the dominant walk direction fastest) is read one aligned 4-byte word at a time. Each walk
-// keeps the word holding its current cell and reads a new one only when its next cell lies in
-// another word, i.e. after a step along a slow axis or every fourth step along the fast one, so
-// the 64 walks of a wavefront issue few reads and touch few cache lines per step. The cell index
-// includes the grid's leading padding, so the cell one step past any face is still inside the
-// allocation and never below 0 (that word is read but not used).
+// Memory: the walk grid (m.walkGrid: each cell a 1-, 2- or 4-bit index into the palette of the
+// materials present, whose sigt are the table `sig`, in LDS in the kernels; axes ordered with the
+// dominant walk direction fastest) is read one aligned 4-byte word at a time; a 128^3 grid of four
+// materials packs into 512 KB. The walk's cell position is a bit offset; each walk keeps the word
+// holding its current cell and reads a new one only when its next cell lies in another word, i.e.
+// after a step along a slow axis or every 32 / bits steps along the fast one, so the 64 walks of a
+// wavefront issue few reads and touch few cache lines per step. The offset includes the grid's
+// leading padding, so the cell one step past any face is still inside the allocation and never
+// below 0 (that word is read but not used).
 //
 // The stages are software-pipelined one step apart: step j reads the word of cell j+1 (when it
-// changes), takes cell j's byte from its word and looks up its sigt, and adds segment(j-1) *
+// changes), takes cell j's field from its word and looks up its sigt, and adds segment(j-1) *
 // sigt(j-1); each read has a whole step of independent work to hide behind. The sum is formed in
 // the same order as a sequential walk and the tauCut test sees the same partial sums, so the
 // result is bit-identical to one.
@@ -277,8 +306,8 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
     const float pp[3] = {p.x, p.y, DIMS == 3 ? p.z : 0.f};
     const float dd[3] = {d.x, d.y, DIMS == 3 ? d.z : 0.f};
     float tf[3] = {3.0e38f, 3.0e38f, 3.0e38f}, td[3] = {0.f, 0.f, 0.f};
-    uint32_t stride[3] = {0u, 0u, 0u};   // walk-grid strides in two's complement
-    uint32_t cell = m.walkPad;
+    uint32_t stride[3] = {0u, 0u, 0u};   // walk-grid bit strides in two's complement
+    uint32_t cell = m.walkOrigin;        // bit offset of the current cell
     float tExit = 3.0e38f;
 #pragma unroll
     for (int a = 0; a < DIMS; ++a) {
@@ -293,8 +322,8 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
         stride[a] = up ? m.walkStride[a] : 0u - m.walkStride[a];
         cell += (uint32_t)b * m.walkStride[a];
     }
-    const uint32_t* w32 = (const uint32_t*)m.walkGrid;
-    const char* sigb = (const char*)sig;
+    const WalkWords words(m);
+    const uint32_t bits = m.walkBits;
     float t0 = tf[0], t1 = tf[1], t2 = tf[2];
     float k0 = 0.f, k1 = 0.f, k2 = 0.f;
     float tcur = 0.f, tau = 0.f;
@@ -311,9 +340,9 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
         const bool m0 = !m2 && !m1;                                                                 \
         CELLN = CELL + (m2 ? stride[2] : (m1 ? stride[1] : stride[0]));                             \
         WORDN = WORD;                                                                               \
-        if ((CELLN ^ CELL) > 3u) {                                                                  \
+        if ((CELLN ^ CELL) > 31u) {                                                                 \
             SSTMC_WALK_INDEX_CHECK(m, CELLN);                                                       \
-            WORDN = w32[CELLN >> 2];                                                                \
+            WORDN = words(CELLN);                                                                   \
         }                                                                                           \
         SEGO = tn - tcur;                                                                           \
         tcur = tn;                                                                                  \
@@ -323,13 +352,13 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
         t0 = fmaf(k0, td[0], tf[0]);                                                                \
         t1 = fmaf(k1, td[1], tf[1]);                                                                \
         if (DIMS == 3) t2 = fmaf(k2, td[2], tf[2]);                                                 \
-        SIGO = *(const float*)(sigb + byteOf(WORD, CELL));                                          \
+        SIGO = sig[cellField(WORD, CELL, bits)];                                                    \
         tau = fmaf(SEGI, SIGI, tau);  /* previous cell */                                           \
         DONE = (tau > tauCut) | (tn >= tExit);                                                      \
     }
     // two steps per iteration with the pipeline registers swapping roles (no register copies)
     SSTMC_WALK_INDEX_CHECK(m, cell);
-    uint32_t ca = cell, cb, wa = w32[cell >> 2], wb;
+    uint32_t ca = cell, cb, wa = words(cell), wb;
     float sga = 0.f, sia = 0.f, sgb, sib;       // nothing before the first cell: adds exactly 0
     float lastSeg = 0.f, lastSig = 0.f;
     bool done;

@@ -85,24 +85,24 @@ __device__ inline R phaseAmp(const ScatterView& s, uint32_t m, double c) {
     return sqrtR<R>(phasePdf<DIMS, R>(s, m, c));
 }
 
-// Voxel walk dispatch: the branch-free FP32 walk on equally spaced grids, the general walk
-// (edge tables, FP64 when R is double) otherwise. `sig` is the block's LDS copy of the
-// per-material sigt table.
+// Voxel walk dispatch: the branch-free FP32 walk over the packed walk grid (equally spaced
+// grids), the general walk (edge tables, FP64 when R is double) otherwise. `sig` is the block's
+// LDS table: per-material sigt, then (FP32) the walk grid's palette sigt.
 template <int DIMS, typename R, typename V>
 __device__ inline R walk(const MediumView& m, const R* sig, const V& p, const V& d, R tauCut, uint32_t* steps) {
     if constexpr (sizeof(R) == 4) {
-        if (m.allUniform) return opticalDepthFast<DIMS>(m, sig, p, d, tauCut, steps);
+        if (m.walkFast) return opticalDepthFast<DIMS>(m, sig + kMaxMaterials, p, d, tauCut, steps);
     }
     return opticalDepth<DIMS, R>(m, sig, p, d, tauCut, steps);
 }
 
-// Per-block LDS copy of the per-material sigt table in R. Every thread of the block must call it.
+// Per-block LDS sigt table in R: kMaxMaterials per-material entries, then kWalkPalette entries
+// of the walk grid's palette. Every thread of the block must call it.
+constexpr int kSigLds = kMaxMaterials + kWalkPalette;
 template <typename R>
 __device__ inline const R* loadSigTable(const MediumView& m, R* lds) {
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < kMaxMaterials; ++i) lds[i] = Real<R>::sigt(m, i);
-    }
+    if (threadIdx.x < kMaxMaterials) lds[threadIdx.x] = Real<R>::sigt(m, threadIdx.x);
+    else if (threadIdx.x < kSigLds) lds[threadIdx.x] = (R)m.walkSig[threadIdx.x - kMaxMaterials];
     __syncthreads();
     return lds;
 }
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
     // when L and V are small)
     const uint32_t lane = threadIdx.x & 63, lpp = prm.lpp, sub = lane & (lpp - 1);
     const uint32_t p = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / lpp) + lane / lpp;
-    __shared__ R sigLds[kMaxMaterials];
+    __shared__ R sigLds[kSigLds];
     const R* sig = loadSigTable<R>(m, sigLds);
     uint32_t steps = 0;
     if (p < prm.npaths) nee_path<DIMS, R>(m, sig, sc, il, vw, fr, vb, prm, out, p, sub, lpp, &steps);
@@ -508,7 +508,7 @@ __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 4 : (SRC != kSrcPoint ? 8 : 6))))
 k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
-    __shared__ R sigLds[kMaxMaterials];
+    __shared__ R sigLds[kSigLds];
     const R* sig = loadSigTable<R>(m, sigLds);
     const ViewUnit u = viewUnit<WAVE>(prm);
     const uint32_t p = u.p, t = u.chunk, P = prm.npaths, V = vw.n;
@@ -548,7 +548,7 @@ template <int DIMS, typename R, int SRC, bool WAVE>
 __global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView sc, SourceView vw, FramesView fr,
                                                        VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
-    __shared__ R sigLds[kMaxMaterials];
+    __shared__ R sigLds[kSigLds];
     __shared__ C acc[kCorrLags][256];
     const R* sig = loadSigTable<R>(m, sigLds);
     const ViewUnit u = viewUnit<WAVE>(prm);

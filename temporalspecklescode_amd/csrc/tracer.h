@@ -213,7 +213,7 @@ private:
 
     const Medium& medium_;
     MediumView mview_{};            // the medium's view plus this tracer's walk grid
-    DevBuf<uint8_t> walkGrid_;
+    DevBuf<uint32_t> walkGrid_;
     const Scattering& sc_;
     const Source& il_;
     const Source& vw_;

@@ -504,17 +504,19 @@ uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint
     uint64_t field = 16ull * T * L * V;
     uint64_t pool = (uint64_t)P * 32 * (20 + 32ull * T + (frames ? 12ull * T : 0));
     uint64_t ws = 2ull * 8 * T * L * V * elem;   // split-K partials, two workspaces
-    uint64_t grids = 3 * cells;                  // material grid + padded walk grid
+    uint64_t grids = 2 * cells;                  // material grid + packed walk grid (<= 4 bits a cell)
     return terms + field + pool + ws + grids + 24ull * P + (1ull << 20);
 }
 
-// The NEE walk grid: 4 x material id per cell (the byte offset of the material's sigt in the
-// kernels' float table), with the axes ordered so that the axis the view walks mostly run along
-// is fastest. A walk then meets its next cell in the same 4-byte word for runs of cells along
-// that axis and re-reads memory only when it leaves the word; the 64 walks of a wavefront, which
-// fan out from one vertex, then issue few reads and touch few cache lines per step. Padding of
-// one slowest-axis layer (rounded up to a word) on both sides keeps the cell one step past any
-// face inside the allocation; the total is a whole number of words.
+// The NEE walk grid: per cell the index of its material in the palette of the materials present,
+// packed 1, 2 or 4 bits a cell (the palette's sigt go to the kernels' LDS table), with the axes
+// ordered so that the axis the view walks mostly run along is fastest. A walk then meets its next
+// cell in the same 4-byte word for runs of up to 32 / bits cells along that axis and re-reads
+// memory only when it leaves the word; the 64 walks of a wavefront, which fan out from one
+// vertex, then issue few reads and touch few cache lines per step. Padding of one slowest-axis
+// layer (rounded up to a word) on both sides keeps the cell one step past any face inside the
+// allocation. Grids whose bit offsets do not fit 32 bits, or with unequal spacing, use the
+// general walk.
 int NextEventEstimation::buildWalkGrid() {
     mview_ = medium_.view();
     const uint32_t* n = mview_.n;
@@ -542,24 +544,41 @@ int NextEventEstimation::buildWalkGrid() {
         S[order[j]] = order[j] < dims_ ? mul : 0u;
         if (order[j] < dims_) mul *= n[order[j]];
     }
+    // palette of the materials present, and the narrowest field that indexes it
     const size_t ncell = (size_t)n[0] * n[1] * n[2];
-    const int slow = dims_ == 3 ? order[2] : order[1];
-    const size_t pad = ((size_t)S[slow] + 3) & ~(size_t)3;
-    const size_t total = (pad + ncell + pad + 3) & ~(size_t)3;
-    if (total >= 0xffffffffull) return SSTMC_INVALID_CONFIG;
-    std::vector<uint8_t> wg(total, 0);
     const std::vector<uint8_t>& g = medium_.hostGrid();
+    bool present[256] = {};
+    for (size_t i = 0; i < ncell; ++i) present[g[i]] = true;
+    uint32_t index[256] = {}, npal = 0;
+    for (int id = 0; id < 256; ++id)
+        if (present[id]) {
+            if (id >= kMaxMaterials || npal >= (uint32_t)kWalkPalette) return SSTMC_INVALID_CONFIG;
+            index[id] = npal;
+            mview_.walkSig[npal++] = mview_.sigtf[id];
+        }
+    const uint32_t bits = npal <= 2 ? 1u : (npal <= 4 ? 2u : 4u);
+    // bit offsets: one layer of the slowest axis of padding (whole words) before and after
+    const int slow = dims_ == 3 ? order[2] : order[1];
+    const uint64_t pad = ((uint64_t)S[slow] * bits + 31) & ~31ull;
+    const uint64_t total = (pad + (uint64_t)ncell * bits + pad + 31) & ~31ull;
+    mview_.walkFast = 0;
+    if (!mview_.allUniform || total >= 0xffffffffull) return SSTMC_NO_ERROR;   // the general walk
+    std::vector<uint32_t> wg(total / 32, 0u);
     for (uint32_t i2 = 0; i2 < n[2]; ++i2)
         for (uint32_t i1 = 0; i1 < n[1]; ++i1)
-            for (uint32_t i0 = 0; i0 < n[0]; ++i0)
-                wg[pad + (size_t)i0 * S[0] + (size_t)i1 * S[1] + (size_t)i2 * S[2]] =
-                    (uint8_t)(4 * g[i0 + (size_t)n[0] * (i1 + (size_t)n[1] * i2)]);
+            for (uint32_t i0 = 0; i0 < n[0]; ++i0) {
+                const uint64_t bit = pad + ((uint64_t)i0 * S[0] + (uint64_t)i1 * S[1] + (uint64_t)i2 * S[2]) * bits;
+                wg[bit >> 5] |= index[g[i0 + (size_t)n[0] * (i1 + (size_t)n[1] * i2)]] << (bit & 31);
+            }
     int e = walkGrid_.upload(wg.data(), wg.size());
     if (e) return e;
     mview_.walkGrid = walkGrid_.p;
-    mview_.walkPad = (uint32_t)pad;
-    for (int a = 0; a < 3; ++a) mview_.walkStride[a] = S[a];
-    bytes_ += total;
+    mview_.walkWords = (uint32_t)(total / 32);
+    mview_.walkOrigin = (uint32_t)pad;
+    mview_.walkBits = bits;
+    for (int a = 0; a < 3; ++a) mview_.walkStride[a] = S[a] * bits;
+    mview_.walkFast = 1;
+    bytes_ += total / 8;
     return SSTMC_NO_ERROR;
 }
 

@@ -1,6 +1,6 @@
 // Host-side check of the FP32 fast voxel walk (opticalDepthFast) on random grids and rays:
 //   * every 4-byte walk-grid word it reads lies inside the padded grid (SSTMC_WALK_INDEX_CHECK),
-//     for walk grids with each axis as the fastest one;
+//     for walk grids with each axis as the fastest one and 1-, 2- and 4-bit cells;
 //   * tau agrees with the general FP32 walk (opticalDepth<3, float>, per-crossing parameters from
 //     the fixed origin) to 1e-5 relative, and with the FP64 general walk to 2e-3 absolute;
 //   * the tau cut: either both walks stop past the cut or both agree as above.
@@ -13,11 +13,11 @@
 #include <random>
 #include <vector>
 
-static size_t g_walk_bytes = 0;   // bytes of the padded walk grid under test
+static size_t g_walk_words = 0;   // words of the padded walk grid under test
 static long g_bad_index = 0;
 #define SSTMC_WALK_INDEX_CHECK(m, cell)                                                         \
     do {                                                                                        \
-        if (((size_t)(cell) & ~(size_t)3) + 4 > g_walk_bytes) ++g_bad_index;                    \
+        if ((size_t)((cell) >> 5) >= g_walk_words) ++g_bad_index;                               \
     } while (0)
 
 #include "../../temporalspecklescode_amd/csrc/common.h"
@@ -25,10 +25,11 @@ static long g_bad_index = 0;
 
 using namespace sstmc;
 
-// Walk grid with axis `fast` fastest, the other axes in increasing order, one slowest-axis layer
-// of padding (rounded up to a word) on both sides, whole words in total.
-static std::vector<uint8_t> walkGrid(const std::vector<uint8_t>& grid, const uint32_t n[3], int dims, int fast,
-                                     uint32_t S[3], uint32_t* pad) {
+// Packed walk grid with axis `fast` fastest, the other axes in increasing order: palette index
+// of each cell (palette = material ids 1..nmat, `bits` bits a cell), one slowest-axis layer of
+// padding (rounded up to a word) on both sides. Strides and the origin in bits.
+static std::vector<uint32_t> walkGrid(const std::vector<uint8_t>& grid, const uint32_t n[3], int dims, int fast,
+                                      uint32_t bits, uint32_t S[3], uint32_t* origin) {
     int order[3] = {fast, -1, -1}, k = 1;
     for (int a = 0; a < 3; ++a)
         if (a != fast) order[k++] = a;
@@ -38,19 +39,21 @@ static std::vector<uint8_t> walkGrid(const std::vector<uint8_t>& grid, const uin
         if (order[j] < dims) mul *= n[order[j]];
     }
     const size_t ncell = (size_t)n[0] * n[1] * n[2];
-    const size_t p = ((size_t)S[dims == 3 ? order[2] : order[1]] + 3) & ~(size_t)3;
-    std::vector<uint8_t> wg((p + ncell + p + 3) & ~(size_t)3, 0);
+    const size_t p = ((size_t)S[dims == 3 ? order[2] : order[1]] * bits + 31) & ~(size_t)31;
+    std::vector<uint32_t> wg(((p + ncell * bits + p + 31) & ~(size_t)31) / 32, 0u);
     for (uint32_t i2 = 0; i2 < n[2]; ++i2)
         for (uint32_t i1 = 0; i1 < n[1]; ++i1)
-            for (uint32_t i0 = 0; i0 < n[0]; ++i0)
-                wg[p + (size_t)i0 * S[0] + (size_t)i1 * S[1] + (size_t)i2 * S[2]] =
-                    (uint8_t)(4 * grid[i0 + (size_t)n[0] * (i1 + (size_t)n[1] * i2)]);
-    *pad = (uint32_t)p;
+            for (uint32_t i0 = 0; i0 < n[0]; ++i0) {
+                const size_t bit = p + ((size_t)i0 * S[0] + (size_t)i1 * S[1] + (size_t)i2 * S[2]) * bits;
+                wg[bit >> 5] |= (uint32_t)(grid[i0 + (size_t)n[0] * (i1 + (size_t)n[1] * i2)] - 1) << (bit & 31);
+            }
+    for (int a = 0; a < 3; ++a) S[a] *= bits;
+    *origin = (uint32_t)p;
     return wg;
 }
 
 template <int DIMS>
-static int run(int n, double h, int trials, unsigned seed, int fast) {
+static int run(int n, double h, int trials, unsigned seed, int fast, int nmat) {
     std::mt19937 rng(seed);
     std::vector<float> ef(n + 1);
     std::vector<double> ed(n + 1);
@@ -58,12 +61,13 @@ static int run(int n, double h, int trials, unsigned seed, int fast) {
     const int nz = DIMS == 3 ? n : 1;
     const size_t nc = (size_t)n * n * nz;
     std::vector<uint8_t> grid(nc);
-    for (auto& g : grid) g = (uint8_t)(1 + rng() % 4);
+    for (auto& g : grid) g = (uint8_t)(1 + rng() % nmat);
     const uint32_t nn[3] = {(uint32_t)n, (uint32_t)n, (uint32_t)nz};
-    uint32_t S[3], pad;
-    std::vector<uint8_t> wg = walkGrid(grid, nn, DIMS, fast, S, &pad);
-    g_walk_bytes = wg.size();
-    const double sig[kMaxMaterials] = {0, 0.21, 1.01, 3.02, 5.05};
+    const uint32_t bits = nmat <= 2 ? 1u : (nmat <= 4 ? 2u : 4u);
+    uint32_t S[3], origin;
+    std::vector<uint32_t> wg = walkGrid(grid, nn, DIMS, fast, bits, S, &origin);
+    g_walk_words = wg.size();
+    const double sig[kMaxMaterials] = {0, 0.21, 1.01, 3.02, 5.05, 0.5, 2.2, 0.05, 7.5, 1.7};
     MediumView m{};
     m.dims = DIMS;
     for (int a = 0; a < 3; ++a) {
@@ -73,10 +77,12 @@ static int run(int n, double h, int trials, unsigned seed, int fast) {
         m.bminf[a] = ef[0]; m.bmaxf[a] = ef[n]; m.bmind[a] = ed[0]; m.bmaxd[a] = ed[n];
         m.uniformAxis[a] = 1; m.e0[a] = ef[0]; m.invH[a] = (float)(1.0 / h); m.h[a] = (float)h;
     }
-    m.allUniform = 1; m.grid = grid.data(); m.walkGrid = wg.data(); m.walkPad = pad;
+    m.allUniform = 1; m.grid = grid.data(); m.walkGrid = wg.data(); m.walkWords = (uint32_t)wg.size();
+    m.walkOrigin = origin; m.walkBits = bits;
+    m.walkFast = 1;
     for (int a = 0; a < 3; ++a) m.walkStride[a] = S[a];
-    float sigf[kMaxMaterials];
-    for (int i = 0; i < kMaxMaterials; ++i) { m.sigtf[i] = (float)sig[i]; m.sigtd[i] = sig[i]; sigf[i] = (float)sig[i]; }
+    for (int i = 0; i < kMaxMaterials; ++i) { m.sigtf[i] = (float)sig[i]; m.sigtd[i] = sig[i]; }
+    for (int i = 0; i < nmat; ++i) m.walkSig[i] = (float)sig[i + 1];   // palette index i = material i + 1
     std::uniform_real_distribution<double> U(ed[0], ed[n]), N(-1, 1);
     long bad = 0;
     double worstRel = 0, worstAbs64 = 0;
@@ -93,7 +99,7 @@ static int run(int n, double h, int trials, unsigned seed, int fast) {
         const d3 pd = mkd(pf.x, pf.y, pf.z), dd = mkd(df.x, df.y, df.z);
         const float cut = (t % 5 == 0) ? 7.5f : 3.0e38f;
         uint32_t s0 = 0, s1 = 0, s2 = 0;
-        const float fast = opticalDepthFast<DIMS>(m, sigf, pf, df, cut, &s0);
+        const float fast = opticalDepthFast<DIMS>(m, m.walkSig, pf, df, cut, &s0);
         const float slow = opticalDepth<DIMS, float>(m, m.sigtf, pf, df, cut, &s1);
         const double ref = opticalDepth<DIMS, double>(m, m.sigtd, pd, dd, 1e300, &s2);
         if (cut < 1e30f && (fast > cut || slow > cut)) {
@@ -109,21 +115,24 @@ static int run(int n, double h, int trials, unsigned seed, int fast) {
             ++bad;
         }
     }
-    printf("DIMS %d n %d h %g fastest axis %d: %d rays, %ld mismatches, %ld out-of-range words, max rel vs FP32 "
-           "general %.2e, max abs vs FP64 %.2e\n", DIMS, n, h, fast, trials, bad, g_bad_index, worstRel, worstAbs64);
+    printf("DIMS %d n %d h %g fastest axis %d, %d materials (%u-bit cells): %d rays, %ld mismatches, %ld "
+           "out-of-range words, max rel vs FP32 general %.2e, max abs vs FP64 %.2e\n", DIMS, n, h, fast, nmat, bits,
+           trials, bad, g_bad_index, worstRel, worstAbs64);
     return bad != 0 || g_bad_index != 0;
 }
 
 int main() {
     int fail = 0;
+    const int mats[3] = {2, 4, 9};   // 1-, 2- and 4-bit cells
     for (int fast = 0; fast < 3; ++fast) {
-        fail |= run<3>(32, 1.0, 40000, 1 + fast, fast);
-        fail |= run<3>(7, 0.37, 40000, 4 + fast, fast);
-        fail |= run<3>(9, 0.1, 20000, 7 + fast, fast);
+        fail |= run<3>(32, 1.0, 40000, 1 + fast, fast, mats[fast]);
+        fail |= run<3>(7, 0.37, 40000, 4 + fast, fast, mats[(fast + 1) % 3]);
+        fail |= run<3>(9, 0.1, 20000, 7 + fast, fast, mats[(fast + 2) % 3]);
     }
     for (int fast = 0; fast < 2; ++fast) {
-        fail |= run<2>(64, 1.0, 40000, 10 + fast, fast);
-        fail |= run<2>(5, 0.25, 40000, 12 + fast, fast);
+        fail |= run<2>(64, 1.0, 40000, 10 + fast, fast, mats[fast]);
+        fail |= run<2>(5, 0.25, 40000, 12 + fast, fast, mats[fast + 1]);
+        fail |= run<2>(33, 0.5, 20000, 14 + fast, fast, mats[2 - fast]);
     }
     printf(fail ? "FAIL\n" : "OK\n");
     return fail;
