@@ -19,7 +19,7 @@
 namespace sstmc {
 
 constexpr int kMaxMaterials = 10;   // MATERIAL_NUM (Simulation.cuh:32)
-constexpr int kWalkPalette = 16;    // palette entries of the packed walk grid (4-bit cells)
+constexpr int kWalkPalette = 16;    // palette entries of the packed walk grid
 constexpr uint32_t kMaxAttempts = 1u << 20;  // first-vertex retry budget per path slot
 
 // ---------------------------------------------------------------- counter-based RNG
@@ -102,7 +102,8 @@ struct MediumView {
     float h[3];                    // spacing of an equally spaced axis
     int allUniform;                // every axis equally spaced
     // NEE walk grid (built per tracer), the material grid packed for the branch-free FP32 walk:
-    // each cell holds its index in a palette of the materials present (walkBits = 1, 2 or 4 bits),
+    // each cell holds 4 x its index in a palette of the materials present, i.e. the byte offset of
+    // its sigt in walkSig (walkBits = 1 bit for one material, 4 bits for up to four, else 8),
     // cells in the order that puts the view walks' dominant axis fastest. Cell (i0, i1, i2) starts
     // at bit walkOrigin + i0*walkStride[0] + i1*walkStride[1] + i2*walkStride[2] (strides in bits)
     // of the little-endian word array walkGrid; walkOrigin padding bits (whole words, at least one

@@ -284,10 +284,11 @@ struct WalkWords {
 // the same expression as the crossings, so the exit test needs no per-axis counters and does not
 // wait for memory.
 //
-// Memory: the walk grid (m.walkGrid: each cell a 1-, 2- or 4-bit index into the palette of the
-// materials present, whose sigt are the table `sig`, in LDS in the kernels; axes ordered with the
-// dominant walk direction fastest) is read one aligned 4-byte word at a time; a 128^3 grid of four
-// materials packs into 512 KB. The walk's cell position is a bit offset; each walk keeps the word
+// Memory: the walk grid (m.walkGrid: each cell a 1-, 4- or 8-bit field holding 4 x its index in
+// the palette of the materials present, i.e. the byte offset of its sigt in the table `sig`, in
+// LDS in the kernels; axes ordered with the dominant walk direction fastest) is read one aligned
+// 4-byte word at a time; a 128^3 grid of four materials packs into 1 MB. The walk's cell position
+// is a bit offset; each walk keeps the word
 // holding its current cell and reads a new one only when its next cell lies in another word, i.e.
 // after a step along a slow axis or every 32 / bits steps along the fast one, so the 64 walks of a
 // wavefront issue few reads and touch few cache lines per step. The offset includes the grid's
@@ -324,6 +325,7 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
     }
     const WalkWords words(m);
     const uint32_t bits = m.walkBits;
+    const char* sigb = (const char*)sig;
     float t0 = tf[0], t1 = tf[1], t2 = tf[2];
     float k0 = 0.f, k1 = 0.f, k2 = 0.f;
     float tcur = 0.f, tau = 0.f;
@@ -352,7 +354,7 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
         t0 = fmaf(k0, td[0], tf[0]);                                                                \
         t1 = fmaf(k1, td[1], tf[1]);                                                                \
         if (DIMS == 3) t2 = fmaf(k2, td[2], tf[2]);                                                 \
-        SIGO = sig[cellField(WORD, CELL, bits)];                                                    \
+        SIGO = *(const float*)(sigb + cellField(WORD, CELL, bits));                                 \
         tau = fmaf(SEGI, SIGI, tau);  /* previous cell */                                           \
         DONE = (tau > tauCut) | (tn >= tExit);                                                      \
     }

@@ -508,8 +508,9 @@ uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint
     return terms + field + pool + ws + grids + 24ull * P + (1ull << 20);
 }
 
-// The NEE walk grid: per cell the index of its material in the palette of the materials present,
-// packed 1, 2 or 4 bits a cell (the palette's sigt go to the kernels' LDS table), with the axes
+// The NEE walk grid: per cell 4 x the index of its material in the palette of the materials
+// present (the byte offset of its sigt in the kernels' LDS table), packed 1 bit a cell for one
+// material, 4 bits for up to four and 8 bits for more, with the axes
 // ordered so that the axis the view walks mostly run along is fastest. A walk then meets its next
 // cell in the same 4-byte word for runs of up to 32 / bits cells along that axis and re-reads
 // memory only when it leaves the word; the 64 walks of a wavefront, which fan out from one
@@ -556,7 +557,8 @@ int NextEventEstimation::buildWalkGrid() {
             index[id] = npal;
             mview_.walkSig[npal++] = mview_.sigtf[id];
         }
-    const uint32_t bits = npal <= 2 ? 1u : (npal <= 4 ? 2u : 4u);
+    // fields hold 4 x palette index (the sigt byte offset): 0 fits one bit, 0..12 four, 0..60 eight
+    const uint32_t bits = npal <= 1 ? 1u : (npal <= 4 ? 4u : 8u);
     // bit offsets: one layer of the slowest axis of padding (whole words) before and after
     const int slow = dims_ == 3 ? order[2] : order[1];
     const uint64_t pad = ((uint64_t)S[slow] * bits + 31) & ~31ull;
@@ -568,7 +570,7 @@ int NextEventEstimation::buildWalkGrid() {
         for (uint32_t i1 = 0; i1 < n[1]; ++i1)
             for (uint32_t i0 = 0; i0 < n[0]; ++i0) {
                 const uint64_t bit = pad + ((uint64_t)i0 * S[0] + (uint64_t)i1 * S[1] + (uint64_t)i2 * S[2]) * bits;
-                wg[bit >> 5] |= index[g[i0 + (size_t)n[0] * (i1 + (size_t)n[1] * i2)]] << (bit & 31);
+                wg[bit >> 5] |= (4u * index[g[i0 + (size_t)n[0] * (i1 + (size_t)n[1] * i2)]]) << (bit & 31);
             }
     int e = walkGrid_.upload(wg.data(), wg.size());
     if (e) return e;

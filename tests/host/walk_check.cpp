@@ -1,6 +1,6 @@
 // Host-side check of the FP32 fast voxel walk (opticalDepthFast) on random grids and rays:
 //   * every 4-byte walk-grid word it reads lies inside the padded grid (SSTMC_WALK_INDEX_CHECK),
-//     for walk grids with each axis as the fastest one and 1-, 2- and 4-bit cells;
+//     for walk grids with each axis as the fastest one and 1-, 4- and 8-bit cells;
 //   * tau agrees with the general FP32 walk (opticalDepth<3, float>, per-crossing parameters from
 //     the fixed origin) to 1e-5 relative, and with the FP64 general walk to 2e-3 absolute;
 //   * the tau cut: either both walks stop past the cut or both agree as above.
@@ -25,8 +25,8 @@ static long g_bad_index = 0;
 
 using namespace sstmc;
 
-// Packed walk grid with axis `fast` fastest, the other axes in increasing order: palette index
-// of each cell (palette = material ids 1..nmat, `bits` bits a cell), one slowest-axis layer of
+// Packed walk grid with axis `fast` fastest, the other axes in increasing order: 4 x palette
+// index of each cell (palette = material ids 1..nmat, `bits` bits a cell), one slowest-axis layer of
 // padding (rounded up to a word) on both sides. Strides and the origin in bits.
 static std::vector<uint32_t> walkGrid(const std::vector<uint8_t>& grid, const uint32_t n[3], int dims, int fast,
                                       uint32_t bits, uint32_t S[3], uint32_t* origin) {
@@ -45,7 +45,7 @@ static std::vector<uint32_t> walkGrid(const std::vector<uint8_t>& grid, const ui
         for (uint32_t i1 = 0; i1 < n[1]; ++i1)
             for (uint32_t i0 = 0; i0 < n[0]; ++i0) {
                 const size_t bit = p + ((size_t)i0 * S[0] + (size_t)i1 * S[1] + (size_t)i2 * S[2]) * bits;
-                wg[bit >> 5] |= (uint32_t)(grid[i0 + (size_t)n[0] * (i1 + (size_t)n[1] * i2)] - 1) << (bit & 31);
+                wg[bit >> 5] |= 4u * (uint32_t)(grid[i0 + (size_t)n[0] * (i1 + (size_t)n[1] * i2)] - 1) << (bit & 31);
             }
     for (int a = 0; a < 3; ++a) S[a] *= bits;
     *origin = (uint32_t)p;
@@ -63,7 +63,7 @@ static int run(int n, double h, int trials, unsigned seed, int fast, int nmat) {
     std::vector<uint8_t> grid(nc);
     for (auto& g : grid) g = (uint8_t)(1 + rng() % nmat);
     const uint32_t nn[3] = {(uint32_t)n, (uint32_t)n, (uint32_t)nz};
-    const uint32_t bits = nmat <= 2 ? 1u : (nmat <= 4 ? 2u : 4u);
+    const uint32_t bits = nmat <= 1 ? 1u : (nmat <= 4 ? 4u : 8u);
     uint32_t S[3], origin;
     std::vector<uint32_t> wg = walkGrid(grid, nn, DIMS, fast, bits, S, &origin);
     g_walk_words = wg.size();
@@ -123,7 +123,7 @@ static int run(int n, double h, int trials, unsigned seed, int fast, int nmat) {
 
 int main() {
     int fail = 0;
-    const int mats[3] = {2, 4, 9};   // 1-, 2- and 4-bit cells
+    const int mats[3] = {1, 4, 9};   // 1-, 4- and 8-bit cells
     for (int fast = 0; fast < 3; ++fast) {
         fail |= run<3>(32, 1.0, 40000, 1 + fast, fast, mats[fast]);
         fail |= run<3>(7, 0.37, 40000, 4 + fast, fast, mats[(fast + 1) % 3]);
