@@ -1,7 +1,8 @@
 // sstmc_oracle.cpp -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
 //
 // Scalar FP64 restatement of the reference's next-event-estimation field tracer
-// (TemporalSampler + NextEventEstimation, non-CBS, one wavelength), used only by tests/,
+// (TemporalSampler + NextEventEstimation, one wavelength; coherent backscattering for field
+// output), used only by tests/,
 // __graft_entry__.smoke() and the cpu_baseline leg of bench.py as the checker and the CPU
 // baseline. It shares nothing with the HIP product except the config struct declared in
 // include/sstmc.h (the boundary format).
