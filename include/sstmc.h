@@ -54,6 +54,7 @@ enum {
 /* ---------------------------------------------------------------- enums */
 enum { SSTMC_MEDIUM_HETEROGENEOUS = 1 };                 /* config.medium.type */
 enum { SSTMC_SCATTERING_ISOTROPIC = 1,                    /* config.scattering.type{m} */
+       SSTMC_SCATTERING_TABULAR = 2,
        SSTMC_SCATTERING_HG = 3 };                         /* values follow ScatteringNS::ScatteringType */
 enum { SSTMC_SOURCE_NONE = 0,                             /* config.illumination/view.type */
        SSTMC_SOURCE_POINT = 1,
@@ -95,8 +96,14 @@ typedef struct sstmc_medium {
 /* config.scattering (MatlabInterface.cu:712-888): one entry per material. */
 typedef struct sstmc_scattering {
     uint32_t materialNum;
-    const int32_t* type;           /* SSTMC_SCATTERING_ISOTROPIC or SSTMC_SCATTERING_HG */
-    const double* g;               /* HG anisotropy per material (ignored for isotropic) */
+    const int32_t* type;           /* SSTMC_SCATTERING_ISOTROPIC, _TABULAR or _HG */
+    const double* g;               /* HG anisotropy per material (ignored otherwise) */
+    /* Tabular amplitude functions (Scattering.cuh:272-429): tabularSize[m] complex samples of
+     * the amplitude over the scattering angle, equally spaced on [0, pi] (3D) or [0, 2 pi] (2D),
+     * for each material m with tabularSize[m] > 0, concatenated in material order as interleaved
+     * (re, im) doubles. NULL when no material is tabular. */
+    const uint32_t* tabularSize;
+    const double* tabular;
 } sstmc_scattering;
 
 /* config.illumination / config.view (MatlabInterface.cu:1062-1215). */

@@ -237,25 +237,70 @@ struct Medium {
 };
 
 // ------------------------------------------------------------------ scattering
-// Scattering.cuh: isotropic and Henyey-Greenstein (setup :500-572, eval :895-1110).
+// Scattering.cuh: isotropic, tabular and Henyey-Greenstein (setup :272-429, :500-572; eval
+// :895-1110).
 struct Scatter {
     int dims;
     std::vector<int> type;  // per material
     std::vector<double> g;
     bool allIsotropic;      // Scattering::isIsotropicScattering() (Scattering.cuh:1505-1527)
+    // tabular: amplitude normalised by the cross section, evaluation pdf and inverse cdf angle
+    struct Table {
+        std::vector<cplx> f;
+        std::vector<double> pdf, icdf;
+    };
+    std::vector<Table> tab;
+
+    // setTabularScattering (Scattering.cuh:272-429): n complex samples over [0, pi] (3D) or
+    // [0, 2 pi] (2D). getSamplePdfRaw (:208-222): |f|^2, times sin(theta) in 3D; the sum
+    // normalises the sampling pdf (:344-348); the cross section (sum / n) 2 pi (times pi in 3D)
+    // normalises f (:351-358); the inverse cdf is lower_bound(cdf, i / (n-1)) scaled to the
+    // angle range (:361-373); the evaluation pdf is getSamplePdfRaw of the normalised f (:377).
+    void setTabular(uint32_t m, const double* fin, uint32_t n) {
+        Table& t = tab[m];
+        std::vector<double> raw(n), cdf(n);
+        double sum = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            double th = ((double)i / (double)(n - 1)) * PI;
+            raw[i] = (fin[2 * i] * fin[2 * i] + fin[2 * i + 1] * fin[2 * i + 1]) * (dims == 2 ? 1.0 : std::sin(th));
+            sum += raw[i];
+        }
+        double acc = 0;
+        for (uint32_t i = 0; i < n; ++i) { acc += raw[i] / sum; cdf[i] = acc; }
+        double cross = (sum / (double)n) * 2.0 * PI * (dims == 2 ? 1.0 : PI);
+        double k = std::sqrt(1.0 / cross), range = dims == 2 ? 2.0 * PI : PI;
+        t.f.resize(n); t.pdf.resize(n); t.icdf.resize(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            double th = ((double)i / (double)(n - 1)) * PI;
+            t.f[i] = cplx(fin[2 * i] * k, fin[2 * i + 1] * k);
+            t.pdf[i] = std::norm(t.f[i]) * (dims == 2 ? 1.0 : std::sin(th));
+            size_t lb = (size_t)(std::lower_bound(cdf.begin(), cdf.end(), (double)i / (double)(n - 1)) - cdf.begin());
+            t.icdf[i] = (double)lb * (range / (double)(n - 1));
+        }
+    }
+    // entry of cosine c: round(safeAcos(c) (n-1) / range) half to even, clamped (Scattering.cuh:
+    // 908-915; safeAcos, round_num Simulation.cuh:274-297)
+    size_t tabEntry(uint32_t m, double c) const {
+        size_t n = tab[m].f.size();
+        double a = c < -1.0 + EPS ? PI : (c > 1.0 - EPS ? 0.0 : std::acos(c));
+        double r = std::nearbyint(a * (double)(n - 1) / (dims == 2 ? 2.0 * PI : PI));
+        return r < 0 ? 0 : (r >= (double)n ? n - 1 : (size_t)r);
+    }
 
     double isoAmp() const { return dims == 3 ? 0.282094791773878143 : 0.398942280401432678; }
     double isoPdf() const { return dims == 3 ? 0.079577471545947668 : 0.159154943091895336; }
     double pdf(uint32_t m, double c) const {
         if (type[m] == SSTMC_SCATTERING_ISOTROPIC) return isoPdf();
+        if (type[m] == SSTMC_SCATTERING_TABULAR) return tab[m].pdf[tabEntry(m, c)];
         double gg = g[m];
         double up = dims == 3 ? (1.0 - gg * gg) / (4.0 * PI) : 2.0 * ((1.0 - gg * gg) / (4.0 * PI));
         double den = std::fma(-2.0 * gg, c, 1.0 + gg * gg);
         return dims == 3 ? up / (den * std::sqrt(den)) : up / den;
     }
-    double amp(uint32_t m, double c) const {
-        if (type[m] == SSTMC_SCATTERING_ISOTROPIC) return isoAmp();
-        return std::sqrt(pdf(m, c));
+    cplx amp(uint32_t m, double c) const {
+        if (type[m] == SSTMC_SCATTERING_ISOTROPIC) return cplx(isoAmp());
+        if (type[m] == SSTMC_SCATTERING_TABULAR) return tab[m].f[tabEntry(m, c)];
+        return cplx(std::sqrt(pdf(m, c)));
     }
     // randomDirection (VectorType.cuh:316-362).
     V3 randomDirection(Stream& s) const {
@@ -274,7 +319,12 @@ struct Scatter {
     V3 sample(uint32_t m, const V3& prev, Stream& s) const {
         if (type[m] == SSTMC_SCATTERING_ISOTROPIC) return randomDirection(s);
         double gg = g[m], c;
-        if (std::fabs(gg) < 0.001) {
+        if (type[m] == SSTMC_SCATTERING_TABULAR) {
+            // inverse cdf at round_num(u n) (Scattering.cuh:1030-1037)
+            size_t n = tab[m].icdf.size();
+            double r = std::nearbyint(s.uniform() * (double)n);
+            c = std::cos(tab[m].icdf[r >= (double)n ? n - 1 : (size_t)r]);
+        } else if (std::fabs(gg) < 0.001) {
             c = s.uniform() * 2.0 - 1.0;
         } else {
             double u = s.uniform();
@@ -549,17 +599,17 @@ struct Problem {
         return geo.normalize(sub(vw.pt[v], x));
     }
     // threePointFunctionSingle (FarField.cuh:546-599, PointSource.cuh:363-446)
-    double fsingle(uint32_t l, uint32_t v, uint32_t m, const V3& x1) const {
+    cplx fsingle(uint32_t l, uint32_t v, uint32_t m, const V3& x1) const {
         if (il.type == SSTMC_SOURCE_POINT && sc.allIsotropic) return sc.isoAmp();
         return sc.amp(m, geo.dot(inDir(l, x1), outDir(v, x1)));
     }
     // illumination f at pL=2 (FarField.cuh:546-556; PointSource.cuh:347-361)
-    double fil(uint32_t l, uint32_t m, const V3& x1, const V3& x2) const {
+    cplx fil(uint32_t l, uint32_t m, const V3& x1, const V3& x2) const {
         if (il.type == SSTMC_SOURCE_POINT && sc.allIsotropic) return sc.isoAmp();
         return sc.amp(m, geo.dot(inDir(l, x1), geo.normalize(sub(x2, x1))));
     }
     // view f at pL>=2: amplitude(-d_v | s_v, x_k, x_{k-1}) (NextEventEstimation.cuh:1560-1563)
-    double fvw(uint32_t v, uint32_t m, const V3& xprev, const V3& xk) const {
+    cplx fvw(uint32_t v, uint32_t m, const V3& xprev, const V3& xk) const {
         if (vw.type == SSTMC_SOURCE_POINT && sc.allIsotropic) return sc.isoAmp();
         V3 a = vw.type == SSTMC_SOURCE_FARFIELD ? scale(-1.0, vw.pt[v])
                                                   : geo.normalize(sub(xk, vw.pt[v]));
@@ -575,7 +625,7 @@ struct Problem {
     }
     // Three-point function of source i of s at vertex p1 with the path continuing to p2
     // (threePointsThroughput, NextEventEstimation.cuh:773-800).
-    double f3(const Source& s, uint32_t i, uint32_t m, const V3& p1, const V3& p2) const {
+    cplx f3(const Source& s, uint32_t i, uint32_t m, const V3& p1, const V3& p2) const {
         if (s.type == SSTMC_SOURCE_POINT && sc.allIsotropic) return sc.isoAmp();
         return sc.amp(m, geo.dot(incoming(s, i, p1), geo.normalize(sub(p2, p1))));
     }
@@ -720,16 +770,29 @@ int build(const sstmc_config* c, Problem& pb) {
     pb.sc.dims = dims;
     pb.sc.type.assign(nmat, 0);
     pb.sc.g.assign(nmat, 0.0);
+    pb.sc.tab.assign(nmat, Scatter::Table());
+    std::vector<size_t> toff(nm, 0);
+    for (uint32_t m = 0, acc = 0; m < nm; ++m) {
+        toff[m] = acc;
+        if (c->scattering.tabularSize) acc += 2 * c->scattering.tabularSize[m];
+    }
     for (uint32_t m = 0; m < nmat; ++m) {
         uint32_t src = nm == 1 ? 0 : m;
         int ty = c->scattering.type[src];
-        if (ty != SSTMC_SCATTERING_ISOTROPIC && ty != SSTMC_SCATTERING_HG) return SSTMC_NOT_SUPPORTED;
+        if (ty != SSTMC_SCATTERING_ISOTROPIC && ty != SSTMC_SCATTERING_HG && ty != SSTMC_SCATTERING_TABULAR)
+            return SSTMC_NOT_SUPPORTED;
         pb.sc.type[m] = ty;
         pb.sc.g[m] = (ty == SSTMC_SCATTERING_HG && c->scattering.g) ? c->scattering.g[src] : 0.0;
+        if (ty == SSTMC_SCATTERING_TABULAR) {
+            if (!c->scattering.tabular || !c->scattering.tabularSize || c->scattering.tabularSize[src] < 2)
+                return SSTMC_INVALID_CONFIG;
+            pb.sc.setTabular(m, c->scattering.tabular + toff[src], c->scattering.tabularSize[src]);
+        }
     }
-    pb.sc.allIsotropic = true;
+    pb.sc.allIsotropic = true;   // tabular never counts as isotropic (Scattering.cuh:478-482)
     for (uint32_t m = 0; m < nmat; ++m)
-        if (!(pb.sc.type[m] == SSTMC_SCATTERING_ISOTROPIC || std::fabs(pb.sc.g[m]) < 0.0001))
+        if (!(pb.sc.type[m] == SSTMC_SCATTERING_ISOTROPIC ||
+              (pb.sc.type[m] == SSTMC_SCATTERING_HG && std::fabs(pb.sc.g[m]) < 0.0001)))
             pb.sc.allIsotropic = false;
     // medium
     const sstmc_medium& md = c->medium;
@@ -1090,7 +1153,32 @@ int oracle_sample_directions(const sstmc_config* cfg, uint32_t m, const double* 
 double oracle_phase_amplitude(const sstmc_config* cfg, uint32_t m, double cosang, int pdf) {
     Problem pb;
     if (build(cfg, pb)) return -1.0;
-    return pdf ? pb.sc.pdf(m, cosang) : pb.sc.amp(m, cosang);
+    return pdf ? pb.sc.pdf(m, cosang) : pb.sc.amp(m, cosang).real();
+}
+
+// Complex amplitude of material m at cosine c: out[0] re, out[1] im.
+int oracle_phase_amplitude_complex(const sstmc_config* cfg, uint32_t m, double cosang, double* out) {
+    Problem pb;
+    int rc = build(cfg, pb);
+    if (rc) return rc;
+    cplx f = pb.sc.amp(m, cosang);
+    out[0] = f.real();
+    out[1] = f.imag();
+    return SSTMC_NO_ERROR;
+}
+
+// The tabular table of material m as built: f (2n doubles, interleaved), evaluation pdf (n),
+// inverse-cdf angles (n). Returns n (0 when m is not tabular); NULL outputs are skipped.
+uint32_t oracle_tabular_table(const sstmc_config* cfg, uint32_t m, double* f, double* pdf, double* icdf) {
+    Problem pb;
+    if (build(cfg, pb) || m >= pb.sc.tab.size()) return 0;
+    const Scatter::Table& t = pb.sc.tab[m];
+    for (size_t i = 0; i < t.f.size(); ++i) {
+        if (f) { f[2 * i] = t.f[i].real(); f[2 * i + 1] = t.f[i].imag(); }
+        if (pdf) pdf[i] = t.pdf[i];
+        if (icdf) icdf[i] = t.icdf[i];
+    }
+    return (uint32_t)t.f.size();
 }
 
 // FarField sampling table of illumination l: tangent, perpT, perpS, minT, maxT, minS, maxS, prob.

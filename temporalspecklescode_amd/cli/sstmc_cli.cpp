@@ -15,7 +15,9 @@
 //   medium: type "Heterogeneous", xAxis, yAxis[, zAxis], materialGrid (flat, x fastest, or nested
 //           [x][y][z]) or materialGridFile (raw bytes, x fastest; materialGridType "uint8"|"uint32"),
 //           sigs, siga
-//   scattering: type ("Isotropic" | "HenyeyGreenstein" | "HG", or a list), amplitudeFunction
+//   scattering: type ("Isotropic" | "Tabular" | "HenyeyGreenstein" | "HG", or a list),
+//               amplitudeFunction (HG: g; Tabular: {"re": [...], "im": [...]}, [[re, im], ...] or
+//               a real list, per material)
 //   illumination / view: type "FarField" (direction{x,y[,z]}) | "PointSource" (location{...}),
 //                        isAreaSource
 //   sampler: type "TemporalSampler", t, D, U{x,y,z}, isTemporalAmplitude
@@ -152,27 +154,62 @@ void fill(HostConfig& h, const Value& cfg, uint32_t dims) {
     const Value& tv = sc.at("type");
     if (tv.kind == Value::String) types.push_back(tv.str);
     else for (const Value& e : tv.arr) types.push_back(e.str);
-    std::vector<double> amps(types.size(), 0.0);
+    // amplitudeFunction per material: one entry per material, or one value for all
+    std::vector<const Value*> ampv(types.size(), nullptr);
     if (sc.has("amplitudeFunction")) {
         const Value& av = sc.at("amplitudeFunction");
-        if (av.kind == Value::Array && av.arr.size() == types.size()) {
-            for (size_t k = 0; k < types.size(); ++k) amps[k] = nums(av.arr[k])[0];
-        } else {
-            amps.assign(types.size(), nums(av)[0]);
-        }
+        const bool perMaterial = av.kind == Value::Array && av.arr.size() == types.size() &&
+                                 (types.size() > 1 || av.arr[0].kind != Value::Number);
+        for (size_t k = 0; k < types.size(); ++k) ampv[k] = perMaterial ? &av.arr[k] : &av;
     }
     std::vector<int32_t> tcode;
-    std::vector<double> g;
+    std::vector<double> g, tab;
+    std::vector<uint32_t> tsize;
     for (size_t k = 0; k < types.size(); ++k) {
+        const std::string mk = std::to_string(k);
         if (types[k] == "Isotropic") {
             tcode.push_back(SSTMC_SCATTERING_ISOTROPIC);
             g.push_back(0.0);
+            tsize.push_back(0);
         } else if (types[k] == "HenyeyGreenstein" || types[k] == "HG") {
             tcode.push_back(SSTMC_SCATTERING_HG);
-            g.push_back(amps[k]);
+            g.push_back(ampv[k] ? nums(*ampv[k])[0] : 0.0);
+            tsize.push_back(0);
+        } else if (types[k] == "Tabular") {
+            // complex amplitude samples over the scattering angle (Scattering.cuh:272-429)
+            std::vector<double> re, im;
+            if (ampv[k] && ampv[k]->kind == Value::Object) {
+                re = nums(ampv[k]->at("re"));
+                im = ampv[k]->has("im") ? nums(ampv[k]->at("im")) : std::vector<double>(re.size(), 0.0);
+            } else if (ampv[k] && ampv[k]->kind == Value::Array && !ampv[k]->arr.empty() &&
+                       ampv[k]->arr[0].kind == Value::Array) {
+                for (const Value& e : ampv[k]->arr) {
+                    std::vector<double> c = nums(e);
+                    re.push_back(c.at(0));
+                    im.push_back(c.size() > 1 ? c[1] : 0.0);
+                }
+            } else if (ampv[k]) {
+                re = nums(*ampv[k]);
+                im.assign(re.size(), 0.0);
+            }
+            bool finite = re.size() == im.size(), nonzero = false;
+            for (size_t i = 0; finite && i < re.size(); ++i) {
+                finite = std::isfinite(re[i]) && std::isfinite(im[i]);
+                nonzero = nonzero || re[i] != 0.0 || im[i] != 0.0;
+            }
+            if (re.size() < 2 || !finite || !nonzero)
+                throw ConfigError("Tabular amplitude function of material " + mk +
+                                  " must hold at least two finite samples, not all zero");
+            tcode.push_back(SSTMC_SCATTERING_TABULAR);
+            g.push_back(0.0);
+            tsize.push_back((uint32_t)re.size());
+            for (size_t i = 0; i < re.size(); ++i) {
+                tab.push_back(re[i]);
+                tab.push_back(im[i]);
+            }
         } else {
-            throw ConfigError("Unsupported scattering type in material " + std::to_string(k) + ". Type entered: " +
-                              types[k] + ". Available scattering types are Isotropic and HenyeyGreenstein (HG)");
+            throw ConfigError("Unsupported scattering type in material " + mk + ". Type entered: " + types[k] +
+                              ". Available scattering types are Isotropic, Tabular and HenyeyGreenstein (HG)");
         }
     }
     const uint32_t nsc = (uint32_t)tcode.size();
@@ -181,6 +218,11 @@ void fill(HostConfig& h, const Value& cfg, uint32_t dims) {
     s.scattering.materialNum = nsc;
     s.scattering.type = h.i.back().data();
     s.scattering.g = h.keep(g);
+    if (!tab.empty()) {
+        h.u.push_back(tsize);
+        s.scattering.tabularSize = h.u.back().data();
+        s.scattering.tabular = h.keep(tab);
+    }
 
     // medium (MatlabInterface.cu:890-1060)
     const Value& md = cfg.at("medium");

@@ -21,6 +21,7 @@ import numpy as np
 # enum values, kept equal to include/sstmc.h
 MEDIUM_HETEROGENEOUS = 1
 SCATTERING_ISOTROPIC = 1
+SCATTERING_TABULAR = 2
 SCATTERING_HG = 3
 SOURCE_NONE, SOURCE_POINT, SOURCE_FARFIELD = 0, 1, 2
 SAMPLER_TEMPORAL = 1
@@ -47,7 +48,8 @@ class Medium(C.Structure):
 
 
 class Scattering(C.Structure):
-    _fields_ = [("materialNum", C.c_uint32), ("type", _I32P), ("g", _DP)]
+    _fields_ = [("materialNum", C.c_uint32), ("type", _I32P), ("g", _DP), ("tabularSize", _U32P),
+                ("tabular", _DP)]
 
 
 class Source(C.Structure):
@@ -127,24 +129,42 @@ class CConfig:
         amps = sc.get("amplitudeFunction", [0.0] * len(types))
         if not isinstance(amps, (list, tuple)):
             amps = [amps]
-        tcode, gvals = [], []
+        elif len(types) == 1 and len(amps) > 1 and np.isscalar(amps[0]):
+            amps = [amps]   # one material given its table as a plain list
+        tcode, gvals, tsize, tables = [], [], [], []
         for i, t in enumerate(types):
             if t == "Isotropic":
                 tcode.append(SCATTERING_ISOTROPIC)
                 gvals.append(0.0)
+                tsize.append(0)
             elif t in ("HenyeyGreenstein", "HG"):
                 tcode.append(SCATTERING_HG)
                 gvals.append(float(np.asarray(amps[i]).ravel()[0]))
+                tsize.append(0)
+            elif t == "Tabular":
+                # complex amplitude samples over the scattering angle (Scattering.cuh:272-429)
+                f = np.asarray(amps[i], dtype=np.complex128).ravel(order="F")
+                if f.size < 2 or not np.all(np.isfinite(f)) or not np.any(f != 0):
+                    raise ValueError("Tabular amplitude function of material %d must hold at least two finite "
+                                     "samples, not all zero" % i)
+                tcode.append(SCATTERING_TABULAR)
+                gvals.append(0.0)
+                tsize.append(f.size)
+                tables.append(np.stack([f.real, f.imag], axis=1).ravel())
             else:
                 raise ValueError(
                     "Unsupported scattering type in material %d. Type entered: %s. Available "
-                    "scattering types are Isotropic and HenyeyGreenstein (HG)" % (i, t))
+                    "scattering types are Isotropic, Tabular and HenyeyGreenstein (HG)" % (i, t))
         n_sc = len(tcode)
         material_num = 2 if n_sc == 1 else n_sc
         ta = self._arr(tcode, np.int32)
         s.scattering.materialNum = n_sc
         s.scattering.type = ta.ctypes.data_as(_I32P)
         s.scattering.g, _ = self._dptr(gvals)
+        if tables:
+            sz = self._arr(tsize, np.uint32)
+            s.scattering.tabularSize = sz.ctypes.data_as(_U32P)
+            s.scattering.tabular, _ = self._dptr(np.concatenate(tables))
 
         # medium (MatlabInterface.cu:890-1060)
         md = cfg["medium"]

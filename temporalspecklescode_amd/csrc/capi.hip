@@ -52,8 +52,12 @@ static int validate(const sstmc_config* c) {
     if (sc.materialNum == 0 || sc.materialNum > (uint32_t)kMaxMaterials || !sc.type) return SSTMC_INVALID_CONFIG;
     uint32_t nmat = sc.materialNum == 1 ? 2 : sc.materialNum;
     for (uint32_t m = 0; m < sc.materialNum; ++m) {
-        if (sc.type[m] != SSTMC_SCATTERING_ISOTROPIC && sc.type[m] != SSTMC_SCATTERING_HG) return SSTMC_NOT_SUPPORTED;
+        if (sc.type[m] != SSTMC_SCATTERING_ISOTROPIC && sc.type[m] != SSTMC_SCATTERING_HG &&
+            sc.type[m] != SSTMC_SCATTERING_TABULAR)
+            return SSTMC_NOT_SUPPORTED;
         if (sc.type[m] == SSTMC_SCATTERING_HG && sc.g && !(std::fabs(sc.g[m]) < 1.0)) return SSTMC_INVALID_CONFIG;
+        if (sc.type[m] == SSTMC_SCATTERING_TABULAR && (!sc.tabularSize || !sc.tabular || sc.tabularSize[m] < 2))
+            return SSTMC_INVALID_CONFIG;
     }
     const sstmc_medium& md = c->medium;
     if (md.type != SSTMC_MEDIUM_HETEROGENEOUS) return SSTMC_NOT_SUPPORTED;
@@ -109,14 +113,22 @@ static int create(const sstmc_config* c, sstmc_handle** out) {
     if (sc.materialNum == 0 || sc.materialNum > (uint32_t)kMaxMaterials || !sc.type) return SSTMC_INVALID_CONFIG;
     uint32_t nmat = sc.materialNum == 1 ? 2 : sc.materialNum;
     h->scattering.reset(new Scattering(dims));
+    std::vector<size_t> tabOff(sc.materialNum, 0);   // start of material m's table in sc.tabular
+    for (uint32_t m = 0, acc = 0; m < sc.materialNum; ++m) {
+        tabOff[m] = acc;
+        if (sc.tabularSize) acc += 2 * sc.tabularSize[m];
+    }
     for (uint32_t m = 0; m < nmat; ++m) {
         uint32_t src = sc.materialNum == 1 ? 0 : m;
         int rc;
         if (sc.type[src] == SSTMC_SCATTERING_ISOTROPIC) rc = h->scattering->setIsotropicScattering(m);
         else if (sc.type[src] == SSTMC_SCATTERING_HG) rc = h->scattering->setHenyeyGreensteinScattering(sc.g ? sc.g[src] : 0.0, m);
+        else if (sc.type[src] == SSTMC_SCATTERING_TABULAR)
+            rc = h->scattering->setTabularScattering(sc.tabular + tabOff[src], sc.tabularSize[src], m);
         else return SSTMC_NOT_SUPPORTED;
         if (rc) return rc;
     }
+    if (int rc = h->scattering->upload()) return rc;
 
     // medium (MatlabInterface.cu:951-1056)
     const sstmc_medium& md = c->medium;

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 namespace sstmc {
 
 constexpr int kMaxMaterials = 10;   // MATERIAL_NUM (Simulation.cuh:32)
@@ -118,13 +120,38 @@ struct MediumView {
 };
 
 // Scattering: Scattering.cuh:15-29, HG table :500-572.
+// Tabular amplitude functions (Scattering.cuh:191-482), in device memory: material m's n[m]
+// entries start at off[m], equally spaced over the scattering angle on [0, pi] (3D) or [0, 2 pi]
+// (2D); f is the amplitude normalised by the cross section, pdf the reference's evaluation pdf
+// (|f|^2, times sin of the angle in 3D), icdf the inverse cdf's angle for entry i of a uniform
+// draw. Kept out of ScatterView so that the kernels' by-value argument stays small.
+struct TabularTables {
+    uint32_t n[kMaxMaterials];
+    uint32_t off[kMaxMaterials];
+    const float4* f;               // (|f|, Re f, Im f, pdf) in FP32
+    const double4* d;              // the same in FP64
+    const double* ph;              // arg f / (2 pi): the amplitude's phase in cycles
+    const float* icdf;
+};
+
 struct ScatterView {
     int dims;
     int allIsotropic;
-    int type[kMaxMaterials];       // 1 isotropic, 3 HG
+    int type[kMaxMaterials];       // 1 isotropic, 2 tabular, 3 HG
     float g[kMaxMaterials];
     double gd[kMaxMaterials];
+    const TabularTables* tab;      // device tables when a material is tabular, else null
 };
+
+// Entry of cosine c in an n-entry tabular amplitude function: the nearest tabulated angle,
+// acos with the reference's end clamps, rounded half to even and clamped to the table
+// (Scattering.cuh:908-915; safeAcos and round_num, Simulation.cuh:274-297).
+__host__ __device__ inline uint32_t tabularEntry(uint32_t n, int dims, double c) {
+    const double a = c < -1.0 + 1e-12 ? 3.14159265358979323846 : (c > 1.0 - 1e-12 ? 0.0 : acos(c));
+    const double x = a * (double)(n - 1) / (dims == 2 ? 6.28318530717958647692 : 3.14159265358979323846);
+    const double r = rint(x);
+    return r < 0 ? 0u : (r >= (double)n ? n - 1 : (uint32_t)r);
+}
 
 // Sources: FarField (FarField.cuh:71-127) / PointSource (PointSource.cuh:176-216).
 // FarField illumination sampling table per source, 20 floats: tangent[3] perpT[3] perpS[3]

@@ -212,10 +212,10 @@ __global__ void __launch_bounds__(256) k_contract_fma(const typename C2T<T>::T* 
 }
 
 // Fixed-order reduction of split partials into the FP64 accumulator, with the elementwise
-// single-scattering factor F (or F^2 for correlations) and a scalar.
+// single-scattering factor F (complex; |F|^2 for correlations, S(0) conj S(t)) and a scalar.
 template <typename C>
 __global__ void __launch_bounds__(256) k_reduce(const C* __restrict__ part, uint32_t splits, uint32_t LV,
-                                                const float* __restrict__ F, const double* __restrict__ Fd, int fsq,
+                                                const float2* __restrict__ F, const double2* __restrict__ Fd, int fsq,
                                                 double fscalar, double* __restrict__ U) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= LV) return;
@@ -225,11 +225,12 @@ __global__ void __launch_bounds__(256) k_reduce(const C* __restrict__ part, uint
         re += (double)c.x;
         im += (double)c.y;
     }
-    double f = fscalar;
-    if (Fd) f *= fsq ? Fd[i] * Fd[i] : Fd[i];
-    else if (F) f *= fsq ? (double)F[i] * F[i] : (double)F[i];
-    U[2 * (size_t)i] += f * re;
-    U[2 * (size_t)i + 1] += f * im;
+    double fr = 1.0, fi = 0.0;
+    if (Fd) { fr = Fd[i].x; fi = Fd[i].y; }
+    else if (F) { fr = (double)F[i].x; fi = (double)F[i].y; }
+    if (fsq) { fr = fr * fr + fi * fi; fi = 0.0; }
+    U[2 * (size_t)i] += fscalar * (fr * re - fi * im);
+    U[2 * (size_t)i + 1] += fscalar * (fr * im + fi * re);
 }
 
 static uint32_t chooseSplits(uint32_t tiles, uint32_t P, uint32_t minPaths) {
@@ -241,7 +242,7 @@ static uint32_t chooseSplits(uint32_t tiles, uint32_t P, uint32_t minPaths) {
 
 // The split-K workspace belongs to the caller (one per tracer), grown here when too small.
 hipError_t launchContract(int kind, const void* A, const void* B, uint32_t P, uint32_t L, uint32_t V, uint32_t nt,
-                          const float* F, const double* Fd, int fsq, double fscalar, double* U, Workspace* ws,
+                          const float2* F, const double2* Fd, int fsq, double fscalar, double* U, Workspace* ws,
                           hipStream_t st) {
     uint32_t tl = kind == kContractBf16x3 ? kTileL : 64, tv = kind == kContractBf16x3 ? kTileV : 64;
     uint32_t tiles = ((L + tl - 1) / tl) * ((V + tv - 1) / tv);
@@ -305,13 +306,25 @@ template <> struct SingleT<double> {
     __device__ static double sq(double x) { return sqrt(x); }
 };
 
+// Single-scattering amplitude at cosine c (complex: a tabular amplitude carries a phase).
 template <int DIMS, typename R>
-__device__ inline R singleAmp(const ScatterView& s, uint32_t m, R c) {
-    if (s.type[m] == 1) return DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678;
-    R g = (R)s.gd[m];
-    R up = DIMS == 3 ? ((R)1 - g * g) / (R)12.566370614359172954 : ((R)1 - g * g) / (R)6.283185307179586477;
-    R den = fma((R)-2 * g, c, (R)1 + g * g);
-    return SingleT<R>::sq(DIMS == 3 ? up / (den * SingleT<R>::sq(den)) : up / den);
+__device__ inline typename SingleT<R>::C singleAmp(const ScatterView& s, uint32_t m, R c) {
+    typename SingleT<R>::C f;
+    f.y = 0;
+    if (s.type[m] == 1) {
+        f.x = DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678;
+    } else if (s.type[m] == 2) {
+        const TabularTables& t = *s.tab;
+        const uint32_t i = t.off[m] + tabularEntry(t.n[m], DIMS, (double)c);
+        if constexpr (sizeof(R) == 4) { f.x = t.f[i].y; f.y = t.f[i].z; }
+        else { f.x = t.d[i].y; f.y = t.d[i].z; }
+    } else {
+        R g = (R)s.gd[m];
+        R up = DIMS == 3 ? ((R)1 - g * g) / (R)12.566370614359172954 : ((R)1 - g * g) / (R)6.283185307179586477;
+        R den = fma((R)-2 * g, c, (R)1 + g * g);
+        f.x = SingleT<R>::sq(DIMS == 3 ? up / (den * SingleT<R>::sq(den)) : up / den);
+    }
+    return f;
 }
 
 template <int DIMS, typename R>
@@ -379,10 +392,12 @@ __global__ void __launch_bounds__(256) k_single_generic(ScatterView sc, SourceVi
         for (int q = 0; q < 16; ++q) {
             C x = sa[q][tl], y = sb[q][tq];
             R c = su[q][tl][0] * sw[q][tq][0] + su[q][tl][1] * sw[q][tq][1] + su[q][tl][2] * sw[q][tq][2];
-            R f = iso ? (DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678) : singleAmp<DIMS, R>(sc, sm[q], c);
+            C f;
+            if (iso) { f.x = DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678; f.y = 0; }
+            else f = singleAmp<DIMS, R>(sc, sm[q], c);
             R pr = x.x * y.x - x.y * y.y, pi = x.x * y.y + x.y * y.x;
-            re = fma(f, pr, re);
-            im = fma(f, pi, im);
+            re = fma(f.x, pr, fma(-f.y, pi, re));
+            im = fma(f.x, pi, fma(f.y, pr, im));
         }
         __syncthreads();
     }
@@ -427,8 +442,8 @@ hipError_t launchSingleGeneric(int dims, int fp64, const ScatterView& sc, const 
                                (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<double2>, dim3((LV + 255) / 256), dim3(256), 0, st,
-                               (const double2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float*)nullptr,
-                               (const double*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
+                               (const double2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,
+                               (const double2*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
     } else {
         if (dims == 2)
             hipLaunchKernelGGL((k_single_generic<2, float>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
@@ -438,8 +453,8 @@ hipError_t launchSingleGeneric(int dims, int fp64, const ScatterView& sc, const 
                                (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st,
-                               (const float2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float*)nullptr,
-                               (const double*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
+                               (const float2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,
+                               (const double2*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
     }
     return hipGetLastError();
 }

@@ -155,7 +155,15 @@ template <int DIMS>
 __device__ inline f3 sampleDirection(const ScatterView& s, uint32_t mat, const f3& prev, Rng& rng) {
     if (s.type[mat] == 1) return randomDirection<DIMS>(rng);
     float g = s.g[mat], c;
-    if (fabsf(g) < 0.001f) {
+    if (s.type[mat] == 2) {
+        // tabular: the inverse cdf at round(u n) (Scattering.cuh:1030-1037); u has 25 significant
+        // bits, so u n is exact in FP64 and the entry is the one an FP64 evaluation picks
+        const TabularTables& t = *s.tab;
+        const uint32_t n = t.n[mat];
+        const double x = rint((double)rng.uniform() * (double)n);
+        const uint32_t i = x >= (double)n ? n - 1 : (uint32_t)x;
+        c = cosf(t.icdf[t.off[mat] + i]);
+    } else if (fabsf(g) < 0.001f) {
         c = rng.uniform() * 2.0f - 1.0f;
     } else {
         float u = rng.uniform();

@@ -62,7 +62,7 @@ struct Workspace {
     size_t bytes = 0;
 };
 hipError_t launchContract(int kind, const void* A, const void* B, uint32_t P, uint32_t L, uint32_t V, uint32_t nt,
-                          const float* F, const double* Fd, int fsq, double fscalar, double* U, Workspace* ws,
+                          const float2* F, const double2* Fd, int fsq, double fscalar, double* U, Workspace* ws,
                           hipStream_t st);
 
 // Generic single scattering (field output): U[t][v][l] += sum_p a[t][p][l] b[t][p][v] f_p(l, v, t).

@@ -72,17 +72,36 @@ template <int DIMS> __device__ inline d3 dnormalize(const d3& a) {
 // evaluated in FP64: for forward-peaked HG it cancels to (1 - g)^2 near c = 1, which FP32 would
 // resolve to ~1e-5 relative only.
 template <int DIMS, typename R>
-__device__ inline R phasePdf(const ScatterView& s, uint32_t m, double c) {
-    if (s.type[m] == 1) return DIMS == 3 ? (R)0.079577471545947668 : (R)0.159154943091895336;
-    const double g = s.gd[m];
+__device__ inline R hgPdf(double g, double c) {
     const R up = (R)(DIMS == 3 ? (1.0 - g * g) / 12.566370614359172954 : (1.0 - g * g) / 6.283185307179586477);
     const R den = (R)fma(-2.0 * g, c, 1.0 + g * g);
     return DIMS == 3 ? up / (den * sqrtR<R>(den)) : up / den;
 }
 template <int DIMS, typename R>
-__device__ inline R phaseAmp(const ScatterView& s, uint32_t m, double c) {
+__device__ inline R phasePdf(const ScatterView& s, uint32_t m, double c) {
+    if (s.type[m] == 1) return DIMS == 3 ? (R)0.079577471545947668 : (R)0.159154943091895336;
+    if (s.type[m] == 2) {   // tabular evaluation pdf (Scattering.cuh:971-980)
+        const TabularTables& t = *s.tab;
+        const uint32_t i = t.off[m] + tabularEntry(t.n[m], DIMS, c);
+        if constexpr (sizeof(R) == 4) return t.f[i].w;
+        else return t.d[i].w;
+    }
+    return hgPdf<DIMS, R>(s.gd[m], c);
+}
+// Amplitude |f| at cosine c; a tabular amplitude's phase (in cycles) is added to *ph
+// (Scattering.cuh:907-916: the tabulated complex amplitude). TAB = false compiles the tabular
+// case out (no material is tabular), keeping its FP64 acos out of the view kernels' registers.
+template <int DIMS, typename R, bool TAB = true>
+__device__ inline R phaseAmp(const ScatterView& s, uint32_t m, double c, double* ph) {
     if (s.type[m] == 1) return DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678;
-    return sqrtR<R>(phasePdf<DIMS, R>(s, m, c));
+    if (TAB && s.type[m] == 2) {
+        const TabularTables& t = *s.tab;
+        const uint32_t i = t.off[m] + tabularEntry(t.n[m], DIMS, c);
+        *ph += t.ph[i];
+        if constexpr (sizeof(R) == 4) return t.f[i].x;
+        else return t.d[i].x;
+    }
+    return sqrtR<R>(hgPdf<DIMS, R>(s.gd[m], c));
 }
 
 // Voxel walk dispatch: the branch-free FP32 walk over the packed walk grid (equally spaced
@@ -310,7 +329,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
     for (uint32_t i = sub; i < L * NT; i += lpp) {
         const uint32_t l = i % L, t = i / L;
         R ga[2], fa[2] = {0, 0};
-        double gp[2];
+        double gp[2], fp[2] = {0, 0};   // fp: phase of a tabular amplitude, in cycles
         const int nf = corr ? 2 : 1;
         for (int f = 0; f < nf; ++f) {
             uint32_t tt = (corr && f == 0) ? 0 : t;
@@ -321,16 +340,17 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
                 const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
                 const d3 pb = fr.temporalAmp ? x2t.pd : x2cd;
                 if (isoPointIl) fa[f] = isoAmp<DIMS, R>();
-                else fa[f] = phaseAmp<DIMS, R>(sc, m1, ddot<DIMS>(illumIn<DIMS>(il, l, pa), dnormalize<DIMS>(dsub(pb, pa))));
+                else fa[f] = phaseAmp<DIMS, R>(sc, m1, ddot<DIMS>(illumIn<DIMS>(il, l, pa), dnormalize<DIMS>(dsub(pb, pa))),
+                                               &fp[f]);
             }
         }
         const size_t o = ((size_t)t * P + p) * L + l;
         if (!corr) {
             a[o] = cis<R>(ga[0] * w1, gp[0] + rho1);
-            A[o] = multi ? cis<R>(ga[0] * fa[0] * w2, gp[0]) : zero;
+            A[o] = multi ? cis<R>(ga[0] * fa[0] * w2, gp[0] + fp[0]) : zero;
         } else {
             a[o] = cis<R>(ga[0] * ga[1] * w1 * w1, gp[0] - gp[1]);
-            A[o] = multi ? cis<R>(ga[0] * fa[0] * ga[1] * fa[1] * w2 * w2, gp[0] - gp[1]) : zero;
+            A[o] = multi ? cis<R>(ga[0] * fa[0] * ga[1] * fa[1] * w2 * w2, gp[0] - gp[1] + fp[0] - fp[1]) : zero;
         }
     }
 
@@ -359,15 +379,17 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         b[o] = corr ? cis<R>(ga[0] * ga[1], gp[0] - gp[1]) : cis<R>(ga[0], gp[0]);
         if (Ar) {   // reversed path: the view connected at x1 with f toward x2 (field output only)
             R fv = 0;
+            double fvp = 0;
             if (multi) {
                 const Geo x1t = vertexFrame(vb, fr, off, t, x1m);
                 const Geo x2t = vertexFrame(vb, fr, off + 1, t, x2m);
                 const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
                 const d3 pb = fr.temporalAmp ? x2t.pd : x2cd;
                 fv = isoPointVw ? isoAmp<DIMS, R>()
-                                : phaseAmp<DIMS, R>(sc, m1, ddot<DIMS>(incoming<DIMS>(vw, v, pa), dnormalize<DIMS>(dsub(pb, pa))));
+                                : phaseAmp<DIMS, R>(sc, m1, ddot<DIMS>(incoming<DIMS>(vw, v, pa), dnormalize<DIMS>(dsub(pb, pa))),
+                                                    &fvp);
             }
-            Ar[o] = multi ? cis<R>(ga[0] * fv * w2, gp[0]) : zero;
+            Ar[o] = multi ? cis<R>(ga[0] * fv * w2, gp[0] + fvp) : zero;
         }
     }
     *stepsOut += steps;
@@ -438,7 +460,7 @@ enum : int { kSrcPoint = 0, kSrcFarView = 1, kSrcFarIllum = 2 };
 // the record's back direction, with the record's phase base added to the source phase. The
 // source's direction or location arrives in registers (vd FP64, vf FP32 direction), loaded once
 // per lane before the vertex loop.
-template <int DIMS, typename R, int SRC>
+template <int DIMS, typename R, int SRC, bool TAB>
 __device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, const ScatterView& sc,
                                             const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
                                             const d3& vd, const f3& vf, uint64_t ok, uint32_t t, const float4& xkm,
@@ -458,7 +480,7 @@ __device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, 
         const d3 av = SRC == kSrcFarIllum  ? vd
                       : SRC == kSrcFarView ? mkd(-vd.x, -vd.y, -vd.z)
                                            : dnormalize<DIMS>(dsub(pk, vd));
-        fv = phaseAmp<DIMS, R>(sc, __float_as_uint(xkm.w), ddot<DIMS>(av, mkd(rec.bx, rec.by, rec.bz)));
+        fv = phaseAmp<DIMS, R, TAB>(sc, __float_as_uint(xkm.w), ddot<DIMS>(av, mkd(rec.bx, rec.by, rec.bz)), &gp);
     }
     return cis<R>(ga * fv, gp + rec.phase);
 }
@@ -503,7 +525,7 @@ __device__ inline ViewUnit viewUnit(const NeeParams& prm) {
 // for FP32 far-field sources (64 VGPRs; the latency of the walk's scattered word reads wants the
 // occupancy), 6 for FP32 point sources (80 VGPRs; their longer per-vertex set-up spills in the
 // vertex loop at 64), 4 for FP64 (no spills; 1.6x faster than 6 waves with spills).
-template <int DIMS, typename R, int SRC, bool WAVE>
+template <int DIMS, typename R, int SRC, bool WAVE, bool TAB>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 4 : (SRC != kSrcPoint ? 8 : 6))))
 k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBatch vb, NeeParams prm, NeeOut out) {
@@ -529,7 +551,7 @@ k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBat
         R accRe = 0, accIm = 0;
         for (uint32_t k = 1; k < K; ++k) {
             const uint64_t ok = off + k;
-            const C g = viewTerm<DIMS, R, SRC>(m, sig, sc, vw, fr, vb, vd, vf, ok, t, vb.xm[ok], invLambda, tauCut,
+            const C g = viewTerm<DIMS, R, SRC, TAB>(m, sig, sc, vw, fr, vb, vd, vf, ok, t, vb.xm[ok], invLambda, tauCut,
                                                isoPoint, &steps);
             accRe += g.x;
             accIm += g.y;
@@ -544,7 +566,7 @@ k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBat
 // (path, chunk of up to kCorrLags frames, block of lppView views): one view per lane; per vertex,
 // G_k(0) is formed once and reused for every frame of the chunk, whose running sums live in LDS
 // (one slot per thread and frame).
-template <int DIMS, typename R, int SRC, bool WAVE>
+template <int DIMS, typename R, int SRC, bool WAVE, bool TAB>
 __global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView sc, SourceView vw, FramesView fr,
                                                        VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
@@ -572,12 +594,12 @@ __global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView
         for (uint32_t k = 1; k < K; ++k) {
             const uint64_t ok = off + k;
             const float4 xkm = vb.xm[ok];
-            const C g0 = viewTerm<DIMS, R, SRC>(m, sig, sc, vw, fr, vb, vd, vf, ok, 0, xkm, invLambda, tauCut,
+            const C g0 = viewTerm<DIMS, R, SRC, TAB>(m, sig, sc, vw, fr, vb, vd, vf, ok, 0, xkm, invLambda, tauCut,
                                                 isoPoint, &steps);
             for (uint32_t j = 0; j < nt; ++j) {
                 const uint32_t t = t0 + j;
                 const C gt = t == 0 ? g0
-                                    : viewTerm<DIMS, R, SRC>(m, sig, sc, vw, fr, vb, vd, vf, ok, t, xkm, invLambda,
+                                    : viewTerm<DIMS, R, SRC, TAB>(m, sig, sc, vw, fr, vb, vd, vf, ok, t, xkm, invLambda,
                                                               tauCut, isoPoint, &steps);
                 C a = acc[j][threadIdx.x];
                 a.x += g0.x * gt.x + g0.y * gt.y;      // G(0) * conj(G(t))
@@ -605,17 +627,21 @@ static void launchView(bool corr, const MediumView& m, const ScatterView& sc, co
                        hipStream_t st) {
     const uint64_t units = (uint64_t)q.npaths * q.viewBlocks * q.frameChunks, upb = 4 * (64 / q.lppView);
     const dim3 grid((uint32_t)((units + upb - 1) / upb)), block(256);
-    const bool wave = q.lppView == 64;
+    const bool wave = q.lppView == 64, tab = sc.tab != nullptr;
+#define SSTMC_VIEW(K, W, T) hipLaunchKernelGGL((K<DIMS, R, SRC, W, T>), grid, block, 0, st, m, sc, src, fr, vb, q, out)
     if (corr) {
         if constexpr (SRC != kSrcFarIllum) {
-            if (wave) hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, SRC, true>), grid, block, 0, st, m, sc, src, fr, vb, q, out);
-            else hipLaunchKernelGGL((k_nee_view_corr<DIMS, R, SRC, false>), grid, block, 0, st, m, sc, src, fr, vb, q, out);
+            if (wave) { if (tab) SSTMC_VIEW(k_nee_view_corr, true, true); else SSTMC_VIEW(k_nee_view_corr, true, false); }
+            else { if (tab) SSTMC_VIEW(k_nee_view_corr, false, true); else SSTMC_VIEW(k_nee_view_corr, false, false); }
         }
     } else if (wave) {
-        hipLaunchKernelGGL((k_nee_view<DIMS, R, SRC, true>), grid, block, 0, st, m, sc, src, fr, vb, q, out);
+        if (tab) SSTMC_VIEW(k_nee_view, true, true);
+        else SSTMC_VIEW(k_nee_view, true, false);
     } else {
-        hipLaunchKernelGGL((k_nee_view<DIMS, R, SRC, false>), grid, block, 0, st, m, sc, src, fr, vb, q, out);
+        if (tab) SSTMC_VIEW(k_nee_view, false, true);
+        else SSTMC_VIEW(k_nee_view, false, false);
     }
+#undef SSTMC_VIEW
 }
 
 template <int DIMS, typename R>

@@ -75,17 +75,35 @@ public:
     explicit Scattering(int dims);
     int setIsotropicScattering(uint32_t materialNum);
     int setHenyeyGreensteinScattering(double g, uint32_t materialNum);
+    // Scattering::setTabularScattering (Scattering.cuh:272-429): f holds n complex samples
+    // (interleaved re, im) of the amplitude over the scattering angle.
+    int setTabularScattering(const double* f, uint32_t n, uint32_t materialNum);
+    // copies the tabular tables to the device (after the last set*; no-op without tables)
+    int upload();
     // Scattering::isIsotropicScattering (Scattering.cuh:1505-1527): HG with |g| < 1e-4 counts.
     bool isIsotropicScattering() const;
     // true when every material in [1, n) has the same phase function
     bool isUniform(uint32_t n) const;
     bool isExactlyIsotropic(uint32_t n) const;
-    double amplitude(uint32_t m, double cosang) const;
+    // amplitude (complex: tabular amplitudes carry a phase) at the scattering cosine
+    void amplitude(uint32_t m, double cosang, double* re, double* im) const;
     const ScatterView& view() const { return v_; }
 
 private:
     ScatterView v_{};
     bool defined_[kMaxMaterials] = {};
+    // tabular tables of every material, host copies (entry order as on the device)
+    std::vector<float4> tabF_;
+    std::vector<double4> tabD_;
+    std::vector<double> tabPh_, tabRaw_;   // tabRaw_: the input samples (uniformity test)
+    std::vector<float> tabIcdf_;
+    uint32_t rawOff_[kMaxMaterials] = {};
+    TabularTables tab_{};               // sizes and offsets (host), device pointers after upload
+    DevBuf<TabularTables> dTab_;
+    DevBuf<float4> dTabF_;
+    DevBuf<double4> dTabD_;
+    DevBuf<double> dTabPh_;
+    DevBuf<float> dTabIcdf_;
 };
 
 // ---------------------------------------------------------------- Sources
@@ -238,8 +256,8 @@ private:
     DevBuf<unsigned char> A_, B_, a_, b_;
     DevBuf<unsigned char> Ar_, Br_;   // coherent backscattering: reversed-path terms
     DevBuf<double> U_;
-    DevBuf<float> Ff_;
-    DevBuf<double> Fd_;
+    DevBuf<float2> Ff_;             // single-scattering F (complex) when separable
+    DevBuf<double2> Fd_;
     Workspace wsMulti_, wsSingle_;
     // per batch: 0 start, 1 after transport, 5 between k_nee and k_nee_view, 2 after NEE,
     // 3 after the multiple-scattering contraction, 4 after the single-scattering one

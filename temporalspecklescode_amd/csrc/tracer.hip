@@ -40,6 +40,10 @@ template struct DevBuf<double>;
 template struct DevBuf<uint8_t>;
 template struct DevBuf<uint32_t>;
 template struct DevBuf<float4>;
+template struct DevBuf<double4>;
+template struct DevBuf<float2>;
+template struct DevBuf<double2>;
+template struct DevBuf<TabularTables>;
 template struct DevBuf<unsigned long long>;
 template struct DevBuf<VertexRec>;
 
@@ -129,33 +133,114 @@ int Scattering::setHenyeyGreensteinScattering(double g, uint32_t m) {
     return SSTMC_NO_ERROR;
 }
 
+// Tabular scattering (Scattering.cuh:272-429), on the host in FP64: the raw sampling pdf |f|^2
+// (times sin(theta) in 3D, theta = i pi / (n-1)) is normalised to sum 1; the cross section
+// (sum / n) 2 pi (times pi in 3D) normalises the amplitude; the inverse cdf of entry i is the
+// first cdf index at or above i / (n-1) (lower_bound), scaled to the angle range; the evaluation
+// pdf is |f|^2 (times sin(theta) in 3D) of the normalised amplitude.
+int Scattering::setTabularScattering(const double* f, uint32_t n, uint32_t m) {
+    if (m >= (uint32_t)kMaxMaterials || n < 2 || !f) return SSTMC_INVALID_CONFIG;
+    const bool d2 = v_.dims == 2;
+    std::vector<double> pdf(n), cdf(n);
+    double sum = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double th = ((double)i / (double)(n - 1)) * kPi;
+        pdf[i] = (f[2 * i] * f[2 * i] + f[2 * i + 1] * f[2 * i + 1]) * (d2 ? 1.0 : std::sin(th));
+        sum += pdf[i];
+    }
+    if (!(sum > 0) || !std::isfinite(sum)) return SSTMC_INVALID_CONFIG;
+    double acc = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        pdf[i] *= 1.0 / sum;
+        acc += pdf[i];
+        cdf[i] = acc;
+    }
+    const double cross = (sum / (double)n) * 2.0 * kPi * (d2 ? 1.0 : kPi);
+    const double scale = std::sqrt(1.0 / cross);
+    const double range = d2 ? 2.0 * kPi : kPi;
+    tab_.n[m] = n;
+    tab_.off[m] = (uint32_t)tabF_.size();
+    rawOff_[m] = (uint32_t)tabRaw_.size();
+    tabRaw_.insert(tabRaw_.end(), f, f + 2 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const double th = ((double)i / (double)(n - 1)) * kPi;
+        const double re = f[2 * i] * scale, im = f[2 * i + 1] * scale;
+        const double a = std::hypot(re, im);
+        const double pe = (re * re + im * im) * (d2 ? 1.0 : std::sin(th));
+        const double q = (double)i / (double)(n - 1);
+        const size_t lb = (size_t)(std::lower_bound(cdf.begin(), cdf.end(), q) - cdf.begin());
+        tabF_.push_back(make_float4((float)a, (float)re, (float)im, (float)pe));
+        tabD_.push_back(make_double4(a, re, im, pe));
+        tabPh_.push_back(a > 0 ? std::atan2(im, re) / (2.0 * kPi) : 0.0);
+        tabIcdf_.push_back((float)((double)lb * (range / (double)(n - 1))));
+    }
+    v_.type[m] = SSTMC_SCATTERING_TABULAR;
+    v_.g[m] = 0.f;
+    v_.gd[m] = 0.0;
+    defined_[m] = true;
+    v_.allIsotropic = isIsotropicScattering() ? 1 : 0;
+    return SSTMC_NO_ERROR;
+}
+
+int Scattering::upload() {
+    if (tabF_.empty()) return SSTMC_NO_ERROR;
+    int e;
+    if ((e = dTabF_.upload(tabF_.data(), tabF_.size())) || (e = dTabD_.upload(tabD_.data(), tabD_.size())) ||
+        (e = dTabPh_.upload(tabPh_.data(), tabPh_.size())) || (e = dTabIcdf_.upload(tabIcdf_.data(), tabIcdf_.size())))
+        return e;
+    tab_.f = dTabF_.p;
+    tab_.d = dTabD_.p;
+    tab_.ph = dTabPh_.p;
+    tab_.icdf = dTabIcdf_.p;
+    if ((e = dTab_.upload(&tab_, 1))) return e;
+    v_.tab = dTab_.p;
+    return SSTMC_NO_ERROR;
+}
+
 bool Scattering::isIsotropicScattering() const {
     for (int m = 0; m < kMaxMaterials; ++m) {
         if (!defined_[m]) continue;
+        if (v_.type[m] == SSTMC_SCATTERING_TABULAR) return false;   // isIsotropicScatteringTabular
         if (v_.type[m] == SSTMC_SCATTERING_HG && !(std::fabs(v_.gd[m]) < 0.0001)) return false;
     }
     return true;
 }
 
 bool Scattering::isUniform(uint32_t n) const {
-    for (uint32_t m = 2; m < n; ++m)
+    for (uint32_t m = 2; m < n; ++m) {
         if (v_.type[m] != v_.type[1] || (v_.type[m] == SSTMC_SCATTERING_HG && v_.gd[m] != v_.gd[1])) return false;
+        if (v_.type[m] == SSTMC_SCATTERING_TABULAR &&
+            (tab_.n[m] != tab_.n[1] ||
+             !std::equal(tabRaw_.begin() + rawOff_[m], tabRaw_.begin() + rawOff_[m] + 2 * (size_t)tab_.n[m],
+                         tabRaw_.begin() + rawOff_[1])))
+            return false;
+    }
     return true;
 }
 
 bool Scattering::isExactlyIsotropic(uint32_t n) const {
     for (uint32_t m = 1; m < n; ++m)
-        if (!(v_.type[m] == SSTMC_SCATTERING_ISOTROPIC || v_.gd[m] == 0.0)) return false;
+        if (!(v_.type[m] == SSTMC_SCATTERING_ISOTROPIC || (v_.type[m] == SSTMC_SCATTERING_HG && v_.gd[m] == 0.0)))
+            return false;
     return true;
 }
 
-double Scattering::amplitude(uint32_t m, double c) const {
-    double iso = v_.dims == 3 ? 0.282094791773878143 : 0.398942280401432678;
-    if (v_.type[m] == SSTMC_SCATTERING_ISOTROPIC) return iso;
+void Scattering::amplitude(uint32_t m, double c, double* re, double* im) const {
+    *im = 0.0;
+    if (v_.type[m] == SSTMC_SCATTERING_ISOTROPIC) {
+        *re = v_.dims == 3 ? 0.282094791773878143 : 0.398942280401432678;
+        return;
+    }
+    if (v_.type[m] == SSTMC_SCATTERING_TABULAR) {
+        const double4 e = tabD_[tab_.off[m] + tabularEntry(tab_.n[m], v_.dims, c)];
+        *re = e.y;
+        *im = e.z;
+        return;
+    }
     double g = v_.gd[m];
     double up = v_.dims == 3 ? (1.0 - g * g) / (4.0 * kPi) : (1.0 - g * g) / (2.0 * kPi);
     double den = std::fma(-2.0 * g, c, 1.0 + g * g);
-    return std::sqrt(v_.dims == 3 ? up / (den * std::sqrt(den)) : up / den);
+    *re = std::sqrt(v_.dims == 3 ? up / (den * std::sqrt(den)) : up / den);
 }
 
 // ====================================================================== Sources
@@ -385,16 +470,17 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
     if ((*err = buildWalkGrid())) return;
     if ((*err = allocate())) return;
     if (needF) {
-        std::vector<float> ff((size_t)L_ * V_);
-        std::vector<double> fd((size_t)L_ * V_);
+        std::vector<float2> ff((size_t)L_ * V_);
+        std::vector<double2> fd((size_t)L_ * V_);
         const std::vector<double>& dl = il_.hostPoints();
         const std::vector<double>& dv = vw_.hostPoints();
         for (uint32_t v = 0; v < V_; ++v)
             for (uint32_t l = 0; l < L_; ++l) {
                 double c = dl[3 * l] * dv[3 * v] + dl[3 * l + 1] * dv[3 * v + 1] + dl[3 * l + 2] * dv[3 * v + 2];
-                double f = sc_.amplitude(1, c);
-                fd[(size_t)v * L_ + l] = f;
-                ff[(size_t)v * L_ + l] = (float)f;
+                double fr, fi;
+                sc_.amplitude(1, c, &fr, &fi);
+                fd[(size_t)v * L_ + l] = make_double2(fr, fi);
+                ff[(size_t)v * L_ + l] = make_float2((float)fr, (float)fi);
             }
         if ((*err = Ff_.upload(ff.data(), ff.size()))) return;
         if ((*err = Fd_.upload(fd.data(), fd.size()))) return;

@@ -234,6 +234,28 @@ def mcx_validation(t, D, MFP, R, h, dims, viSep, batch=10000, full_iteration=1, 
     }
 
 
+def mudiff_cbs(amp, deg, x, y, mfp, batch: int = 1000, renderings: int = 1, cbs: bool = True, seed: int = 1) -> dict:
+    """The field-output configuration of the reference's examples/mudiffCBSValidation2D.m (lines 10-57,
+    96-117): a 2D single-cell slab of mean free path `mfp` with the tabular amplitude function
+    `amp` in both materials, one far-field illumination along +y and far-field views at the
+    angles `deg` (radians) from +y, with or without coherent backscattering."""
+    amp = np.asarray(amp, np.complex128).ravel()
+    deg = np.asarray(deg, np.float64).ravel()
+    sigs = np.array([0.0, 1.0 / float(mfp)])
+    return {
+        "simulation": {"batchSize": batch, "fullIteration": 1, "renderingsNum": renderings, "seed": seed,
+                       "lambda": 1.0},
+        "medium": {"type": "Heterogeneous", "xAxis": np.asarray(x, np.float64).ravel(),
+                   "yAxis": np.asarray(y, np.float64).ravel(), "materialGrid": np.ones((1, 1), np.uint32),
+                   "sigs": sigs, "siga": sigs * 0},
+        "scattering": {"type": ["Tabular", "Tabular"], "amplitudeFunction": [amp, amp]},
+        "illumination": {"type": "FarField", "direction": {"x": [0.0], "y": [1.0]}},
+        "view": {"type": "FarField", "direction": {"x": np.sin(deg), "y": np.cos(deg)}},
+        "sampler": _static_sampler(2),
+        "tracer": {"type": "nee", "isCBS": bool(cbs)},
+    }
+
+
 def serving_stream(n_requests: int, seed: int = 7, max_paths_log2: int = 22, max_lv: int = 512):
     """Mixed request stream: (arrival_offset_s, dims, config) tuples.
 

@@ -83,3 +83,41 @@ def cbs_backscatter(paths=256, n=12, nd=6, g=0.8, seed=9, point=False):
     cfg["scattering"]["amplitudeFunction"] = [g] * 4
     cfg["tracer"] = {"type": "nee", "isCBS": True}
     return cfg
+
+
+def smooth_table(n, dims, k=0):
+    """A smooth complex amplitude over the scattering angle ([0, pi] in 3D, [0, 2 pi] in 2D),
+    forward-peaked, with a k-dependent shape and phase (k picks one of several materials)."""
+    th = np.arange(n) / (n - 1) * (np.pi if dims == 3 else 2 * np.pi)
+    mag = 1.0 + 0.6 * np.cos(th) + 0.15 * (k + 1) * np.cos(2 * th) + 1.2 * np.exp(-(th / 0.4) ** 2)
+    if dims == 2:
+        mag = mag + 1.2 * np.exp(-((th - 2 * np.pi) / 0.4) ** 2)
+    return mag * np.exp(1j * (0.5 * (k + 1) * np.sin(th) + 0.3 * k))
+
+
+def with_tabular(cfg, dims, uniform=False, mixed=False):
+    """The same configuration with tabular scattering in every material (one shared table when
+    uniform, else one per material); mixed keeps material 2's original phase function."""
+    sc = cfg["scattering"]
+    types = sc["type"] if isinstance(sc["type"], list) else [sc["type"]]
+    nm = len(types)
+    amps = list(sc.get("amplitudeFunction", [0.0] * nm))
+    new_t, new_a = [], []
+    for m in range(nm):
+        if mixed and m == 2:
+            new_t.append(types[m])
+            new_a.append(amps[m])
+        else:
+            new_t.append("Tabular")
+            new_a.append(smooth_table(601 + (0 if uniform else 37 * m), dims, 0 if uniform else m))
+    cfg["scattering"] = {"type": new_t, "amplitudeFunction": new_a}
+    return cfg
+
+
+def tabular_mudiff_small(paths=256, nview=24, cbs=True, seed=13):
+    """The mudiffCBSValidation2D.m geometry (2D single-cell slab, one plane wave, views around the
+    circle) with a smooth synthetic table and fewer views."""
+    deg = np.linspace(0, 2 * np.pi, nview)
+    cfg = W.mudiff_cbs(smooth_table(721, 2), deg, [-20.0, 20.0], [-20.0, 20.0], 10.0, batch=paths, cbs=cbs,
+                       seed=seed)
+    return cfg

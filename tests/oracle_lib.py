@@ -152,6 +152,30 @@ def phase(cfg, dims, m, cosang, pdf=False):
     return lib().oracle_phase_amplitude(cc.ptr(), m, float(cosang), 1 if pdf else 0)
 
 
+def amplitude_complex(cfg, dims, m, cosangs):
+    """Complex amplitude of material m at each cosine."""
+    cc = CConfig(cfg, dims)
+    out = np.zeros(2)
+    res = []
+    for c in np.atleast_1d(cosangs):
+        rc = lib().oracle_phase_amplitude_complex(cc.ptr(), C.c_uint32(m), C.c_double(float(c)), _p(out))
+        if rc:
+            raise RuntimeError("oracle_phase_amplitude_complex failed with code %d" % rc)
+        res.append(out[0] + 1j * out[1])
+    return np.array(res)
+
+
+def tabular_table(cfg, dims, m):
+    """Material m's tabular table as the oracle builds it: (f complex, evaluation pdf, icdf angles)."""
+    cc = CConfig(cfg, dims)
+    L = lib()
+    L.oracle_tabular_table.restype = C.c_uint32
+    n = L.oracle_tabular_table(cc.ptr(), C.c_uint32(m), None, None, None)
+    f = np.zeros(2 * n); pdf = np.zeros(n); icdf = np.zeros(n)
+    L.oracle_tabular_table(cc.ptr(), C.c_uint32(m), _p(f), _p(pdf), _p(icdf))
+    return f[0::2] + 1j * f[1::2], pdf, icdf
+
+
 def farfield_table(cfg, dims, l):
     cc = CConfig(cfg, dims)
     out = np.zeros(16)

@@ -79,6 +79,9 @@ def test_python_mirror_messages():
     bad = dict(base); bad["sampler"] = dict(base["sampler"]); bad["sampler"]["D"] = [0.0, -1.0]
     with pytest.raises(ValueError, match="non-negative"):
         config.CConfig(bad, 2)
-    bad = dict(base); bad["scattering"] = {"type": ["Tabular", "Tabular"]}
+    bad = dict(base); bad["scattering"] = {"type": ["vMF", "vMF"]}
     with pytest.raises(ValueError, match="Unsupported scattering"):
+        config.CConfig(bad, 2)
+    bad = dict(base); bad["scattering"] = {"type": ["Tabular", "Tabular"], "amplitudeFunction": [[1.0], [1.0]]}
+    with pytest.raises(ValueError, match="Tabular amplitude function of material 0"):
         config.CConfig(bad, 2)
