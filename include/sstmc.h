@@ -55,7 +55,8 @@ enum {
 enum { SSTMC_MEDIUM_HETEROGENEOUS = 1 };                 /* config.medium.type */
 enum { SSTMC_SCATTERING_ISOTROPIC = 1,                    /* config.scattering.type{m} */
        SSTMC_SCATTERING_TABULAR = 2,
-       SSTMC_SCATTERING_HG = 3 };                         /* values follow ScatteringNS::ScatteringType */
+       SSTMC_SCATTERING_HG = 3,
+       SSTMC_SCATTERING_VMF = 4 };                        /* values follow ScatteringNS::ScatteringType */
 enum { SSTMC_SOURCE_NONE = 0,                             /* config.illumination/view.type */
        SSTMC_SOURCE_POINT = 1,
        SSTMC_SOURCE_FARFIELD = 2 };
@@ -96,7 +97,7 @@ typedef struct sstmc_medium {
 /* config.scattering (MatlabInterface.cu:712-888): one entry per material. */
 typedef struct sstmc_scattering {
     uint32_t materialNum;
-    const int32_t* type;           /* SSTMC_SCATTERING_ISOTROPIC, _TABULAR or _HG */
+    const int32_t* type;           /* SSTMC_SCATTERING_ISOTROPIC, _TABULAR, _HG or _VMF */
     const double* g;               /* HG anisotropy per material (ignored otherwise) */
     /* Tabular amplitude functions (Scattering.cuh:272-429): tabularSize[m] complex samples of
      * the amplitude over the scattering angle, equally spaced on [0, pi] (3D) or [0, 2 pi] (2D),
@@ -104,6 +105,14 @@ typedef struct sstmc_scattering {
      * (re, im) doubles. NULL when no material is tabular. */
     const uint32_t* tabularSize;
     const double* tabular;
+    /* von Mises-Fisher mixtures (Scattering.cuh:613-884): vmfMixtures[m] mixtures for each
+     * material m with vmfMixtures[m] > 0; their mixtureMu (kappa), mixtureC and mixtureAlpha
+     * concatenated in material order. NULL when no material is vMF. In 2D a vMF material scatters
+     * isotropically (Scattering.cuh:898-903, 1094-1099). */
+    const uint32_t* vmfMixtures;
+    const double* vmfMu;
+    const double* vmfC;
+    const double* vmfAlpha;
 } sstmc_scattering;
 
 /* config.illumination / config.view (MatlabInterface.cu:1062-1215). */

@@ -250,6 +250,74 @@ struct Scatter {
         std::vector<double> pdf, icdf;
     };
     std::vector<Table> tab;
+    // von Mises-Fisher mixtures (Scattering.cuh:613-884): per mixture kappa and C + log(alpha);
+    // per pair i + M j the summed concentration, its log normalisation, the normalised weight and
+    // the exclusive cumulative weight
+    struct Vmf {
+        std::vector<double> kappa, c, pairMu, pairLogNorm, pairPdf, pairCdf;
+        bool iso = true;
+    };
+    std::vector<Vmf> vmf;
+
+    // vMFnormalizationLog in 3D (SpecialFunctions.cuh:187-201): log(kappa / (4 pi sinh kappa))
+    // with the reference's small- and large-kappa branches.
+    static double vmfNormLog(double x) {
+        if (x < EPS) return -std::log(4.0 * PI);
+        if (x < 5.0) return std::log(x) - std::log(4.0 * PI * std::sinh(x));
+        return std::log(x) - std::log(2.0 * PI) - x;
+    }
+    void setVmf(uint32_t m, const double* mu, const double* C, const double* alpha, uint32_t M) {
+        Vmf& v = vmf[m];
+        std::vector<double> w(M * M);
+        double sum = 0;
+        for (uint32_t i = 0; i < M; ++i) {
+            v.kappa.push_back(mu[i]);
+            v.c.push_back(C[i] + std::log(alpha[i]));
+            if (std::fabs(mu[i]) > 0.0001) v.iso = false;
+        }
+        for (uint32_t k = 0; k < M * M; ++k) {
+            uint32_t i = k % M, j = k / M;
+            v.pairMu.push_back(mu[i] + mu[j]);
+            v.pairLogNorm.push_back(vmfNormLog(mu[i] + mu[j]));
+            w[k] = std::exp(C[i] + std::log(alpha[i]) + C[j] + std::log(alpha[j]) - vmfNormLog(mu[i] + mu[j]));
+            sum += w[k];
+        }
+        double acc = 0;
+        for (uint32_t k = 0; k < M * M; ++k) {
+            v.pairPdf.push_back(w[k] / sum);
+            v.pairCdf.push_back(acc);
+            acc += w[k] / sum;
+        }
+    }
+    // binarySearchKernel (Simulation.cuh:325-362)
+    static size_t cdfSearch(const std::vector<double>& a, double x) {
+        size_t r = a.size() - 1, l = 0;
+        if (a[0] > x) return 0;
+        if (a[r] < x) return r;
+        while (l <= r) {
+            size_t m = l + (r - l) / 2;
+            if (m == 0) return 0;
+            if (a[m] == x) return m;
+            if (a[m - 1] <= x && a[m] > x) return m - 1;
+            if (a[m] < x) l = m + 1;
+            else r = m - 1;
+        }
+        return 0;
+    }
+    // random_vMF_cosine in 3D (VectorType.cuh:169-205): Wood's rejection sampler
+    static double vmfCosine(double kappa, Stream& s) {
+        double b = -kappa + std::hypot(kappa, 1.0);
+        double x0 = (1.0 - b) / (1.0 + b);
+        double sc = kappa * x0 + 2.0 * std::log(1.0 - x0 * x0);
+        double w, t, u;
+        do {
+            double z = s.uniform();
+            u = s.uniform();
+            w = (1.0 - (1.0 + b) * z) / (1.0 - (1.0 - b) * z);
+            t = kappa * w + 2.0 * std::log(1.0 - x0 * w) - sc;
+        } while (t < std::log(u));
+        return w;
+    }
 
     // setTabularScattering (Scattering.cuh:272-429): n complex samples over [0, pi] (3D) or
     // [0, 2 pi] (2D). getSamplePdfRaw (:208-222): |f|^2, times sin(theta) in 3D; the sum
@@ -290,16 +358,27 @@ struct Scatter {
     double isoAmp() const { return dims == 3 ? 0.282094791773878143 : 0.398942280401432678; }
     double isoPdf() const { return dims == 3 ? 0.079577471545947668 : 0.159154943091895336; }
     double pdf(uint32_t m, double c) const {
-        if (type[m] == SSTMC_SCATTERING_ISOTROPIC) return isoPdf();
+        if (type[m] == SSTMC_SCATTERING_ISOTROPIC || (type[m] == SSTMC_SCATTERING_VMF && dims == 2)) return isoPdf();
         if (type[m] == SSTMC_SCATTERING_TABULAR) return tab[m].pdf[tabEntry(m, c)];
+        if (type[m] == SSTMC_SCATTERING_VMF) {   // Scattering.cuh:1003-1017
+            const Vmf& v = vmf[m];
+            double p = 0;
+            for (size_t k = 0; k < v.pairMu.size(); ++k) p += v.pairPdf[k] * std::exp(c * v.pairMu[k] + v.pairLogNorm[k]);
+            return p;
+        }
         double gg = g[m];
         double up = dims == 3 ? (1.0 - gg * gg) / (4.0 * PI) : 2.0 * ((1.0 - gg * gg) / (4.0 * PI));
         double den = std::fma(-2.0 * gg, c, 1.0 + gg * gg);
         return dims == 3 ? up / (den * std::sqrt(den)) : up / den;
     }
     cplx amp(uint32_t m, double c) const {
-        if (type[m] == SSTMC_SCATTERING_ISOTROPIC) return cplx(isoAmp());
+        if (type[m] == SSTMC_SCATTERING_ISOTROPIC || (type[m] == SSTMC_SCATTERING_VMF && dims == 2)) return cplx(isoAmp());
         if (type[m] == SSTMC_SCATTERING_TABULAR) return tab[m].f[tabEntry(m, c)];
+        if (type[m] == SSTMC_SCATTERING_VMF) {   // Scattering.cuh:932-945
+            double a = 0;
+            for (size_t i = 0; i < vmf[m].kappa.size(); ++i) a += std::exp(c * vmf[m].kappa[i] + vmf[m].c[i]);
+            return cplx(a);
+        }
         return cplx(std::sqrt(pdf(m, c)));
     }
     // randomDirection (VectorType.cuh:316-362).
@@ -317,9 +396,13 @@ struct Scatter {
     // sampleDeviceImplementation (Scattering.cuh:1091-1109) with sampleCosine (:1009-1063)
     // and vector_rotation (VectorType.cuh:261-282).
     V3 sample(uint32_t m, const V3& prev, Stream& s) const {
-        if (type[m] == SSTMC_SCATTERING_ISOTROPIC) return randomDirection(s);
+        if (type[m] == SSTMC_SCATTERING_ISOTROPIC || (type[m] == SSTMC_SCATTERING_VMF && dims == 2))
+            return randomDirection(s);
         double gg = g[m], c;
-        if (type[m] == SSTMC_SCATTERING_TABULAR) {
+        if (type[m] == SSTMC_SCATTERING_VMF) {   // Scattering.cuh:1066-1077
+            size_t j = cdfSearch(vmf[m].pairCdf, s.uniform());
+            c = vmfCosine(vmf[m].pairMu[j], s);
+        } else if (type[m] == SSTMC_SCATTERING_TABULAR) {
             // inverse cdf at round_num(u n) (Scattering.cuh:1030-1037)
             size_t n = tab[m].icdf.size();
             double r = std::nearbyint(s.uniform() * (double)n);
@@ -771,15 +854,19 @@ int build(const sstmc_config* c, Problem& pb) {
     pb.sc.type.assign(nmat, 0);
     pb.sc.g.assign(nmat, 0.0);
     pb.sc.tab.assign(nmat, Scatter::Table());
-    std::vector<size_t> toff(nm, 0);
-    for (uint32_t m = 0, acc = 0; m < nm; ++m) {
+    pb.sc.vmf.assign(nmat, Scatter::Vmf());
+    std::vector<size_t> toff(nm, 0), voff(nm, 0);
+    for (uint32_t m = 0, acc = 0, accv = 0; m < nm; ++m) {
         toff[m] = acc;
+        voff[m] = accv;
         if (c->scattering.tabularSize) acc += 2 * c->scattering.tabularSize[m];
+        if (c->scattering.vmfMixtures) accv += c->scattering.vmfMixtures[m];
     }
     for (uint32_t m = 0; m < nmat; ++m) {
         uint32_t src = nm == 1 ? 0 : m;
         int ty = c->scattering.type[src];
-        if (ty != SSTMC_SCATTERING_ISOTROPIC && ty != SSTMC_SCATTERING_HG && ty != SSTMC_SCATTERING_TABULAR)
+        if (ty != SSTMC_SCATTERING_ISOTROPIC && ty != SSTMC_SCATTERING_HG && ty != SSTMC_SCATTERING_TABULAR &&
+            ty != SSTMC_SCATTERING_VMF)
             return SSTMC_NOT_SUPPORTED;
         pb.sc.type[m] = ty;
         pb.sc.g[m] = (ty == SSTMC_SCATTERING_HG && c->scattering.g) ? c->scattering.g[src] : 0.0;
@@ -788,11 +875,19 @@ int build(const sstmc_config* c, Problem& pb) {
                 return SSTMC_INVALID_CONFIG;
             pb.sc.setTabular(m, c->scattering.tabular + toff[src], c->scattering.tabularSize[src]);
         }
+        if (ty == SSTMC_SCATTERING_VMF) {
+            const sstmc_scattering& q = c->scattering;
+            if (!q.vmfMixtures || !q.vmfMu || !q.vmfC || !q.vmfAlpha || q.vmfMixtures[src] == 0)
+                return SSTMC_INVALID_CONFIG;
+            pb.sc.setVmf(m, q.vmfMu + voff[src], q.vmfC + voff[src], q.vmfAlpha + voff[src], q.vmfMixtures[src]);
+        }
     }
-    pb.sc.allIsotropic = true;   // tabular never counts as isotropic (Scattering.cuh:478-482)
+    // tabular never counts as isotropic (Scattering.cuh:478-482); vMF when every |kappa| <= 1e-4
+    pb.sc.allIsotropic = true;
     for (uint32_t m = 0; m < nmat; ++m)
         if (!(pb.sc.type[m] == SSTMC_SCATTERING_ISOTROPIC ||
-              (pb.sc.type[m] == SSTMC_SCATTERING_HG && std::fabs(pb.sc.g[m]) < 0.0001)))
+              (pb.sc.type[m] == SSTMC_SCATTERING_HG && std::fabs(pb.sc.g[m]) < 0.0001) ||
+              (pb.sc.type[m] == SSTMC_SCATTERING_VMF && pb.sc.vmf[m].iso)))
             pb.sc.allIsotropic = false;
     // medium
     const sstmc_medium& md = c->medium;

@@ -15,9 +15,10 @@
 //   medium: type "Heterogeneous", xAxis, yAxis[, zAxis], materialGrid (flat, x fastest, or nested
 //           [x][y][z]) or materialGridFile (raw bytes, x fastest; materialGridType "uint8"|"uint32"),
 //           sigs, siga
-//   scattering: type ("Isotropic" | "Tabular" | "HenyeyGreenstein" | "HG", or a list),
-//               amplitudeFunction (HG: g; Tabular: {"re": [...], "im": [...]}, [[re, im], ...] or
-//               a real list, per material)
+//   scattering: type ("Isotropic" | "Tabular" | "HenyeyGreenstein" | "HG" | "VonMisesFisher" |
+//               "vMF", or a list), amplitudeFunction (HG: g; Tabular: {"re": [...], "im": [...]},
+//               [[re, im], ...] or a real list; vMF: {"mixtureMu", "mixtureC", "mixtureAlpha"};
+//               per material)
 //   illumination / view: type "FarField" (direction{x,y[,z]}) | "PointSource" (location{...}),
 //                        isAreaSource
 //   sampler: type "TemporalSampler", t, D, U{x,y,z}, isTemporalAmplitude
@@ -163,10 +164,11 @@ void fill(HostConfig& h, const Value& cfg, uint32_t dims) {
         for (size_t k = 0; k < types.size(); ++k) ampv[k] = perMaterial ? &av.arr[k] : &av;
     }
     std::vector<int32_t> tcode;
-    std::vector<double> g, tab;
-    std::vector<uint32_t> tsize;
+    std::vector<double> g, tab, vmu, vc, valpha;
+    std::vector<uint32_t> tsize, vnum;
     for (size_t k = 0; k < types.size(); ++k) {
         const std::string mk = std::to_string(k);
+        vnum.push_back(0);
         if (types[k] == "Isotropic") {
             tcode.push_back(SSTMC_SCATTERING_ISOTROPIC);
             g.push_back(0.0);
@@ -175,6 +177,28 @@ void fill(HostConfig& h, const Value& cfg, uint32_t dims) {
             tcode.push_back(SSTMC_SCATTERING_HG);
             g.push_back(ampv[k] ? nums(*ampv[k])[0] : 0.0);
             tsize.push_back(0);
+        } else if (types[k] == "VonMisesFisher" || types[k] == "vMF") {
+            // mixture of von Mises-Fisher lobes (MatlabInterface.cu:827-866)
+            std::vector<double> mu, cc, al;
+            if (ampv[k] && ampv[k]->kind == Value::Object) {
+                if (ampv[k]->has("mixtureMu")) mu = nums(ampv[k]->at("mixtureMu"));
+                if (ampv[k]->has("mixtureC")) cc = nums(ampv[k]->at("mixtureC"));
+                if (ampv[k]->has("mixtureAlpha")) al = nums(ampv[k]->at("mixtureAlpha"));
+            }
+            if (mu.empty() || cc.size() != mu.size() || al.size() != mu.size())
+                throw ConfigError("vMF amplitude function of material " + mk +
+                                  " needs mixtureMu, mixtureC and mixtureAlpha of equal, non-zero length");
+            for (size_t i = 0; i < mu.size(); ++i)
+                if (!(al[i] > 0) || !std::isfinite(mu[i]) || !std::isfinite(cc[i]))
+                    throw ConfigError("vMF amplitude function of material " + mk +
+                                      ": mixtureAlpha must be positive and mixtureMu, mixtureC finite");
+            tcode.push_back(SSTMC_SCATTERING_VMF);
+            g.push_back(0.0);
+            tsize.push_back(0);
+            vnum.back() = (uint32_t)mu.size();
+            vmu.insert(vmu.end(), mu.begin(), mu.end());
+            vc.insert(vc.end(), cc.begin(), cc.end());
+            valpha.insert(valpha.end(), al.begin(), al.end());
         } else if (types[k] == "Tabular") {
             // complex amplitude samples over the scattering angle (Scattering.cuh:272-429)
             std::vector<double> re, im;
@@ -209,7 +233,8 @@ void fill(HostConfig& h, const Value& cfg, uint32_t dims) {
             }
         } else {
             throw ConfigError("Unsupported scattering type in material " + mk + ". Type entered: " + types[k] +
-                              ". Available scattering types are Isotropic, Tabular and HenyeyGreenstein (HG)");
+                              ". Available scattering types are Isotropic, Tabular, HenyeyGreenstein (HG) and "
+                              "VonMisesFisher (vMF)");
         }
     }
     const uint32_t nsc = (uint32_t)tcode.size();
@@ -222,6 +247,13 @@ void fill(HostConfig& h, const Value& cfg, uint32_t dims) {
         h.u.push_back(tsize);
         s.scattering.tabularSize = h.u.back().data();
         s.scattering.tabular = h.keep(tab);
+    }
+    if (!vmu.empty()) {
+        h.u.push_back(vnum);
+        s.scattering.vmfMixtures = h.u.back().data();
+        s.scattering.vmfMu = h.keep(vmu);
+        s.scattering.vmfC = h.keep(vc);
+        s.scattering.vmfAlpha = h.keep(valpha);
     }
 
     // medium (MatlabInterface.cu:890-1060)

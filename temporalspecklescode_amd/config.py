@@ -23,6 +23,7 @@ MEDIUM_HETEROGENEOUS = 1
 SCATTERING_ISOTROPIC = 1
 SCATTERING_TABULAR = 2
 SCATTERING_HG = 3
+SCATTERING_VMF = 4
 SOURCE_NONE, SOURCE_POINT, SOURCE_FARFIELD = 0, 1, 2
 SAMPLER_TEMPORAL = 1
 TRACER_NEE = 1
@@ -49,7 +50,7 @@ class Medium(C.Structure):
 
 class Scattering(C.Structure):
     _fields_ = [("materialNum", C.c_uint32), ("type", _I32P), ("g", _DP), ("tabularSize", _U32P),
-                ("tabular", _DP)]
+                ("tabular", _DP), ("vmfMixtures", _U32P), ("vmfMu", _DP), ("vmfC", _DP), ("vmfAlpha", _DP)]
 
 
 class Source(C.Structure):
@@ -132,29 +133,47 @@ class CConfig:
         elif len(types) == 1 and len(amps) > 1 and np.isscalar(amps[0]):
             amps = [amps]   # one material given its table as a plain list
         tcode, gvals, tsize, tables = [], [], [], []
+        vnum, vmu, vc, valpha = [], [], [], []
         for i, t in enumerate(types):
+            g, nt, nv = 0.0, 0, 0
             if t == "Isotropic":
-                tcode.append(SCATTERING_ISOTROPIC)
-                gvals.append(0.0)
-                tsize.append(0)
+                code = SCATTERING_ISOTROPIC
             elif t in ("HenyeyGreenstein", "HG"):
-                tcode.append(SCATTERING_HG)
-                gvals.append(float(np.asarray(amps[i]).ravel()[0]))
-                tsize.append(0)
+                code = SCATTERING_HG
+                g = float(np.asarray(amps[i]).ravel()[0])
             elif t == "Tabular":
                 # complex amplitude samples over the scattering angle (Scattering.cuh:272-429)
                 f = np.asarray(amps[i], dtype=np.complex128).ravel(order="F")
                 if f.size < 2 or not np.all(np.isfinite(f)) or not np.any(f != 0):
                     raise ValueError("Tabular amplitude function of material %d must hold at least two finite "
                                      "samples, not all zero" % i)
-                tcode.append(SCATTERING_TABULAR)
-                gvals.append(0.0)
-                tsize.append(f.size)
+                code, nt = SCATTERING_TABULAR, f.size
                 tables.append(np.stack([f.real, f.imag], axis=1).ravel())
+            elif t in ("VonMisesFisher", "vMF"):
+                # mixture of von Mises-Fisher lobes (MatlabInterface.cu:827-866)
+                a = amps[i] if isinstance(amps[i], dict) else {}
+                mu = np.asarray(a.get("mixtureMu", []), np.float64).ravel()
+                cc = np.asarray(a.get("mixtureC", []), np.float64).ravel()
+                al = np.asarray(a.get("mixtureAlpha", []), np.float64).ravel()
+                if mu.size == 0 or cc.size != mu.size or al.size != mu.size:
+                    raise ValueError("vMF amplitude function of material %d needs mixtureMu, mixtureC and "
+                                     "mixtureAlpha of equal, non-zero length" % i)
+                if not np.all(al > 0) or not np.all(np.isfinite(mu)) or not np.all(np.isfinite(cc)):
+                    raise ValueError("vMF amplitude function of material %d: mixtureAlpha must be positive and "
+                                     "mixtureMu, mixtureC finite" % i)
+                code, nv = SCATTERING_VMF, mu.size
+                vmu.append(mu)
+                vc.append(cc)
+                valpha.append(al)
             else:
                 raise ValueError(
                     "Unsupported scattering type in material %d. Type entered: %s. Available "
-                    "scattering types are Isotropic, Tabular and HenyeyGreenstein (HG)" % (i, t))
+                    "scattering types are Isotropic, Tabular, HenyeyGreenstein (HG) and VonMisesFisher (vMF)"
+                    % (i, t))
+            tcode.append(code)
+            gvals.append(g)
+            tsize.append(nt)
+            vnum.append(nv)
         n_sc = len(tcode)
         material_num = 2 if n_sc == 1 else n_sc
         ta = self._arr(tcode, np.int32)
@@ -165,6 +184,12 @@ class CConfig:
             sz = self._arr(tsize, np.uint32)
             s.scattering.tabularSize = sz.ctypes.data_as(_U32P)
             s.scattering.tabular, _ = self._dptr(np.concatenate(tables))
+        if vmu:
+            vn = self._arr(vnum, np.uint32)
+            s.scattering.vmfMixtures = vn.ctypes.data_as(_U32P)
+            s.scattering.vmfMu, _ = self._dptr(np.concatenate(vmu))
+            s.scattering.vmfC, _ = self._dptr(np.concatenate(vc))
+            s.scattering.vmfAlpha, _ = self._dptr(np.concatenate(valpha))
 
         # medium (MatlabInterface.cu:890-1060)
         md = cfg["medium"]

@@ -53,8 +53,11 @@ static int validate(const sstmc_config* c) {
     uint32_t nmat = sc.materialNum == 1 ? 2 : sc.materialNum;
     for (uint32_t m = 0; m < sc.materialNum; ++m) {
         if (sc.type[m] != SSTMC_SCATTERING_ISOTROPIC && sc.type[m] != SSTMC_SCATTERING_HG &&
-            sc.type[m] != SSTMC_SCATTERING_TABULAR)
+            sc.type[m] != SSTMC_SCATTERING_TABULAR && sc.type[m] != SSTMC_SCATTERING_VMF)
             return SSTMC_NOT_SUPPORTED;
+        if (sc.type[m] == SSTMC_SCATTERING_VMF &&
+            (!sc.vmfMixtures || !sc.vmfMu || !sc.vmfC || !sc.vmfAlpha || sc.vmfMixtures[m] == 0))
+            return SSTMC_INVALID_CONFIG;
         if (sc.type[m] == SSTMC_SCATTERING_HG && sc.g && !(std::fabs(sc.g[m]) < 1.0)) return SSTMC_INVALID_CONFIG;
         if (sc.type[m] == SSTMC_SCATTERING_TABULAR && (!sc.tabularSize || !sc.tabular || sc.tabularSize[m] < 2))
             return SSTMC_INVALID_CONFIG;
@@ -113,10 +116,12 @@ static int create(const sstmc_config* c, sstmc_handle** out) {
     if (sc.materialNum == 0 || sc.materialNum > (uint32_t)kMaxMaterials || !sc.type) return SSTMC_INVALID_CONFIG;
     uint32_t nmat = sc.materialNum == 1 ? 2 : sc.materialNum;
     h->scattering.reset(new Scattering(dims));
-    std::vector<size_t> tabOff(sc.materialNum, 0);   // start of material m's table in sc.tabular
-    for (uint32_t m = 0, acc = 0; m < sc.materialNum; ++m) {
+    std::vector<size_t> tabOff(sc.materialNum, 0), vmfOff(sc.materialNum, 0);   // starts in the arrays
+    for (uint32_t m = 0, acc = 0, accv = 0; m < sc.materialNum; ++m) {
         tabOff[m] = acc;
+        vmfOff[m] = accv;
         if (sc.tabularSize) acc += 2 * sc.tabularSize[m];
+        if (sc.vmfMixtures) accv += sc.vmfMixtures[m];
     }
     for (uint32_t m = 0; m < nmat; ++m) {
         uint32_t src = sc.materialNum == 1 ? 0 : m;
@@ -125,6 +130,9 @@ static int create(const sstmc_config* c, sstmc_handle** out) {
         else if (sc.type[src] == SSTMC_SCATTERING_HG) rc = h->scattering->setHenyeyGreensteinScattering(sc.g ? sc.g[src] : 0.0, m);
         else if (sc.type[src] == SSTMC_SCATTERING_TABULAR)
             rc = h->scattering->setTabularScattering(sc.tabular + tabOff[src], sc.tabularSize[src], m);
+        else if (sc.type[src] == SSTMC_SCATTERING_VMF)
+            rc = h->scattering->setVonMisesFisherScattering(sc.vmfMu + vmfOff[src], sc.vmfC + vmfOff[src],
+                                                            sc.vmfAlpha + vmfOff[src], sc.vmfMixtures[src], m);
         else return SSTMC_NOT_SUPPORTED;
         if (rc) return rc;
     }

@@ -120,27 +120,41 @@ struct MediumView {
 };
 
 // Scattering: Scattering.cuh:15-29, HG table :500-572.
-// Tabular amplitude functions (Scattering.cuh:191-482), in device memory: material m's n[m]
-// entries start at off[m], equally spaced over the scattering angle on [0, pi] (3D) or [0, 2 pi]
-// (2D); f is the amplitude normalised by the cross section, pdf the reference's evaluation pdf
-// (|f|^2, times sin of the angle in 3D), icdf the inverse cdf's angle for entry i of a uniform
-// draw. Kept out of ScatterView so that the kernels' by-value argument stays small.
-struct TabularTables {
+// Tabulated phase functions, in device memory (kept out of ScatterView so that the kernels'
+// by-value argument stays small).
+// Tabular (Scattering.cuh:191-482): material m's n[m] entries start at off[m], equally spaced
+// over the scattering angle on [0, pi] (3D) or [0, 2 pi] (2D); f is the amplitude normalised by
+// the cross section, pdf the reference's evaluation pdf (|f|^2, times sin of the angle in 3D),
+// icdf the inverse cdf's angle for entry i of a uniform draw.
+// von Mises-Fisher mixtures (Scattering.cuh:613-884): material m's vmfM[m] mixtures start at
+// vmfOff[m] (kappa and C + log alpha); its vmfM[m]^2 mixture pairs (i + M j) start at
+// vmfPairOff[m] (kappa_i + kappa_j, the log normalisation of that sum, the pair's weight and the
+// exclusive cumulative weight).
+struct PhaseTables {
     uint32_t n[kMaxMaterials];
     uint32_t off[kMaxMaterials];
     const float4* f;               // (|f|, Re f, Im f, pdf) in FP32
     const double4* d;              // the same in FP64
     const double* ph;              // arg f / (2 pi): the amplitude's phase in cycles
     const float* icdf;
+    uint32_t vmfM[kMaxMaterials];
+    uint32_t vmfOff[kMaxMaterials];
+    uint32_t vmfPairOff[kMaxMaterials];
+    const double* vmfKappa;
+    const double* vmfC;
+    const double* vmfPairMu;
+    const double* vmfPairLogNorm;
+    const double* vmfPairPdf;
+    const double* vmfPairCdf;
 };
 
 struct ScatterView {
     int dims;
     int allIsotropic;
-    int type[kMaxMaterials];       // 1 isotropic, 2 tabular, 3 HG
+    int type[kMaxMaterials];       // 1 isotropic, 2 tabular, 3 HG, 4 vMF mixture
     float g[kMaxMaterials];
     double gd[kMaxMaterials];
-    const TabularTables* tab;      // device tables when a material is tabular, else null
+    const PhaseTables* tab;        // device tables when a material is tabular or vMF, else null
 };
 
 // Entry of cosine c in an n-entry tabular amplitude function: the nearest tabulated angle,
@@ -151,6 +165,35 @@ __host__ __device__ inline uint32_t tabularEntry(uint32_t n, int dims, double c)
     const double x = a * (double)(n - 1) / (dims == 2 ? 6.28318530717958647692 : 3.14159265358979323846);
     const double r = rint(x);
     return r < 0 ? 0u : (r >= (double)n ? n - 1 : (uint32_t)r);
+}
+
+// 3D vMF mixture amplitude sum_i exp(c kappa_i + C_i + log alpha_i) (Scattering.cuh:932-945).
+__host__ __device__ inline double vmfAmplitude(const PhaseTables& t, uint32_t m, double c) {
+    double a = 0;
+    for (uint32_t i = t.vmfOff[m], e = t.vmfOff[m] + t.vmfM[m]; i < e; ++i) a += exp(c * t.vmfKappa[i] + t.vmfC[i]);
+    return a;
+}
+// 3D vMF mixture pdf sum_ij w_ij exp(c (kappa_i + kappa_j) + lognorm_ij) (Scattering.cuh:1003-1017).
+__host__ __device__ inline double vmfPdf(const PhaseTables& t, uint32_t m, double c) {
+    double p = 0;
+    for (uint32_t i = t.vmfPairOff[m], e = t.vmfPairOff[m] + t.vmfM[m] * t.vmfM[m]; i < e; ++i)
+        p += t.vmfPairPdf[i] * exp(c * t.vmfPairMu[i] + t.vmfPairLogNorm[i]);
+    return p;
+}
+// binarySearchKernel (Simulation.cuh:325-362): the interval of a sorted n-entry array holding x.
+__host__ __device__ inline uint32_t cdfSearch(const double* a, uint32_t n, double x) {
+    uint32_t r = n - 1, l = 0;
+    if (a[0] > x) return 0;
+    if (a[r] < x) return r;
+    while (l <= r) {
+        const uint32_t m = l + (r - l) / 2;
+        if (m == 0) return 0;
+        if (a[m] == x) return m;
+        if (a[m - 1] <= x && a[m] > x) return m - 1;
+        if (a[m] < x) l = m + 1;
+        else r = m - 1;
+    }
+    return 0;
 }
 
 // Sources: FarField (FarField.cuh:71-127) / PointSource (PointSource.cuh:176-216).

@@ -314,10 +314,12 @@ __device__ inline typename SingleT<R>::C singleAmp(const ScatterView& s, uint32_
     if (s.type[m] == 1) {
         f.x = DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678;
     } else if (s.type[m] == 2) {
-        const TabularTables& t = *s.tab;
+        const PhaseTables& t = *s.tab;
         const uint32_t i = t.off[m] + tabularEntry(t.n[m], DIMS, (double)c);
         if constexpr (sizeof(R) == 4) { f.x = t.f[i].y; f.y = t.f[i].z; }
         else { f.x = t.d[i].y; f.y = t.d[i].z; }
+    } else if (s.type[m] == 4) {
+        f.x = DIMS == 3 ? (R)vmfAmplitude(*s.tab, m, (double)c) : (R)0.398942280401432678;
     } else {
         R g = (R)s.gd[m];
         R up = DIMS == 3 ? ((R)1 - g * g) / (R)12.566370614359172954 : ((R)1 - g * g) / (R)6.283185307179586477;

@@ -149,16 +149,38 @@ __device__ inline f3 randomDirection(Rng& rng) {
     return mk(st * cp, st * sp, ct);
 }
 
+// random_vMF_cosine (VectorType.cuh:169-205), 3D: Wood's rejection sampler for the cosine of a
+// von Mises-Fisher law of concentration kappa, in FP64.
+__device__ inline double vmfCosine3(double kappa, Rng& rng) {
+    const double b = -kappa + hypot(kappa, 1.0);
+    const double x0 = (1.0 - b) / (1.0 + b);
+    const double sc = kappa * x0 + 2.0 * log(1.0 - x0 * x0);
+    double w, t, u;
+    do {
+        const double z = rng.uniform();
+        u = rng.uniform();
+        w = (1.0 - (1.0 + b) * z) / (1.0 - (1.0 - b) * z);
+        t = kappa * w + 2.0 * log(1.0 - x0 * w) - sc;
+    } while (t < log(u));
+    return w;
+}
+
 // sampleDeviceImplementation (Scattering.cuh:1091-1109): HG inverse CDF (:1039-1063) and
 // vector_rotation (VectorType.cuh:261-282) about the incoming direction.
 template <int DIMS>
 __device__ inline f3 sampleDirection(const ScatterView& s, uint32_t mat, const f3& prev, Rng& rng) {
-    if (s.type[mat] == 1) return randomDirection<DIMS>(rng);
+    if (s.type[mat] == 1 || (DIMS == 2 && s.type[mat] == 4)) return randomDirection<DIMS>(rng);
     float g = s.g[mat], c;
-    if (s.type[mat] == 2) {
+    if (s.type[mat] == 4) {
+        // vMF mixture: a pair by its cumulative weight, then its cosine (Scattering.cuh:1066-1077)
+        const PhaseTables& t = *s.tab;
+        const uint32_t j = t.vmfPairOff[mat] +
+                           cdfSearch(t.vmfPairCdf + t.vmfPairOff[mat], t.vmfM[mat] * t.vmfM[mat], (double)rng.uniform());
+        c = (float)vmfCosine3(t.vmfPairMu[j], rng);
+    } else if (s.type[mat] == 2) {
         // tabular: the inverse cdf at round(u n) (Scattering.cuh:1030-1037); u has 25 significant
         // bits, so u n is exact in FP64 and the entry is the one an FP64 evaluation picks
-        const TabularTables& t = *s.tab;
+        const PhaseTables& t = *s.tab;
         const uint32_t n = t.n[mat];
         const double x = rint((double)rng.uniform() * (double)n);
         const uint32_t i = x >= (double)n ? n - 1 : (uint32_t)x;

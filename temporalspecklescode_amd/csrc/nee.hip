@@ -81,26 +81,31 @@ template <int DIMS, typename R>
 __device__ inline R phasePdf(const ScatterView& s, uint32_t m, double c) {
     if (s.type[m] == 1) return DIMS == 3 ? (R)0.079577471545947668 : (R)0.159154943091895336;
     if (s.type[m] == 2) {   // tabular evaluation pdf (Scattering.cuh:971-980)
-        const TabularTables& t = *s.tab;
+        const PhaseTables& t = *s.tab;
         const uint32_t i = t.off[m] + tabularEntry(t.n[m], DIMS, c);
         if constexpr (sizeof(R) == 4) return t.f[i].w;
         else return t.d[i].w;
     }
+    if (s.type[m] == 4)     // vMF mixture: isotropic in 2D (Scattering.cuh:963-968)
+        return DIMS == 3 ? (R)vmfPdf(*s.tab, m, c) : (R)0.159154943091895336;
     return hgPdf<DIMS, R>(s.gd[m], c);
 }
 // Amplitude |f| at cosine c; a tabular amplitude's phase (in cycles) is added to *ph
-// (Scattering.cuh:907-916: the tabulated complex amplitude). TAB = false compiles the tabular
-// case out (no material is tabular), keeping its FP64 acos out of the view kernels' registers.
+// (Scattering.cuh:907-916: the tabulated complex amplitude). TAB = false compiles the tabular and
+// vMF cases out (no material uses them), keeping their FP64 code out of the view kernels'
+// registers.
 template <int DIMS, typename R, bool TAB = true>
 __device__ inline R phaseAmp(const ScatterView& s, uint32_t m, double c, double* ph) {
     if (s.type[m] == 1) return DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678;
     if (TAB && s.type[m] == 2) {
-        const TabularTables& t = *s.tab;
+        const PhaseTables& t = *s.tab;
         const uint32_t i = t.off[m] + tabularEntry(t.n[m], DIMS, c);
         *ph += t.ph[i];
         if constexpr (sizeof(R) == 4) return t.f[i].x;
         else return t.d[i].x;
     }
+    if (TAB && s.type[m] == 4)   // vMF mixture: isotropic in 2D (Scattering.cuh:898-903, 932-945)
+        return DIMS == 3 ? (R)vmfAmplitude(*s.tab, m, c) : (R)0.398942280401432678;
     return sqrtR<R>(hgPdf<DIMS, R>(s.gd[m], c));
 }
 

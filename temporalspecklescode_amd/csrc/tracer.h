@@ -78,6 +78,10 @@ public:
     // Scattering::setTabularScattering (Scattering.cuh:272-429): f holds n complex samples
     // (interleaved re, im) of the amplitude over the scattering angle.
     int setTabularScattering(const double* f, uint32_t n, uint32_t materialNum);
+    // Scattering::setVonMisesFisherScattering (Scattering.cuh:635-817): M mixtures with
+    // concentrations mu, log-scales C and weights alpha.
+    int setVonMisesFisherScattering(const double* mu, const double* C, const double* alpha, uint32_t M,
+                                    uint32_t materialNum);
     // copies the tabular tables to the device (after the last set*; no-op without tables)
     int upload();
     // Scattering::isIsotropicScattering (Scattering.cuh:1505-1527): HG with |g| < 1e-4 counts.
@@ -97,9 +101,13 @@ private:
     std::vector<double4> tabD_;
     std::vector<double> tabPh_, tabRaw_;   // tabRaw_: the input samples (uniformity test)
     std::vector<float> tabIcdf_;
+    std::vector<double> vmfKappa_, vmfC_, vmfRaw_, vmfPairMu_, vmfPairLogNorm_, vmfPairPdf_, vmfPairCdf_;
+    uint32_t vmfRawOff_[kMaxMaterials] = {};
+    bool vmfIso_[kMaxMaterials] = {};
+    DevBuf<double> dVmf_;   // the vMF arrays, one allocation
     uint32_t rawOff_[kMaxMaterials] = {};
-    TabularTables tab_{};               // sizes and offsets (host), device pointers after upload
-    DevBuf<TabularTables> dTab_;
+    PhaseTables tab_{};               // sizes and offsets (host), device pointers after upload
+    DevBuf<PhaseTables> dTab_;
     DevBuf<float4> dTabF_;
     DevBuf<double4> dTabD_;
     DevBuf<double> dTabPh_;

@@ -43,7 +43,7 @@ template struct DevBuf<float4>;
 template struct DevBuf<double4>;
 template struct DevBuf<float2>;
 template struct DevBuf<double2>;
-template struct DevBuf<TabularTables>;
+template struct DevBuf<PhaseTables>;
 template struct DevBuf<unsigned long long>;
 template struct DevBuf<VertexRec>;
 
@@ -182,16 +182,86 @@ int Scattering::setTabularScattering(const double* f, uint32_t n, uint32_t m) {
     return SSTMC_NO_ERROR;
 }
 
+// log of the 3D vMF normalisation kappa / (4 pi sinh kappa) (vMFnormalizationLog,
+// SpecialFunctions.cuh:187-201), with the reference's small- and large-kappa branches.
+static double vmfNormLog3(double x) {
+    if (x < 1e-12) return -std::log(4.0 * kPi);
+    if (x < 5.0) return std::log(x) - std::log(4.0 * kPi * std::sinh(x));
+    return std::log(x) - std::log(2.0 * kPi) - x;
+}
+
+// Scattering.cuh:635-817: per mixture kappa and C + log(alpha); per pair (i + M j) the summed
+// concentration, its log normalisation, the normalised weight exp(C_i + C_j + log alpha_i alpha_j -
+// lognorm) and the exclusive cumulative weight (thrust::exclusive_scan).
+int Scattering::setVonMisesFisherScattering(const double* mu, const double* C, const double* alpha, uint32_t M,
+                                            uint32_t m) {
+    if (m >= (uint32_t)kMaxMaterials || M == 0 || !mu || !C || !alpha) return SSTMC_INVALID_CONFIG;
+    for (uint32_t i = 0; i < M; ++i)
+        if (!(alpha[i] > 0) || !std::isfinite(mu[i]) || !std::isfinite(C[i])) return SSTMC_INVALID_CONFIG;
+    tab_.vmfM[m] = M;
+    tab_.vmfOff[m] = (uint32_t)vmfKappa_.size();
+    tab_.vmfPairOff[m] = (uint32_t)vmfPairMu_.size();
+    vmfRawOff_[m] = (uint32_t)vmfRaw_.size();
+    bool iso = true;
+    for (uint32_t i = 0; i < M; ++i) {
+        vmfKappa_.push_back(mu[i]);
+        vmfC_.push_back(C[i] + std::log(alpha[i]));
+        vmfRaw_.insert(vmfRaw_.end(), {mu[i], C[i], alpha[i]});
+        if (std::fabs(mu[i]) > 0.0001) iso = false;
+    }
+    std::vector<double> w(M * M);
+    double sum = 0;
+    for (uint32_t k = 0; k < M * M; ++k) {
+        const uint32_t i = k % M, j = k / M;
+        const double s = mu[i] + mu[j];
+        vmfPairMu_.push_back(s);
+        vmfPairLogNorm_.push_back(vmfNormLog3(s));
+        w[k] = std::exp(C[i] + std::log(alpha[i]) + C[j] + std::log(alpha[j]) - vmfNormLog3(s));
+        sum += w[k];
+    }
+    if (!(sum > 0) || !std::isfinite(sum)) return SSTMC_INVALID_CONFIG;
+    double acc = 0;
+    for (uint32_t k = 0; k < M * M; ++k) {
+        vmfPairPdf_.push_back(w[k] / sum);
+        vmfPairCdf_.push_back(acc);
+        acc += w[k] / sum;
+    }
+    vmfIso_[m] = iso;
+    v_.type[m] = SSTMC_SCATTERING_VMF;
+    v_.g[m] = 0.f;
+    v_.gd[m] = 0.0;
+    defined_[m] = true;
+    v_.allIsotropic = isIsotropicScattering() ? 1 : 0;
+    return SSTMC_NO_ERROR;
+}
+
 int Scattering::upload() {
-    if (tabF_.empty()) return SSTMC_NO_ERROR;
+    if (tabF_.empty() && vmfKappa_.empty()) return SSTMC_NO_ERROR;
     int e;
-    if ((e = dTabF_.upload(tabF_.data(), tabF_.size())) || (e = dTabD_.upload(tabD_.data(), tabD_.size())) ||
-        (e = dTabPh_.upload(tabPh_.data(), tabPh_.size())) || (e = dTabIcdf_.upload(tabIcdf_.data(), tabIcdf_.size())))
-        return e;
-    tab_.f = dTabF_.p;
-    tab_.d = dTabD_.p;
-    tab_.ph = dTabPh_.p;
-    tab_.icdf = dTabIcdf_.p;
+    if (!tabF_.empty()) {
+        if ((e = dTabF_.upload(tabF_.data(), tabF_.size())) || (e = dTabD_.upload(tabD_.data(), tabD_.size())) ||
+            (e = dTabPh_.upload(tabPh_.data(), tabPh_.size())) ||
+            (e = dTabIcdf_.upload(tabIcdf_.data(), tabIcdf_.size())))
+            return e;
+        tab_.f = dTabF_.p;
+        tab_.d = dTabD_.p;
+        tab_.ph = dTabPh_.p;
+        tab_.icdf = dTabIcdf_.p;
+    }
+    if (!vmfKappa_.empty()) {
+        const size_t nm = vmfKappa_.size(), np = vmfPairMu_.size();
+        std::vector<double> all;
+        all.reserve(2 * nm + 4 * np);
+        for (const std::vector<double>* v : {&vmfKappa_, &vmfC_, &vmfPairMu_, &vmfPairLogNorm_, &vmfPairPdf_, &vmfPairCdf_})
+            all.insert(all.end(), v->begin(), v->end());
+        if ((e = dVmf_.upload(all.data(), all.size()))) return e;
+        tab_.vmfKappa = dVmf_.p;
+        tab_.vmfC = dVmf_.p + nm;
+        tab_.vmfPairMu = dVmf_.p + 2 * nm;
+        tab_.vmfPairLogNorm = dVmf_.p + 2 * nm + np;
+        tab_.vmfPairPdf = dVmf_.p + 2 * nm + 2 * np;
+        tab_.vmfPairCdf = dVmf_.p + 2 * nm + 3 * np;
+    }
     if ((e = dTab_.upload(&tab_, 1))) return e;
     v_.tab = dTab_.p;
     return SSTMC_NO_ERROR;
@@ -201,6 +271,7 @@ bool Scattering::isIsotropicScattering() const {
     for (int m = 0; m < kMaxMaterials; ++m) {
         if (!defined_[m]) continue;
         if (v_.type[m] == SSTMC_SCATTERING_TABULAR) return false;   // isIsotropicScatteringTabular
+        if (v_.type[m] == SSTMC_SCATTERING_VMF && !vmfIso_[m]) return false;   // all |kappa| <= 1e-4
         if (v_.type[m] == SSTMC_SCATTERING_HG && !(std::fabs(v_.gd[m]) < 0.0001)) return false;
     }
     return true;
@@ -214,21 +285,33 @@ bool Scattering::isUniform(uint32_t n) const {
              !std::equal(tabRaw_.begin() + rawOff_[m], tabRaw_.begin() + rawOff_[m] + 2 * (size_t)tab_.n[m],
                          tabRaw_.begin() + rawOff_[1])))
             return false;
+        if (v_.type[m] == SSTMC_SCATTERING_VMF &&
+            (tab_.vmfM[m] != tab_.vmfM[1] ||
+             !std::equal(vmfRaw_.begin() + vmfRawOff_[m], vmfRaw_.begin() + vmfRawOff_[m] + 3 * (size_t)tab_.vmfM[m],
+                         vmfRaw_.begin() + vmfRawOff_[1])))
+            return false;
     }
     return true;
 }
 
 bool Scattering::isExactlyIsotropic(uint32_t n) const {
     for (uint32_t m = 1; m < n; ++m)
-        if (!(v_.type[m] == SSTMC_SCATTERING_ISOTROPIC || (v_.type[m] == SSTMC_SCATTERING_HG && v_.gd[m] == 0.0)))
+        if (!(v_.type[m] == SSTMC_SCATTERING_ISOTROPIC || (v_.type[m] == SSTMC_SCATTERING_HG && v_.gd[m] == 0.0) ||
+              (v_.type[m] == SSTMC_SCATTERING_VMF && v_.dims == 2)))
             return false;
     return true;
 }
 
 void Scattering::amplitude(uint32_t m, double c, double* re, double* im) const {
     *im = 0.0;
-    if (v_.type[m] == SSTMC_SCATTERING_ISOTROPIC) {
+    if (v_.type[m] == SSTMC_SCATTERING_ISOTROPIC || (v_.type[m] == SSTMC_SCATTERING_VMF && v_.dims == 2)) {
         *re = v_.dims == 3 ? 0.282094791773878143 : 0.398942280401432678;
+        return;
+    }
+    if (v_.type[m] == SSTMC_SCATTERING_VMF) {
+        double a = 0;
+        for (uint32_t i = tab_.vmfOff[m]; i < tab_.vmfOff[m] + tab_.vmfM[m]; ++i) a += std::exp(c * vmfKappa_[i] + vmfC_[i]);
+        *re = a;
         return;
     }
     if (v_.type[m] == SSTMC_SCATTERING_TABULAR) {

@@ -121,3 +121,40 @@ def tabular_mudiff_small(paths=256, nview=24, cbs=True, seed=13):
     cfg = W.mudiff_cbs(smooth_table(721, 2), deg, [-20.0, 20.0], [-20.0, 20.0], 10.0, batch=paths, cbs=cbs,
                        seed=seed)
     return cfg
+
+
+def _vmf_lobe_c(k):
+    """log of the factor that makes one lobe's |exp(k c + C)|^2 integrate to 1 over the sphere."""
+    x = 2.0 * k
+    return 0.5 * (-np.log(4 * np.pi) if x < 1e-12 else np.log(x) - np.log(4 * np.pi * np.sinh(x)))
+
+
+def _vmf_mixture(mu, alpha):
+    return {"mixtureMu": list(mu), "mixtureC": [_vmf_lobe_c(k) for k in mu], "mixtureAlpha": list(alpha)}
+
+
+VMF_MIXTURES = [
+    _vmf_mixture([1.5, 9.0], [0.6, 0.4]),
+    _vmf_mixture([0.0, 3.0, 12.0], [1.0, 0.5, 2.0]),
+    _vmf_mixture([4.0], [1.0]),
+    _vmf_mixture([2.0, 6.0], [0.3, 0.7]),
+]
+
+
+def with_vmf(cfg, uniform=False, mixed=False):
+    """The same configuration with von Mises-Fisher mixtures in every material (one shared mixture
+    when uniform, else one per material); mixed keeps material 2's original phase function."""
+    sc = cfg["scattering"]
+    types = sc["type"] if isinstance(sc["type"], list) else [sc["type"]]
+    nm = len(types)
+    amps = list(sc.get("amplitudeFunction", [0.0] * nm))
+    new_t, new_a = [], []
+    for m in range(nm):
+        if mixed and m == 2:
+            new_t.append(types[m])
+            new_a.append(amps[m])
+        else:
+            new_t.append("vMF")
+            new_a.append(VMF_MIXTURES[0 if uniform else m % len(VMF_MIXTURES)])
+    cfg["scattering"] = {"type": new_t, "amplitudeFunction": new_a}
+    return cfg

@@ -79,7 +79,7 @@ def test_python_mirror_messages():
     bad = dict(base); bad["sampler"] = dict(base["sampler"]); bad["sampler"]["D"] = [0.0, -1.0]
     with pytest.raises(ValueError, match="non-negative"):
         config.CConfig(bad, 2)
-    bad = dict(base); bad["scattering"] = {"type": ["vMF", "vMF"]}
+    bad = dict(base); bad["scattering"] = {"type": ["Mie", "Mie"]}
     with pytest.raises(ValueError, match="Unsupported scattering"):
         config.CConfig(bad, 2)
     bad = dict(base); bad["scattering"] = {"type": ["Tabular", "Tabular"], "amplitudeFunction": [[1.0], [1.0]]}

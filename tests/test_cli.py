@@ -74,7 +74,7 @@ def test_invalid_configs_match_python_messages(tmp_path, mutate):
     elif mutate == "batch0":
         c["simulation"]["batchSize"] = 0
     elif mutate == "scatter_type":
-        c["scattering"]["type"] = ["vMF"]
+        c["scattering"]["type"] = ["Mie"]
     elif mutate == "source_type":
         c["view"]["type"] = "GaussianBeam"
     elif mutate == "sizes":
