@@ -166,18 +166,19 @@ __device__ inline double vmfCosine3(double kappa, Rng& rng) {
 }
 
 // sampleDeviceImplementation (Scattering.cuh:1091-1109): HG inverse CDF (:1039-1063) and
-// vector_rotation (VectorType.cuh:261-282) about the incoming direction.
-template <int DIMS>
+// vector_rotation (VectorType.cuh:261-282) about the incoming direction. TAB = false compiles the
+// tabular and vMF samplers out (no material uses them).
+template <int DIMS, bool TAB = true>
 __device__ inline f3 sampleDirection(const ScatterView& s, uint32_t mat, const f3& prev, Rng& rng) {
-    if (s.type[mat] == 1 || (DIMS == 2 && s.type[mat] == 4)) return randomDirection<DIMS>(rng);
+    if (s.type[mat] == 1 || (TAB && DIMS == 2 && s.type[mat] == 4)) return randomDirection<DIMS>(rng);
     float g = s.g[mat], c;
-    if (s.type[mat] == 4) {
+    if (TAB && s.type[mat] == 4) {
         // vMF mixture: a pair by its cumulative weight, then its cosine (Scattering.cuh:1066-1077)
         const PhaseTables& t = *s.tab;
         const uint32_t j = t.vmfPairOff[mat] +
                            cdfSearch(t.vmfPairCdf + t.vmfPairOff[mat], t.vmfM[mat] * t.vmfM[mat], (double)rng.uniform());
         c = (float)vmfCosine3(t.vmfPairMu[j], rng);
-    } else if (s.type[mat] == 2) {
+    } else if (TAB && s.type[mat] == 2) {
         // tabular: the inverse cdf at round(u n) (Scattering.cuh:1030-1037); u has 25 significant
         // bits, so u n is exact in FP64 and the entry is the one an FP64 evaluation picks
         const PhaseTables& t = *s.tab;

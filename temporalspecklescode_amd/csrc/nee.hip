@@ -77,16 +77,16 @@ __device__ inline R hgPdf(double g, double c) {
     const R den = (R)fma(-2.0 * g, c, 1.0 + g * g);
     return DIMS == 3 ? up / (den * sqrtR<R>(den)) : up / den;
 }
-template <int DIMS, typename R>
+template <int DIMS, typename R, bool TAB = true>
 __device__ inline R phasePdf(const ScatterView& s, uint32_t m, double c) {
     if (s.type[m] == 1) return DIMS == 3 ? (R)0.079577471545947668 : (R)0.159154943091895336;
-    if (s.type[m] == 2) {   // tabular evaluation pdf (Scattering.cuh:971-980)
+    if (TAB && s.type[m] == 2) {   // tabular evaluation pdf (Scattering.cuh:971-980)
         const PhaseTables& t = *s.tab;
         const uint32_t i = t.off[m] + tabularEntry(t.n[m], DIMS, c);
         if constexpr (sizeof(R) == 4) return t.f[i].w;
         else return t.d[i].w;
     }
-    if (s.type[m] == 4)     // vMF mixture: isotropic in 2D (Scattering.cuh:963-968)
+    if (TAB && s.type[m] == 4)   // vMF mixture: isotropic in 2D (Scattering.cuh:963-968)
         return DIMS == 3 ? (R)vmfPdf(*s.tab, m, c) : (R)0.159154943091895336;
     return hgPdf<DIMS, R>(s.gd[m], c);
 }
@@ -248,7 +248,7 @@ __device__ inline typename Cx<R>::T cis(R amp, double phaseCycles) {
 
 // First-vertex work of one path, on a group of lpp lanes: path probabilities, illumination terms
 // a and A, single-scattering view terms b (and zero B rows when the path has one vertex).
-template <int DIMS, typename R>
+template <int DIMS, typename R, bool TAB>
 __device__ inline void nee_path(const MediumView& m, const R* sig, const ScatterView& sc, const SourceView& il,
                                 const SourceView& vw,
                                 const FramesView& fr, const VertexBatch& vb, const NeeParams& prm, const NeeOut& out,
@@ -320,7 +320,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         s1 += pr;
         if (multi) {
             d3 in = illumIn<DIMS>(il, l, x1c.pd);
-            s2 += pr * phasePdf<DIMS, R>(sc, m1, ddot<DIMS>(in, seg12));
+            s2 += pr * phasePdf<DIMS, R, TAB>(sc, m1, ddot<DIMS>(in, seg12));
         }
     }
     s1 = groupSum(s1, lpp);
@@ -345,7 +345,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
                 const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
                 const d3 pb = fr.temporalAmp ? x2t.pd : x2cd;
                 if (isoPointIl) fa[f] = isoAmp<DIMS, R>();
-                else fa[f] = phaseAmp<DIMS, R>(sc, m1, ddot<DIMS>(illumIn<DIMS>(il, l, pa), dnormalize<DIMS>(dsub(pb, pa))),
+                else fa[f] = phaseAmp<DIMS, R, TAB>(sc, m1, ddot<DIMS>(illumIn<DIMS>(il, l, pa), dnormalize<DIMS>(dsub(pb, pa))),
                                                &fp[f]);
             }
         }
@@ -391,7 +391,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
                 const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
                 const d3 pb = fr.temporalAmp ? x2t.pd : x2cd;
                 fv = isoPointVw ? isoAmp<DIMS, R>()
-                                : phaseAmp<DIMS, R>(sc, m1, ddot<DIMS>(incoming<DIMS>(vw, v, pa), dnormalize<DIMS>(dsub(pb, pa))),
+                                : phaseAmp<DIMS, R, TAB>(sc, m1, ddot<DIMS>(incoming<DIMS>(vw, v, pa), dnormalize<DIMS>(dsub(pb, pa))),
                                                     &fvp);
             }
             Ar[o] = multi ? cis<R>(ga[0] * fv * w2, gp[0] + fvp) : zero;
@@ -400,7 +400,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
     *stepsOut += steps;
 }
 
-template <int DIMS, typename R, int IT>
+template <int DIMS, typename R, bool TAB>
 __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, SourceView il, SourceView vw, FramesView fr,
                                              VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
@@ -411,7 +411,7 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
     __shared__ R sigLds[kSigLds];
     const R* sig = loadSigTable<R>(m, sigLds);
     uint32_t steps = 0;
-    if (p < prm.npaths) nee_path<DIMS, R>(m, sig, sc, il, vw, fr, vb, prm, out, p, sub, lpp, &steps);
+    if (p < prm.npaths) nee_path<DIMS, R, TAB>(m, sig, sc, il, vw, fr, vb, prm, out, p, sub, lpp, &steps);
     uint32_t tot = waveSum(steps);
     if (lane == 0 && out.steps) atomicAdd(out.steps, (unsigned long long)tot);
 }
@@ -655,7 +655,8 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
                        hipStream_t st, hipEvent_t between) {
     const uint32_t ppb = 4 * (64 / p.lpp);   // paths per 256-thread block
     dim3 grid((p.npaths + ppb - 1) / ppb), block(256);
-    hipLaunchKernelGGL((k_nee<DIMS, R, 1>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    if (sc.tab) hipLaunchKernelGGL((k_nee<DIMS, R, true>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    else hipLaunchKernelGGL((k_nee<DIMS, R, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
     hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, vb, fr, p.npaths,
                        1.0 / p.lambda);
     if (between) hipEventRecord(between, st);

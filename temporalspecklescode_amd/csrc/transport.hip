@@ -85,7 +85,7 @@ __device__ inline void frameDeltas(const FramesView& fr, uint64_t seed, uint64_t
     }
 }
 
-template <int DIMS>
+template <int DIMS, bool TAB>
 __global__ void __launch_bounds__(256) k_transport(MediumView m, ScatterView sc, SourceView il, FramesView fr,
                                                    VertexBatch vb, uint64_t seed, uint64_t pathBegin,
                                                    uint32_t npaths, int write) {
@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(256) k_transport(MediumView m, ScatterView sc,
         if (k > 1) {
             ev = kEventVertex | k;
             Rng rng(seed, path, ev, kPurposeTransport);
-            f3 nd = sampleDirection<DIMS>(sc, mat, dir, rng);
+            f3 nd = sampleDirection<DIMS, TAB>(sc, mat, dir, rng);
             f3 nx;
             uint32_t nm = woodcock<DIMS>(m, x, nd, rng, &nx);
             if (nm == 0) break;
@@ -217,10 +217,12 @@ hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc,
                            const FramesView& fr, const VertexBatch& vb, uint64_t seed, uint64_t pathBegin,
                            uint32_t npaths, int write, hipStream_t st) {
     dim3 grid((npaths + 255) / 256), block(256);
-    if (dims == 2)
-        hipLaunchKernelGGL(k_transport<2>, grid, block, 0, st, m, sc, il, fr, vb, seed, pathBegin, npaths, write);
-    else
-        hipLaunchKernelGGL(k_transport<3>, grid, block, 0, st, m, sc, il, fr, vb, seed, pathBegin, npaths, write);
+    const bool tab = sc.tab != nullptr;
+#define SSTMC_TRANSPORT(D, T) \
+    hipLaunchKernelGGL((k_transport<D, T>), grid, block, 0, st, m, sc, il, fr, vb, seed, pathBegin, npaths, write)
+    if (dims == 2) { if (tab) SSTMC_TRANSPORT(2, true); else SSTMC_TRANSPORT(2, false); }
+    else { if (tab) SSTMC_TRANSPORT(3, true); else SSTMC_TRANSPORT(3, false); }
+#undef SSTMC_TRANSPORT
     return hipGetLastError();
 }
 
