@@ -60,7 +60,8 @@ enum { SSTMC_SCATTERING_ISOTROPIC = 1,                    /* config.scattering.t
 enum { SSTMC_SOURCE_NONE = 0,                             /* config.illumination/view.type */
        SSTMC_SOURCE_POINT = 1,
        SSTMC_SOURCE_FARFIELD = 2 };
-enum { SSTMC_SAMPLER_TEMPORAL = 1 };                      /* config.sampler.type */
+enum { SSTMC_SAMPLER_TEMPORAL = 1,                        /* config.sampler.type: TemporalSampler */
+       SSTMC_SAMPLER_TEMPORAL_CORRELATION = 2 };          /* TemporalCorrelationSampler */
 enum { SSTMC_TRACER_NEE = 1 };                            /* config.tracer.type */
 enum { SSTMC_OUTPUT_FIELD = 0,                            /* u(l,v,t): the field per frame */
        SSTMC_OUTPUT_CORRELATION = 1 };                    /* C(l,v,d) = sum u(t=0) conj(u(t=d)), per-sub-path estimator */
@@ -125,22 +126,25 @@ typedef struct sstmc_source {
     int32_t isAreaSource;          /* PointSource only (PointSource.cuh:140-147) */
 } sstmc_source;
 
-/* config.sampler, TemporalSampler (MatlabInterface.cu:1217-1312). */
+/* config.sampler (MatlabInterface.cu:1217-1368). TemporalSampler: t are frame times and u(l,v,t)
+ * the field per frame. TemporalCorrelationSampler (TemporalSamplerCorrelation.cuh): t are time
+ * lags tau and u(l,v,t) the analytic correlation sum u1(-tau/2) conj(u2(+tau/2)) between the
+ * illumination/view pair and the illumination2/view2 pair. */
 typedef struct sstmc_sampler {
-    int32_t type;                  /* SSTMC_SAMPLER_TEMPORAL */
-    uint32_t tNum;                 /* frames Nt */
-    const double* t;               /* frame times */
+    int32_t type;                  /* SSTMC_SAMPLER_TEMPORAL or SSTMC_SAMPLER_TEMPORAL_CORRELATION */
+    uint32_t tNum;                 /* frames (or lags) Nt */
+    const double* t;               /* frame times (or lags) */
     const double* D;               /* Brownian coefficient per material */
     const double* Ux;              /* linear velocity per material */
     const double* Uy;
     const double* Uz;
-    int32_t isTemporalAmplitude;   /* default 1: g/f use displaced positions */
+    int32_t isTemporalAmplitude;   /* TemporalSampler, default 1: g/f use displaced positions */
 } sstmc_sampler;
 
 /* config.tracer (MatlabInterface.cu:1374-1457). */
 typedef struct sstmc_tracer {
     int32_t type;                  /* SSTMC_TRACER_NEE */
-    int32_t isCBS;                 /* must be 0 (coherent backscattering not built) */
+    int32_t isCBS;                 /* coherent backscattering (field output of the TemporalSampler) */
     int32_t output;                /* SSTMC_OUTPUT_FIELD or SSTMC_OUTPUT_CORRELATION */
 } sstmc_tracer;
 
@@ -161,6 +165,11 @@ typedef struct sstmc_config {
     sstmc_sampler sampler;
     sstmc_tracer tracer;
     sstmc_precision precision;
+    /* config.illumination2 / config.view2 (MatlabInterface.cu:1062-1090, 1313-1331): the u2
+     * sources of the TemporalCorrelationSampler, of the same type, count and area flag as
+     * illumination / view; type SSTMC_SOURCE_NONE (zero) when absent. */
+    sstmc_source illumination2;
+    sstmc_source view2;
 } sstmc_config;
 
 typedef struct sstmc_handle sstmc_handle;

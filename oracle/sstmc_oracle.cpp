@@ -475,10 +475,14 @@ struct Problem {
     uint64_t seed;
     int output;                // SSTMC_OUTPUT_FIELD / _CORRELATION
     bool cbs = false;          // coherent backscattering: add the reversed path of every path
+    // TemporalCorrelationSampler: il holds the illumination (u1) sources then the illumination2
+    // (u2) ones, vw the view then the view2 ones; ticks are the time lags
+    bool acorr = false;
+    std::vector<double> ticks;
     uint32_t maxAttempts = 1u << 20;
 
-    uint32_t L() const { return il.n(); }
-    uint32_t V() const { return vw.n(); }
+    uint32_t L() const { return acorr ? il.n() / 2 : il.n(); }
+    uint32_t V() const { return acorr ? vw.n() / 2 : vw.n(); }
     uint32_t T() const { return fr.nt; }
 
     // projectBoxToLine (MediumInterface.cuh:156-199) with pointToLineProjectionDistance (:137-153).
@@ -543,8 +547,12 @@ struct Problem {
 
     // sampleFarFieldKernel (FarField.cuh:129-205) / PointSourceNS::sampleFirstPointKernel
     // (PointSource.cuh:72-90): source point x0 and direction of the first flight.
+    // With the TemporalCorrelationSampler the source is drawn from the u1 and u2 sources together:
+    // u1 or u2 with probability 1/2 each, as the reference's random split of a batch between
+    // u1IlluminationsHandle and u2IlluminationsHandle (TemporalSamplerCorrelation.cuh:611-676)
+    // gives every path slot on average.
     void sampleSource(Stream& s, V3* x0, V3* dir) const {
-        uint32_t l = s.index(L());
+        uint32_t l = s.index(il.n());
         if (il.type == SSTMC_SOURCE_POINT) {
             *x0 = il.pt[l];
             *dir = sc.randomDirection(s);
@@ -617,7 +625,8 @@ struct Problem {
             ++*misses;
         }
         v1.rho = Stream(seed, path, event, 2).uniform();
-        frames(path, event, v1);
+        if (acorr) v1.xt.assign(fr.nt, v1.x);   // the correlation sampler displaces analytically
+        else frames(path, event, v1);
         v1.len.assign(fr.nt, 0.0);
         vs.push_back(v1);
         for (uint32_t k = 2;; ++k) {
@@ -631,7 +640,8 @@ struct Problem {
             Vertex v;
             v.x = x; v.dirIn = d; v.m = (uint32_t)m;
             v.rho = Stream(seed, path, ev, 2).uniform();
-            frames(path, ev, v);
+            if (acorr) v.xt.assign(fr.nt, v.x);
+            else frames(path, ev, v);
             v.len.resize(fr.nt);
             for (uint32_t t = 0; t < fr.nt; ++t)
                 v.len[t] = prev.len[t] + geo.norm(sub(v.xt[t], prev.xt[t]));
@@ -729,6 +739,7 @@ struct Problem {
     // Per-path terms of Appendix A (NextEventEstimation.cuh:1016-1136,1138-1458;
     // TemporalSampler.cuh:668-719).
     void terms(const std::vector<Vertex>& vs, Terms& out, uint64_t* steps) const {
+        if (acorr) { termsAcorr(vs, out, steps); return; }
         uint32_t nl = L(), nv = V(), nt = T();
         const Vertex& v1 = vs[0];
         out.A.assign((size_t)nt * nl, cplx(0));
@@ -812,6 +823,110 @@ struct Problem {
                     out.A[t * nl + l] = out.A[l] * std::conj(out.A[t * nl + l]);
         }
     }
+    // ---------------------------------------------------------------- TemporalCorrelationSampler
+    // Per-path terms of the analytic correlation u1(-tau/2) conj u2(+tau/2)
+    // (TemporalSamplerCorrelation.cuh; NextEventEstimation.cuh:1016-1136 single scattering,
+    // :1138-1200 and :1400-1425 multiple scattering, :1461-1601 order of evaluation). For lag tau_t:
+    //  * g of u1 sees a vertex of material m at x - (tau/2) U_m, g of u2 at x + (tau/2) U_m
+    //    (copyPointsToBuffersThroughput, TemporalSamplerCorrelation.cuh:228-250);
+    //  * the three-point functions of u1 and u2 both see x - (tau/2) U_m: the sign test of
+    //    copyPointsToBuffersThreePoints (:253-313) is always true (quirk Q3);
+    //  * Brownian motion enters as exp(-k^2 |D_m tau| |q|^2), q the momentum transfer at a vertex
+    //    from the unshifted positions: at x1 against the mean u1/u2 illumination direction (C1_l1l2),
+    //    at x_k against the mean view direction (C1_v1v2), both through
+    //    temporalCorrelationSourceContributionMultiple (SourceInterface.cuh:280-352); at the inner
+    //    vertices through momentTransfer (copyBuffersToPoints_momentumTransfer, :144-179), applied
+    //    once, on u1 (pathContriobutionKernel, :384-450); single scattering through
+    //    temporalCorrelationSourceContributionSingle (SourceInterface.cuh:217-277);
+    //  * the path phase 2 pi L_k / lambda multiplies u1 and u2 alike and cancels;
+    //  * the path probabilities are the means of the u1 and u2 first-point probabilities, without the
+    //    square root (pathSingleProbability, :783-829; NextEventEstimation.cuh:851-880, 930-977); the
+    //    second-point factor uses the u1 illuminations' directions for both (:833-856,
+    //    secondPointProbability of illuminationsHandle);
+    //  * single scattering with a scalar Fs (point illumination and view, isotropic scattering,
+    //    SamplerInterface.cuh:155, PointSource.cuh:62) is Fs^2 / p1 g_l1 g_v1 conj(g_l2 g_v2):
+    //    u2's Fs and the single-scattering temporal factor are not used (NextEventEstimation.cuh:
+    //    308-324, 1020-1029, 1094-1118).
+    V3 shifted(const V3& x, uint32_t m, double s) const { return add(x, scale(s, fr.U[m])); }
+    void termsAcorr(const std::vector<Vertex>& vs, Terms& out, uint64_t* steps) const {
+        const uint32_t nl = L(), nv = V(), nt = T();
+        const double kw = 2.0 * PI / lambda, k2 = kw * kw;
+        const Vertex& v1 = vs[0];
+        out.A.assign((size_t)nt * nl, cplx(0));
+        out.B.assign((size_t)nt * nv, cplx(0));
+        out.S.assign((size_t)nt * nv * nl, cplx(0));
+        out.multiple = vs.size() >= 2;
+        std::vector<double> prl(2 * nl);
+        double p1 = 0;
+        for (uint32_t i = 0; i < 2 * nl; ++i) { prl[i] = pr(i, v1.x, steps); p1 += prl[i]; }
+        p1 /= 2.0 * nl;
+        const bool scalarFs = il.type == SSTMC_SOURCE_POINT && vw.type == SSTMC_SOURCE_POINT && sc.allIsotropic;
+        const double iso = sc.isoAmp();
+        std::vector<cplx> gl1(nl), gl2(nl), gv1(nv), gv2(nv);
+        for (uint32_t t = 0; t < nt; ++t) {
+            const double s = 0.5 * ticks[t];
+            const double dtD = k2 * std::fabs(fr.D[v1.m] * ticks[t]);
+            const V3 xa = shifted(v1.x, v1.m, -s), xb = shifted(v1.x, v1.m, s);
+            for (uint32_t l = 0; l < nl; ++l) { gl1[l] = g(il, l, xa, steps); gl2[l] = g(il, nl + l, xb, steps); }
+            for (uint32_t v = 0; v < nv; ++v) { gv1[v] = g(vw, v, xa, steps); gv2[v] = g(vw, nv + v, xb, steps); }
+            for (uint32_t v = 0; v < nv; ++v)
+                for (uint32_t l = 0; l < nl; ++l) {
+                    const cplx u1 = gl1[l] * gv1[v], u2 = gl2[l] * gv2[v];
+                    cplx sv;
+                    if (scalarFs) {
+                        sv = iso * iso / p1 * u1 * std::conj(u2);
+                    } else {
+                        const cplx f1 = fsingle(l, v, v1.m, xa), f2 = fsingle(nl + l, nv + v, v1.m, xa);
+                        const V3 q = scale(0.5, sub(add(inDir(l, v1.x), inDir(nl + l, v1.x)),
+                                                    add(outDir(v, v1.x), outDir(nv + v, v1.x))));
+                        sv = u1 * f1 / p1 * std::conj(u2 * f2 * std::exp(-dtD * geo.dot(q, q)));
+                    }
+                    out.S[((size_t)t * nv + v) * nl + l] = sv;
+                }
+        }
+        if (!out.multiple) return;
+        const Vertex& v2 = vs[1];
+        const V3 d12 = geo.normalize(sub(v2.x, v1.x));
+        double p2 = 0;
+        for (uint32_t i = 0; i < 2 * nl; ++i) p2 += prl[i] * sc.pdf(v1.m, geo.dot(inDir(i % nl, v1.x), d12));
+        p2 /= 2.0 * nl;
+        for (uint32_t t = 0; t < nt; ++t) {
+            const double s = 0.5 * ticks[t];
+            const double dtD = k2 * std::fabs(fr.D[v1.m] * ticks[t]);
+            const V3 x1a = shifted(v1.x, v1.m, -s), x1b = shifted(v1.x, v1.m, s), x2a = shifted(v2.x, v2.m, -s);
+            for (uint32_t l = 0; l < nl; ++l) {
+                const cplx a1 = g(il, l, x1a, steps) * f3(il, l, v1.m, x1a, x2a) / p2;
+                const cplx a2 = g(il, nl + l, x1b, steps) * f3(il, nl + l, v1.m, x1a, x2a);
+                const V3 q = sub(scale(0.5, add(inDir(l, v1.x), inDir(nl + l, v1.x))), d12);
+                out.A[(size_t)t * nl + l] = a1 * std::conj(a2) * std::exp(-dtD * geo.dot(q, q));
+            }
+        }
+        double mt = 0, len = 0;   // inner momentum transfer sum_j D_j |q_j|^2, path length from x1
+        for (size_t k = 1; k < vs.size(); ++k) {
+            const Vertex& xk = vs[k];
+            const Vertex& xp = vs[k - 1];
+            len += geo.norm(sub(xk.x, xp.x));
+            if (k >= 2) {
+                const V3 q = sub(geo.normalize(sub(xp.x, vs[k - 2].x)), geo.normalize(sub(xk.x, xp.x)));
+                mt += geo.dot(q, q) * fr.D[xp.m];
+            }
+            const V3 back = geo.normalize(sub(xp.x, xk.x));
+            const cplx e(std::cos(2.0 * PI * len / lambda), std::sin(2.0 * PI * len / lambda));
+            for (uint32_t t = 0; t < nt; ++t) {
+                const double s = 0.5 * ticks[t];
+                const double dtD = k2 * std::fabs(fr.D[xk.m] * ticks[t]);
+                const double inner = std::exp(-k2 * std::fabs(ticks[t]) * mt);
+                const V3 xka = shifted(xk.x, xk.m, -s), xkb = shifted(xk.x, xk.m, s), xpa = shifted(xp.x, xp.m, -s);
+                for (uint32_t v = 0; v < nv; ++v) {
+                    const cplx G1 = g(vw, v, xka, steps) * fvw(v, xk.m, xpa, xka) * e * inner;
+                    const cplx G2 = g(vw, nv + v, xkb, steps) * fvw(nv + v, xk.m, xpa, xka) * e;
+                    const V3 q = sub(scale(-0.5, add(outDir(v, xk.x), outDir(nv + v, xk.x))), back);
+                    out.B[(size_t)t * nv + v] += G1 * std::conj(G2) * std::exp(-dtD * geo.dot(q, q));
+                }
+            }
+        }
+    }
+
     // Adds the path's contribution to U[t][v][l].
     void accumulate(const Terms& tm, std::vector<cplx>& U) const {
         uint32_t nl = L(), nv = V(), nt = T();
@@ -918,27 +1033,40 @@ int build(const sstmc_config* c, Problem& pb) {
     med.maxSigt = 0;
     for (uint32_t m = 0; m < nmat; ++m) med.maxSigt = std::max(med.maxSigt, med.sigt(m));
     if (!(med.maxSigt > 0)) return SSTMC_INVALID_CONFIG;
-    // sources
+    // sources; the TemporalCorrelationSampler appends illumination2 / view2 (same type, count and
+    // area flag, MatlabInterface.cu:1313-1331)
+    const sstmc_sampler& sp = c->sampler;
+    if (sp.type != SSTMC_SAMPLER_TEMPORAL && sp.type != SSTMC_SAMPLER_TEMPORAL_CORRELATION) return SSTMC_NOT_SUPPORTED;
+    pb.acorr = sp.type == SSTMC_SAMPLER_TEMPORAL_CORRELATION;
+    if (pb.acorr && pb.cbs) return SSTMC_NOT_SUPPORTED;
+    if (pb.acorr && pb.output != SSTMC_OUTPUT_FIELD) return SSTMC_INVALID_CONFIG;
     const sstmc_source* srcs[2] = {&c->illumination, &c->view};
+    const sstmc_source* srcs2[2] = {&c->illumination2, &c->view2};
     Source* dst[2] = {&pb.il, &pb.vw};
     for (int i = 0; i < 2; ++i) {
-        const sstmc_source& s = *srcs[i];
         Source& d = *dst[i];
-        d.type = s.type;
+        const sstmc_source& s0 = *srcs[i];
+        d.type = s0.type;
         d.illum = i == 0;
-        d.area = s.isAreaSource != 0;
-        if ((s.type != SSTMC_SOURCE_FARFIELD && s.type != SSTMC_SOURCE_POINT) || s.num == 0 ||
-            !s.x || !s.y || (dims == 3 && !s.z))
-            return SSTMC_INVALID_CONFIG;
-        d.pt.resize(s.num);
-        for (uint32_t k = 0; k < s.num; ++k) {
-            V3 p(s.x[k], s.y[k], dims == 3 ? s.z[k] : 0.0);
-            d.pt[k] = s.type == SSTMC_SOURCE_FARFIELD ? pb.geo.normalize(p) : p;
+        d.area = s0.isAreaSource != 0;
+        d.pt.clear();
+        for (int part = 0; part < (pb.acorr ? 2 : 1); ++part) {
+            const sstmc_source& s = part == 0 ? s0 : *srcs2[i];
+            if (part == 1 && s.type == SSTMC_SOURCE_NONE) return SSTMC_INVALID_CONFIG;
+            if (part == 1 && (s.type != s0.type || (s.isAreaSource != 0) != d.area)) return SSTMC_NOT_SUPPORTED;
+            if (part == 1 && s.num != s0.num) return SSTMC_INVALID_CONFIG;
+            if ((s.type != SSTMC_SOURCE_FARFIELD && s.type != SSTMC_SOURCE_POINT) || s.num == 0 ||
+                !s.x || !s.y || (dims == 3 && !s.z))
+                return SSTMC_INVALID_CONFIG;
+            for (uint32_t k = 0; k < s.num; ++k) {
+                V3 p(s.x[k], s.y[k], dims == 3 ? s.z[k] : 0.0);
+                d.pt.push_back(s.type == SSTMC_SOURCE_FARFIELD ? pb.geo.normalize(p) : p);
+            }
         }
-        if (s.type == SSTMC_SOURCE_FARFIELD && i == 0) pb.buildFarField(d);
+        if (d.type == SSTMC_SOURCE_FARFIELD && i == 0) pb.buildFarField(d);
     }
-    // sampler: time table (TemporalSampler.cuh:941-1132)
-    const sstmc_sampler& sp = c->sampler;
+    // sampler: time table (TemporalSampler.cuh:941-1132); the correlation sampler takes the lags
+    // as they are (TemporalSamplerCorrelation.cuh:455-530)
     if (sp.tNum == 0 || !sp.t || !sp.D) return SSTMC_INVALID_CONFIG;
     Frames& fr = pb.fr;
     fr.nt = sp.tNum;
@@ -949,6 +1077,10 @@ int build(const sstmc_config* c, Problem& pb) {
     fr.U.assign(nmat, V3());
     for (uint32_t m = 0; m < nmat; ++m)
         fr.U[m] = V3(sp.Ux ? sp.Ux[m] : 0.0, sp.Uy ? sp.Uy[m] : 0.0, (dims == 3 && sp.Uz) ? sp.Uz[m] : 0.0);
+    if (pb.acorr) {
+        pb.ticks.assign(sp.t, sp.t + sp.tNum);
+        return SSTMC_NO_ERROR;
+    }
     double mean = 0;
     for (uint32_t t = 0; t < fr.nt; ++t) mean += sp.t[t];
     mean /= fr.nt;

@@ -6,7 +6,9 @@ MatlabInterface.cu:274-1563): ``simulation``, ``medium``, ``scattering``, ``illu
 ``medium.materialGrid`` may be an (nx, ny[, nz]) array indexed like MATLAB (x fastest in memory
 after Fortran-order flattening) or an already-flat vector.  Extra, MI355X-specific keys live
 under ``config['precision']`` (``terms``: 'fp32'|'fp64', ``contraction``: 'bf16x3'|'fp32'|'fp64')
-and ``config['tracer']['output']`` ('field'|'correlation').
+and ``config['tracer']['output']`` ('field'|'correlation').  ``sampler.type`` is
+'TemporalSampler' (default) or 'TemporalCorrelationSampler', which takes ``illumination2`` and
+``view2`` as the second source pair.
 
 Validation follows the reference's MEX checks where they exist (material ids in 1..M-1,
 MatlabInterface.cu:975-992; D >= 0, :1293-1297; batchSize > 0, :1497-1509) and raises
@@ -25,7 +27,7 @@ SCATTERING_TABULAR = 2
 SCATTERING_HG = 3
 SCATTERING_VMF = 4
 SOURCE_NONE, SOURCE_POINT, SOURCE_FARFIELD = 0, 1, 2
-SAMPLER_TEMPORAL = 1
+SAMPLER_TEMPORAL, SAMPLER_TEMPORAL_CORRELATION = 1, 2
 TRACER_NEE = 1
 OUTPUT_FIELD, OUTPUT_CORRELATION = 0, 1
 TERMS_FP32, TERMS_FP64 = 0, 1
@@ -74,7 +76,8 @@ class Precision(C.Structure):
 class Config(C.Structure):
     _fields_ = [("dims", C.c_uint32), ("simulation", Simulation), ("medium", Medium),
                 ("scattering", Scattering), ("illumination", Source), ("view", Source),
-                ("sampler", Sampler), ("tracer", Tracer), ("precision", Precision)]
+                ("sampler", Sampler), ("tracer", Tracer), ("precision", Precision),
+                ("illumination2", Source), ("view2", Source)]
 
 
 class CConfig:
@@ -234,11 +237,28 @@ class CConfig:
         for name in ("illumination", "view"):
             self._source(getattr(s, name), cfg[name], dims, name)
 
-        # sampler (MatlabInterface.cu:1217-1312)
+        # sampler (MatlabInterface.cu:1217-1368)
         sp = cfg["sampler"]
-        if sp.get("type", "TemporalSampler") != "TemporalSampler":
-            raise ValueError("Unsupported sampler type. Available sampler type is TemporalSampler")
-        s.sampler.type = SAMPLER_TEMPORAL
+        stype = sp.get("type", "TemporalSampler")
+        if stype == "TemporalSampler":
+            s.sampler.type = SAMPLER_TEMPORAL
+        elif stype == "TemporalCorrelationSampler":
+            s.sampler.type = SAMPLER_TEMPORAL_CORRELATION
+            # MatlabInterface.cu:1313-1331
+            if cfg.get("illumination2") is None or cfg.get("view2") is None:
+                raise ValueError("For correlation sampler, source.illuminationsHandle2 and "
+                                 "source.viewsHandle2 must be entered")
+            self._source(s.illumination2, cfg["illumination2"], dims, "illumination2")
+            self._source(s.view2, cfg["view2"], dims, "view2")
+            if s.illumination2.num != s.illumination.num:
+                raise ValueError("Illumination 1 in size of %d, Illumination 2 in size of %d. Both illuminations "
+                                 "must be at the same size" % (s.illumination.num, s.illumination2.num))
+            if s.view2.num != s.view.num:
+                raise ValueError("View 1 in size of %d, View 2 in size of %d. Both views must be at the same size"
+                                 % (s.view.num, s.view2.num))
+        else:
+            raise ValueError("Unsupported sapmler type. Available sampler types are TemporalSampler and "
+                             "TemporalCorrelationSampler")
         t = np.asarray(sp.get("t", [0.0]), np.float64).ravel()
         s.sampler.t, s.sampler.tNum = self._dptr(t)
         D = np.asarray(sp.get("D", np.zeros(material_num)), np.float64).ravel()

@@ -84,6 +84,9 @@ def terms_from_vertices(cfg: dict, dims: int, k, off, pos, mat, rho, xt, ln):
     rho = np.ascontiguousarray(rho, np.float64)
     xt = np.ascontiguousarray(xt, np.float64)
     ln = np.ascontiguousarray(ln, np.float64)
+    nvtx = int(off[-1])
+    if pos.shape != (nvtx, 3) or xt.shape != (nvtx, nt, 3) or ln.shape != (nvtx, nt) or mat.size != nvtx:
+        raise ValueError("vertex arrays do not match %d vertices x %d frames" % (nvtx, nt))
     a = np.zeros((n, nt, nl, 2)); b = np.zeros((n, nt, nv, 2)); s = np.zeros((n, nt, nv, nl, 2))
     rc = lib().oracle_terms_from_vertices(cc.ptr(), C.c_uint32(n), _p(k), _p(off), _p(pos), _p(mat), _p(rho),
                                           _p(xt), _p(ln), _p(a), _p(b), _p(s))

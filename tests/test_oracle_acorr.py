@@ -1,0 +1,271 @@
+"""The TemporalCorrelationSampler (analytic temporal correlation, TemporalSamplerCorrelation.cuh;
+SourceInterface.cuh:217-352; NextEventEstimation.cuh:1016-1200, 1400-1425) in the CPU restatement.
+
+Pinned against independent routes to the same quantity, never against a retyped formula:
+  * linear motion: with every scatterer moving at one velocity U and far-field sources, the
+    analytic terms at lag tau equal, path by path on identical vertices, the explicit-frame
+    estimator (the TemporalSampler with output 'correlation') with its two frames at -tau/2 and
+    +tau/2 (translation keeps every direction, so quirk Q3 does not enter);
+  * Brownian motion: the analytic exp(-k^2 |D tau| |q|^2) factors are the expectation, over
+    Gaussian displacements of variance 2 D tau per axis, of the explicit-frame estimator on the
+    same vertices (its phases change by sum_j Q_j . Delta_j); checked for far-field and point
+    sources, 2D and 3D, materials with different D;
+  * the reference's own known answer: examples/mcxValidation3D.m runs this sampler
+    (correlationAlgorithmValidation) against mcx_res (tests/golden/mcx_validation.json);
+  * the u1/u2 source pair: at lag 0 without motion the analytic correlation between two source
+    sets is the mean, over field renderings with both sets as sources, of u(l1, v1) conj u(l2, v2);
+  * the reference's scalar-Fs single scattering (point sources, isotropic scattering) carries no
+    temporal factor.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from temporalspecklescode_amd import workloads as W
+
+
+def _slab(dims, seed, n=6, far=True, g=0.6, lam=1.0, nl=2, nv=3):
+    edges = np.linspace(-3.0, 3.0, n + 1)
+    grid = W.random_materials((n,) * dims, seed, 3)
+    rng = np.random.default_rng(seed)
+    cfg = {
+        "simulation": {"batchSize": 256, "seed": seed, "lambda": lam},
+        "medium": {"type": "Heterogeneous", "xAxis": edges, "yAxis": edges, "materialGrid": grid,
+                   "sigs": [0.0, 0.5, 0.9, 1.4], "siga": [0.0, 0.01, 0.02, 0.0]},
+        "scattering": {"type": ["HG"] * 4, "amplitudeFunction": [g, g, 0.3, g]},
+        "tracer": {"type": "nee"},
+    }
+    if dims == 3:
+        cfg["medium"]["zAxis"] = edges
+
+    def dirs(k, lo):
+        a = rng.uniform(-0.25, 0.25, k) + lo
+        if dims == 2:
+            return {"x": np.sin(a), "y": np.cos(a)}
+        b = rng.uniform(-0.25, 0.25, k)
+        d = np.stack([np.sin(a) * np.cos(b), np.sin(b), np.cos(a) * np.cos(b)])
+        return {"x": d[0], "y": d[1], "z": d[2]}
+
+    def points(k, side):
+        p = {"x": rng.uniform(-2, 2, k), "y": rng.uniform(-2, 2, k)}
+        if dims == 3:
+            p["z"] = np.full(k, 3.5 * side)
+        else:
+            p["y"] = np.full(k, 3.5 * side)
+        return p
+
+    if far:
+        cfg["illumination"] = {"type": "FarField", "direction": dirs(nl, 0.0)}
+        cfg["view"] = {"type": "FarField", "direction": dirs(nv, 0.1)}
+    else:
+        cfg["illumination"] = {"type": "PointSource", "location": points(nl, -1)}
+        cfg["view"] = {"type": "PointSource", "location": points(nv, 1)}
+    return cfg
+
+
+def _sampler(kind, t, D, U=(0.0, 0.0, 0.0), amp=True):
+    nm = len(D)
+    s = {"type": kind, "t": np.asarray(t, np.float64), "D": np.asarray(D, np.float64),
+         "U": {"x": np.full(nm, U[0]), "y": np.full(nm, U[1]), "z": np.full(nm, U[2])}}
+    if kind == "TemporalSampler":
+        s["isTemporalAmplitude"] = amp
+    return s
+
+
+def _acorr(cfg, t, D, U=(0.0, 0.0, 0.0), il2=None, vw2=None):
+    c = dict(cfg)
+    c["sampler"] = _sampler("TemporalCorrelationSampler", t, D, U)
+    c["tracer"] = {"type": "nee"}
+    c["illumination2"] = il2 if il2 is not None else cfg["illumination"]
+    c["view2"] = vw2 if vw2 is not None else cfg["view"]
+    return c
+
+
+def _terms(cfg, dims, o, xt=None, ln=None):
+    nt = len(cfg["sampler"]["t"])
+    if xt is None:
+        xt = o["xt"] if o["xt"].shape[1] == nt else np.repeat(o["pos"][:, None, :], nt, axis=1)
+        ln = o["len"] if o["len"].shape[1] == nt else np.zeros((o["pos"].shape[0], nt))
+    return O.terms_from_vertices(cfg, dims, o["k"], o["off"], o["pos"], o["mat"], o["rho"], xt, ln)
+
+
+def _per_path_u(A, B, S):
+    """U_p[v, l] = S_p + A_p (x) B_p for every path, lag axis first: [t][p][v][l]."""
+    return np.transpose(S, (1, 0, 2, 3)) + np.einsum("ptl,ptv->tpvl", A, B)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_linear_motion_equals_explicit_frames_at_half_lags(dims):
+    cfg = _slab(dims, 11 + dims)
+    U = (0.13, -0.07, 0.05 if dims == 3 else 0.0)
+    lags = [0.0, 0.7, 2.5]
+    ca = _acorr(cfg, lags, [0.0] * 4, U)
+    o = O.paths(dict(cfg, sampler=_sampler("TemporalSampler", [0.0], [0.0] * 4)), dims, 0, 256)
+    A, B, S = _terms(ca, dims, o)
+    ua = _per_path_u(A, B, S)
+    assert np.count_nonzero(o["k"] >= 3) > 50
+    for i, tau in enumerate(lags):
+        ce = dict(cfg, sampler=_sampler("TemporalSampler", [-tau / 2, tau / 2], [0.0] * 4, U))
+        ce["tracer"] = {"type": "nee", "output": "correlation"}
+        oe = O.paths(ce, dims, 0, 256)       # the same vertices: the frames have their own stream
+        assert np.array_equal(oe["k"], o["k"]) and np.array_equal(oe["pos"], o["pos"])
+        Ae, Be, Se = _terms(ce, dims, oe)
+        se = Se[:, :1] * np.conj(Se)
+        ue = _per_path_u(Ae, Be, se)[1]
+        scale = np.abs(ua[i]).max()
+        assert np.abs(ua[i] - ue).max() < 1e-10 * scale, tau
+        assert np.abs(A[:, i] - Ae[:, 1]).max() < 1e-10 * np.abs(A[:, i]).max()
+        assert np.abs(B[:, i] - Be[:, 1]).max() < 1e-10 * np.abs(B[:, i]).max()
+    # motion matters: the lags differ from lag 0
+    assert np.abs(ua[2].sum(0) - ua[0].sum(0)).max() > 1e-3 * np.abs(ua[0].sum(0)).max()
+
+
+@pytest.mark.parametrize("dims,far", [(3, True), (3, False), (2, True)])
+def test_brownian_is_the_mean_over_gaussian_displacements(dims, far):
+    lam, tau = 0.01, 1.0
+    k = 2 * math.pi / lam
+    D = np.array([0.0, 0.1, 0.25, 0.05]) / (k * k * tau)   # k^2 D tau per vertex: 0.1 .. 0.25
+    cfg = _slab(dims, 21 + dims + 2 * far, far=far, lam=lam)
+    ca = _acorr(cfg, [tau], D)
+    ce = dict(cfg, sampler=_sampler("TemporalSampler", [0.0, tau], D, amp=False))
+    ce["tracer"] = {"type": "nee", "output": "correlation"}
+    o = O.paths(dict(cfg, sampler=_sampler("TemporalSampler", [0.0], [0.0] * 4)), dims, 0, 200)
+    A, B, S = _terms(ca, dims, o)
+    ua = _per_path_u(A, B, S)[0].sum(0)
+    pos = o["pos"]
+    sd = np.sqrt(2 * D[o["mat"]] * tau)
+    rng = np.random.default_rng(7)
+    off = o["off"].astype(np.int64)
+    first = np.zeros(pos.shape[0], bool)
+    first[off[:-1][o["k"] > 0]] = True
+    draws = []
+    for _ in range(300):
+        x1 = pos + rng.standard_normal(pos.shape) * sd[:, None]
+        if dims == 2:
+            x1[:, 2] = 0.0
+        xt = np.stack([pos, x1], axis=1)
+        seg = np.zeros((pos.shape[0], 2))
+        seg[1:] = np.linalg.norm(np.diff(xt, axis=0), axis=2)
+        seg[first] = 0.0
+        ln = np.zeros_like(seg)
+        for a, b in zip(off[:-1], off[1:]):
+            ln[a:b] = np.cumsum(seg[a:b], axis=0)
+        Ae, Be, Se = _terms(ce, dims, o, xt, ln)
+        draws.append(_per_path_u(Ae, Be, Se[:, :1] * np.conj(Se))[1].sum(0))
+    draws = np.array(draws)
+    m = draws.mean(0)
+    se = draws.std(0, ddof=1) / np.sqrt(len(draws))
+    err = np.abs(m - ua)
+    assert np.all(err < 4 * se + 2e-3 * np.abs(ua).max()), (err / np.abs(ua).max())
+    # the decorrelation is large enough to be seen: lag tau is well below lag 0
+    u0 = _per_path_u(*_terms(_acorr(cfg, [0.0], D), dims, o))[0].sum(0)
+    assert np.abs(ua).sum() < 0.9 * np.abs(u0).sum()
+
+
+def test_mcx_known_answer():
+    """examples/mcxValidation3D.m with config.sampler.type = 'TemporalCorrelationSampler'
+    (illumination2 = illumination, view2 = view): C = u / iter against mcx_res. The level of the
+    curve carries the estimator's heavy-tailed path noise and is checked within 4 standard errors;
+    its shape C(t) / C(0), whose noise is an order of magnitude smaller, within 5 %."""
+    import json, os
+    r = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mcx_validation.json")))
+    res = []
+    for s in range(8):
+        cfg = W.mcx_validation(r["t"], r["D"], r["MFP"], r["R"], r["h"], r["dims"], r["viSep"], batch=20000,
+                               seed=200 + s)
+        cfg = _acorr(cfg, cfg["sampler"]["t"], cfg["sampler"]["D"])
+        u, it, _ = O.render(cfg, 3)
+        res.append(u[0, 0, :].real / it)
+    res = np.array(res)
+    mcx = np.array(r["mcx_res"])
+    m = res.mean(0)
+    se = res.std(0, ddof=1) / np.sqrt(len(res))
+    assert np.all(np.abs(m - mcx) < 4 * se)
+    shape = (m / m[0]) / (mcx / mcx[0])
+    assert np.abs(shape - 1).max() < 0.05, shape
+    assert m[0] > 2 * m[-1]
+
+
+@pytest.mark.parametrize("far,thin", [(True, False), (False, False), (False, True)])
+def test_source_pair_is_the_field_cross_correlation(far, thin):
+    """u1 = (illumination, view), u2 = (illumination2, view2), lag 0, no motion, isotropic
+    scattering (the reference's second-point probability takes u1's directions for both sets,
+    which matters only for anisotropic scattering): the analytic correlation equals the field
+    estimator's E[u(l, v) conj u(L + l, V + v)] / iter with both sets as sources. Paths are
+    independent and each path's field has zero mean (its random phases), so that expectation is the
+    per-path sum sum_p u_p(l, v) conj u_p(L + l, V + v) / iter, taken here from the field
+    estimator's per-path terms. Far field: u2 is u1 rotated by a few hundredths of a radian. Point
+    sources: the u2 illuminations sit half a unit farther from the box, which changes their
+    first-point probabilities by tens of percent near the entry face while the fields stay
+    correlated (wavelength 4); in a thin medium single scattering, and with it the first-point
+    probability, carries most of the result."""
+    dims = 2
+    base = _slab(dims, 31, nl=2, nv=2, far=far, lam=1.0 if far else 4.0)
+    base["scattering"] = {"type": ["Isotropic"] * 4}
+    base["medium"]["sigs"] = [0.0, 0.03, 0.05, 0.07] if thin else [0.0, 0.3, 0.5, 0.7]
+    il1, vw1 = base["illumination"], base["view"]
+    key = "direction" if far else "location"
+    if far:
+        rot = lambda d, a: {"x": np.cos(a) * d["x"] - np.sin(a) * d["y"], "y": np.sin(a) * d["x"] + np.cos(a) * d["y"]}
+        il2 = {"type": "FarField", "direction": rot(il1["direction"], 0.03)}
+        vw2 = {"type": "FarField", "direction": rot(vw1["direction"], -0.02)}
+    else:
+        il2 = {"type": "PointSource", "location": {"x": il1["location"]["x"], "y": il1["location"]["y"] - 0.5}}
+        vw2 = vw1
+    ca = _acorr(base, [0.0], [0.0] * 4, il2=il2, vw2=vw2)
+    acc = []
+    for s in range(32):
+        ca["simulation"] = dict(base["simulation"], batchSize=8192, seed=100 + s)
+        u, it, _ = O.render(ca, dims)
+        acc.append(u[:, :, 0] / it)
+    acc = np.array(acc)
+    cf = dict(base, sampler=_sampler("TemporalSampler", [0.0], [0.0] * 4))
+    cf["illumination"] = {"type": il1["type"], key: {k: np.concatenate([il1[key][k], il2[key][k]]) for k in ("x", "y")}}
+    cf["view"] = {"type": vw1["type"], key: {k: np.concatenate([vw1[key][k], vw2[key][k]]) for k in ("x", "y")}}
+    xs = []
+    for c in range(32):
+        o = O.paths(cf, dims, c * 8192, 8192)
+        up = o["s"][:, 0].transpose(0, 2, 1) + o["a"][:, 0, :, None] * o["b"][:, 0, None, :]   # [p][l][v]
+        xs.append((up[:, :2, :2] * np.conj(up[:, 2:, 2:])).sum(0) / (8192 + o["misses"].sum()))
+    xs = np.array(xs)
+    ma, sa = acc.mean(0), acc.std(0, ddof=1) / np.sqrt(len(acc))
+    mf, sf = xs.mean(0), xs.std(0, ddof=1) / np.sqrt(len(xs))
+    tol = 4 * np.sqrt(np.abs(sa) ** 2 + np.abs(sf) ** 2)
+    assert np.all(np.abs(ma - mf) < tol), (ma, mf, tol)
+    assert np.all(tol < 0.04 * np.abs(ma))      # a resolution that sees a 5 % bias
+    # the pair is correlated well above the noise
+    assert np.all(np.abs(ma) > 8 * np.abs(sa))
+
+
+def test_scalar_fs_single_scattering_keeps_no_temporal_factor():
+    """Point illumination and view with isotropic scattering: the reference forms single
+    scattering from Fs^2 / p1 and never applies u2's Fs nor the single-scattering temporal factor
+    (NextEventEstimation.cuh:1020-1029, 1094-1118), so S does not decorrelate; with an anisotropic
+    phase function (non-scalar Fs) it does."""
+    dims = 3
+    lam = 0.01
+    D = np.array([0.0, 1.0, 1.0, 1.0]) / (2 * math.pi / lam) ** 2
+    for iso in (True, False):
+        cfg = _slab(dims, 41, far=False, lam=lam)
+        if iso:
+            cfg["scattering"] = {"type": ["Isotropic"] * 4}
+        o = O.paths(dict(cfg, sampler=_sampler("TemporalSampler", [0.0], [0.0] * 4)), dims, 0, 64)
+        A, B, S = _terms(_acorr(cfg, [0.0, 1.0], D), dims, o)
+        if iso:
+            assert np.abs(S[:, 1] - S[:, 0]).max() == 0.0
+        else:
+            assert np.abs(S[:, 1]).sum() < 0.8 * np.abs(S[:, 0]).sum()
+        assert np.abs(A[:, 1]).sum() < 0.8 * np.abs(A[:, 0]).sum()
+
+
+def test_config_checks():
+    cfg = _slab(2, 3)
+    bad = _acorr(cfg, [0.0], [0.0] * 4)
+    del bad["view2"]
+    with pytest.raises(ValueError, match="illuminationsHandle2"):
+        O.render(bad, 2)
+    bad = _acorr(cfg, [0.0], [0.0] * 4, vw2={"type": "FarField", "direction": {"x": [0.0], "y": [1.0]}})
+    with pytest.raises(ValueError, match="Both views must be at the same size"):
+        O.render(bad, 2)
