@@ -21,7 +21,8 @@
 //               per material)
 //   illumination / view: type "FarField" (direction{x,y[,z]}) | "PointSource" (location{...}),
 //                        isAreaSource
-//   sampler: type "TemporalSampler", t, D, U{x,y,z}, isTemporalAmplitude
+//   sampler: type "TemporalSampler" | "TemporalCorrelationSampler", t, D, U{x,y,z},
+//            isTemporalAmplitude; the correlation sampler also takes illumination2 and view2
 //   tracer: type "nee", isCBS, output "field" | "correlation"
 //   precision (this implementation): terms "fp32"|"fp64", contraction "bf16x3"|"fp32"|"fp64"
 //   dims (optional; else --dims; else 3 when medium.zAxis is present)
@@ -309,11 +310,30 @@ void fill(HostConfig& h, const Value& cfg, uint32_t dims) {
     source(h, s.illumination, cfg.at("illumination"), dims, "illumination");
     source(h, s.view, cfg.at("view"), dims, "view");
 
-    // sampler (MatlabInterface.cu:1217-1312)
+    // sampler (MatlabInterface.cu:1217-1368)
     const Value& sp = cfg.at("sampler");
-    if (str(sp, "type", "TemporalSampler") != "TemporalSampler")
-        throw ConfigError("Unsupported sampler type. Available sampler type is TemporalSampler");
-    s.sampler.type = SSTMC_SAMPLER_TEMPORAL;
+    const std::string stype = str(sp, "type", "TemporalSampler");
+    if (stype == "TemporalSampler") {
+        s.sampler.type = SSTMC_SAMPLER_TEMPORAL;
+    } else if (stype == "TemporalCorrelationSampler") {
+        s.sampler.type = SSTMC_SAMPLER_TEMPORAL_CORRELATION;
+        // MatlabInterface.cu:1313-1331
+        if (!cfg.has("illumination2") || !cfg.has("view2"))
+            throw ConfigError("For correlation sampler, source.illuminationsHandle2 and source.viewsHandle2 must be "
+                              "entered");
+        source(h, s.illumination2, cfg.at("illumination2"), dims, "illumination2");
+        source(h, s.view2, cfg.at("view2"), dims, "view2");
+        if (s.illumination2.num != s.illumination.num)
+            throw ConfigError("Illumination 1 in size of " + std::to_string(s.illumination.num) +
+                              ", Illumination 2 in size of " + std::to_string(s.illumination2.num) +
+                              ". Both illuminations must be at the same size");
+        if (s.view2.num != s.view.num)
+            throw ConfigError("View 1 in size of " + std::to_string(s.view.num) + ", View 2 in size of " +
+                              std::to_string(s.view2.num) + ". Both views must be at the same size");
+    } else {
+        throw ConfigError("Unsupported sapmler type. Available sampler types are TemporalSampler and "
+                          "TemporalCorrelationSampler");
+    }
     std::vector<double> t = sp.has("t") ? nums(sp.at("t")) : std::vector<double>{0.0};
     s.sampler.tNum = (uint32_t)t.size();
     s.sampler.t = h.keep(t);

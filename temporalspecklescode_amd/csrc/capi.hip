@@ -37,6 +37,26 @@ static int buildSource(const sstmc_source& s, const Medium& med, int dims, bool 
     return err;
 }
 
+// The TemporalCorrelationSampler's sources: the u1 sources then the u2 ones in one source object
+// (validated to share type, count and area flag), so that the first vertex is drawn from both sets
+// and the kernels address u2 source i as entry n + i.
+static int buildSourcePair(const sstmc_source& s1, const sstmc_source& s2, const Medium& med, int dims, bool illum,
+                           std::unique_ptr<Source>* out) {
+    const uint32_t n = s1.num;
+    std::vector<double> x(2 * (size_t)n), y(2 * (size_t)n), z(2 * (size_t)n, 0.0);
+    for (uint32_t i = 0; i < n; ++i) {
+        x[i] = s1.x[i]; x[n + i] = s2.x[i];
+        y[i] = s1.y[i]; y[n + i] = s2.y[i];
+        if (dims == 3) { z[i] = s1.z[i]; z[n + i] = s2.z[i]; }
+    }
+    sstmc_source both = s1;
+    both.num = 2 * n;
+    both.x = x.data();
+    both.y = y.data();
+    both.z = z.data();
+    return buildSource(both, med, dims, illum, out);
+}
+
 // Host-only validation, before anything touches the device (MatlabInterface.cu:712-1563).
 static int validate(const sstmc_config* c) {
     if (!c) return SSTMC_INVALID_CONFIG;
@@ -88,10 +108,29 @@ static int validate(const sstmc_config* c) {
         if (s->num == 0 || !s->x || !s->y || (dims == 3 && !s->z)) return SSTMC_INVALID_CONFIG;
     }
     const sstmc_sampler& sp = c->sampler;
-    if (sp.type != SSTMC_SAMPLER_TEMPORAL && sp.type != 0) return SSTMC_NOT_SUPPORTED;
+    if (sp.type != SSTMC_SAMPLER_TEMPORAL && sp.type != SSTMC_SAMPLER_TEMPORAL_CORRELATION && sp.type != 0)
+        return SSTMC_NOT_SUPPORTED;
     if (sp.tNum == 0 || !sp.t || !sp.D) return SSTMC_INVALID_CONFIG;
     for (uint32_t m = 0; m < nmat; ++m)
         if (!(sp.D[m] >= 0)) return SSTMC_INVALID_CONFIG;   // MatlabInterface.cu:1293-1297
+    if (sp.type == SSTMC_SAMPLER_TEMPORAL_CORRELATION) {
+        // the correlation form of coherent backscattering (CBS02-CBS04, NextEventEstimation.cuh:
+        // 1237-1383) is not built; the sampler's output is already a correlation
+        if (c->tracer.isCBS) return SSTMC_NOT_SUPPORTED;
+        if (c->tracer.output != SSTMC_OUTPUT_FIELD) return SSTMC_INVALID_CONFIG;
+        // illumination2 / view2: present, and as large as illumination / view (MatlabInterface.cu:
+        // 1313-1331); the u1 and u2 sets share one source type and area flag here
+        const sstmc_source* pair[2] = {&c->illumination2, &c->view2};
+        for (int i = 0; i < 2; ++i) {
+            const sstmc_source* s2 = pair[i];
+            const sstmc_source* s1 = srcs[i];
+            if (s2->type != SSTMC_SOURCE_FARFIELD && s2->type != SSTMC_SOURCE_POINT) return SSTMC_INVALID_CONFIG;
+            if (s2->num != s1->num || !s2->x || !s2->y || (dims == 3 && !s2->z)) return SSTMC_INVALID_CONFIG;
+            if (s2->type != s1->type || (s2->isAreaSource != 0) != (s1->isAreaSource != 0)) return SSTMC_NOT_SUPPORTED;
+        }
+        if ((uint64_t)c->illumination.num * 2 > 0xffffffffull || (uint64_t)c->view.num * 2 > 0xffffffffull)
+            return SSTMC_INVALID_CONFIG;
+    }
     if (c->simulation.lambda < 0 || !std::isfinite(c->simulation.lambda)) return SSTMC_INVALID_CONFIG;
     if (c->precision.terms != SSTMC_TERMS_FP32 && c->precision.terms != SSTMC_TERMS_FP64) return SSTMC_INVALID_CONFIG;
     if (c->precision.contraction < SSTMC_CONTRACT_BF16X3 || c->precision.contraction > SSTMC_CONTRACT_FP64)
@@ -162,14 +201,20 @@ static int create(const sstmc_config* c, sstmc_handle** out) {
     if ((err = h->medium->setMaterial(md.sigs, md.siga, nmat))) return err;
 
     // sources (MatlabInterface.cu:1062-1215)
-    if ((err = buildSource(c->illumination, *h->medium, dims, true, &h->illumination))) return err;
-    if ((err = buildSource(c->view, *h->medium, dims, false, &h->view))) return err;
-
-    // sampler (MatlabInterface.cu:1217-1312)
     const sstmc_sampler& sp = c->sampler;
-    if (sp.type != SSTMC_SAMPLER_TEMPORAL && sp.type != 0) return SSTMC_NOT_SUPPORTED;
+    const bool correlation = sp.type == SSTMC_SAMPLER_TEMPORAL_CORRELATION;
+    if (correlation) {
+        if ((err = buildSourcePair(c->illumination, c->illumination2, *h->medium, dims, true, &h->illumination)))
+            return err;
+        if ((err = buildSourcePair(c->view, c->view2, *h->medium, dims, false, &h->view))) return err;
+    } else {
+        if ((err = buildSource(c->illumination, *h->medium, dims, true, &h->illumination))) return err;
+        if ((err = buildSource(c->view, *h->medium, dims, false, &h->view))) return err;
+    }
+
+    // sampler (MatlabInterface.cu:1217-1368)
     if (sp.tNum == 0 || !sp.t || !sp.D) return SSTMC_INVALID_CONFIG;
-    h->sampler.reset(new TemporalSampler(&err, sp.t, sp.tNum, sp.isTemporalAmplitude != 0));
+    h->sampler.reset(new TemporalSampler(&err, sp.t, sp.tNum, sp.isTemporalAmplitude != 0, correlation));
     if (err) return err;
     for (uint32_t m = 0; m < nmat; ++m) {
         if (sp.D[m] < 0) return SSTMC_INVALID_CONFIG;   // MatlabInterface.cu:1293-1297

@@ -213,14 +213,21 @@ struct SourceView {
 };
 
 // Temporal frames: TemporalSampler.cuh:14-38 + the time table built in its ctor (:941-1132).
+// With the TemporalCorrelationSampler (analytic != 0; TemporalSamplerCorrelation.cuh:14-31) the
+// nt entries are time lags, no vertex is displaced (trivial) and the kernels apply the motion
+// analytically from ticks, Dd and Ud.
 struct FramesView {
-    uint32_t nt;                   // output frames
+    uint32_t nt;                   // output frames (or lags)
     uint32_t neg;                  // slots of the negative chain
     const float* dt;               // per chain slot
     int trivial;                   // every displacement is exactly zero (dt*U == 0 and D*dt == 0)
     int temporalAmp;               // isTemporalAmplitude
     float D[kMaxMaterials];
     float U[kMaxMaterials][3];
+    int analytic;                  // TemporalCorrelationSampler
+    const double* ticks;           // the lags tau (analytic), nt entries
+    double Dd[kMaxMaterials];      // D and U in FP64 (analytic)
+    double Ud[kMaxMaterials][3];
 };
 
 // One batch of vertices (SoA), written by the transport kernel.

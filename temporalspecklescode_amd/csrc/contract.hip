@@ -300,10 +300,12 @@ template <typename R> struct SingleT;
 template <> struct SingleT<float> {
     typedef float2 C;
     __device__ static float sq(float x) { return sqrtf(x); }
+    __device__ static float ex(float x) { return __expf(x); }
 };
 template <> struct SingleT<double> {
     typedef double2 C;
     __device__ static double sq(double x) { return sqrt(x); }
+    __device__ static double ex(double x) { return exp(x); }
 };
 
 // Single-scattering amplitude at cosine c (complex: a tabular amplitude carries a phase).
@@ -453,6 +455,180 @@ hipError_t launchSingleGeneric(int dims, int fp64, const ScatterView& sc, const 
         else
             hipLaunchKernelGGL((k_single_generic<3, float>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
                                (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr);
+        for (uint32_t t = 0; t < NT; ++t)
+            hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st,
+                               (const float2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,
+                               (const double2*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- correlation sampler single scattering
+// S_p[l,v,t] = a_p[l,t] b_p[v,t] Fs_u1(l, v) conj(Fs_u2(l, v)) exp(-k^2 |D tau_t| |q|^2) for the
+// TemporalCorrelationSampler (NextEventEstimation.cuh:1016-1136): the sources hold u1 (i < L, V)
+// then u2 (L + i, V + i); both amplitudes see x1 at -tau/2 (copyPointsToBuffersThreePoints,
+// TemporalSamplerCorrelation.cuh:289-313, quirk Q3); q = (i_l1 + i_l2 - o_v1 - o_v2) / 2 from the
+// unshifted x1, i the incoming and o the outgoing (toward the view) directions
+// (temporalCorrelationSourceContributionSingle, SourceInterface.cuh:217-277). Block = 16 x 16 (l, v)
+// outputs; per chunk of 16 paths the directions, terms and decorrelation rates are staged in LDS.
+__device__ inline void unitTo(const double* src, const double x[3], bool toward, int dims, double out[3]) {
+    // n(x - src) (toward = false) or n(src - x) (toward = true)
+    double q[3] = {x[0] - src[0], x[1] - src[1], dims == 3 ? x[2] - src[2] : 0.0};
+    if (toward) { q[0] = -q[0]; q[1] = -q[1]; q[2] = -q[2]; }
+    const double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    if (n > 0) { out[0] = q[0] / n; out[1] = q[1] / n; out[2] = q[2] / n; }
+    else { out[0] = 0; out[1] = 0; out[2] = 1; }
+}
+
+template <int DIMS, typename R>
+__global__ void __launch_bounds__(256) k_single_corrsampler(ScatterView sc, SourceView il, SourceView vw,
+                                                            FramesView fr, VertexBatch vb,
+                                                            const typename SingleT<R>::C* __restrict__ a,
+                                                            const typename SingleT<R>::C* __restrict__ b, uint32_t P,
+                                                            uint32_t pathsPerSplit, uint32_t splits, double kw2,
+                                                            typename SingleT<R>::C* __restrict__ part) {
+    typedef typename SingleT<R>::C C;
+    __shared__ R su1[16][16][3], su2[16][16][3], sw1[16][16][3], sw2[16][16][3], sh[16][16][3], sk[16][16][3];
+    __shared__ C sa[16][16], sb[16][16];
+    __shared__ uint32_t sm[16];
+    __shared__ R sr[16];
+    const uint32_t L = il.n / 2, V = vw.n / 2;
+    const uint32_t tl = threadIdx.x & 15, tq = threadIdx.x >> 4;
+    const uint32_t split = blockIdx.z % splits, t = blockIdx.z / splits;
+    const uint32_t l = blockIdx.x * 16 + tl, v = blockIdx.y * 16 + tq;
+    const uint32_t pBegin = split * pathsPerSplit, pEnd = min(P, pBegin + pathsPerSplit);
+    const bool iso = il.type == 1 && sc.allIsotropic;
+    const double tau = fr.ticks[t], s = 0.5 * tau;
+    R re = 0, im = 0;
+    for (uint32_t pc = pBegin; pc < pEnd; pc += 16) {
+        const uint32_t p = pc + tq;
+        const uint32_t ls = blockIdx.x * 16 + tl, vs = blockIdx.y * 16 + tl;
+        C av, bv;
+        av.x = av.y = 0; bv.x = bv.y = 0;
+        double x1[3] = {0, 0, 0}, xa[3] = {0, 0, 0};
+        uint32_t m1 = 0;
+        double rate = 0;
+        const bool ok = p < pEnd && vb.k[p] > 0 && (uint64_t)vb.off[P] <= vb.cap;
+        if (ok) {
+            const float4 xm = vb.xm[vb.off[p]];
+            m1 = __float_as_uint(xm.w);
+            x1[0] = xm.x; x1[1] = xm.y; x1[2] = xm.z;
+            for (int c = 0; c < 3; ++c) xa[c] = x1[c] - s * fr.Ud[m1][c];
+            rate = kw2 * fabs(fr.Dd[m1] * tau);
+            if (ls < L) av = a[((size_t)t * P + p) * L + ls];
+            if (vs < V) bv = b[((size_t)t * P + p) * V + vs];
+        }
+        // incoming directions of u1 / u2 illumination ls at x1 shifted (amplitudes) and their mean
+        // at x1 (decorrelation); outgoing directions toward u1 / u2 view vs likewise
+        double u1[3] = {0, 0, 1}, u2[3] = {0, 0, 1}, h[3] = {0, 0, 0};
+        double w1[3] = {0, 0, 1}, w2[3] = {0, 0, 1}, g[3] = {0, 0, 0};
+        if (ls < L) {
+            const double* d1 = il.pd + 3 * (size_t)ls;
+            const double* d2 = il.pd + 3 * (size_t)(L + ls);
+            double c1[3], c2[3];
+            if (il.type == 2) {
+                for (int c = 0; c < 3; ++c) { u1[c] = c1[c] = d1[c]; u2[c] = c2[c] = d2[c]; }
+            } else {
+                unitTo(d1, xa, false, DIMS, u1); unitTo(d2, xa, false, DIMS, u2);
+                unitTo(d1, x1, false, DIMS, c1); unitTo(d2, x1, false, DIMS, c2);
+            }
+            for (int c = 0; c < 3; ++c) h[c] = 0.5 * (c1[c] + c2[c]);
+        }
+        if (vs < V) {
+            const double* d1 = vw.pd + 3 * (size_t)vs;
+            const double* d2 = vw.pd + 3 * (size_t)(V + vs);
+            double c1[3], c2[3];
+            if (vw.type == 2) {
+                for (int c = 0; c < 3; ++c) { w1[c] = c1[c] = d1[c]; w2[c] = c2[c] = d2[c]; }
+            } else {
+                unitTo(d1, xa, true, DIMS, w1); unitTo(d2, xa, true, DIMS, w2);
+                unitTo(d1, x1, true, DIMS, c1); unitTo(d2, x1, true, DIMS, c2);
+            }
+            for (int c = 0; c < 3; ++c) g[c] = 0.5 * (c1[c] + c2[c]);
+        }
+        if (DIMS == 2) { u1[2] = u2[2] = w1[2] = w2[2] = h[2] = g[2] = 0; }
+        for (int c = 0; c < 3; ++c) {
+            su1[tq][tl][c] = (R)u1[c]; su2[tq][tl][c] = (R)u2[c]; sh[tq][tl][c] = (R)h[c];
+            sw1[tq][tl][c] = (R)w1[c]; sw2[tq][tl][c] = (R)w2[c]; sk[tq][tl][c] = (R)g[c];
+        }
+        sa[tq][tl] = av;
+        sb[tq][tl] = bv;
+        if (tl == 0) { sm[tq] = m1; sr[tq] = (R)rate; }
+        __syncthreads();
+        for (int q = 0; q < 16; ++q) {
+            const C x = sa[q][tl], y = sb[q][tq];
+            C f1, f2;
+            if (iso) {
+                f1.x = f2.x = DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678;
+                f1.y = f2.y = 0;
+            } else {
+                const R c1 = su1[q][tl][0] * sw1[q][tq][0] + su1[q][tl][1] * sw1[q][tq][1] + su1[q][tl][2] * sw1[q][tq][2];
+                const R c2 = su2[q][tl][0] * sw2[q][tq][0] + su2[q][tl][1] * sw2[q][tq][1] + su2[q][tl][2] * sw2[q][tq][2];
+                f1 = singleAmp<DIMS, R>(sc, sm[q], c1);
+                f2 = singleAmp<DIMS, R>(sc, sm[q], c2);
+            }
+            const R dx = sh[q][tl][0] - sk[q][tq][0], dy = sh[q][tl][1] - sk[q][tq][1], dz = sh[q][tl][2] - sk[q][tq][2];
+            const R w = SingleT<R>::ex(-sr[q] * (dx * dx + dy * dy + dz * dz));
+            // F = f1 conj(f2) w; S = a b F
+            const R fr_ = (f1.x * f2.x + f1.y * f2.y) * w, fi = (f1.y * f2.x - f1.x * f2.y) * w;
+            const R pr = x.x * y.x - x.y * y.y, pi = x.x * y.y + x.y * y.x;
+            re = fma(fr_, pr, fma(-fi, pi, re));
+            im = fma(fr_, pi, fma(fi, pr, im));
+        }
+        __syncthreads();
+    }
+    if (l < L && v < V) {
+        C c;
+        c.x = re;
+        c.y = im;
+        part[((size_t)blockIdx.z) * L * V + (size_t)v * L + l] = c;
+    }
+}
+
+hipError_t launchSingleCorrelation(int dims, int fp64, const ScatterView& sc, const SourceView& il,
+                                   const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
+                                   const void* b, uint32_t P, double lambda, double* U, Workspace* ws,
+                                   hipStream_t st) {
+    const uint32_t L = il.n / 2, V = vw.n / 2, NT = fr.nt;
+    uint32_t tiles = ((L + 15) / 16) * ((V + 15) / 16);
+    uint32_t splits = chooseSplits(tiles, P, 256);
+    uint32_t pps = ((P + splits - 1) / splits + 15) / 16 * 16;
+    splits = (P + pps - 1) / pps;
+    if (splits < 1) splits = 1;
+    size_t elem = fp64 ? 16 : 8;
+    size_t need = (size_t)splits * NT * L * V * elem;
+    if (ws->bytes < need) {
+        if (ws->ptr) {
+            hipStreamSynchronize(st);
+            hipFree(ws->ptr);
+        }
+        ws->ptr = nullptr;
+        ws->bytes = 0;
+        hipError_t e = hipMalloc(&ws->ptr, need);
+        if (e != hipSuccess) return e;
+        ws->bytes = need;
+    }
+    const double kw2 = 39.478417604357434 / (lambda * lambda);
+    dim3 grid((L + 15) / 16, (V + 15) / 16, splits * NT);
+    const uint32_t LV = L * V;
+    if (fp64) {
+        if (dims == 2)
+            hipLaunchKernelGGL((k_single_corrsampler<2, double>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
+                               (const double2*)a, (const double2*)b, P, pps, splits, kw2, (double2*)ws->ptr);
+        else
+            hipLaunchKernelGGL((k_single_corrsampler<3, double>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
+                               (const double2*)a, (const double2*)b, P, pps, splits, kw2, (double2*)ws->ptr);
+        for (uint32_t t = 0; t < NT; ++t)
+            hipLaunchKernelGGL(k_reduce<double2>, dim3((LV + 255) / 256), dim3(256), 0, st,
+                               (const double2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,
+                               (const double2*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
+    } else {
+        if (dims == 2)
+            hipLaunchKernelGGL((k_single_corrsampler<2, float>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
+                               (const float2*)a, (const float2*)b, P, pps, splits, kw2, (float2*)ws->ptr);
+        else
+            hipLaunchKernelGGL((k_single_corrsampler<3, float>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
+                               (const float2*)a, (const float2*)b, P, pps, splits, kw2, (float2*)ws->ptr);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st,
                                (const float2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,

@@ -8,6 +8,9 @@ namespace sstmc {
 // NEE outputs of one batch (element type float2 for FP32 terms, double2 for FP64 terms).
 // Layouts are frame-major so each frame is a (P x L) / (P x V) row-major operand of the
 // contraction: A[t][p][l], B[t][p][v], a[t][p][l], b[t][p][v].
+// With the TemporalCorrelationSampler each term is already a u1 conj(u2) product at lag t
+// (A: with 1 / p2 and the decorrelation at x1; B: summed over k >= 2 with the decorrelation at x_k
+// and at the inner vertices; a: g_l1 conj(g_l2) / p1; b: g_v1 conj(g_v2)).
 struct NeeOut {
     void* A;          // multiple-scattering illumination terms (or A(0) conj A(t) for correlation)
     void* B;          // multiple-scattering view terms (or sum_k G_k(0) conj G_k(t))
@@ -23,7 +26,8 @@ struct NeeOut {
 // How single scattering is contracted (decided on the host from the source/scattering types).
 enum SingleMode : int {
     kSingleSeparable = 0,  // S = (a^T b) o F with F a fixed L x V matrix or scalar
-    kSingleGeneric = 1     // S_p[l,v] needs f(l -> x1 -> v) per path (fused kernel)
+    kSingleGeneric = 1,    // S_p[l,v] needs f(l -> x1 -> v) per path (fused kernel)
+    kSingleCorrelation = 2 // TemporalCorrelationSampler: f1 conj(f2) times the decorrelation at x1
 };
 
 struct NeeParams {
@@ -38,6 +42,7 @@ struct NeeParams {
     uint32_t viewBlocks;   // view kernels: blocks of lppView sources per path and frame chunk
     uint32_t frameChunks;  // view kernels: frame chunks per path (field output: one frame each,
                            // correlation: up to 16 frames each)
+    uint32_t sameViews;    // TemporalCorrelationSampler: view2 holds the same sources as view
 };
 
 hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc, const SourceView& il,
@@ -69,6 +74,13 @@ hipError_t launchContract(int kind, const void* A, const void* B, uint32_t P, ui
 hipError_t launchSingleGeneric(int dims, int fp64, const ScatterView& sc, const SourceView& il,
                                const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
                                const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st);
+
+// Single scattering of the TemporalCorrelationSampler (sources il / vw hold u1 then u2):
+// U[t][v][l] += sum_p a[t][p][l] b[t][p][v] Fs_u1 conj(Fs_u2) exp(-k^2 |D tau_t| |q|^2).
+hipError_t launchSingleCorrelation(int dims, int fp64, const ScatterView& sc, const SourceView& il,
+                                   const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
+                                   const void* b, uint32_t P, double lambda, double* U, Workspace* ws,
+                                   hipStream_t st);
 
 // U += 0 helpers
 hipError_t launchZeroU(double* U, size_t n, hipStream_t st);

@@ -418,6 +418,156 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
 
 
 
+// ---------------------------------------------------------------- TemporalCorrelationSampler
+// Position of a vertex seen at time s by the correlation sampler: x + s U_m, FP64 (exact sum of
+// the FP32 position and the FP64 shift) with its FP32 rounding for the walk.
+__device__ inline Geo shiftGeo(const float4& xm, const double* U, double s) {
+    Geo g;
+    g.pd = mkd((double)xm.x + s * U[0], (double)xm.y + s * U[1], (double)xm.z + s * U[2]);
+    g.pf = mk((float)g.pd.x, (float)g.pd.y, (float)g.pd.z);
+    return g;
+}
+
+// First-vertex work of one path for the TemporalCorrelationSampler, on a group of lpp lanes. The
+// sources hold the u1 sources (i < L) then the u2 ones (L + i). For lag tau: u1's g sees x1 at
+// -tau/2, u2's g at +tau/2 (copyPointsToBuffersThroughput, TemporalSamplerCorrelation.cuh:228-250);
+// the three-point functions of both see x1 and x2 at -tau/2 (copyPointsToBuffersThreePoints,
+// :253-313, quirk Q3); the decorrelation at x1 is exp(-k^2 |D tau| |q|^2) with q the mean u1/u2
+// incoming direction minus n(x2 - x1), from the unshifted positions (C1_l1l2,
+// temporalCorrelationSourceContributionMultiple, SourceInterface.cuh:280-352). Path
+// probabilities: means over the 2L sources (pathSingleProbability, :783-829), the second-point
+// factor with the u1 directions for both halves (pathMultipleProbability, :833-856), no square
+// root (NextEventEstimation.cuh:851-880, 930-977). Outputs:
+//   a[t][p][l] = g_l1 conj(g_l2) / p1,     b[t][p][v] = g_v1 conj(g_v2)
+//   A[t][p][l] = g_l1 f_l1 conj(g_l2 f_l2) exp(-k^2 |D1 tau| |q|^2) / p2
+template <int DIMS, typename R, bool TAB>
+__device__ inline void nee_path_corr(const MediumView& m, const R* sig, const ScatterView& sc, const SourceView& il,
+                                     const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
+                                     const NeeParams& prm, const NeeOut& out, uint32_t p, uint32_t sub, uint32_t lpp,
+                                     uint32_t* stepsOut) {
+    typedef typename Cx<R>::T C;
+    const uint32_t P = prm.npaths, L = il.n / 2, V = vw.n / 2, NT = fr.nt;
+    const double invLambda = 1.0 / prm.lambda;
+    const double kw2 = 39.478417604357434 * invLambda * invLambda;   // (2 pi / lambda)^2
+    const R tauCutFF = sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD;
+    const R tauInf = (R)3e38;
+    C* A = (C*)out.A;
+    C* B = (C*)out.B;
+    C* a = (C*)out.a;
+    C* b = (C*)out.b;
+    const bool poolOk = (uint64_t)vb.off[prm.npaths] <= vb.cap;
+    const uint32_t K = poolOk ? vb.k[p] : 0u;
+    const uint64_t off = vb.off[p];
+    const C zero = Cx<R>::make(0, 0);
+    if (K == 0) {
+        for (uint32_t i = sub; i < L * NT; i += lpp) {
+            const size_t o = ((size_t)(i / L) * P + p) * L + i % L;
+            A[o] = zero;
+            a[o] = zero;
+        }
+        for (uint32_t i = sub; i < V * NT; i += lpp) {
+            const size_t o = ((size_t)(i / V) * P + p) * V + i % V;
+            B[o] = zero;
+            b[o] = zero;
+        }
+        return;
+    }
+    uint32_t steps = 0;
+    const float4 x1m = vb.xm[off];
+    const uint32_t m1 = __float_as_uint(x1m.w);
+    const bool multi = K >= 2;
+    const float4 x2m = multi ? vb.xm[off + 1] : x1m;
+    const uint32_t m2 = __float_as_uint(x2m.w);
+    const Geo x1c = {mk(x1m.x, x1m.y, x1m.z), mkd(x1m.x, x1m.y, x1m.z)};
+    const d3 seg12 = dnormalize<DIMS>(dsub(mkd(x2m.x, x2m.y, x2m.z), x1c.pd));
+
+    R s1 = 0, s2 = 0;
+    for (uint32_t l = sub; l < 2 * L; l += lpp) {
+        R pr;
+        if (il.type == 2) {
+            typename Pos<R>::V p1 = Pos<R>::pick(x1c.pf, x1c.pd);
+            typename Pos<R>::V w;
+            if (sizeof(R) == 4) w = Pos<R>::fromD(mkd(-(double)il.pf[3 * l], -(double)il.pf[3 * l + 1], -(double)il.pf[3 * l + 2]));
+            else w = Pos<R>::fromD(mkd(-il.pd[3 * l], -il.pd[3 * l + 1], -il.pd[3 * l + 2]));
+            R tau = walk<DIMS, R>(m, sig, p1, w, sizeof(R) == 4 ? (R)prm.tauCutF : tauInf, &steps);
+            pr = expR<R>(-tau) * (sizeof(R) == 4 ? (R)il.probf[l] : (R)il.probd[l]);
+        } else {
+            d3 sp = mkd(il.pd[3 * l], il.pd[3 * l + 1], DIMS == 3 ? il.pd[3 * l + 2] : 0.0);
+            d3 q = dsub(sp, x1c.pd);
+            double rr = 1.0 / sqrt(ddot<DIMS>(q, q));
+            typename Pos<R>::V w = Pos<R>::fromD(mkd(q.x * rr, q.y * rr, q.z * rr));
+            R tau = walk<DIMS, R>(m, sig, Pos<R>::pick(x1c.pf, x1c.pd), w, tauInf, &steps);
+            double fac = DIMS == 3 ? rr * rr / 12.566370614359172954 : rr * 25.132741228718345908;
+            pr = expR<R>(-tau) * (R)fac;
+        }
+        s1 += pr;
+        if (multi) s2 += pr * phasePdf<DIMS, R, TAB>(sc, m1, ddot<DIMS>(illumIn<DIMS>(il, l % L, x1c.pd), seg12));
+    }
+    s1 = groupSum(s1, lpp);
+    s2 = groupSum(s2, lpp);
+    const R p1 = s1 / (R)(2 * L), p2 = s2 / (R)(2 * L);
+    const R w1 = p1 > 0 ? (R)1 / p1 : (R)0;
+    const R w2 = p2 > 0 ? (R)1 / p2 : (R)0;
+    const bool isoPointIl = il.type == 1 && sc.allIsotropic;
+    const R tauIl = il.type == 2 ? tauCutFF : tauInf, tauVw = vw.type == 2 ? tauCutFF : tauInf;
+
+    for (uint32_t i = sub; i < L * NT; i += lpp) {
+        const uint32_t l = i % L, t = i / L;
+        const double tau = fr.ticks[t], s = 0.5 * tau;
+        const Geo x1a = shiftGeo(x1m, fr.Ud[m1], -s), x1b = shiftGeo(x1m, fr.Ud[m1], s);
+        R g1, g2;
+        double gp1, gp2;
+        gTerm<DIMS, R>(m, sig, il, l, x1a, invLambda, tauIl, &steps, &g1, &gp1);
+        gTerm<DIMS, R>(m, sig, il, L + l, x1b, invLambda, tauIl, &steps, &g2, &gp2);
+        const size_t o = ((size_t)t * P + p) * L + l;
+        a[o] = cis<R>(g1 * g2 * w1, gp1 - gp2);
+        if (multi) {
+            const Geo x2a = shiftGeo(x2m, fr.Ud[m2], -s);
+            const d3 dir = dnormalize<DIMS>(dsub(x2a.pd, x1a.pd));
+            R f1, f2;
+            double fp1 = 0, fp2 = 0;
+            if (isoPointIl) {
+                f1 = f2 = isoAmp<DIMS, R>();
+            } else {
+                f1 = phaseAmp<DIMS, R, TAB>(sc, m1, ddot<DIMS>(illumIn<DIMS>(il, l, x1a.pd), dir), &fp1);
+                f2 = phaseAmp<DIMS, R, TAB>(sc, m1, ddot<DIMS>(illumIn<DIMS>(il, L + l, x1a.pd), dir), &fp2);
+            }
+            const d3 i1 = illumIn<DIMS>(il, l, x1c.pd), i2 = illumIn<DIMS>(il, L + l, x1c.pd);
+            const d3 q = mkd(0.5 * (i1.x + i2.x) - seg12.x, 0.5 * (i1.y + i2.y) - seg12.y, 0.5 * (i1.z + i2.z) - seg12.z);
+            const double e = kw2 * fabs(fr.Dd[m1] * tau) * ddot<DIMS>(q, q);
+            A[o] = cis<R>(g1 * f1 * g2 * f2 * w2 * expR<R>((R)-e), gp1 + fp1 - gp2 - fp2);
+        } else {
+            A[o] = zero;
+        }
+    }
+    if (!multi)
+        for (uint32_t i = sub; i < V * NT; i += lpp) B[((size_t)(i / V) * P + p) * V + i % V] = zero;
+    for (uint32_t i = sub; i < V * NT; i += lpp) {
+        const uint32_t v = i % V, t = i / V;
+        const double s = 0.5 * fr.ticks[t];
+        const Geo x1a = shiftGeo(x1m, fr.Ud[m1], -s), x1b = shiftGeo(x1m, fr.Ud[m1], s);
+        R g1, g2;
+        double gp1, gp2;
+        gTerm<DIMS, R>(m, sig, vw, v, x1a, invLambda, tauVw, &steps, &g1, &gp1);
+        gTerm<DIMS, R>(m, sig, vw, V + v, x1b, invLambda, tauVw, &steps, &g2, &gp2);
+        b[((size_t)t * P + p) * V + v] = cis<R>(g1 * g2, gp1 - gp2);
+    }
+    *stepsOut += steps;
+}
+
+template <int DIMS, typename R, bool TAB>
+__global__ void __launch_bounds__(256) k_nee_corr(MediumView m, ScatterView sc, SourceView il, SourceView vw,
+                                                  FramesView fr, VertexBatch vb, NeeParams prm, NeeOut out) {
+    const uint32_t lane = threadIdx.x & 63, lpp = prm.lpp, sub = lane & (lpp - 1);
+    const uint32_t p = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / lpp) + lane / lpp;
+    __shared__ R sigLds[kSigLds];
+    const R* sig = loadSigTable<R>(m, sigLds);
+    uint32_t steps = 0;
+    if (p < prm.npaths) nee_path_corr<DIMS, R, TAB>(m, sig, sc, il, vw, fr, vb, prm, out, p, sub, lpp, &steps);
+    uint32_t tot = waveSum(steps);
+    if (lane == 0 && out.steps) atomicAdd(out.steps, (unsigned long long)tot);
+}
+
 // Vertex records for the view terms: one thread per path walks its vertices once per frame and
 // writes, for every vertex k >= 2, the unit vector back to the previous vertex (the direction the
 // view-side HG amplitude is evaluated against, FarField.cuh:546-599 / PointSource.cuh:347-446)
@@ -618,6 +768,139 @@ __global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView
     if ((threadIdx.x & 63) == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
 }
 
+// One view term of the correlation sampler: g of the source at xg times the amplitude at pf
+// against the back direction, as amplitude and phase in cycles (the path phase, common to u1 and
+// u2, is left out: it cancels in u1 conj(u2)).
+template <int DIMS, typename R, int SRC, bool TAB>
+__device__ inline void corrViewG(const MediumView& m, const R* sig, const ScatterView& sc, const SourceView& vw,
+                                 const d3& vd, const f3& vf, const Geo& xg, const d3& pf, const d3& back, uint32_t mat,
+                                 double invLambda, R tauCut, bool isoPoint, uint32_t* steps, R* amp, double* ph) {
+    R ga;
+    double gp;
+    if constexpr (SRC == kSrcPoint) gPoint<DIMS, R>(m, sig, vd, vw.area != 0, xg, invLambda, steps, &ga, &gp);
+    else gFar<DIMS, R>(m, sig, vd, vf, false, xg, invLambda, tauCut, steps, &ga, &gp);
+    R fv;
+    if (isoPoint) {
+        fv = isoAmp<DIMS, R>();
+    } else {
+        const d3 av = SRC == kSrcFarView ? mkd(-vd.x, -vd.y, -vd.z) : dnormalize<DIMS>(dsub(pf, vd));
+        fv = phaseAmp<DIMS, R, TAB>(sc, mat, ddot<DIMS>(av, back), &gp);
+    }
+    *amp = ga * fv;
+    *ph = gp;
+}
+
+// Multiple-scattering view terms of the TemporalCorrelationSampler: per (path, chunk of up to
+// kCorrLags lags, block of views), one view per lane,
+//   B[t][p][v] = sum_{k>=2} G_k,v1(-tau/2) conj G_k,v2(+tau/2) exp(-k^2 |tau| (D_k |q_k|^2 + mt_k))
+// with q_k = -(o_v1 + o_v2) / 2 - n(x_{k-1} - x_k) (o: direction toward the view; C1_v1v2) and
+// mt_k = sum_{j=2}^{k-1} D_j |n(x_j - x_{j-1}) - n(x_{j+1} - x_j)|^2 (momentTransfer, applied on u1
+// only: pathContriobutionKernel, TemporalSamplerCorrelation.cuh:384-450), all from unshifted
+// positions. u1's g sees x_k at -tau/2, u2's at +tau/2; both amplitudes see x_{k-1} and x_k at
+// -tau/2 (quirk Q3). Vertices whose own and previous materials are at rest are lag-independent
+// except for the exponential: their G products are formed once for all the chunk's lags (once,
+// not twice, when view2 holds the same sources as view).
+template <int DIMS, typename R, int SRC, bool WAVE, bool TAB>
+__global__ void __launch_bounds__(256) k_nee_view_corrsampler(MediumView m, ScatterView sc, SourceView vw,
+                                                              FramesView fr, VertexBatch vb, NeeParams prm,
+                                                              NeeOut out) {
+    typedef typename Cx<R>::T C;
+    __shared__ R sigLds[kSigLds];
+    __shared__ C acc[kCorrLags][256];
+    const R* sig = loadSigTable<R>(m, sigLds);
+    const ViewUnit u = viewUnit<WAVE>(prm);
+    const uint32_t p = u.p, P = prm.npaths, V = vw.n / 2, NT = fr.nt;
+    const bool poolOk = (uint64_t)vb.off[P] <= vb.cap;
+    const uint32_t K = (p < P && poolOk) ? vb.k[p] : 0u;
+    const uint64_t off = p < P ? vb.off[p] : 0;
+    const uint32_t t0 = u.chunk * kCorrLags;
+    const uint32_t nt = t0 < NT ? (NT - t0 < kCorrLags ? NT - t0 : kCorrLags) : 0u;
+    const double invLambda = 1.0 / prm.lambda;
+    const double kw2 = 39.478417604357434 * invLambda * invLambda;
+    const R tauCut = SRC != kSrcPoint ? (sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD) : (R)3e38;
+    C* B = (C*)out.B;
+    const bool isoPoint = SRC == kSrcPoint && sc.allIsotropic;
+    const bool same = prm.sameViews != 0;
+    uint32_t steps = 0;
+    const uint32_t v = u.blk * (WAVE ? 64u : prm.lppView) + u.sub;
+    const bool active = v < V && K >= 2 && nt > 0;
+    const uint32_t vv = active ? v : 0;
+    const d3 vd1 = sourceD<DIMS>(vw, vv), vd2 = sourceD<DIMS>(vw, V + vv);
+    const f3 vf1 = SRC != kSrcPoint && sizeof(R) == 4 ? sourceF(vw, vv) : mk(0.f, 0.f, 0.f);
+    const f3 vf2 = SRC != kSrcPoint && sizeof(R) == 4 ? sourceF(vw, V + vv) : mk(0.f, 0.f, 0.f);
+    if (active) {
+        for (uint32_t j = 0; j < nt; ++j) acc[j][threadIdx.x] = Cx<R>::make(0, 0);
+        double mt = 0;
+        d3 prevB = mkd(0.0, 0.0, 0.0);
+        float4 xpm = vb.xm[off];
+        for (uint32_t k = 1; k < K; ++k) {
+            const uint64_t ok = off + k;
+            const float4 xkm = vb.xm[ok];
+            const uint32_t mat = __float_as_uint(xkm.w), mp = __float_as_uint(xpm.w);
+            const VertexRec rec = vb.rec[ok];
+            const d3 bk = mkd(rec.bx, rec.by, rec.bz);   // n(x_{k-1} - x_k)
+            if (k >= 2) {
+                const d3 dq = dsub(bk, prevB);
+                mt += fr.Dd[mp] * ddot<DIMS>(dq, dq);
+            }
+            prevB = bk;
+            const d3 xk = mkd(xkm.x, xkm.y, xkm.z);
+            const d3 o1 = SRC == kSrcFarView ? vd1 : dnormalize<DIMS>(dsub(vd1, xk));
+            const d3 o2 = SRC == kSrcFarView ? vd2 : dnormalize<DIMS>(dsub(vd2, xk));
+            const d3 q = mkd(-0.5 * (o1.x + o2.x) - bk.x, -0.5 * (o1.y + o2.y) - bk.y, -0.5 * (o1.z + o2.z) - bk.z);
+            const double rate = kw2 * (fr.Dd[mat] * ddot<DIMS>(q, q) + mt);   // times |tau|
+            const double* Uk = fr.Ud[mat];
+            const double* Up = fr.Ud[mp];
+            const bool moving = Uk[0] != 0.0 || Uk[1] != 0.0 || Uk[2] != 0.0 || Up[0] != 0.0 || Up[1] != 0.0 ||
+                                Up[2] != 0.0;
+            if (!moving) {
+                const Geo xg = {mk(xkm.x, xkm.y, xkm.z), xk};
+                R a1, a2;
+                double ph1, ph2;
+                corrViewG<DIMS, R, SRC, TAB>(m, sig, sc, vw, vd1, vf1, xg, xk, bk, mat, invLambda, tauCut, isoPoint,
+                                             &steps, &a1, &ph1);
+                if (same) {
+                    a2 = a1;
+                    ph2 = ph1;
+                } else {
+                    corrViewG<DIMS, R, SRC, TAB>(m, sig, sc, vw, vd2, vf2, xg, xk, bk, mat, invLambda, tauCut, isoPoint,
+                                                 &steps, &a2, &ph2);
+                }
+                const C g = cis<R>(a1 * a2, ph1 - ph2);
+                for (uint32_t j = 0; j < nt; ++j) {
+                    const R w = expR<R>((R)(-rate * fabs(fr.ticks[t0 + j])));
+                    C c = acc[j][threadIdx.x];
+                    c.x += g.x * w;
+                    c.y += g.y * w;
+                    acc[j][threadIdx.x] = c;
+                }
+            } else {
+                for (uint32_t j = 0; j < nt; ++j) {
+                    const double tau = fr.ticks[t0 + j], s = 0.5 * tau;
+                    const Geo xa = shiftGeo(xkm, Uk, -s), xb = shiftGeo(xkm, Uk, s);
+                    const Geo xpa = shiftGeo(xpm, Up, -s);
+                    const d3 back = dnormalize<DIMS>(dsub(xpa.pd, xa.pd));
+                    R a1, a2;
+                    double ph1, ph2;
+                    corrViewG<DIMS, R, SRC, TAB>(m, sig, sc, vw, vd1, vf1, xa, xa.pd, back, mat, invLambda, tauCut,
+                                                 isoPoint, &steps, &a1, &ph1);
+                    corrViewG<DIMS, R, SRC, TAB>(m, sig, sc, vw, vd2, vf2, xb, xa.pd, back, mat, invLambda, tauCut,
+                                                 isoPoint, &steps, &a2, &ph2);
+                    const C g = cis<R>(a1 * a2 * expR<R>((R)(-rate * fabs(tau))), ph1 - ph2);
+                    C c = acc[j][threadIdx.x];
+                    c.x += g.x;
+                    c.y += g.y;
+                    acc[j][threadIdx.x] = c;
+                }
+            }
+            xpm = xkm;
+        }
+        for (uint32_t j = 0; j < nt; ++j) B[((size_t)(t0 + j) * P + p) * V + v] = acc[j][threadIdx.x];
+    }
+    uint32_t tot = waveSum(steps);
+    if ((threadIdx.x & 63) == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
+}
+
 // Lanes per view-kernel work unit for n sources: a wavefront, halved (down to 8) while half still
 // covers them, so that several units share a wavefront when there are few sources.
 static inline void viewLanes(uint32_t n, NeeParams* q) {
@@ -649,12 +932,39 @@ static void launchView(bool corr, const MediumView& m, const ScatterView& sc, co
 #undef SSTMC_VIEW
 }
 
+template <int DIMS, typename R, int SRC>
+static void launchViewCorrSampler(const MediumView& m, const ScatterView& sc, const SourceView& src,
+                                  const FramesView& fr, const VertexBatch& vb, const NeeParams& q, const NeeOut& out,
+                                  hipStream_t st) {
+    const uint64_t units = (uint64_t)q.npaths * q.viewBlocks * q.frameChunks, upb = 4 * (64 / q.lppView);
+    const dim3 grid((uint32_t)((units + upb - 1) / upb)), block(256);
+    const bool wave = q.lppView == 64, tab = sc.tab != nullptr;
+#define SSTMC_VIEWC(W, T) \
+    hipLaunchKernelGGL((k_nee_view_corrsampler<DIMS, R, SRC, W, T>), grid, block, 0, st, m, sc, src, fr, vb, q, out)
+    if (wave) { if (tab) SSTMC_VIEWC(true, true); else SSTMC_VIEWC(true, false); }
+    else { if (tab) SSTMC_VIEWC(false, true); else SSTMC_VIEWC(false, false); }
+#undef SSTMC_VIEWC
+}
+
 template <int DIMS, typename R>
 static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceView& il, const SourceView& vw,
                        const FramesView& fr, const VertexBatch& vb, const NeeParams& p, const NeeOut& out,
                        hipStream_t st, hipEvent_t between) {
     const uint32_t ppb = 4 * (64 / p.lpp);   // paths per 256-thread block
     dim3 grid((p.npaths + ppb - 1) / ppb), block(256);
+    if (fr.analytic) {
+        // TemporalCorrelationSampler: the vertices never move, one record per vertex
+        if (sc.tab) hipLaunchKernelGGL((k_nee_corr<DIMS, R, true>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+        else hipLaunchKernelGGL((k_nee_corr<DIMS, R, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+        FramesView one = fr;
+        one.nt = 1;
+        hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, vb, one, p.npaths,
+                           1.0 / p.lambda);
+        if (between) hipEventRecord(between, st);
+        if (vw.type == 2) launchViewCorrSampler<DIMS, R, kSrcFarView>(m, sc, vw, fr, vb, p, out, st);
+        else launchViewCorrSampler<DIMS, R, kSrcPoint>(m, sc, vw, fr, vb, p, out, st);
+        return;
+    }
     if (sc.tab) hipLaunchKernelGGL((k_nee<DIMS, R, true>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
     else hipLaunchKernelGGL((k_nee<DIMS, R, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
     hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, vb, fr, p.npaths,
@@ -685,10 +995,16 @@ hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView&
     // field output and (path, chunk of up to kCorrLags frames, block of views) for correlation.
     NeeParams q = p;
     uint32_t want = il.n > vw.n * fr.nt ? il.n : vw.n * fr.nt;
+    if (fr.analytic) {   // sources hold u1 then u2; items: 2L probabilities, L x T and V x T terms
+        const uint32_t L = il.n / 2, V = vw.n / 2;
+        want = il.n;
+        if (L * fr.nt > want) want = L * fr.nt;
+        if (V * fr.nt > want) want = V * fr.nt;
+    }
     q.lpp = 64;
     while (q.lpp > 8 && q.lpp / 2 >= want) q.lpp /= 2;
-    const bool corr = p.output == 1;
-    viewLanes(vw.n, &q);
+    const bool corr = p.output == 1 || fr.analytic;
+    viewLanes(fr.analytic ? vw.n / 2 : vw.n, &q);
     q.frameChunks = corr ? (fr.nt + kCorrLags - 1) / kCorrLags : fr.nt;
     if (dims == 2) {
         if (fp64) launchNeeT<2, double>(m, sc, il, vw, fr, vb, q, out, st, between);

@@ -150,19 +150,26 @@ public:
 };
 
 // ---------------------------------------------------------------- Sampler
-// TemporalSampler (TemporalSampler.cuh:55-137, time table :941-1132).
+// TemporalSampler (TemporalSampler.cuh:55-137, time table :941-1132), or with `correlation` the
+// TemporalCorrelationSampler (TemporalSamplerCorrelation.cuh:41-106, 455-594): the ticks are
+// then time lags, kept as they are, and every vertex stays at its sampled position.
 class TemporalSampler {
 public:
-    TemporalSampler(int* err, const double* ticks, uint32_t ticksNum, bool isTemporalAmplitude);
+    TemporalSampler(int* err, const double* ticks, uint32_t ticksNum, bool isTemporalAmplitude,
+                    bool correlation = false);
     int setMaterial(double D, double ux, double uy, double uz, uint32_t materialNum);
     void finalize(uint32_t materialNum);
     const FramesView& view() const { return v_; }
     uint32_t frames() const { return v_.nt; }
+    bool correlation() const { return v_.analytic != 0; }
+    // some material moves (U != 0): the correlation sampler's g and f then see shifted positions
+    bool moving(uint32_t materialNum) const;
 
 private:
     FramesView v_{};
     std::vector<double> dt_;
     DevBuf<float> dtDev_;
+    DevBuf<double> ticksDev_;
     std::vector<double> D_, U_;
 };
 
@@ -249,6 +256,7 @@ private:
     uint32_t L_, V_, T_, P_;
     int singleMode_;
     double fscalar_ = 1.0;
+    bool sameViews_ = false;        // correlation sampler: view2 holds the same sources as view
     bool fp64_;
     int contractKind_;
     hipStream_t stream_ = nullptr;

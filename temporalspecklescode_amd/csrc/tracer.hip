@@ -448,10 +448,20 @@ PointSource::PointSource(int* err, int dims, const double* x, const double* y, c
 }
 
 // ====================================================================== TemporalSampler
-TemporalSampler::TemporalSampler(int* err, const double* t, uint32_t nt, bool isTemporalAmplitude) {
+TemporalSampler::TemporalSampler(int* err, const double* t, uint32_t nt, bool isTemporalAmplitude,
+                                 bool correlation) {
     *err = SSTMC_NO_ERROR;
     v_.nt = nt;
     v_.temporalAmp = isTemporalAmplitude ? 1 : 0;
+    D_.assign(kMaxMaterials, 0.0);
+    U_.assign(3 * kMaxMaterials, 0.0);
+    if (correlation) {
+        v_.analytic = 1;
+        v_.trivial = 1;
+        if ((*err = ticksDev_.upload(t, nt))) return;
+        v_.ticks = ticksDev_.p;
+        return;
+    }
     // time table of the TemporalSampler constructor (TemporalSampler.cuh:941-1132): frames below
     // the mean form the negative chain, the rest the positive chain, each ordered by distance to
     // the mean; a slot's step is its time minus the previous slot's (or minus the mean).
@@ -491,8 +501,6 @@ TemporalSampler::TemporalSampler(int* err, const double* t, uint32_t nt, bool is
     for (uint32_t k = 0; k < nt; ++k) f[k] = (float)dt_[k];
     if ((*err = dtDev_.upload(f.data(), nt))) return;
     v_.dt = dtDev_.p;
-    D_.assign(kMaxMaterials, 0.0);
-    U_.assign(3 * kMaxMaterials, 0.0);
 }
 
 int TemporalSampler::setMaterial(double D, double ux, double uy, double uz, uint32_t m) {
@@ -501,10 +509,19 @@ int TemporalSampler::setMaterial(double D, double ux, double uy, double uz, uint
     U_[3 * m] = ux; U_[3 * m + 1] = uy; U_[3 * m + 2] = uz;
     v_.D[m] = (float)D;
     v_.U[m][0] = (float)ux; v_.U[m][1] = (float)uy; v_.U[m][2] = (float)uz;
+    v_.Dd[m] = D;
+    v_.Ud[m][0] = ux; v_.Ud[m][1] = uy; v_.Ud[m][2] = uz;
     return SSTMC_NO_ERROR;
 }
 
+bool TemporalSampler::moving(uint32_t materialNum) const {
+    for (uint32_t m = 1; m < materialNum && m < (uint32_t)kMaxMaterials; ++m)
+        if (U_[3 * m] != 0.0 || U_[3 * m + 1] != 0.0 || U_[3 * m + 2] != 0.0) return true;
+    return false;
+}
+
 void TemporalSampler::finalize(uint32_t materialNum) {
+    if (v_.analytic) return;   // the correlation sampler never displaces a vertex
     bool trivial = true;
     for (uint32_t k = 0; k < v_.nt; ++k)
         for (uint32_t m = 1; m < materialNum; ++m) {
@@ -521,8 +538,10 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
     : medium_(medium), sc_(scattering), il_(illumination), vw_(view), sampler_(sampler), opt_(opt) {
     seed_ = opt.seed;
     dims_ = medium.dims();
-    L_ = il_.size();
-    V_ = vw_.size();
+    // the correlation sampler's sources hold the u1 sources then the u2 ones
+    const bool acorr = sampler_.correlation();
+    L_ = acorr ? il_.size() / 2 : il_.size();
+    V_ = acorr ? vw_.size() / 2 : vw_.size();
     T_ = sampler_.frames();
     P_ = opt.batchSize;
     fp64_ = opt.terms == SSTMC_TERMS_FP64;
@@ -545,7 +564,21 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
         singleMode_ = kSingleSeparable;
         fscalar_ = iso;
     }
-    if (opt_.output == SSTMC_OUTPUT_CORRELATION) {
+    if (acorr) {
+        // TemporalCorrelationSampler single scattering (NextEventEstimation.cuh:1016-1136): with a
+        // scalar Fs (point illumination and view, isotropic scattering: Sampler::isScalarFs,
+        // SamplerInterface.cuh:155, PointSource.cuh:62) it is Fs^2 a^T b; otherwise each (l, v, path)
+        // takes Fs_u1 conj(Fs_u2) and the decorrelation at x1
+        needF = false;
+        fscalar_ = 1.0;
+        singleMode_ = kSingleCorrelation;
+        if (il_.type() == SSTMC_SOURCE_POINT && vw_.type() == SSTMC_SOURCE_POINT && sc_.isIsotropicScattering()) {
+            singleMode_ = kSingleSeparable;
+            fscalar_ = iso * iso;
+        }
+        const std::vector<double>& hv = vw_.hostPoints();
+        sameViews_ = std::equal(hv.begin(), hv.begin() + 3 * (size_t)V_, hv.begin() + 3 * (size_t)V_);
+    } else if (opt_.output == SSTMC_OUTPUT_CORRELATION) {
         if (singleMode_ != kSingleSeparable) { *err = SSTMC_NOT_SUPPORTED; return; }
         fscalar_ *= fscalar_;
     }
@@ -599,13 +632,15 @@ VertexBatch NextEventEstimation::vertexBatch() const {
 
 int NextEventEstimation::growPool(uint64_t cap) {
     int e;
-    if ((e = xm_.alloc(cap)) || (e = rho_.alloc(cap)) || (e = rec_.alloc(cap * T_))) return e;
+    // vertex records per frame; the correlation sampler's vertices do not move: one record each
+    const uint64_t recT = sampler_.correlation() ? 1 : T_;
+    if ((e = xm_.alloc(cap)) || (e = rho_.alloc(cap)) || (e = rec_.alloc(cap * recT))) return e;
     if (!sampler_.view().trivial) {
         if ((e = delta_.alloc(cap * T_ * 3))) return e;
     } else {
         delta_.release();
     }
-    const uint64_t perVertex = 20 + 32 * (uint64_t)T_ + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_);
+    const uint64_t perVertex = 20 + 32 * recT + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_);
     bytes_ += (cap - cap_) * perVertex;
     cap_ = cap;
     return SSTMC_NO_ERROR;
@@ -618,6 +653,9 @@ int NextEventEstimation::contractSingle(uint32_t npaths, double* U) {
     if (singleMode_ == kSingleSeparable) {
         he = launchContract(contractKind_, a_.p, b_.p, npaths, L_, V_, T_, Ff_.p, Fd_.p,
                             opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0, fscalar_, U, &wsSingle_, stream_);
+    } else if (singleMode_ == kSingleCorrelation) {
+        he = launchSingleCorrelation(dims_, fp64_ ? 1 : 0, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p,
+                                     npaths, opt_.lambda, U, &wsSingle_, stream_);
     } else {
         he = launchSingleGeneric(dims_, fp64_ ? 1 : 0, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p, npaths, U,
                                  &wsSingle_, stream_);
@@ -646,6 +684,7 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     prm.tauCutF = 210.0f;     // exp(-105) rounds to zero in FP32: far-field walks stop there
     prm.tauCutD = 1.0e300;
     prm.npaths = npaths;
+    prm.sameViews = sameViews_ ? 1u : 0u;
     NeeOut out{A_.p, B_.p, a_.p, b_.p, Ar_.p, Br_.p, totals_.p + 2};
     if (launchNee(dims_, fp64_ ? 1 : 0, m, sc, il, vw, fr, vb, prm, out, stream_, e ? e[5] : nullptr) != hipSuccess)
         return SSTMC_KERNEL_ERROR;

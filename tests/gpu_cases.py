@@ -158,3 +158,33 @@ def with_vmf(cfg, uniform=False, mixed=False):
             new_a.append(VMF_MIXTURES[0 if uniform else m % len(VMF_MIXTURES)])
     cfg["scattering"] = {"type": new_t, "amplitudeFunction": new_a}
     return cfg
+
+
+def corr_sampler(cfg, lags, D, U=None, il2=None, vw2=None):
+    """The same medium, scattering and sources under the TemporalCorrelationSampler (lags `lags`,
+    Brownian D and velocity U per material); illumination2 / view2 default to the u1 sources."""
+    c = dict(cfg)
+    nm = len(D)
+    U = np.zeros((nm, 3)) if U is None else np.asarray(U, np.float64)
+    c["sampler"] = {"type": "TemporalCorrelationSampler", "t": np.asarray(lags, np.float64),
+                    "D": np.asarray(D, np.float64), "U": {"x": U[:, 0], "y": U[:, 1], "z": U[:, 2]}}
+    c["tracer"] = {"type": "nee"}
+    c["illumination2"] = il2 if il2 is not None else cfg["illumination"]
+    c["view2"] = vw2 if vw2 is not None else cfg["view"]
+    return c
+
+
+def perturbed(src, seed, amount):
+    """A second source set near `src`: far-field directions tilted by about `amount` radians,
+    point locations moved by about `amount`."""
+    rng = np.random.default_rng(seed)
+    key = "direction" if src["type"] == "FarField" else "location"
+    loc = {k: np.array(v, np.float64) for k, v in src[key].items()}
+    for k in loc:
+        loc[k] = loc[k] + rng.uniform(-amount, amount, loc[k].shape)
+    if key == "direction":
+        n = np.sqrt(sum(loc[k] ** 2 for k in loc))
+        loc = {k: loc[k] / n for k in loc}
+    out = dict(src)
+    out[key] = loc
+    return out

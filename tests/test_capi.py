@@ -85,3 +85,27 @@ def test_python_mirror_messages():
     bad = dict(base); bad["scattering"] = {"type": ["Tabular", "Tabular"], "amplitudeFunction": [[1.0], [1.0]]}
     with pytest.raises(ValueError, match="Tabular amplitude function of material 0"):
         config.CConfig(bad, 2)
+
+
+def test_correlation_sampler_config_checks(native):
+    """TemporalCorrelationSampler: illumination2 / view2 must be present and as large as
+    illumination / view (MatlabInterface.cu:1313-1331); u1 and u2 share one source type here; its
+    coherent-backscattering form is not built."""
+    base = W.c1(paths=64, batch=64)
+    cs = dict(base, sampler=dict(base["sampler"], type="TemporalCorrelationSampler"),
+              illumination2=base["illumination"], view2=base["view"])
+    assert native.sstmc_validate(config.CConfig(cs, 2).ptr()) == 0
+    cc = config.CConfig(cs, 2)
+    cc.struct.view2.type = 0                          # absent in the C struct
+    assert native.sstmc_validate(cc.ptr()) == 7
+    cc = config.CConfig(cs, 2)
+    cc.struct.illumination2.num = 3                   # sizes differ
+    assert native.sstmc_validate(cc.ptr()) == 7
+    pt = dict(cs, view2={"type": "PointSource", "location": {"x": np.zeros(16), "y": np.full(16, 30.0)}})
+    assert native.sstmc_validate(config.CConfig(pt, 2).ptr()) == 5   # u2 of another type
+    cb = dict(cs, tracer={"type": "nee", "isCBS": True})
+    assert native.sstmc_validate(config.CConfig(cb, 2).ptr()) == 5
+    with pytest.raises(ValueError, match="illuminationsHandle2"):
+        config.CConfig({k: v for k, v in cs.items() if k != "view2"}, 2)
+    with pytest.raises(ValueError, match="Available sampler types are TemporalSampler and TemporalCorrelationSampler"):
+        config.CConfig(dict(base, sampler=dict(base["sampler"], type="Other")), 2)

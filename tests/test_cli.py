@@ -107,3 +107,21 @@ def test_bad_json_and_usage(tmp_path):
     assert r.returncode == 7 and "JSON" in r.stderr
     r = subprocess.run([U.CLI], capture_output=True, text=True)
     assert r.returncode == 2
+
+
+def test_correlation_sampler_configs(tmp_path):
+    c = W.c1(paths=1024, batch=1024)
+    cs = dict(c, sampler=dict(c["sampler"], type="TemporalCorrelationSampler"), illumination2=c["illumination"],
+              view2=c["view"])
+    r = U.run(cs, tmp_path, "--check", dims=2)
+    assert r.returncode == 0, r.stderr
+    for drop, msg in (("view2", "illuminationsHandle2"),):
+        bad = {k: v for k, v in cs.items() if k != drop}
+        assert msg in _py_error(bad, 2)
+        r = U.run(bad, tmp_path, "--check", dims=2)
+        assert r.returncode == 7 and msg in r.stderr, r.stderr
+    small = dict(cs, view2={"type": "FarField", "direction": {"x": [0.0], "y": [1.0]}})
+    msg = _py_error(small, 2)
+    assert "Both views must be at the same size" in msg
+    r = U.run(small, tmp_path, "--check", dims=2)
+    assert r.returncode == 7 and msg in r.stderr, (msg, r.stderr)
