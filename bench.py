@@ -48,6 +48,9 @@ def workload(name: str, batch: int, steps: int):
     if name == "corr128":
         cfg = W.c3(paths=batch * steps, batch=batch)
         return cfg, 3
+    if name == "acorr128":
+        cfg = W.c3_analytic(paths=batch * steps, batch=batch)
+        return cfg, 3
     raise SystemExit("unknown workload %s" % name)
 
 

@@ -585,9 +585,83 @@ __global__ void __launch_bounds__(256) k_single_corrsampler(ScatterView sc, Sour
     }
 }
 
+// Far-field sources and one phase function for every material: Fs_u1 conj(Fs_u2) and |q|^2 depend
+// on (l, v) only and the decorrelation rate on the first vertex's material and the lag only, so each
+// thread keeps its (l, v) constants, the block a per-material table of exp(-rate_m |q|^2), and the
+// path loop is one complex multiply-accumulate per path.
+template <int DIMS, typename R>
+__global__ void __launch_bounds__(256) k_single_corrsampler_far(ScatterView sc, SourceView il, SourceView vw,
+                                                                FramesView fr, VertexBatch vb,
+                                                                const typename SingleT<R>::C* __restrict__ a,
+                                                                const typename SingleT<R>::C* __restrict__ b,
+                                                                uint32_t P, uint32_t pathsPerSplit, uint32_t splits,
+                                                                double kw2, typename SingleT<R>::C* __restrict__ part) {
+    typedef typename SingleT<R>::C C;
+    __shared__ C sa[16][16], sb[16][16];
+    __shared__ uint32_t sm[16];
+    __shared__ R ex[kMaxMaterials][256];
+    const uint32_t L = il.n / 2, V = vw.n / 2;
+    const uint32_t tl = threadIdx.x & 15, tq = threadIdx.x >> 4;
+    const uint32_t split = blockIdx.z % splits, t = blockIdx.z / splits;
+    const uint32_t l = blockIdx.x * 16 + tl, v = blockIdx.y * 16 + tq;
+    const uint32_t pBegin = split * pathsPerSplit, pEnd = min(P, pBegin + pathsPerSplit);
+    const double tau = fr.ticks[t];
+    C f0;
+    f0.x = f0.y = 0;
+    {
+        const uint32_t ll = l < L ? l : 0, vv = v < V ? v : 0;
+        const double* i1 = il.pd + 3 * (size_t)ll;
+        const double* i2 = il.pd + 3 * (size_t)(L + ll);
+        const double* o1 = vw.pd + 3 * (size_t)vv;
+        const double* o2 = vw.pd + 3 * (size_t)(V + vv);
+        const C f1 = singleAmp<DIMS, R>(sc, 1, (R)(i1[0] * o1[0] + i1[1] * o1[1] + i1[2] * o1[2]));
+        const C f2 = singleAmp<DIMS, R>(sc, 1, (R)(i2[0] * o2[0] + i2[1] * o2[1] + i2[2] * o2[2]));
+        f0.x = f1.x * f2.x + f1.y * f2.y;
+        f0.y = f1.y * f2.x - f1.x * f2.y;
+        double q2 = 0;
+        for (int c = 0; c < DIMS; ++c) {
+            const double d = 0.5 * (i1[c] + i2[c] - o1[c] - o2[c]);
+            q2 += d * d;
+        }
+        for (int m = 0; m < kMaxMaterials; ++m)
+            ex[m][threadIdx.x] = SingleT<R>::ex((R)(-kw2 * fabs(fr.Dd[m] * tau) * q2));
+    }
+    R re = 0, im = 0;
+    for (uint32_t pc = pBegin; pc < pEnd; pc += 16) {
+        const uint32_t p = pc + tq;
+        const uint32_t ls = blockIdx.x * 16 + tl, vs = blockIdx.y * 16 + tl;
+        C av, bv;
+        av.x = av.y = 0; bv.x = bv.y = 0;
+        uint32_t m1 = 0;
+        if (p < pEnd && vb.k[p] > 0 && (uint64_t)vb.off[P] <= vb.cap) {
+            m1 = __float_as_uint(vb.xm[vb.off[p]].w);
+            if (ls < L) av = a[((size_t)t * P + p) * L + ls];
+            if (vs < V) bv = b[((size_t)t * P + p) * V + vs];
+        }
+        sa[tq][tl] = av;
+        sb[tq][tl] = bv;
+        if (tl == 0) sm[tq] = m1;
+        __syncthreads();
+        for (int q = 0; q < 16; ++q) {
+            const C x = sa[q][tl], y = sb[q][tq];
+            const R w = ex[sm[q]][threadIdx.x];
+            const R pr = x.x * y.x - x.y * y.y, pi = x.x * y.y + x.y * y.x;
+            re = fma(w, pr, re);
+            im = fma(w, pi, im);
+        }
+        __syncthreads();
+    }
+    if (l < L && v < V) {
+        C c;
+        c.x = f0.x * re - f0.y * im;
+        c.y = f0.x * im + f0.y * re;
+        part[((size_t)blockIdx.z) * L * V + (size_t)v * L + l] = c;
+    }
+}
+
 hipError_t launchSingleCorrelation(int dims, int fp64, const ScatterView& sc, const SourceView& il,
                                    const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
-                                   const void* b, uint32_t P, double lambda, double* U, Workspace* ws,
+                                   const void* b, uint32_t P, double lambda, int uniformFar, double* U, Workspace* ws,
                                    hipStream_t st) {
     const uint32_t L = il.n / 2, V = vw.n / 2, NT = fr.nt;
     uint32_t tiles = ((L + 15) / 16) * ((V + 15) / 16);
@@ -611,29 +685,31 @@ hipError_t launchSingleCorrelation(int dims, int fp64, const ScatterView& sc, co
     const double kw2 = 39.478417604357434 / (lambda * lambda);
     dim3 grid((L + 15) / 16, (V + 15) / 16, splits * NT);
     const uint32_t LV = L * V;
+#define SSTMC_SINGLE_CS(D, R, C2)                                                                          \
+    do {                                                                                                   \
+        if (uniformFar)                                                                                    \
+            hipLaunchKernelGGL((k_single_corrsampler_far<D, R>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,  \
+                               (const C2*)a, (const C2*)b, P, pps, splits, kw2, (C2*)ws->ptr);             \
+        else                                                                                               \
+            hipLaunchKernelGGL((k_single_corrsampler<D, R>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,      \
+                               (const C2*)a, (const C2*)b, P, pps, splits, kw2, (C2*)ws->ptr);             \
+    } while (0)
     if (fp64) {
-        if (dims == 2)
-            hipLaunchKernelGGL((k_single_corrsampler<2, double>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
-                               (const double2*)a, (const double2*)b, P, pps, splits, kw2, (double2*)ws->ptr);
-        else
-            hipLaunchKernelGGL((k_single_corrsampler<3, double>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
-                               (const double2*)a, (const double2*)b, P, pps, splits, kw2, (double2*)ws->ptr);
+        if (dims == 2) SSTMC_SINGLE_CS(2, double, double2);
+        else SSTMC_SINGLE_CS(3, double, double2);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<double2>, dim3((LV + 255) / 256), dim3(256), 0, st,
                                (const double2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,
                                (const double2*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
     } else {
-        if (dims == 2)
-            hipLaunchKernelGGL((k_single_corrsampler<2, float>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
-                               (const float2*)a, (const float2*)b, P, pps, splits, kw2, (float2*)ws->ptr);
-        else
-            hipLaunchKernelGGL((k_single_corrsampler<3, float>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
-                               (const float2*)a, (const float2*)b, P, pps, splits, kw2, (float2*)ws->ptr);
+        if (dims == 2) SSTMC_SINGLE_CS(2, float, float2);
+        else SSTMC_SINGLE_CS(3, float, float2);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st,
                                (const float2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,
                                (const double2*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
     }
+    #undef SSTMC_SINGLE_CS
     return hipGetLastError();
 }
 

@@ -77,10 +77,11 @@ hipError_t launchSingleGeneric(int dims, int fp64, const ScatterView& sc, const 
 
 // Single scattering of the TemporalCorrelationSampler (sources il / vw hold u1 then u2):
 // U[t][v][l] += sum_p a[t][p][l] b[t][p][v] Fs_u1 conj(Fs_u2) exp(-k^2 |D tau_t| |q|^2).
+// uniformFar: far-field illuminations and views and one phase function for every material.
 hipError_t launchSingleCorrelation(int dims, int fp64, const ScatterView& sc, const SourceView& il,
                                    const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
-                                   const void* b, uint32_t P, double lambda, double* U, Workspace* ws,
-                                   hipStream_t st);
+                                   const void* b, uint32_t P, double lambda, int uniformFar, double* U,
+                                   Workspace* ws, hipStream_t st);
 
 // U += 0 helpers
 hipError_t launchZeroU(double* U, size_t n, hipStream_t st);

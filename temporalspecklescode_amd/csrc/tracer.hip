@@ -654,8 +654,10 @@ int NextEventEstimation::contractSingle(uint32_t npaths, double* U) {
         he = launchContract(contractKind_, a_.p, b_.p, npaths, L_, V_, T_, Ff_.p, Fd_.p,
                             opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0, fscalar_, U, &wsSingle_, stream_);
     } else if (singleMode_ == kSingleCorrelation) {
+        const bool uniformFar = il_.type() == SSTMC_SOURCE_FARFIELD && vw_.type() == SSTMC_SOURCE_FARFIELD &&
+                                sc_.isUniform(medium_.materialNum());
         he = launchSingleCorrelation(dims_, fp64_ ? 1 : 0, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p,
-                                     npaths, opt_.lambda, U, &wsSingle_, stream_);
+                                     npaths, opt_.lambda, uniformFar ? 1 : 0, U, &wsSingle_, stream_);
     } else {
         he = launchSingleGeneric(dims_, fp64_ ? 1 : 0, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p, npaths, U,
                                  &wsSingle_, stream_);

@@ -177,6 +177,17 @@ def c3(paths: int = 1 << 22, batch: int = 1 << 15, seed: int = 4, nlv: int = 128
     }
 
 
+def c3_analytic(paths: int = 1 << 22, batch: int = 1 << 15, seed: int = 4, nlv: int = 128, nt: int = 9) -> dict:
+    """The c3 slab under the TemporalCorrelationSampler (analytic decorrelation; illumination2 =
+    illumination, view2 = view, lags 0..nt-1)."""
+    cfg = c3(paths=paths, batch=batch, seed=seed, nlv=nlv, nt=nt)
+    cfg["sampler"] = dict(cfg["sampler"], type="TemporalCorrelationSampler")
+    cfg["tracer"] = {"type": "nee"}
+    cfg["illumination2"] = cfg["illumination"]
+    cfg["view2"] = cfg["view"]
+    return cfg
+
+
 def c4(paths: int = 1 << 24, batch: int = 1 << 16, seed: int = 6, perlin_seed: int = 5, n: int = 128,
        nside: int = 32) -> dict:
     """Transmission matrix: 128^3 Perlin-thresholded grid, 1024 x 1024 plane waves."""
