@@ -188,6 +188,37 @@ def c3_analytic(paths: int = 1 << 22, batch: int = 1 << 15, seed: int = 4, nlv: 
     return cfg
 
 
+def spatio_temporal_2d(amp, pixels: int = 51, motion: str = "Brownian-Linear", batch: int = 100,
+                       full_iteration: int = 100, seed: int = 1, mfp: float = 40.0, ux: float = 1.0,
+                       D: float = 0.015) -> dict:
+    """The configuration of the reference's examples/spatioTemporalCorrelation2D.m (lines 8-108): a
+    2D single-cell slab [-50, 50] x [-20, 20] of mean free path `mfp` with the tabular amplitude
+    `amp` in both materials; TemporalCorrelationSampler with the u1 illumination and view fixed at
+    the first pixel and the u2 ones swept over `pixels` positions 80 apart, 30 below the slab
+    (reflection); lags 0..2; Brownian D and/or linear velocity ux along x by `motion`."""
+    x = np.linspace(-1.0, 1.0, pixels)
+    sep, dil, dvw = 80.0, 30.0, 30.0
+
+    def pts(xs, y):
+        return {"type": "PointSource", "location": {"x": np.asarray(xs, np.float64), "y": np.full(pixels, y)}}
+
+    sig = np.array([0.0, 1.0 / mfp])
+    return {
+        "simulation": {"batchSize": batch, "fullIteration": full_iteration, "seed": seed, "lambda": 1.0},
+        "scattering": {"type": ["Tabular", "Tabular"], "amplitudeFunction": [amp, amp]},
+        "medium": {"type": "Heterogeneous", "xAxis": [-50.0, 50.0], "yAxis": [-20.0, 20.0],
+                   "materialGrid": np.ones((1, 1), np.uint32), "sigs": sig, "siga": sig * 0},
+        "illumination": pts(0 * x + x[0] - sep / 2, -dil),
+        "illumination2": pts(x - sep / 2, -dil),
+        "view": pts(0 * x + x[0] + sep / 2, -dvw),
+        "view2": pts(x + sep / 2, -dvw),
+        "sampler": {"type": "TemporalCorrelationSampler", "t": np.linspace(0.0, 2.0, pixels),
+                    "D": np.array([0.0, D if motion != "Linear" else 0.0]),
+                    "U": {"x": np.array([0.0, ux if motion != "Brownian" else 0.0]), "y": np.zeros(2)}},
+        "tracer": {"type": "nee"},
+    }
+
+
 def c4(paths: int = 1 << 24, batch: int = 1 << 16, seed: int = 6, perlin_seed: int = 5, n: int = 128,
        nside: int = 32) -> dict:
     """Transmission matrix: 128^3 Perlin-thresholded grid, 1024 x 1024 plane waves."""

@@ -119,3 +119,17 @@ def test_mcx_known_answer_on_gpu():
     assert np.all(np.abs(m - mcx) < 4 * se)
     shape = (m / m[0]) / (mcx / mcx[0])
     assert np.abs(shape - 1).max() < 0.03, shape
+
+
+@pytest.mark.parametrize("motion", ["Linear", "Brownian-Linear"])
+def test_spatio_temporal_example_matches_oracle(motion):
+    """examples/spatioTemporalCorrelation2D.m (11 pixels, its tabular amplitude from
+    inPhasefunction.mat): FP64 terms and batch correlation against the restatement."""
+    amp = np.load(os.path.join(HERE, "golden", "in_phase_function.npz"))["amp"]
+    cfg = W.spatio_temporal_2d(amp, pixels=11, motion=motion, batch=512, full_iteration=1, seed=3)
+    cfg["precision"] = {"terms": "fp64", "contraction": "fp64"}
+    with Tracer(cfg, 2) as tr:
+        d = tr.debug_batch(0, 512)
+    A, B, S, U = _oracle_u(cfg, 2, d, d["a"], d["b"])
+    assert _rel(d["a"], A) < 1e-10 and _rel(d["b"], B) < 1e-10
+    assert _rel(d["u"], U) < 1e-10
