@@ -59,7 +59,8 @@ enum { SSTMC_SCATTERING_ISOTROPIC = 1,                    /* config.scattering.t
        SSTMC_SCATTERING_VMF = 4 };                        /* values follow ScatteringNS::ScatteringType */
 enum { SSTMC_SOURCE_NONE = 0,                             /* config.illumination/view.type */
        SSTMC_SOURCE_POINT = 1,
-       SSTMC_SOURCE_FARFIELD = 2 };
+       SSTMC_SOURCE_FARFIELD = 2,
+       SSTMC_SOURCE_GAUSSIAN_BEAM = 3 };                  /* 3D; both illumination and view */
 enum { SSTMC_SAMPLER_TEMPORAL = 1,                        /* config.sampler.type: TemporalSampler */
        SSTMC_SAMPLER_TEMPORAL_CORRELATION = 2 };          /* TemporalCorrelationSampler */
 enum { SSTMC_TRACER_NEE = 1 };                            /* config.tracer.type */
@@ -118,12 +119,18 @@ typedef struct sstmc_scattering {
 
 /* config.illumination / config.view (MatlabInterface.cu:1062-1215). */
 typedef struct sstmc_source {
-    int32_t type;                  /* SSTMC_SOURCE_FARFIELD or SSTMC_SOURCE_POINT */
-    uint32_t num;                  /* number of directions / locations */
-    const double* x;               /* direction.x or location.x */
+    int32_t type;                  /* SSTMC_SOURCE_FARFIELD, _POINT or _GAUSSIAN_BEAM */
+    uint32_t num;                  /* number of directions / locations / beams */
+    const double* x;               /* direction.x, location.x or focalPoint.x */
     const double* y;
     const double* z;               /* ignored in 2D */
     int32_t isAreaSource;          /* PointSource only (PointSource.cuh:140-147) */
+    /* GaussianBeam only (MatlabInterface.cu:1153-1190, GaussianBeam.cuh:1026-1196): the beam
+     * directions focalDirection.{x,y,z} (unit vectors) and the aperture */
+    const double* dirX;
+    const double* dirY;
+    const double* dirZ;
+    double aperture;
 } sstmc_source;
 
 /* config.sampler (MatlabInterface.cu:1217-1368). TemporalSampler: t are frame times and u(l,v,t)

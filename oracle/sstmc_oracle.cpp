@@ -90,6 +90,42 @@ inline V3 add(const V3& a, const V3& b) { return V3(a[0] + b[0], a[1] + b[1], a[
 inline V3 sub(const V3& a, const V3& b) { return V3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
 inline V3 scale(double s, const V3& a) { return V3(s * a[0], s * a[1], s * a[2]); }
 
+// ------------------------------------------------------------------ complex helpers
+// ComplexType.cuh:84-215 and VectorType.cuh:389-520 (complex 3-vectors), as GaussianBeam uses them.
+struct CV3 {
+    cplx c[3];
+};
+inline CV3 cvec(const V3& re, const V3& im) { return {{cplx(re[0], im[0]), cplx(re[1], im[1]), cplx(re[2], im[2])}}; }
+// cfma(real a, real vector b, complex vector c) = a b + c
+inline CV3 cfma(double a, const V3& b, const CV3& c) {
+    return {{cplx(std::fma(a, b[0], c.c[0].real()), c.c[0].imag()), cplx(std::fma(a, b[1], c.c[1].real()), c.c[1].imag()),
+             cplx(std::fma(a, b[2], c.c[2].real()), c.c[2].imag())}};
+}
+// cfma(complex a, complex vector b, complex vector c) = a b + c
+inline CV3 cfma(cplx a, const CV3& b, const CV3& c) { return {{a * b.c[0] + c.c[0], a * b.c[1] + c.c[1], a * b.c[2] + c.c[2]}}; }
+inline cplx sumSquare(const CV3& v) { return v.c[0] * v.c[0] + v.c[1] * v.c[1] + v.c[2] * v.c[2]; }
+inline cplx dotR(const CV3& v, const V3& w) { return v.c[0] * w[0] + v.c[1] * w[1] + v.c[2] * w[2]; }
+// realMult(a, v): the real part of a v, per component
+inline V3 realMult(cplx a, const CV3& v) {
+    return V3((a * v.c[0]).real(), (a * v.c[1]).real(), (a * v.c[2]).real());
+}
+// complexSqrt / rComplexSqrt: the principal square root (boost's eval_sqrt) and its reciprocal
+inline cplx csqrtRef(cplx a) {
+    if (a.real() == 0 && a.imag() == 0) return cplx(0, 0);
+    double s = std::sqrt((std::fabs(a.real()) + std::sqrt(a.real() * a.real() + a.imag() * a.imag())) / 2.0);
+    if (a.real() >= 0) return cplx(s, a.imag() / (2.0 * s));
+    return cplx(std::fabs(a.imag()) / (2.0 * s), std::copysign(s, a.imag()));
+}
+inline cplx rcsqrtRef(cplx a) { return 1.0 / csqrtRef(a); }
+// complex2timesSinh(z, r) = exp(r + z) - exp(r - z), each exponential dropped when its real part
+// is at most -10 (ComplexType.cuh:170-203)
+inline cplx twoSinh(cplx z, cplx r) {
+    cplx p = r + z, n = r - z, out(0, 0);
+    if (p.real() > -10.0) out += std::exp(p);
+    if (n.real() > -10.0) out -= std::exp(n);
+    return out;
+}
+
 struct Problem;
 
 // VectorType.cuh:95-165 device semantics: dot, norm, normalize (zero vector -> +z / +y).
@@ -418,6 +454,10 @@ struct Scatter {
                 c = (1.0 + gg * gg - sv * sv) / (2.0 * gg);
             }
         }
+        return rotate(prev, c, s);
+    }
+    // vector_rotation (VectorType.cuh:261-282): a direction at cosine c about prev.
+    V3 rotate(const V3& prev, double c, Stream& s) const {
         double sa = (s.uniform() < 0.5 ? -1.0 : 1.0) * std::sqrt(1.0 - c * c);
         if (dims == 2)
             return V3(prev[0] * c - prev[1] * sa, prev[0] * sa + prev[1] * c, 0);
@@ -442,6 +482,16 @@ struct Source {
     // FarField sampling tables (fillBuffers, FarField.cuh:71-127)
     std::vector<V3> invDir, tangent, perpT, perpS;
     std::vector<double> minT, maxT, minS, maxS, prob;
+    // GaussianBeam (GaussianBeam.cuh, 3D): pt holds the focal points, dir the beam directions as
+    // given; gammaA = 1 / aperture^2, apNorm = vMFnormalizationLog(gammaA) (:1053-1062)
+    std::vector<V3> dir;
+    double gammaA = 0, apNorm = 0;
+    // illumination sampling tables, per beam (updateSampler, :1281-1613; zSamplesNum = 1000)
+    uint32_t zB = 1000;
+    std::vector<uint32_t> focalMat;
+    std::vector<double> zmin, zmax, zdt;
+    std::vector<V3> xyb0, xyb1;
+    std::vector<std::vector<double>> zpdf, zicdf, apdf, aicdf;   // [beam][mix * zB + b], [beam][mix]
     uint32_t n() const { return (uint32_t)pt.size(); }
 };
 
@@ -505,6 +555,282 @@ struct Problem {
         }
         *tmin = lo; *tmax = hi;
     }
+
+    // ---------------------------------------------------------------- GaussianBeam (3D)
+    V3 meanView;   // mean view beam direction (GaussianBeam.cuh:1176-1194), the proposal's axis
+    uint32_t matAt(const V3& p) const { return med.inside(p) ? med.material(p) : 0u; }
+    double waveK() const { return 2.0 * PI / lambda; }
+    // orthogonalBase (VectorType.cuh:293-311)
+    static void orthoBase(const V3& v, V3* b0, V3* b1) {
+        if (std::fabs(1.0 - std::fabs(v[2])) <= EPS) { *b0 = V3(1, 0, 0); *b1 = V3(0, 1, 0); return; }
+        V3 a(-v[1], v[0], 0.0);
+        double n = std::sqrt(a[0] * a[0] + a[1] * a[1]);
+        a = V3(a[0] / n, a[1] / n, 0.0);
+        *b0 = a;
+        *b1 = V3(a[1] * v[2] - a[2] * v[1], a[2] * v[0] - a[0] * v[2], a[0] * v[1] - a[1] * v[0]);
+    }
+    // pointToLineProjectionDistance (MediumInterface.cuh:138-153)
+    double lineDistance(const V3& p, const V3& p0, const V3& v) const {
+        V3 o;
+        for (int a = 0; a < 3; ++a) {
+            double acc = 0;
+            for (int b = 0; b < 3; ++b) acc += v[a] * v[b] * (p[b] - p0[b]);
+            o[a] = acc;
+        }
+        return o[0] * v[0] + o[1] * v[1] + o[2] * v[2];
+    }
+    // updateSampler (GaussianBeam.cuh:1281-1613): per beam, the focal material's mixture weights
+    // exp(C + log-normalisation(|mu|)) (alphaPdfIcdfKernel, :254-295, with its large-kappa term
+    // log(2 pi) * kappa), zB points along the beam across the box (initSamplingTable,
+    // fillTimesInSamplingTable, :177-224) and per mixture the z density attenuation / w(z)^2
+    // (samplingPdfKernel, :297-324), exclusive-scanned and normalised (:336-355).
+    int buildGaussian(Source& s) {
+        const uint32_t nb = s.n(), B = s.zB;
+        const double k = waveK();
+        s.focalMat.resize(nb); s.zmin.resize(nb); s.zmax.resize(nb); s.zdt.resize(nb);
+        s.xyb0.resize(nb); s.xyb1.resize(nb);
+        s.zpdf.assign(nb, {}); s.zicdf.assign(nb, {}); s.apdf.assign(nb, {}); s.aicdf.assign(nb, {});
+        for (uint32_t l = 0; l < nb; ++l) {
+            const V3 f = s.pt[l], d = s.dir[l];
+            const uint32_t fm = matAt(f);
+            if (fm == 0 || sc.type[fm] != SSTMC_SCATTERING_VMF) return SSTMC_INVALID_CONFIG;
+            s.focalMat[l] = fm;
+            projectBox(f, d, &s.zmin[l], &s.zmax[l]);
+            s.zdt[l] = (s.zmax[l] - s.zmin[l]) / (double)B;
+            orthoBase(d, &s.xyb0[l], &s.xyb1[l]);
+            const Scatter::Vmf& v = sc.vmf[fm];
+            const uint32_t M = (uint32_t)v.kappa.size();
+            std::vector<double>& ap = s.apdf[l];
+            double asum = 0;
+            for (uint32_t i = 0; i < M; ++i) {
+                const double kap = std::fabs(v.kappa[i]);
+                const double ln = kap < 1e-5 ? -std::log(4.0 * PI)
+                                             : std::log(kap) - (kap > 10.0 ? std::log(2.0 * PI) * kap
+                                                                           : std::log(4.0 * PI * std::sinh(kap)));
+                ap.push_back(std::exp(v.c[i] + ln));
+                asum += ap.back();
+            }
+            double acc = 0;
+            for (uint32_t i = 0; i < M; ++i) {
+                ap[i] /= asum;
+                s.aicdf[l].push_back(acc);
+                acc += ap[i];
+            }
+            std::vector<double>& zp = s.zpdf[l];
+            std::vector<double>& zi = s.zicdf[l];
+            zp.assign((size_t)M * B, 0.0);
+            zi.assign((size_t)M * B, 0.0);
+            for (uint32_t b = 0; b < B; ++b) {
+                const double t = s.zmin[l] + (double)b * s.zdt[l];
+                const V3 z = add(f, scale(t, d));
+                const double att = std::exp(-med.opticalDepth(z, scale(-1.0, d), nullptr));
+                const double dz = geo.dot(sub(z, f), d);
+                for (uint32_t i = 0; i < M; ++i) {
+                    const double wz2 = (s.gammaA + std::fabs(v.kappa[i])) / (k * k) + dz * dz / s.gammaA;
+                    zp[(size_t)i * B + b] = att / wz2;
+                }
+            }
+            for (uint32_t i = 0; i < M; ++i) {
+                double run = 0;
+                for (uint32_t b = 0; b < B; ++b) { zi[(size_t)i * B + b] = run; run += zp[(size_t)i * B + b]; }
+                const double sum = zp[(size_t)i * B + B - 1] + zi[(size_t)i * B + B - 1];
+                for (uint32_t b = 0; b < B; ++b) {
+                    zp[(size_t)i * B + b] *= (double)(B - 1) / sum;
+                    zi[(size_t)i * B + b] /= sum;
+                }
+            }
+        }
+        return SSTMC_NO_ERROR;
+    }
+    // sampleFirstBeam (GaussianBeam.cuh:357-418): a beam, a mixture by its weight, a z bin by the
+    // mixture's table moved uniformly within the bin, then a Gaussian transverse offset of width
+    // w(z); the material is the one at the point (getMaterialInfo, :1642; 0 outside: a miss).
+    V3 sampleGaussFirst(Stream& s, uint32_t* beam) const {
+        const Source& g = il;
+        const uint32_t l = s.index(g.n());
+        *beam = l;
+        const uint32_t B = g.zB;
+        const uint32_t fm = g.focalMat[l];
+        const uint32_t mix = (uint32_t)Scatter::cdfSearch(g.aicdf[l], s.uniform());
+        const std::vector<double> zi(g.zicdf[l].begin() + (size_t)mix * B, g.zicdf[l].begin() + (size_t)(mix + 1) * B);
+        const uint32_t zb = (uint32_t)Scatter::cdfSearch(zi, s.uniform());
+        const V3 d = g.dir[l];
+        const V3 z = add(add(g.pt[l], scale(g.zmin[l] + (double)zb * g.zdt[l], d)), scale(s.uniform() * g.zdt[l], d));
+        const double k = waveK();
+        const double gs = std::fabs(sc.vmf[fm].kappa[mix]);
+        const double dz = geo.dot(sub(z, g.pt[l]), d);
+        const double wz = std::sqrt((g.gammaA + gs) / (k * k) + dz * dz / g.gammaA);
+        const double u1 = s.uniform(), u2 = s.uniform();
+        const double r = std::sqrt(-2.0 * std::log(u1));
+        const double n1 = r * std::cos(2.0 * PI * u2), n2 = r * std::sin(2.0 * PI * u2);
+        return add(z, scale(wz, add(scale(n1, g.xyb0[l]), scale(n2, g.xyb1[l]))));
+    }
+    // firstPointProbabilityKernel (GaussianBeam.cuh:420-487): the mixture-weighted z-table value
+    // at the point's bin times the transverse Gaussian, over sigs(m) times the box depth in z.
+    double prGauss(uint32_t l, const V3& x1) const {
+        const Source& g = il;
+        const uint32_t B = g.zB, fm = g.focalMat[l];
+        const V3 f = g.pt[l], d = g.dir[l];
+        const double dz = lineDistance(x1, f, d);
+        const V3 zv = sub(add(f, scale(dz, d)), x1);
+        const double dxy2 = geo.dot(zv, zv);
+        uint32_t zb;
+        if (dz < g.zmin[l]) zb = 0;
+        else if (dz > g.zmax[l]) zb = B - 1;
+        else zb = std::min(B - 1, (uint32_t)((dz - g.zmin[l]) / g.zdt[l]));   // the reference reads bin B at dz = tMax
+        const double k = waveK();
+        const Scatter::Vmf& v = sc.vmf[fm];
+        double pdf = 0;
+        for (size_t i = 0; i < v.kappa.size(); ++i) {
+            const double wz2 = (g.gammaA + std::fabs(v.kappa[i])) / (k * k) + dz * dz / g.gammaA;
+            pdf += g.apdf[l][i] * g.zpdf[l][i * B + zb] * std::exp(-dxy2 / (2.0 * wz2)) / (2.0 * wz2 * PI);
+        }
+        const uint32_t m = matAt(x1);
+        return pdf / (med.sigs[m] * (med.bmax[2] - med.bmin[2]));
+    }
+    // computeCdfBuffer (GaussianBeam.cuh:489-621): the second-point proposal of beam l at x0, a
+    // mixture over pairs (i, j) of the focal material's lobes of vMF lobes with axis mu_ij and
+    // weight alpha_ij (normalised), from the beam's complex throughput gamma_a d + i k (x0 - focal)
+    // convolved about the mean view direction; log_C = 1 / (4 pi) when |mu_ij| < 1e-4 (as written).
+    struct Conv {
+        std::vector<V3> mu;
+        std::vector<double> w, logC;
+    };
+    void gaussConv(uint32_t l, const V3& x0, Conv* out) const {
+        const Source& g = il;
+        const Scatter::Vmf& v = sc.vmf[g.focalMat[l]];
+        const uint32_t M = (uint32_t)v.kappa.size();
+        const double k = waveK();
+        const CV3 T = cvec(scale(g.gammaA, g.dir[l]), scale(k, sub(x0, g.pt[l])));
+        std::vector<V3> cm(M);
+        std::vector<double> rc(M);
+        for (uint32_t i = 0; i < M; ++i) {
+            const double gs = v.kappa[i];
+            if (std::fabs(gs) < 0.000000001) {
+                const cplx ct = csqrtRef(sumSquare(T));
+                rc[i] = ct.real() - std::log(std::abs(ct)) + v.c[i];
+                cm[i] = V3(0, 0, 0);
+            } else {
+                const cplx gob = gs * rcsqrtRef(sumSquare(cfma(gs, meanView, T)));
+                cm[i] = realMult(gob, T);
+                const double rk = 1.0 / geo.norm(cm[i]);
+                const V3 w = scale(rk, cm[i]);
+                const cplx ct = csqrtRef(sumSquare(cfma(gs, w, T)));
+                rc[i] = ct.real() - std::log(std::abs(ct)) + v.c[i] - geo.dot(w, cm[i]);
+            }
+        }
+        out->mu.resize((size_t)M * M);
+        out->w.resize((size_t)M * M);
+        out->logC.resize((size_t)M * M);
+        double sum = 0;
+        for (uint32_t q = 0; q < M * M; ++q) {
+            const uint32_t i = q % M, j = q / M;
+            const V3 mu = add(cm[i], cm[j]);
+            const double kap = geo.norm(mu);
+            const double lc = kap < 0.0001 ? 1.0 / (4.0 * PI) : Scatter::vmfNormLog(kap);
+            out->mu[q] = mu;
+            out->logC[q] = lc;
+            out->w[q] = std::exp(rc[i] + rc[j] - lc - 2.0 * g.gammaA);
+            sum += out->w[q];
+        }
+        for (double& w : out->w) w /= sum;
+    }
+    // sampleCdfBuffer (GaussianBeam.cuh:636-691): a pair by the first inclusive partial sum above
+    // u, then a vMF direction about its axis (the z axis when |mu| < 1e-4).
+    V3 sampleGaussSecond(uint32_t l, const V3& x0, Stream& s) const {
+        if (sc.allIsotropic) return sc.randomDirection(s);   // isotropicGaussianSampling (:1001-1011)
+        Conv cv;
+        gaussConv(l, x0, &cv);
+        const double u = s.uniform();
+        size_t q = 0;
+        double acc = 0;
+        for (q = 0; q < cv.w.size(); ++q) {
+            if (u < acc) break;
+            acc += cv.w[q];
+        }
+        q -= 1;
+        V3 mu = cv.mu[q];
+        const double kap = geo.norm(mu);
+        mu = kap < 0.0001 ? V3(0, 0, 1) : scale(1.0 / kap, mu);
+        return sc.rotate(mu, Scatter::vmfCosine(kap, s), s);
+    }
+    // sampleDirectionProbability (GaussianBeam.cuh:693-752): the proposal density of the direction
+    // x0 -> x1 under beam l's pair mixture; isotropicGaussianProbability (:1013-1023) otherwise.
+    double pdirGauss(uint32_t l, const V3& x0, const V3& x1) const {
+        if (sc.allIsotropic) return 1.0 / (4.0 * PI);
+        Conv cv;
+        gaussConv(l, x0, &cv);
+        const V3 d = geo.normalize(sub(x1, x0));
+        double p = 0;
+        for (size_t q = 0; q < cv.w.size(); ++q) p += cv.w[q] * std::exp(geo.dot(cv.mu[q], d) + cv.logC[q]);
+        return p;
+    }
+    // gGaussianBeam with the attenuation along the beam (GaussianBeam.cuh:795-816, 1942-1992): the
+    // square root of the transmittance toward the beam's source side (illumination: -d; view: +d).
+    cplx gGauss(const Source& s, uint32_t i, const V3& x, uint64_t* steps) const {
+        if (matAt(x) == 0) return cplx(0, 0);
+        const V3 w = s.illum ? scale(-1.0, s.dir[i]) : s.dir[i];
+        return cplx(std::sqrt(std::exp(-med.opticalDepth(x, w, steps))), 0.0);
+    }
+    // fGaussianBeam (GaussianBeam.cuh:818-892): the vMF lobes of x0's material convolved with the
+    // beam's complex throughput gamma_a d + i k (x0 - focal) (k negative for a view), w0 the
+    // direction x0 -> p2 (illumination) or p2 -> x0 (view):
+    //   sum_j exp(apNorm + log 2 pi + C_j) 2 sinh(s_j) / s_j,  s_j = sqrt((gamma_j w0 + T).(gamma_j w0 + T))
+    cplx fGauss(const Source& s, uint32_t i, const V3& x0, const V3& p2) const {
+        const uint32_t m = matAt(x0);
+        if (m == 0) return cplx(0, 0);
+        const V3 w0 = s.illum ? geo.normalize(sub(p2, x0)) : geo.normalize(sub(x0, p2));
+        const double k = s.illum ? waveK() : -waveK();
+        const CV3 T = cvec(scale(s.gammaA, s.dir[i]), scale(k, sub(x0, s.pt[i])));
+        const double tc = s.apNorm + std::log(2.0 * PI);
+        const Scatter::Vmf& v = sc.vmf[m];
+        cplx f(0, 0);
+        for (size_t j = 0; j < v.kappa.size(); ++j) {
+            const double lognu = tc + v.c[j];
+            const cplx sm = csqrtRef(sumSquare(cfma(v.kappa[j], w0, T)));
+            if (std::fabs(sm.real()) < 0.0001 && std::fabs(sm.imag()) < 0.0001) f += 2.0 * std::exp(lognu);
+            else f += twoSinh(sm, cplx(lognu, 0)) / sm;
+        }
+        return f;
+    }
+    // fsGaussianBeam (GaussianBeam.cuh:894-999): illumination beam a and view beam b through x0's
+    // lobes, the inner convolution taken about b's direction.
+    cplx fsGauss(uint32_t la, uint32_t vb, const V3& x0) const {
+        const Source& A = il;
+        const Source& Bs = vw;
+        const uint32_t m = matAt(x0);
+        const double k = waveK();
+        const CV3 aT = cvec(scale(A.gammaA, A.dir[la]), scale(k, sub(x0, A.pt[la])));
+        const CV3 bT = cvec(scale(Bs.gammaA, Bs.dir[vb]), scale(k, sub(Bs.pt[vb], x0)));
+        const double tc = A.apNorm + Bs.apNorm + 2.0 * std::log(2.0 * PI);
+        const Scatter::Vmf& v = sc.vmf[m];
+        cplx fs(0, 0);
+        for (size_t j = 0; j < v.kappa.size(); ++j) {
+            double gs = v.kappa[j];
+            const double lognu = tc + v.c[j];
+            cplx C, sm, c(0, 0);
+            if (std::fabs(gs) < 0.000001) {
+                C = csqrtRef(sumSquare(aT));
+                sm = csqrtRef(sumSquare(bT));
+            } else {
+                const cplx gob = gs * rcsqrtRef(sumSquare(cfma(gs, Bs.dir[vb], aT)));
+                sm = csqrtRef(sumSquare(cfma(gob, aT, bT)));
+                const V3 w = realMult(gob, aT);
+                const double ra = 1.0 / geo.norm(w);
+                gs *= ra;
+                C = csqrtRef(sumSquare(cfma(gs, w, aT)));
+                c = (-ra) * gob * dotR(aT, w);
+            }
+            const bool smZero = std::fabs(sm.real()) <= 0.0001 && std::fabs(sm.imag()) <= 0.0001;
+            const bool cZero = std::fabs(C.real()) <= 0.0001 && std::fabs(C.imag()) <= 0.0001;
+            if (!smZero && !cZero) fs += twoSinh(C + sm, c + lognu) / (C * sm);
+            else if (smZero && cZero) fs += 4.0 * std::exp(c + lognu);
+            else if (smZero) fs += 2.0 * twoSinh(C, c + lognu) / C;
+            else fs += 2.0 * twoSinh(sm, cplx(lognu, 0)) / sm;
+        }
+        return fs;
+    }
+    bool gauss() const { return il.type == SSTMC_SOURCE_GAUSSIAN_BEAM; }
 
     // FarFieldNS::fillBuffers (FarField.cuh:71-127).
     void buildFarField(Source& s) {
@@ -614,9 +940,17 @@ struct Problem {
         *misses = 0;
         Vertex v1;
         uint32_t event = 0;
+        uint32_t beam = 0;
         for (;; ++event) {
             if (event >= maxAttempts) return SSTMC_NO_HIT;
             Stream s(seed, path, event, 0);
+            if (gauss()) {   // the first point drawn from the beam profile (sampleFirstBeam)
+                const V3 x = sampleGaussFirst(s, &beam);
+                const uint32_t m = matAt(x);
+                if (m > 0) { v1.x = x; v1.dirIn = il.dir[beam]; v1.m = m; break; }
+                ++*misses;
+                continue;
+            }
             V3 x0, d;
             sampleSource(s, &x0, &d);
             V3 x;
@@ -633,7 +967,9 @@ struct Problem {
             const Vertex& prev = vs.back();
             uint32_t ev = 0x80000000u | k;
             Stream s(seed, path, ev, 0);
-            V3 d = sc.sample(prev.m, prev.dirIn, s);
+            // a GaussianBeam illumination proposes the second point's direction itself
+            // (sampleSecondPoint, GaussianBeam.cuh:1673-1748)
+            V3 d = (gauss() && k == 2) ? sampleGaussSecond(beam, prev.x, s) : sc.sample(prev.m, prev.dirIn, s);
             V3 x;
             int m = med.woodcock(prev.x, d, s, &x);
             if (m <= 0) break;
@@ -653,6 +989,7 @@ struct Problem {
     // ---------------------------------------------------------------- NEE terms
     // g: FarField gDirectionKernel (FarField.cuh:223-263) / gPointKernel (PointSource.cuh:117-172)
     cplx g(const Source& s, uint32_t i, const V3& x, uint64_t* steps) const {
+        if (s.type == SSTMC_SOURCE_GAUSSIAN_BEAM) return gGauss(s, i, x, steps);
         if (s.type == SSTMC_SOURCE_FARFIELD) {
             const V3& d = s.pt[i];
             V3 wd = s.illum ? scale(-1.0, d) : d;
@@ -674,6 +1011,7 @@ struct Problem {
     }
     // firstPointProbability (FarField.cuh:469-499, PointSource.cuh:263-294, quirk Q7 in 2D).
     double pr(uint32_t l, const V3& x1, uint64_t* steps) const {
+        if (il.type == SSTMC_SOURCE_GAUSSIAN_BEAM) return prGauss(l, x1);
         if (il.type == SSTMC_SOURCE_FARFIELD)
             return std::exp(-med.opticalDepth(x1, scale(-1.0, il.pt[l]), steps)) * il.prob[l];
         V3 q = sub(il.pt[l], x1);
@@ -693,16 +1031,19 @@ struct Problem {
     }
     // threePointFunctionSingle (FarField.cuh:546-599, PointSource.cuh:363-446)
     cplx fsingle(uint32_t l, uint32_t v, uint32_t m, const V3& x1) const {
+        if (il.type == SSTMC_SOURCE_GAUSSIAN_BEAM) return fsGauss(l, v, x1);
         if (il.type == SSTMC_SOURCE_POINT && sc.allIsotropic) return sc.isoAmp();
         return sc.amp(m, geo.dot(inDir(l, x1), outDir(v, x1)));
     }
     // illumination f at pL=2 (FarField.cuh:546-556; PointSource.cuh:347-361)
     cplx fil(uint32_t l, uint32_t m, const V3& x1, const V3& x2) const {
+        if (il.type == SSTMC_SOURCE_GAUSSIAN_BEAM) return fGauss(il, l, x1, x2);
         if (il.type == SSTMC_SOURCE_POINT && sc.allIsotropic) return sc.isoAmp();
         return sc.amp(m, geo.dot(inDir(l, x1), geo.normalize(sub(x2, x1))));
     }
     // view f at pL>=2: amplitude(-d_v | s_v, x_k, x_{k-1}) (NextEventEstimation.cuh:1560-1563)
     cplx fvw(uint32_t v, uint32_t m, const V3& xprev, const V3& xk) const {
+        if (vw.type == SSTMC_SOURCE_GAUSSIAN_BEAM) return fGauss(vw, v, xk, xprev);
         if (vw.type == SSTMC_SOURCE_POINT && sc.allIsotropic) return sc.isoAmp();
         V3 a = vw.type == SSTMC_SOURCE_FARFIELD ? scale(-1.0, vw.pt[v])
                                                   : geo.normalize(sub(xk, vw.pt[v]));
@@ -767,8 +1108,14 @@ struct Problem {
         if (!out.multiple) return;
         const Vertex& v2 = vs[1];
         double p2 = 0;
-        for (uint32_t l = 0; l < nl; ++l)
-            p2 += prl[l] * sc.pdf(v1.m, geo.dot(inDir(l, v1.x), geo.normalize(sub(v2.x, v1.x))));
+        for (uint32_t l = 0; l < nl; ++l) {
+            if (gauss()) {   // secondPointProbability (GaussianBeam.cuh:1835-1940); 0 becomes 1e20
+                double q = prl[l] * pdirGauss(l, v1.x, v2.x);
+                p2 += (q == 0 && !sc.allIsotropic) ? 1e20 : q;
+            } else {
+                p2 += prl[l] * sc.pdf(v1.m, geo.dot(inDir(l, v1.x), geo.normalize(sub(v2.x, v1.x))));
+            }
+        }
         p2 /= nl;
         double w2 = 1.0 / std::sqrt(p2);
         for (uint32_t t = 0; t < nt; ++t) {
@@ -1055,15 +1402,39 @@ int build(const sstmc_config* c, Problem& pb) {
             if (part == 1 && s.type == SSTMC_SOURCE_NONE) return SSTMC_INVALID_CONFIG;
             if (part == 1 && (s.type != s0.type || (s.isAreaSource != 0) != d.area)) return SSTMC_NOT_SUPPORTED;
             if (part == 1 && s.num != s0.num) return SSTMC_INVALID_CONFIG;
-            if ((s.type != SSTMC_SOURCE_FARFIELD && s.type != SSTMC_SOURCE_POINT) || s.num == 0 ||
-                !s.x || !s.y || (dims == 3 && !s.z))
+            if ((s.type != SSTMC_SOURCE_FARFIELD && s.type != SSTMC_SOURCE_POINT &&
+                 s.type != SSTMC_SOURCE_GAUSSIAN_BEAM) || s.num == 0 || !s.x || !s.y || (dims == 3 && !s.z))
                 return SSTMC_INVALID_CONFIG;
             for (uint32_t k = 0; k < s.num; ++k) {
                 V3 p(s.x[k], s.y[k], dims == 3 ? s.z[k] : 0.0);
                 d.pt.push_back(s.type == SSTMC_SOURCE_FARFIELD ? pb.geo.normalize(p) : p);
             }
+            if (s.type == SSTMC_SOURCE_GAUSSIAN_BEAM) {
+                // GaussianBeam (MatlabInterface.cu:1153-1190; GaussianBeam.cuh:1026-1062): 3D, focal
+                // points and beam directions as given, gamma_a = 1 / aperture^2, the aperture
+                // normalisation vMFnormalizationLog(gamma_a)
+                if (dims != 3 || !s.dirX || !s.dirY || !s.dirZ || !(s.aperture > 0)) return SSTMC_INVALID_CONFIG;
+                for (uint32_t k = 0; k < s.num; ++k) d.dir.push_back(V3(s.dirX[k], s.dirY[k], s.dirZ[k]));
+                d.gammaA = 1.0 / (s.aperture * s.aperture);
+                d.apNorm = Scatter::vmfNormLog(d.gammaA);
+            }
         }
         if (d.type == SSTMC_SOURCE_FARFIELD && i == 0) pb.buildFarField(d);
+    }
+    if (pb.il.type == SSTMC_SOURCE_GAUSSIAN_BEAM || pb.vw.type == SSTMC_SOURCE_GAUSSIAN_BEAM) {
+        // both roles GaussianBeam (fsGaussianBeam pairs two beams, GaussianBeam.cuh:2040-2096), vMF
+        // lobes in every material (updateSampler, :1283-1290), the TemporalSampler without
+        // coherent backscattering
+        if (pb.il.type != pb.vw.type || pb.acorr || pb.cbs) return SSTMC_NOT_SUPPORTED;
+        for (uint32_t m = 1; m < nmat; ++m)
+            if (pb.sc.type[m] != SSTMC_SCATTERING_VMF) return SSTMC_NOT_SUPPORTED;
+        // the mean view beam direction, the axis of the second-point proposal (:1176-1194); the
+        // reference draws a random axis per batch when the mean vanishes, here the z axis
+        V3 mv(0, 0, 0);
+        for (const V3& dv : pb.vw.dir) mv = add(mv, scale(1.0 / pb.vw.n(), dv));
+        const double r = pb.geo.norm(mv);
+        pb.meanView = r < 0.0001 ? V3(0, 0, 1) : scale(1.0 / r, mv);
+        if (int e = pb.buildGaussian(pb.il)) return e;
     }
     // sampler: time table (TemporalSampler.cuh:941-1132); the correlation sampler takes the lags
     // as they are (TemporalSamplerCorrelation.cuh:455-530)
@@ -1330,6 +1701,76 @@ int oracle_cbs_terms_from_vertices(const sstmc_config* cfg, uint32_t n, const ui
             term_br[2 * ((size_t)i * nt * nl + j)] = x.real();
             term_br[2 * ((size_t)i * nt * nl + j) + 1] = x.imag();
         }
+    }
+    return SSTMC_NO_ERROR;
+}
+
+// --- GaussianBeam pieces, for quadrature and sampling checks --------------------------------
+// f of illumination (view = 0) or view (view = 1) beam i at x0 with the path continuing to / coming
+// from p2; fs of illumination beam l and view beam v at x0; out = (re, im).
+int oracle_gauss_f(const sstmc_config* cfg, int view, uint32_t i, const double* x0, const double* p2, double* out) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    if (!pb.gauss()) return SSTMC_INVALID_CONFIG;
+    cplx f = pb.fGauss(view ? pb.vw : pb.il, i, V3(x0[0], x0[1], x0[2]), V3(p2[0], p2[1], p2[2]));
+    out[0] = f.real();
+    out[1] = f.imag();
+    return SSTMC_NO_ERROR;
+}
+int oracle_gauss_fs(const sstmc_config* cfg, uint32_t l, uint32_t v, const double* x0, double* out) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    if (!pb.gauss()) return SSTMC_INVALID_CONFIG;
+    cplx f = pb.fsGauss(l, v, V3(x0[0], x0[1], x0[2]));
+    out[0] = f.real();
+    out[1] = f.imag();
+    return SSTMC_NO_ERROR;
+}
+// first-point probability of beam l at x1 (n points), second-point proposal density of beam l
+// at x0 toward each x1 (n points)
+int oracle_gauss_pr(const sstmc_config* cfg, uint32_t l, uint32_t n, const double* x1, double* out) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    if (!pb.gauss()) return SSTMC_INVALID_CONFIG;
+    for (uint32_t i = 0; i < n; ++i) out[i] = pb.prGauss(l, V3(x1[3 * i], x1[3 * i + 1], x1[3 * i + 2]));
+    return SSTMC_NO_ERROR;
+}
+int oracle_gauss_pdir(const sstmc_config* cfg, uint32_t l, const double* x0, uint32_t n, const double* x1,
+                      double* out) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    if (!pb.gauss()) return SSTMC_INVALID_CONFIG;
+    for (uint32_t i = 0; i < n; ++i)
+        out[i] = pb.pdirGauss(l, V3(x0[0], x0[1], x0[2]), V3(x1[3 * i], x1[3 * i + 1], x1[3 * i + 2]));
+    return SSTMC_NO_ERROR;
+}
+// n first points drawn from the beam profile (path ids 0..n-1, no rejection outside the medium),
+// their beams; n second-point directions of beam l at x0
+int oracle_gauss_sample_first(const sstmc_config* cfg, uint32_t n, double* pts, uint32_t* beams) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    if (!pb.gauss()) return SSTMC_INVALID_CONFIG;
+    for (uint32_t i = 0; i < n; ++i) {
+        Stream s(pb.seed, i, 0, 0);
+        V3 x = pb.sampleGaussFirst(s, &beams[i]);
+        for (int a = 0; a < 3; ++a) pts[3 * i + a] = x[a];
+    }
+    return SSTMC_NO_ERROR;
+}
+int oracle_gauss_sample_second(const sstmc_config* cfg, uint32_t l, const double* x0, uint32_t n, double* dirs) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    if (!pb.gauss()) return SSTMC_INVALID_CONFIG;
+    for (uint32_t i = 0; i < n; ++i) {
+        Stream s(pb.seed, i, 0x80000002u, 0);
+        V3 d = pb.sampleGaussSecond(l, V3(x0[0], x0[1], x0[2]), s);
+        for (int a = 0; a < 3; ++a) dirs[3 * i + a] = d[a];
     }
     return SSTMC_NO_ERROR;
 }

@@ -20,7 +20,8 @@
 //               [[re, im], ...] or a real list; vMF: {"mixtureMu", "mixtureC", "mixtureAlpha"};
 //               per material)
 //   illumination / view: type "FarField" (direction{x,y[,z]}) | "PointSource" (location{...}),
-//                        isAreaSource
+//                        isAreaSource | "GaussianBeam" (focalPoint{x,y,z}, focalDirection{x,y,z},
+//                        aperture; 3D)
 //   sampler: type "TemporalSampler" | "TemporalCorrelationSampler", t, D, U{x,y,z},
 //            isTemporalAmplitude; the correlation sampler also takes illumination2 and view2
 //   tracer: type "nee", isCBS, output "field" | "correlation"
@@ -96,9 +97,26 @@ void source(HostConfig& h, sstmc_source& dst, const Value& src, uint32_t dims, c
     } else if (ty == "FarField") {
         dst.type = SSTMC_SOURCE_FARFIELD;
         loc = &src.at("direction");
+    } else if (ty == "GaussianBeam") {
+        // MatlabInterface.cu:1153-1190: focalPoint, focalDirection (one per focal point), aperture
+        if (dims != 3) throw ConfigError("GaussianBeam sources are supported in 3D");
+        dst.type = SSTMC_SOURCE_GAUSSIAN_BEAM;
+        loc = &src.at("focalPoint");
+        const Value& fd = src.at("focalDirection");
+        const size_t n = nums(loc->at("x")).size();
+        const char* keys[3] = {"x", "y", "z"};
+        const double** dirs[3] = {&dst.dirX, &dst.dirY, &dst.dirZ};
+        for (int a = 0; a < 3; ++a) {
+            std::vector<double> v = nums(fd.at(keys[a]));
+            if (v.size() != n)
+                throw ConfigError(std::string(name) + " focalDirection must hold one direction per focal point");
+            *dirs[a] = h.keep(v);
+        }
+        dst.aperture = num(src.at("aperture"), "aperture");
+        if (!(dst.aperture > 0)) throw ConfigError(std::string(name) + " aperture must be positive");
     } else {
         throw ConfigError(std::string("Unsupported ") + name + " source type " + ty +
-                          ". Available types are PointSource and FarField");
+                          ". Available types are PointSource, FarField and GaussianBeam");
     }
     std::vector<double> xs = nums(loc->at("x")), ys = nums(loc->at("y"));
     if (xs.size() != ys.size()) throw ConfigError(std::string(name) + " x and y must have the same size");

@@ -26,7 +26,7 @@ SCATTERING_ISOTROPIC = 1
 SCATTERING_TABULAR = 2
 SCATTERING_HG = 3
 SCATTERING_VMF = 4
-SOURCE_NONE, SOURCE_POINT, SOURCE_FARFIELD = 0, 1, 2
+SOURCE_NONE, SOURCE_POINT, SOURCE_FARFIELD, SOURCE_GAUSSIAN_BEAM = 0, 1, 2, 3
 SAMPLER_TEMPORAL, SAMPLER_TEMPORAL_CORRELATION = 1, 2
 TRACER_NEE = 1
 OUTPUT_FIELD, OUTPUT_CORRELATION = 0, 1
@@ -57,7 +57,7 @@ class Scattering(C.Structure):
 
 class Source(C.Structure):
     _fields_ = [("type", C.c_int32), ("num", C.c_uint32), ("x", _DP), ("y", _DP), ("z", _DP),
-                ("isAreaSource", C.c_int32)]
+                ("isAreaSource", C.c_int32), ("dirX", _DP), ("dirY", _DP), ("dirZ", _DP), ("aperture", C.c_double)]
 
 
 class Sampler(C.Structure):
@@ -294,9 +294,26 @@ class CConfig:
         elif ty == "FarField":
             dst.type = SOURCE_FARFIELD
             loc = src["direction"]
+        elif ty == "GaussianBeam":
+            # MatlabInterface.cu:1153-1190: focalPoint, focalDirection (one per focal point), aperture
+            if dims != 3:
+                raise ValueError("GaussianBeam sources are supported in 3D")
+            dst.type = SOURCE_GAUSSIAN_BEAM
+            loc = src["focalPoint"]
+            fd = src["focalDirection"]
+            n = np.atleast_1d(np.asarray(loc["x"])).size
+            for k, attr in (("x", "dirX"), ("y", "dirY"), ("z", "dirZ")):
+                v = np.atleast_1d(np.asarray(fd[k], np.float64)).ravel()
+                if v.size != n:
+                    raise ValueError("%s focalDirection must hold one direction per focal point" % name)
+                setattr(dst, attr, self._dptr(v)[0])
+            ap = float(np.asarray(src["aperture"]).ravel()[0])
+            if not ap > 0:
+                raise ValueError("%s aperture must be positive" % name)
+            dst.aperture = ap
         else:
-            raise ValueError("Unsupported %s source type %s. Available types are PointSource and "
-                             "FarField" % (name, ty))
+            raise ValueError("Unsupported %s source type %s. Available types are PointSource, FarField and "
+                             "GaussianBeam" % (name, ty))
         xs = np.atleast_1d(np.asarray(loc["x"], np.float64)).ravel()
         ys = np.atleast_1d(np.asarray(loc["y"], np.float64)).ravel()
         if xs.size != ys.size:
@@ -314,7 +331,8 @@ class CConfig:
 def sizes(cfg: dict, dims: int):
     """(Nl, Nv, Nt, renderingsNum) of a config dict."""
     def n(src):
-        loc = src["location"] if src["type"] == "PointSource" else src["direction"]
+        loc = {"PointSource": "location", "GaussianBeam": "focalPoint"}.get(src["type"], "direction")
+        loc = src[loc]
         return int(np.atleast_1d(np.asarray(loc["x"])).size)
     nt = int(np.asarray(cfg["sampler"].get("t", [0.0])).size)
     rn = int((cfg.get("simulation") or {}).get("renderingsNum", 1))

@@ -219,6 +219,64 @@ def spatio_temporal_2d(amp, pixels: int = 51, motion: str = "Brownian-Linear", b
     }
 
 
+CONFOCAL_LOBES = [   # examples/confocalRenderings.m lines 16-47: vMF lobes of the five materials
+    {"mixtureMu": [0.0], "mixtureC": [-1.2655121234846453964889], "mixtureAlpha": [1.0]},
+    {"mixtureMu": [0, 7.864623088553429, 1.432699943392017, 34.940513109419960, 1.582028646657162e+02,
+                   7.527763768351407e+02, -0.026667703987043],
+     "mixtureC": [-2.531024246969291, -7.640125542225978, -2.852369324003370, -33.224743185787034,
+                  -1.549768635690816e+02, -7.479904856941710e+02, -2.531142771898976],
+     "mixtureAlpha": [0.096076950725656, 0.250010487228936, 0.504158076691359, 0.163528156678287,
+                      0.097875866684034, 0.031641230957240, 0.223222884401916]},
+    {"mixtureMu": [0, 1.764406405245083e+02, 1.382801893975425, 36.855826580175176, 7.949781817711435,
+                   -0.078158767086068],
+     "mixtureC": [-2.531024246969291, -1.731055330854194e+02, -2.831561238911424, -35.086689924296930,
+                  -7.714514400144169, -2.532042171876453],
+     "mixtureAlpha": [0.105617258338313, 0.067057292201984, 0.758751296773679, 0.206483373674102,
+                      0.359274660451266, 0.326443781481564]},
+    {"mixtureMu": [0, 8.567948260089990, 40.747148947304666, 1.512176451996613, -0.006999141602579],
+     "mixtureC": [-2.531024246969291, -8.257797032110451, -38.877640140958086, -2.886694147327419,
+                  -2.531032411619821],
+     "mixtureAlpha": [0.253871838456329, 0.451553613239657, 0.137923738827578, 1.057954808434358,
+                      0.476692395924039]},
+    {"mixtureMu": [0, 11.579090874816515, 1.779309977673783, 67.550452715965830, 4.215886314000809e+02,
+                   2.882436622668382e+03, -0.005285363127619],
+     "mixtureC": [-2.531024246969291, -10.967766980382587, -3.012069941353215, -65.175455016052640,
+                  -4.173824784349965e+02, -2.876308108469822e+03, -2.531028902808854],
+     "mixtureAlpha": [0.063275130856032, 0.164645985522835, 0.319201646728633, 0.107094124396163,
+                      0.065917372206027, 0.026062432689170, 0.164887237242181]},
+]
+
+
+def confocal(focal_x=np.arange(-50.0, 51.0, 1.0), focal_z=np.arange(0.0, 21.0, 2.0), tau_x=(-1.0, 0.0, 1.0),
+             batch: int = 50, full_iteration: int = 100, seed: int = 1, aperture: float = 0.4) -> dict:
+    """The configuration of the reference's examples/confocalRenderings.m (lines 8-95): four cells of
+    vMF-mixture materials in [-100, 100] x [-100, 100] x [0, 20], GaussianBeam illuminations focused
+    on an (x, z) grid along +z, GaussianBeam views on the same grid shifted by tau_x in x."""
+    fx, fz = np.asarray(focal_x, np.float64), np.asarray(focal_z, np.float64)
+    xl, zl = np.meshgrid(fx, fz, indexing="ij")
+    xv, zv, tv = np.meshgrid(fx, fz, np.asarray(tau_x, np.float64), indexing="ij")
+    xl, zl = xl.ravel(order="F"), zl.ravel(order="F")
+    xv, zv, tv = xv.ravel(order="F"), zv.ravel(order="F"), tv.ravel(order="F")
+    sigs = np.array([0, 1 / 5, 1 / 10, 1 / 4, 1 / 20])
+
+    def beams(x, z):
+        n = x.size
+        return {"type": "GaussianBeam", "focalPoint": {"x": x, "y": np.zeros(n), "z": z},
+                "focalDirection": {"x": np.zeros(n), "y": np.zeros(n), "z": np.ones(n)}, "aperture": aperture}
+
+    return {
+        "simulation": {"batchSize": batch, "fullIteration": full_iteration, "seed": seed, "lambda": 1.0},
+        "scattering": {"type": ["vMF"] * 5, "amplitudeFunction": [dict(x) for x in CONFOCAL_LOBES]},
+        "medium": {"type": "Heterogeneous", "xAxis": [-100.0, 0.0, 100.0], "yAxis": [-100.0, 100.0],
+                   "zAxis": [0.0, 10.0, 20.0], "materialGrid": np.array([1, 2, 3, 4], np.uint32).reshape(2, 1, 2, order="F"),
+                   "sigs": sigs, "siga": sigs * 0},
+        "illumination": beams(xl, zl),
+        "view": beams(xv + tv, zv),
+        "sampler": _static_sampler(5),
+        "tracer": {"type": "nee"},
+    }
+
+
 def c4(paths: int = 1 << 24, batch: int = 1 << 16, seed: int = 6, perlin_seed: int = 5, n: int = 128,
        nside: int = 32) -> dict:
     """Transmission matrix: 128^3 Perlin-thresholded grid, 1024 x 1024 plane waves."""

@@ -198,3 +198,63 @@ def frame_table(cfg, dims):
     if rc:
         raise RuntimeError("oracle_frame_table failed with code %d" % rc)
     return dt, neg.value
+
+
+def gauss_f(cfg, view, i, x0, p2):
+    cc = CConfig(cfg, 3)   # keeps the arrays behind the struct's pointers alive
+    out = np.zeros(2)
+    rc = lib().oracle_gauss_f(cc.ptr(), C.c_int(view), C.c_uint32(i), _p(np.asarray(x0, np.float64)),
+                              _p(np.asarray(p2, np.float64)), _p(out))
+    if rc:
+        raise RuntimeError("oracle_gauss_f failed with code %d" % rc)
+    return out[0] + 1j * out[1]
+
+
+def gauss_fs(cfg, l, v, x0):
+    cc = CConfig(cfg, 3)   # keeps the arrays behind the struct's pointers alive
+    out = np.zeros(2)
+    rc = lib().oracle_gauss_fs(cc.ptr(), C.c_uint32(l), C.c_uint32(v), _p(np.asarray(x0, np.float64)),
+                               _p(out))
+    if rc:
+        raise RuntimeError("oracle_gauss_fs failed with code %d" % rc)
+    return out[0] + 1j * out[1]
+
+
+def gauss_pr(cfg, l, x1):
+    cc = CConfig(cfg, 3)   # keeps the arrays behind the struct's pointers alive
+    x1 = np.ascontiguousarray(np.atleast_2d(x1), np.float64)
+    out = np.zeros(x1.shape[0])
+    rc = lib().oracle_gauss_pr(cc.ptr(), C.c_uint32(l), C.c_uint32(x1.shape[0]), _p(x1), _p(out))
+    if rc:
+        raise RuntimeError("oracle_gauss_pr failed with code %d" % rc)
+    return out
+
+
+def gauss_pdir(cfg, l, x0, x1):
+    cc = CConfig(cfg, 3)   # keeps the arrays behind the struct's pointers alive
+    x1 = np.ascontiguousarray(np.atleast_2d(x1), np.float64)
+    out = np.zeros(x1.shape[0])
+    rc = lib().oracle_gauss_pdir(cc.ptr(), C.c_uint32(l), _p(np.asarray(x0, np.float64)),
+                                 C.c_uint32(x1.shape[0]), _p(x1), _p(out))
+    if rc:
+        raise RuntimeError("oracle_gauss_pdir failed with code %d" % rc)
+    return out
+
+
+def gauss_sample_first(cfg, n):
+    cc = CConfig(cfg, 3)   # keeps the arrays behind the struct's pointers alive
+    pts = np.zeros((n, 3)); beams = np.zeros(n, np.uint32)
+    rc = lib().oracle_gauss_sample_first(cc.ptr(), C.c_uint32(n), _p(pts), _p(beams))
+    if rc:
+        raise RuntimeError("oracle_gauss_sample_first failed with code %d" % rc)
+    return pts, beams
+
+
+def gauss_sample_second(cfg, l, x0, n):
+    cc = CConfig(cfg, 3)   # keeps the arrays behind the struct's pointers alive
+    d = np.zeros((n, 3))
+    rc = lib().oracle_gauss_sample_second(cc.ptr(), C.c_uint32(l), _p(np.asarray(x0, np.float64)),
+                                          C.c_uint32(n), _p(d))
+    if rc:
+        raise RuntimeError("oracle_gauss_sample_second failed with code %d" % rc)
+    return d

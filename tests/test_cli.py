@@ -76,7 +76,7 @@ def test_invalid_configs_match_python_messages(tmp_path, mutate):
     elif mutate == "scatter_type":
         c["scattering"]["type"] = ["Mie"]
     elif mutate == "source_type":
-        c["view"]["type"] = "GaussianBeam"
+        c["view"]["type"] = "Pencil"
     elif mutate == "sizes":
         c["view"]["direction"] = {"x": [0.0, 0.1], "y": [1.0]}
     msg = _py_error(c, 2)
