@@ -25,10 +25,13 @@ struct sstmc_handle {
     std::unique_ptr<NextEventEstimation> tracer;
 };
 
-static int buildSource(const sstmc_source& s, const Medium& med, int dims, bool illum, std::unique_ptr<Source>* out) {
+static int buildSource(const sstmc_source& s, const Medium& med, int dims, bool illum, std::unique_ptr<Source>* out,
+                       const Scattering* sc = nullptr, const sstmc_source* views = nullptr, double lambda = 1.0) {
     if (s.num == 0 || !s.x || !s.y || (dims == 3 && !s.z)) return SSTMC_INVALID_CONFIG;
     int err = SSTMC_NO_ERROR;
-    if (s.type == SSTMC_SOURCE_FARFIELD)
+    if (s.type == SSTMC_SOURCE_GAUSSIAN_BEAM && sc)
+        out->reset(new GaussianBeam(&err, med, *sc, s, views, lambda, illum));
+    else if (s.type == SSTMC_SOURCE_FARFIELD)
         out->reset(new FarField(&err, med, dims, s.x, s.y, s.z, s.num, illum));
     else if (s.type == SSTMC_SOURCE_POINT)
         out->reset(new PointSource(&err, dims, s.x, s.y, s.z, s.num, s.isAreaSource != 0, illum));
@@ -104,8 +107,24 @@ static int validate(const sstmc_config* c) {
     if (!(maj > 0)) return SSTMC_INVALID_CONFIG;
     const sstmc_source* srcs[2] = {&c->illumination, &c->view};
     for (const sstmc_source* s : srcs) {
-        if (s->type != SSTMC_SOURCE_FARFIELD && s->type != SSTMC_SOURCE_POINT) return SSTMC_INVALID_CONFIG;
+        if (s->type != SSTMC_SOURCE_FARFIELD && s->type != SSTMC_SOURCE_POINT && s->type != SSTMC_SOURCE_GAUSSIAN_BEAM)
+            return SSTMC_INVALID_CONFIG;
         if (s->num == 0 || !s->x || !s->y || (dims == 3 && !s->z)) return SSTMC_INVALID_CONFIG;
+        // GaussianBeam (MatlabInterface.cu:1153-1190): 3D, one direction per focal point, aperture > 0
+        if (s->type == SSTMC_SOURCE_GAUSSIAN_BEAM &&
+            (dims != 3 || !s->dirX || !s->dirY || !s->dirZ || !(s->aperture > 0)))
+            return SSTMC_INVALID_CONFIG;
+    }
+    if (c->illumination.type == SSTMC_SOURCE_GAUSSIAN_BEAM || c->view.type == SSTMC_SOURCE_GAUSSIAN_BEAM) {
+        // built: GaussianBeam illumination and view (fsGaussianBeam pairs two beams,
+        // GaussianBeam.cuh:2040-2096), every material a vMF mixture of at most 16 lobes
+        // (updateSampler, :1283-1290), field output of the TemporalSampler without coherent
+        // backscattering
+        if (c->illumination.type != c->view.type || c->tracer.isCBS || c->tracer.output != SSTMC_OUTPUT_FIELD ||
+            c->sampler.type == SSTMC_SAMPLER_TEMPORAL_CORRELATION)
+            return SSTMC_NOT_SUPPORTED;
+        for (uint32_t m = 0; m < sc.materialNum; ++m)
+            if (sc.type[m] != SSTMC_SCATTERING_VMF || sc.vmfMixtures[m] > 16) return SSTMC_NOT_SUPPORTED;
     }
     const sstmc_sampler& sp = c->sampler;
     if (sp.type != SSTMC_SAMPLER_TEMPORAL && sp.type != SSTMC_SAMPLER_TEMPORAL_CORRELATION && sp.type != 0)
@@ -208,8 +227,12 @@ static int create(const sstmc_config* c, sstmc_handle** out) {
             return err;
         if ((err = buildSourcePair(c->view, c->view2, *h->medium, dims, false, &h->view))) return err;
     } else {
-        if ((err = buildSource(c->illumination, *h->medium, dims, true, &h->illumination))) return err;
-        if ((err = buildSource(c->view, *h->medium, dims, false, &h->view))) return err;
+        const double lam = c->simulation.lambda > 0 ? c->simulation.lambda : 1.0;
+        if ((err = buildSource(c->illumination, *h->medium, dims, true, &h->illumination, h->scattering.get(), &c->view,
+                               lam)))
+            return err;
+        if ((err = buildSource(c->view, *h->medium, dims, false, &h->view, h->scattering.get(), nullptr, lam)))
+            return err;
     }
 
     // sampler (MatlabInterface.cu:1217-1368)

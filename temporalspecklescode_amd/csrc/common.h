@@ -200,8 +200,36 @@ __host__ __device__ inline uint32_t cdfSearch(const double* a, uint32_t n, doubl
 // FarField illumination sampling table per source, 20 floats: tangent[3] perpT[3] perpS[3]
 // minT maxT minS maxS invDir[3] dir[3] -- precomputed on the host in FP64.
 constexpr int kFarFieldTable = 20;
+// GaussianBeam sources (GaussianBeam.cuh, 3D), FP64: focal points and beam directions,
+// gammaA = 1 / aperture^2, apNorm = vMFnormalizationLog(gammaA) (:1053-1062) and the wave number
+// 2 pi / lambda. An illumination also carries the sampling tables of updateSampler (:1281-1613),
+// built on the host: per beam its focal material, the beam's extent [zmin, zmax] across the box
+// in B bins of zdt and a transverse basis (xyb, 6 per beam); per mixture of the focal material
+// (row mixOff[l] + i) its weight and exclusive cumulative weight (apdf, aicdf) and B z-densities
+// with their exclusive cumulative sums (zpdf, zicdf); prDen[m] = sigs(m) times the box depth in z
+// (the first-point probability's denominator) and meanView, the mean view beam direction (the
+// axis of the second-point proposal).
+struct GaussBeams {
+    const double* focal;           // n x 3
+    const double* dir;             // n x 3
+    double gammaA, apNorm, waveK;
+    d3 meanView;
+    uint32_t B;
+    const uint32_t* focalMat;      // [n]
+    const uint32_t* mixOff;        // [n]
+    const double* zmin;            // [n]
+    const double* zmax;
+    const double* zdt;
+    const double* xyb;             // n x 6
+    const double* apdf;            // [rows]
+    const double* aicdf;
+    const double* zpdf;            // [rows x B]
+    const double* zicdf;
+    double prDen[kMaxMaterials];
+};
+
 struct SourceView {
-    int type;                      // 1 point, 2 far field
+    int type;                      // 1 point, 2 far field, 3 Gaussian beam
     int illum;                     // 1 illumination, 0 view
     int area;                      // PointSource isAreaSource
     uint32_t n;
@@ -210,6 +238,7 @@ struct SourceView {
     const float* table;            // FarField illumination sampling table (n x 20), else null
     const float* probf;            // FarField 1 / projected area, n
     const double* probd;
+    GaussBeams g;                  // GaussianBeam (pf / pd then hold the focal points)
 };
 
 // Temporal frames: TemporalSampler.cuh:14-38 + the time table built in its ctor (:941-1132).

@@ -13,6 +13,7 @@
 // Split-K partial sums go to a workspace and are reduced in a fixed order, so a result does not
 // depend on scheduling: a request gives the same bits alone or in a mix.
 #include "common.h"
+#include "gauss.h"
 #include "kernels.h"
 
 namespace sstmc {
@@ -455,6 +456,116 @@ hipError_t launchSingleGeneric(int dims, int fp64, const ScatterView& sc, const 
         else
             hipLaunchKernelGGL((k_single_generic<3, float>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
                                (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr);
+        for (uint32_t t = 0; t < NT; ++t)
+            hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st,
+                               (const float2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,
+                               (const double2*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- GaussianBeam single scattering
+// S_p[l,v] = a_p[l] b_p[v] fs(l, v, x1) with fs the two beams' throughputs through the vMF lobes of
+// x1's material (fsGaussianBeam, GaussianBeam.cuh:894-999), FP64 whatever the term precision.
+// Block = 16 x 16 (l, v) outputs; per chunk of 16 paths the first vertices, their materials and
+// the terms are staged in LDS.
+template <typename R>
+__global__ void __launch_bounds__(256) k_single_gauss(MediumView m, ScatterView sc, SourceView il, SourceView vw,
+                                                      FramesView fr, VertexBatch vb,
+                                                      const typename SingleT<R>::C* __restrict__ a,
+                                                      const typename SingleT<R>::C* __restrict__ b, uint32_t P,
+                                                      uint32_t pathsPerSplit, uint32_t splits,
+                                                      typename SingleT<R>::C* __restrict__ part) {
+    typedef typename SingleT<R>::C C;
+    __shared__ double sx[16][3];
+    __shared__ uint32_t sm[16];
+    __shared__ C sa[16][16], sb[16][16];
+    const uint32_t L = il.n, V = vw.n;
+    const uint32_t tl = threadIdx.x & 15, tq = threadIdx.x >> 4;
+    const uint32_t split = blockIdx.z % splits, t = blockIdx.z / splits;
+    const uint32_t l = blockIdx.x * 16 + tl, v = blockIdx.y * 16 + tq;
+    const uint32_t pBegin = split * pathsPerSplit, pEnd = min(P, pBegin + pathsPerSplit);
+    const PhaseTables& tab = *sc.tab;
+    double re = 0, im = 0;
+    for (uint32_t pc = pBegin; pc < pEnd; pc += 16) {
+        const uint32_t p = pc + tq;
+        const uint32_t ls = blockIdx.x * 16 + tl, vs = blockIdx.y * 16 + tl;
+        C av, bv;
+        av.x = av.y = 0; bv.x = bv.y = 0;
+        const bool ok = p < pEnd && vb.k[p] > 0 && (uint64_t)vb.off[P] <= vb.cap;
+        if (ok) {
+            const uint64_t o = vb.off[p];
+            if (tl == 0) {
+                const float4 xm = vb.xm[o];
+                d3 x1 = mkd(xm.x, xm.y, xm.z);
+                if (fr.temporalAmp && !fr.trivial) {
+                    const float* d = vb.delta + (o * fr.nt + t) * 3;
+                    x1 = mkd(x1.x + (double)d[0], x1.y + (double)d[1], x1.z + (double)d[2]);
+                }
+                sx[tq][0] = x1.x; sx[tq][1] = x1.y; sx[tq][2] = x1.z;
+                sm[tq] = gbMaterial(m, x1);
+            }
+            if (ls < L) av = a[((size_t)t * P + p) * L + ls];
+            if (vs < V) bv = b[((size_t)t * P + p) * V + vs];
+        } else if (tl == 0) {
+            sm[tq] = 0;
+        }
+        sa[tq][tl] = av;
+        sb[tq][tl] = bv;
+        __syncthreads();
+        if (l < L && v < V)
+            for (int q = 0; q < 16; ++q) {
+                if (sm[q] == 0) continue;
+                const Cd f = gbFs(il.g, vw.g, tab, l, v, sm[q], mkd(sx[q][0], sx[q][1], sx[q][2]), il.g.waveK);
+                const C x = sa[q][tl], y = sb[q][tq];
+                const double pr = (double)x.x * y.x - (double)x.y * y.y, pi = (double)x.x * y.y + (double)x.y * y.x;
+                re += f.re * pr - f.im * pi;
+                im += f.re * pi + f.im * pr;
+            }
+        __syncthreads();
+    }
+    if (l < L && v < V) {
+        C c;
+        c.x = (R)re;
+        c.y = (R)im;
+        part[((size_t)blockIdx.z) * L * V + (size_t)v * L + l] = c;
+    }
+}
+
+hipError_t launchSingleGauss(int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
+                             const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
+                             const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st) {
+    const uint32_t L = il.n, V = vw.n, NT = fr.nt;
+    uint32_t tiles = ((L + 15) / 16) * ((V + 15) / 16);
+    uint32_t splits = chooseSplits(tiles, P, 64);
+    uint32_t pps = ((P + splits - 1) / splits + 15) / 16 * 16;
+    splits = (P + pps - 1) / pps;
+    if (splits < 1) splits = 1;
+    size_t elem = fp64 ? 16 : 8;
+    size_t need = (size_t)splits * NT * L * V * elem;
+    if (ws->bytes < need) {
+        if (ws->ptr) {
+            hipStreamSynchronize(st);
+            hipFree(ws->ptr);
+        }
+        ws->ptr = nullptr;
+        ws->bytes = 0;
+        hipError_t e = hipMalloc(&ws->ptr, need);
+        if (e != hipSuccess) return e;
+        ws->bytes = need;
+    }
+    dim3 grid((L + 15) / 16, (V + 15) / 16, splits * NT);
+    const uint32_t LV = L * V;
+    if (fp64) {
+        hipLaunchKernelGGL(k_single_gauss<double>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb, (const double2*)a,
+                           (const double2*)b, P, pps, splits, (double2*)ws->ptr);
+        for (uint32_t t = 0; t < NT; ++t)
+            hipLaunchKernelGGL(k_reduce<double2>, dim3((LV + 255) / 256), dim3(256), 0, st,
+                               (const double2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,
+                               (const double2*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
+    } else {
+        hipLaunchKernelGGL(k_single_gauss<float>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb, (const float2*)a,
+                           (const float2*)b, P, pps, splits, (float2*)ws->ptr);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st,
                                (const float2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,

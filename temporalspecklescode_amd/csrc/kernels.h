@@ -27,7 +27,8 @@ struct NeeOut {
 enum SingleMode : int {
     kSingleSeparable = 0,  // S = (a^T b) o F with F a fixed L x V matrix or scalar
     kSingleGeneric = 1,    // S_p[l,v] needs f(l -> x1 -> v) per path (fused kernel)
-    kSingleCorrelation = 2 // TemporalCorrelationSampler: f1 conj(f2) times the decorrelation at x1
+    kSingleCorrelation = 2, // TemporalCorrelationSampler: f1 conj(f2) times the decorrelation at x1
+    kSingleGauss = 3       // GaussianBeam sources: fsGaussianBeam per (l, v, path)
 };
 
 struct NeeParams {
@@ -82,6 +83,16 @@ hipError_t launchSingleCorrelation(int dims, int fp64, const ScatterView& sc, co
                                    const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
                                    const void* b, uint32_t P, double lambda, int uniformFar, double* U,
                                    Workspace* ws, hipStream_t st);
+
+// GaussianBeam set-up: per point i the material there (0 outside) and exp(-tau) along -dirs[i]
+// (3D, FP64).
+hipError_t launchGaussSetup(const MediumView& m, const double* pts, const double* dirs, uint32_t n, double* att,
+                            uint32_t* mat, hipStream_t st);
+// Single scattering between GaussianBeam illuminations and views (field output):
+// U[t][v][l] += sum_p a[t][p][l] b[t][p][v] fs(l, v, x1).
+hipError_t launchSingleGauss(int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
+                             const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
+                             const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st);
 
 // U += 0 helpers
 hipError_t launchZeroU(double* U, size_t n, hipStream_t st);

@@ -17,6 +17,7 @@
 // phase (source phase, rho, path length / lambda) is summed and reduced mod 1 in FP64, and path
 // lengths are accumulated in FP64 from the FP32 transport positions.
 #include "common.h"
+#include "gauss.h"
 #include "geometry.h"
 #include "kernels.h"
 
@@ -197,9 +198,25 @@ __device__ inline f3 sourceF(const SourceView& s, uint32_t i) {
     const float* q = s.pf + 3 * (size_t)i;
     return mk(q[0], q[1], q[2]);
 }
+// GaussianBeam: gGaussianBeam with the attenuation along the beam (GaussianBeam.cuh:795-816,
+// 1942-1992), the square root of the transmittance toward the beam's source side (illumination
+// against the beam direction, view along it), zero outside the medium; no phase.
+template <int DIMS, typename R>
+__device__ inline void gGauss(const MediumView& m, const R* sig, const d3& dir, bool illum, const Geo& x, R tauCut,
+                              uint32_t* steps, R* amp, double* ph) {
+    const d3 w = illum ? mkd(-dir.x, -dir.y, -dir.z) : dir;
+    const R tau = walk<DIMS, R>(m, sig, Pos<R>::pick(x.pf, x.pd), Pos<R>::fromD(w), tauCut, steps);
+    *amp = gbMaterial(m, x.pd) ? expR<R>(-(R)0.5 * tau) : (R)0;
+    *ph = 0;
+}
 template <int DIMS, typename R>
 __device__ inline void gTerm(const MediumView& m, const R* sig, const SourceView& s, uint32_t i, const Geo& x,
                              double invLambda, R tauCut, uint32_t* steps, R* amp, double* ph) {
+    if constexpr (DIMS == 3)
+        if (s.type == 3) {
+            gGauss<DIMS, R>(m, sig, gbDir(s.g, i), s.illum != 0, x, tauCut, steps, amp, ph);
+            return;
+        }
     if (s.type == 2)
         gFar<DIMS, R>(m, sig, sourceD<DIMS>(s, i), sizeof(R) == 4 ? sourceF(s, i) : mk(0.f, 0.f, 0.f), s.illum != 0,
                       x, invLambda, tauCut, steps, amp, ph);
@@ -298,8 +315,22 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
 
     // ---- path probabilities from the central first vertex (pathProbabilitySingle/Multiple)
     R s1 = 0, s2 = 0;
+    const bool gauss = TAB && DIMS == 3 && il.type == 3;
+    const uint32_t m1g = gauss ? gbMaterial(m, x1c.pd) : 0u;
     for (uint32_t l = sub; l < L; l += lpp) {
         R pr;
+        if (gauss) {
+            // firstPointProbability and secondPointProbability (GaussianBeam.cuh:420-487,
+            // 1835-1940): a zero second-point density becomes 1e20 unless every material scatters
+            // isotropically
+            const double q = gbPr(il.g, *sc.tab, l, x1c.pd, m1g);
+            s1 += (R)q;
+            if (multi) {
+                const double q2 = q * gbPdir(il.g, *sc.tab, sc.allIsotropic != 0, l, x1c.pd, seg12);
+                s2 += (q2 == 0 && !sc.allIsotropic) ? (R)1e20 : (R)q2;
+            }
+            continue;
+        }
         if (il.type == 2) {
             typename Pos<R>::V p1 = Pos<R>::pick(x1c.pf, x1c.pd);
             typename Pos<R>::V w;
@@ -339,12 +370,20 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         for (int f = 0; f < nf; ++f) {
             uint32_t tt = (corr && f == 0) ? 0 : t;
             Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
-            gTerm<DIMS, R>(m, sig, il, l, x1t, invLambda, il.type == 2 ? tauCutFF : tauInf, &steps, &ga[f], &gp[f]);
+            gTerm<DIMS, R>(m, sig, il, l, x1t, invLambda, il.type != 1 ? tauCutFF : tauInf, &steps, &ga[f], &gp[f]);
             if (multi) {
                 Geo x2t = vertexFrame(vb, fr, off + 1, tt, x2m);
                 const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
                 const d3 pb = fr.temporalAmp ? x2t.pd : x2cd;
-                if (isoPointIl) fa[f] = isoAmp<DIMS, R>();
+                if (gauss) {   // fGaussianBeam (GaussianBeam.cuh:818-892) of x1's material toward x2
+                    const uint32_t mg = gbMaterial(m, pa);
+                    if (mg) {
+                        const Cd fz = gbF(il.g, *sc.tab, gbFocal(il.g, l), gbDir(il.g, l), mg, pa,
+                                          dnormalize<DIMS>(dsub(pb, pa)), il.g.waveK);
+                        fa[f] = (R)cabsd(fz);
+                        fp[f] = atan2(fz.im, fz.re) * 0.15915494309189533577;
+                    }
+                } else if (isoPointIl) fa[f] = isoAmp<DIMS, R>();
                 else fa[f] = phaseAmp<DIMS, R, TAB>(sc, m1, ddot<DIMS>(illumIn<DIMS>(il, l, pa), dnormalize<DIMS>(dsub(pb, pa))),
                                                &fp[f]);
             }
@@ -378,7 +417,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         for (int f = 0; f < nf; ++f) {
             uint32_t tt = (corr && f == 0) ? 0 : t;
             Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
-            gTerm<DIMS, R>(m, sig, vw, v, x1t, invLambda, vw.type == 2 ? tauCutFF : tauInf, &steps, &ga[f], &gp[f]);
+            gTerm<DIMS, R>(m, sig, vw, v, x1t, invLambda, vw.type != 1 ? tauCutFF : tauInf, &steps, &ga[f], &gp[f]);
         }
         const size_t o = ((size_t)t * P + p) * V + v;
         b[o] = corr ? cis<R>(ga[0] * ga[1], gp[0] - gp[1]) : cis<R>(ga[0], gp[0]);
@@ -609,7 +648,14 @@ __global__ void __launch_bounds__(256) k_vertex_prep(VertexBatch vb, FramesView 
 // flips the walk direction, the phase sign and the incoming direction) is no runtime branch in the
 // hot loop: point sources; FarField views (the NEE view terms); FarField illuminations (the
 // reversed paths of coherent backscattering).
-enum : int { kSrcPoint = 0, kSrcFarView = 1, kSrcFarIllum = 2 };
+enum : int { kSrcPoint = 0, kSrcFarView = 1, kSrcFarIllum = 2, kSrcGauss = 3 };
+
+// Beam direction of GaussianBeam source i (other kinds: unused, vd returned).
+template <int SRC>
+__device__ inline d3 sourceDir(const SourceView& s, uint32_t i, const d3& vd) {
+    if constexpr (SRC == kSrcGauss) return gbDir(s.g, i);
+    else return vd;
+}
 
 // G_k of one source at one frame: attenuated source term g(x_k^t) times the HG amplitude against
 // the record's back direction, with the record's phase base added to the source phase. The
@@ -618,10 +664,23 @@ enum : int { kSrcPoint = 0, kSrcFarView = 1, kSrcFarIllum = 2 };
 template <int DIMS, typename R, int SRC, bool TAB>
 __device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, const ScatterView& sc,
                                             const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
-                                            const d3& vd, const f3& vf, uint64_t ok, uint32_t t, const float4& xkm,
-                                            double invLambda, R tauCut, bool isoPoint, uint32_t* steps) {
+                                            const d3& vd, const d3& vdir, const f3& vf, uint64_t ok, uint32_t t,
+                                            const float4& xkm, double invLambda, R tauCut, bool isoPoint,
+                                            uint32_t* steps) {
     const Geo xk = vertexFrame(vb, fr, ok, t, xkm);
     const VertexRec rec = vb.rec[ok * fr.nt + t];
+    if constexpr (SRC == kSrcGauss) {
+        // g along the view beam times fGaussianBeam of x_k's material against the direction from
+        // x_{k-1} (focal point vd, beam direction vdir, negative wave number)
+        R ga;
+        double gp;
+        gGauss<DIMS, R>(m, sig, vdir, false, xk, tauCut, steps, &ga, &gp);
+        const d3 pk = fr.temporalAmp ? xk.pd : mkd(xkm.x, xkm.y, xkm.z);
+        const uint32_t mg = gbMaterial(m, pk);
+        if (mg == 0 || ga == (R)0) return Cx<R>::make(0, 0);
+        const Cd fz = gbF(vw.g, *sc.tab, vd, vdir, mg, pk, mkd(-rec.bx, -rec.by, -rec.bz), -vw.g.waveK);
+        return cis<R>(ga * (R)cabsd(fz), atan2(fz.im, fz.re) * 0.15915494309189533577 + rec.phase);
+    }
     R ga;
     double gp;
     if constexpr (SRC == kSrcPoint) gPoint<DIMS, R>(m, sig, vd, vw.area != 0, xk, invLambda, steps, &ga, &gp);
@@ -682,7 +741,7 @@ __device__ inline ViewUnit viewUnit(const NeeParams& prm) {
 // vertex loop at 64), 4 for FP64 (no spills; 1.6x faster than 6 waves with spills).
 template <int DIMS, typename R, int SRC, bool WAVE, bool TAB>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 4 : (SRC != kSrcPoint ? 8 : 6))))
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 || SRC == kSrcGauss ? 4 : (SRC != kSrcPoint ? 8 : 6))))
 k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
     __shared__ R sigLds[kSigLds];
@@ -701,13 +760,15 @@ k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBat
     const bool active = v < V && K >= 2;
     // direction (far field) or location (point) of this lane's source
     const d3 vd = sourceD<DIMS>(vw, active ? v : 0);
-    const f3 vf = SRC != kSrcPoint && sizeof(R) == 4 ? sourceF(vw, active ? v : 0) : mk(0.f, 0.f, 0.f);
+    const f3 vf = SRC != kSrcPoint && SRC != kSrcGauss && sizeof(R) == 4 ? sourceF(vw, active ? v : 0)
+                                                                         : mk(0.f, 0.f, 0.f);
+    const d3 vdir = sourceDir<SRC>(vw, active ? v : 0, vd);
     if (active) {
         R accRe = 0, accIm = 0;
         for (uint32_t k = 1; k < K; ++k) {
             const uint64_t ok = off + k;
-            const C g = viewTerm<DIMS, R, SRC, TAB>(m, sig, sc, vw, fr, vb, vd, vf, ok, t, vb.xm[ok], invLambda, tauCut,
-                                               isoPoint, &steps);
+            const C g = viewTerm<DIMS, R, SRC, TAB>(m, sig, sc, vw, fr, vb, vd, vdir, vf, ok, t, vb.xm[ok], invLambda,
+                                                    tauCut, isoPoint, &steps);
             accRe += g.x;
             accIm += g.y;
         }
@@ -749,13 +810,13 @@ __global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView
         for (uint32_t k = 1; k < K; ++k) {
             const uint64_t ok = off + k;
             const float4 xkm = vb.xm[ok];
-            const C g0 = viewTerm<DIMS, R, SRC, TAB>(m, sig, sc, vw, fr, vb, vd, vf, ok, 0, xkm, invLambda, tauCut,
-                                                isoPoint, &steps);
+            const C g0 = viewTerm<DIMS, R, SRC, TAB>(m, sig, sc, vw, fr, vb, vd, vd, vf, ok, 0, xkm, invLambda, tauCut,
+                                                     isoPoint, &steps);
             for (uint32_t j = 0; j < nt; ++j) {
                 const uint32_t t = t0 + j;
                 const C gt = t == 0 ? g0
-                                    : viewTerm<DIMS, R, SRC, TAB>(m, sig, sc, vw, fr, vb, vd, vf, ok, t, xkm, invLambda,
-                                                              tauCut, isoPoint, &steps);
+                                    : viewTerm<DIMS, R, SRC, TAB>(m, sig, sc, vw, fr, vb, vd, vd, vf, ok, t, xkm,
+                                                                  invLambda, tauCut, isoPoint, &steps);
                 C a = acc[j][threadIdx.x];
                 a.x += g0.x * gt.x + g0.y * gt.y;      // G(0) * conj(G(t))
                 a.y += g0.y * gt.x - g0.x * gt.y;
@@ -918,7 +979,7 @@ static void launchView(bool corr, const MediumView& m, const ScatterView& sc, co
     const bool wave = q.lppView == 64, tab = sc.tab != nullptr;
 #define SSTMC_VIEW(K, W, T) hipLaunchKernelGGL((K<DIMS, R, SRC, W, T>), grid, block, 0, st, m, sc, src, fr, vb, q, out)
     if (corr) {
-        if constexpr (SRC != kSrcFarIllum) {
+        if constexpr (SRC != kSrcFarIllum && SRC != kSrcGauss) {
             if (wave) { if (tab) SSTMC_VIEW(k_nee_view_corr, true, true); else SSTMC_VIEW(k_nee_view_corr, true, false); }
             else { if (tab) SSTMC_VIEW(k_nee_view_corr, false, true); else SSTMC_VIEW(k_nee_view_corr, false, false); }
         }
@@ -971,7 +1032,9 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
                        1.0 / p.lambda);
     if (between) hipEventRecord(between, st);
     const bool corr = p.output == 1;
-    if (vw.type == 2) launchView<DIMS, R, kSrcFarView>(corr, m, sc, vw, fr, vb, p, out, st);
+    if (vw.type == 3) {
+        if constexpr (DIMS == 3) launchView<DIMS, R, kSrcGauss>(corr, m, sc, vw, fr, vb, p, out, st);
+    } else if (vw.type == 2) launchView<DIMS, R, kSrcFarView>(corr, m, sc, vw, fr, vb, p, out, st);
     else launchView<DIMS, R, kSrcPoint>(corr, m, sc, vw, fr, vb, p, out, st);
     if (out.Br) {
         // coherent backscattering: the same per-vertex sums with the illumination connected at

@@ -92,6 +92,8 @@ public:
     // amplitude (complex: tabular amplitudes carry a phase) at the scattering cosine
     void amplitude(uint32_t m, double cosang, double* re, double* im) const;
     const ScatterView& view() const { return v_; }
+    // the vMF lobes of material m (kappa, C + log alpha); 0 when m is not a vMF mixture
+    uint32_t vmfLobes(uint32_t m, const double** kappa, const double** c) const;
 
 private:
     ScatterView v_{};
@@ -147,6 +149,21 @@ class PointSource : public Source {
 public:
     PointSource(int* err, int dims, const double* x, const double* y, const double* z, uint32_t n, bool areaSource,
                 bool illumination);
+};
+
+// GaussianBeam (GaussianBeam.cuh:1026-1613), 3D: beams focused at the given points along the
+// given directions with aperture a (gamma_a = 1 / a^2). An illumination also builds the sampling
+// tables of updateSampler on the host in FP64 (the attenuation along each beam on the device), the
+// axis of its second-point proposal being the mean of `views`' beam directions. Every material
+// holding a focal point must be a vMF mixture (at most 16 lobes).
+class GaussianBeam : public Source {
+public:
+    GaussianBeam(int* err, const Medium& medium, const Scattering& scattering, const sstmc_source& s,
+                 const sstmc_source* views, double lambda, bool illumination);
+
+private:
+    DevBuf<double> gd_;
+    DevBuf<uint32_t> gu_;
 };
 
 // ---------------------------------------------------------------- Sampler

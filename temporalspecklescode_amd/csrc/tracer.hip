@@ -1,5 +1,6 @@
 // tracer.hip -- host implementation of the plugin classes and the per-batch driver.
 #include "tracer.h"
+#include "gauss.h"
 
 #include <algorithm>
 #include <cmath>
@@ -180,14 +181,6 @@ int Scattering::setTabularScattering(const double* f, uint32_t n, uint32_t m) {
     defined_[m] = true;
     v_.allIsotropic = isIsotropicScattering() ? 1 : 0;
     return SSTMC_NO_ERROR;
-}
-
-// log of the 3D vMF normalisation kappa / (4 pi sinh kappa) (vMFnormalizationLog,
-// SpecialFunctions.cuh:187-201), with the reference's small- and large-kappa branches.
-static double vmfNormLog3(double x) {
-    if (x < 1e-12) return -std::log(4.0 * kPi);
-    if (x < 5.0) return std::log(x) - std::log(4.0 * kPi * std::sinh(x));
-    return std::log(x) - std::log(2.0 * kPi) - x;
 }
 
 // Scattering.cuh:635-817: per mixture kappa and C + log(alpha); per pair (i + M j) the summed
@@ -447,6 +440,213 @@ PointSource::PointSource(int* err, int dims, const double* x, const double* y, c
     v_.pd = pd_.p;
 }
 
+// ====================================================================== GaussianBeam
+uint32_t Scattering::vmfLobes(uint32_t m, const double** kappa, const double** c) const {
+    if (m >= (uint32_t)kMaxMaterials || v_.type[m] != SSTMC_SCATTERING_VMF) return 0;
+    *kappa = vmfKappa_.data() + tab_.vmfOff[m];
+    *c = vmfC_.data() + tab_.vmfOff[m];
+    return tab_.vmfM[m];
+}
+
+GaussianBeam::GaussianBeam(int* err, const Medium& medium, const Scattering& scattering, const sstmc_source& s,
+                           const sstmc_source* views, double lambda, bool illumination) {
+    *err = SSTMC_NO_ERROR;
+    const uint32_t n = s.num;
+    const double PI = 3.14159265358979323846;
+    v_.type = SSTMC_SOURCE_GAUSSIAN_BEAM;
+    v_.illum = illumination ? 1 : 0;
+    v_.n = n;
+    if (medium.dims() != 3 || n == 0 || !s.dirX || !s.dirY || !s.dirZ || !(s.aperture > 0) || !(lambda > 0)) {
+        *err = SSTMC_INVALID_CONFIG;
+        return;
+    }
+    pts_.assign(3 * (size_t)n, 0.0);
+    std::vector<double> dir(3 * (size_t)n);
+    std::vector<float> pf(3 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        pts_[3 * i] = s.x[i]; pts_[3 * i + 1] = s.y[i]; pts_[3 * i + 2] = s.z[i];
+        dir[3 * i] = s.dirX[i]; dir[3 * i + 1] = s.dirY[i]; dir[3 * i + 2] = s.dirZ[i];
+        for (int a = 0; a < 3; ++a) pf[3 * i + a] = (float)pts_[3 * i + a];
+    }
+    if ((*err = pd_.upload(pts_.data(), pts_.size()))) return;
+    if ((*err = pf_.upload(pf.data(), pf.size()))) return;
+    v_.pf = pf_.p;
+    v_.pd = pd_.p;
+    GaussBeams& g = v_.g;
+    g.gammaA = 1.0 / (s.aperture * s.aperture);
+    g.apNorm = vmfNormLog3(g.gammaA);
+    g.waveK = 2.0 * PI / lambda;
+    g.focal = pd_.p;
+    g.meanView = mkd(0.0, 0.0, 1.0);
+    if (!illumination) {
+        if ((*err = gd_.upload(dir.data(), dir.size()))) return;
+        g.dir = gd_.p;
+        return;
+    }
+    // mean view beam direction (GaussianBeam.cuh:1176-1194); the z axis when it vanishes
+    if (views && views->num > 0 && views->dirX && views->dirY && views->dirZ) {
+        double mv[3] = {0, 0, 0};
+        for (uint32_t i = 0; i < views->num; ++i) {
+            const double w = 1.0 / (double)views->num;
+            mv[0] = mv[0] + w * views->dirX[i];
+            mv[1] = mv[1] + w * views->dirY[i];
+            mv[2] = mv[2] + w * views->dirZ[i];
+        }
+        const double r = std::sqrt(mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2]);
+        if (!(r < 0.0001)) g.meanView = mkd(mv[0] * (1.0 / r), mv[1] * (1.0 / r), mv[2] * (1.0 / r));
+    }
+    // updateSampler (GaussianBeam.cuh:1281-1613): zB = 1000 points along each beam across the box
+    // (initSamplingTable, fillTimesInSamplingTable, :177-224), the projection of the box corners on
+    // the beam giving its extent
+    const uint32_t B = 1000;
+    g.B = B;
+    const MediumView& mv = medium.view();
+    std::vector<double> zmin(n), zmax(n), zdt(n), xyb(6 * (size_t)n);
+    std::vector<double> pts(3 * (size_t)n * (B + 1)), wd(3 * (size_t)n * (B + 1));
+    for (uint32_t l = 0; l < n; ++l) {
+        const double* f = &pts_[3 * l];
+        const double* d = &dir[3 * l];
+        double lo = 0, hi = 0;
+        for (int c = 0; c < 8; ++c) {
+            const double p[3] = {(c & 1) ? mv.bmaxd[0] : mv.bmind[0], (c & 2) ? mv.bmaxd[1] : mv.bmind[1],
+                                 (c & 4) ? mv.bmaxd[2] : mv.bmind[2]};
+            double o[3];
+            for (int a = 0; a < 3; ++a) {
+                double acc = 0;
+                for (int b = 0; b < 3; ++b) acc += d[a] * d[b] * (p[b] - f[b]);
+                o[a] = acc;
+            }
+            const double t = o[0] * d[0] + o[1] * d[1] + o[2] * d[2];
+            if (c == 0 || t < lo) lo = t;
+            if (c == 0 || t > hi) hi = t;
+        }
+        zmin[l] = lo;
+        zmax[l] = hi;
+        zdt[l] = (hi - lo) / (double)B;
+        // orthogonalBase (VectorType.cuh:293-311)
+        double* b0 = &xyb[6 * l];
+        double* b1 = &xyb[6 * l + 3];
+        if (std::fabs(1.0 - std::fabs(d[2])) <= 1e-12) {
+            b0[0] = 1; b0[1] = 0; b0[2] = 0;
+            b1[0] = 0; b1[1] = 1; b1[2] = 0;
+        } else {
+            double a0 = -d[1], a1 = d[0];
+            const double nn = std::sqrt(a0 * a0 + a1 * a1);
+            a0 = a0 / nn;
+            a1 = a1 / nn;
+            b0[0] = a0; b0[1] = a1; b0[2] = 0.0;
+            b1[0] = a1 * d[2] - 0.0 * d[1];
+            b1[1] = 0.0 * d[0] - a0 * d[2];
+            b1[2] = a0 * d[1] - a1 * d[0];
+        }
+        // point 0: the focal point (its material); points 1..B: the z samples
+        const size_t o = (size_t)l * (B + 1);
+        for (int a = 0; a < 3; ++a) { pts[3 * o + a] = f[a]; wd[3 * o + a] = d[a]; }
+        for (uint32_t b = 0; b < B; ++b) {
+            const double t = zmin[l] + (double)b * zdt[l];
+            for (int a = 0; a < 3; ++a) {
+                pts[3 * (o + 1 + b) + a] = f[a] + t * d[a];
+                wd[3 * (o + 1 + b) + a] = d[a];
+            }
+        }
+    }
+    const uint32_t np = n * (B + 1);
+    DevBuf<double> dp, dw, datt;
+    DevBuf<uint32_t> dmat;
+    if ((*err = dp.upload(pts.data(), pts.size())) || (*err = dw.upload(wd.data(), wd.size())) ||
+        (*err = datt.alloc(np)) || (*err = dmat.alloc(np)))
+        return;
+    std::vector<double> att(np);
+    std::vector<uint32_t> mat(np);
+    if (launchGaussSetup(mv, dp.p, dw.p, np, datt.p, dmat.p, nullptr) != hipSuccess ||
+        hipMemcpy(att.data(), datt.p, np * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(mat.data(), dmat.p, np * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) {
+        *err = SSTMC_DEVICE_ERROR;
+        return;
+    }
+    // per beam: the focal material's mixture weights exp(C + log-normalisation(|mu|))
+    // (alphaPdfIcdfKernel, :254-295, with its large-kappa term log(2 pi) * kappa kept as written)
+    // and per mixture the z density attenuation / w(z)^2 (samplingPdfKernel, :297-324),
+    // exclusive-scanned and normalised, the density scaled by B - 1 (:336-355)
+    std::vector<uint32_t> focalMat(n), mixOff(n);
+    std::vector<double> apdf, aicdf, zpdf, zicdf;
+    const double k = g.waveK;
+    for (uint32_t l = 0; l < n; ++l) {
+        const size_t o = (size_t)l * (B + 1);
+        const uint32_t fm = mat[o];
+        const double* kap = nullptr;
+        const double* cc = nullptr;
+        const uint32_t M = fm ? scattering.vmfLobes(fm, &kap, &cc) : 0;
+        if (M == 0) { *err = SSTMC_INVALID_CONFIG; return; }
+        if (M > kGaussLobes) { *err = SSTMC_NOT_SUPPORTED; return; }
+        focalMat[l] = fm;
+        mixOff[l] = (uint32_t)apdf.size();
+        const size_t a0 = apdf.size();
+        double asum = 0;
+        for (uint32_t i = 0; i < M; ++i) {
+            const double ka = std::fabs(kap[i]);
+            const double ln = ka < 1e-5 ? -std::log(4.0 * PI)
+                                        : std::log(ka) - (ka > 10.0 ? std::log(2.0 * PI) * ka
+                                                                    : std::log(4.0 * PI * std::sinh(ka)));
+            apdf.push_back(std::exp(cc[i] + ln));
+            asum += apdf.back();
+        }
+        double acc = 0;
+        for (uint32_t i = 0; i < M; ++i) {
+            apdf[a0 + i] /= asum;
+            aicdf.push_back(acc);
+            acc += apdf[a0 + i];
+        }
+        const size_t z0 = zpdf.size();
+        zpdf.resize(z0 + (size_t)M * B);
+        zicdf.resize(z0 + (size_t)M * B);
+        const double* f = &pts_[3 * l];
+        const double* d = &dir[3 * l];
+        for (uint32_t b = 0; b < B; ++b) {
+            const double* z = &pts[3 * (o + 1 + b)];
+            const double dz = (z[0] - f[0]) * d[0] + (z[1] - f[1]) * d[1] + (z[2] - f[2]) * d[2];
+            for (uint32_t i = 0; i < M; ++i) {
+                const double wz2 = (g.gammaA + std::fabs(kap[i])) / (k * k) + dz * dz / g.gammaA;
+                zpdf[z0 + (size_t)i * B + b] = att[o + 1 + b] / wz2;
+            }
+        }
+        for (uint32_t i = 0; i < M; ++i) {
+            double* zp = &zpdf[z0 + (size_t)i * B];
+            double* zi = &zicdf[z0 + (size_t)i * B];
+            double run = 0;
+            for (uint32_t b = 0; b < B; ++b) { zi[b] = run; run += zp[b]; }
+            const double sum = zp[B - 1] + zi[B - 1];
+            for (uint32_t b = 0; b < B; ++b) {
+                zp[b] *= (double)(B - 1) / sum;
+                zi[b] /= sum;
+            }
+        }
+    }
+    for (uint32_t m = 0; m < medium.materialNum() && m < (uint32_t)kMaxMaterials; ++m)
+        g.prDen[m] = medium.sigs(m) * (mv.bmaxd[2] - mv.bmind[2]);
+    // one FP64 allocation: dir, zmin, zmax, zdt, xyb, apdf, aicdf, zpdf, zicdf; one for the ids
+    std::vector<double> all;
+    std::vector<size_t> at;
+    for (const std::vector<double>* v : {&dir, &zmin, &zmax, &zdt, &xyb, &apdf, &aicdf, &zpdf, &zicdf}) {
+        at.push_back(all.size());
+        all.insert(all.end(), v->begin(), v->end());
+    }
+    std::vector<uint32_t> ids(focalMat);
+    ids.insert(ids.end(), mixOff.begin(), mixOff.end());
+    if ((*err = gd_.upload(all.data(), all.size())) || (*err = gu_.upload(ids.data(), ids.size()))) return;
+    g.dir = gd_.p + at[0];
+    g.zmin = gd_.p + at[1];
+    g.zmax = gd_.p + at[2];
+    g.zdt = gd_.p + at[3];
+    g.xyb = gd_.p + at[4];
+    g.apdf = gd_.p + at[5];
+    g.aicdf = gd_.p + at[6];
+    g.zpdf = gd_.p + at[7];
+    g.zicdf = gd_.p + at[8];
+    g.focalMat = gu_.p;
+    g.mixOff = gu_.p + n;
+}
+
 // ====================================================================== TemporalSampler
 TemporalSampler::TemporalSampler(int* err, const double* t, uint32_t nt, bool isTemporalAmplitude,
                                  bool correlation) {
@@ -554,7 +754,15 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
     double iso = dims_ == 3 ? 0.282094791773878143 : 0.398942280401432678;
     singleMode_ = kSingleGeneric;
     bool needF = false;
-    if (il_.type() == SSTMC_SOURCE_POINT && sc_.isIsotropicScattering()) {
+    if (il_.type() == SSTMC_SOURCE_GAUSSIAN_BEAM || vw_.type() == SSTMC_SOURCE_GAUSSIAN_BEAM) {
+        // both roles GaussianBeam (fsGaussianBeam pairs two beams), field output of the
+        // TemporalSampler without coherent backscattering
+        if (il_.type() != vw_.type() || acorr || opt.cbs || opt.output != SSTMC_OUTPUT_FIELD || dims_ != 3) {
+            *err = SSTMC_NOT_SUPPORTED;
+            return;
+        }
+        singleMode_ = kSingleGauss;
+    } else if (il_.type() == SSTMC_SOURCE_POINT && sc_.isIsotropicScattering()) {
         singleMode_ = kSingleSeparable;
         fscalar_ = iso;
     } else if (il_.type() == SSTMC_SOURCE_FARFIELD && vw_.type() == SSTMC_SOURCE_FARFIELD && sc_.isUniform(nm)) {
@@ -653,6 +861,9 @@ int NextEventEstimation::contractSingle(uint32_t npaths, double* U) {
     if (singleMode_ == kSingleSeparable) {
         he = launchContract(contractKind_, a_.p, b_.p, npaths, L_, V_, T_, Ff_.p, Fd_.p,
                             opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0, fscalar_, U, &wsSingle_, stream_);
+    } else if (singleMode_ == kSingleGauss) {
+        he = launchSingleGauss(fp64_ ? 1 : 0, mview_, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p, npaths,
+                               U, &wsSingle_, stream_);
     } else if (singleMode_ == kSingleCorrelation) {
         const bool uniformFar = il_.type() == SSTMC_SOURCE_FARFIELD && vw_.type() == SSTMC_SOURCE_FARFIELD &&
                                 sc_.isUniform(medium_.materialNum());

@@ -8,6 +8,7 @@
 // their offsets. The counter-based stream makes both passes draw identical numbers; the write pass
 // never stores beyond the counted vertices and raises an overflow flag if the pool is too small.
 #include "common.h"
+#include "gauss.h"
 #include "geometry.h"
 #include "kernels.h"
 
@@ -85,7 +86,11 @@ __device__ inline void frameDeltas(const FramesView& fr, uint64_t seed, uint64_t
     }
 }
 
-template <int DIMS, bool TAB>
+// GB: GaussianBeam illumination (3D), whose first point is drawn from the beam profile
+// (sampleFirstBeam, GaussianBeam.cuh:357-418; a point outside the medium is a miss, no absorption
+// test) and whose second direction comes from the beam's lobe-pair proposal
+// (sampleSecondPoint, :1673-1748).
+template <int DIMS, bool TAB, bool GB>
 __global__ void __launch_bounds__(256) k_transport(MediumView m, ScatterView sc, SourceView il, FramesView fr,
                                                    VertexBatch vb, uint64_t seed, uint64_t pathBegin,
                                                    uint32_t npaths, int write) {
@@ -95,7 +100,7 @@ __global__ void __launch_bounds__(256) k_transport(MediumView m, ScatterView sc,
     uint32_t misses = 0;
     uint32_t event = 0;
     f3 x, dir;
-    uint32_t mat = 0;
+    uint32_t mat = 0, beam = 0;
     for (;;) {
         if (event >= kMaxAttempts) {
             atomicOr(vb.status + 1, 1u);
@@ -103,9 +108,16 @@ __global__ void __launch_bounds__(256) k_transport(MediumView m, ScatterView sc,
             return;
         }
         Rng rng(seed, path, event, kPurposeTransport);
-        f3 x0;
-        sampleSource<DIMS>(il, m, rng, &x0, &dir);
-        mat = woodcock<DIMS>(m, x0, dir, rng, &x);
+        if constexpr (GB) {
+            const d3 xd = gbSampleFirst(il.g, *sc.tab, il.n, rng, &beam);
+            x = mk((float)xd.x, (float)xd.y, (float)xd.z);
+            dir = mk((float)il.g.dir[3 * beam], (float)il.g.dir[3 * beam + 1], (float)il.g.dir[3 * beam + 2]);
+            mat = gbMaterial(m, xd);
+        } else {
+            f3 x0;
+            sampleSource<DIMS>(il, m, rng, &x0, &dir);
+            mat = woodcock<DIMS>(m, x0, dir, rng, &x);
+        }
         if (mat > 0) break;
         ++misses;
         ++event;
@@ -124,7 +136,13 @@ __global__ void __launch_bounds__(256) k_transport(MediumView m, ScatterView sc,
         if (k > 1) {
             ev = kEventVertex | k;
             Rng rng(seed, path, ev, kPurposeTransport);
-            f3 nd = sampleDirection<DIMS, TAB>(sc, mat, dir, rng);
+            f3 nd;
+            if constexpr (GB) {
+                if (k == 2) nd = gbSampleSecond(il.g, *sc.tab, sc.allIsotropic != 0, beam, mkd(x.x, x.y, x.z), rng);
+                else nd = sampleDirection<DIMS, TAB>(sc, mat, dir, rng);
+            } else {
+                nd = sampleDirection<DIMS, TAB>(sc, mat, dir, rng);
+            }
             f3 nx;
             uint32_t nm = woodcock<DIMS>(m, x, nd, rng, &nx);
             if (nm == 0) break;
@@ -218,11 +236,32 @@ hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc,
                            uint32_t npaths, int write, hipStream_t st) {
     dim3 grid((npaths + 255) / 256), block(256);
     const bool tab = sc.tab != nullptr;
-#define SSTMC_TRANSPORT(D, T) \
-    hipLaunchKernelGGL((k_transport<D, T>), grid, block, 0, st, m, sc, il, fr, vb, seed, pathBegin, npaths, write)
-    if (dims == 2) { if (tab) SSTMC_TRANSPORT(2, true); else SSTMC_TRANSPORT(2, false); }
-    else { if (tab) SSTMC_TRANSPORT(3, true); else SSTMC_TRANSPORT(3, false); }
+#define SSTMC_TRANSPORT(D, T, G) \
+    hipLaunchKernelGGL((k_transport<D, T, G>), grid, block, 0, st, m, sc, il, fr, vb, seed, pathBegin, npaths, write)
+    if (il.type == 3) {
+        if (dims != 3 || !tab) return hipErrorInvalidValue;
+        SSTMC_TRANSPORT(3, true, true);
+    } else if (dims == 2) { if (tab) SSTMC_TRANSPORT(2, true, false); else SSTMC_TRANSPORT(2, false, false); }
+    else { if (tab) SSTMC_TRANSPORT(3, true, false); else SSTMC_TRANSPORT(3, false, false); }
 #undef SSTMC_TRANSPORT
+    return hipGetLastError();
+}
+
+// GaussianBeam set-up (updateSampler's attenuation along the beam and getMaterialInfo of the focal
+// points, GaussianBeam.cuh:1281-1613): per point the material there and exp(-tau) toward -dirs.
+__global__ void __launch_bounds__(256) k_gauss_setup(MediumView m, const double* pts, const double* dirs, uint32_t n,
+                                                     double* att, uint32_t* mat) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const d3 p = mkd(pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]);
+    const d3 w = mkd(-dirs[3 * (size_t)i], -dirs[3 * (size_t)i + 1], -dirs[3 * (size_t)i + 2]);
+    mat[i] = gbMaterial(m, p);
+    att[i] = exp(-opticalDepth<3, double, d3>(m, m.sigtd, p, w, 1e300, nullptr));
+}
+
+hipError_t launchGaussSetup(const MediumView& m, const double* pts, const double* dirs, uint32_t n, double* att,
+                            uint32_t* mat, hipStream_t st) {
+    hipLaunchKernelGGL(k_gauss_setup, dim3((n + 255) / 256), dim3(256), 0, st, m, pts, dirs, n, att, mat);
     return hipGetLastError();
 }
 

@@ -109,3 +109,23 @@ def test_correlation_sampler_config_checks(native):
         config.CConfig({k: v for k, v in cs.items() if k != "view2"}, 2)
     with pytest.raises(ValueError, match="Available sampler types are TemporalSampler and TemporalCorrelationSampler"):
         config.CConfig(dict(base, sampler=dict(base["sampler"], type="Other")), 2)
+
+
+def test_gaussian_beam_config_checks(native):
+    """GaussianBeam (MatlabInterface.cu:1153-1190): 3D, a direction per focal point, aperture > 0;
+    built for GaussianBeam illumination and view together, vMF mixtures of at most 16 lobes, field
+    output of the TemporalSampler without coherent backscattering."""
+    cfg = W.confocal(focal_x=[0.0, 2.0], focal_z=[8.0], tau_x=[0.0], batch=64, full_iteration=1)
+    assert native.sstmc_validate(config.CConfig(cfg, 3).ptr()) == 0
+    cc = config.CConfig(cfg, 3)
+    cc.struct.illumination.aperture = 0.0
+    assert native.sstmc_validate(cc.ptr()) == 7
+    ff = dict(cfg, view={"type": "FarField", "direction": {"x": [0.0], "y": [0.0], "z": [1.0]}})
+    assert native.sstmc_validate(config.CConfig(ff, 3).ptr()) == 5
+    corr = dict(cfg, tracer={"type": "nee", "output": "correlation"})
+    assert native.sstmc_validate(config.CConfig(corr, 3).ptr()) == 5
+    hg = dict(cfg, scattering={"type": ["HG"] * 5, "amplitudeFunction": [0.8] * 5})
+    assert native.sstmc_validate(config.CConfig(hg, 3).ptr()) == 5
+    flat = dict(cfg, medium=dict(cfg["medium"], materialGrid=np.array([[1, 2]], np.uint32)))
+    with pytest.raises(ValueError, match="GaussianBeam sources are supported in 3D"):
+        config.CConfig(flat, 2)
