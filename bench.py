@@ -51,6 +51,10 @@ def workload(name: str, batch: int, steps: int):
     if name == "acorr128":
         cfg = W.c3_analytic(paths=batch * steps, batch=batch)
         return cfg, 3
+    if name == "confocal":
+        # the confocal example's geometry: 1111 GaussianBeam illuminations, 3333 GaussianBeam views
+        cfg = W.confocal(batch=batch, full_iteration=steps)
+        return cfg, 3
     raise SystemExit("unknown workload %s" % name)
 
 

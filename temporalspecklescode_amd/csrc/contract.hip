@@ -532,9 +532,133 @@ __global__ void __launch_bounds__(256) k_single_gauss(MediumView m, ScatterView 
     }
 }
 
-hipError_t launchSingleGauss(int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
-                             const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
-                             const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st) {
+// The same sum when every view beam has one direction d_v (the confocal geometry): then the inner
+// convolution of fsGaussianBeam (its factor gob, the axis w, C and the exponent c) depends on the
+// illumination, the lobe and the path only. Per path, the 16 x M (illumination, lobe) records of the
+// block are formed once in LDS (one per thread) with the real parts and unit phases of r + C and
+// r - C, r = c + log nu; per (l, v, lobe) there remain sm = sqrt((gob aT + bT).(gob aT + bT)), one
+// sine-cosine pair of Im sm, two real exponentials exp(Re(r +- C) +- Re sm) (each exponent whole, as
+// the reference forms it, so that no factor overflows alone) and the division by C sm. A term is dropped when the real part of its exponent is at most -10,
+// as complex2timesSinh drops it; the rare zero cases of C or sm take the reference's special forms.
+template <typename R>
+__global__ void __launch_bounds__(256) k_single_gauss_uv(MediumView m, ScatterView sc, SourceView il, SourceView vw,
+                                                         FramesView fr, VertexBatch vb,
+                                                         const typename SingleT<R>::C* __restrict__ a,
+                                                         const typename SingleT<R>::C* __restrict__ b, uint32_t P,
+                                                         uint32_t pathsPerSplit, uint32_t splits,
+                                                         typename SingleT<R>::C* __restrict__ part) {
+    typedef typename SingleT<R>::C C;
+    __shared__ double2 sGob[kGaussLobes][16], sC[kGaussLobes][16], sUp[kGaussLobes][16], sUm[kGaussLobes][16],
+        sR[kGaussLobes][16];
+    __shared__ double sReP[kGaussLobes][16], sReM[kGaussLobes][16], sNu[kGaussLobes];
+    __shared__ double sx[3];
+    __shared__ uint32_t sm1;
+    const uint32_t L = il.n, V = vw.n;
+    const uint32_t tl = threadIdx.x & 15, tq = threadIdx.x >> 4;
+    const uint32_t split = blockIdx.z % splits, t = blockIdx.z / splits;
+    const uint32_t l = blockIdx.x * 16 + tl, v = blockIdx.y * 16 + tq;
+    const uint32_t pBegin = split * pathsPerSplit, pEnd = min(P, pBegin + pathsPerSplit);
+    const PhaseTables& tab = *sc.tab;
+    const GaussBeams& ga = il.g;
+    const GaussBeams& gb = vw.g;
+    const double k = ga.waveK;
+    const d3 db = gbDir(gb, 0);
+    const double tc = ga.apNorm + gb.apNorm + 2.0 * 1.8378770664093454836;
+    const bool poolOk = (uint64_t)vb.off[P] <= vb.cap;
+    double re = 0, im = 0;
+    for (uint32_t p = pBegin; p < pEnd; ++p) {
+        if (!poolOk || vb.k[p] == 0) continue;   // uniform over the block
+        const uint64_t o = vb.off[p];
+        if (threadIdx.x == 0) {
+            const float4 xm = vb.xm[o];
+            d3 x1 = mkd(xm.x, xm.y, xm.z);
+            if (fr.temporalAmp && !fr.trivial) {
+                const float* d = vb.delta + (o * fr.nt + t) * 3;
+                x1 = mkd(x1.x + (double)d[0], x1.y + (double)d[1], x1.z + (double)d[2]);
+            }
+            sx[0] = x1.x; sx[1] = x1.y; sx[2] = x1.z;
+            sm1 = gbMaterial(m, x1);
+        }
+        __syncthreads();
+        const uint32_t mat = sm1;
+        const d3 x1 = mkd(sx[0], sx[1], sx[2]);
+        const uint32_t lo = mat ? tab.vmfOff[mat] : 0u, M = mat ? tab.vmfM[mat] : 0u;
+        // (illumination tl, lobe tq) record
+        if (tq < M && l < L) {
+            const CV aT = CV{scale3(ga.gammaA, gbDir(ga, l)), scale3(k, sub3(x1, gbFocal(ga, l)))};
+            const double gs0 = tab.vmfKappa[lo + tq];
+            const double lognu = tc + tab.vmfC[lo + tq];
+            Cd gob = cd(0, 0), Cc, c = cd(0, 0);
+            if (fabs(gs0) < 0.000001) {
+                Cc = csqrtd(sumSq(aT));
+            } else {
+                gob = gs0 * rcsqrtd(sumSq(cfmaR(gs0, db, aT)));
+                const d3 w = realMultV(gob, aT);
+                const double ra = 1.0 / sqrt(dot3(w, w));
+                Cc = csqrtd(sumSq(cfmaR(gs0 * ra, w, aT)));
+                c = (-ra) * (gob * dotCV(aT, w));
+            }
+            const Cd r = c + cd(lognu);
+            sGob[tq][tl] = make_double2(gob.re, gob.im);
+            sC[tq][tl] = make_double2(Cc.re, Cc.im);
+            sR[tq][tl] = make_double2(r.re, r.im);
+            const Cd rp = r + Cc, rm = r - Cc;
+            double sn, cs;
+            sincos(rp.im, &sn, &cs);
+            sUp[tq][tl] = make_double2(cs, sn);
+            sincos(rm.im, &sn, &cs);
+            sUm[tq][tl] = make_double2(cs, sn);
+            sReP[tq][tl] = rp.re;
+            sReM[tq][tl] = rm.re;
+            if (tl == 0) sNu[tq] = lognu;
+        }
+        __syncthreads();
+        if (l < L && v < V && M > 0) {
+            const CV aT = CV{scale3(ga.gammaA, gbDir(ga, l)), scale3(k, sub3(x1, gbFocal(ga, l)))};
+            const CV bT = CV{scale3(gb.gammaA, gbDir(gb, v)), scale3(k, sub3(gbFocal(gb, v), x1))};
+            Cd fs = cd(0, 0);
+            for (uint32_t j = 0; j < M; ++j) {
+                const double gs0 = tab.vmfKappa[lo + j];
+                Cd sm;
+                if (fabs(gs0) < 0.000001) sm = csqrtd(sumSq(bT));
+                else sm = csqrtd(sumSq(cfmaC(cd(sGob[j][tl].x, sGob[j][tl].y), aT, bT)));
+                const Cd Cc = cd(sC[j][tl].x, sC[j][tl].y);
+                const bool smZero = fabs(sm.re) <= 0.0001 && fabs(sm.im) <= 0.0001;
+                const bool cZero = fabs(Cc.re) <= 0.0001 && fabs(Cc.im) <= 0.0001;
+                if (!smZero && !cZero) {
+                    // exp(r + C + sm) - exp(r - C - sm), each dropped at real part <= -10
+                    double sn, cs;
+                    sincos(sm.im, &sn, &cs);
+                    Cd num = cd(0, 0);
+                    const double ep = sReP[j][tl] + sm.re, em = sReM[j][tl] - sm.re;
+                    if (ep > -10.0) num = exp(ep) * (cd(sUp[j][tl].x, sUp[j][tl].y) * cd(cs, sn));
+                    if (em > -10.0) num = num - exp(em) * (cd(sUm[j][tl].x, sUm[j][tl].y) * cd(cs, -sn));
+                    fs = fs + cdiv(num, Cc * sm);
+                } else {
+                    const Cd r = cd(sR[j][tl].x, sR[j][tl].y);
+                    if (smZero && cZero) fs = fs + 4.0 * cexpd(r);
+                    else if (smZero) fs = fs + cdiv(2.0 * twoSinhd(Cc, r), Cc);
+                    else fs = fs + cdiv(2.0 * twoSinhd(sm, cd(sNu[j])), sm);
+                }
+            }
+            const C x = a[((size_t)t * P + p) * L + l], y = b[((size_t)t * P + p) * V + v];
+            const double pr = (double)x.x * y.x - (double)x.y * y.y, pi = (double)x.x * y.y + (double)x.y * y.x;
+            re += fs.re * pr - fs.im * pi;
+            im += fs.re * pi + fs.im * pr;
+        }
+        __syncthreads();
+    }
+    if (l < L && v < V) {
+        C c;
+        c.x = (R)re;
+        c.y = (R)im;
+        part[((size_t)blockIdx.z) * L * V + (size_t)v * L + l] = c;
+    }
+}
+
+hipError_t launchSingleGauss(int fp64, int uniformViewDir, const MediumView& m, const ScatterView& sc,
+                             const SourceView& il, const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
+                             const void* a, const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st) {
     const uint32_t L = il.n, V = vw.n, NT = fr.nt;
     uint32_t tiles = ((L + 15) / 16) * ((V + 15) / 16);
     uint32_t splits = chooseSplits(tiles, P, 64);
@@ -557,14 +681,20 @@ hipError_t launchSingleGauss(int fp64, const MediumView& m, const ScatterView& s
     dim3 grid((L + 15) / 16, (V + 15) / 16, splits * NT);
     const uint32_t LV = L * V;
     if (fp64) {
-        hipLaunchKernelGGL(k_single_gauss<double>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb, (const double2*)a,
+        if (uniformViewDir)
+            hipLaunchKernelGGL(k_single_gauss_uv<double>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb,
+                               (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr);
+        else hipLaunchKernelGGL(k_single_gauss<double>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb, (const double2*)a,
                            (const double2*)b, P, pps, splits, (double2*)ws->ptr);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<double2>, dim3((LV + 255) / 256), dim3(256), 0, st,
                                (const double2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,
                                (const double2*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
     } else {
-        hipLaunchKernelGGL(k_single_gauss<float>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb, (const float2*)a,
+        if (uniformViewDir)
+            hipLaunchKernelGGL(k_single_gauss_uv<float>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb,
+                               (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr);
+        else hipLaunchKernelGGL(k_single_gauss<float>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb, (const float2*)a,
                            (const float2*)b, P, pps, splits, (float2*)ws->ptr);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st,

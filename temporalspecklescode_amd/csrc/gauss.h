@@ -24,7 +24,9 @@ __host__ __device__ inline Cd cdiv(Cd a, Cd b) {
 __host__ __device__ inline double cabsd(Cd a) { return sqrt(a.re * a.re + a.im * a.im); }
 __host__ __device__ inline Cd cexpd(Cd a) {
     const double e = exp(a.re);
-    return cd(e * cos(a.im), e * sin(a.im));
+    double sn, cs;
+    sincos(a.im, &sn, &cs);
+    return cd(e * cs, e * sn);
 }
 // principal square root (ComplexType.cuh:116-131) and its reciprocal
 __host__ __device__ inline Cd csqrtd(Cd a) {

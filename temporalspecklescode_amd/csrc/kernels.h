@@ -89,10 +89,11 @@ hipError_t launchSingleCorrelation(int dims, int fp64, const ScatterView& sc, co
 hipError_t launchGaussSetup(const MediumView& m, const double* pts, const double* dirs, uint32_t n, double* att,
                             uint32_t* mat, hipStream_t st);
 // Single scattering between GaussianBeam illuminations and views (field output):
-// U[t][v][l] += sum_p a[t][p][l] b[t][p][v] fs(l, v, x1).
-hipError_t launchSingleGauss(int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
-                             const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
-                             const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st);
+// U[t][v][l] += sum_p a[t][p][l] b[t][p][v] fs(l, v, x1). uniformViewDir: every view beam has the
+// direction of view 0 (the per-illumination part of fs is then shared by the views).
+hipError_t launchSingleGauss(int fp64, int uniformViewDir, const MediumView& m, const ScatterView& sc,
+                             const SourceView& il, const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
+                             const void* a, const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st);
 
 // U += 0 helpers
 hipError_t launchZeroU(double* U, size_t n, hipStream_t st);

@@ -209,14 +209,13 @@ __device__ inline void gGauss(const MediumView& m, const R* sig, const d3& dir, 
     *amp = gbMaterial(m, x.pd) ? expR<R>(-(R)0.5 * tau) : (R)0;
     *ph = 0;
 }
-template <int DIMS, typename R>
+template <int DIMS, typename R, bool GB = false>
 __device__ inline void gTerm(const MediumView& m, const R* sig, const SourceView& s, uint32_t i, const Geo& x,
                              double invLambda, R tauCut, uint32_t* steps, R* amp, double* ph) {
-    if constexpr (DIMS == 3)
-        if (s.type == 3) {
-            gGauss<DIMS, R>(m, sig, gbDir(s.g, i), s.illum != 0, x, tauCut, steps, amp, ph);
-            return;
-        }
+    if constexpr (GB && DIMS == 3) {
+        gGauss<DIMS, R>(m, sig, gbDir(s.g, i), s.illum != 0, x, tauCut, steps, amp, ph);
+        return;
+    }
     if (s.type == 2)
         gFar<DIMS, R>(m, sig, sourceD<DIMS>(s, i), sizeof(R) == 4 ? sourceF(s, i) : mk(0.f, 0.f, 0.f), s.illum != 0,
                       x, invLambda, tauCut, steps, amp, ph);
@@ -265,7 +264,7 @@ __device__ inline typename Cx<R>::T cis(R amp, double phaseCycles) {
 
 // First-vertex work of one path, on a group of lpp lanes: path probabilities, illumination terms
 // a and A, single-scattering view terms b (and zero B rows when the path has one vertex).
-template <int DIMS, typename R, bool TAB>
+template <int DIMS, typename R, bool TAB, bool GB = false>
 __device__ inline void nee_path(const MediumView& m, const R* sig, const ScatterView& sc, const SourceView& il,
                                 const SourceView& vw,
                                 const FramesView& fr, const VertexBatch& vb, const NeeParams& prm, const NeeOut& out,
@@ -315,7 +314,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
 
     // ---- path probabilities from the central first vertex (pathProbabilitySingle/Multiple)
     R s1 = 0, s2 = 0;
-    const bool gauss = TAB && DIMS == 3 && il.type == 3;
+    constexpr bool gauss = GB && TAB && DIMS == 3;   // GaussianBeam illumination and view
     const uint32_t m1g = gauss ? gbMaterial(m, x1c.pd) : 0u;
     for (uint32_t l = sub; l < L; l += lpp) {
         R pr;
@@ -370,7 +369,8 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         for (int f = 0; f < nf; ++f) {
             uint32_t tt = (corr && f == 0) ? 0 : t;
             Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
-            gTerm<DIMS, R>(m, sig, il, l, x1t, invLambda, il.type != 1 ? tauCutFF : tauInf, &steps, &ga[f], &gp[f]);
+            gTerm<DIMS, R, gauss>(m, sig, il, l, x1t, invLambda, il.type != 1 ? tauCutFF : tauInf, &steps, &ga[f],
+                                  &gp[f]);
             if (multi) {
                 Geo x2t = vertexFrame(vb, fr, off + 1, tt, x2m);
                 const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
@@ -417,7 +417,8 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         for (int f = 0; f < nf; ++f) {
             uint32_t tt = (corr && f == 0) ? 0 : t;
             Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
-            gTerm<DIMS, R>(m, sig, vw, v, x1t, invLambda, vw.type != 1 ? tauCutFF : tauInf, &steps, &ga[f], &gp[f]);
+            gTerm<DIMS, R, gauss>(m, sig, vw, v, x1t, invLambda, vw.type != 1 ? tauCutFF : tauInf, &steps, &ga[f],
+                                  &gp[f]);
         }
         const size_t o = ((size_t)t * P + p) * V + v;
         b[o] = corr ? cis<R>(ga[0] * ga[1], gp[0] - gp[1]) : cis<R>(ga[0], gp[0]);
@@ -439,7 +440,8 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
     *stepsOut += steps;
 }
 
-template <int DIMS, typename R, bool TAB>
+// GB: GaussianBeam sources (their FP64 beam functions kept out of the other instantiations)
+template <int DIMS, typename R, bool TAB, bool GB>
 __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, SourceView il, SourceView vw, FramesView fr,
                                              VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
@@ -450,7 +452,7 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
     __shared__ R sigLds[kSigLds];
     const R* sig = loadSigTable<R>(m, sigLds);
     uint32_t steps = 0;
-    if (p < prm.npaths) nee_path<DIMS, R, TAB>(m, sig, sc, il, vw, fr, vb, prm, out, p, sub, lpp, &steps);
+    if (p < prm.npaths) nee_path<DIMS, R, TAB, GB>(m, sig, sc, il, vw, fr, vb, prm, out, p, sub, lpp, &steps);
     uint32_t tot = waveSum(steps);
     if (lane == 0 && out.steps) atomicAdd(out.steps, (unsigned long long)tot);
 }
@@ -741,7 +743,7 @@ __device__ inline ViewUnit viewUnit(const NeeParams& prm) {
 // vertex loop at 64), 4 for FP64 (no spills; 1.6x faster than 6 waves with spills).
 template <int DIMS, typename R, int SRC, bool WAVE, bool TAB>
 __global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 || SRC == kSrcGauss ? 4 : (SRC != kSrcPoint ? 8 : 6))))
+__attribute__((amdgpu_waves_per_eu(SRC == kSrcGauss ? 2 : (sizeof(R) == 8 ? 4 : (SRC != kSrcPoint ? 8 : 6)))))
 k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBatch vb, NeeParams prm, NeeOut out) {
     typedef typename Cx<R>::T C;
     __shared__ R sigLds[kSigLds];
@@ -1026,8 +1028,10 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
         else launchViewCorrSampler<DIMS, R, kSrcPoint>(m, sc, vw, fr, vb, p, out, st);
         return;
     }
-    if (sc.tab) hipLaunchKernelGGL((k_nee<DIMS, R, true>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
-    else hipLaunchKernelGGL((k_nee<DIMS, R, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    if (DIMS == 3 && il.type == 3 && sc.tab)
+        hipLaunchKernelGGL((k_nee<DIMS, R, true, DIMS == 3>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    else if (sc.tab) hipLaunchKernelGGL((k_nee<DIMS, R, true, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    else hipLaunchKernelGGL((k_nee<DIMS, R, false, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
     hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, vb, fr, p.npaths,
                        1.0 / p.lambda);
     if (between) hipEventRecord(between, st);

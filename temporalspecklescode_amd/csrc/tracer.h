@@ -160,8 +160,10 @@ class GaussianBeam : public Source {
 public:
     GaussianBeam(int* err, const Medium& medium, const Scattering& scattering, const sstmc_source& s,
                  const sstmc_source* views, double lambda, bool illumination);
+    const std::vector<double>& directions() const { return dirs_; }   // beam directions, n x 3 (host)
 
 private:
+    std::vector<double> dirs_;
     DevBuf<double> gd_;
     DevBuf<uint32_t> gu_;
 };

@@ -468,6 +468,7 @@ GaussianBeam::GaussianBeam(int* err, const Medium& medium, const Scattering& sca
         dir[3 * i] = s.dirX[i]; dir[3 * i + 1] = s.dirY[i]; dir[3 * i + 2] = s.dirZ[i];
         for (int a = 0; a < 3; ++a) pf[3 * i + a] = (float)pts_[3 * i + a];
     }
+    dirs_ = dir;
     if ((*err = pd_.upload(pts_.data(), pts_.size()))) return;
     if ((*err = pf_.upload(pf.data(), pf.size()))) return;
     v_.pf = pf_.p;
@@ -862,7 +863,10 @@ int NextEventEstimation::contractSingle(uint32_t npaths, double* U) {
         he = launchContract(contractKind_, a_.p, b_.p, npaths, L_, V_, T_, Ff_.p, Fd_.p,
                             opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0, fscalar_, U, &wsSingle_, stream_);
     } else if (singleMode_ == kSingleGauss) {
-        he = launchSingleGauss(fp64_ ? 1 : 0, mview_, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p, npaths,
+        const std::vector<double>& vd = static_cast<const GaussianBeam&>(vw_).directions();
+        bool uniformDir = true;
+        for (size_t i = 3; i < vd.size() && uniformDir; ++i) uniformDir = vd[i] == vd[i % 3];
+        he = launchSingleGauss(fp64_ ? 1 : 0, uniformDir ? 1 : 0, mview_, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p, npaths,
                                U, &wsSingle_, stream_);
     } else if (singleMode_ == kSingleCorrelation) {
         const bool uniformFar = il_.type() == SSTMC_SOURCE_FARFIELD && vw_.type() == SSTMC_SOURCE_FARFIELD &&

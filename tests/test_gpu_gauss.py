@@ -27,14 +27,26 @@ def _small(batch=128, frames=False, seed=1, shift=1.0):
     return cfg
 
 
-@pytest.mark.parametrize("frames", [False, True])
-def test_fp64_terms_match_oracle(frames):
+def _tilted(cfg, seed=7):
+    """View beams along individually tilted unit directions (the general fs kernel; with one shared
+    view direction the per-illumination part of fs is formed once per path and shared)."""
+    n = len(cfg["view"]["focalPoint"]["x"])
+    d = np.random.default_rng(seed).normal(size=(n, 3)) * [0.15, 0.15, 0.0] + [0.0, 0.0, 1.0]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    cfg["view"]["focalDirection"] = {"x": d[:, 0], "y": d[:, 1], "z": d[:, 2]}
+    return cfg
+
+
+@pytest.mark.parametrize("frames,tilted", [(False, False), (True, False), (False, True)])
+def test_fp64_terms_match_oracle(frames, tilted):
     cfg = _small(128, frames)
+    if tilted:
+        cfg = _tilted(cfg)
     cfg["precision"] = {"terms": "fp64", "contraction": "fp64"}
     n = cfg["simulation"]["batchSize"]
     with Tracer(cfg, 3) as tr:
         d = tr.debug_batch(0, n)
-    print("frames", frames, "vertices", int(d["off"][-1]), flush=True)
+    print("frames", frames, "tilted", tilted, "vertices", int(d["off"][-1]), flush=True)
     A, B, S, U = _oracle_u(cfg, 3, d, d["a"], d["b"])
     assert np.linalg.norm(A) > 0 and np.linalg.norm(B) > 0
     assert _rel(d["a"], A) < 1e-9
@@ -42,9 +54,12 @@ def test_fp64_terms_match_oracle(frames):
     assert _rel(d["u"], U) < 1e-9
 
 
-def test_fp32_terms():
+@pytest.mark.parametrize("tilted", [False, True])
+def test_fp32_terms(tilted):
     """FP32 walks and products over FP64 beam functions: batch field within 1e-4."""
     cfg = _small(128)
+    if tilted:
+        cfg = _tilted(cfg)
     cfg["precision"] = {"terms": "fp32", "contraction": "fp32"}
     n = cfg["simulation"]["batchSize"]
     with Tracer(cfg, 3) as tr:
@@ -80,3 +95,16 @@ def test_render_gating_and_iter():
     same = np.abs([u[i, i] for i in range(L)])
     shifted = np.abs([u[i, L + i] for i in range(L)])
     assert same.mean() > 3 * shifted.mean()
+
+
+def test_wide_grid_fp64_matches_oracle():
+    """Beams up to 100 wavelengths from the first vertex: large imaginary throughputs, whose shared
+    fs factors must keep each exponent whole (finite field, FP64 parity)."""
+    cfg = W.confocal(focal_x=np.array([-50.0, -25.0, 0.0, 25.0, 50.0]), focal_z=np.array([2.0, 10.0, 18.0]),
+                     tau_x=(-1.0, 0.0, 1.0), batch=512, full_iteration=1, seed=11)
+    cfg["precision"] = {"terms": "fp64", "contraction": "fp64"}
+    with Tracer(cfg, 3) as tr:
+        d = tr.debug_batch(0, 512)
+    assert np.isfinite(d["u"]).all()
+    A, B, S, U = _oracle_u(cfg, 3, d, d["a"], d["b"])
+    assert _rel(d["u"], U) < 1e-9
