@@ -34,6 +34,10 @@ def test_check_valid_configs(tmp_path):
     r = U.run(c2, tmp_path, "--check")           # dims from the presence of zAxis
     assert r.returncode == 0, r.stderr
     assert json.loads(r.stdout)["shape"] == [64, 64, 1, 1] and json.loads(r.stdout)["dims"] == 3
+    gb = W.confocal(focal_x=[-2.0, 0.0, 2.0], focal_z=[4.0, 12.0], tau_x=(0.0, 1.0), batch=64, full_iteration=1)
+    r = U.run(gb, tmp_path, "--check")           # GaussianBeam illumination and view
+    assert r.returncode == 0, r.stderr
+    assert json.loads(r.stdout)["shape"] == [6, 12, 1, 1]
 
 
 def test_nested_flat_and_file_grids_agree(tmp_path):
