@@ -337,9 +337,9 @@ __global__ void __launch_bounds__(256) k_single_generic(ScatterView sc, SourceVi
                                                         VertexBatch vb, const typename SingleT<R>::C* __restrict__ a,
                                                         const typename SingleT<R>::C* __restrict__ b, uint32_t P,
                                                         uint32_t pathsPerSplit, uint32_t splits,
-                                                        typename SingleT<R>::C* __restrict__ part) {
+                                                        typename SingleT<R>::C* __restrict__ part, int corr) {
     typedef typename SingleT<R>::C C;
-    __shared__ R su[16][16][3], sw[16][16][3];
+    __shared__ R su[16][16][3], sw[16][16][3], su0[16][16][3], sw0[16][16][3];
     __shared__ C sa[16][16], sb[16][16];
     __shared__ uint32_t sm[16];
     const uint32_t L = il.n, V = vw.n;
@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(256) k_single_generic(ScatterView sc, SourceVi
         const uint32_t ls = blockIdx.x * 16 + tl, vs = blockIdx.y * 16 + tl;
         C av, bv;
         av.x = av.y = 0; bv.x = bv.y = 0;
-        double x1[3] = {0, 0, 0};
+        double x1[3] = {0, 0, 0}, x10[3] = {0, 0, 0};
         uint32_t m1 = 0;
         bool ok = p < pEnd && vb.k[p] > 0 && (uint64_t)vb.off[P] <= vb.cap;
         if (ok) {
@@ -362,34 +362,45 @@ __global__ void __launch_bounds__(256) k_single_generic(ScatterView sc, SourceVi
             const float4 xm = vb.xm[o];
             m1 = __float_as_uint(xm.w);
             x1[0] = xm.x; x1[1] = xm.y; x1[2] = xm.z;
+            x10[0] = x1[0]; x10[1] = x1[1]; x10[2] = x1[2];
             if (fr.temporalAmp && !fr.trivial) {
                 const float* d = vb.delta + (o * fr.nt + t) * 3;
+                const float* d0 = vb.delta + o * fr.nt * 3;
                 x1[0] += (double)d[0]; x1[1] += (double)d[1]; x1[2] += (double)d[2];
+                x10[0] += (double)d0[0]; x10[1] += (double)d0[1]; x10[2] += (double)d0[2];
             }
             if (ls < L) av = a[((size_t)t * P + p) * L + ls];
             if (vs < V) bv = b[((size_t)t * P + p) * V + vs];
         }
-        // incoming direction of illumination ls at x1, outgoing direction toward view vs
-        double u[3] = {0, 0, 1}, w[3] = {0, 0, 1};
-        if (ls < L) {
-            if (il.type == 2) { u[0] = il.pd[3 * ls]; u[1] = il.pd[3 * ls + 1]; u[2] = il.pd[3 * ls + 2]; }
-            else {
-                double q[3] = {x1[0] - il.pd[3 * ls], x1[1] - il.pd[3 * ls + 1], DIMS == 3 ? x1[2] - il.pd[3 * ls + 2] : 0.0};
-                double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
-                if (n > 0) { u[0] = q[0] / n; u[1] = q[1] / n; u[2] = q[2] / n; }
+        // incoming direction of illumination ls at x1, outgoing direction toward view vs (frame t;
+        // with the correlation output also at frame 0)
+        for (int f = 0; f < (corr ? 2 : 1); ++f) {
+            const double* xp = f == 0 ? x1 : x10;
+            double u[3] = {0, 0, 1}, w[3] = {0, 0, 1};
+            if (ls < L) {
+                if (il.type == 2) { u[0] = il.pd[3 * ls]; u[1] = il.pd[3 * ls + 1]; u[2] = il.pd[3 * ls + 2]; }
+                else {
+                    double q[3] = {xp[0] - il.pd[3 * ls], xp[1] - il.pd[3 * ls + 1],
+                                   DIMS == 3 ? xp[2] - il.pd[3 * ls + 2] : 0.0};
+                    double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+                    if (n > 0) { u[0] = q[0] / n; u[1] = q[1] / n; u[2] = q[2] / n; }
+                }
             }
-        }
-        if (vs < V) {
-            if (vw.type == 2) { w[0] = vw.pd[3 * vs]; w[1] = vw.pd[3 * vs + 1]; w[2] = vw.pd[3 * vs + 2]; }
-            else {
-                double q[3] = {vw.pd[3 * vs] - x1[0], vw.pd[3 * vs + 1] - x1[1], DIMS == 3 ? vw.pd[3 * vs + 2] - x1[2] : 0.0};
-                double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
-                if (n > 0) { w[0] = q[0] / n; w[1] = q[1] / n; w[2] = q[2] / n; }
+            if (vs < V) {
+                if (vw.type == 2) { w[0] = vw.pd[3 * vs]; w[1] = vw.pd[3 * vs + 1]; w[2] = vw.pd[3 * vs + 2]; }
+                else {
+                    double q[3] = {vw.pd[3 * vs] - xp[0], vw.pd[3 * vs + 1] - xp[1],
+                                   DIMS == 3 ? vw.pd[3 * vs + 2] - xp[2] : 0.0};
+                    double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+                    if (n > 0) { w[0] = q[0] / n; w[1] = q[1] / n; w[2] = q[2] / n; }
+                }
             }
+            if (DIMS == 2) { u[2] = 0; w[2] = 0; }
+            R(*du)[3] = f == 0 ? su[tq] : su0[tq];
+            R(*dw)[3] = f == 0 ? sw[tq] : sw0[tq];
+            du[tl][0] = (R)u[0]; du[tl][1] = (R)u[1]; du[tl][2] = (R)u[2];
+            dw[tl][0] = (R)w[0]; dw[tl][1] = (R)w[1]; dw[tl][2] = (R)w[2];
         }
-        if (DIMS == 2) { u[2] = 0; w[2] = 0; }
-        su[tq][tl][0] = (R)u[0]; su[tq][tl][1] = (R)u[1]; su[tq][tl][2] = (R)u[2];
-        sw[tq][tl][0] = (R)w[0]; sw[tq][tl][1] = (R)w[1]; sw[tq][tl][2] = (R)w[2];
         sa[tq][tl] = av;
         sb[tq][tl] = bv;
         if (tl == 0) sm[tq] = m1;
@@ -400,6 +411,15 @@ __global__ void __launch_bounds__(256) k_single_generic(ScatterView sc, SourceVi
             C f;
             if (iso) { f.x = DIMS == 3 ? (R)0.282094791773878143 : (R)0.398942280401432678; f.y = 0; }
             else f = singleAmp<DIMS, R>(sc, sm[q], c);
+            if (corr) {   // S(0) conj S(t): the amplitude at frame 0 times the conjugate at frame t
+                const R c0 = su0[q][tl][0] * sw0[q][tq][0] + su0[q][tl][1] * sw0[q][tq][1] + su0[q][tl][2] * sw0[q][tq][2];
+                C f0;
+                if (iso) { f0.x = f.x; f0.y = 0; }
+                else f0 = singleAmp<DIMS, R>(sc, sm[q], c0);
+                const R fr0 = f0.x * f.x + f0.y * f.y, fi0 = f0.y * f.x - f0.x * f.y;
+                f.x = fr0;
+                f.y = fi0;
+            }
             R pr = x.x * y.x - x.y * y.y, pi = x.x * y.y + x.y * y.x;
             re = fma(f.x, pr, fma(-f.y, pi, re));
             im = fma(f.x, pi, fma(f.y, pr, im));
@@ -416,7 +436,7 @@ __global__ void __launch_bounds__(256) k_single_generic(ScatterView sc, SourceVi
 
 hipError_t launchSingleGeneric(int dims, int fp64, const ScatterView& sc, const SourceView& il,
                                const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
-                               const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st) {
+                               const void* b, uint32_t P, int corr, double* U, Workspace* ws, hipStream_t st) {
     const uint32_t L = il.n, V = vw.n, NT = fr.nt;
     uint32_t tiles = ((L + 15) / 16) * ((V + 15) / 16);
     uint32_t splits = chooseSplits(tiles, P, 256);
@@ -441,10 +461,10 @@ hipError_t launchSingleGeneric(int dims, int fp64, const ScatterView& sc, const 
     if (fp64) {
         if (dims == 2)
             hipLaunchKernelGGL((k_single_generic<2, double>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
-                               (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr);
+                               (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr, corr);
         else
             hipLaunchKernelGGL((k_single_generic<3, double>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
-                               (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr);
+                               (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr, corr);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<double2>, dim3((LV + 255) / 256), dim3(256), 0, st,
                                (const double2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,
@@ -452,10 +472,10 @@ hipError_t launchSingleGeneric(int dims, int fp64, const ScatterView& sc, const 
     } else {
         if (dims == 2)
             hipLaunchKernelGGL((k_single_generic<2, float>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
-                               (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr);
+                               (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr, corr);
         else
             hipLaunchKernelGGL((k_single_generic<3, float>), grid, dim3(256), 0, st, sc, il, vw, fr, vb,
-                               (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr);
+                               (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr, corr);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st,
                                (const float2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,

@@ -71,10 +71,11 @@ hipError_t launchContract(int kind, const void* A, const void* B, uint32_t P, ui
                           const float2* F, const double2* Fd, int fsq, double fscalar, double* U, Workspace* ws,
                           hipStream_t st);
 
-// Generic single scattering (field output): U[t][v][l] += sum_p a[t][p][l] b[t][p][v] f_p(l, v, t).
+// Generic single scattering: U[t][v][l] += sum_p a[t][p][l] b[t][p][v] f_p(l, v, t) (field output),
+// or with corr f_p(l, v, 0) conj f_p(l, v, t) (correlation output, a and b holding the lag products).
 hipError_t launchSingleGeneric(int dims, int fp64, const ScatterView& sc, const SourceView& il,
                                const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const void* a,
-                               const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st);
+                               const void* b, uint32_t P, int corr, double* U, Workspace* ws, hipStream_t st);
 
 // Single scattering of the TemporalCorrelationSampler (sources il / vw hold u1 then u2):
 // U[t][v][l] += sum_p a[t][p][l] b[t][p][v] Fs_u1 conj(Fs_u2) exp(-k^2 |D tau_t| |q|^2).

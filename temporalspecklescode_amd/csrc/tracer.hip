@@ -788,7 +788,8 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
         const std::vector<double>& hv = vw_.hostPoints();
         sameViews_ = std::equal(hv.begin(), hv.begin() + 3 * (size_t)V_, hv.begin() + 3 * (size_t)V_);
     } else if (opt_.output == SSTMC_OUTPUT_CORRELATION) {
-        if (singleMode_ != kSingleSeparable) { *err = SSTMC_NOT_SUPPORTED; return; }
+        // S(0) conj S(t): a scalar or fixed F enters squared, a per-path f as f(0) conj f(t)
+        if (singleMode_ != kSingleSeparable && singleMode_ != kSingleGeneric) { *err = SSTMC_NOT_SUPPORTED; return; }
         fscalar_ *= fscalar_;
     }
     if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) { *err = SSTMC_DEVICE_ERROR; return; }
@@ -874,7 +875,8 @@ int NextEventEstimation::contractSingle(uint32_t npaths, double* U) {
         he = launchSingleCorrelation(dims_, fp64_ ? 1 : 0, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p,
                                      npaths, opt_.lambda, uniformFar ? 1 : 0, U, &wsSingle_, stream_);
     } else {
-        he = launchSingleGeneric(dims_, fp64_ ? 1 : 0, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p, npaths, U,
+        he = launchSingleGeneric(dims_, fp64_ ? 1 : 0, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p, npaths,
+                                 opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0, U,
                                  &wsSingle_, stream_);
     }
     return he == hipSuccess ? SSTMC_NO_ERROR : SSTMC_KERNEL_ERROR;

@@ -162,3 +162,25 @@ def test_render_deterministic_and_iter():
     # the oracle on the same counter-based paths: same miss count
     uo, ito, _ = O.render(cfg, 2)
     assert abs(ito - it1) <= 0.01 * 4096
+
+
+@pytest.mark.parametrize("case", ["moving_points_amp", "moving_points", "tabular_points_2d"])
+def test_correlation_output_with_per_path_single_scattering(case):
+    """Correlation output where the single-scattering amplitude depends on the first vertex (point
+    sources with HG or tabular scattering): sum_p a b f(0) conj f(t), with f(t) at the displaced
+    first vertex when the amplitude follows the scatterers."""
+    if case == "tabular_points_2d":
+        cfg, dims = G.with_tabular(G.hetero2_points(128), 2), 2
+        cfg["sampler"] = {"type": "TemporalSampler", "t": np.array([0.0, 1.0, 2.0]), "D": np.full(4, 2e-3),
+                          "U": {"x": np.zeros(4), "y": np.zeros(4)}, "isTemporalAmplitude": True}
+    else:
+        cfg, dims = G.moving_ff_points(128, temporal_amp=case == "moving_points_amp"), 3
+    cfg["tracer"] = {"type": "nee", "output": "correlation"}
+    cfg["precision"] = {"terms": "fp64", "contraction": "fp64"}
+    n = cfg["simulation"]["batchSize"]
+    with Tracer(cfg, dims) as tr:
+        d = tr.debug_batch(0, n)
+    print("case", case, "vertices", int(d["off"][-1]), flush=True)
+    A, B, S, U = _oracle_u(cfg, dims, d, d["a"], d["b"])
+    assert np.linalg.norm(S) > 0
+    assert _rel(d["u"], U) < 1e-10
