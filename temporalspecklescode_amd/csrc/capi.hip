@@ -118,9 +118,8 @@ static int validate(const sstmc_config* c) {
     if (c->illumination.type == SSTMC_SOURCE_GAUSSIAN_BEAM || c->view.type == SSTMC_SOURCE_GAUSSIAN_BEAM) {
         // built: GaussianBeam illumination and view (fsGaussianBeam pairs two beams,
         // GaussianBeam.cuh:2040-2096), every material a vMF mixture of at most 16 lobes
-        // (updateSampler, :1283-1290), field output of the TemporalSampler without coherent
-        // backscattering
-        if (c->illumination.type != c->view.type || c->tracer.isCBS || c->tracer.output != SSTMC_OUTPUT_FIELD ||
+        // (updateSampler, :1283-1290), the TemporalSampler without coherent backscattering
+        if (c->illumination.type != c->view.type || c->tracer.isCBS ||
             c->sampler.type == SSTMC_SAMPLER_TEMPORAL_CORRELATION)
             return SSTMC_NOT_SUPPORTED;
         for (uint32_t m = 0; m < sc.materialNum; ++m)

@@ -495,10 +495,10 @@ __global__ void __launch_bounds__(256) k_single_gauss(MediumView m, ScatterView 
                                                       const typename SingleT<R>::C* __restrict__ a,
                                                       const typename SingleT<R>::C* __restrict__ b, uint32_t P,
                                                       uint32_t pathsPerSplit, uint32_t splits,
-                                                      typename SingleT<R>::C* __restrict__ part) {
+                                                      typename SingleT<R>::C* __restrict__ part, int corr) {
     typedef typename SingleT<R>::C C;
-    __shared__ double sx[16][3];
-    __shared__ uint32_t sm[16];
+    __shared__ double sx[16][3], sx0[16][3];
+    __shared__ uint32_t sm[16], sm0[16];
     __shared__ C sa[16][16], sb[16][16];
     const uint32_t L = il.n, V = vw.n;
     const uint32_t tl = threadIdx.x & 15, tq = threadIdx.x >> 4;
@@ -517,18 +517,23 @@ __global__ void __launch_bounds__(256) k_single_gauss(MediumView m, ScatterView 
             const uint64_t o = vb.off[p];
             if (tl == 0) {
                 const float4 xm = vb.xm[o];
-                d3 x1 = mkd(xm.x, xm.y, xm.z);
+                d3 x1 = mkd(xm.x, xm.y, xm.z), x10 = x1;
                 if (fr.temporalAmp && !fr.trivial) {
                     const float* d = vb.delta + (o * fr.nt + t) * 3;
+                    const float* d0 = vb.delta + o * fr.nt * 3;
                     x1 = mkd(x1.x + (double)d[0], x1.y + (double)d[1], x1.z + (double)d[2]);
+                    x10 = mkd(x10.x + (double)d0[0], x10.y + (double)d0[1], x10.z + (double)d0[2]);
                 }
                 sx[tq][0] = x1.x; sx[tq][1] = x1.y; sx[tq][2] = x1.z;
                 sm[tq] = gbMaterial(m, x1);
+                sx0[tq][0] = x10.x; sx0[tq][1] = x10.y; sx0[tq][2] = x10.z;
+                sm0[tq] = gbMaterial(m, x10);
             }
             if (ls < L) av = a[((size_t)t * P + p) * L + ls];
             if (vs < V) bv = b[((size_t)t * P + p) * V + vs];
         } else if (tl == 0) {
             sm[tq] = 0;
+            sm0[tq] = 0;
         }
         sa[tq][tl] = av;
         sb[tq][tl] = bv;
@@ -536,7 +541,15 @@ __global__ void __launch_bounds__(256) k_single_gauss(MediumView m, ScatterView 
         if (l < L && v < V)
             for (int q = 0; q < 16; ++q) {
                 if (sm[q] == 0) continue;
-                const Cd f = gbFs(il.g, vw.g, tab, l, v, sm[q], mkd(sx[q][0], sx[q][1], sx[q][2]), il.g.waveK);
+                Cd f = gbFs(il.g, vw.g, tab, l, v, sm[q], mkd(sx[q][0], sx[q][1], sx[q][2]), il.g.waveK);
+                if (corr) {   // S(0) conj S(t): fs at frame 0 times the conjugate at frame t
+                    const bool same = sx0[q][0] == sx[q][0] && sx0[q][1] == sx[q][1] && sx0[q][2] == sx[q][2];
+                    const Cd f0 = same ? f
+                                       : (sm0[q] ? gbFs(il.g, vw.g, tab, l, v, sm0[q], mkd(sx0[q][0], sx0[q][1], sx0[q][2]),
+                                                        il.g.waveK)
+                                                 : cd(0, 0));
+                    f = cd(f0.re * f.re + f0.im * f.im, f0.im * f.re - f0.re * f.im);
+                }
                 const C x = sa[q][tl], y = sb[q][tq];
                 const double pr = (double)x.x * y.x - (double)x.y * y.y, pi = (double)x.x * y.y + (double)x.y * y.x;
                 re += f.re * pr - f.im * pi;
@@ -676,9 +689,10 @@ __global__ void __launch_bounds__(256) k_single_gauss_uv(MediumView m, ScatterVi
     }
 }
 
-hipError_t launchSingleGauss(int fp64, int uniformViewDir, const MediumView& m, const ScatterView& sc,
+hipError_t launchSingleGauss(int fp64, int uniformViewDir, int corr, const MediumView& m, const ScatterView& sc,
                              const SourceView& il, const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
                              const void* a, const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st) {
+    if (corr) uniformViewDir = 0;   // the shared-direction kernel forms the field fs only
     const uint32_t L = il.n, V = vw.n, NT = fr.nt;
     uint32_t tiles = ((L + 15) / 16) * ((V + 15) / 16);
     uint32_t splits = chooseSplits(tiles, P, 64);
@@ -705,7 +719,7 @@ hipError_t launchSingleGauss(int fp64, int uniformViewDir, const MediumView& m, 
             hipLaunchKernelGGL(k_single_gauss_uv<double>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb,
                                (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr);
         else hipLaunchKernelGGL(k_single_gauss<double>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb, (const double2*)a,
-                           (const double2*)b, P, pps, splits, (double2*)ws->ptr);
+                           (const double2*)b, P, pps, splits, (double2*)ws->ptr, corr);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<double2>, dim3((LV + 255) / 256), dim3(256), 0, st,
                                (const double2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,
@@ -715,7 +729,7 @@ hipError_t launchSingleGauss(int fp64, int uniformViewDir, const MediumView& m, 
             hipLaunchKernelGGL(k_single_gauss_uv<float>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb,
                                (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr);
         else hipLaunchKernelGGL(k_single_gauss<float>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb, (const float2*)a,
-                           (const float2*)b, P, pps, splits, (float2*)ws->ptr);
+                           (const float2*)b, P, pps, splits, (float2*)ws->ptr, corr);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st,
                                (const float2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,

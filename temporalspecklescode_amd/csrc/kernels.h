@@ -89,10 +89,11 @@ hipError_t launchSingleCorrelation(int dims, int fp64, const ScatterView& sc, co
 // (3D, FP64).
 hipError_t launchGaussSetup(const MediumView& m, const double* pts, const double* dirs, uint32_t n, double* att,
                             uint32_t* mat, hipStream_t st);
-// Single scattering between GaussianBeam illuminations and views (field output):
-// U[t][v][l] += sum_p a[t][p][l] b[t][p][v] fs(l, v, x1). uniformViewDir: every view beam has the
-// direction of view 0 (the per-illumination part of fs is then shared by the views).
-hipError_t launchSingleGauss(int fp64, int uniformViewDir, const MediumView& m, const ScatterView& sc,
+// Single scattering between GaussianBeam illuminations and views:
+// U[t][v][l] += sum_p a[t][p][l] b[t][p][v] fs(l, v, x1), or with corr fs(x1(0)) conj fs(x1(t)).
+// uniformViewDir: every view beam has the direction of view 0 (the per-illumination part of fs is
+// then shared by the views; field output).
+hipError_t launchSingleGauss(int fp64, int uniformViewDir, int corr, const MediumView& m, const ScatterView& sc,
                              const SourceView& il, const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
                              const void* a, const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st);
 

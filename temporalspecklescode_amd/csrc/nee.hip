@@ -806,18 +806,19 @@ __global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView
     const bool active = v < V && K >= 2 && t0 < NT;
     const uint32_t vv = active ? v : 0;
     const d3 vd = sourceD<DIMS>(vw, vv);
-    const f3 vf = SRC != kSrcPoint && sizeof(R) == 4 ? sourceF(vw, vv) : mk(0.f, 0.f, 0.f);
+    const f3 vf = SRC != kSrcPoint && SRC != kSrcGauss && sizeof(R) == 4 ? sourceF(vw, vv) : mk(0.f, 0.f, 0.f);
+    const d3 vdir = sourceDir<SRC>(vw, vv, vd);
     if (active) {
         for (uint32_t j = 0; j < nt; ++j) acc[j][threadIdx.x] = Cx<R>::make(0, 0);
         for (uint32_t k = 1; k < K; ++k) {
             const uint64_t ok = off + k;
             const float4 xkm = vb.xm[ok];
-            const C g0 = viewTerm<DIMS, R, SRC, TAB>(m, sig, sc, vw, fr, vb, vd, vd, vf, ok, 0, xkm, invLambda, tauCut,
-                                                     isoPoint, &steps);
+            const C g0 = viewTerm<DIMS, R, SRC, TAB>(m, sig, sc, vw, fr, vb, vd, vdir, vf, ok, 0, xkm, invLambda,
+                                                     tauCut, isoPoint, &steps);
             for (uint32_t j = 0; j < nt; ++j) {
                 const uint32_t t = t0 + j;
                 const C gt = t == 0 ? g0
-                                    : viewTerm<DIMS, R, SRC, TAB>(m, sig, sc, vw, fr, vb, vd, vd, vf, ok, t, xkm,
+                                    : viewTerm<DIMS, R, SRC, TAB>(m, sig, sc, vw, fr, vb, vd, vdir, vf, ok, t, xkm,
                                                                   invLambda, tauCut, isoPoint, &steps);
                 C a = acc[j][threadIdx.x];
                 a.x += g0.x * gt.x + g0.y * gt.y;      // G(0) * conj(G(t))
@@ -981,10 +982,16 @@ static void launchView(bool corr, const MediumView& m, const ScatterView& sc, co
     const bool wave = q.lppView == 64, tab = sc.tab != nullptr;
 #define SSTMC_VIEW(K, W, T) hipLaunchKernelGGL((K<DIMS, R, SRC, W, T>), grid, block, 0, st, m, sc, src, fr, vb, q, out)
     if (corr) {
-        if constexpr (SRC != kSrcFarIllum && SRC != kSrcGauss) {
+        if constexpr (SRC == kSrcGauss) {   // GaussianBeam sources imply vMF tables
+            if (wave) SSTMC_VIEW(k_nee_view_corr, true, true);
+            else SSTMC_VIEW(k_nee_view_corr, false, true);
+        } else if constexpr (SRC != kSrcFarIllum) {
             if (wave) { if (tab) SSTMC_VIEW(k_nee_view_corr, true, true); else SSTMC_VIEW(k_nee_view_corr, true, false); }
             else { if (tab) SSTMC_VIEW(k_nee_view_corr, false, true); else SSTMC_VIEW(k_nee_view_corr, false, false); }
         }
+    } else if constexpr (SRC == kSrcGauss) {
+        if (wave) SSTMC_VIEW(k_nee_view, true, true);
+        else SSTMC_VIEW(k_nee_view, false, true);
     } else if (wave) {
         if (tab) SSTMC_VIEW(k_nee_view, true, true);
         else SSTMC_VIEW(k_nee_view, true, false);

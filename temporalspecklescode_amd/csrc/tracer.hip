@@ -756,9 +756,9 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
     singleMode_ = kSingleGeneric;
     bool needF = false;
     if (il_.type() == SSTMC_SOURCE_GAUSSIAN_BEAM || vw_.type() == SSTMC_SOURCE_GAUSSIAN_BEAM) {
-        // both roles GaussianBeam (fsGaussianBeam pairs two beams), field output of the
-        // TemporalSampler without coherent backscattering
-        if (il_.type() != vw_.type() || acorr || opt.cbs || opt.output != SSTMC_OUTPUT_FIELD || dims_ != 3) {
+        // both roles GaussianBeam (fsGaussianBeam pairs two beams), the TemporalSampler without
+        // coherent backscattering
+        if (il_.type() != vw_.type() || acorr || opt.cbs || dims_ != 3) {
             *err = SSTMC_NOT_SUPPORTED;
             return;
         }
@@ -789,7 +789,10 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
         sameViews_ = std::equal(hv.begin(), hv.begin() + 3 * (size_t)V_, hv.begin() + 3 * (size_t)V_);
     } else if (opt_.output == SSTMC_OUTPUT_CORRELATION) {
         // S(0) conj S(t): a scalar or fixed F enters squared, a per-path f as f(0) conj f(t)
-        if (singleMode_ != kSingleSeparable && singleMode_ != kSingleGeneric) { *err = SSTMC_NOT_SUPPORTED; return; }
+        if (singleMode_ != kSingleSeparable && singleMode_ != kSingleGeneric && singleMode_ != kSingleGauss) {
+            *err = SSTMC_NOT_SUPPORTED;
+            return;
+        }
         fscalar_ *= fscalar_;
     }
     if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) { *err = SSTMC_DEVICE_ERROR; return; }
@@ -867,7 +870,7 @@ int NextEventEstimation::contractSingle(uint32_t npaths, double* U) {
         const std::vector<double>& vd = static_cast<const GaussianBeam&>(vw_).directions();
         bool uniformDir = true;
         for (size_t i = 3; i < vd.size() && uniformDir; ++i) uniformDir = vd[i] == vd[i % 3];
-        he = launchSingleGauss(fp64_ ? 1 : 0, uniformDir ? 1 : 0, mview_, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p, npaths,
+        he = launchSingleGauss(fp64_ ? 1 : 0, uniformDir ? 1 : 0, opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0, mview_, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p, npaths,
                                U, &wsSingle_, stream_);
     } else if (singleMode_ == kSingleCorrelation) {
         const bool uniformFar = il_.type() == SSTMC_SOURCE_FARFIELD && vw_.type() == SSTMC_SOURCE_FARFIELD &&

@@ -113,8 +113,8 @@ def test_correlation_sampler_config_checks(native):
 
 def test_gaussian_beam_config_checks(native):
     """GaussianBeam (MatlabInterface.cu:1153-1190): 3D, a direction per focal point, aperture > 0;
-    built for GaussianBeam illumination and view together, vMF mixtures of at most 16 lobes, field
-    output of the TemporalSampler without coherent backscattering."""
+    built for GaussianBeam illumination and view together, vMF mixtures of at most 16 lobes, the
+    TemporalSampler without coherent backscattering."""
     cfg = W.confocal(focal_x=[0.0, 2.0], focal_z=[8.0], tau_x=[0.0], batch=64, full_iteration=1)
     assert native.sstmc_validate(config.CConfig(cfg, 3).ptr()) == 0
     cc = config.CConfig(cfg, 3)
@@ -123,7 +123,9 @@ def test_gaussian_beam_config_checks(native):
     ff = dict(cfg, view={"type": "FarField", "direction": {"x": [0.0], "y": [0.0], "z": [1.0]}})
     assert native.sstmc_validate(config.CConfig(ff, 3).ptr()) == 5
     corr = dict(cfg, tracer={"type": "nee", "output": "correlation"})
-    assert native.sstmc_validate(config.CConfig(corr, 3).ptr()) == 5
+    assert native.sstmc_validate(config.CConfig(corr, 3).ptr()) == 0
+    cbs = dict(cfg, tracer={"type": "nee", "isCBS": True})
+    assert native.sstmc_validate(config.CConfig(cbs, 3).ptr()) == 5
     hg = dict(cfg, scattering={"type": ["HG"] * 5, "amplitudeFunction": [0.8] * 5})
     assert native.sstmc_validate(config.CConfig(hg, 3).ptr()) == 5
     flat = dict(cfg, medium=dict(cfg["medium"], materialGrid=np.array([[1, 2]], np.uint32)))

@@ -108,3 +108,19 @@ def test_wide_grid_fp64_matches_oracle():
     assert np.isfinite(d["u"]).all()
     A, B, S, U = _oracle_u(cfg, 3, d, d["a"], d["b"])
     assert _rel(d["u"], U) < 1e-9
+
+
+@pytest.mark.parametrize("amp", [False, True])
+def test_correlation_output(amp):
+    """Correlation output over moving frames: view sums G(0) conj G(t) with the beams' complex f,
+    single scattering fs(0) conj fs(t) (fs at the displaced first vertex when the amplitude follows
+    the scatterers)."""
+    cfg = _small(128, frames=True)
+    cfg["sampler"]["isTemporalAmplitude"] = amp
+    cfg["tracer"] = {"type": "nee", "output": "correlation"}
+    cfg["precision"] = {"terms": "fp64", "contraction": "fp64"}
+    with Tracer(cfg, 3) as tr:
+        d = tr.debug_batch(0, 128)
+    A, B, S, U = _oracle_u(cfg, 3, d, d["a"], d["b"])
+    assert np.linalg.norm(S) > 0 and np.linalg.norm(B) > 0
+    assert _rel(d["u"], U) < 1e-9
