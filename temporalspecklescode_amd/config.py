@@ -100,7 +100,12 @@ class CConfig:
         return arr.ctypes.data_as(_DP), arr.size
 
     def ptr(self):
-        return C.byref(self.struct)
+        """Pointer to the C struct. It holds this object, and with it every array the struct points
+        into, for as long as the pointer itself is referenced (so `CConfig(cfg, d).ptr()` passed
+        straight to a call stays valid during the call)."""
+        p = C.pointer(self.struct)
+        p._owner = self
+        return p
 
     def _fill(self, cfg, dims):
         s = self.struct

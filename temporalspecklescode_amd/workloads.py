@@ -277,6 +277,38 @@ def confocal(focal_x=np.arange(-50.0, 51.0, 1.0), focal_z=np.arange(0.0, 21.0, 2
     }
 
 
+def gauss_transmission_matrix(pixels: int = 51, dp: float = 0.2, backscattering: bool = True, batch: int = 1,
+                              full_iteration: int = 1000, seed: int = 1, aperture: float = 0.4) -> dict:
+    """The configuration of the reference's examples/transmissionMatrixRendering.m (lines 1-95): a
+    2000 x 2000 x 100 slab of one material (mean free path 40, isotropic vMF lobes), pixels x pixels
+    GaussianBeam illuminations focused on the z = 0 face along +z, GaussianBeam views on the same
+    grid (backscattering: at z = 0 looking along -z; transmissive: at z = 100 along +z), five
+    frames of Brownian motion."""
+    r = (np.arange(pixels, dtype=np.float64) - (pixels - 1) / 2) * dp
+    X, Y = np.meshgrid(r, r, indexing="ij")
+    x, y = X.ravel(order="F"), Y.ravel(order="F")
+    n = x.size
+    iso = {"mixtureMu": [0.0], "mixtureC": [-1.2655121234846453964889], "mixtureAlpha": [1.0]}
+    sigs = np.array([0.0, 1 / 40])
+    vz, vd = (0.0, -1.0) if backscattering else (100.0, 1.0)
+    return {
+        "simulation": {"batchSize": batch, "fullIteration": full_iteration, "seed": seed, "lambda": 1.0},
+        "scattering": {"type": ["vMF"] * 2, "amplitudeFunction": [dict(iso), dict(iso)]},
+        "medium": {"type": "Heterogeneous", "xAxis": [-1000.0, 1000.0], "yAxis": [-1000.0, 1000.0],
+                   "zAxis": [0.0, 100.0], "materialGrid": np.ones((1, 1, 1), np.uint32), "sigs": sigs,
+                   "siga": sigs * 0},
+        "illumination": {"type": "GaussianBeam", "focalPoint": {"x": x, "y": y, "z": np.zeros(n)},
+                         "focalDirection": {"x": np.zeros(n), "y": np.zeros(n), "z": np.ones(n)},
+                         "aperture": aperture},
+        "view": {"type": "GaussianBeam", "focalPoint": {"x": x, "y": y, "z": np.full(n, vz)},
+                 "focalDirection": {"x": np.zeros(n), "y": np.zeros(n), "z": np.full(n, vd)},
+                 "aperture": aperture},
+        "sampler": {"type": "TemporalSampler", "t": np.arange(0.0, 1.01, 0.25), "D": np.array([0.0, 1e-3]),
+                    "U": {"x": np.zeros(2), "y": np.zeros(2), "z": np.zeros(2)}},
+        "tracer": {"type": "nee"},
+    }
+
+
 def c4(paths: int = 1 << 24, batch: int = 1 << 16, seed: int = 6, perlin_seed: int = 5, n: int = 128,
        nside: int = 32) -> dict:
     """Transmission matrix: 128^3 Perlin-thresholded grid, 1024 x 1024 plane waves."""

@@ -124,3 +124,21 @@ def test_correlation_output(amp):
     A, B, S, U = _oracle_u(cfg, 3, d, d["a"], d["b"])
     assert np.linalg.norm(S) > 0 and np.linalg.norm(B) > 0
     assert _rel(d["u"], U) < 1e-9
+
+
+@pytest.mark.parametrize("backscattering", [True, False])
+def test_transmission_matrix_example(backscattering):
+    """The reference's transmissionMatrixRendering.m geometry (5 x 5 pixels here): isotropic vMF
+    lobes everywhere, so the second direction is uniform and its density 1 / (4 pi); focal points on
+    the z = 0 face; views looking back along -z or through the slab; five Brownian frames."""
+    cfg = W.gauss_transmission_matrix(pixels=5, dp=2.0, backscattering=backscattering, batch=256,
+                                      full_iteration=1, seed=4)
+    cfg["precision"] = {"terms": "fp64", "contraction": "fp64"}
+    with Tracer(cfg, 3) as tr:
+        d = tr.debug_batch(0, 256)
+    o, same_k, match = _transport_match(cfg, 3, d, 256, tol=2e-3)
+    print("backscattering", backscattering, "vertices", int(d["off"][-1]), "match", np.mean(match), flush=True)
+    assert np.mean(match) >= 0.95
+    A, B, S, U = _oracle_u(cfg, 3, d, d["a"], d["b"])
+    assert np.linalg.norm(S) > 0 and np.linalg.norm(B) > 0
+    assert _rel(d["u"], U) < 1e-9
