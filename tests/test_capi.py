@@ -131,3 +131,16 @@ def test_gaussian_beam_config_checks(native):
     flat = dict(cfg, medium=dict(cfg["medium"], materialGrid=np.array([[1, 2]], np.uint32)))
     with pytest.raises(ValueError, match="GaussianBeam sources are supported in 3D"):
         config.CConfig(flat, 2)
+
+
+def test_config_pointer_keeps_its_arrays(native):
+    """A pointer from a temporary CConfig holds the config (and the arrays its struct points into)
+    until the pointer is released: validation through it after garbage collection and new
+    allocations still sees the original arrays."""
+    import gc
+    p = config.CConfig(W.gauss_transmission_matrix(pixels=9, dp=1.0, batch=16, full_iteration=1), 3).ptr()
+    gc.collect()
+    junk = [np.full(1 << 16, 7.0) for _ in range(64)]
+    assert native.sstmc_validate(p) == 0
+    assert p.contents.illumination.num == 81 and p.contents.illumination.aperture == 0.4
+    del junk
