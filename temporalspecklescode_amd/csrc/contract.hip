@@ -565,6 +565,8 @@ __global__ void __launch_bounds__(256) k_single_gauss(MediumView m, ScatterView 
     }
 }
 
+constexpr uint32_t kFsFrames = 8;   // frames sharing one fs evaluation in k_single_gauss_uv (F)
+
 // The same sum when every view beam has one direction d_v (the confocal geometry): then the inner
 // convolution of fsGaussianBeam (its factor gob, the axis w, C and the exponent c) depends on the
 // illumination, the lobe and the path only. Per path, the 16 x M (illumination, lobe) records of the
@@ -573,13 +575,13 @@ __global__ void __launch_bounds__(256) k_single_gauss(MediumView m, ScatterView 
 // sine-cosine pair of Im sm, two real exponentials exp(Re(r +- C) +- Re sm) (each exponent whole, as
 // the reference forms it, so that no factor overflows alone) and the division by C sm. A term is dropped when the real part of its exponent is at most -10,
 // as complex2timesSinh drops it; the rare zero cases of C or sm take the reference's special forms.
-template <typename R>
+template <typename R, uint32_t F>
 __global__ void __launch_bounds__(256) k_single_gauss_uv(MediumView m, ScatterView sc, SourceView il, SourceView vw,
                                                          FramesView fr, VertexBatch vb,
                                                          const typename SingleT<R>::C* __restrict__ a,
                                                          const typename SingleT<R>::C* __restrict__ b, uint32_t P,
                                                          uint32_t pathsPerSplit, uint32_t splits,
-                                                         typename SingleT<R>::C* __restrict__ part) {
+                                                         typename SingleT<R>::C* __restrict__ part, uint32_t fpb) {
     typedef typename SingleT<R>::C C;
     __shared__ double2 sGob[kGaussLobes][16], sC[kGaussLobes][16], sUp[kGaussLobes][16], sUm[kGaussLobes][16],
         sR[kGaussLobes][16];
@@ -588,7 +590,10 @@ __global__ void __launch_bounds__(256) k_single_gauss_uv(MediumView m, ScatterVi
     __shared__ uint32_t sm1;
     const uint32_t L = il.n, V = vw.n;
     const uint32_t tl = threadIdx.x & 15, tq = threadIdx.x >> 4;
-    const uint32_t split = blockIdx.z % splits, t = blockIdx.z / splits;
+    // frames t0 .. t0 + nf - 1 of this block share fs (fpb > 1: the first vertex does not move with
+    // the frame); fpb = 1 takes the first vertex of frame t0
+    const uint32_t split = blockIdx.z % splits, t0 = (blockIdx.z / splits) * fpb;
+    const uint32_t nf = fr.nt - t0 < fpb ? fr.nt - t0 : fpb;
     const uint32_t l = blockIdx.x * 16 + tl, v = blockIdx.y * 16 + tq;
     const uint32_t pBegin = split * pathsPerSplit, pEnd = min(P, pBegin + pathsPerSplit);
     const PhaseTables& tab = *sc.tab;
@@ -598,15 +603,17 @@ __global__ void __launch_bounds__(256) k_single_gauss_uv(MediumView m, ScatterVi
     const d3 db = gbDir(gb, 0);
     const double tc = ga.apNorm + gb.apNorm + 2.0 * 1.8378770664093454836;
     const bool poolOk = (uint64_t)vb.off[P] <= vb.cap;
-    double re = 0, im = 0;
+    double re[F], im[F];
+#pragma unroll
+    for (uint32_t j = 0; j < F; ++j) re[j] = im[j] = 0;
     for (uint32_t p = pBegin; p < pEnd; ++p) {
         if (!poolOk || vb.k[p] == 0) continue;   // uniform over the block
         const uint64_t o = vb.off[p];
         if (threadIdx.x == 0) {
             const float4 xm = vb.xm[o];
             d3 x1 = mkd(xm.x, xm.y, xm.z);
-            if (fr.temporalAmp && !fr.trivial) {
-                const float* d = vb.delta + (o * fr.nt + t) * 3;
+            if (fpb == 1 && fr.temporalAmp && !fr.trivial) {
+                const float* d = vb.delta + (o * fr.nt + t0) * 3;
                 x1 = mkd(x1.x + (double)d[0], x1.y + (double)d[1], x1.z + (double)d[2]);
             }
             sx[0] = x1.x; sx[1] = x1.y; sx[2] = x1.z;
@@ -674,18 +681,27 @@ __global__ void __launch_bounds__(256) k_single_gauss_uv(MediumView m, ScatterVi
                     else fs = fs + cdiv(2.0 * twoSinhd(sm, cd(sNu[j])), sm);
                 }
             }
-            const C x = a[((size_t)t * P + p) * L + l], y = b[((size_t)t * P + p) * V + v];
-            const double pr = (double)x.x * y.x - (double)x.y * y.y, pi = (double)x.x * y.y + (double)x.y * y.x;
-            re += fs.re * pr - fs.im * pi;
-            im += fs.re * pi + fs.im * pr;
+#pragma unroll
+            for (uint32_t j = 0; j < F; ++j) {
+                if (j >= nf) break;
+                const size_t t = t0 + j;
+                const C x = a[(t * P + p) * L + l], y = b[(t * P + p) * V + v];
+                const double pr = (double)x.x * y.x - (double)x.y * y.y, pi = (double)x.x * y.y + (double)x.y * y.x;
+                re[j] += fs.re * pr - fs.im * pi;
+                im[j] += fs.re * pi + fs.im * pr;
+            }
         }
         __syncthreads();
     }
     if (l < L && v < V) {
-        C c;
-        c.x = (R)re;
-        c.y = (R)im;
-        part[((size_t)blockIdx.z) * L * V + (size_t)v * L + l] = c;
+#pragma unroll
+        for (uint32_t j = 0; j < F; ++j) {
+            if (j >= nf) break;
+            C c;
+            c.x = (R)re[j];
+            c.y = (R)im[j];
+            part[((size_t)(t0 + j) * splits + split) * L * V + (size_t)v * L + l] = c;
+        }
     }
 }
 
@@ -713,11 +729,18 @@ hipError_t launchSingleGauss(int fp64, int uniformViewDir, int corr, const Mediu
         ws->bytes = need;
     }
     dim3 grid((L + 15) / 16, (V + 15) / 16, splits * NT);
+    // frames per block of the shared-direction kernel: all frames share fs unless the amplitude
+    // follows a moving first vertex
+    const uint32_t fpb = (fr.temporalAmp && !fr.trivial) ? 1u : (NT < kFsFrames ? NT : kFsFrames);
+    const dim3 gridUv((L + 15) / 16, (V + 15) / 16, splits * ((NT + fpb - 1) / fpb));
     const uint32_t LV = L * V;
     if (fp64) {
-        if (uniformViewDir)
-            hipLaunchKernelGGL(k_single_gauss_uv<double>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb,
-                               (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr);
+        if (uniformViewDir && fpb == 1)
+            hipLaunchKernelGGL((k_single_gauss_uv<double, 1>), gridUv, dim3(256), 0, st, m, sc, il, vw, fr, vb,
+                               (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr, fpb);
+        else if (uniformViewDir)
+            hipLaunchKernelGGL((k_single_gauss_uv<double, kFsFrames>), gridUv, dim3(256), 0, st, m, sc, il, vw, fr, vb,
+                               (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr, fpb);
         else hipLaunchKernelGGL(k_single_gauss<double>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb, (const double2*)a,
                            (const double2*)b, P, pps, splits, (double2*)ws->ptr, corr);
         for (uint32_t t = 0; t < NT; ++t)
@@ -725,9 +748,12 @@ hipError_t launchSingleGauss(int fp64, int uniformViewDir, int corr, const Mediu
                                (const double2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,
                                (const double2*)nullptr, 0, 1.0, U + 2 * (size_t)t * LV);
     } else {
-        if (uniformViewDir)
-            hipLaunchKernelGGL(k_single_gauss_uv<float>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb,
-                               (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr);
+        if (uniformViewDir && fpb == 1)
+            hipLaunchKernelGGL((k_single_gauss_uv<float, 1>), gridUv, dim3(256), 0, st, m, sc, il, vw, fr, vb,
+                               (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr, fpb);
+        else if (uniformViewDir)
+            hipLaunchKernelGGL((k_single_gauss_uv<float, kFsFrames>), gridUv, dim3(256), 0, st, m, sc, il, vw, fr, vb,
+                               (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr, fpb);
         else hipLaunchKernelGGL(k_single_gauss<float>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb, (const float2*)a,
                            (const float2*)b, P, pps, splits, (float2*)ws->ptr, corr);
         for (uint32_t t = 0; t < NT; ++t)
