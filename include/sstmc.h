@@ -160,6 +160,13 @@ typedef struct sstmc_tracer {
 typedef struct sstmc_precision {
     int32_t terms;                 /* SSTMC_TERMS_FP32 (default) or SSTMC_TERMS_FP64 */
     int32_t contraction;           /* SSTMC_CONTRACT_BF16X3 (default), _FP32 or _FP64 */
+    /* Error-budgeted attenuation cut: an NEE term (g of one illumination or view at one vertex)
+     * whose optical depth to the source exceeds tauCut is dropped, and its voxel walk stops
+     * there; the dropped amplitude is below exp(-tauCut / 2) times the term's unattenuated one.
+     * 0 (default) is exact: walks run to the box (FP32 walks stop at 210, past which exp(-tau/2)
+     * is exactly 0 in FP32). The reference walks every term to the box
+     * (HeterogeneousMedium.cuh:386-573). */
+    double tauCut;
 } sstmc_precision;
 
 typedef struct sstmc_config {
@@ -268,6 +275,12 @@ int sstmc_debug_batch(sstmc_handle* h, uint64_t path_begin, uint32_t npaths,
  * The multiple-scattering field is (sum_p A^T B + sum_p Br^T Ar) / sqrt(2). */
 int sstmc_debug_cbs(sstmc_handle* h, uint32_t npaths, double* term_ar, double* term_br);
 
+/* The reference optical depth of the relative attenuation cut (precision.tauCut) in the last
+ * batch traced (sstmc_debug_batch or the last batch of sstmc_trace): the least view optical
+ * depth its sample found; the cut dropped the view-side terms whose depth exceeded
+ * tau_ref + tauCut. SSTMC_INVALID_CONFIG when the handle has no cut. */
+int sstmc_debug_tau_ref(const sstmc_handle* h, float* tau_ref);
+
 /* Contracts caller-supplied terms on the device with the handle's contraction precision:
  * u[t][v][l] += sum_p a[p][t][l] * b[p][t][v] (complex, interleaved doubles in and out).
  * Exposes kernel (c) alone for accuracy tests against an FP64 GEMM. */
@@ -289,14 +302,17 @@ typedef struct sstmc_server_stats {
 
 int sstmc_server_create(uint32_t device, uint64_t hbm_budget_bytes, uint32_t max_inflight,
                         sstmc_server** out);
-/* Queues one rendering of cfg (fullIteration batches); returns a request id. */
+/* Queues cfg: renderingsNum renderings of fullIteration batches each, rendering i at seed + i, as
+ * sstmc_render does (MatlabInterface.cu:1797-1800); returns a request id. The config is copied
+ * whole (every array it points into), so the caller may free or reuse its arrays on return. */
 int sstmc_submit(sstmc_server* s, const sstmc_config* cfg, uint64_t* request_id);
 /* Advances the loop without blocking; returns SSTMC_NO_ERROR when the request is done,
  * SSTMC_NOT_READY otherwise. */
 int sstmc_poll(sstmc_server* s, uint64_t request_id);
 /* Blocks until the request is done. */
 int sstmc_wait(sstmc_server* s, uint64_t request_id);
-/* Copies the finished result (2*Nl*Nv*Nt doubles, iter) and releases its device memory. */
+/* Copies the finished result and releases its device memory. u: 2*Nl*Nv*Nt*renderingsNum doubles
+ * laid out u(l, v, t, i) as sstmc_render writes them; iter (may be NULL): renderingsNum doubles. */
 int sstmc_fetch(sstmc_server* s, uint64_t request_id, double* u, double* iter);
 int sstmc_server_stats_get(sstmc_server* s, sstmc_server_stats* out);
 void sstmc_server_destroy(sstmc_server* s);

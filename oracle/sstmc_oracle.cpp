@@ -525,6 +525,38 @@ struct Problem {
     uint64_t seed;
     int output;                // SSTMC_OUTPUT_FIELD / _CORRELATION
     bool cbs = false;          // coherent backscattering: add the reversed path of every path
+    // precision.tauCut = delta: an unweighted view-side term (g of a view) whose optical depth
+    // exceeds tauRef + delta is dropped (delta = 0: none), tauRef being the batch's least view
+    // optical depth over the sample of tauReference; terms that carry the path weight
+    // (illumination terms at x1, the reversed path's view term at x1) and the reversed path's
+    // illumination terms are never cut. Not in the reference (it walks every term to the box);
+    // restated from include/sstmc.h so the GPU's cut mode can be checked term by term.
+    double tauCut = 0.0;
+    double tauRef = 0.0;
+    double cutT(double tau, bool cut) const {
+        return (cut && tauCut > 0 && tau > tauRef + tauCut) ? 0.0 : std::exp(-tau);
+    }
+    // The batch's least view optical depth over every pathStride-th path (pathStride = P / 4096
+    // for batches of more than 4096 paths, else 1), its vertices at frame 0, and every view (at
+    // most 64) or 64 views spread evenly (view lane * n / 64).
+    double tauReference(const std::vector<std::vector<Vertex>>& batch) const {
+        const size_t P = batch.size();
+        const size_t stride = P > 4096 ? P / 4096 : 1;
+        const uint32_t n = vw.n();
+        const uint32_t lanes = n < 64 ? n : 64;
+        double best = 3.0e38;
+        for (size_t i = 0; i < P; i += stride)
+            for (const Vertex& x : batch[i])
+                for (uint32_t lane = 0; lane < lanes; ++lane) {
+                    const uint32_t v = n <= 64 ? lane : (uint32_t)(((uint64_t)lane * n) / 64);
+                    V3 w;
+                    if (vw.type == SSTMC_SOURCE_FARFIELD) w = vw.pt[v];
+                    else if (vw.type == SSTMC_SOURCE_GAUSSIAN_BEAM) w = vw.dir[v];
+                    else w = geo.normalize(sub(vw.pt[v], x.xt[0]));
+                    best = std::min(best, med.opticalDepth(x.xt[0], w, nullptr));
+                }
+        return best;
+    }
     // TemporalCorrelationSampler: il holds the illumination (u1) sources then the illumination2
     // (u2) ones, vw the view then the view2 ones; ticks are the time lags
     bool acorr = false;
@@ -767,10 +799,10 @@ struct Problem {
     }
     // gGaussianBeam with the attenuation along the beam (GaussianBeam.cuh:795-816, 1942-1992): the
     // square root of the transmittance toward the beam's source side (illumination: -d; view: +d).
-    cplx gGauss(const Source& s, uint32_t i, const V3& x, uint64_t* steps) const {
+    cplx gGauss(const Source& s, uint32_t i, const V3& x, uint64_t* steps, bool cut = false) const {
         if (matAt(x) == 0) return cplx(0, 0);
         const V3 w = s.illum ? scale(-1.0, s.dir[i]) : s.dir[i];
-        return cplx(std::sqrt(std::exp(-med.opticalDepth(x, w, steps))), 0.0);
+        return cplx(std::sqrt(cutT(med.opticalDepth(x, w, steps), cut)), 0.0);
     }
     // fGaussianBeam (GaussianBeam.cuh:818-892): the vMF lobes of x0's material convolved with the
     // beam's complex throughput gamma_a d + i k (x0 - focal) (k negative for a view), w0 the
@@ -988,12 +1020,13 @@ struct Problem {
 
     // ---------------------------------------------------------------- NEE terms
     // g: FarField gDirectionKernel (FarField.cuh:223-263) / gPointKernel (PointSource.cuh:117-172)
-    cplx g(const Source& s, uint32_t i, const V3& x, uint64_t* steps) const {
-        if (s.type == SSTMC_SOURCE_GAUSSIAN_BEAM) return gGauss(s, i, x, steps);
+    // cut: an unweighted term, dropped past precision.tauCut
+    cplx g(const Source& s, uint32_t i, const V3& x, uint64_t* steps, bool cut = false) const {
+        if (s.type == SSTMC_SOURCE_GAUSSIAN_BEAM) return gGauss(s, i, x, steps, cut);
         if (s.type == SSTMC_SOURCE_FARFIELD) {
             const V3& d = s.pt[i];
             V3 wd = s.illum ? scale(-1.0, d) : d;
-            double T = std::exp(-med.opticalDepth(x, wd, steps));
+            double T = cutT(med.opticalDepth(x, wd, steps), cut);
             double a = std::sqrt(T);
             double ph = (s.illum ? 1.0 : -1.0) * 2.0 * geo.dot(d, x) / lambda;
             return cplx(a * std::cos(PI * ph), a * std::sin(PI * ph));
@@ -1001,7 +1034,7 @@ struct Problem {
         V3 q = sub(s.pt[i], x);
         double r = geo.norm(q);
         double rr = 1.0 / r;
-        double T = std::exp(-med.opticalDepth(x, geo.normalize(q), steps));
+        double T = cutT(med.opticalDepth(x, geo.normalize(q), steps), cut);
         double at = T * rr;
         if (dims == 3) at *= rr * sc.isoAmp();
         if (s.area) at *= rr * std::fabs(dims == 3 ? x[2] - s.pt[i][2] : x[1] - s.pt[i][1]);
@@ -1095,7 +1128,7 @@ struct Problem {
         std::vector<cplx> gl((size_t)nt * nl), gv1((size_t)nt * nv);
         for (uint32_t t = 0; t < nt; ++t) {
             for (uint32_t l = 0; l < nl; ++l) gl[t * nl + l] = g(il, l, v1.xt[t], steps);
-            for (uint32_t v = 0; v < nv; ++v) gv1[t * nv + v] = g(vw, v, v1.xt[t], steps);
+            for (uint32_t v = 0; v < nv; ++v) gv1[t * nv + v] = g(vw, v, v1.xt[t], steps, !cbs);
         }
         cplx e1(std::cos(2.0 * PI * v1.rho), std::sin(2.0 * PI * v1.rho));
         for (uint32_t t = 0; t < nt; ++t) {
@@ -1156,7 +1189,7 @@ struct Problem {
                 double ph = xk.rho + xk.len[t] / lambda;
                 cplx e(std::cos(2.0 * PI * ph), std::sin(2.0 * PI * ph));
                 for (uint32_t v = 0; v < nv; ++v)
-                    G[t * nv + v] = g(vw, v, xk.xt[t], steps) * fvw(v, xk.m, pp, pk) * e;
+                    G[t * nv + v] = g(vw, v, xk.xt[t], steps, true) * fvw(v, xk.m, pp, pk) * e;
             }
             for (uint32_t t = 0; t < nt; ++t)
                 for (uint32_t v = 0; v < nv; ++v) {
@@ -1215,7 +1248,7 @@ struct Problem {
             const double dtD = k2 * std::fabs(fr.D[v1.m] * ticks[t]);
             const V3 xa = shifted(v1.x, v1.m, -s), xb = shifted(v1.x, v1.m, s);
             for (uint32_t l = 0; l < nl; ++l) { gl1[l] = g(il, l, xa, steps); gl2[l] = g(il, nl + l, xb, steps); }
-            for (uint32_t v = 0; v < nv; ++v) { gv1[v] = g(vw, v, xa, steps); gv2[v] = g(vw, nv + v, xb, steps); }
+            for (uint32_t v = 0; v < nv; ++v) { gv1[v] = g(vw, v, xa, steps, true); gv2[v] = g(vw, nv + v, xb, steps, true); }
             for (uint32_t v = 0; v < nv; ++v)
                 for (uint32_t l = 0; l < nl; ++l) {
                     const cplx u1 = gl1[l] * gv1[v], u2 = gl2[l] * gv2[v];
@@ -1265,8 +1298,8 @@ struct Problem {
                 const double inner = std::exp(-k2 * std::fabs(ticks[t]) * mt);
                 const V3 xka = shifted(xk.x, xk.m, -s), xkb = shifted(xk.x, xk.m, s), xpa = shifted(xp.x, xp.m, -s);
                 for (uint32_t v = 0; v < nv; ++v) {
-                    const cplx G1 = g(vw, v, xka, steps) * fvw(v, xk.m, xpa, xka) * e * inner;
-                    const cplx G2 = g(vw, nv + v, xkb, steps) * fvw(nv + v, xk.m, xpa, xka) * e;
+                    const cplx G1 = g(vw, v, xka, steps, true) * fvw(v, xk.m, xpa, xka) * e * inner;
+                    const cplx G2 = g(vw, nv + v, xkb, steps, true) * fvw(nv + v, xk.m, xpa, xka) * e;
                     const V3 q = sub(scale(-0.5, add(outDir(v, xk.x), outDir(nv + v, xk.x))), back);
                     out.B[(size_t)t * nv + v] += G1 * std::conj(G2) * std::exp(-dtD * geo.dot(q, q));
                 }
@@ -1307,6 +1340,7 @@ int build(const sstmc_config* c, Problem& pb) {
     pb.seed = c->simulation.seed;
     pb.output = c->tracer.output;
     pb.cbs = c->tracer.isCBS != 0;
+    pb.tauCut = c->precision.tauCut;
     if (pb.cbs && pb.output == SSTMC_OUTPUT_CORRELATION) return SSTMC_NOT_SUPPORTED;
     // scattering: one cell -> materials 0 and 1 (MatlabInterface.cu:750-760, 878-881)
     uint32_t nm = c->scattering.materialNum;
@@ -1510,6 +1544,43 @@ int oracle_render(const sstmc_config* cfg, double* u, double* iter, uint64_t* ve
     std::vector<std::vector<cplx>> Ut(nth, std::vector<cplx>(usz, cplx(0)));
     std::vector<uint64_t> miss(nth, 0), nvtx(nth, 0);
     std::vector<int> errs(nth, 0);
+    if (pb.tauCut > 0) {
+        // the relative attenuation cut needs each batch's reference before its terms: batch by
+        // batch, transport of the whole batch, then its tauReference, then the terms
+        std::vector<std::vector<Vertex>> batch(pb.P);
+        std::vector<uint32_t> ms(pb.P);
+        for (uint32_t bi = 0; bi < pb.fullIter; ++bi) {
+            const uint64_t b0 = (uint64_t)bi * pb.P;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nth) schedule(dynamic, 16)
+#endif
+            for (int64_t i = 0; i < (int64_t)pb.P; ++i) {
+                int tid = 0;
+#ifdef _OPENMP
+                tid = omp_get_thread_num();
+#endif
+                int rc = pb.transport(b0 + (uint64_t)i, batch[i], &ms[i]);
+                if (rc) errs[tid] = rc;
+            }
+            for (int t = 0; t < nth; ++t)
+                if (errs[t]) return errs[t];
+            pb.tauRef = pb.tauReference(batch);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nth) schedule(static)
+#endif
+            for (int64_t i = 0; i < (int64_t)pb.P; ++i) {
+                int tid = 0;
+#ifdef _OPENMP
+                tid = omp_get_thread_num();
+#endif
+                Problem::Terms tm;
+                miss[tid] += ms[i];
+                nvtx[tid] += batch[i].size();
+                pb.terms(batch[i], tm, nullptr);
+                pb.accumulate(tm, Ut[tid]);
+            }
+        }
+    } else
 #ifdef _OPENMP
 #pragma omp parallel num_threads(nth)
 #endif
@@ -1588,6 +1659,7 @@ int oracle_paths(const sstmc_config* cfg, uint64_t begin, uint32_t n, uint64_t* 
         }
     }
     path_off[n] = off;
+    if (!term_a && !term_b && !term_s) return SSTMC_NO_ERROR;   // vertices only
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 4)
 #endif
@@ -1617,13 +1689,50 @@ int oracle_paths(const sstmc_config* cfg, uint64_t begin, uint32_t n, uint64_t* 
 int oracle_terms_from_vertices(const sstmc_config* cfg, uint32_t n, const uint32_t* path_k,
                                const uint64_t* path_off, const double* vtx_pos, const uint32_t* vtx_mat,
                                const double* vtx_rho, const double* vtx_xt, const double* vtx_len,
-                               double* term_a, double* term_b, double* term_s) {
+                               double* term_a, double* term_b, double* term_s, double* term_s_sum,
+                               double tau_ref) {
     Problem pb;
     int err = build(cfg, pb);
     if (err) return err;
     uint32_t nl = pb.L(), nv = pb.V(), nt = pb.T();
+    // the relative attenuation cut: the given reference (NaN: this batch's own tauReference)
+    if (pb.tauCut > 0) {
+        if (std::isnan(tau_ref)) {
+            std::vector<std::vector<Vertex>> batch(n);
+            for (uint32_t i = 0; i < n; ++i)
+                for (uint32_t k = 0; k < path_k[i]; ++k) {
+                    const uint64_t o = path_off[i] + k;
+                    Vertex v;
+                    v.x = V3(vtx_pos[3 * o], vtx_pos[3 * o + 1], vtx_pos[3 * o + 2]);
+                    v.xt.assign(1, V3(vtx_xt[o * nt * 3], vtx_xt[o * nt * 3 + 1], vtx_xt[o * nt * 3 + 2]));
+                    batch[i].push_back(v);
+                }
+            pb.tauRef = pb.tauReference(batch);
+        } else {
+            pb.tauRef = tau_ref;
+        }
+    }
+    // term_s_sum: sum over the n paths of the single-scattering terms, [t][v][l] complex, for
+    // problems whose per-path terms are too large to return (each thread sums a contiguous range
+    // of paths; the partial sums are added in thread order, so the result depends only on n and
+    // the thread count)
+    const size_t nsum = (size_t)nt * nv * nl;
+    int nthr = 1;
 #ifdef _OPENMP
-#pragma omp parallel for schedule(dynamic, 4)
+    if (term_s_sum) nthr = omp_get_max_threads();
+#endif
+    std::vector<std::vector<cplx>> part(term_s_sum ? nthr : 0);
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthr)
+#endif
+    {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    if (term_s_sum) part[tid].assign(nsum, cplx(0));
+#ifdef _OPENMP
+#pragma omp for schedule(static)
 #endif
     for (int64_t i = 0; i < (int64_t)n; ++i) {
         std::vector<Vertex> vs(path_k[i]);
@@ -1655,6 +1764,22 @@ int oracle_terms_from_vertices(const sstmc_config* cfg, uint32_t n, const uint32
                 term_s[2 * ((size_t)i * nt * nv * nl + j)] = tm.S[j].real();
                 term_s[2 * ((size_t)i * nt * nv * nl + j) + 1] = tm.S[j].imag();
             }
+        if (term_s_sum) {
+            // correlation output: the path's single scattering enters as S(0) conj S(t), as in
+            // Problem::accumulate
+            const bool corr = pb.output == SSTMC_OUTPUT_CORRELATION;
+            for (size_t j = 0; j < tm.S.size() && j < nsum; ++j)
+                part[tid][j] += corr ? tm.S[j % ((size_t)nv * nl)] * std::conj(tm.S[j]) : tm.S[j];
+        }
+    }
+    }
+    if (term_s_sum) {
+        for (size_t j = 0; j < nsum; ++j) {
+            cplx acc(0);
+            for (int q = 0; q < nthr; ++q) acc += part[q][j];
+            term_s_sum[2 * j] = acc.real();
+            term_s_sum[2 * j + 1] = acc.imag();
+        }
     }
     return SSTMC_NO_ERROR;
 }

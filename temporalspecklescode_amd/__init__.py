@@ -129,6 +129,12 @@ class Tracer:
         _native.check(_native.lib().sstmc_debug_cbs(self._h, C.c_uint32(n), _p(ar), _p(br)), "sstmc_debug_cbs")
         return ar[..., 0] + 1j * ar[..., 1], br[..., 0] + 1j * br[..., 1]
 
+    def debug_tau_ref(self) -> float:
+        """Reference optical depth of the relative attenuation cut in the last batch traced."""
+        r = C.c_float()
+        _native.check(_native.lib().sstmc_debug_tau_ref(self._h, C.byref(r)), "sstmc_debug_tau_ref")
+        return float(r.value)
+
     def debug_contract(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
         """u[l, v, t] = sum_p a[p, t, l] b[p, t, v] with this tracer's contraction kernel."""
         n = a.shape[0]

@@ -25,7 +25,8 @@
 //   sampler: type "TemporalSampler" | "TemporalCorrelationSampler", t, D, U{x,y,z},
 //            isTemporalAmplitude; the correlation sampler also takes illumination2 and view2
 //   tracer: type "nee", isCBS, output "field" | "correlation"
-//   precision (this implementation): terms "fp32"|"fp64", contraction "bf16x3"|"fp32"|"fp64"
+//   precision (this implementation): terms "fp32"|"fp64", contraction "bf16x3"|"fp32"|"fp64",
+//   tauCut (optical depth past which an NEE term is dropped; 0, the default, is exact)
 //   dims (optional; else --dims; else 3 when medium.zAxis is present)
 #include <chrono>
 #include <cmath>
@@ -387,6 +388,7 @@ void fill(HostConfig& h, const Value& cfg, uint32_t dims) {
     else if (con == "fp32") s.precision.contraction = SSTMC_CONTRACT_FP32;
     else if (con == "fp64") s.precision.contraction = SSTMC_CONTRACT_FP64;
     else throw ConfigError("precision.contraction must be bf16x3, fp32 or fp64");
+    s.precision.tauCut = numOr(pr, "tauCut", 0.0);
 }
 
 // .npy v1.0, complex128, Fortran order

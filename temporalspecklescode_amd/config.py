@@ -5,7 +5,8 @@ MatlabInterface.cu:274-1563): ``simulation``, ``medium``, ``scattering``, ``illu
 ``view``, ``sampler``, ``tracer``.  Numeric arrays may be lists or numpy arrays;
 ``medium.materialGrid`` may be an (nx, ny[, nz]) array indexed like MATLAB (x fastest in memory
 after Fortran-order flattening) or an already-flat vector.  Extra, MI355X-specific keys live
-under ``config['precision']`` (``terms``: 'fp32'|'fp64', ``contraction``: 'bf16x3'|'fp32'|'fp64')
+under ``config['precision']`` (``terms``: 'fp32'|'fp64', ``contraction``: 'bf16x3'|'fp32'|'fp64',
+``tauCut``: the optical depth past which an NEE term is dropped, 0 = exact)
 and ``config['tracer']['output']`` ('field'|'correlation').  ``sampler.type`` is
 'TemporalSampler' (default) or 'TemporalCorrelationSampler', which takes ``illumination2`` and
 ``view2`` as the second source pair.
@@ -70,7 +71,7 @@ class Tracer(C.Structure):
 
 
 class Precision(C.Structure):
-    _fields_ = [("terms", C.c_int32), ("contraction", C.c_int32)]
+    _fields_ = [("terms", C.c_int32), ("contraction", C.c_int32), ("tauCut", C.c_double)]
 
 
 class Config(C.Structure):
@@ -290,6 +291,10 @@ class CConfig:
         s.precision.terms = {"fp32": TERMS_FP32, "fp64": TERMS_FP64}[pr.get("terms", "fp32")]
         s.precision.contraction = {"bf16x3": CONTRACT_BF16X3, "fp32": CONTRACT_FP32,
                                    "fp64": CONTRACT_FP64}[pr.get("contraction", "bf16x3")]
+        cut = float(pr.get("tauCut", 0.0))
+        if not (cut >= 0 and np.isfinite(cut)):
+            raise ValueError("precision.tauCut must be a finite non-negative optical depth (0: exact)")
+        s.precision.tauCut = cut
 
     def _source(self, dst, src, dims, name):
         ty = src["type"]

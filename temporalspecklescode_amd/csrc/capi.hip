@@ -153,6 +153,7 @@ static int validate(const sstmc_config* c) {
     if (c->precision.terms != SSTMC_TERMS_FP32 && c->precision.terms != SSTMC_TERMS_FP64) return SSTMC_INVALID_CONFIG;
     if (c->precision.contraction < SSTMC_CONTRACT_BF16X3 || c->precision.contraction > SSTMC_CONTRACT_FP64)
         return SSTMC_INVALID_CONFIG;
+    if (!(c->precision.tauCut >= 0) || !std::isfinite(c->precision.tauCut)) return SSTMC_INVALID_CONFIG;
     return SSTMC_NO_ERROR;
 }
 
@@ -257,6 +258,7 @@ static int create(const sstmc_config* c, sstmc_handle** out) {
     opt.cbs = c->tracer.isCBS ? 1 : 0;
     opt.terms = c->precision.terms;
     opt.contraction = c->precision.contraction;
+    opt.tauCut = c->precision.tauCut;
     h->tracer.reset(new NextEventEstimation(&err, *h->medium, *h->scattering, *h->illumination, *h->view,
                                             *h->sampler, opt));
     if (err) return err;
@@ -368,6 +370,12 @@ int sstmc_debug_cbs(sstmc_handle* h, uint32_t npaths, double* term_ar, double* t
     if (!h) return SSTMC_INVALID_CONFIG;
     hipSetDevice(h->device);
     return h->tracer->debugCbsTerms(npaths, term_ar, term_br);
+}
+
+int sstmc_debug_tau_ref(const sstmc_handle* h, float* tau_ref) {
+    if (!h || !tau_ref) return SSTMC_INVALID_CONFIG;
+    hipSetDevice(h->device);
+    return h->tracer->tauRef(tau_ref);
 }
 
 int sstmc_debug_contract(sstmc_handle* h, uint32_t npaths, const double* a, const double* b, double* u) {

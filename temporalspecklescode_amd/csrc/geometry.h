@@ -66,6 +66,19 @@ __host__ __device__ inline int locate(const MediumView& m, int a, R x) {
     return lo;
 }
 
+// locate for an equally spaced axis (m.uniformAxis[a]): the same cell as locate, without the
+// general binary search (the fast walk only runs on grids whose every axis is equally spaced).
+template <typename R>
+__host__ __device__ inline int locateUniform(const MediumView& m, int a, R x) {
+    const R* e = Real<R>::edges(m, a);
+    const int n = (int)m.n[a];
+    const float f = ((float)x - m.e0[a]) * m.invH[a];
+    int b = f <= 0.f ? 0 : (f >= m.nf[a] ? n - 1 : (int)f);
+    if (b > 0 && x < e[b]) --b;
+    if (b < n - 1 && x >= e[b + 1]) ++b;
+    return b;
+}
+
 __host__ __device__ inline uint32_t cellIndex(const MediumView& m, int bx, int by, int bz) {
     return (uint32_t)bx + m.n[0] * ((uint32_t)by + m.n[1] * (uint32_t)bz);
 }
@@ -331,12 +344,44 @@ struct WalkWords {
 // sigt(j-1); each read has a whole step of independent work to hide behind. The sum is formed in
 // the same order as a sequential walk and the tauCut test sees the same partial sums, so the
 // result is bit-identical to one.
+// The direction-only part of the walk's set-up: per axis the inverse direction, the parameter
+// step between faces, the walk-grid bit stride (two's complement) and the stepping sense. A
+// kernel whose lanes each walk one direction from many points forms it once per lane.
+struct WalkDir {
+    float inv[3], td[3];
+    uint32_t stride[3];
+    int up[3];
+};
+template <int DIMS>
+__host__ __device__ inline WalkDir walkDir(const MediumView& m, const f3& d) {
+    const float dd[3] = {d.x, d.y, DIMS == 3 ? d.z : 0.f};
+    WalkDir w;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        w.inv[a] = fabsf(dd[a]) < 1e-12f ? (dd[a] >= 0.f ? 1e12f : -1e12f) : 1.0f / dd[a];
+        w.up[a] = dd[a] >= 0.f;
+        w.td[a] = m.h[a] * fabsf(w.inv[a]);
+        w.stride[a] = w.up[a] ? m.walkStride[a] : 0u - m.walkStride[a];
+    }
+    return w;
+}
+
+template <int DIMS>
+__host__ __device__ inline float opticalDepthFastDir(const MediumView& m, const float* sig, const f3& p,
+                                                     const WalkDir& wd, float tauCut, uint32_t* steps);
+
 template <int DIMS>
 __host__ __device__ inline float opticalDepthFast(const MediumView& m, const float* sig, const f3& p, const f3& d,
                                                   float tauCut, uint32_t* steps) {
+    return opticalDepthFastDir<DIMS>(m, sig, p, walkDir<DIMS>(m, d), tauCut, steps);
+}
+
+// The walk from p with the direction part wd already formed (walkDir).
+template <int DIMS>
+__host__ __device__ inline float opticalDepthFastDir(const MediumView& m, const float* sig, const f3& p,
+                                                     const WalkDir& wd, float tauCut, uint32_t* steps) {
     if (!inside<DIMS, float>(m, p)) return 0.f;
     const float pp[3] = {p.x, p.y, DIMS == 3 ? p.z : 0.f};
-    const float dd[3] = {d.x, d.y, DIMS == 3 ? d.z : 0.f};
     float tf[3] = {3.0e38f, 3.0e38f, 3.0e38f}, td[3] = {0.f, 0.f, 0.f};
     uint32_t stride[3] = {0u, 0u, 0u};   // walk-grid bit strides in two's complement
     uint32_t cell = m.walkOrigin;        // bit offset of the current cell
@@ -344,14 +389,16 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
 #pragma unroll
     for (int a = 0; a < DIMS; ++a) {
         const int n = (int)m.n[a];
-        const int b = locate<float>(m, a, pp[a]);
-        const float inv = fabsf(dd[a]) < 1e-12f ? (dd[a] >= 0.f ? 1e12f : -1e12f) : 1.0f / dd[a];
-        const int up = dd[a] >= 0.f;
-        tf[a] = (m.ef[a][b + up] - pp[a]) * inv;
-        td[a] = m.h[a] * fabsf(inv);
+        const int b = locateUniform<float>(m, a, pp[a]);   // the fast walk runs on equally spaced grids
+        const int up = wd.up[a];
+        // both faces of the cell (edges hold n + 1 entries), then the one ahead: when p is shared
+        // by a wavefront's walks these are two scalar loads rather than a gather
+        const float elo = m.ef[a][b], ehi = m.ef[a][b + 1];
+        tf[a] = ((up ? ehi : elo) - pp[a]) * wd.inv[a];
+        td[a] = wd.td[a];
         // faces crossed before the one that leaves the box along this axis
         tExit = fminf(tExit, fmaf((float)(up ? n - 1 - b : b), td[a], tf[a]));
-        stride[a] = up ? m.walkStride[a] : 0u - m.walkStride[a];
+        stride[a] = wd.stride[a];
         cell += (uint32_t)b * m.walkStride[a];
     }
     const WalkWords words(m);

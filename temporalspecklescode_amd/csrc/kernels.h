@@ -35,8 +35,14 @@ struct NeeParams {
     double lambda;
     int output;            // 0 field, 1 correlation
     int singleMode;
-    float tauCutF;         // early stop of far-field walks: exp(-tau/2) is exactly 0 in FP32 beyond it
-    double tauCutD;
+    // Walks stop once tau passes a cut past which the term they form is exactly zero: FP32 210
+    // (exp(-tau/2) is 0 in FP32 beyond it), FP64 never. With precision.tauCut = delta > 0 the
+    // unweighted view-side terms (b, B: g of a view) use the relative cut tauRef[0] + delta, where
+    // tauRef[0] is the batch's least view optical depth found by k_tau_ref; terms that carry the
+    // path weight (a, A, Ar), the reversed path's Br and the path probabilities stay exact.
+    float exactCutF;
+    const float* tauRef;   // device scalar, or null (no relative cut)
+    float tauCutRel;       // delta
     uint32_t npaths;
     uint32_t lpp;          // k_nee: lanes per path (power of two, 8..64), set by launchNee
     uint32_t lppView;      // k_nee_view: lanes per work unit (power of two, 8..64), set by launchNee
@@ -50,11 +56,13 @@ hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc,
                            const FramesView& fr, const VertexBatch& vb, uint64_t seed, uint64_t pathBegin,
                            uint32_t npaths, int write, hipStream_t st);
 hipError_t launchScan(const uint32_t* k, uint32_t* off, uint32_t* blockSums, uint32_t n, hipStream_t st);
+// Adds a batch's misses and vertices to acc[0], acc[1] and its vertex count to the maximum in
+// acc[3]; a batch whose vertices exceed `cap` is flagged in the `overflow` bitmap instead.
 hipError_t launchBatchTotals(const uint32_t* misses, const uint32_t* off, uint32_t n, unsigned long long* acc,
-                             hipStream_t st);
+                             uint64_t cap, uint32_t* overflow, uint32_t batch, float* tauRef, hipStream_t st);
 
-// k_nee (first vertex) then k_nee_view (vertices k >= 2); `between`, when not null, is recorded
-// on `st` between the two launches.
+// [k_tau_ref when p.tauRef is set,] k_nee (first vertex) then k_nee_view (vertices k >= 2);
+// `between`, when not null, is recorded on `st` between the two launches.
 hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
                      const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
                      const NeeOut& out, hipStream_t st, hipEvent_t between = nullptr);

@@ -170,12 +170,12 @@ __device__ inline void gFar(const MediumView& m, const R* sig, const d3& dd, con
     if constexpr (sizeof(R) == 4) w = illum ? mk(-df.x, -df.y, -df.z) : df;
     else w = illum ? mkd(-dd.x, -dd.y, -dd.z) : dd;
     const R tau = walk<DIMS, R>(m, sig, p, w, tauCut, steps);
-    *amp = expR<R>(-(R)0.5 * tau);
+    *amp = tau > tauCut ? (R)0 : expR<R>(-(R)0.5 * tau);
 }
 // PointSource: gPointKernel (PointSource.cuh:117-172), from the source location sp (z = 0 in 2D).
 template <int DIMS, typename R>
 __device__ inline void gPoint(const MediumView& m, const R* sig, const d3& sp, bool area, const Geo& x,
-                              double invLambda, uint32_t* steps, R* amp, double* ph) {
+                              double invLambda, R tauCut, uint32_t* steps, R* amp, double* ph) {
     typedef typename Pos<R>::V V;
     const V p = Pos<R>::pick(x.pf, x.pd);
     const d3 q = dsub(sp, x.pd);
@@ -183,10 +183,10 @@ __device__ inline void gPoint(const MediumView& m, const R* sig, const d3& sp, b
     const double rr = 1.0 / r;
     *ph = r * invLambda;
     const V w = Pos<R>::fromD(mkd(q.x * rr, q.y * rr, q.z * rr));
-    const R tau = walk<DIMS, R>(m, sig, p, w, (R)3e38, steps);
+    const R tau = walk<DIMS, R>(m, sig, p, w, tauCut, steps);
     double fac = DIMS == 3 ? rr * 0.531125966013598 /* sqrt(ISOTROPIC_AMPLITUDE) */ : sqrt(rr);
     if (area) fac *= sqrt(rr * fabs(DIMS == 3 ? x.pd.z - sp.z : x.pd.y - sp.y));
-    *amp = expR<R>(-(R)0.5 * tau) * (R)fac;
+    *amp = tau > tauCut ? (R)0 : expR<R>(-(R)0.5 * tau) * (R)fac;
 }
 // FP64 direction or location of source i (z = 0 for a 2D location) and its FP32 direction.
 template <int DIMS>
@@ -206,7 +206,7 @@ __device__ inline void gGauss(const MediumView& m, const R* sig, const d3& dir, 
                               uint32_t* steps, R* amp, double* ph) {
     const d3 w = illum ? mkd(-dir.x, -dir.y, -dir.z) : dir;
     const R tau = walk<DIMS, R>(m, sig, Pos<R>::pick(x.pf, x.pd), Pos<R>::fromD(w), tauCut, steps);
-    *amp = gbMaterial(m, x.pd) ? expR<R>(-(R)0.5 * tau) : (R)0;
+    *amp = (gbMaterial(m, x.pd) && !(tau > tauCut)) ? expR<R>(-(R)0.5 * tau) : (R)0;
     *ph = 0;
 }
 template <int DIMS, typename R, bool GB = false>
@@ -219,7 +219,7 @@ __device__ inline void gTerm(const MediumView& m, const R* sig, const SourceView
     if (s.type == 2)
         gFar<DIMS, R>(m, sig, sourceD<DIMS>(s, i), sizeof(R) == 4 ? sourceF(s, i) : mk(0.f, 0.f, 0.f), s.illum != 0,
                       x, invLambda, tauCut, steps, amp, ph);
-    else gPoint<DIMS, R>(m, sig, sourceD<DIMS>(s, i), s.area != 0, x, invLambda, steps, amp, ph);
+    else gPoint<DIMS, R>(m, sig, sourceD<DIMS>(s, i), s.area != 0, x, invLambda, tauCut, steps, amp, ph);
 }
 
 // Incoming unit direction at x from source i as the three-point functions use it: FarField
@@ -255,6 +255,16 @@ template <typename T> __device__ inline T groupSum(T v, uint32_t lpp) {
 
 __device__ inline double frac1(double x) { return x - floor(x); }
 
+// The cut of the unweighted view-side terms (b, B): exact (FP32 210, past which exp(-tau/2) is
+// 0 in FP32; FP64 none), or the relative cut tauRef + delta of precision.tauCut.
+template <typename R>
+__device__ inline R viewCut(const NeeParams& prm) {
+    const R exact = sizeof(R) == 4 ? (R)prm.exactCutF : (R)1e300;
+    if (!prm.tauRef) return exact;
+    const R c = (R)(*prm.tauRef) + (R)prm.tauCutRel;
+    return c < exact ? c : exact;
+}
+
 template <typename R>
 __device__ inline typename Cx<R>::T cis(R amp, double phaseCycles) {
     R s, c;
@@ -273,7 +283,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
     const uint32_t P = prm.npaths, L = il.n, V = vw.n, NT = fr.nt;
     const bool corr = prm.output == 1;
     const double invLambda = 1.0 / prm.lambda;
-    const R tauCutFF = sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD;
+    const R tauCutFF = viewCut<R>(prm);
     C* A = (C*)out.A;
     C* B = (C*)out.B;
     C* a = (C*)out.a;
@@ -311,6 +321,11 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
     const d3 x2cd = mkd(x2m.x, x2m.y, x2m.z);
     const d3 seg12 = dnormalize<DIMS>(dsub(x2cd, x1c.pd));
     const R tauInf = (R)3e38;
+    // Illumination terms at x1 (a, A) and the reversed path's view term at x1 (Ar) carry the path
+    // weight 1 / sqrt(p), which the same attenuations set: a deep x1 has tiny g and a huge weight,
+    // and its terms are O(1). They are walked exactly. The cut of precision.tauCut applies to the
+    // unweighted terms only: b (and the view kernels' B, Br).
+    const R tauExact = sizeof(R) == 4 ? (R)prm.exactCutF : tauInf;
 
     // ---- path probabilities from the central first vertex (pathProbabilitySingle/Multiple)
     R s1 = 0, s2 = 0;
@@ -335,7 +350,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
             typename Pos<R>::V w;
             if (sizeof(R) == 4) w = Pos<R>::fromD(mkd(-(double)il.pf[3 * l], -(double)il.pf[3 * l + 1], -(double)il.pf[3 * l + 2]));
             else w = Pos<R>::fromD(mkd(-il.pd[3 * l], -il.pd[3 * l + 1], -il.pd[3 * l + 2]));
-            R tau = walk<DIMS, R>(m, sig, p1, w, sizeof(R) == 4 ? (R)prm.tauCutF : tauInf, &steps);
+            R tau = walk<DIMS, R>(m, sig, p1, w, sizeof(R) == 4 ? (R)prm.exactCutF : tauInf, &steps);
             pr = expR<R>(-tau) * (sizeof(R) == 4 ? (R)il.probf[l] : (R)il.probd[l]);
         } else {
             d3 sp = mkd(il.pd[3 * l], il.pd[3 * l + 1], DIMS == 3 ? il.pd[3 * l + 2] : 0.0);
@@ -369,7 +384,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         for (int f = 0; f < nf; ++f) {
             uint32_t tt = (corr && f == 0) ? 0 : t;
             Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
-            gTerm<DIMS, R, gauss>(m, sig, il, l, x1t, invLambda, il.type != 1 ? tauCutFF : tauInf, &steps, &ga[f],
+            gTerm<DIMS, R, gauss>(m, sig, il, l, x1t, invLambda, tauExact, &steps, &ga[f],
                                   &gp[f]);
             if (multi) {
                 Geo x2t = vertexFrame(vb, fr, off + 1, tt, x2m);
@@ -417,7 +432,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         for (int f = 0; f < nf; ++f) {
             uint32_t tt = (corr && f == 0) ? 0 : t;
             Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
-            gTerm<DIMS, R, gauss>(m, sig, vw, v, x1t, invLambda, vw.type != 1 ? tauCutFF : tauInf, &steps, &ga[f],
+            gTerm<DIMS, R, gauss>(m, sig, vw, v, x1t, invLambda, Ar ? tauExact : tauCutFF, &steps, &ga[f],
                                   &gp[f]);
         }
         const size_t o = ((size_t)t * P + p) * V + v;
@@ -490,7 +505,7 @@ __device__ inline void nee_path_corr(const MediumView& m, const R* sig, const Sc
     const uint32_t P = prm.npaths, L = il.n / 2, V = vw.n / 2, NT = fr.nt;
     const double invLambda = 1.0 / prm.lambda;
     const double kw2 = 39.478417604357434 * invLambda * invLambda;   // (2 pi / lambda)^2
-    const R tauCutFF = sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD;
+    const R tauCutFF = viewCut<R>(prm);
     const R tauInf = (R)3e38;
     C* A = (C*)out.A;
     C* B = (C*)out.B;
@@ -530,7 +545,7 @@ __device__ inline void nee_path_corr(const MediumView& m, const R* sig, const Sc
             typename Pos<R>::V w;
             if (sizeof(R) == 4) w = Pos<R>::fromD(mkd(-(double)il.pf[3 * l], -(double)il.pf[3 * l + 1], -(double)il.pf[3 * l + 2]));
             else w = Pos<R>::fromD(mkd(-il.pd[3 * l], -il.pd[3 * l + 1], -il.pd[3 * l + 2]));
-            R tau = walk<DIMS, R>(m, sig, p1, w, sizeof(R) == 4 ? (R)prm.tauCutF : tauInf, &steps);
+            R tau = walk<DIMS, R>(m, sig, p1, w, sizeof(R) == 4 ? (R)prm.exactCutF : tauInf, &steps);
             pr = expR<R>(-tau) * (sizeof(R) == 4 ? (R)il.probf[l] : (R)il.probd[l]);
         } else {
             d3 sp = mkd(il.pd[3 * l], il.pd[3 * l + 1], DIMS == 3 ? il.pd[3 * l + 2] : 0.0);
@@ -550,7 +565,8 @@ __device__ inline void nee_path_corr(const MediumView& m, const R* sig, const Sc
     const R w1 = p1 > 0 ? (R)1 / p1 : (R)0;
     const R w2 = p2 > 0 ? (R)1 / p2 : (R)0;
     const bool isoPointIl = il.type == 1 && sc.allIsotropic;
-    const R tauIl = il.type == 2 ? tauCutFF : tauInf, tauVw = vw.type == 2 ? tauCutFF : tauInf;
+    // illumination terms carry the path weight: walked exactly; view terms take the cut
+    const R tauIl = sizeof(R) == 4 ? (R)prm.exactCutF : tauInf, tauVw = tauCutFF;
 
     for (uint32_t i = sub; i < L * NT; i += lpp) {
         const uint32_t l = i % L, t = i / L;
@@ -685,7 +701,7 @@ __device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, 
     }
     R ga;
     double gp;
-    if constexpr (SRC == kSrcPoint) gPoint<DIMS, R>(m, sig, vd, vw.area != 0, xk, invLambda, steps, &ga, &gp);
+    if constexpr (SRC == kSrcPoint) gPoint<DIMS, R>(m, sig, vd, vw.area != 0, xk, invLambda, tauCut, steps, &ga, &gp);
     else gFar<DIMS, R>(m, sig, vd, vf, SRC == kSrcFarIllum, xk, invLambda, tauCut, steps, &ga, &gp);
     R fv;
     if (isoPoint) {
@@ -754,7 +770,7 @@ k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBat
     const uint32_t K = (p < P && poolOk) ? vb.k[p] : 0u;   // K < 2: k_nee wrote the zero rows
     const uint64_t off = p < P ? vb.off[p] : 0;
     const double invLambda = 1.0 / prm.lambda;
-    const R tauCut = SRC != kSrcPoint ? (sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD) : (R)3e38;
+    const R tauCut = viewCut<R>(prm);
     C* B = (C*)out.B;
     const bool isoPoint = SRC == kSrcPoint && sc.allIsotropic;
     uint32_t steps = 0;
@@ -780,6 +796,85 @@ k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBat
     if ((threadIdx.x & 63) == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
 }
 
+// HG amplitude constants per material for the far-field view kernel: sqrt((1 - g^2) / 4 pi),
+// (1 - g)^2 and 2 g, so that |f| = C den^(-3/4) with den = (1 - g)^2 + 2 g (1 - c). An isotropic
+// material is g = 0.
+struct HgConst {
+    float c, a, b;
+};
+
+// Multiple-scattering view terms B for far-field sources on equally spaced grids, FP32 terms,
+// HG or isotropic materials, one wavefront per work unit (the common case of the measured
+// transmission-matrix workload; other cases take k_nee_view). Same unit and sums as k_nee_view,
+// with the per-(vertex, source) work cut to what depends on both:
+//   * the walk's direction part (inverse direction, face step, bit stride), the FP64 phase
+//     direction d / lambda and the HG incoming direction are formed once per lane, before the
+//     vertex loop; the vertex (wave-uniform) part of the walk's set-up is one locate per axis;
+//   * 1 - cos of the HG amplitude is 0.5 |a - b|^2 in FP32, which keeps its relative accuracy in
+//     the forward peak where 1 + g^2 - 2 g c cancels; |f| = C rsqrt(den) sqrt(rsqrt(den));
+//   * exp(i 2 pi phase) is the hardware sine and cosine (argument in turns) of the FP64-reduced
+//     phase (measured on gfx950: within 1.3e-7 of sin, cos over [0, 1));
+//   * a vertex whose every lane's term is zero (past the attenuation cut, or outside) skips the
+//     amplitude and phase.
+template <int SRC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+k_nee_view_far(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBatch vb, NeeParams prm, NeeOut out) {
+    static_assert(SRC == kSrcFarView || SRC == kSrcFarIllum, "far-field sources only");
+    __shared__ float sigLds[kSigLds];
+    __shared__ HgConst hg[kMaxMaterials];
+    if (threadIdx.x < kMaxMaterials) {
+        const double g = sc.type[threadIdx.x] == 3 ? sc.gd[threadIdx.x] : 0.0;
+        hg[threadIdx.x] = HgConst{(float)sqrt((1.0 - g * g) / 12.566370614359172954), (float)((1.0 - g) * (1.0 - g)),
+                                  (float)(2.0 * g)};
+    }
+    const float* sig = loadSigTable<float>(m, sigLds);   // synchronises the block
+    const ViewUnit u = viewUnit<true>(prm);
+    const uint32_t p = u.p, t = u.chunk, P = prm.npaths, V = vw.n;
+    const bool poolOk = (uint64_t)vb.off[P] <= vb.cap;
+    const uint32_t K = (p < P && poolOk) ? vb.k[p] : 0u;
+    const uint64_t off = p < P ? vb.off[p] : 0;
+    const float tauCut = viewCut<float>(prm);
+    float2* B = (float2*)out.B;
+    uint32_t steps = 0;
+    const uint32_t v = u.blk * 64u + u.sub;
+    if (v < V && K >= 2) {
+        const d3 vd = sourceD<3>(vw, v);
+        const f3 vf = sourceF(vw, v);
+        // walk along the view direction (view) or against it (illumination); the phase is
+        // -d.x / lambda (view) or +d.x / lambda (illumination); the HG incoming direction is -d
+        // (view) or d (illumination) (FarField.cuh:223-263, 546-599)
+        const float sgn = SRC == kSrcFarIllum ? -1.f : 1.f;
+        const WalkDir wd = walkDir<3>(m, mk(sgn * vf.x, sgn * vf.y, sgn * vf.z));
+        const double ks = (SRC == kSrcFarIllum ? 1.0 : -1.0) / prm.lambda;
+        const d3 kd = mkd(vd.x * ks, vd.y * ks, vd.z * ks);
+        const float as = SRC == kSrcFarIllum ? 1.f : -1.f;
+        const f3 av = mk(as * (float)vd.x, as * (float)vd.y, as * (float)vd.z);
+        float accRe = 0.f, accIm = 0.f;
+        for (uint32_t k = 1; k < K; ++k) {
+            const uint64_t ok = off + k;
+            const float4 xkm = vb.xm[ok];
+            const Geo xk = vertexFrame(vb, fr, ok, t, xkm);
+            const float tau = opticalDepthFastDir<3>(m, sig + kMaxMaterials, xk.pf, wd, tauCut, &steps);
+            const float amp = tau > tauCut ? 0.f : __expf(-0.5f * tau);
+            if (__ballot(amp != 0.f) == 0) continue;   // every term of this vertex is zero
+            const VertexRec rec = vb.rec[ok * fr.nt + t];
+            const HgConst h = hg[__float_as_uint(xkm.w)];
+            const float dx = av.x - (float)rec.bx, dy = av.y - (float)rec.by, dz = av.z - (float)rec.bz;
+            const float den = fmaf(h.b, 0.5f * fmaf(dx, dx, fmaf(dy, dy, dz * dz)), h.a);
+            const float r = __builtin_amdgcn_rsqf(den);
+            const float fv = h.c * r * __builtin_amdgcn_sqrtf(r);
+            const double ph = kd.x * xk.pd.x + kd.y * xk.pd.y + kd.z * xk.pd.z + rec.phase;
+            const float turns = (float)(ph - floor(ph));
+            const float a = amp * fv;
+            accRe = fmaf(a, __builtin_amdgcn_cosf(turns), accRe);
+            accIm = fmaf(a, __builtin_amdgcn_sinf(turns), accIm);
+        }
+        B[((size_t)t * P + p) * V + v] = make_float2(accRe, accIm);
+    }
+    uint32_t tot = waveSum(steps);
+    if ((threadIdx.x & 63) == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
+}
+
 // Correlation view terms sum_k G_k(0) conj G_k(t) (the per-sub-path estimator). Work unit =
 // (path, chunk of up to kCorrLags frames, block of lppView views): one view per lane; per vertex,
 // G_k(0) is formed once and reused for every frame of the chunk, whose running sums live in LDS
@@ -798,7 +893,7 @@ __global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView
     const uint64_t off = p < P ? vb.off[p] : 0;
     const uint32_t t0 = u.chunk * kCorrLags, nt = NT - t0 < kCorrLags ? NT - t0 : kCorrLags;
     const double invLambda = 1.0 / prm.lambda;
-    const R tauCut = SRC != kSrcPoint ? (sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD) : (R)3e38;
+    const R tauCut = viewCut<R>(prm);
     C* B = (C*)out.B;
     const bool isoPoint = SRC == kSrcPoint && sc.allIsotropic;
     uint32_t steps = 0;
@@ -841,7 +936,7 @@ __device__ inline void corrViewG(const MediumView& m, const R* sig, const Scatte
                                  double invLambda, R tauCut, bool isoPoint, uint32_t* steps, R* amp, double* ph) {
     R ga;
     double gp;
-    if constexpr (SRC == kSrcPoint) gPoint<DIMS, R>(m, sig, vd, vw.area != 0, xg, invLambda, steps, &ga, &gp);
+    if constexpr (SRC == kSrcPoint) gPoint<DIMS, R>(m, sig, vd, vw.area != 0, xg, invLambda, tauCut, steps, &ga, &gp);
     else gFar<DIMS, R>(m, sig, vd, vf, false, xg, invLambda, tauCut, steps, &ga, &gp);
     R fv;
     if (isoPoint) {
@@ -881,7 +976,7 @@ __global__ void __launch_bounds__(256) k_nee_view_corrsampler(MediumView m, Scat
     const uint32_t nt = t0 < NT ? (NT - t0 < kCorrLags ? NT - t0 : kCorrLags) : 0u;
     const double invLambda = 1.0 / prm.lambda;
     const double kw2 = 39.478417604357434 * invLambda * invLambda;
-    const R tauCut = SRC != kSrcPoint ? (sizeof(R) == 4 ? (R)prm.tauCutF : (R)prm.tauCutD) : (R)3e38;
+    const R tauCut = viewCut<R>(prm);
     C* B = (C*)out.B;
     const bool isoPoint = SRC == kSrcPoint && sc.allIsotropic;
     const bool same = prm.sameViews != 0;
@@ -965,6 +1060,77 @@ __global__ void __launch_bounds__(256) k_nee_view_corrsampler(MediumView m, Scat
     if ((threadIdx.x & 63) == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
 }
 
+// The reference of the relative attenuation cut: the batch's least view optical depth. One
+// wavefront per sampled path (every pathStride-th), one view per lane (every view when there are at
+// most 64, else 64 spread evenly), the path's vertices k >= 1 at frame 0, FP32 walks. A walk stops
+// once it passes the least depth found so far (by its lane, or by the batch as last read), which
+// cannot change the minimum: the least pair is walked to its end. The result is the exact minimum
+// over the pairs evaluated, hence at or above the least depth over all pairs, so the cut
+// tauRef + delta never drops a term that the same cut about the true least depth would keep.
+template <int DIMS, int SRC>
+__global__ void __launch_bounds__(256) k_tau_ref(MediumView m, SourceView vw, FramesView fr, VertexBatch vb,
+                                                 NeeParams prm, uint32_t pathStride) {
+    __shared__ float sigLds[kSigLds];
+    const float* sig = loadSigTable<float>(m, sigLds);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t unit = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+    const uint64_t p64 = (uint64_t)unit * pathStride;
+    const uint32_t P = prm.npaths;
+    const bool poolOk = (uint64_t)vb.off[P] <= vb.cap;
+    const uint32_t K = (p64 < P && poolOk) ? vb.k[(uint32_t)p64] : 0u;
+    const uint32_t n = vw.n;
+    unsigned int* ref = (unsigned int*)prm.tauRef;   // non-negative floats order as their bits
+    float best = 3.0e38f, published = 3.0e38f;
+    if (K > 0) {
+        const uint64_t off = vb.off[(uint32_t)p64];
+        const bool active = lane < n;
+        const uint32_t v = !active ? 0u : (n <= 64 ? lane : (uint32_t)(((uint64_t)lane * n) / 64u));
+        const d3 vd = sourceD<DIMS>(vw, v);
+        for (uint32_t k = 0; k < K; ++k) {
+            if ((k & 7u) == 0)
+                best = fminf(best, __uint_as_float(__hip_atomic_load(ref, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
+            const uint64_t o = off + k;
+            const Geo x = vertexFrame(vb, fr, o, 0, vb.xm[o]);
+            float tau = 3.0e38f;
+            if (active) {
+                f3 w;
+                if constexpr (SRC == kSrcPoint) {
+                    const d3 q = dsub(vd, x.pd);
+                    const double rr = 1.0 / sqrt(ddot<DIMS>(q, q));
+                    w = mk((float)(q.x * rr), (float)(q.y * rr), (float)(q.z * rr));
+                } else if constexpr (SRC == kSrcGauss) {
+                    const d3 b = gbDir(vw.g, v);
+                    w = mk((float)b.x, (float)b.y, (float)b.z);
+                } else {
+                    w = sourceF(vw, v);
+                }
+                tau = walk<DIMS, float>(m, sig, x.pf, w, best, nullptr);
+            }
+            best = fminf(best, tau);
+            if ((k & 7u) == 7u || k + 1 == K) {
+                float wb = best;
+#pragma unroll
+                for (int s2 = 32; s2 > 0; s2 >>= 1) wb = fminf(wb, __shfl_xor(wb, s2, 64));
+                if (lane == 0 && wb < published) {
+                    atomicMin(ref, __float_as_uint(wb));
+                    published = wb;
+                }
+            }
+        }
+    }
+}
+
+template <int DIMS>
+static void launchTauRef(const MediumView& m, const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
+                         const NeeParams& p, hipStream_t st) {
+    const uint32_t stride = p.npaths > 4096 ? p.npaths / 4096 : 1u;
+    const uint32_t units = (p.npaths + stride - 1) / stride;
+    const dim3 grid((units + 3) / 4), block(256);
+    if (vw.type == 2) hipLaunchKernelGGL((k_tau_ref<DIMS, kSrcFarView>), grid, block, 0, st, m, vw, fr, vb, p, stride);
+    else if (vw.type == 1) hipLaunchKernelGGL((k_tau_ref<DIMS, kSrcPoint>), grid, block, 0, st, m, vw, fr, vb, p, stride);
+    else if constexpr (DIMS == 3) hipLaunchKernelGGL((k_tau_ref<DIMS, kSrcGauss>), grid, block, 0, st, m, vw, fr, vb, p, stride);
+}
+
 // Lanes per view-kernel work unit for n sources: a wavefront, halved (down to 8) while half still
 // covers them, so that several units share a wavefront when there are few sources.
 static inline void viewLanes(uint32_t n, NeeParams* q) {
@@ -992,6 +1158,11 @@ static void launchView(bool corr, const MediumView& m, const ScatterView& sc, co
     } else if constexpr (SRC == kSrcGauss) {
         if (wave) SSTMC_VIEW(k_nee_view, true, true);
         else SSTMC_VIEW(k_nee_view, false, true);
+    } else if constexpr (DIMS == 3 && sizeof(R) == 4 && (SRC == kSrcFarView || SRC == kSrcFarIllum)) {
+        // far-field sources, FP32 terms: the specialised kernel when its preconditions hold
+        if (wave && !tab && m.walkFast) hipLaunchKernelGGL((k_nee_view_far<SRC>), grid, block, 0, st, m, sc, src, fr, vb, q, out);
+        else if (wave) { if (tab) SSTMC_VIEW(k_nee_view, true, true); else SSTMC_VIEW(k_nee_view, true, false); }
+        else { if (tab) SSTMC_VIEW(k_nee_view, false, true); else SSTMC_VIEW(k_nee_view, false, false); }
     } else if (wave) {
         if (tab) SSTMC_VIEW(k_nee_view, true, true);
         else SSTMC_VIEW(k_nee_view, true, false);
@@ -1022,6 +1193,7 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
                        hipStream_t st, hipEvent_t between) {
     const uint32_t ppb = 4 * (64 / p.lpp);   // paths per 256-thread block
     dim3 grid((p.npaths + ppb - 1) / ppb), block(256);
+    if (p.tauRef) launchTauRef<DIMS>(m, vw, fr, vb, p, st);   // the relative cut's reference
     if (fr.analytic) {
         // TemporalCorrelationSampler: the vertices never move, one record per vertex
         if (sc.tab) hipLaunchKernelGGL((k_nee_corr<DIMS, R, true>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
@@ -1053,6 +1225,7 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
         NeeParams q = p;
         viewLanes(il.n, &q);
         q.frameChunks = fr.nt;
+        q.tauRef = nullptr;   // the reversed path's Br stays exact
         NeeOut o = out;
         o.B = out.Br;
         if (il.type == 2) launchView<DIMS, R, kSrcFarIllum>(false, m, sc, il, fr, vb, q, o, st);

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/sstmc.h"
+#include "config_copy.h"
 #include "tracer.h"
 
 using namespace sstmc;
@@ -39,47 +40,6 @@ int configValidate(const sstmc_config* c);
 
 namespace {
 
-// Deep copy of a config: the caller may free its arrays after sstmc_submit returns.
-struct ConfigCopy {
-    sstmc_config c{};
-    std::vector<double> ax[3], sigs, siga, g, src[2][3], t, D, U[3];
-    std::vector<uint32_t> grid;
-    std::vector<int32_t> stype;
-
-    template <typename T>
-    static const T* keep(std::vector<T>& dst, const T* p, size_t n) {
-        if (!p) return nullptr;
-        dst.assign(p, p + n);
-        return dst.data();
-    }
-    explicit ConfigCopy(const sstmc_config& in) : c(in) {
-        int dims = (int)in.dims;
-        const sstmc_medium& m = in.medium;
-        c.medium.xAxis = keep(ax[0], m.xAxis, m.xAxisSize);
-        c.medium.yAxis = keep(ax[1], m.yAxis, m.yAxisSize);
-        c.medium.zAxis = dims == 3 ? keep(ax[2], m.zAxis, m.zAxisSize) : nullptr;
-        size_t ncell = (size_t)(m.xAxisSize - 1) * (m.yAxisSize - 1) * (dims == 3 ? m.zAxisSize - 1 : 1);
-        c.medium.materialGrid = keep(grid, m.materialGrid, ncell);
-        c.medium.sigs = keep(sigs, m.sigs, m.materialNum);
-        c.medium.siga = keep(siga, m.siga, m.materialNum);
-        c.scattering.type = keep(stype, in.scattering.type, in.scattering.materialNum);
-        c.scattering.g = keep(g, in.scattering.g, in.scattering.materialNum);
-        const sstmc_source* s[2] = {&in.illumination, &in.view};
-        sstmc_source* d[2] = {&c.illumination, &c.view};
-        for (int i = 0; i < 2; ++i) {
-            d[i]->x = keep(src[i][0], s[i]->x, s[i]->num);
-            d[i]->y = keep(src[i][1], s[i]->y, s[i]->num);
-            d[i]->z = dims == 3 ? keep(src[i][2], s[i]->z, s[i]->num) : nullptr;
-        }
-        uint32_t nmat = m.materialNum;
-        c.sampler.t = keep(t, in.sampler.t, in.sampler.tNum);
-        c.sampler.D = keep(D, in.sampler.D, nmat);
-        c.sampler.Ux = keep(U[0], in.sampler.Ux, nmat);
-        c.sampler.Uy = keep(U[1], in.sampler.Uy, nmat);
-        c.sampler.Uz = dims == 3 ? keep(U[2], in.sampler.Uz, nmat) : nullptr;
-    }
-};
-
 enum State { kQueued, kActive, kDone, kFailed, kFetched };
 
 struct Request {
@@ -87,14 +47,16 @@ struct Request {
     std::unique_ptr<ConfigCopy> cfg;
     State state = kQueued;
     int error = SSTMC_NO_ERROR;
-    uint64_t footprint = 0;
+    uint64_t footprint = 0;            // bytes counted in the server's inUse while active
     sstmc_handle* handle = nullptr;
     NextEventEstimation* tracer = nullptr;
     uint32_t nextBatch = 0, batches = 0;
+    uint32_t renderings = 1, rendering = 0;   // renderingsNum; the one in flight
+    uint64_t seed = 0;
     std::deque<hipEvent_t> inflight;
-    double* result = nullptr;          // device copy of the field after completion
-    size_t resultDoubles = 0;
-    double iter = 0;
+    double* result = nullptr;          // device: renderingsNum fields, u(l, v, t, i) like sstmc_render
+    size_t fieldDoubles = 0;           // doubles of one rendering's field
+    std::vector<double> iter;          // one per rendering
     uint64_t paths = 0, contributions = 0;
     Clock::time_point submitted, done;
 };
@@ -121,7 +83,10 @@ struct sstmc_server {
     void run();
     void admit();
     int pump(Request& r);
-    void complete(Request& r);
+    void renderingDone(Request& r);
+    void fail(Request& r, int rc);
+    void release(Request& r);
+    void account(Request& r);
 };
 
 static uint64_t estimate(const sstmc_config& c) {
@@ -135,8 +100,50 @@ static uint64_t estimate(const sstmc_config& c) {
     }
     const uint64_t cells = (uint64_t)(c.medium.xAxisSize - 1) * (c.medium.yAxisSize - 1) *
                            (c.dims == 3 ? (uint64_t)(c.medium.zAxisSize - 1) : 1ull);
+    const uint32_t R = c.simulation.renderingsNum ? c.simulation.renderingsNum : 1;
     return NextEventEstimation::footprint(P, c.illumination.num, c.view.num, c.sampler.tNum, fp64, frames, cells,
-                                          c.tracer.isCBS != 0);
+                                          c.tracer.isCBS != 0) +
+           16ull * c.illumination.num * c.view.num * c.sampler.tNum * R;   // the result buffer
+}
+
+// Brings the request's share of inUse up to what its tracer holds now (the vertex pool is sized
+// from the first batch's exact vertex count, and grows after an overflow; the split-K workspaces
+// are allocated by the first contraction). Caller holds mu.
+void sstmc_server::account(Request& r) {
+    if (!r.tracer) return;
+    const uint64_t now = r.tracer->deviceBytes() + r.fieldDoubles * r.renderings * sizeof(double);
+    if (now > r.footprint) {
+        inUse += now - r.footprint;
+        r.footprint = now;
+    }
+}
+
+// Frees everything the request holds on the device except its finished result: the events of
+// batches in flight (after they complete), the tracer. Caller holds mu.
+void sstmc_server::release(Request& r) {
+    for (hipEvent_t e : r.inflight) {
+        hipEventSynchronize(e);
+        hipEventDestroy(e);
+    }
+    r.inflight.clear();
+    if (r.handle) sstmc_destroy(r.handle);
+    r.handle = nullptr;
+    r.tracer = nullptr;
+}
+
+void sstmc_server::fail(Request& r, int rc) {
+    const bool wasActive = r.state == kActive;
+    release(r);
+    if (r.result) hipFree(r.result);
+    r.result = nullptr;
+    if (wasActive) {
+        inUse -= r.footprint;
+        active.erase(std::find(active.begin(), active.end(), r.id));
+    }
+    r.state = kFailed;
+    r.error = rc ? rc : SSTMC_MISC_ERROR;
+    r.done = Clock::now();
+    cv.notify_all();
 }
 
 void sstmc_server::admit() {
@@ -148,81 +155,90 @@ void sstmc_server::admit() {
         sstmc_handle* h = nullptr;
         int rc = sstmc_create(&r.cfg->c, &h);
         if (rc) {
-            r.state = kFailed;
-            r.error = rc;
-            r.done = Clock::now();
-            cv.notify_all();
+            fail(r, rc);
             continue;
         }
         r.handle = h;
         r.tracer = handleTracer(h);
         r.batches = r.tracer->batches();
         r.nextBatch = 0;
-        rc = r.tracer->begin();
-        if (rc) {
-            r.state = kFailed;
-            r.error = rc;
-            sstmc_destroy(h);
-            r.handle = nullptr;
-            cv.notify_all();
-            continue;
-        }
+        r.rendering = 0;
+        r.fieldDoubles = 2ull * r.tracer->L() * r.tracer->V() * r.tracer->T();
+        r.iter.assign(r.renderings, 0.0);
         r.state = kActive;
         inUse += r.footprint;
         active.push_back(r.id);
+        if (hipMalloc((void**)&r.result, r.fieldDoubles * r.renderings * sizeof(double)) != hipSuccess) {
+            r.result = nullptr;
+            fail(r, SSTMC_ALLOCATION_ERROR);
+            continue;
+        }
+        r.tracer->updateSeed(r.seed);   // rendering 0: the config's seed (MatlabInterface.cu:1799)
+        if ((rc = r.tracer->begin())) {
+            fail(r, rc);
+            continue;
+        }
+        account(r);
         if (!anyAdmitted) { firstAdmit = Clock::now(); anyAdmitted = true; }
     }
 }
 
-// Queue batches (at most two in flight) and retire finished ones. Returns 1 when the request's
-// last batch has completed.
+// Queue batches (at most two in flight) and retire finished ones. Returns 1 when the current
+// rendering's last batch has completed, -1 on an error (r.error set).
 int sstmc_server::pump(Request& r) {
-    while (!r.inflight.empty() && hipEventQuery(r.inflight.front()) == hipSuccess) {
+    while (!r.inflight.empty()) {
+        const hipError_t q = hipEventQuery(r.inflight.front());
+        if (q == hipErrorNotReady) break;
         hipEventDestroy(r.inflight.front());
         r.inflight.pop_front();
+        if (q != hipSuccess) { r.error = SSTMC_KERNEL_ERROR; return -1; }
     }
     while (r.nextBatch < r.batches && r.inflight.size() < 2) {
         int rc = r.tracer->enqueueBatch(r.nextBatch++);
         if (rc) { r.error = rc; return -1; }
         hipEvent_t e;
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { r.error = SSTMC_DEVICE_ERROR; return -1; }
-        hipEventRecord(e, r.tracer->stream());
+        if (hipEventRecord(e, r.tracer->stream()) != hipSuccess) {
+            hipEventDestroy(e);
+            r.error = SSTMC_DEVICE_ERROR;
+            return -1;
+        }
         r.inflight.push_back(e);
     }
     return (r.nextBatch == r.batches && r.inflight.empty()) ? 1 : 0;
 }
 
-void sstmc_server::complete(Request& r) {
-    size_t n = 2ull * r.tracer->L() * r.tracer->V() * r.tracer->T();
-    int rc = hipMalloc((void**)&r.result, n * sizeof(double)) == hipSuccess ? SSTMC_NO_ERROR : SSTMC_ALLOCATION_ERROR;
-    if (!rc) rc = r.tracer->finish(r.result, true);
-    if (rc == SSTMC_NOT_READY) {       // vertex pool overflow: the pool was grown, re-run the request
-        hipFree(r.result);
-        r.result = nullptr;
-        r.tracer->begin();
+// The current rendering's batches are done: finish() re-runs any batch that overflowed the vertex
+// pool and copies the field into the request's result; then either the next rendering starts at
+// seed + i (MatlabInterface.cu:1799) or the request completes and its tracer is freed.
+void sstmc_server::renderingDone(Request& r) {
+    int rc = r.tracer->finish(r.result + r.fieldDoubles * r.rendering, true);
+    if (rc) {
+        fail(r, rc);
+        return;
+    }
+    account(r);
+    r.iter[r.rendering] = (double)r.tracer->paths() + (double)r.tracer->getZeroContributionPaths();
+    r.paths += r.tracer->paths();
+    r.contributions += r.tracer->paths() * (uint64_t)r.tracer->L() * r.tracer->V() * r.tracer->T();
+    if (++r.rendering < r.renderings) {
+        r.tracer->updateSeed(r.seed + r.rendering);
+        if ((rc = r.tracer->begin())) {
+            fail(r, rc);
+            return;
+        }
         r.nextBatch = 0;
         return;
     }
-    r.resultDoubles = n;
-    r.iter = (double)r.tracer->paths() + (double)r.tracer->getZeroContributionPaths();
-    r.paths = r.tracer->paths();
-    r.contributions = r.paths * (uint64_t)r.tracer->L() * r.tracer->V() * r.tracer->T();
-    sstmc_destroy(r.handle);
-    r.handle = nullptr;
-    r.tracer = nullptr;
+    release(r);
     inUse -= r.footprint;
-    r.done = Clock::now();
-    if (rc) {
-        r.state = kFailed;
-        r.error = rc;
-    } else {
-        r.state = kDone;
-        latMs.push_back(std::chrono::duration<double, std::milli>(r.done - r.submitted).count());
-        donePaths += r.paths;
-        doneContrib += r.contributions;
-        lastDone = r.done;
-    }
     active.erase(std::find(active.begin(), active.end(), r.id));
+    r.done = Clock::now();
+    r.state = kDone;
+    latMs.push_back(std::chrono::duration<double, std::milli>(r.done - r.submitted).count());
+    donePaths += r.paths;
+    doneContrib += r.contributions;
+    lastDone = r.done;
     cv.notify_all();
 }
 
@@ -240,28 +256,14 @@ void sstmc_server::run() {
             Request& r = *reqs[id];
             if (r.state != kActive) continue;
             int st = pump(r);
-            if (st < 0) {
-                sstmc_destroy(r.handle);
-                r.handle = nullptr;
-                inUse -= r.footprint;
-                r.state = kFailed;
-                r.done = Clock::now();
-                active.erase(std::find(active.begin(), active.end(), id));
-                cv.notify_all();
-            } else if (st == 1) {
-                complete(r);
-            }
+            if (st < 0) fail(r, r.error);
+            else if (st == 1) renderingDone(r);
         }
         lk.unlock();
         std::this_thread::sleep_for(std::chrono::microseconds(50));
         lk.lock();
     }
-    for (uint64_t id : active) {
-        Request& r = *reqs[id];
-        for (hipEvent_t e : r.inflight) { hipEventSynchronize(e); hipEventDestroy(e); }
-        if (r.handle) sstmc_destroy(r.handle);
-        r.handle = nullptr;
-    }
+    for (uint64_t id : std::vector<uint64_t>(active)) release(*reqs[id]);
 }
 
 extern "C" {
@@ -292,6 +294,8 @@ int sstmc_submit(sstmc_server* s, const sstmc_config* cfg, uint64_t* request_id)
     r->cfg.reset(new ConfigCopy(*cfg));
     r->footprint = estimate(r->cfg->c);
     if (r->footprint > s->budget) return SSTMC_BUDGET_EXCEEDED;
+    r->renderings = cfg->simulation.renderingsNum ? cfg->simulation.renderingsNum : 1;
+    r->seed = cfg->simulation.seed;
     r->submitted = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
     r->id = s->nextId++;
@@ -332,9 +336,10 @@ int sstmc_fetch(sstmc_server* s, uint64_t id, double* u, double* iter) {
     if (r.state == kFailed) return r.error ? r.error : SSTMC_MISC_ERROR;
     if (r.state != kDone) return SSTMC_NOT_READY;
     hipSetDevice(s->device);
-    if (hipMemcpy(u, r.result, r.resultDoubles * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(u, r.result, r.fieldDoubles * r.renderings * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
         return SSTMC_DEVICE_ERROR;
-    if (iter) *iter = r.iter;
+    if (iter)
+        for (uint32_t i = 0; i < r.renderings; ++i) iter[i] = r.iter[i];
     hipFree(r.result);
     r.result = nullptr;
     r.state = kFetched;

@@ -202,6 +202,7 @@ struct TracerOptions {
     int cbs;           // coherent backscattering (field output only)
     int terms;         // SSTMC_TERMS_*
     int contraction;   // SSTMC_CONTRACT_*
+    double tauCut;     // precision.tauCut (0: exact)
 };
 
 class Tracer {
@@ -227,9 +228,9 @@ public:
     ~NextEventEstimation() override;
     int trace(double* u) override;
     // Asynchronous form used by the serving loop: begin() clears the accumulators, enqueueBatch()
-    // queues one batch on the tracer's stream, finish() checks the device status (returns
-    // SSTMC_NOT_READY after a vertex-pool overflow, after which the pool has been grown and the
-    // trace must be re-run) and copies the field to host or device memory.
+    // queues one batch on the tracer's stream, finish() waits for the stream, re-runs the batches
+    // whose vertices overflowed the pool (after growing it) and copies the field to host or
+    // device memory.
     int begin();
     int enqueueBatch(uint32_t b);
     int enqueue();
@@ -242,6 +243,8 @@ public:
                    uint64_t* pathOff, float* vtxPos, uint32_t* vtxMat, float* vtxRho, float* vtxXt, double* vtxLen,
                    double* termA, double* termB, double* termSa, double* termSb, double* uBatch);
     int debugContract(uint32_t npaths, const double* a, const double* b, double* u);
+    // the last batch's reference of the relative attenuation cut
+    int tauRef(float* out) const;
     // coherent-backscattering terms (Ar [p][t][v], Br [p][t][l]) of the last debugBatch of n paths
     int debugCbsTerms(uint32_t n, double* termAr, double* termBr) const;
     void timing(sstmc_timing* out) const;
@@ -249,7 +252,8 @@ public:
     uint32_t L() const { return L_; }
     uint32_t V() const { return V_; }
     uint32_t T() const { return T_; }
-    uint64_t deviceBytes() const { return bytes_; }
+    // device bytes held now: buffers, vertex pool, walk grid and split-K workspaces
+    uint64_t deviceBytes() const { return bytes_ + wsMulti_.bytes + wsSingle_.bytes; }
     uint64_t paths() const { return (uint64_t)P_ * opt_.fullIteration; }
 
 private:
@@ -282,6 +286,8 @@ private:
     uint64_t bytes_ = 0;
 
     DevBuf<uint32_t> k_, misses_, off_, blockSums_, status_;
+    DevBuf<uint32_t> overflow_;           // bit b: batch b's vertices did not fit the pool
+    DevBuf<float> tauRef_;                // the batch's least view optical depth (relative cut)
     DevBuf<unsigned long long> totals_;   // [0] misses, [1] vertices, [2] NEE voxel steps
     DevBuf<float4> xm_;
     DevBuf<float> rho_, delta_;

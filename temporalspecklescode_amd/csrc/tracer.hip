@@ -828,7 +828,8 @@ NextEventEstimation::~NextEventEstimation() {
 int NextEventEstimation::allocate() {
     int e;
     if ((e = k_.alloc(P_)) || (e = misses_.alloc(P_)) || (e = off_.alloc(P_ + 1)) ||
-        (e = blockSums_.alloc((P_ + 1023) / 1024 + 1)) || (e = status_.alloc(4)) || (e = totals_.alloc(kTotals)))
+        (e = blockSums_.alloc((P_ + 1023) / 1024 + 1)) || (e = status_.alloc(4)) || (e = totals_.alloc(kTotals)) ||
+        (e = overflow_.alloc((opt_.fullIteration + 31) / 32)) || (e = tauRef_.alloc(1)))
         return e;
     size_t elem = fp64_ ? 16 : 8;
     size_t na = (size_t)T_ * P_ * L_ * elem, nb = (size_t)T_ * P_ * V_ * elem;
@@ -897,14 +898,18 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     if (launchTransport(dims_, m, sc, il, fr, vb, seed_, pathBegin, npaths, 0, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
     if (launchScan(k_.p, off_.p, blockSums_.p, npaths, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
     if (launchTransport(dims_, m, sc, il, fr, vb, seed_, pathBegin, npaths, 1, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
-    if (launchBatchTotals(misses_.p, off_.p, npaths, totals_.p, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    if (launchBatchTotals(misses_.p, off_.p, npaths, totals_.p, cap_, overflow_.p, b, tauRef_.p, stream_) != hipSuccess)
+        return SSTMC_KERNEL_ERROR;
     if (e) hipEventRecord(e[1], stream_);
     NeeParams prm;
     prm.lambda = opt_.lambda;
     prm.output = opt_.output;
     prm.singleMode = singleMode_;
-    prm.tauCutF = 210.0f;     // exp(-105) rounds to zero in FP32: far-field walks stop there
-    prm.tauCutD = 1.0e300;
+    // walks are exact by default (FP32: exp(-105) rounds to zero, so walks stop at 210); with
+    // precision.tauCut the view-side terms take the batch-relative cut (k_tau_ref)
+    prm.exactCutF = 210.0f;
+    prm.tauRef = opt_.tauCut > 0 ? tauRef_.p : nullptr;
+    prm.tauCutRel = (float)opt_.tauCut;
     prm.npaths = npaths;
     prm.sameViews = sameViews_ ? 1u : 0u;
     NeeOut out{A_.p, B_.p, a_.p, b_.p, Ar_.p, Br_.p, totals_.p + 2};
@@ -1052,6 +1057,8 @@ int NextEventEstimation::begin() {
     if (hipMemsetAsync(status_.p, 0, 4 * sizeof(uint32_t), stream_) != hipSuccess) return SSTMC_DEVICE_ERROR;
     if (hipMemsetAsync(totals_.p, 0, kTotals * sizeof(unsigned long long), stream_) != hipSuccess)
         return SSTMC_DEVICE_ERROR;
+    if (hipMemsetAsync(overflow_.p, 0, overflow_.n * sizeof(uint32_t), stream_) != hipSuccess)
+        return SSTMC_DEVICE_ERROR;
     if (launchZeroU(U_.p, U_.n, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
     return SSTMC_NO_ERROR;
 }
@@ -1069,18 +1076,36 @@ int NextEventEstimation::enqueue() {
     return SSTMC_NO_ERROR;
 }
 
+// A batch whose vertices overflowed the pool added nothing to U and was not counted (every
+// kernel after the transport sees the overflow and writes zero terms); it is re-run here after the
+// pool has grown to the largest batch seen plus a margin. The re-run adds its share to U after the
+// other batches rather than in batch order; pool sizes depend on the config alone, so a config
+// still gives the same bits every time, alone or in a serving mix.
 int NextEventEstimation::finish(double* u, bool deviceDst) {
-    if (hipStreamSynchronize(stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
     uint32_t st[4];
     unsigned long long tot[kTotals];
-    if (hipMemcpy(st, status_.p, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
-    if (hipMemcpy(tot, totals_.p, sizeof(tot), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
-    if (st[1]) return SSTMC_NO_HIT;
-    if (st[0] & 2u) return SSTMC_KERNEL_ERROR;   // count and write passes disagreed
-    if (st[0] & 1u) {                            // pool overflow: grow from the largest batch, re-run
-        uint64_t want = std::max<uint64_t>(cap_ * 2, (uint64_t)tot[3] + tot[3] / 4 + 1024);
+    for (int round = 0;; ++round) {
+        if (hipStreamSynchronize(stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+        if (hipMemcpy(st, status_.p, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
+        if (hipMemcpy(tot, totals_.p, sizeof(tot), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
+        if (st[1]) return SSTMC_NO_HIT;
+        if (st[0] & 2u) return SSTMC_KERNEL_ERROR;   // count and write passes disagreed
+        if (!(st[0] & 1u)) break;
+        // the pool now holds the largest batch with margin, so a re-run batch fits; a second
+        // overflow would mean the count pass changed, which it cannot
+        if (round == 2) return SSTMC_ALLOCATION_ERROR;
+        std::vector<uint32_t> bits(overflow_.n);
+        if (hipMemcpy(bits.data(), overflow_.p, bits.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            return SSTMC_DEVICE_ERROR;
+        const uint64_t want = std::max<uint64_t>(cap_ + cap_ / 2, (uint64_t)tot[3] + tot[3] / 4 + 1024);
         int rc = growPool(want);
-        return rc ? rc : SSTMC_NOT_READY;
+        if (rc) return rc;
+        if (hipMemsetAsync(status_.p, 0, sizeof(uint32_t), stream_) != hipSuccess ||
+            hipMemsetAsync(overflow_.p, 0, overflow_.n * sizeof(uint32_t), stream_) != hipSuccess)
+            return SSTMC_DEVICE_ERROR;
+        for (uint32_t b = 0; b < opt_.fullIteration; ++b)
+            if (bits[b >> 5] & (1u << (b & 31)))
+                if ((rc = runBatch((uint64_t)b * P_, P_, U_.p, true, b))) return rc;
     }
     zeroContributionPaths_ = tot[0];
     photonsBudget_ = tot[1];
@@ -1110,13 +1135,8 @@ int NextEventEstimation::finish(double* u, bool deviceDst) {
 }
 
 int NextEventEstimation::trace(double* u) {
-    for (int attempt = 0; attempt < 8; ++attempt) {
-        int rc = enqueue();
-        if (rc) return rc;
-        rc = finish(u);
-        if (rc != SSTMC_NOT_READY) return rc;   // NOT_READY: the pool was grown, run again
-    }
-    return SSTMC_ALLOCATION_ERROR;
+    int rc = enqueue();
+    return rc ? rc : finish(u);
 }
 
 void NextEventEstimation::timing(sstmc_timing* out) const { *out = last_; }
@@ -1159,6 +1179,7 @@ int NextEventEstimation::debugBatch(uint64_t pathBegin, uint32_t n, uint64_t* nv
     for (int attempt = 0; attempt < 8; ++attempt) {
         hipMemsetAsync(status_.p, 0, 4 * sizeof(uint32_t), stream_);
         hipMemsetAsync(totals_.p, 0, kTotals * sizeof(unsigned long long), stream_);
+        hipMemsetAsync(overflow_.p, 0, overflow_.n * sizeof(uint32_t), stream_);
         launchZeroU(Ub.p, Ub.n, stream_);
         if ((rc = runBatch(pathBegin, n, Ub.p, false, 0))) return rc;
         if (hipStreamSynchronize(stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
@@ -1231,6 +1252,13 @@ int NextEventEstimation::debugBatch(uint64_t pathBegin, uint32_t n, uint64_t* nv
         return SSTMC_NO_ERROR;
     }
     return SSTMC_ALLOCATION_ERROR;
+}
+
+int NextEventEstimation::tauRef(float* out) const {
+    if (!(opt_.tauCut > 0)) return SSTMC_INVALID_CONFIG;
+    if (hipStreamSynchronize(stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    return hipMemcpy(out, tauRef_.p, sizeof(float), hipMemcpyDeviceToHost) == hipSuccess ? SSTMC_NO_ERROR
+                                                                                     : SSTMC_DEVICE_ERROR;
 }
 
 int NextEventEstimation::debugContract(uint32_t n, const double* a, const double* b, double* u) {

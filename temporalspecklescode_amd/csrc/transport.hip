@@ -212,9 +212,14 @@ __global__ void __launch_bounds__(1024) k_scan_add(uint32_t* out, const uint32_t
 }
 
 // Per-batch totals: misses, vertex count and the largest batch, in 64-bit counters (one block,
-// launched in stream order after each batch, so the updates never race).
+// launched in stream order after each batch, so the updates never race). A batch whose vertices
+// did not fit the pool contributes nothing (every later kernel sees the overflow and writes zero
+// terms): it is flagged in `overflow` (bit `batch`) for the host to re-run after growing the pool,
+// and its misses and vertices are counted by that re-run instead.
 __global__ void __launch_bounds__(256) k_batch_totals(const uint32_t* misses, const uint32_t* off, uint32_t n,
-                                                      unsigned long long* acc) {
+                                                      unsigned long long* acc, uint64_t cap, uint32_t* overflow,
+                                                      uint32_t batch, float* tauRef) {
+    if (tauRef && threadIdx.x == 0) *tauRef = 3.0e38f;   // the batch's k_tau_ref lowers it
     __shared__ unsigned long long s[256];
     unsigned long long v = 0;
     for (uint32_t i = threadIdx.x; i < n; i += 256) v += misses[i];
@@ -225,8 +230,12 @@ __global__ void __launch_bounds__(256) k_batch_totals(const uint32_t* misses, co
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        acc[0] += s[0];
-        acc[1] += off[n];
+        if ((uint64_t)off[n] <= cap) {
+            acc[0] += s[0];
+            acc[1] += off[n];
+        } else {
+            overflow[batch >> 5] |= 1u << (batch & 31);
+        }
         if (off[n] > acc[3]) acc[3] = off[n];   // largest batch, sizes the vertex pool
     }
 }
@@ -274,8 +283,8 @@ hipError_t launchScan(const uint32_t* k, uint32_t* off, uint32_t* blockSums, uin
 }
 
 hipError_t launchBatchTotals(const uint32_t* misses, const uint32_t* off, uint32_t n, unsigned long long* acc,
-                             hipStream_t st) {
-    hipLaunchKernelGGL(k_batch_totals, dim3(1), dim3(256), 0, st, misses, off, n, acc);
+                             uint64_t cap, uint32_t* overflow, uint32_t batch, float* tauRef, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_totals, dim3(1), dim3(256), 0, st, misses, off, n, acc, cap, overflow, batch, tauRef);
     return hipGetLastError();
 }
 

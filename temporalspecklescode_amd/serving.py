@@ -43,8 +43,7 @@ class Server:
         cc = CConfig(config, dims)
         rid = C.c_uint64()
         _native.check(_native.lib().sstmc_submit(self._h, cc.ptr(), C.byref(rid)), "sstmc_submit")
-        nl, nv, nt, _ = sizes(config, dims)
-        self._meta[rid.value] = (nl, nv, nt)
+        self._meta[rid.value] = sizes(config, dims)
         return rid.value
 
     def poll(self, rid: int) -> bool:
@@ -58,12 +57,17 @@ class Server:
         _native.check(_native.lib().sstmc_wait(self._h, C.c_uint64(rid)), "sstmc_wait")
 
     def fetch(self, rid: int):
-        nl, nv, nt = self._meta[rid]
-        u = np.zeros(2 * nl * nv * nt)
-        it = C.c_double()
-        _native.check(_native.lib().sstmc_fetch(self._h, C.c_uint64(rid), u.ctypes.data_as(C.c_void_p), C.byref(it)),
-                      "sstmc_fetch")
-        return (u[0::2] + 1j * u[1::2]).reshape(nt, nv, nl).transpose(2, 1, 0), it.value
+        """(u, iter) like ``sstmc2D/3D``: u of shape (Nl, Nv, Nt) and iter a float for one
+        rendering; (Nl, Nv, Nt, renderingsNum) and an array of renderingsNum values otherwise."""
+        nl, nv, nt, rn = self._meta[rid]
+        u = np.zeros(2 * nl * nv * nt * rn)
+        it = np.zeros(rn)
+        _native.check(_native.lib().sstmc_fetch(self._h, C.c_uint64(rid), u.ctypes.data_as(C.c_void_p),
+                                                it.ctypes.data_as(C.c_void_p)), "sstmc_fetch")
+        u = (u[0::2] + 1j * u[1::2]).reshape(rn, nt, nv, nl).transpose(3, 2, 1, 0)
+        if rn == 1:
+            return u[..., 0], float(it[0])
+        return u, it
 
     def stats(self) -> dict:
         s = _native.ServerStats()

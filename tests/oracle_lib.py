@@ -47,8 +47,9 @@ def render(cfg: dict, dims: int, nthreads: int = 0):
     return uc, it.value, nvtx.value
 
 
-def paths(cfg: dict, dims: int, begin: int, n: int):
-    """Per-path vertices and terms for global path ids [begin, begin+n)."""
+def paths(cfg: dict, dims: int, begin: int, n: int, terms: bool = True):
+    """Per-path vertices and terms for global path ids [begin, begin+n) (vertices only when
+    terms is False)."""
     cc = CConfig(cfg, dims)
     nl, nv, nt, _ = sizes(cfg, dims)
     nvtx = C.c_uint64()
@@ -60,20 +61,27 @@ def paths(cfg: dict, dims: int, begin: int, n: int):
     out = dict(
         k=np.zeros(n, np.uint32), misses=np.zeros(n, np.uint32), off=np.zeros(n + 1, np.uint64),
         pos=np.zeros((m, 3)), mat=np.zeros(m, np.uint32), rho=np.zeros(m), xt=np.zeros((m, nt, 3)),
-        len=np.zeros((m, nt)), a=np.zeros((n, nt, nl, 2)), b=np.zeros((n, nt, nv, 2)),
-        s=np.zeros((n, nt, nv, nl, 2)))
+        len=np.zeros((m, nt)))
+    if terms:
+        out.update(a=np.zeros((n, nt, nl, 2)), b=np.zeros((n, nt, nv, 2)), s=np.zeros((n, nt, nv, nl, 2)))
+    tp = [_p(out[key]) if terms else None for key in ("a", "b", "s")]
     rc = L.oracle_paths(cc.ptr(), C.c_uint64(begin), C.c_uint32(n), C.byref(nvtx),
                         _p(out["k"]), _p(out["misses"]), _p(out["off"]), _p(out["pos"]), _p(out["mat"]),
-                        _p(out["rho"]), _p(out["xt"]), _p(out["len"]), _p(out["a"]), _p(out["b"]), _p(out["s"]))
+                        _p(out["rho"]), _p(out["xt"]), _p(out["len"]), *tp)
     if rc:
         raise RuntimeError("oracle_paths failed with code %d" % rc)
-    for key in ("a", "b", "s"):
-        out[key] = out[key][..., 0] + 1j * out[key][..., 1]
+    if terms:
+        for key in ("a", "b", "s"):
+            out[key] = out[key][..., 0] + 1j * out[key][..., 1]
     return out
 
 
-def terms_from_vertices(cfg: dict, dims: int, k, off, pos, mat, rho, xt, ln):
-    """FP64 NEE terms for given vertices (positions as float64)."""
+def terms_from_vertices(cfg: dict, dims: int, k, off, pos, mat, rho, xt, ln, per_path_single=True, tau_ref=None):
+    """FP64 NEE terms for given vertices (positions as float64): (A [n, Nt, Nl], B [n, Nt, Nv], S).
+    S is the per-path single-scattering term [n, Nt, Nv, Nl], or with per_path_single=False its
+    sum over the paths [Nt, Nv, Nl] (S(0) conj S(t) summed for the correlation output), for
+    problems whose per-path S is too large to hold. With precision.tauCut, tau_ref is the batch's
+    reference optical depth of the relative cut (None: computed from these vertices)."""
     cc = CConfig(cfg, dims)
     nl, nv, nt, _ = sizes(cfg, dims)
     n = len(k)
@@ -87,9 +95,16 @@ def terms_from_vertices(cfg: dict, dims: int, k, off, pos, mat, rho, xt, ln):
     nvtx = int(off[-1])
     if pos.shape != (nvtx, 3) or xt.shape != (nvtx, nt, 3) or ln.shape != (nvtx, nt) or mat.size != nvtx:
         raise ValueError("vertex arrays do not match %d vertices x %d frames" % (nvtx, nt))
-    a = np.zeros((n, nt, nl, 2)); b = np.zeros((n, nt, nv, 2)); s = np.zeros((n, nt, nv, nl, 2))
+    a = np.zeros((n, nt, nl, 2)); b = np.zeros((n, nt, nv, 2))
+    if per_path_single:
+        s = np.zeros((n, nt, nv, nl, 2))
+        ps, pss = _p(s), None
+    else:
+        s = np.zeros((nt, nv, nl, 2))
+        ps, pss = None, _p(s)
     rc = lib().oracle_terms_from_vertices(cc.ptr(), C.c_uint32(n), _p(k), _p(off), _p(pos), _p(mat), _p(rho),
-                                          _p(xt), _p(ln), _p(a), _p(b), _p(s))
+                                          _p(xt), _p(ln), _p(a), _p(b), ps, pss,
+                                          C.c_double(float("nan") if tau_ref is None else tau_ref))
     if rc:
         raise RuntimeError("oracle_terms_from_vertices failed with code %d" % rc)
     return a[..., 0] + 1j * a[..., 1], b[..., 0] + 1j * b[..., 1], s[..., 0] + 1j * s[..., 1]

@@ -35,8 +35,8 @@ def _oracle_u(cfg, dims, d, a, b, s_unused=None):
     return A, B, S, U
 
 
-def _transport_match(cfg, dims, d, n, tol):
-    o = O.paths(cfg, dims, 0, n)
+def _transport_match(cfg, dims, d, n, tol, terms=True):
+    o = O.paths(cfg, dims, 0, n, terms=terms)
     same_k = d["k"] == o["k"]
     match = np.zeros(n, bool)
     for i in np.nonzero(same_k)[0]:
