@@ -48,7 +48,8 @@ enum {
     SSTMC_INVALID_CONFIG = 7,      /* config validation (MatlabInterface.cu mexErrMsgIdAndTxt sites) */
     SSTMC_NO_HIT = 8,              /* first-vertex sampling exceeded its retry budget */
     SSTMC_BUDGET_EXCEEDED = 9,     /* serving: request larger than the admission budget */
-    SSTMC_NOT_READY = 10           /* serving: request still in flight */
+    SSTMC_NOT_READY = 10,          /* serving: request still in flight */
+    SSTMC_SCRATCH_EXCEEDED = 11    /* sstmc_contract_noalloc: partials exceed its fixed scratch */
 };
 
 /* ---------------------------------------------------------------- enums */
@@ -286,6 +287,21 @@ int sstmc_debug_tau_ref(const sstmc_handle* h, float* tau_ref);
  * Exposes kernel (c) alone for accuracy tests against an FP64 GEMM. */
 int sstmc_debug_contract(sstmc_handle* h, uint32_t npaths, const double* a, const double* b,
                          double* u);
+
+/* Allocation-free contraction. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It runs the contraction of sstmc_debug_contract, (c), on terms the caller already
+ * holds on the current device and adds into the caller's device accumulator:
+ *   u[t][v][l] += sum_p a[t][p][l] * b[t][p][v]   (complex, interleaved)
+ * a, b: complex64 terms (interleaved floats) for SSTMC_CONTRACT_BF16X3 and _FP32, complex128 for
+ * _FP64; u: complex128. Enqueued on `stream` (a hipStream_t; NULL: the default stream), not
+ * synchronised. It allocates nothing: its split-K partials live in a fixed 64 MiB array per device
+ * (held from the device's first call until the process ends); a problem whose L x V partials of
+ * one split exceed it (L * V > 8 Mi complex64, 4 Mi complex128) returns SSTMC_SCRATCH_EXCEEDED,
+ * and larger path counts take fewer split slices. Calls that overlap on one device share that
+ * array and race: serialise them (one stream, or events). */
+int sstmc_contract_noalloc(int32_t contraction, uint32_t npaths, uint32_t L, uint32_t V, uint32_t nt,
+                           const void* a, const void* b, double* u, void* stream);
 
 /* ---------------------------------------------------------------- serving */
 /* Device-resident request loop with continuous batching (no counterpart in the reference,

@@ -330,6 +330,9 @@ const char* sstmc_error_string(int code) {
         case SSTMC_NO_HIT: return "No path reached the medium within the retry budget";
         case SSTMC_BUDGET_EXCEEDED: return "Request larger than the admission budget";
         case SSTMC_NOT_READY: return "Request still in flight";
+        case SSTMC_SCRATCH_EXCEEDED:
+            return "Contraction too large for the allocation-free entry point: one split's L x V partials exceed "
+                   "its fixed 64 MiB scratch";
         default: return "Unknown error";
     }
 }

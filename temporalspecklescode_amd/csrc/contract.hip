@@ -12,6 +12,9 @@
 //   * fp32 / fp64: LDS-tiled FMA kernels, the accuracy references.
 // Split-K partial sums go to a workspace and are reduced in a fixed order, so a result does not
 // depend on scheduling: a request gives the same bits alone or in a mix.
+#include <mutex>
+
+#include "../../include/sstmc.h"
 #include "common.h"
 #include "gauss.h"
 #include "kernels.h"
@@ -241,56 +244,134 @@ static uint32_t chooseSplits(uint32_t tiles, uint32_t P, uint32_t minPaths) {
     return s < 1 ? 1 : s;
 }
 
-// The split-K workspace belongs to the caller (one per tracer), grown here when too small.
-hipError_t launchContract(int kind, const void* A, const void* B, uint32_t P, uint32_t L, uint32_t V, uint32_t nt,
-                          const float2* F, const double2* Fd, int fsq, double fscalar, double* U, Workspace* ws,
-                          hipStream_t st) {
-    uint32_t tl = kind == kContractBf16x3 ? kTileL : 64, tv = kind == kContractBf16x3 ? kTileV : 64;
-    uint32_t tiles = ((L + tl - 1) / tl) * ((V + tv - 1) / tv);
+// Split-K plan: `splits` slices of `pps` paths each (whole LDS chunks), at most maxSplits.
+struct SplitPlan {
+    uint32_t splits, pps;
+    size_t bytes;   // partials: splits x L x V complex
+};
+static SplitPlan planSplits(int kind, uint32_t P, uint32_t L, uint32_t V, uint32_t maxSplits) {
+    const uint32_t tl = kind == kContractBf16x3 ? kTileL : 64, tv = kind == kContractBf16x3 ? kTileV : 64;
+    const uint32_t tiles = ((L + tl - 1) / tl) * ((V + tv - 1) / tv);
     uint32_t splits = chooseSplits(tiles, P, kind == kContractBf16x3 ? 512 : 256);
+    if (splits > maxSplits) splits = maxSplits;
+    if (splits < 1) splits = 1;
     uint32_t pps = (P + splits - 1) / splits;
     if (kind == kContractBf16x3) pps = (pps + kChunk - 1) / kChunk * kChunk;
     else pps = (pps + 15) / 16 * 16;
     splits = (P + pps - 1) / pps;
     if (splits < 1) splits = 1;
-    size_t elem = kind == kContractFp64 ? 16 : 8;
-    size_t need = (size_t)splits * L * V * elem;
-    if (ws->bytes < need) {
+    const size_t elem = kind == kContractFp64 ? 16 : 8;
+    return SplitPlan{splits, pps, (size_t)splits * L * V * elem};
+}
+
+static hipError_t launchContractInto(int kind, const void* A, const void* B, uint32_t P, uint32_t L, uint32_t V,
+                                     uint32_t nt, const float2* F, const double2* Fd, int fsq, double fscalar,
+                                     double* U, const SplitPlan& plan, void* partials, hipStream_t st);
+
+// The split-K workspace belongs to the caller (one per tracer), grown here when too small.
+hipError_t launchContract(int kind, const void* A, const void* B, uint32_t P, uint32_t L, uint32_t V, uint32_t nt,
+                          const float2* F, const double2* Fd, int fsq, double fscalar, double* U, Workspace* ws,
+                          hipStream_t st) {
+    const SplitPlan plan = planSplits(kind, P, L, V, 0xffffffffu);
+    if (ws->bytes < plan.bytes) {
         if (ws->ptr) {
             hipStreamSynchronize(st);
             hipFree(ws->ptr);
         }
         ws->ptr = nullptr;
         ws->bytes = 0;
-        hipError_t e = hipMalloc(&ws->ptr, need);
+        hipError_t e = hipMalloc(&ws->ptr, plan.bytes);
         if (e != hipSuccess) return e;
-        ws->bytes = need;
+        ws->bytes = plan.bytes;
     }
+    return launchContractInto(kind, A, B, P, L, V, nt, F, Fd, fsq, fscalar, U, plan, ws->ptr, st);
+}
+
+static hipError_t launchContractInto(int kind, const void* A, const void* B, uint32_t P, uint32_t L, uint32_t V,
+                                     uint32_t nt, const float2* F, const double2* Fd, int fsq, double fscalar,
+                                     double* U, const SplitPlan& plan, void* partials, hipStream_t st) {
+    const uint32_t tl = kind == kContractBf16x3 ? kTileL : 64, tv = kind == kContractBf16x3 ? kTileV : 64;
+    const uint32_t splits = plan.splits, pps = plan.pps;
     const uint32_t LV = L * V;
     for (uint32_t t = 0; t < nt; ++t) {
         dim3 grid((L + tl - 1) / tl, (V + tv - 1) / tv, splits);
         if (kind == kContractBf16x3) {
             const float2* a = (const float2*)A + (size_t)t * P * L;
             const float2* b = (const float2*)B + (size_t)t * P * V;
-            hipLaunchKernelGGL(k_contract_bf16x3, grid, dim3(256), 0, st, a, b, P, L, V, pps, (float2*)ws->ptr);
-            hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st, (const float2*)ws->ptr,
+            hipLaunchKernelGGL(k_contract_bf16x3, grid, dim3(256), 0, st, a, b, P, L, V, pps, (float2*)partials);
+            hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st, (const float2*)partials,
                                splits, LV, F, Fd, fsq, fscalar, U + 2 * (size_t)t * LV);
         } else if (kind == kContractFp32) {
             const float2* a = (const float2*)A + (size_t)t * P * L;
             const float2* b = (const float2*)B + (size_t)t * P * V;
-            hipLaunchKernelGGL(k_contract_fma<float>, grid, dim3(256), 0, st, a, b, P, L, V, pps, (float2*)ws->ptr);
-            hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st, (const float2*)ws->ptr,
+            hipLaunchKernelGGL(k_contract_fma<float>, grid, dim3(256), 0, st, a, b, P, L, V, pps, (float2*)partials);
+            hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st, (const float2*)partials,
                                splits, LV, F, Fd, fsq, fscalar, U + 2 * (size_t)t * LV);
         } else {
             const double2* a = (const double2*)A + (size_t)t * P * L;
             const double2* b = (const double2*)B + (size_t)t * P * V;
-            hipLaunchKernelGGL(k_contract_fma<double>, grid, dim3(256), 0, st, a, b, P, L, V, pps, (double2*)ws->ptr);
-            hipLaunchKernelGGL(k_reduce<double2>, dim3((LV + 255) / 256), dim3(256), 0, st, (const double2*)ws->ptr,
+            hipLaunchKernelGGL(k_contract_fma<double>, grid, dim3(256), 0, st, a, b, P, L, V, pps, (double2*)partials);
+            hipLaunchKernelGGL(k_reduce<double2>, dim3((LV + 255) / 256), dim3(256), 0, st, (const double2*)partials,
                                splits, LV, F, Fd, fsq, fscalar, U + 2 * (size_t)t * LV);
         }
     }
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------- allocation-free contraction
+// sstmc_contract_noalloc keeps its split-K partials in this fixed array instead of a workspace it
+// allocates: each device the library runs on holds one copy (64 MiB) until the process ends, and
+// every call on a device uses that device's copy. The kernels write every partial they later
+// reduce, so nothing in it needs clearing between calls; calls that overlap on one device race.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// The device address of `scratch` on the current device, taken once per device with it current
+// (on the host the array's name is a stand-in, not a device address).
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratchMu;
+static void* g_scratchPtr[kMaxDevices] = {};
+
+static hipError_t scratchAddress(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratchPtr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratchPtr[dev] = p;
+    }
+    *out = g_scratchPtr[dev];
+    return hipSuccess;
+}
+
+}  // namespace sstmc
+
+extern "C" int sstmc_contract_noalloc(int32_t contraction, uint32_t npaths, uint32_t L, uint32_t V, uint32_t nt,
+                                      const void* a, const void* b, double* u, void* stream) {
+    using namespace sstmc;
+    if (npaths == 0 || L == 0 || V == 0 || nt == 0 || !a || !b || !u) return SSTMC_INVALID_CONFIG;
+    int kind;
+    if (contraction == SSTMC_CONTRACT_BF16X3) kind = kContractBf16x3;
+    else if (contraction == SSTMC_CONTRACT_FP32) kind = kContractFp32;
+    else if (contraction == SSTMC_CONTRACT_FP64) kind = kContractFp64;
+    else return SSTMC_INVALID_CONFIG;
+    // one split's partials (L x V complex) must fit; more splits take what the array holds
+    const size_t slice = (size_t)L * V * (kind == kContractFp64 ? 16u : 8u);
+    if (slice > kScratchBytes) return SSTMC_SCRATCH_EXCEEDED;
+    const SplitPlan plan = planSplits(kind, npaths, L, V, (uint32_t)(kScratchBytes / slice));
+    if (plan.bytes > kScratchBytes) return SSTMC_SCRATCH_EXCEEDED;
+    void* ptr = nullptr;
+    if (scratchAddress(&ptr) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    const hipError_t e = launchContractInto(kind, a, b, npaths, L, V, nt, nullptr, nullptr, 0, 1.0, u, plan, ptr,
+                                            (hipStream_t)stream);
+    return e == hipSuccess ? SSTMC_NO_ERROR : SSTMC_KERNEL_ERROR;
+}
+
+namespace sstmc {
 
 // ---------------------------------------------------------------- generic single scattering
 // S_p[l,v,t] = a_p[l,t] b_p[v,t] f(l -> x1^t -> v) when f depends on the first vertex (point
