@@ -6,8 +6,9 @@
 // resident at once. A worker thread:
 //   * admits queued requests in arrival order while the HBM footprint of the admitted set stays
 //     within the budget and fewer than max_inflight are active;
-//   * keeps up to two batches in flight per active request, round-robin over requests, so work
-//     that arrives while others run is interleaved with it on the device;
+//   * keeps up to three batches in flight per active request (one per stage of its tracer's
+//     pipeline), round-robin over requests, so work that arrives while others run is interleaved
+//     with it on the device;
 //   * on completion copies the field into a small device buffer owned by the request and frees
 //     the tracer, so finished results stay on the device until fetched.
 // A request's result depends only on its config: batches, random numbers and reductions are
@@ -183,8 +184,8 @@ void sstmc_server::admit() {
     }
 }
 
-// Queue batches (at most two in flight) and retire finished ones. Returns 1 when the current
-// rendering's last batch has completed, -1 on an error (r.error set).
+// Queue batches (at most three in flight: one per pipeline stage) and retire finished ones.
+// Returns 1 when the current rendering's last batch has completed, -1 on an error (r.error set).
 int sstmc_server::pump(Request& r) {
     while (!r.inflight.empty()) {
         const hipError_t q = hipEventQuery(r.inflight.front());
@@ -193,12 +194,12 @@ int sstmc_server::pump(Request& r) {
         r.inflight.pop_front();
         if (q != hipSuccess) { r.error = SSTMC_KERNEL_ERROR; return -1; }
     }
-    while (r.nextBatch < r.batches && r.inflight.size() < 2) {
+    while (r.nextBatch < r.batches && r.inflight.size() < 3) {
         int rc = r.tracer->enqueueBatch(r.nextBatch++);
         if (rc) { r.error = rc; return -1; }
         hipEvent_t e;
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { r.error = SSTMC_DEVICE_ERROR; return -1; }
-        if (hipEventRecord(e, r.tracer->stream()) != hipSuccess) {
+        if (hipEventRecord(e, r.tracer->doneStream()) != hipSuccess) {
             hipEventDestroy(e);
             r.error = SSTMC_DEVICE_ERROR;
             return -1;

@@ -228,9 +228,11 @@ public:
     ~NextEventEstimation() override;
     int trace(double* u) override;
     // Asynchronous form used by the serving loop: begin() clears the accumulators, enqueueBatch()
-    // queues one batch on the tracer's stream, finish() waits for the stream, re-runs the batches
-    // whose vertices overflowed the pool (after growing it) and copies the field to host or
-    // device memory.
+    // queues one batch on the tracer's three streams (transport, NEE, contraction; consecutive
+    // batches alternate between two buffer sets, so the transport of batch b + 1 and the
+    // contraction of batch b - 1 overlap the NEE of batch b), finish() waits for the streams,
+    // re-runs the batches whose vertices overflowed the pool (after growing it) and copies the
+    // field to host or device memory.
     int begin();
     int enqueueBatch(uint32_t b);
     int enqueue();
@@ -249,6 +251,8 @@ public:
     int debugCbsTerms(uint32_t n, double* termAr, double* termBr) const;
     void timing(sstmc_timing* out) const;
     hipStream_t stream() const { return stream_; }
+    // the stream a batch's last stage (its contraction) runs on
+    hipStream_t doneStream() const { return streamC_; }
     uint32_t L() const { return L_; }
     uint32_t V() const { return V_; }
     uint32_t T() const { return T_; }
@@ -260,12 +264,33 @@ private:
     int allocate();
     int growPool(uint64_t cap);
     int sizePool();
-    VertexBatch vertexBatch() const;
+    // Per-batch device state. Batch b uses set b & 1: its per-path arrays, vertex pool, term
+    // slabs and cut reference, so that two batches can be in different stages at once.
+    struct BatchSet {
+        DevBuf<uint32_t> k, misses, off, blockSums;
+        DevBuf<float4> xm;
+        DevBuf<float> rho, delta;
+        DevBuf<VertexRec> rec;
+        DevBuf<float> tauRef;                 // the batch's least view optical depth (relative cut)
+        DevBuf<unsigned char> A, B, a, b;
+        DevBuf<unsigned char> Ar, Br;         // coherent backscattering: reversed-path terms
+        hipEvent_t transported = nullptr;     // transport done (transport stream)
+        hipEvent_t traced = nullptr;          // NEE done (NEE stream)
+        hipEvent_t released = nullptr;        // contraction done: the set may be reused
+        bool used = false;
+    };
+    VertexBatch vertexBatch(int set) const;
     int buildWalkGrid();
     void fetchTerms(const DevBuf<unsigned char>& src, uint32_t n, uint32_t width, double* dst) const;
     uint32_t lastDebugPaths_ = 0;
-    int runBatch(uint64_t pathBegin, uint32_t npaths, double* U, bool timed, uint32_t b);
-    int contractSingle(uint32_t npaths, double* U);
+    int lastSet_ = 0;
+    // pipelined: the three stages on their own streams, ordered by the set's events; otherwise
+    // everything on stream_ (parity inspection)
+    int runBatch(uint64_t pathBegin, uint32_t npaths, double* U, bool timed, uint32_t b, int set, bool pipelined);
+    int contractSingle(uint32_t npaths, double* U, int set, hipStream_t st);
+    int syncAll() const;
+    int orderAfterNee();
+    hipEvent_t cleared_ = nullptr;   // begin()'s clears on the NEE stream
 
     const Medium& medium_;
     MediumView mview_{};            // the medium's view plus this tracer's walk grid
@@ -282,27 +307,25 @@ private:
     bool sameViews_ = false;        // correlation sampler: view2 holds the same sources as view
     bool fp64_;
     int contractKind_;
-    hipStream_t stream_ = nullptr;
+    hipStream_t stream_ = nullptr;    // NEE (and every stage when not pipelined)
+    hipStream_t streamT_ = nullptr;   // transport
+    hipStream_t streamC_ = nullptr;   // contraction
     uint64_t bytes_ = 0;
 
-    DevBuf<uint32_t> k_, misses_, off_, blockSums_, status_;
+    BatchSet sets_[2];
+    DevBuf<uint32_t> status_;
     DevBuf<uint32_t> overflow_;           // bit b: batch b's vertices did not fit the pool
-    DevBuf<float> tauRef_;                // the batch's least view optical depth (relative cut)
     DevBuf<unsigned long long> totals_;   // [0] misses, [1] vertices, [2] NEE voxel steps
-    DevBuf<float4> xm_;
-    DevBuf<float> rho_, delta_;
-    DevBuf<VertexRec> rec_;
     uint64_t cap_ = 0;
     bool poolSized_ = false;   // sizePool ran: the pool holds a batch of this request with margin
-    DevBuf<unsigned char> A_, B_, a_, b_;
-    DevBuf<unsigned char> Ar_, Br_;   // coherent backscattering: reversed-path terms
     DevBuf<double> U_;
     DevBuf<float2> Ff_;             // single-scattering F (complex) when separable
     DevBuf<double2> Fd_;
     Workspace wsMulti_, wsSingle_;
-    // per batch: 0 start, 1 after transport, 5 between k_nee and k_nee_view, 2 after NEE,
-    // 3 after the multiple-scattering contraction, 4 after the single-scattering one
-    static constexpr size_t kEventsPerBatch = 6;
+    // per batch: 0 transport start, 1 transport end (transport stream); 6 NEE start, 5 between
+    // k_nee and k_nee_view, 2 NEE end (NEE stream); 7 contraction start, 3 after the
+    // multiple-scattering contraction, 4 after the single-scattering one (contraction stream)
+    static constexpr size_t kEventsPerBatch = 8;
     // device totals: 0 missed attempts, 1 vertices, 2 k_nee voxel steps, 3 largest batch (vertices),
     // 4 k_nee_view voxel steps
     static constexpr size_t kTotals = 5;

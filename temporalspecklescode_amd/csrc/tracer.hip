@@ -795,7 +795,19 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
         }
         fscalar_ *= fscalar_;
     }
-    if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) { *err = SSTMC_DEVICE_ERROR; return; }
+    if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&streamT_, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&streamC_, hipStreamNonBlocking) != hipSuccess) {
+        *err = SSTMC_DEVICE_ERROR;
+        return;
+    }
+    for (BatchSet& bs : sets_)
+        if (hipEventCreateWithFlags(&bs.transported, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&bs.traced, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&bs.released, hipEventDisableTiming) != hipSuccess) {
+            *err = SSTMC_DEVICE_ERROR;
+            return;
+        }
     if ((*err = buildWalkGrid())) return;
     if ((*err = allocate())) return;
     if (needF) {
@@ -818,89 +830,129 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
 }
 
 NextEventEstimation::~NextEventEstimation() {
-    if (stream_) hipStreamSynchronize(stream_);
+    syncAll();
     for (hipEvent_t e : ev_) hipEventDestroy(e);
+    for (BatchSet& bs : sets_)
+        for (hipEvent_t e : {bs.transported, bs.traced, bs.released})
+            if (e) hipEventDestroy(e);
+    if (cleared_) hipEventDestroy(cleared_);
     if (wsMulti_.ptr) hipFree(wsMulti_.ptr);
     if (wsSingle_.ptr) hipFree(wsSingle_.ptr);
-    if (stream_) hipStreamDestroy(stream_);
+    for (hipStream_t st : {stream_, streamT_, streamC_})
+        if (st) hipStreamDestroy(st);
+}
+
+int NextEventEstimation::syncAll() const {
+    int rc = SSTMC_NO_ERROR;
+    for (hipStream_t st : {streamT_, stream_, streamC_})
+        if (st && hipStreamSynchronize(st) != hipSuccess) rc = SSTMC_KERNEL_ERROR;
+    return rc;
 }
 
 int NextEventEstimation::allocate() {
     int e;
-    if ((e = k_.alloc(P_)) || (e = misses_.alloc(P_)) || (e = off_.alloc(P_ + 1)) ||
-        (e = blockSums_.alloc((P_ + 1023) / 1024 + 1)) || (e = status_.alloc(4)) || (e = totals_.alloc(kTotals)) ||
-        (e = overflow_.alloc((opt_.fullIteration + 31) / 32)) || (e = tauRef_.alloc(1)))
+    if ((e = status_.alloc(4)) || (e = totals_.alloc(kTotals)) || (e = overflow_.alloc((opt_.fullIteration + 31) / 32)))
         return e;
     size_t elem = fp64_ ? 16 : 8;
     size_t na = (size_t)T_ * P_ * L_ * elem, nb = (size_t)T_ * P_ * V_ * elem;
-    if ((e = A_.alloc(na)) || (e = a_.alloc(na)) || (e = B_.alloc(nb)) || (e = b_.alloc(nb))) return e;
-    if (opt_.cbs && ((e = Ar_.alloc(nb)) || (e = Br_.alloc(na)))) return e;
+    for (BatchSet& bs : sets_) {
+        if ((e = bs.k.alloc(P_)) || (e = bs.misses.alloc(P_)) || (e = bs.off.alloc(P_ + 1)) ||
+            (e = bs.blockSums.alloc((P_ + 1023) / 1024 + 1)) || (e = bs.tauRef.alloc(1)))
+            return e;
+        if ((e = bs.A.alloc(na)) || (e = bs.a.alloc(na)) || (e = bs.B.alloc(nb)) || (e = bs.b.alloc(nb))) return e;
+        if (opt_.cbs && ((e = bs.Ar.alloc(nb)) || (e = bs.Br.alloc(na)))) return e;
+        bytes_ += (opt_.cbs ? 3 : 2) * (na + nb) + 20 * (size_t)P_;
+    }
     if ((e = U_.alloc(2 * (size_t)T_ * L_ * V_))) return e;
-    bytes_ += (opt_.cbs ? 3 : 2) * (na + nb) + 16 * (size_t)T_ * L_ * V_ + 20 * (size_t)P_;
+    bytes_ += 16 * (size_t)T_ * L_ * V_;
     return growPool((uint64_t)P_ * 32);
 }
 
-VertexBatch NextEventEstimation::vertexBatch() const {
-    return VertexBatch{k_.p, misses_.p, off_.p, xm_.p, rho_.p, delta_.p, cap_, status_.p, rec_.p};
+VertexBatch NextEventEstimation::vertexBatch(int set) const {
+    const BatchSet& bs = sets_[set];
+    return VertexBatch{bs.k.p, bs.misses.p, bs.off.p, bs.xm.p, bs.rho.p, bs.delta.p, cap_, status_.p, bs.rec.p};
 }
 
+// Both sets' vertex pools hold `cap` vertices; the streams must be idle.
 int NextEventEstimation::growPool(uint64_t cap) {
     int e;
     // vertex records per frame; the correlation sampler's vertices do not move: one record each
     const uint64_t recT = sampler_.correlation() ? 1 : T_;
-    if ((e = xm_.alloc(cap)) || (e = rho_.alloc(cap)) || (e = rec_.alloc(cap * recT))) return e;
-    if (!sampler_.view().trivial) {
-        if ((e = delta_.alloc(cap * T_ * 3))) return e;
-    } else {
-        delta_.release();
+    for (BatchSet& bs : sets_) {
+        if ((e = bs.xm.alloc(cap)) || (e = bs.rho.alloc(cap)) || (e = bs.rec.alloc(cap * recT))) return e;
+        if (!sampler_.view().trivial) {
+            if ((e = bs.delta.alloc(cap * T_ * 3))) return e;
+        } else {
+            bs.delta.release();
+        }
     }
     const uint64_t perVertex = 20 + 32 * recT + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_);
-    bytes_ += (cap - cap_) * perVertex;
+    bytes_ += 2 * (cap - cap_) * perVertex;
     cap_ = cap;
     return SSTMC_NO_ERROR;
 }
 
-int NextEventEstimation::contractSingle(uint32_t npaths, double* U) {
+int NextEventEstimation::contractSingle(uint32_t npaths, double* U, int set, hipStream_t st) {
     FramesView fr = sampler_.view();
-    VertexBatch vb = vertexBatch();
+    VertexBatch vb = vertexBatch(set);
+    const BatchSet& bs = sets_[set];
     hipError_t he;
     if (singleMode_ == kSingleSeparable) {
-        he = launchContract(contractKind_, a_.p, b_.p, npaths, L_, V_, T_, Ff_.p, Fd_.p,
-                            opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0, fscalar_, U, &wsSingle_, stream_);
+        he = launchContract(contractKind_, bs.a.p, bs.b.p, npaths, L_, V_, T_, Ff_.p, Fd_.p,
+                            opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0, fscalar_, U, &wsSingle_, st);
     } else if (singleMode_ == kSingleGauss) {
         const std::vector<double>& vd = static_cast<const GaussianBeam&>(vw_).directions();
         bool uniformDir = true;
         for (size_t i = 3; i < vd.size() && uniformDir; ++i) uniformDir = vd[i] == vd[i % 3];
-        he = launchSingleGauss(fp64_ ? 1 : 0, uniformDir ? 1 : 0, opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0, mview_, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p, npaths,
-                               U, &wsSingle_, stream_);
+        he = launchSingleGauss(fp64_ ? 1 : 0, uniformDir ? 1 : 0, opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0,
+                               mview_, sc_.view(), il_.view(), vw_.view(), fr, vb, bs.a.p, bs.b.p, npaths, U,
+                               &wsSingle_, st);
     } else if (singleMode_ == kSingleCorrelation) {
         const bool uniformFar = il_.type() == SSTMC_SOURCE_FARFIELD && vw_.type() == SSTMC_SOURCE_FARFIELD &&
                                 sc_.isUniform(medium_.materialNum());
-        he = launchSingleCorrelation(dims_, fp64_ ? 1 : 0, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p,
-                                     npaths, opt_.lambda, uniformFar ? 1 : 0, U, &wsSingle_, stream_);
+        he = launchSingleCorrelation(dims_, fp64_ ? 1 : 0, sc_.view(), il_.view(), vw_.view(), fr, vb, bs.a.p, bs.b.p,
+                                     npaths, opt_.lambda, uniformFar ? 1 : 0, U, &wsSingle_, st);
     } else {
-        he = launchSingleGeneric(dims_, fp64_ ? 1 : 0, sc_.view(), il_.view(), vw_.view(), fr, vb, a_.p, b_.p, npaths,
-                                 opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0, U,
-                                 &wsSingle_, stream_);
+        he = launchSingleGeneric(dims_, fp64_ ? 1 : 0, sc_.view(), il_.view(), vw_.view(), fr, vb, bs.a.p, bs.b.p,
+                                 npaths, opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0, U, &wsSingle_, st);
     }
     return he == hipSuccess ? SSTMC_NO_ERROR : SSTMC_KERNEL_ERROR;
 }
 
-int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U, bool timed, uint32_t b) {
+// One batch in three stages: transport (free flight, vertex offsets, totals), NEE (cut reference,
+// first-vertex terms, vertex records, view terms) and contraction (multiple then single
+// scattering into U). Pipelined, each stage runs on its own stream: the transport of a set waits
+// until the set's previous batch has been contracted, the NEE waits for its transport, the
+// contraction for its NEE; batches alternate between the two sets, and the contractions keep
+// batch order on their one stream, so U is summed in the same order as without the pipeline.
+int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U, bool timed, uint32_t b, int set,
+                                  bool pipelined) {
     const MediumView& m = mview_;
     const ScatterView& sc = sc_.view();
     const SourceView& il = il_.view();
     const SourceView& vw = vw_.view();
     const FramesView& fr = sampler_.view();
-    VertexBatch vb = vertexBatch();
+    BatchSet& bs = sets_[set];
+    VertexBatch vb = vertexBatch(set);
+    const hipStream_t sT = pipelined ? streamT_ : stream_, sN = stream_, sC = pipelined ? streamC_ : stream_;
     hipEvent_t* e = timed ? &ev_[(size_t)b * kEventsPerBatch] : nullptr;
-    if (e) hipEventRecord(e[0], stream_);
-    if (launchTransport(dims_, m, sc, il, fr, vb, seed_, pathBegin, npaths, 0, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
-    if (launchScan(k_.p, off_.p, blockSums_.p, npaths, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
-    if (launchTransport(dims_, m, sc, il, fr, vb, seed_, pathBegin, npaths, 1, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
-    if (launchBatchTotals(misses_.p, off_.p, npaths, totals_.p, cap_, overflow_.p, b, tauRef_.p, stream_) != hipSuccess)
+    lastSet_ = set;
+
+    // ---- transport
+    if (bs.used && hipStreamWaitEvent(sT, bs.released, 0) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    bs.used = true;
+    if (e) hipEventRecord(e[0], sT);
+    if (launchTransport(dims_, m, sc, il, fr, vb, seed_, pathBegin, npaths, 0, sT) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    if (launchScan(bs.k.p, bs.off.p, bs.blockSums.p, npaths, sT) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    if (launchTransport(dims_, m, sc, il, fr, vb, seed_, pathBegin, npaths, 1, sT) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    if (launchBatchTotals(bs.misses.p, bs.off.p, npaths, totals_.p, cap_, overflow_.p, b, bs.tauRef.p, sT) != hipSuccess)
         return SSTMC_KERNEL_ERROR;
-    if (e) hipEventRecord(e[1], stream_);
+    if (e) hipEventRecord(e[1], sT);
+    if (hipEventRecord(bs.transported, sT) != hipSuccess) return SSTMC_DEVICE_ERROR;
+
+    // ---- NEE
+    if (pipelined && hipStreamWaitEvent(sN, bs.transported, 0) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    if (e) hipEventRecord(e[6], sN);
     NeeParams prm;
     prm.lambda = opt_.lambda;
     prm.output = opt_.output;
@@ -908,39 +960,46 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     // walks are exact by default (FP32: exp(-105) rounds to zero, so walks stop at 210); with
     // precision.tauCut the view-side terms take the batch-relative cut (k_tau_ref)
     prm.exactCutF = 210.0f;
-    prm.tauRef = opt_.tauCut > 0 ? tauRef_.p : nullptr;
+    prm.tauRef = opt_.tauCut > 0 ? bs.tauRef.p : nullptr;
     prm.tauCutRel = (float)opt_.tauCut;
     prm.npaths = npaths;
     prm.sameViews = sameViews_ ? 1u : 0u;
-    NeeOut out{A_.p, B_.p, a_.p, b_.p, Ar_.p, Br_.p, totals_.p + 2};
-    if (launchNee(dims_, fp64_ ? 1 : 0, m, sc, il, vw, fr, vb, prm, out, stream_, e ? e[5] : nullptr) != hipSuccess)
+    NeeOut out{bs.A.p, bs.B.p, bs.a.p, bs.b.p, bs.Ar.p, bs.Br.p, totals_.p + 2};
+    if (launchNee(dims_, fp64_ ? 1 : 0, m, sc, il, vw, fr, vb, prm, out, sN, e ? e[5] : nullptr) != hipSuccess)
         return SSTMC_KERNEL_ERROR;
-    if (e) hipEventRecord(e[2], stream_);
-    // multiple scattering: U += sum_p A^T B; with coherent backscattering the reversed paths add
-    // sum_p Br^T Ar and both are scaled by 1/sqrt(2) (NextEventEstimation.cuh:1385-1391,1448-1454)
+    if (e) hipEventRecord(e[2], sN);
+    if (hipEventRecord(bs.traced, sN) != hipSuccess) return SSTMC_DEVICE_ERROR;
+
+    // ---- contraction: multiple scattering U += sum_p A^T B; with coherent backscattering the
+    // reversed paths add sum_p Br^T Ar and both are scaled by 1/sqrt(2)
+    // (NextEventEstimation.cuh:1385-1391,1448-1454); then single scattering
+    if (pipelined && hipStreamWaitEvent(sC, bs.traced, 0) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    if (e) hipEventRecord(e[7], sC);
     const double msScale = opt_.cbs ? 0.70710678118654752440 : 1.0;
-    if (launchContract(contractKind_, A_.p, B_.p, npaths, L_, V_, T_, nullptr, nullptr, 0, msScale, U, &wsMulti_,
-                       stream_) != hipSuccess)
+    if (launchContract(contractKind_, bs.A.p, bs.B.p, npaths, L_, V_, T_, nullptr, nullptr, 0, msScale, U, &wsMulti_,
+                       sC) != hipSuccess)
         return SSTMC_KERNEL_ERROR;
-    if (opt_.cbs && launchContract(contractKind_, Br_.p, Ar_.p, npaths, L_, V_, T_, nullptr, nullptr, 0, msScale, U,
-                                   &wsMulti_, stream_) != hipSuccess)
+    if (opt_.cbs && launchContract(contractKind_, bs.Br.p, bs.Ar.p, npaths, L_, V_, T_, nullptr, nullptr, 0, msScale,
+                                   U, &wsMulti_, sC) != hipSuccess)
         return SSTMC_KERNEL_ERROR;
-    if (e) hipEventRecord(e[3], stream_);
-    int rc = contractSingle(npaths, U);
+    if (e) hipEventRecord(e[3], sC);
+    int rc = contractSingle(npaths, U, set, sC);
     if (rc) return rc;
-    if (e) hipEventRecord(e[4], stream_);
+    if (e) hipEventRecord(e[4], sC);
+    if (hipEventRecord(bs.released, sC) != hipSuccess) return SSTMC_DEVICE_ERROR;
     return SSTMC_NO_ERROR;
 }
 
 uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames,
                                         uint64_t cells, bool cbs) {
     uint64_t elem = fp64 ? 16 : 8;
-    uint64_t terms = (cbs ? 3ull : 2ull) * T * P * (uint64_t)(L + V) * elem;
+    // two batch sets (the pipeline's): terms, per-path arrays and vertex pools
+    uint64_t terms = 2 * (cbs ? 3ull : 2ull) * T * P * (uint64_t)(L + V) * elem;
     uint64_t field = 16ull * T * L * V;
-    uint64_t pool = (uint64_t)P * 32 * (20 + 32ull * T + (frames ? 12ull * T : 0));
+    uint64_t pool = 2 * (uint64_t)P * 32 * (20 + 32ull * T + (frames ? 12ull * T : 0));
     uint64_t ws = 2ull * 8 * T * L * V * elem;   // split-K partials, two workspaces
     uint64_t grids = 2 * cells;                  // material grid + packed walk grid (<= 4 bits a cell)
-    return terms + field + pool + ws + grids + 24ull * P + (1ull << 20);
+    return terms + field + pool + ws + grids + 48ull * P + (1ull << 20);
 }
 
 // The NEE walk grid: per cell 4 x the index of its material in the palette of the materials
@@ -1025,13 +1084,15 @@ int NextEventEstimation::buildWalkGrid() {
 // re-run by finish().
 int NextEventEstimation::sizePool() {
     if (poolSized_) return SSTMC_NO_ERROR;
+    if (syncAll()) return SSTMC_KERNEL_ERROR;
     const FramesView& fr = sampler_.view();
-    VertexBatch vb = vertexBatch();
+    VertexBatch vb = vertexBatch(0);
+    BatchSet& bs = sets_[0];
     if (launchTransport(dims_, mview_, sc_.view(), il_.view(), fr, vb, seed_, 0, P_, 0, stream_) != hipSuccess ||
-        launchScan(k_.p, off_.p, blockSums_.p, P_, stream_) != hipSuccess)
+        launchScan(bs.k.p, bs.off.p, bs.blockSums.p, P_, stream_) != hipSuccess)
         return SSTMC_KERNEL_ERROR;
     uint32_t need = 0;
-    if (hipMemcpyAsync(&need, off_.p + P_, sizeof(need), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+    if (hipMemcpyAsync(&need, bs.off.p + P_, sizeof(need), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
         hipStreamSynchronize(stream_) != hipSuccess)
         return SSTMC_DEVICE_ERROR;
     const uint64_t want = (uint64_t)need + need / 4 + 1024;
@@ -1060,12 +1121,22 @@ int NextEventEstimation::begin() {
     if (hipMemsetAsync(overflow_.p, 0, overflow_.n * sizeof(uint32_t), stream_) != hipSuccess)
         return SSTMC_DEVICE_ERROR;
     if (launchZeroU(U_.p, U_.n, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+    return orderAfterNee();
+}
+
+// The transport and contraction streams wait for the work already queued on the NEE stream (the
+// clears of the totals, flags and U).
+int NextEventEstimation::orderAfterNee() {
+    if (!cleared_ && hipEventCreateWithFlags(&cleared_, hipEventDisableTiming) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    if (hipEventRecord(cleared_, stream_) != hipSuccess || hipStreamWaitEvent(streamT_, cleared_, 0) != hipSuccess ||
+        hipStreamWaitEvent(streamC_, cleared_, 0) != hipSuccess)
+        return SSTMC_DEVICE_ERROR;
     return SSTMC_NO_ERROR;
 }
 
 int NextEventEstimation::enqueueBatch(uint32_t b) {
     if (b >= opt_.fullIteration) return SSTMC_INVALID_CONFIG;
-    return runBatch((uint64_t)b * P_, P_, U_.p, true, b);
+    return runBatch((uint64_t)b * P_, P_, U_.p, true, b, (int)(b & 1u), true);
 }
 
 int NextEventEstimation::enqueue() {
@@ -1085,7 +1156,7 @@ int NextEventEstimation::finish(double* u, bool deviceDst) {
     uint32_t st[4];
     unsigned long long tot[kTotals];
     for (int round = 0;; ++round) {
-        if (hipStreamSynchronize(stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
+        if (syncAll()) return SSTMC_KERNEL_ERROR;
         if (hipMemcpy(st, status_.p, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
         if (hipMemcpy(tot, totals_.p, sizeof(tot), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
         if (st[1]) return SSTMC_NO_HIT;
@@ -1103,9 +1174,10 @@ int NextEventEstimation::finish(double* u, bool deviceDst) {
         if (hipMemsetAsync(status_.p, 0, sizeof(uint32_t), stream_) != hipSuccess ||
             hipMemsetAsync(overflow_.p, 0, overflow_.n * sizeof(uint32_t), stream_) != hipSuccess)
             return SSTMC_DEVICE_ERROR;
+        if ((rc = orderAfterNee())) return rc;
         for (uint32_t b = 0; b < opt_.fullIteration; ++b)
             if (bits[b >> 5] & (1u << (b & 31)))
-                if ((rc = runBatch((uint64_t)b * P_, P_, U_.p, true, b))) return rc;
+                if ((rc = runBatch((uint64_t)b * P_, P_, U_.p, true, b, (int)(b & 1u), true))) return rc;
     }
     zeroContributionPaths_ = tot[0];
     photonsBudget_ = tot[1];
@@ -1118,9 +1190,9 @@ int NextEventEstimation::finish(double* u, bool deviceDst) {
         float ms;
         hipEvent_t* e = &ev_[(size_t)b * kEventsPerBatch];
         if (hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess) t.transport_ms += ms;
-        if (hipEventElapsedTime(&ms, e[1], e[2]) == hipSuccess) t.nee_ms += ms;
+        if (hipEventElapsedTime(&ms, e[6], e[2]) == hipSuccess) t.nee_ms += ms;
         if (hipEventElapsedTime(&ms, e[5], e[2]) == hipSuccess) t.nee_view_ms += ms;
-        if (hipEventElapsedTime(&ms, e[2], e[3]) == hipSuccess) t.contract_ms += ms;
+        if (hipEventElapsedTime(&ms, e[7], e[3]) == hipSuccess) t.contract_ms += ms;
         if (hipEventElapsedTime(&ms, e[3], e[4]) == hipSuccess) t.single_ms += ms;
     }
     float ms;
@@ -1163,8 +1235,8 @@ void NextEventEstimation::fetchTerms(const DevBuf<unsigned char>& src, uint32_t 
 
 int NextEventEstimation::debugCbsTerms(uint32_t n, double* termAr, double* termBr) const {
     if (!opt_.cbs || n != lastDebugPaths_ || n == 0) return SSTMC_INVALID_CONFIG;
-    if (termAr) fetchTerms(Ar_, n, V_, termAr);
-    if (termBr) fetchTerms(Br_, n, L_, termBr);
+    if (termAr) fetchTerms(sets_[0].Ar, n, V_, termAr);
+    if (termBr) fetchTerms(sets_[0].Br, n, L_, termBr);
     return SSTMC_NO_ERROR;
 }
 
@@ -1176,12 +1248,14 @@ int NextEventEstimation::debugBatch(uint64_t pathBegin, uint32_t n, uint64_t* nv
     DevBuf<double> Ub;
     int rc;
     if ((rc = Ub.alloc(U_.n))) return rc;
+    if (syncAll()) return SSTMC_KERNEL_ERROR;
+    const BatchSet& bs = sets_[0];
     for (int attempt = 0; attempt < 8; ++attempt) {
         hipMemsetAsync(status_.p, 0, 4 * sizeof(uint32_t), stream_);
         hipMemsetAsync(totals_.p, 0, kTotals * sizeof(unsigned long long), stream_);
         hipMemsetAsync(overflow_.p, 0, overflow_.n * sizeof(uint32_t), stream_);
         launchZeroU(Ub.p, Ub.n, stream_);
-        if ((rc = runBatch(pathBegin, n, Ub.p, false, 0))) return rc;
+        if ((rc = runBatch(pathBegin, n, Ub.p, false, 0, 0, false))) return rc;
         if (hipStreamSynchronize(stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
         uint32_t st[4];
         unsigned long long tot[kTotals];
@@ -1194,9 +1268,9 @@ int NextEventEstimation::debugBatch(uint64_t pathBegin, uint32_t n, uint64_t* nv
             continue;
         }
         std::vector<uint32_t> k(n), ms(n), off(n + 1);
-        hipMemcpy(k.data(), k_.p, n * 4, hipMemcpyDeviceToHost);
-        hipMemcpy(ms.data(), misses_.p, n * 4, hipMemcpyDeviceToHost);
-        hipMemcpy(off.data(), off_.p, (n + 1) * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(k.data(), bs.k.p, n * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(ms.data(), bs.misses.p, n * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(off.data(), bs.off.p, (n + 1) * 4, hipMemcpyDeviceToHost);
         uint64_t nv = off[n];
         if (nvtxOut) *nvtxOut = nv;
         if (!pathK) return SSTMC_NO_ERROR;
@@ -1204,12 +1278,12 @@ int NextEventEstimation::debugBatch(uint64_t pathBegin, uint32_t n, uint64_t* nv
         pathOff[n] = nv;
         std::vector<float4> xm(nv);
         std::vector<float> rho(nv), dl;
-        hipMemcpy(xm.data(), xm_.p, nv * sizeof(float4), hipMemcpyDeviceToHost);
-        hipMemcpy(rho.data(), rho_.p, nv * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(xm.data(), bs.xm.p, nv * sizeof(float4), hipMemcpyDeviceToHost);
+        hipMemcpy(rho.data(), bs.rho.p, nv * 4, hipMemcpyDeviceToHost);
         const FramesView& fr = sampler_.view();
         if (!fr.trivial) {
             dl.resize(nv * T_ * 3);
-            hipMemcpy(dl.data(), delta_.p, dl.size() * 4, hipMemcpyDeviceToHost);
+            hipMemcpy(dl.data(), bs.delta.p, dl.size() * 4, hipMemcpyDeviceToHost);
         }
         for (uint64_t o = 0; o < nv; ++o) {
             vtxPos[3 * o] = xm[o].x; vtxPos[3 * o + 1] = xm[o].y; vtxPos[3 * o + 2] = xm[o].z;
@@ -1244,10 +1318,10 @@ int NextEventEstimation::debugBatch(uint64_t pathBegin, uint32_t n, uint64_t* nv
             fetchTerms(src, n, width, dst);
         };
         lastDebugPaths_ = n;
-        if (termA) fetch(A_, L_, termA);
-        if (termB) fetch(B_, V_, termB);
-        if (termSa) fetch(a_, L_, termSa);
-        if (termSb) fetch(b_, V_, termSb);
+        if (termA) fetch(bs.A, L_, termA);
+        if (termB) fetch(bs.B, V_, termB);
+        if (termSa) fetch(bs.a, L_, termSa);
+        if (termSb) fetch(bs.b, V_, termSb);
         if (uBatch) hipMemcpy(uBatch, Ub.p, Ub.n * sizeof(double), hipMemcpyDeviceToHost);
         return SSTMC_NO_ERROR;
     }
@@ -1256,13 +1330,15 @@ int NextEventEstimation::debugBatch(uint64_t pathBegin, uint32_t n, uint64_t* nv
 
 int NextEventEstimation::tauRef(float* out) const {
     if (!(opt_.tauCut > 0)) return SSTMC_INVALID_CONFIG;
-    if (hipStreamSynchronize(stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
-    return hipMemcpy(out, tauRef_.p, sizeof(float), hipMemcpyDeviceToHost) == hipSuccess ? SSTMC_NO_ERROR
-                                                                                     : SSTMC_DEVICE_ERROR;
+    if (syncAll()) return SSTMC_KERNEL_ERROR;
+    return hipMemcpy(out, sets_[lastSet_].tauRef.p, sizeof(float), hipMemcpyDeviceToHost) == hipSuccess
+               ? SSTMC_NO_ERROR
+               : SSTMC_DEVICE_ERROR;
 }
 
 int NextEventEstimation::debugContract(uint32_t n, const double* a, const double* b, double* u) {
     if (n == 0) return SSTMC_INVALID_CONFIG;
+    if (syncAll()) return SSTMC_KERNEL_ERROR;
     size_t na = (size_t)T_ * n * L_, nb = (size_t)T_ * n * V_;
     DevBuf<unsigned char> da, db;
     DevBuf<double> du;
