@@ -1,20 +1,27 @@
 """Benchmark: path-contributions/s of the NEE speckle tracer on one MI355X.
 
-Workload (default): the 1024 x 1024 transmission-matrix render -- 128^3 voxel grid with four
-Perlin-thresholded materials, HG g = 0.9, 1024 plane-wave illuminations x 1024 plane-wave views,
-one frame, lambda = 1. A "step" is one batch of ``--batch`` paths (default 2^16) traced end to end:
-free flight, NEE to every illumination and view, and the complex contraction into the 1024 x 1024
-field. The timed region is ``--steps`` batches on the device with the medium and tables already
+Workload (default `tm1024`): the 1024 x 1024 transmission-matrix render -- 128^3 voxel grid with
+four Perlin-thresholded materials, HG g = 0.9, 1024 plane-wave illuminations x 1024 plane-wave
+views, one frame, lambda = 1. A "step" is one batch of ``--batch`` paths (default 2^16) traced end
+to end: free flight, NEE to every illumination and view, and the complex contraction into the
+1024 x 1024 field. The timed region is ``--steps`` batches with the medium and tables already
 resident in HBM; the value is paths * L * V * frames / wall time.
 
-The JSON line also carries:
-  roofline      the dominant kernel's achieved rate against its MI355X peak (HIP events on the
-                tracer's stream); the contraction's MFMA rate is reported beside it;
-  accuracy      relative Frobenius error of the mixed-precision field against the FP64 terms +
-                FP64 contraction on the same paths (one batch);
-  cpu_baseline  the FP64 CPU restatement (OpenMP) on a bounded sample of the same workload;
-  plain_gpu     the same tracer without split precision (FP32 terms + FP32 contraction, and FP64
-                terms + FP64 contraction) on the same workload, with mixed/plain ratios.
+The measured form is the mixed-precision one: FP32 transport and terms, FP64 path length and
+phase, the batch-relative attenuation cut (precision.tauCut, default 24: view-side terms
+attenuated below exp(-12) of the batch's least attenuated view term are dropped) and the bf16x3
+split-product MFMA contraction. The JSON line also carries:
+  roofline              the dominant kernel (the view-term kernel) against HBM, with its VALU
+                        issue rate beside it, from HIP events on its stream;
+  roofline_contraction  the contraction's algorithmic MFMA rate;
+  accuracy              the field of the measured form against FP64 terms and FP64 contraction
+                        without the cut, on one whole batch of the same paths (GPU), and against
+                        the FP64 CPU restatement on a small batch (cpu_baseline leg);
+  plain_gpu             the same tracer without split precision or cut (FP32 terms + FP32
+                        contraction, and FP64 + FP64), with mixed/plain ratios;
+  serving               a replay of the mixed request stream (three preloaded media, L and V in
+                        [16, 512], 1-8 lags, Poisson arrivals) with p50/p99 latency;
+  cpu_baseline          the FP64 CPU restatement (OpenMP) on a bounded sample of the workload.
 """
 from __future__ import annotations
 
@@ -32,40 +39,55 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 VALU_PEAK_LANE_INSTR = 157.3e12 / 2       # published FP32 vector peak in lane-FMAs (2 flops each) per s
+METRIC = "path-contributions/sec (L×V complex) at err≤1e-3 vs FP64; p50/p99 render ms"
 
 
 def workload(name: str, batch: int, steps: int):
     from temporalspecklescode_amd import workloads as W
-    if name == "tm1024":
-        cfg = W.c4(paths=batch * steps, batch=batch)
-        return cfg, 3
-    if name == "hetero64":
-        cfg = W.c2(paths=batch * steps, batch=batch)
-        return cfg, 3
-    if name == "slab2d":
-        cfg = W.c1(paths=batch * steps, batch=batch)
-        return cfg, 2
-    if name == "corr128":
-        cfg = W.c3(paths=batch * steps, batch=batch)
-        return cfg, 3
-    if name == "acorr128":
-        cfg = W.c3_analytic(paths=batch * steps, batch=batch)
-        return cfg, 3
-    if name == "confocal":
+    makers = {
+        "tm1024": lambda: W.c4(paths=batch * steps, batch=batch),
+        "hetero64": lambda: W.c2(paths=batch * steps, batch=batch),
+        "slab2d": lambda: W.c1(paths=batch * steps, batch=batch),
+        "corr128": lambda: W.c3(paths=batch * steps, batch=batch),
+        "acorr128": lambda: W.c3_analytic(paths=batch * steps, batch=batch),
         # the confocal example's geometry: 1111 GaussianBeam illuminations, 3333 GaussianBeam views
-        cfg = W.confocal(batch=batch, full_iteration=steps)
-        return cfg, 3
-    raise SystemExit("unknown workload %s" % name)
+        "confocal": lambda: W.confocal(batch=batch, full_iteration=steps),
+    }
+    if name not in makers:
+        raise SystemExit("unknown workload %s" % name)
+    return makers[name](), 2 if name == "slab2d" else 3
 
 
-def cpu_baseline(name: str, seconds: float, threads: int):
+def host_cpu():
+    """CPU model, the host's logical CPUs and the CPUs this process may use (its share of the box)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
+
+
+def cpu_baseline(name: str, seconds: float, threads: int, mixed_prec: dict):
+    """The FP64 CPU restatement (OpenMP) rendering a growing sample of the workload's paths until
+    it has run for about seconds/4; its rate is the baseline. The same leg checks the measured
+    form's field against the restatement: a 128-path batch traced on the GPU in the measured form
+    and the restatement's FP64 terms on the GPU's own vertices."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
+    from temporalspecklescode_amd import Tracer
     from temporalspecklescode_amd.config import sizes
     n = 8
     cfg, dims = workload(name, n, 1)
     nl, nv, nt, _ = sizes(cfg, dims)
-    elapsed = 0.0
     while True:
         cfg, dims = workload(name, n, 1)
         t0 = time.perf_counter()
@@ -75,42 +97,63 @@ def cpu_baseline(name: str, seconds: float, threads: int):
             break
         n *= 4
     rate = n / elapsed
-    return {"value": rate * nl * nv * nt, "unit": "path-contributions/s", "cores": threads, "kind": "port",
-            "sample": "%d paths of the same workload (seed and batch as the GPU run's first paths), %.1f s, "
-                      "FP64 scalar restatement, OpenMP" % (n, elapsed),
-            "paths_per_s": rate}
+    model, nproc, usable = host_cpu()
+    out = {"value": rate * nl * nv * nt, "unit": "path-contributions/s", "cores": threads, "kind": "port",
+           "sample": "%d paths of the same workload (the GPU run's seed, path ids 0..%d), %.1f s, FP64 scalar "
+                     "restatement, OpenMP, %d threads" % (n, n - 1, elapsed, threads),
+           "paths_per_s": rate, "cpu_model": model, "nproc": nproc, "cpus_usable": usable}
+    # accuracy of the measured form against the restatement on the same vertices
+    m = 128
+    acfg, dims = workload(name, m, 1)
+    acfg["precision"] = mixed_prec
+    with Tracer(acfg, dims) as tr:
+        d = tr.debug_batch(0, m)
+    A, B, S = O.terms_from_vertices(dict(acfg, precision={"terms": "fp64", "contraction": "fp64"}), dims, d["k"],
+                                    d["off"], d["pos"].astype(np.float64), d["mat"], d["rho"].astype(np.float64),
+                                    d["xt"], d["len"], per_path_single=False)
+    U = S.transpose(2, 1, 0).copy()
+    for t in range(A.shape[1]):
+        U[:, :, t] += A[:, t, :].T @ B[:, t, :]
+    out["check_vs_oracle"] = {"paths": m, "rel_fro_err": float(np.linalg.norm(d["u"] - U) / np.linalg.norm(U))}
+    return out
+
+
+def accuracy(cfg: dict, dims: int, batch: int):
+    """The measured form's batch field against FP64 terms + FP64 contraction without the cut, on
+    the same batch of paths (path ids 0..batch-1)."""
+    from temporalspecklescode_amd import Tracer
+    a = dict(cfg)
+    a["simulation"] = dict(cfg["simulation"], fullIteration=1, batchSize=batch)
+    with Tracer(a, dims) as tm:
+        dm = tm.debug_batch(0, batch)
+    a["precision"] = {"terms": "fp64", "contraction": "fp64"}
+    with Tracer(a, dims) as t64:
+        d64 = t64.debug_batch(0, batch)
+    err = float(np.linalg.norm(dm["u"] - d64["u"]) / max(1e-300, np.linalg.norm(d64["u"])))
+    return {"rel_fro_err_vs_fp64": err, "paths": batch, "same_vertices": bool(np.array_equal(dm["k"], d64["k"])),
+            "reference": "FP64 terms + FP64 contraction, no attenuation cut, same vertices (GPU; the FP64 path "
+                         "matches the CPU restatement to 1e-10 at this geometry, tests/test_gpu_workloads.py)"}
 
 
 def plain_gpu(cfg: dict, dims: int, batches: int, mixed_value: float):
-    """The same tracer without split precision (the plain GPU tracer): FP32 transport, FP64
-    path length and phase, and (i) FP32 terms with an FP32 FMA contraction, (ii) FP64 terms with
-    an FP64 contraction. Each is timed over `batches` batches (the kernels are already loaded by the
-    mixed run) and its field error is taken against FP64 terms + FP64 contraction on the same
-    4096 paths."""
+    """The same tracer without split precision or cut: FP32 terms + FP32 contraction, and FP64
+    terms + FP64 contraction, each timed over `batches` batches after one untimed trace."""
     from temporalspecklescode_amd import Tracer
     from temporalspecklescode_amd.config import sizes
     nl, nv, nt, _ = sizes(cfg, dims)
     batch = cfg["simulation"]["batchSize"]
     out = {}
-    ref = None
-    for name, prec in (("fp64", {"terms": "fp64", "contraction": "fp64"}), ("fp32", {"terms": "fp32", "contraction": "fp32"})):
+    for name, prec in (("fp32", {"terms": "fp32", "contraction": "fp32"}),
+                       ("fp64", {"terms": "fp64", "contraction": "fp64"})):
         c = dict(cfg, precision=prec)
         c["simulation"] = dict(cfg["simulation"], fullIteration=batches)
         with Tracer(c, dims) as tr:
+            tr.trace()
             t0 = time.perf_counter()
             tr.trace()
             wall = time.perf_counter() - t0
-        a = dict(c)
-        n = min(batch, 4096)
-        a["simulation"] = dict(cfg["simulation"], fullIteration=1, batchSize=n)
-        with Tracer(a, dims) as td:
-            u = td.debug_batch(0, n)["u"]
-        if ref is None:
-            ref = u
-        err = float(np.linalg.norm(u - ref) / max(1e-300, np.linalg.norm(ref)))
         value = batch * batches * nl * nv * nt / wall
-        out[name] = {"value": value, "ms_per_step": wall * 1e3 / batches, "steps": batches,
-                     "rel_fro_err_vs_fp64": err, "precision": prec}
+        out[name] = {"value": value, "ms_per_step": wall * 1e3 / batches, "steps": batches, "precision": prec}
         print("bench: plain %s %.3e path-contributions/s (%.1f ms/batch)" % (name, value, wall * 1e3 / batches),
               file=sys.stderr, flush=True)
     out["mixed_over_plain_fp32"] = mixed_value / out["fp32"]["value"]
@@ -118,25 +161,28 @@ def plain_gpu(cfg: dict, dims: int, batches: int, mixed_value: float):
     return out
 
 
-def serving(n_requests: int, max_paths_log2: int):
-    """Replays the mixed request stream (Poisson arrivals, 2D/3D, three media, L and V in
-    [16, 512], 1-8 lags of linear motion) through the continuous-batching server; returns
-    latency percentiles and throughput."""
+def serving(n_requests: int, max_paths_log2: int, prec: dict):
+    """Replays the mixed request stream (Poisson arrivals at 20/s, 2D/3D, three preloaded media,
+    L and V log-uniform in [16, 512], 1-8 lags of linear motion) through the continuous-batching
+    server in the measured form; latency percentiles and throughput."""
     from temporalspecklescode_amd import workloads as W
     from temporalspecklescode_amd.serving import Server
-    from temporalspecklescode_amd.config import sizes
     stream = W.serving_stream(n_requests, max_paths_log2=max_paths_log2)
-    t0 = time.perf_counter()
-    ids = []
-    contrib = 0
+    for _, _, cfg in stream:
+        cfg["precision"] = dict(prec)
     with Server(max_inflight=8) as srv:
+        media = {}
+        for _, dims, cfg in stream:   # preload each distinct medium once
+            media.setdefault((dims, len(cfg["medium"]["xAxis"])), (dims, cfg))
+        for dims, cfg in media.values():
+            srv.preload(cfg, dims)
+        t0 = time.perf_counter()
+        ids = []
         for (arrival, dims, cfg) in stream:
             now = time.perf_counter() - t0
             if arrival > now:
                 time.sleep(arrival - now)
             ids.append(srv.submit(cfg, dims))
-            nl, nv, nt, _ = sizes(cfg, dims)
-            contrib += cfg["simulation"]["batchSize"] * cfg["simulation"]["fullIteration"] * nl * nv * nt
         for rid in ids:
             srv.wait(rid)
             srv.fetch(rid)
@@ -145,8 +191,9 @@ def serving(n_requests: int, max_paths_log2: int):
     print("bench: served %d requests in %.2f s, p50 %.1f ms p99 %.1f ms" % (n_requests, wall, st["p50_ms"],
                                                                          st["p99_ms"]), file=sys.stderr, flush=True)
     return {"requests": n_requests, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
-            "paths_per_s": st["paths_per_s"], "contributions_per_s": st["contributions_per_s"],
-            "wall_s": wall, "max_paths": 1 << max_paths_log2, "arrival_rate_per_s": 20.0}
+            "paths_per_s": st["paths_per_s"], "contributions_per_s": st["contributions_per_s"], "wall_s": wall,
+            "max_paths": 1 << max_paths_log2, "arrival_rate_per_s": 20.0, "preloaded_media": len(media),
+            "precision": prec}
 
 
 def main():
@@ -157,28 +204,31 @@ def main():
     ap.add_argument("--batch", type=int, default=1 << 16)
     ap.add_argument("--workload", default="tm1024")
     ap.add_argument("--contraction", default="bf16x3")
+    ap.add_argument("--tau-cut", type=float, default=24.0, help="precision.tauCut of the measured form (0: exact)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--serving", type=int, default=0, help="requests of the mixed stream to serve after the "
+    ap.add_argument("--serving", type=int, default=32, help="requests of the mixed stream to serve after the "
                     "throughput measurement (0: skip)")
     ap.add_argument("--serving-max-paths-log2", type=int, default=18)
     ap.add_argument("--plain-steps", type=int, default=2, help="batches timed for each plain (no split "
-                    "precision) variant of the same workload (0: skip)")
+                    "precision, no cut) variant of the same workload (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: runs on one GPU; sharding paths over GPUs by path-id range is not implemented, "
-              "so --gpus %d is not measured" % args.gpus, file=sys.stderr)
+        print("bench.py: this build runs and measures one GPU; paths would shard over GPUs by path-id range "
+              "with one all-reduce of the field, which is not implemented, so --gpus %d is not measured"
+              % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from temporalspecklescode_amd import Tracer
     from temporalspecklescode_amd.config import sizes
 
     cfg, dims = workload(args.workload, args.batch, args.steps)
-    cfg["precision"] = {"terms": "fp32", "contraction": args.contraction}
+    prec = {"terms": "fp32", "contraction": args.contraction, "tauCut": args.tau_cut}
+    cfg["precision"] = prec
     nl, nv, nt, _ = sizes(cfg, dims)
-    print("bench: workload %s, batch %d, steps %d, L %d V %d T %d" % (args.workload, args.batch, args.steps,
-                                                                     nl, nv, nt), file=sys.stderr, flush=True)
+    print("bench: workload %s, batch %d, steps %d, L %d V %d T %d, precision %s" % (
+        args.workload, args.batch, args.steps, nl, nv, nt, json.dumps(prec)), file=sys.stderr, flush=True)
     if args.warmup > 0:
         wcfg = dict(cfg)
         wcfg["simulation"] = dict(cfg["simulation"], fullIteration=args.warmup, seed=cfg["simulation"]["seed"] + 1000)
@@ -195,79 +245,71 @@ def main():
     value = contributions / wall
     print("bench: timed %.3f s, stages %s" % (wall, json.dumps(tm)), file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel (k_nee_view, the multiple-scattering view terms: >90% of the
-    # device time), timed by HIP events around its launches on the tracer's stream
+    # roofline of the dominant kernel: the multiple-scattering view terms (k_nee_view_far for
+    # far-field views in FP32), timed by HIP events around its launches on its stream
     nb = max(1, tm["batches"])
     view_ms = tm["nee_view_ms"] / nb
-    con_ms = (tm["contract_ms"] + tm["single_ms"]) / nb
     vtx_launch = tm["vertices"] / nb
     # algorithmic bytes of the tracing stage: 8 (L+V) T per path (complex64 term slabs written
     # once for the contraction) + (64 + 16 T) per vertex (vertex state in and out)
     alg_bytes = args.batch * 8.0 * (nl + nv) * nt + vtx_launch * (64.0 + 16.0 * nt)
     gbs = alg_bytes / (view_ms * 1e-3) / 1e9
-    steps_s = tm["dda_steps_view"] / max(1e-9, tm["nee_view_ms"] * 1e-3)   # k_nee_view's own walks
+    steps_s = tm["dda_steps_view"] / max(1e-9, tm["nee_view_ms"] * 1e-3)
     prof = {}
     tpath = os.path.join(ROOT, "profiles", "counters_%s.json" % args.workload)
     if os.path.exists(tpath):
         with open(tpath) as f:
             prof = json.load(f)
-    traffic = None
-    if prof.get("batch") == args.batch and "k_nee_view_hbm_bytes" in prof:
-        traffic = prof["k_nee_view_hbm_bytes"]
-    roof = {"kernel": "k_nee_view", "bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 6), "traffic": traffic, "avg_launch_ms": round(view_ms, 3),
-            "algorithmic_bytes_per_launch": alg_bytes, "voxel_steps_per_s": steps_s}
-    # the walk is VALU-issue-bound, not HBM-bound: VALU lane-instructions per voxel step (from the
-    # SQ_INSTS_VALU counter over the same kernel, inactive lanes included) against MI355X's
-    # published FP32 vector peak counted in lane-instructions (157.3 TFLOP/s / 2 flops per FMA)
-    if "k_nee_view_valu_lane_instr_per_step" in prof:
-        ips = prof["k_nee_view_valu_lane_instr_per_step"]
+    same_form = prof.get("batch") == args.batch and prof.get("precision") == prec
+    roof = {"kernel": prof.get("view_kernel", "k_nee_view_far"), "bound": "hbm", "achieved": round(gbs, 3),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 6),
+            "traffic": prof.get("view_hbm_bytes") if same_form else None, "avg_launch_ms": round(view_ms, 3),
+            "algorithmic_bytes_per_launch": alg_bytes, "voxel_steps_per_s": steps_s,
+            "limiter": "valu issue (the walk's grid reads hit L2; see valu)"}
+    # the walk is bound by VALU issue, not HBM: VALU lane-instructions per voxel step (SQ_INSTS_VALU
+    # over the same kernel, inactive lanes included) against MI355X's published FP32 vector peak
+    # counted in lane-instructions (157.3 TFLOP/s / 2 flops per FMA)
+    if same_form and "view_valu_lane_instr_per_step" in prof:
+        ips = prof["view_valu_lane_instr_per_step"]
         ach = ips * steps_s
         roof["valu"] = {"lane_instr_per_step": ips, "achieved": ach, "peak": VALU_PEAK_LANE_INSTR,
                         "unit": "lane-instr/s", "frac": round(ach / VALU_PEAK_LANE_INSTR, 4)}
-    flops = 2 * 8.0 * args.batch * nl * nv * nt   # multiple + single contraction, 8 flops each
+    flops = 8.0 * args.batch * nl * nv * nt   # per contraction launch set: 8 flops per path-contribution
+    con_ms = tm["contract_ms"] / nb
     con_tflops = flops / (con_ms * 1e-3) / 1e12
     roof_con = {"kernel": "k_contract_bf16x3" if args.contraction == "bf16x3" else "k_contract_fma",
                 "bound": "mfma", "achieved": round(con_tflops, 3), "peak": MFMA_BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(con_tflops / MFMA_BF16_PEAK_TFLOPS, 6), "traffic": None,
-                "avg_launch_ms": round(con_ms / 2, 4)}
-
-    accuracy = None
-    if not args.no_accuracy:
-        acfg = dict(cfg)
-        acfg["simulation"] = dict(cfg["simulation"], fullIteration=1)
-        n = min(args.batch, 4096)
-        acfg["simulation"]["batchSize"] = n
-        with Tracer(acfg, dims) as tm32:
-            d32 = tm32.debug_batch(0, n)
-        acfg["precision"] = {"terms": "fp64", "contraction": "fp64"}
-        with Tracer(acfg, dims) as tm64:
-            d64 = tm64.debug_batch(0, n)
-        same = np.array_equal(d32["k"], d64["k"])
-        err = float(np.linalg.norm(d32["u"] - d64["u"]) / max(1e-300, np.linalg.norm(d64["u"])))
-        accuracy = {"rel_fro_err_vs_fp64": err, "paths": n, "same_vertices": bool(same)}
+                "avg_launch_ms": round(con_ms, 4)}
+    if same_form and "contract_mfma_busy_pct" in prof:
+        roof_con["mfma_busy_pct_counters"] = prof["contract_mfma_busy_pct"]
 
     out = {
-        "metric": "path-contributions/sec (L×V complex) at err≤1e-3 vs FP64; p50/p99 render ms",
+        "metric": METRIC,
         "value": value, "unit": "path-contributions/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 transport+terms, fp64 phase, bf16x3 contraction" if args.contraction == "bf16x3" else "fp32",
+        "dtype": "fp32 transport+terms, fp64 phase, bf16x3 contraction" if args.contraction == "bf16x3" else
+                 "fp32 transport+terms, fp64 phase, %s contraction" % args.contraction,
         "data": "synthetic (seeded Perlin medium, counter-based Philox paths)",
         "config": {"workload": args.workload, "batch_paths": args.batch, "L": nl, "V": nv, "frames": nt,
-                   "paths": paths, "contraction": args.contraction},
+                   "paths": paths, "contraction": args.contraction, "tau_cut": args.tau_cut},
         "paths_per_s": paths / wall, "vertices_per_s": tm["vertices"] / wall,
         "roofline": roof, "roofline_contraction": roof_con,
-        "accuracy": accuracy,
-        "stage_ms": {k: tm[k] for k in ("transport_ms", "nee_ms", "nee_view_ms", "contract_ms", "single_ms")},
+        "stage_ms": {k: tm[k] for k in ("transport_ms", "nee_ms", "nee_view_ms", "contract_ms", "single_ms",
+                                        "total_ms")},
     }
+    if not args.no_accuracy:
+        out["accuracy"] = accuracy(cfg, dims, args.batch)
+        print("bench: accuracy %s" % json.dumps(out["accuracy"]), file=sys.stderr, flush=True)
     if args.plain_steps > 0:
         out["plain_gpu"] = plain_gpu(cfg, dims, args.plain_steps, value)
     if args.serving > 0:
-        out["serving"] = serving(args.serving, args.serving_max_paths_log2)
+        out["serving"] = serving(args.serving, args.serving_max_paths_log2, prec)
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        threads = max(1, min(threads, 16))
-        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds, threads)
+        _, _, usable = host_cpu()
+        threads = int(os.environ.get("OMP_NUM_THREADS", usable))
+        threads = max(1, min(threads, usable))
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds, threads, prec)
     print(json.dumps(out), flush=True)
 
 

@@ -318,6 +318,12 @@ typedef struct sstmc_server_stats {
 
 int sstmc_server_create(uint32_t device, uint64_t hbm_budget_bytes, uint32_t max_inflight,
                         sstmc_server** out);
+/* Builds the medium of cfg (grid, cross sections and its NEE walk grids) on the server's device
+ * and keeps it in the server's medium cache, so that requests on the same medium (equal edges,
+ * grid and cross sections) start without copying or building it. Requests also add the media
+ * they build to the cache (at most 16 media; the least recently used one no request holds is
+ * dropped). No counterpart in the reference. */
+int sstmc_server_preload(sstmc_server* s, const sstmc_config* cfg);
 /* Queues cfg: renderingsNum renderings of fullIteration batches each, rendering i at seed + i, as
  * sstmc_render does (MatlabInterface.cu:1797-1800); returns a request id. The config is copied
  * whole (every array it points into), so the caller may free or reuse its arrays on return. */

@@ -17,7 +17,7 @@ EXPORTS = [
     "sstmc_last_timing", "sstmc_debug_batch", "sstmc_debug_contract", "sstmc_server_create",
     "sstmc_submit", "sstmc_poll", "sstmc_wait", "sstmc_fetch", "sstmc_server_stats_get",
     "sstmc_server_destroy", "sstmc_validate", "sstmc_debug_cbs", "sstmc_debug_tau_ref",
-    "sstmc_contract_noalloc",
+    "sstmc_contract_noalloc", "sstmc_server_preload",
 ]
 
 
@@ -67,6 +67,7 @@ def lib() -> C.CDLL:
     L.sstmc_contract_noalloc.argtypes = [C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, vp, vp]
     L.sstmc_server_create.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(vp)]
     L.sstmc_submit.argtypes = [vp, vp, C.POINTER(C.c_uint64)]
+    L.sstmc_server_preload.argtypes = [vp, vp]
     L.sstmc_poll.argtypes = [vp, C.c_uint64]
     L.sstmc_wait.argtypes = [vp, C.c_uint64]
     L.sstmc_fetch.argtypes = [vp, C.c_uint64, vp, vp]

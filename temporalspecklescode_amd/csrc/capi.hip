@@ -19,7 +19,7 @@ struct sstmc_handle {
     uint32_t renderings = 1;
     uint64_t seed = 0;
     std::unique_ptr<Scattering> scattering;
-    std::unique_ptr<Medium> medium;
+    std::shared_ptr<Medium> medium;   // shared with the serving loop's medium cache
     std::unique_ptr<Source> illumination, view;
     std::unique_ptr<TemporalSampler> sampler;
     std::unique_ptr<NextEventEstimation> tracer;
@@ -61,7 +61,9 @@ static int buildSourcePair(const sstmc_source& s1, const sstmc_source& s2, const
 }
 
 // Host-only validation, before anything touches the device (MatlabInterface.cu:712-1563).
-static int validate(const sstmc_config* c) {
+// checkGrid = false skips the per-cell checks of the material grid (a medium taken from the
+// serving loop's cache, whose grid was checked when it was built).
+static int validate(const sstmc_config* c, bool checkGrid = true) {
     if (!c) return SSTMC_INVALID_CONFIG;
     int dims = (int)c->dims;
     if (dims != 2 && dims != 3) return SSTMC_INVALID_CONFIG;
@@ -96,9 +98,10 @@ static int validate(const sstmc_config* c) {
             if (!(ax[a][i] > ax[a][i - 1])) return SSTMC_INVALID_CONFIG;
         ncell *= sz[a] - 1;
     }
-    if (!md.materialGrid || !md.sigs || md.materialNum != nmat) return SSTMC_INVALID_CONFIG;
-    for (size_t i = 0; i < ncell; ++i)   // material 0 only outside the grid (MatlabInterface.cu:975-992)
-        if (md.materialGrid[i] == 0 || md.materialGrid[i] >= nmat) return SSTMC_INVALID_CONFIG;
+    if ((checkGrid && !md.materialGrid) || !md.sigs || md.materialNum != nmat) return SSTMC_INVALID_CONFIG;
+    if (checkGrid)
+        for (size_t i = 0; i < ncell; ++i)   // material 0 only outside the grid (MatlabInterface.cu:975-992)
+            if (md.materialGrid[i] == 0 || md.materialGrid[i] >= nmat) return SSTMC_INVALID_CONFIG;
     double maj = 0;
     for (uint32_t m = 0; m < nmat; ++m) {
         if (!(md.sigs[m] >= 0) || (md.siga && !(md.siga[m] >= 0))) return SSTMC_INVALID_CONFIG;
@@ -157,10 +160,26 @@ static int validate(const sstmc_config* c) {
     return SSTMC_NO_ERROR;
 }
 
-static int create(const sstmc_config* c, sstmc_handle** out) {
+// HeterogeneousMedium of a validated config (MatlabInterface.cu:951-1056).
+static int buildMedium(const sstmc_config* c, std::shared_ptr<Medium>* out) {
+    const int dims = (int)c->dims;
+    const sstmc_medium& md = c->medium;
+    const uint32_t nmat = c->scattering.materialNum == 1 ? 2 : c->scattering.materialNum;
+    int err = SSTMC_NO_ERROR;
+    std::shared_ptr<Medium> m(new HeterogeneousMedium(&err, dims, md.materialGrid, md.xAxis, md.xAxisSize, md.yAxis,
+                                                      md.yAxisSize, md.zAxis, md.zAxisSize));
+    if (err) return err;
+    if ((err = m->setMaterial(md.sigs, md.siga, nmat))) return err;
+    *out = std::move(m);
+    return SSTMC_NO_ERROR;
+}
+
+// sstmc_create; with `shared` the handle uses that medium (built from an identical medium
+// config) instead of building one, and the config's material grid is not read.
+static int create(const sstmc_config* c, sstmc_handle** out, std::shared_ptr<Medium> shared = nullptr) {
     if (!c || !out) return SSTMC_INVALID_CONFIG;
     *out = nullptr;
-    int vrc = validate(c);
+    int vrc = validate(c, !shared);
     if (vrc) return vrc;
     int dims = (int)c->dims;
     std::unique_ptr<sstmc_handle> h(new sstmc_handle());
@@ -196,28 +215,10 @@ static int create(const sstmc_config* c, sstmc_handle** out) {
     }
     if (int rc = h->scattering->upload()) return rc;
 
-    // medium (MatlabInterface.cu:951-1056)
-    const sstmc_medium& md = c->medium;
-    if (md.type != SSTMC_MEDIUM_HETEROGENEOUS) return SSTMC_NOT_SUPPORTED;
-    const double* ax[3] = {md.xAxis, md.yAxis, md.zAxis};
-    uint32_t sz[3] = {md.xAxisSize, md.yAxisSize, md.zAxisSize};
-    size_t ncell = 1;
-    for (int a = 0; a < dims; ++a) {
-        if (!ax[a] || sz[a] < 2) return SSTMC_INVALID_CONFIG;
-        for (uint32_t i = 1; i < sz[a]; ++i)
-            if (!(ax[a][i] > ax[a][i - 1])) return SSTMC_INVALID_CONFIG;
-        ncell *= sz[a] - 1;
-    }
-    if (!md.materialGrid || !md.sigs || md.materialNum != nmat) return SSTMC_INVALID_CONFIG;
-    for (size_t i = 0; i < ncell; ++i)
-        if (md.materialGrid[i] == 0 || md.materialGrid[i] >= nmat) return SSTMC_INVALID_CONFIG;
-    for (uint32_t m = 0; m < nmat; ++m)
-        if (md.sigs[m] < 0 || (md.siga && md.siga[m] < 0)) return SSTMC_INVALID_CONFIG;
+    // medium (MatlabInterface.cu:951-1056), validated above
     int err = SSTMC_NO_ERROR;
-    h->medium.reset(new HeterogeneousMedium(&err, dims, md.materialGrid, md.xAxis, md.xAxisSize, md.yAxis,
-                                            md.yAxisSize, md.zAxis, md.zAxisSize));
-    if (err) return err;
-    if ((err = h->medium->setMaterial(md.sigs, md.siga, nmat))) return err;
+    if (shared) h->medium = std::move(shared);
+    else if ((err = buildMedium(c, &h->medium))) return err;
 
     // sources (MatlabInterface.cu:1062-1215)
     const sstmc_sampler& sp = c->sampler;
@@ -269,6 +270,23 @@ static int create(const sstmc_config* c, sstmc_handle** out) {
 namespace sstmc {
 NextEventEstimation* handleTracer(sstmc_handle* h) { return h ? h->tracer.get() : nullptr; }
 int configValidate(const sstmc_config* c) { return validate(c); }
+int mediumBuild(const sstmc_config* c, std::shared_ptr<Medium>* out) {
+    int rc = validate(c);
+    if (rc) return rc;
+    if (hipSetDevice((int)c->simulation.deviceNumber) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    try {
+        return buildMedium(c, out);
+    } catch (...) {
+        return SSTMC_ALLOCATION_ERROR;
+    }
+}
+int createShared(const sstmc_config* c, std::shared_ptr<Medium> medium, sstmc_handle** out) {
+    try {
+        return create(c, out, std::move(medium));
+    } catch (...) {
+        return SSTMC_ALLOCATION_ERROR;
+    }
+}
 }  // namespace sstmc
 
 extern "C" {

@@ -20,14 +20,16 @@ namespace sstmc {
 struct ConfigCopy {
     sstmc_config c{};
 
-    explicit ConfigCopy(const sstmc_config& in) : c(in) {
+    // withGrid = false leaves the material grid out (c.medium.materialGrid = NULL): a request whose
+    // medium the server already holds does not need it
+    explicit ConfigCopy(const sstmc_config& in, bool withGrid = true) : c(in) {
         const int dims = (int)in.dims;
         const sstmc_medium& m = in.medium;
         c.medium.xAxis = keep(ax_[0], m.xAxis, m.xAxisSize);
         c.medium.yAxis = keep(ax_[1], m.yAxis, m.yAxisSize);
         c.medium.zAxis = dims == 3 ? keep(ax_[2], m.zAxis, m.zAxisSize) : nullptr;
         const size_t ncell = (size_t)(m.xAxisSize - 1) * (m.yAxisSize - 1) * (dims == 3 ? m.zAxisSize - 1 : 1);
-        c.medium.materialGrid = keep(grid_, m.materialGrid, ncell);
+        c.medium.materialGrid = withGrid ? keep(grid_, m.materialGrid, ncell) : nullptr;
         c.medium.sigs = keep(sigs_, m.sigs, m.materialNum);
         c.medium.siga = keep(siga_, m.siga, m.materialNum);
 

@@ -10,13 +10,17 @@
 //     pipeline), round-robin over requests, so work that arrives while others run is interleaved
 //     with it on the device;
 //   * on completion copies the field into a small device buffer owned by the request and frees
-//     the tracer, so finished results stay on the device until fetched.
+//     the tracer, so finished results stay on the device until fetched;
+//   * keeps the media it has built (grid, cross sections, walk grids) in a cache keyed by their
+//     content, preloadable with sstmc_server_preload, so requests on a medium already resident
+//     share it instead of copying and building it again.
 // A request's result depends only on its config: batches, random numbers and reductions are
 // per request and in a fixed order, so it is bit-identical alone and in any mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -32,20 +36,79 @@
 using namespace sstmc;
 typedef std::chrono::steady_clock Clock;
 
-extern "C" int sstmc_create(const sstmc_config* cfg, sstmc_handle** out);
 extern "C" void sstmc_destroy(sstmc_handle* h);
 namespace sstmc {
 NextEventEstimation* handleTracer(sstmc_handle* h);
 int configValidate(const sstmc_config* c);
+int mediumBuild(const sstmc_config* c, std::shared_ptr<Medium>* out);
+int createShared(const sstmc_config* c, std::shared_ptr<Medium> medium, sstmc_handle** out);
 }
 
 namespace {
+
+// What identifies a medium: dimensions, edges, material grid and cross sections. Its hash keys
+// the cache; a hit is confirmed by comparing the whole identity.
+struct MediumId {
+    uint32_t dims = 0, nmat = 0;
+    std::vector<double> axes[3], sigs, siga;
+    std::vector<uint8_t> grid;
+    uint64_t hash = 0;
+
+    explicit MediumId(const sstmc_config& c) {
+        const sstmc_medium& m = c.medium;
+        dims = c.dims;
+        nmat = m.materialNum;
+        const double* ax[3] = {m.xAxis, m.yAxis, dims == 3 ? m.zAxis : nullptr};
+        const uint32_t sz[3] = {m.xAxisSize, m.yAxisSize, dims == 3 ? m.zAxisSize : 0u};
+        size_t ncell = 1;
+        uint64_t h = 0x9E3779B97F4A7C15ull ^ dims;
+        auto mix = [&h](uint64_t v) {
+            h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+            h *= 0xff51afd7ed558ccdull;
+            h ^= h >> 33;
+        };
+        for (int a = 0; a < 3; ++a) {
+            if (ax[a]) axes[a].assign(ax[a], ax[a] + sz[a]);
+            for (double e : axes[a]) { uint64_t b; std::memcpy(&b, &e, 8); mix(b); }
+            mix(sz[a]);
+            if (sz[a]) ncell *= sz[a] - 1;
+        }
+        grid.resize(ncell);
+        for (size_t i = 0; i < ncell; ++i) grid[i] = (uint8_t)m.materialGrid[i];
+        for (size_t i = 0; i + 8 <= ncell; i += 8) { uint64_t b; std::memcpy(&b, grid.data() + i, 8); mix(b); }
+        for (size_t i = ncell & ~(size_t)7; i < ncell; ++i) mix(grid[i]);
+        sigs.assign(m.sigs, m.sigs + nmat);
+        siga.assign(nmat, 0.0);
+        if (m.siga) siga.assign(m.siga, m.siga + nmat);
+        for (uint32_t i = 0; i < nmat; ++i) {
+            uint64_t b;
+            std::memcpy(&b, &sigs[i], 8); mix(b);
+            std::memcpy(&b, &siga[i], 8); mix(b);
+        }
+        hash = h;
+    }
+    bool operator==(const MediumId& o) const {
+        if (hash != o.hash || dims != o.dims || nmat != o.nmat || grid != o.grid || sigs != o.sigs || siga != o.siga)
+            return false;
+        for (int a = 0; a < 3; ++a)
+            if (axes[a] != o.axes[a]) return false;
+        return true;
+    }
+};
+
+struct CachedMedium {
+    std::unique_ptr<MediumId> id;
+    std::shared_ptr<Medium> medium;
+    uint64_t lastUse = 0;
+};
 
 enum State { kQueued, kActive, kDone, kFailed, kFetched };
 
 struct Request {
     uint64_t id;
     std::unique_ptr<ConfigCopy> cfg;
+    std::shared_ptr<Medium> medium;    // from the cache (then cfg holds no grid)
+    std::unique_ptr<MediumId> mediumId;   // when the medium is not cached yet
     State state = kQueued;
     int error = SSTMC_NO_ERROR;
     uint64_t footprint = 0;            // bytes counted in the server's inUse while active
@@ -80,7 +143,12 @@ struct sstmc_server {
     uint64_t donePaths = 0, doneContrib = 0;
     Clock::time_point firstAdmit, lastDone;
     bool anyAdmitted = false;
+    std::vector<CachedMedium> media;   // guarded by mu
+    uint64_t useClock = 0;
+    static constexpr size_t kMaxMedia = 16;
 
+    std::shared_ptr<Medium> findMedium(const MediumId& id);
+    void keepMedium(std::unique_ptr<MediumId> id, std::shared_ptr<Medium> m);
     void run();
     void admit();
     int pump(Request& r);
@@ -135,6 +203,7 @@ void sstmc_server::release(Request& r) {
 void sstmc_server::fail(Request& r, int rc) {
     const bool wasActive = r.state == kActive;
     release(r);
+    r.medium.reset();
     if (r.result) hipFree(r.result);
     r.result = nullptr;
     if (wasActive) {
@@ -147,14 +216,48 @@ void sstmc_server::fail(Request& r, int rc) {
     cv.notify_all();
 }
 
+// A cached medium equal to `id`, or null. Caller holds mu.
+std::shared_ptr<Medium> sstmc_server::findMedium(const MediumId& id) {
+    for (CachedMedium& cm : media)
+        if (*cm.id == id) {
+            cm.lastUse = ++useClock;
+            return cm.medium;
+        }
+    return nullptr;
+}
+
+// Adds a medium to the cache; when full, drops the least recently used one that no request holds.
+// Caller holds mu.
+void sstmc_server::keepMedium(std::unique_ptr<MediumId> id, std::shared_ptr<Medium> m) {
+    if (findMedium(*id)) return;
+    if (media.size() >= kMaxMedia) {
+        size_t victim = media.size();
+        for (size_t i = 0; i < media.size(); ++i)
+            if (media[i].medium.use_count() == 1 && (victim == media.size() || media[i].lastUse < media[victim].lastUse))
+                victim = i;
+        if (victim == media.size()) return;   // all in use: not cached
+        media.erase(media.begin() + victim);
+    }
+    media.push_back(CachedMedium{std::move(id), std::move(m), ++useClock});
+}
+
 void sstmc_server::admit() {
     // caller holds mu
     while (!queue.empty() && active.size() < maxInflight) {
         Request& r = *reqs[queue.front()];
         if (inUse + r.footprint > budget && !active.empty()) break;   // wait for memory to free up
         queue.pop_front();
+        int rc = SSTMC_NO_ERROR;
+        if (!r.medium) {
+            // a medium submitted after this request, or preloaded since, may be cached by now
+            r.medium = findMedium(*r.mediumId);
+            if (!r.medium) {
+                rc = mediumBuild(&r.cfg->c, &r.medium);
+                if (!rc) keepMedium(std::move(r.mediumId), r.medium);
+            }
+        }
         sstmc_handle* h = nullptr;
-        int rc = sstmc_create(&r.cfg->c, &h);
+        if (!rc) rc = createShared(&r.cfg->c, r.medium, &h);
         if (rc) {
             fail(r, rc);
             continue;
@@ -232,6 +335,7 @@ void sstmc_server::renderingDone(Request& r) {
         return;
     }
     release(r);
+    r.medium.reset();
     inUse -= r.footprint;
     active.erase(std::find(active.begin(), active.end(), r.id));
     r.done = Clock::now();
@@ -292,7 +396,13 @@ int sstmc_submit(sstmc_server* s, const sstmc_config* cfg, uint64_t* request_id)
     int rc = configValidate(cfg);
     if (rc) return rc;
     std::unique_ptr<Request> r(new Request());
-    r->cfg.reset(new ConfigCopy(*cfg));
+    std::unique_ptr<MediumId> mid(new MediumId(*cfg));
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->medium = s->findMedium(*mid);
+    }
+    r->cfg.reset(new ConfigCopy(*cfg, !r->medium));
+    if (!r->medium) r->mediumId = std::move(mid);
     r->footprint = estimate(r->cfg->c);
     if (r->footprint > s->budget) return SSTMC_BUDGET_EXCEEDED;
     r->renderings = cfg->simulation.renderingsNum ? cfg->simulation.renderingsNum : 1;
@@ -304,6 +414,26 @@ int sstmc_submit(sstmc_server* s, const sstmc_config* cfg, uint64_t* request_id)
     s->queue.push_back(r->id);
     s->reqs[r->id] = std::move(r);
     s->cv.notify_all();
+    return SSTMC_NO_ERROR;
+}
+
+int sstmc_server_preload(sstmc_server* s, const sstmc_config* cfg) {
+    if (!s || !cfg) return SSTMC_INVALID_CONFIG;
+    std::unique_ptr<MediumId> mid(new MediumId(*cfg));
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->findMedium(*mid)) return SSTMC_NO_ERROR;
+    }
+    std::shared_ptr<Medium> m;
+    int rc = mediumBuild(cfg, &m);
+    if (rc) return rc;
+    // the walk grids of every axis, so no request waits for one
+    for (int a = 0; a < (int)cfg->dims; ++a) {
+        MediumView mv = m->view();
+        if ((rc = m->walkGrid(a, &mv))) return rc;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->keepMedium(std::move(mid), std::move(m));
     return SSTMC_NO_ERROR;
 }
 

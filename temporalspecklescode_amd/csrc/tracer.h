@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/sstmc.h"
@@ -47,11 +49,28 @@ public:
     double sigs(uint32_t m) const { return sigs_[m]; }
     // material id per cell (x fastest) on the host
     virtual const std::vector<uint8_t>& hostGrid() const = 0;
+    // The packed NEE walk grid with axis `fast` fastest, built on first use and kept (tracers
+    // that share the medium share it): fills the walk fields of *mv, walkFast = 0 when the fast
+    // walk does not apply. Thread-safe.
+    int walkGrid(int fast, MediumView* mv) const;
+    uint64_t walkBytes() const;
 
 protected:
     MediumView v_{};
     uint32_t nmat_ = 0;
     std::vector<double> sigs_, siga_;
+
+private:
+    struct WalkCache {
+        bool built = false;
+        int fast = 0;
+        DevBuf<uint32_t> buf;
+        uint32_t words = 0, origin = 0, bits = 0, stride[3] = {0, 0, 0};
+        float sig[kWalkPalette] = {};
+    };
+    int buildWalk(int fast, WalkCache* out) const;   // caller holds walkMu_
+    mutable std::mutex walkMu_;
+    mutable WalkCache walk_[3];
 };
 
 // HeterogeneousMedium (HeterogeneousMedium.cuh:52-233): rectilinear grid of material ids.
@@ -293,8 +312,7 @@ private:
     hipEvent_t cleared_ = nullptr;   // begin()'s clears on the NEE stream
 
     const Medium& medium_;
-    MediumView mview_{};            // the medium's view plus this tracer's walk grid
-    DevBuf<uint32_t> walkGrid_;
+    MediumView mview_{};            // the medium's view plus the walk grid for this tracer's views
     const Scattering& sc_;
     const Source& il_;
     const Source& vw_;

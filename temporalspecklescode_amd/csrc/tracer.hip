@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace sstmc {
 
@@ -1011,10 +1012,87 @@ uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint
 // vertex, then issue few reads and touch few cache lines per step. Padding of one slowest-axis
 // layer (rounded up to a word) on both sides keeps the cell one step past any face inside the
 // allocation. Grids whose bit offsets do not fit 32 bits, or with unequal spacing, use the
-// general walk.
+// general walk. Built once per fast axis and kept by the medium, so tracers (requests) that share
+// a medium share its walk grids.
+int Medium::walkGrid(int fast, MediumView* mv) const {
+    std::lock_guard<std::mutex> lk(walkMu_);
+    WalkCache& wc = walk_[fast];
+    if (!wc.built) {
+        const int rc = buildWalk(fast, &wc);
+        if (rc) return rc;
+    }
+    for (int i = 0; i < kWalkPalette; ++i) mv->walkSig[i] = wc.sig[i];
+    mv->walkFast = wc.fast;
+    mv->walkGrid = wc.buf.p;
+    mv->walkWords = wc.words;
+    mv->walkOrigin = wc.origin;
+    mv->walkBits = wc.bits;
+    for (int a = 0; a < 3; ++a) mv->walkStride[a] = wc.stride[a];
+    return SSTMC_NO_ERROR;
+}
+
+int Medium::buildWalk(int fast, WalkCache* out) const {
+    WalkCache& wc = *out;
+    const int dims = v_.dims;
+    const uint32_t* n = v_.n;
+    int order[3] = {fast, -1, -1}, k = 1;
+    for (int a = 0; a < 3; ++a)
+        if (a != fast) order[k++] = a;
+    uint32_t S[3] = {0, 0, 0}, mul = 1;
+    for (int j = 0; j < 3; ++j) {
+        S[order[j]] = order[j] < dims ? mul : 0u;
+        if (order[j] < dims) mul *= n[order[j]];
+    }
+    // palette of the materials present, and the narrowest field that indexes it
+    const size_t ncell = (size_t)n[0] * n[1] * n[2];
+    const std::vector<uint8_t>& g = hostGrid();
+    bool present[256] = {};
+    for (size_t i = 0; i < ncell; ++i) present[g[i]] = true;
+    uint32_t index[256] = {}, npal = 0;
+    for (int i = 0; i < kWalkPalette; ++i) wc.sig[i] = 0.f;
+    for (int id = 0; id < 256; ++id)
+        if (present[id]) {
+            if (id >= kMaxMaterials || npal >= (uint32_t)kWalkPalette) return SSTMC_INVALID_CONFIG;
+            index[id] = npal;
+            wc.sig[npal++] = v_.sigtf[id];
+        }
+    // fields hold 4 x palette index (the sigt byte offset): 0 fits one bit, 0..12 four, 0..60 eight
+    const uint32_t bits = npal <= 1 ? 1u : (npal <= 4 ? 4u : 8u);
+    // bit offsets: one layer of the slowest axis of padding (whole words) before and after
+    const int slow = dims == 3 ? order[2] : order[1];
+    const uint64_t pad = ((uint64_t)S[slow] * bits + 31) & ~31ull;
+    const uint64_t total = (pad + (uint64_t)ncell * bits + pad + 31) & ~31ull;
+    wc.fast = 0;
+    if (v_.allUniform && total < 0xffffffffull) {
+        std::vector<uint32_t> wg(total / 32, 0u);
+        for (uint32_t i2 = 0; i2 < n[2]; ++i2)
+            for (uint32_t i1 = 0; i1 < n[1]; ++i1)
+                for (uint32_t i0 = 0; i0 < n[0]; ++i0) {
+                    const uint64_t bit =
+                        pad + ((uint64_t)i0 * S[0] + (uint64_t)i1 * S[1] + (uint64_t)i2 * S[2]) * bits;
+                    wg[bit >> 5] |= (4u * index[g[i0 + (size_t)n[0] * (i1 + (size_t)n[1] * i2)]]) << (bit & 31);
+                }
+        int e = wc.buf.upload(wg.data(), wg.size());
+        if (e) return e;
+        wc.words = (uint32_t)(total / 32);
+        wc.origin = (uint32_t)pad;
+        wc.bits = bits;
+        for (int a = 0; a < 3; ++a) wc.stride[a] = S[a] * bits;
+        wc.fast = 1;
+    }
+    wc.built = true;
+    return SSTMC_NO_ERROR;
+}
+
+uint64_t Medium::walkBytes() const {
+    std::lock_guard<std::mutex> lk(walkMu_);
+    uint64_t b = 0;
+    for (const WalkCache& wc : walk_) b += wc.buf.n * sizeof(uint32_t);
+    return b;
+}
+
 int NextEventEstimation::buildWalkGrid() {
     mview_ = medium_.view();
-    const uint32_t* n = mview_.n;
     // dominant axis of the view walks: far field, mean |direction component|; points, mean
     // |component| of the unit vector from the box centre to the sensor
     double w[3] = {0, 0, 0}, c[3];
@@ -1031,51 +1109,7 @@ int NextEventEstimation::buildWalkGrid() {
     int fast = 0;
     for (int a = 1; a < dims_; ++a)
         if (w[a] > w[fast]) fast = a;
-    int order[3] = {fast, -1, -1}, k = 1;
-    for (int a = 0; a < 3; ++a)
-        if (a != fast) order[k++] = a;
-    uint32_t S[3] = {0, 0, 0}, mul = 1;
-    for (int j = 0; j < 3; ++j) {
-        S[order[j]] = order[j] < dims_ ? mul : 0u;
-        if (order[j] < dims_) mul *= n[order[j]];
-    }
-    // palette of the materials present, and the narrowest field that indexes it
-    const size_t ncell = (size_t)n[0] * n[1] * n[2];
-    const std::vector<uint8_t>& g = medium_.hostGrid();
-    bool present[256] = {};
-    for (size_t i = 0; i < ncell; ++i) present[g[i]] = true;
-    uint32_t index[256] = {}, npal = 0;
-    for (int id = 0; id < 256; ++id)
-        if (present[id]) {
-            if (id >= kMaxMaterials || npal >= (uint32_t)kWalkPalette) return SSTMC_INVALID_CONFIG;
-            index[id] = npal;
-            mview_.walkSig[npal++] = mview_.sigtf[id];
-        }
-    // fields hold 4 x palette index (the sigt byte offset): 0 fits one bit, 0..12 four, 0..60 eight
-    const uint32_t bits = npal <= 1 ? 1u : (npal <= 4 ? 4u : 8u);
-    // bit offsets: one layer of the slowest axis of padding (whole words) before and after
-    const int slow = dims_ == 3 ? order[2] : order[1];
-    const uint64_t pad = ((uint64_t)S[slow] * bits + 31) & ~31ull;
-    const uint64_t total = (pad + (uint64_t)ncell * bits + pad + 31) & ~31ull;
-    mview_.walkFast = 0;
-    if (!mview_.allUniform || total >= 0xffffffffull) return SSTMC_NO_ERROR;   // the general walk
-    std::vector<uint32_t> wg(total / 32, 0u);
-    for (uint32_t i2 = 0; i2 < n[2]; ++i2)
-        for (uint32_t i1 = 0; i1 < n[1]; ++i1)
-            for (uint32_t i0 = 0; i0 < n[0]; ++i0) {
-                const uint64_t bit = pad + ((uint64_t)i0 * S[0] + (uint64_t)i1 * S[1] + (uint64_t)i2 * S[2]) * bits;
-                wg[bit >> 5] |= (4u * index[g[i0 + (size_t)n[0] * (i1 + (size_t)n[1] * i2)]]) << (bit & 31);
-            }
-    int e = walkGrid_.upload(wg.data(), wg.size());
-    if (e) return e;
-    mview_.walkGrid = walkGrid_.p;
-    mview_.walkWords = (uint32_t)(total / 32);
-    mview_.walkOrigin = (uint32_t)pad;
-    mview_.walkBits = bits;
-    for (int a = 0; a < 3; ++a) mview_.walkStride[a] = S[a] * bits;
-    mview_.walkFast = 1;
-    bytes_ += total / 8;
-    return SSTMC_NO_ERROR;
+    return medium_.walkGrid(fast, &mview_);
 }
 
 // Sizes the vertex pool from the exact vertex count of the first batch (its count pass and scan,

@@ -39,6 +39,10 @@ class Server:
         except Exception:
             pass
 
+    def preload(self, config: dict, dims: int):
+        """Builds the medium of `config` on the device and keeps it for later requests."""
+        _native.check(_native.lib().sstmc_server_preload(self._h, CConfig(config, dims).ptr()), "sstmc_server_preload")
+
     def submit(self, config: dict, dims: int) -> int:
         cc = CConfig(config, dims)
         rid = C.c_uint64()

@@ -81,3 +81,64 @@ def test_admission_by_footprint():
         with pytest.raises(_native.SstmcError) as e:
             srv.submit(cfg, dims)
         assert e.value.code == 9
+
+
+def test_preloaded_and_cached_media_give_the_same_results():
+    """Requests on a preloaded medium, and on a medium another request built, share the server's
+    device copy (their configs' grids are not read again) and still give the request's own result."""
+    base = G.hetero3_small(512)
+    base["simulation"]["batchSize"] = 256
+    base["simulation"]["fullIteration"] = 2
+    other = G.hetero3_small(512, point=True, seed=8)
+    reqs = []
+    for i in range(3):
+        c = dict(base)
+        c["simulation"] = dict(base["simulation"], seed=100 + i)
+        reqs.append((c, 3))
+    reqs.append((other, 3))
+    reqs.append((dict(other, simulation=dict(other["simulation"], seed=77)), 3))
+    alone = []
+    for cfg, dims in reqs:
+        with Tracer(cfg, dims) as tr:
+            alone.append(tr.trace())
+    with Server(max_inflight=3) as srv:
+        srv.preload(base, 3)
+        ids = [srv.submit(cfg, dims) for cfg, dims in reqs]
+        outs = []
+        for rid in ids:
+            srv.wait(rid)
+            outs.append(srv.fetch(rid))
+    for (ua, ia), (um, im) in zip(alone, outs):
+        assert ia == im
+        assert np.array_equal(ua, um)
+
+
+def test_mixed_stream_subset_alone_and_served():
+    """A subset of the benchmark's mixed request stream (2D and 3D, the three media, L and V in
+    [16, 512], 1-8 lags of linear motion, the measured precision form) served with preloaded
+    media: every result equal to the request traced alone."""
+    stream = W.serving_stream(6, max_paths_log2=16)
+    for _, _, cfg in stream:
+        cfg["precision"] = {"terms": "fp32", "contraction": "bf16x3", "tauCut": 24.0}
+    alone = []
+    for i, (_, dims, cfg) in enumerate(stream):
+        print("request", i, "dims", dims, "L x V", len(cfg["illumination"]["direction"]["x"]), flush=True)
+        with Tracer(cfg, dims) as tr:
+            alone.append(tr.trace())
+    with Server(max_inflight=4) as srv:
+        seen = set()
+        for _, dims, cfg in stream:
+            key = (dims, len(cfg["medium"]["xAxis"]))
+            if key not in seen:
+                seen.add(key)
+                srv.preload(cfg, dims)
+        ids = [srv.submit(cfg, dims) for _, dims, cfg in stream]
+        outs = []
+        for rid in ids:
+            srv.wait(rid)
+            outs.append(srv.fetch(rid))
+        st = srv.stats()
+    for (ua, ia), (um, im) in zip(alone, outs):
+        assert ia == im
+        assert np.array_equal(ua, um)
+    print("p50 %.1f ms p99 %.1f ms" % (st["p50_ms"], st["p99_ms"]), flush=True)
