@@ -13,7 +13,8 @@ attenuated below exp(-12) of the batch's least attenuated view term are dropped)
 split-product MFMA contraction. The JSON line also carries:
   roofline              the dominant kernel (the view-term kernel) against HBM, with its VALU
                         issue rate beside it, from HIP events on its stream;
-  roofline_contraction  the contraction's algorithmic MFMA rate;
+  roofline_contraction  the contraction's algorithmic MFMA rate in the pipeline, MFMA busy from the
+                        counters, and its rate alone;
   accuracy              the field of the measured form against FP64 terms and FP64 contraction
                         without the cut, on one whole batch of the same paths (GPU), and against
                         the FP64 CPU restatement on a small batch (cpu_baseline leg);
@@ -277,12 +278,22 @@ def main():
     flops = 8.0 * args.batch * nl * nv * nt   # per contraction launch set: 8 flops per path-contribution
     con_ms = tm["contract_ms"] / nb
     con_tflops = flops / (con_ms * 1e-3) / 1e12
-    roof_con = {"kernel": "k_contract_bf16x3" if args.contraction == "bf16x3" else "k_contract_fma",
+    # the multiple-scattering contraction as it runs in the pipeline (split, GEMM and reduce, timed
+    # by events on the contraction stream while the next batch's NEE runs on another stream)
+    roof_con = {"kernel": "k_split_bf16 + k_contract_bf16x3_planes + k_reduce" if args.contraction == "bf16x3"
+                else "k_contract_fma",
                 "bound": "mfma", "achieved": round(con_tflops, 3), "peak": MFMA_BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(con_tflops / MFMA_BF16_PEAK_TFLOPS, 6), "traffic": None,
-                "avg_launch_ms": round(con_ms, 4)}
+                "avg_launch_ms": round(con_ms, 4), "timed": "in the pipeline, concurrent with the NEE stage"}
     if same_form and "contract_mfma_busy_pct" in prof:
         roof_con["mfma_busy_pct_counters"] = prof["contract_mfma_busy_pct"]
+        # HBM bytes of one contraction: the GEMM's plus both operands' splits
+        roof_con["traffic"] = (prof.get("contract_fetch_bytes", 0) + prof.get("contract_write_bytes", 0) +
+                               2 * (prof.get("split_fetch_bytes", 0) + prof.get("split_write_bytes", 0)))
+    if same_form and "contract_alone_total_ms" in prof:
+        roof_con["alone"] = {"ms": prof["contract_alone_total_ms"], "tflops": prof["contract_alone_tflops"],
+                             "gemm_tflops": prof["contract_alone_gemm_tflops"],
+                             "source": "profiles/counters_tm1024.json (kernel trace, no concurrent work)"}
 
     out = {
         "metric": METRIC,

@@ -35,6 +35,21 @@ __device__ inline uint32_t bf16bits(float x) {
 __device__ inline float bf16val(uint32_t h) { return __uint_as_float(h << 16); }
 
 // ---------------------------------------------------------------- bf16x3 MFMA kernel
+// The dispatcher hands workgroup n (x fastest, then y, then z) to XCD n % 8, and each XCD has its
+// own L2. With the natural order XCD k would run l-tile k of every (v-tile, split), so every XCD
+// would read all of B from memory. Renumbering gives XCD k a contiguous run of the logical order
+// (whole splits of all tiles): the workgroups that run together on an XCD share one split's A
+// and B chunks in its L2, and each operand is read from memory about once. Results do not depend
+// on where a tile runs.
+constexpr uint32_t kXcds = 8;
+struct TileIdx { uint32_t x, y, z; };
+__device__ inline TileIdx xcdTile() {
+    const uint32_t gx = gridDim.x, gy = gridDim.y, n = gx * gy * gridDim.z;
+    uint32_t id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if (n % kXcds == 0) id = (id % kXcds) * (n / kXcds) + id / kXcds;
+    return TileIdx{id % gx, (id / gx) % gy, id / (gx * gy)};
+}
+
 constexpr int kTileL = 128, kTileV = 128, kChunk = 32;         // paths per LDS chunk
 constexpr int kRowBytes = kChunk * 2 * 2 + 16;                  // k' = 2*paths bf16 + 16 B pad
 constexpr int kRowWords = kRowBytes / 4;
@@ -45,10 +60,9 @@ __global__ void __launch_bounds__(256) k_contract_bf16x3(const float2* __restric
     __shared__ uint32_t sAh[kTileL * kRowWords], sAl[kTileL * kRowWords];
     __shared__ uint32_t sBh[kTileV * kRowWords], sBl[kTileV * kRowWords];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t l0 = blockIdx.x * kTileL, v0 = blockIdx.y * kTileV;
-    const uint32_t split = blockIdx.z % gridDim.z;
-    const uint32_t t = 0;  // one frame per launch (the host offsets A, B and part)
-    (void)t;
+    const TileIdx tile = xcdTile();
+    const uint32_t l0 = tile.x * kTileL, v0 = tile.y * kTileV;
+    const uint32_t split = tile.z;   // one frame per launch (the host offsets A, B and part)
     const uint32_t pBegin = split * pathsPerSplit;
     const uint32_t pEnd = min(P, pBegin + pathsPerSplit);
     const uint32_t wl = (wave & 1) * 64, wv = (wave >> 1) * 64;   // wave sub-tile origin
@@ -138,7 +152,172 @@ __global__ void __launch_bounds__(256) k_contract_bf16x3(const float2* __restric
     }
     // C layout of a 32x32 f32 MFMA tile: lane holds column (lane % 32), rows
     // 8*(r/4) + 4*(lane/32) + r%4 for accumulator r.
-    float2* out = part + (size_t)blockIdx.z * L * V;
+    float2* out = part + (size_t)split * L * V;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const uint32_t v = v0 + wv + 32 * j + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t l = l0 + wl + 32 * i + 8 * (r / 4) + 4 * (lane >> 5) + (r & 3);
+                if (l < L && v < V) out[(size_t)v * L + l] = make_float2(accRe[i][j][r], accIm[i][j][r]);
+            }
+        }
+}
+
+// ---------------------------------------------------------------- bf16x3 on pre-split operands
+// The same product from operands split once per frame into bf16 hi/lo planes: hi = bf16(re) |
+// bf16(im) << 16 per path, lo likewise for the residuals x - hi. A plane is stored chunk-major:
+// chunk k (paths 32k .. 32k+31) holds `rows` rows (an l or a v each) of 32 paths, 128 bytes a row,
+// so the 128 rows a GEMM tile reads for one chunk are one contiguous 16 KB run. Rows past L (V)
+// and paths past P are zeros. The GEMM then loads 16-byte runs of 4 paths (8 lanes: one row's
+// 128-byte chunk) with no conversion. Its 72 KB of LDS (one chunk of the four planes) and at most
+// 256 registers a lane let two workgroups share a CU, so one's loads and barriers overlap the
+// other's MFMAs (measured on the 65536 x 1024 x 1024 batch: 1.36 ms against 1.68 ms for one
+// workgroup a CU with two LDS buffers and register prefetch). The MFMA sequence (chunks in order,
+// 4 k-steps each, lo*hi, hi*lo, hi*hi per tile) is that of k_contract_bf16x3, so both give the
+// same bits.
+__device__ inline size_t planeIndex(uint32_t rows, uint32_t c, uint32_t p) {
+    return ((size_t)(p / kChunk) * rows + c) * kChunk + (p % kChunk);
+}
+
+// One workgroup splits 64 paths x 64 rows: reads along the rows (coalesced float2), transposes
+// through LDS, writes 2 chunks x 64 rows x 128 bytes per plane.
+__global__ void __launch_bounds__(256) k_split_bf16(const float2* __restrict__ X, uint32_t P, uint32_t C,
+                                                    uint32_t rows, uint32_t* __restrict__ hi,
+                                                    uint32_t* __restrict__ lo) {
+    __shared__ uint32_t th[64][65], tl[64][65];
+    const uint32_t p0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // all 16 loads of a thread are issued before the first conversion
+    float2 v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t p = p0 + ty + 4 * k, c = c0 + tx;
+        v[k] = (p < P && c < C) ? X[(size_t)p * C + c] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t hr = bf16bits(v[k].x), hm = bf16bits(v[k].y);
+        th[tx][ty + 4 * k] = hr | (hm << 16);
+        tl[tx][ty + 4 * k] = bf16bits(v[k].x - bf16val(hr)) | (bf16bits(v[k].y - bf16val(hm)) << 16);
+    }
+    __syncthreads();
+    // every (row, path) of the padded planes belongs to exactly one workgroup: no bounds checks
+#pragma unroll
+    for (uint32_t j = ty; j < 64; j += 4) {
+        const size_t o = planeIndex(rows, c0 + j, p0 + tx);
+        hi[o] = th[j][tx];
+        lo[o] = tl[j][tx];
+    }
+}
+
+__global__ void __launch_bounds__(256, 2) k_contract_bf16x3_planes(const uint32_t* __restrict__ Ah,
+                                                                const uint32_t* __restrict__ Al,
+                                                                const uint32_t* __restrict__ Bh,
+                                                                const uint32_t* __restrict__ Bl, uint32_t rowsA,
+                                                                uint32_t rowsB,
+                                                                uint32_t P, uint32_t L, uint32_t V,
+                                                                uint32_t pathsPerSplit, float2* __restrict__ part) {
+    __shared__ uint32_t sAh[kTileL * kRowWords], sAl[kTileL * kRowWords];
+    __shared__ uint32_t sBh[kTileV * kRowWords], sBl[kTileV * kRowWords];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const TileIdx tile = xcdTile();
+    const uint32_t l0 = tile.x * kTileL, v0 = tile.y * kTileV, split = tile.z;
+    const uint32_t pBegin = split * pathsPerSplit;
+    const uint32_t pEnd = min(P, pBegin + pathsPerSplit);
+    const uint32_t nChunks = (pEnd - pBegin + kChunk - 1) / kChunk;   // the planes hold zeros past P
+    const uint32_t wl = (wave & 1) * 64, wv = (wave >> 1) * 64;
+    f32x16 accRe[2][2], accIm[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) { accRe[i][j][r] = 0.f; accIm[i][j][r] = 0.f; }
+
+    // loader: segment `seg` (4 paths = 16 bytes) of rows rbase + 32 q, q = 0..3, of all 4 planes
+    const uint32_t seg = tid & 7, rbase = tid >> 3;
+    auto stage = [&](uint32_t pc) {
+        u32x4 rg[4][4];   // [plane][q]
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const size_t ra = planeIndex(rowsA, l0 + rbase + 32 * q, pc + 4 * seg);
+            const size_t rb = planeIndex(rowsB, v0 + rbase + 32 * q, pc + 4 * seg);
+            rg[0][q] = *(const u32x4*)(Ah + ra);
+            rg[1][q] = *(const u32x4*)(Al + ra);
+            rg[2][q] = *(const u32x4*)(Bh + rb);
+            rg[3][q] = *(const u32x4*)(Bl + rb);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t w = (rbase + 32 * q) * kRowWords + 4 * seg;
+            *(u32x4*)&sAh[w] = rg[0][q];
+            *(u32x4*)&sAl[w] = rg[1][q];
+            *(u32x4*)&sBh[w] = rg[2][q];
+            *(u32x4*)&sBl[w] = rg[3][q];
+        }
+    };
+    // one k-step's operands as read from LDS: A hi/lo of the wave's 2 row tiles, B hi/lo of its 2
+    // column tiles
+    struct Frag { u32x4 ah[2], al[2], bh[2], bl[2]; };
+    auto readFrag = [&](Frag& f, int ks) {
+        const uint32_t kw = ks * 8 + (lane >> 5) * 4;   // word offset of this lane's 8 k' values
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const uint32_t ra = (wl + 32 * i + (lane & 31)) * kRowWords + kw;
+            const uint32_t rb = (wv + 32 * i + (lane & 31)) * kRowWords + kw;
+            f.ah[i] = *(const u32x4*)&sAh[ra];
+            f.al[i] = *(const u32x4*)&sAl[ra];
+            f.bh[i] = *(const u32x4*)&sBh[rb];
+            f.bl[i] = *(const u32x4*)&sBl[rb];
+        }
+    };
+    auto mfmas = [&](const Frag& f) {
+        bf16x8 bReH[2], bReL[2], bImH[2], bImL[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            u32x4 hre, lre, him, lim;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                hre[q] = f.bh[j][q] ^ 0x80000000u;                       // (re, -im)
+                lre[q] = f.bl[j][q] ^ 0x80000000u;
+                him[q] = (f.bh[j][q] >> 16) | (f.bh[j][q] << 16);        // (im, re)
+                lim[q] = (f.bl[j][q] >> 16) | (f.bl[j][q] << 16);
+            }
+            bReH[j] = __builtin_bit_cast(bf16x8, hre);
+            bReL[j] = __builtin_bit_cast(bf16x8, lre);
+            bImH[j] = __builtin_bit_cast(bf16x8, him);
+            bImL[j] = __builtin_bit_cast(bf16x8, lim);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const bf16x8 aH = __builtin_bit_cast(bf16x8, f.ah[i]), aL = __builtin_bit_cast(bf16x8, f.al[i]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                accRe[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aL, bReH[j], accRe[i][j], 0, 0, 0);
+                accRe[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aH, bReL[j], accRe[i][j], 0, 0, 0);
+                accRe[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aH, bReH[j], accRe[i][j], 0, 0, 0);
+                accIm[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aL, bImH[j], accIm[i][j], 0, 0, 0);
+                accIm[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aH, bImL[j], accIm[i][j], 0, 0, 0);
+                accIm[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aH, bImH[j], accIm[i][j], 0, 0, 0);
+            }
+        }
+    };
+    for (uint32_t c = 0; c < nChunks; ++c) {
+        stage(pBegin + c * kChunk);
+        __syncthreads();
+        // 4 k-steps (k' = 16) per chunk of 32 paths
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            Frag f;
+            readFrag(f, ks);
+            mfmas(f);
+        }
+        __syncthreads();
+    }
+    float2* out = part + (size_t)split * L * V;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -244,12 +423,18 @@ static uint32_t chooseSplits(uint32_t tiles, uint32_t P, uint32_t minPaths) {
     return s < 1 ? 1 : s;
 }
 
-// Split-K plan: `splits` slices of `pps` paths each (whole LDS chunks), at most maxSplits.
+// Split-K plan: `splits` slices of `pps` paths each (whole LDS chunks), at most maxSplits; for
+// bf16x3 with `planes`, also the bf16 hi/lo operand planes (rows padded to the tile, paths to
+// ldp) that follow the partials in the workspace.
 struct SplitPlan {
     uint32_t splits, pps;
-    size_t bytes;   // partials: splits x L x V complex
+    size_t bytes;        // partials: splits x L x V complex
+    uint32_t ldp = 0;    // planes: padded path count (0: no planes, the GEMM converts as it loads)
+    uint32_t rowsA = 0, rowsB = 0;
+    size_t planeOffset = 0, planeBytes = 0;
+    size_t total() const { return planeBytes ? planeOffset + planeBytes : bytes; }
 };
-static SplitPlan planSplits(int kind, uint32_t P, uint32_t L, uint32_t V, uint32_t maxSplits) {
+static SplitPlan planSplits(int kind, uint32_t P, uint32_t L, uint32_t V, uint32_t maxSplits, bool planes = false) {
     const uint32_t tl = kind == kContractBf16x3 ? kTileL : 64, tv = kind == kContractBf16x3 ? kTileV : 64;
     const uint32_t tiles = ((L + tl - 1) / tl) * ((V + tv - 1) / tv);
     uint32_t splits = chooseSplits(tiles, P, kind == kContractBf16x3 ? 512 : 256);
@@ -261,7 +446,15 @@ static SplitPlan planSplits(int kind, uint32_t P, uint32_t L, uint32_t V, uint32
     splits = (P + pps - 1) / pps;
     if (splits < 1) splits = 1;
     const size_t elem = kind == kContractFp64 ? 16 : 8;
-    return SplitPlan{splits, pps, (size_t)splits * L * V * elem};
+    SplitPlan plan{splits, pps, (size_t)splits * L * V * elem};
+    if (planes && kind == kContractBf16x3) {
+        plan.ldp = (P + 63) / 64 * 64;
+        plan.rowsA = (L + kTileL - 1) / kTileL * kTileL;
+        plan.rowsB = (V + kTileV - 1) / kTileV * kTileV;
+        plan.planeOffset = (plan.bytes + 255) / 256 * 256;
+        plan.planeBytes = 2ull * ((size_t)plan.rowsA + plan.rowsB) * plan.ldp * sizeof(uint32_t);
+    }
+    return plan;
 }
 
 static hipError_t launchContractInto(int kind, const void* A, const void* B, uint32_t P, uint32_t L, uint32_t V,
@@ -272,17 +465,17 @@ static hipError_t launchContractInto(int kind, const void* A, const void* B, uin
 hipError_t launchContract(int kind, const void* A, const void* B, uint32_t P, uint32_t L, uint32_t V, uint32_t nt,
                           const float2* F, const double2* Fd, int fsq, double fscalar, double* U, Workspace* ws,
                           hipStream_t st) {
-    const SplitPlan plan = planSplits(kind, P, L, V, 0xffffffffu);
-    if (ws->bytes < plan.bytes) {
+    const SplitPlan plan = planSplits(kind, P, L, V, 0xffffffffu, true);
+    if (ws->bytes < plan.total()) {
         if (ws->ptr) {
             hipStreamSynchronize(st);
             hipFree(ws->ptr);
         }
         ws->ptr = nullptr;
         ws->bytes = 0;
-        hipError_t e = hipMalloc(&ws->ptr, plan.bytes);
+        hipError_t e = hipMalloc(&ws->ptr, plan.total());
         if (e != hipSuccess) return e;
-        ws->bytes = plan.bytes;
+        ws->bytes = plan.total();
     }
     return launchContractInto(kind, A, B, P, L, V, nt, F, Fd, fsq, fscalar, U, plan, ws->ptr, st);
 }
@@ -295,7 +488,22 @@ static hipError_t launchContractInto(int kind, const void* A, const void* B, uin
     const uint32_t LV = L * V;
     for (uint32_t t = 0; t < nt; ++t) {
         dim3 grid((L + tl - 1) / tl, (V + tv - 1) / tv, splits);
-        if (kind == kContractBf16x3) {
+        if (kind == kContractBf16x3 && plan.ldp) {
+            const float2* a = (const float2*)A + (size_t)t * P * L;
+            const float2* b = (const float2*)B + (size_t)t * P * V;
+            uint32_t* ah = (uint32_t*)((char*)partials + plan.planeOffset);
+            uint32_t* al = ah + (size_t)plan.rowsA * plan.ldp;
+            uint32_t* bh = al + (size_t)plan.rowsA * plan.ldp;
+            uint32_t* bl = bh + (size_t)plan.rowsB * plan.ldp;
+            hipLaunchKernelGGL(k_split_bf16, dim3(plan.ldp / 64, plan.rowsA / 64), dim3(256), 0, st, a, P, L,
+                               plan.rowsA, ah, al);
+            hipLaunchKernelGGL(k_split_bf16, dim3(plan.ldp / 64, plan.rowsB / 64), dim3(256), 0, st, b, P, V,
+                               plan.rowsB, bh, bl);
+            hipLaunchKernelGGL(k_contract_bf16x3_planes, grid, dim3(256), 0, st, ah, al, bh, bl, plan.rowsA,
+                               plan.rowsB, P, L, V, pps, (float2*)partials);
+            hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st, (const float2*)partials,
+                               splits, LV, F, Fd, fsq, fscalar, U + 2 * (size_t)t * LV);
+        } else if (kind == kContractBf16x3) {
             const float2* a = (const float2*)A + (size_t)t * P * L;
             const float2* b = (const float2*)B + (size_t)t * P * V;
             hipLaunchKernelGGL(k_contract_bf16x3, grid, dim3(256), 0, st, a, b, P, L, V, pps, (float2*)partials);

@@ -999,6 +999,7 @@ uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint
     uint64_t field = 16ull * T * L * V;
     uint64_t pool = 2 * (uint64_t)P * 32 * (20 + 32ull * T + (frames ? 12ull * T : 0));
     uint64_t ws = 2ull * 8 * T * L * V * elem;   // split-K partials, two workspaces
+    if (!fp64) ws += 2ull * 8 * ((uint64_t)P + 64) * (L + V + 256);   // and their bf16 hi/lo planes
     uint64_t grids = 2 * cells;                  // material grid + packed walk grid (<= 4 bits a cell)
     return terms + field + pool + ws + grids + 48ull * P + (1ull << 20);
 }
