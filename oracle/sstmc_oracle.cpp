@@ -1107,6 +1107,9 @@ struct Problem {
         // later vertex (Gl with f toward the previous vertex and the path phase).
         std::vector<cplx> Ar;  // [t][v]
         std::vector<cplx> Br;  // [t][l]
+        // TemporalCorrelationSampler with coherent backscattering: the two mixed pairings
+        // CBS02 + CBS03, which do not factor over (l, v), summed over the path's vertices
+        std::vector<cplx> X;   // [t][v][l]
         bool multiple = false;
     };
 
@@ -1228,6 +1231,17 @@ struct Problem {
     //    u2's Fs and the single-scattering temporal factor are not used (NextEventEstimation.cuh:
     //    308-324, 1020-1029, 1094-1118).
     V3 shifted(const V3& x, uint32_t m, double s) const { return add(x, scale(s, fr.U[m])); }
+    // Temporal factor of a connection of u1 source (s1, i1) and u2 source (s2, i2) at vertex p1
+    // whose path continues to p2: exp(-dtD |(e1 + e2) / 2 - n(p2 - p1)|^2), e the incoming unit
+    // direction of each source at p1 (a far-field view enters as minus its direction), all from
+    // unshifted positions (temporalCorrelationSourceContributionMultiple,
+    // SourceInterface.cuh:280-352; connection types C1-C4, TemporalSamplerCorrelation.cuh:975-1043).
+    double temporalT(const Source& s1, uint32_t i1, const Source& s2, uint32_t i2, const V3& p1, const V3& p2,
+                     double dtD) const {
+        const V3 e1 = incoming(s1, i1, p1), e2 = incoming(s2, i2, p1);
+        const V3 q = sub(scale(0.5, add(e1, e2)), geo.normalize(sub(p2, p1)));
+        return std::exp(-dtD * geo.dot(q, q));
+    }
     void termsAcorr(const std::vector<Vertex>& vs, Terms& out, uint64_t* steps) const {
         const uint32_t nl = L(), nv = V(), nt = T();
         const double kw = 2.0 * PI / lambda, k2 = kw * kw;
@@ -1281,6 +1295,35 @@ struct Problem {
                 out.A[(size_t)t * nl + l] = a1 * std::conj(a2) * std::exp(-dtD * geo.dot(q, q));
             }
         }
+        // coherent backscattering (NextEventEstimation.cuh:1237-1378): the first vertex's u1 and
+        // u2 terms of either source, kept apart for the mixed pairings; the view side carries the
+        // path weight like the illumination side (Gv0 *= 1/p, :991-1000)
+        std::vector<cplx> fa1, fa2, fc1, fc2;
+        if (cbs) {
+            fa1.assign((size_t)nt * nl, cplx(0)); fa2.assign((size_t)nt * nl, cplx(0));
+            fc1.assign((size_t)nt * nv, cplx(0)); fc2.assign((size_t)nt * nv, cplx(0));
+            out.Ar.assign((size_t)nt * nv, cplx(0));
+            out.Br.assign((size_t)nt * nl, cplx(0));
+            out.X.assign((size_t)nt * nv * nl, cplx(0));
+            for (uint32_t t = 0; t < nt; ++t) {
+                const double s = 0.5 * ticks[t];
+                const double dtD = k2 * std::fabs(fr.D[v1.m] * ticks[t]);
+                const V3 x1a = shifted(v1.x, v1.m, -s), x1b = shifted(v1.x, v1.m, s), x2a = shifted(v2.x, v2.m, -s);
+                for (uint32_t l = 0; l < nl; ++l) {
+                    fa1[t * nl + l] = g(il, l, x1a, steps) * f3(il, l, v1.m, x1a, x2a) / p2;
+                    fa2[t * nl + l] = g(il, nl + l, x1b, steps) * f3(il, nl + l, v1.m, x1a, x2a);
+                }
+                for (uint32_t v = 0; v < nv; ++v) {
+                    fc1[t * nv + v] = g(vw, v, x1a, steps) * f3(vw, v, v1.m, x1a, x2a) / p2;
+                    fc2[t * nv + v] = g(vw, nv + v, x1b, steps) * f3(vw, nv + v, v1.m, x1a, x2a);
+                    // CBS04's first-vertex half: Gv0 conj(u2Gv0) C4_v1v2
+                    out.Ar[t * nv + v] = fc1[t * nv + v] * std::conj(fc2[t * nv + v]) *
+                                         temporalT(vw, v, vw, nv + v, v1.x, v2.x, dtD);
+                }
+            }
+        }
+        std::vector<cplx> S2, S3;   // per-path sums over vertices of the mixed pairings' last halves
+        if (cbs) { S2.assign((size_t)nt * nv * nl, cplx(0)); S3.assign((size_t)nt * nv * nl, cplx(0)); }
         double mt = 0, len = 0;   // inner momentum transfer sum_j D_j |q_j|^2, path length from x1
         for (size_t k = 1; k < vs.size(); ++k) {
             const Vertex& xk = vs[k];
@@ -1303,6 +1346,44 @@ struct Problem {
                     const V3 q = sub(scale(-0.5, add(outDir(v, xk.x), outDir(nv + v, xk.x))), back);
                     out.B[(size_t)t * nv + v] += G1 * std::conj(G2) * std::exp(-dtD * geo.dot(q, q));
                 }
+                if (!cbs) continue;
+                // the reversed path's last connection: the illuminations at x_k with their f toward
+                // x_{k-1}, u1 carrying the inner momentum transfer (pathContriobutionKernel)
+                std::vector<cplx> gl1(nl), gl2(nl), gv1(nv), gv2(nv);
+                for (uint32_t l = 0; l < nl; ++l) {
+                    gl1[l] = g(il, l, xka, steps) * f3(il, l, xk.m, xka, xpa) * e * inner;
+                    gl2[l] = g(il, nl + l, xkb, steps) * f3(il, nl + l, xk.m, xka, xpa) * e;
+                    // CBS04's last half: Gl conj(u2Gl) C4_l1l2
+                    out.Br[(size_t)t * nl + l] +=
+                        gl1[l] * std::conj(gl2[l]) * temporalT(il, l, il, nl + l, xk.x, xp.x, dtD);
+                }
+                for (uint32_t v = 0; v < nv; ++v) {
+                    gv1[v] = g(vw, v, xka, steps, true) * fvw(v, xk.m, xpa, xka) * e * inner;
+                    gv2[v] = g(vw, nv + v, xkb, steps, true) * fvw(nv + v, xk.m, xpa, xka) * e;
+                }
+                for (uint32_t v = 0; v < nv; ++v)
+                    for (uint32_t l = 0; l < nl; ++l) {
+                        const size_t i = ((size_t)t * nv + v) * nl + l;
+                        // CBS02: (v1 -> ... -> l1) conj(l2 -> ... -> v2), last connection C2_l1v2
+                        S2[i] += gl1[l] * std::conj(gv2[v]) * temporalT(il, l, vw, nv + v, xk.x, xp.x, dtD);
+                        // CBS03: (l1 -> ... -> v1) conj(v2 -> ... -> l2), last connection C3_v1l2
+                        S3[i] += std::conj(gl2[l]) * gv1[v] * temporalT(vw, v, il, nl + l, xk.x, xp.x, dtD);
+                    }
+            }
+        }
+        if (cbs) {
+            for (uint32_t t = 0; t < nt; ++t) {
+                const double dtD = k2 * std::fabs(fr.D[v1.m] * ticks[t]);
+                for (uint32_t v = 0; v < nv; ++v)
+                    for (uint32_t l = 0; l < nl; ++l) {
+                        const size_t i = ((size_t)t * nv + v) * nl + l;
+                        // first connections C2_v1l2 and C3_l1v2 at x1 toward x2
+                        const cplx f2 = std::conj(fa2[t * nl + l]) * fc1[t * nv + v] *
+                                        temporalT(vw, v, il, nl + l, v1.x, v2.x, dtD);
+                        const cplx f3v = fa1[t * nl + l] * std::conj(fc2[t * nv + v]) *
+                                         temporalT(il, l, vw, nv + v, v1.x, v2.x, dtD);
+                        out.X[i] = f2 * S2[i] + f3v * S3[i];
+                    }
             }
         }
     }
@@ -1318,9 +1399,14 @@ struct Problem {
                     if (output == SSTMC_OUTPUT_CORRELATION) s = tm.S[(size_t)v * nl + l] * std::conj(s);
                     U[i] += s;
                     if (tm.multiple && !cbs) U[i] += tm.A[t * nl + l] * tm.B[t * nv + v];
+                    // correlation CBS: v = 1/2 * (CBS01 + CBS02 + CBS03 + CBS04) for multiple
+                    // scattering (NextEventEstimation.cuh:1380-1391)
+                    else if (tm.multiple && acorr)
+                        U[i] += (tm.A[t * nl + l] * tm.B[t * nv + v] + tm.Br[t * nl + l] * tm.Ar[t * nv + v] +
+                                 tm.X[i]) * 0.5;
                     // field CBS: v = 1/sqrt(2) * (direct + reversed) for multiple scattering
                     // (NextEventEstimation.cuh:1385-1391, 1448-1454)
-                    if (tm.multiple && cbs)
+                    else if (tm.multiple && cbs)
                         U[i] += (tm.A[t * nl + l] * tm.B[t * nv + v] + tm.Br[t * nl + l] * tm.Ar[t * nv + v]) *
                                 0.70710678118654752440;
                 }
@@ -1419,7 +1505,6 @@ int build(const sstmc_config* c, Problem& pb) {
     const sstmc_sampler& sp = c->sampler;
     if (sp.type != SSTMC_SAMPLER_TEMPORAL && sp.type != SSTMC_SAMPLER_TEMPORAL_CORRELATION) return SSTMC_NOT_SUPPORTED;
     pb.acorr = sp.type == SSTMC_SAMPLER_TEMPORAL_CORRELATION;
-    if (pb.acorr && pb.cbs) return SSTMC_NOT_SUPPORTED;
     if (pb.acorr && pb.output != SSTMC_OUTPUT_FIELD) return SSTMC_INVALID_CONFIG;
     const sstmc_source* srcs[2] = {&c->illumination, &c->view};
     const sstmc_source* srcs2[2] = {&c->illumination2, &c->view2};
@@ -1826,6 +1911,59 @@ int oracle_cbs_terms_from_vertices(const sstmc_config* cfg, uint32_t n, const ui
             term_br[2 * ((size_t)i * nt * nl + j)] = x.real();
             term_br[2 * ((size_t)i * nt * nl + j) + 1] = x.imag();
         }
+    }
+    return SSTMC_NO_ERROR;
+}
+
+// TemporalCorrelationSampler with coherent backscattering: the mixed pairings CBS02 + CBS03
+// summed over the given paths, x_sum [t][v][l] complex interleaved (unscaled: the field takes
+// half of it), with the same vertex inputs as oracle_terms_from_vertices.
+int oracle_cbs_cross_from_vertices(const sstmc_config* cfg, uint32_t n, const uint32_t* path_k,
+                                   const uint64_t* path_off, const double* vtx_pos, const uint32_t* vtx_mat,
+                                   const double* vtx_rho, const double* vtx_xt, const double* vtx_len, double* x_sum) {
+    Problem pb;
+    int err = build(cfg, pb);
+    if (err) return err;
+    if (!pb.cbs || !pb.acorr) return SSTMC_INVALID_CONFIG;
+    const uint32_t nl = pb.L(), nv = pb.V(), nt = pb.T();
+    const size_t nx = (size_t)nt * nv * nl;
+    int nthr = 1;
+#ifdef _OPENMP
+    nthr = omp_get_max_threads();
+#endif
+    std::vector<std::vector<cplx>> part(nthr, std::vector<cplx>(nx, cplx(0)));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (int64_t i = 0; i < (int64_t)n; ++i) {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        std::vector<Vertex> vs(path_k[i]);
+        for (uint32_t k = 0; k < path_k[i]; ++k) {
+            const uint64_t o = path_off[i] + k;
+            Vertex& v = vs[k];
+            v.x = V3(vtx_pos[3 * o], vtx_pos[3 * o + 1], vtx_pos[3 * o + 2]);
+            v.m = vtx_mat[o];
+            v.rho = vtx_rho[o];
+            v.xt.resize(nt);
+            v.len.resize(nt);
+            for (uint32_t t = 0; t < nt; ++t) {
+                v.xt[t] = V3(vtx_xt[(o * nt + t) * 3], vtx_xt[(o * nt + t) * 3 + 1], vtx_xt[(o * nt + t) * 3 + 2]);
+                v.len[t] = vtx_len[o * nt + t];
+            }
+        }
+        if (vs.size() < 2) continue;
+        Problem::Terms tm;
+        pb.terms(vs, tm, nullptr);
+        for (size_t j = 0; j < nx; ++j) part[tid][j] += tm.X[j];
+    }
+    for (size_t j = 0; j < nx; ++j) {
+        cplx acc(0);
+        for (int q = 0; q < nthr; ++q) acc += part[q][j];
+        x_sum[2 * j] = acc.real();
+        x_sum[2 * j + 1] = acc.imag();
     }
     return SSTMC_NO_ERROR;
 }

@@ -126,6 +126,21 @@ def cbs_terms_from_vertices(cfg: dict, dims: int, k, off, pos, mat, rho, xt, ln)
     return ar[..., 0] + 1j * ar[..., 1], br[..., 0] + 1j * br[..., 1]
 
 
+def cbs_cross_from_vertices(cfg: dict, dims: int, k, off, pos, mat, rho, xt, ln):
+    """TemporalCorrelationSampler with coherent backscattering: the mixed pairings CBS02 + CBS03
+    summed over the given paths, [Nl, Nv, Nt] (unscaled; the field takes half)."""
+    cc = CConfig(cfg, dims)
+    nl, nv, nt, _ = sizes(cfg, dims)
+    args = [np.ascontiguousarray(x, dt) for x, dt in ((k, np.uint32), (off, np.uint64), (pos, np.float64),
+                                                      (mat, np.uint32), (rho, np.float64), (xt, np.float64),
+                                                      (ln, np.float64))]
+    x = np.zeros((nt, nv, nl, 2))
+    rc = lib().oracle_cbs_cross_from_vertices(cc.ptr(), C.c_uint32(len(k)), *[_p(a) for a in args], _p(x))
+    if rc:
+        raise RuntimeError("oracle_cbs_cross_from_vertices failed with code %d" % rc)
+    return (x[..., 0] + 1j * x[..., 1]).transpose(2, 1, 0)
+
+
 def philox(ctr, key):
     c = (C.c_uint32 * 4)(*ctr)
     k = (C.c_uint32 * 2)(*key)
