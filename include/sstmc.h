@@ -152,7 +152,8 @@ typedef struct sstmc_sampler {
 /* config.tracer (MatlabInterface.cu:1374-1457). */
 typedef struct sstmc_tracer {
     int32_t type;                  /* SSTMC_TRACER_NEE */
-    int32_t isCBS;                 /* coherent backscattering (field output of the TemporalSampler) */
+    int32_t isCBS;                 /* coherent backscattering: field CBS (TemporalSampler) or the
+                                      correlation pairings CBS01-CBS04 (TemporalCorrelationSampler) */
     int32_t output;                /* SSTMC_OUTPUT_FIELD or SSTMC_OUTPUT_CORRELATION */
 } sstmc_tracer;
 
@@ -273,7 +274,11 @@ int sstmc_debug_batch(sstmc_handle* h, uint64_t path_begin, uint32_t npaths,
  * (NextEventEstimation.cuh:1212-1235, 1432-1443).
  *  term_ar[P*Nt*Nv*2]  view connected at x1: g_v(x1) f_v(x1 -> x2) w2 (zero when K < 2)
  *  term_br[P*Nt*Nl*2]  sum_{k>=2} g_l(x_k) f_l(x_k -> x_{k-1}) exp(i 2 pi (rho_k + L_k / lambda))
- * The multiple-scattering field is (sum_p A^T B + sum_p Br^T Ar) / sqrt(2). */
+ * The multiple-scattering field is (sum_p A^T B + sum_p Br^T Ar) / sqrt(2).
+ * Under the TemporalCorrelationSampler the two hold CBS04's halves, Gv0 conj(u2Gv0) C4_v1v2 at x1
+ * (with the path weight) and sum_{k>=2} Gl conj(u2Gl) C4_l1l2 at x_k (NextEventEstimation.cuh:
+ * 1332-1378); the multiple-scattering correlation is (sum_p A^T B + sum_p Br^T Ar + CBS02 +
+ * CBS03) / 2, the mixed pairings CBS02 + CBS03 formed per (l, v) on the device. */
 int sstmc_debug_cbs(sstmc_handle* h, uint32_t npaths, double* term_ar, double* term_br);
 
 /* The reference optical depth of the relative attenuation cut (precision.tauCut) in the last

@@ -135,9 +135,8 @@ static int validate(const sstmc_config* c, bool checkGrid = true) {
     for (uint32_t m = 0; m < nmat; ++m)
         if (!(sp.D[m] >= 0)) return SSTMC_INVALID_CONFIG;   // MatlabInterface.cu:1293-1297
     if (sp.type == SSTMC_SAMPLER_TEMPORAL_CORRELATION) {
-        // the correlation form of coherent backscattering (CBS02-CBS04, NextEventEstimation.cuh:
-        // 1237-1383) is not built; the sampler's output is already a correlation
-        if (c->tracer.isCBS) return SSTMC_NOT_SUPPORTED;
+        // the sampler's output is already a correlation (with isCBS: CBS01-CBS04,
+        // NextEventEstimation.cuh:1138-1391)
         if (c->tracer.output != SSTMC_OUTPUT_FIELD) return SSTMC_INVALID_CONFIG;
         // illumination2 / view2: present, and as large as illumination / view (MatlabInterface.cu:
         // 1313-1331); the u1 and u2 sets share one source type and area flag here

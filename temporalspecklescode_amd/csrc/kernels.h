@@ -21,6 +21,15 @@ struct NeeOut {
     void* Br;         // [t][p][l] illumination connected at x_k, k >= 2, summed with the path phase
     unsigned long long* steps;   // voxel steps of the NEE walks (accumulated): k_nee adds to
                                  // steps[0], k_nee_view to steps[2]
+    // TemporalCorrelationSampler with coherent backscattering (null otherwise; Ar then holds
+    // Gv0 conj(u2Gv0) C4_v1v2 and Br sum_k Gl conj(u2Gl) C4_l1l2): the first vertex's u1 and u2
+    // terms of either source kept apart, F1l / F2l [t][p][l] (u1 with the path weight), F1v / F2v
+    // [t][p][v]; and the u1 / u2 terms of every vertex k >= 1 without the decorrelation, views
+    // Gv1 / Gv2 [t][vertex][v], illuminations Gl1 / Gl2 [t][vertex][l] (vertex < gcap, u1 without
+    // the inner momentum transfer, neither with the path phase, which cancels in every pairing)
+    void *F1l = nullptr, *F2l = nullptr, *F1v = nullptr, *F2v = nullptr;
+    void *Gv1 = nullptr, *Gv2 = nullptr, *Gl1 = nullptr, *Gl2 = nullptr;
+    uint64_t gcap = 0;
 };
 
 // How single scattering is contracted (decided on the host from the source/scattering types).
@@ -50,6 +59,7 @@ struct NeeParams {
     uint32_t frameChunks;  // view kernels: frame chunks per path (field output: one frame each,
                            // correlation: up to 16 frames each)
     uint32_t sameViews;    // TemporalCorrelationSampler: view2 holds the same sources as view
+    uint32_t sameIllums;   // TemporalCorrelationSampler: illumination2 holds the same sources
 };
 
 hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc, const SourceView& il,
@@ -66,6 +76,21 @@ hipError_t launchBatchTotals(const uint32_t* misses, const uint32_t* off, uint32
 hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
                      const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
                      const NeeOut& out, hipStream_t st, hipEvent_t between = nullptr);
+
+// TemporalCorrelationSampler with coherent backscattering: the mixed pairings CBS02 + CBS03 of
+// every path with at least two vertices, from the NeeOut first-vertex (F*) and per-vertex (G*)
+// terms, added to U[t][v][l] with the factor 1/2; deterministic (split over paths, slices summed
+// in order into `part`, at most maxSplits slices of nt * L * V complex FP64).
+struct CbsCross {
+    const void *F1l, *F2l, *F1v, *F2v;
+    const void *Gv1, *Gv2, *Gl1, *Gl2;
+    uint64_t gcap;
+};
+hipError_t launchCbsCross(int dims, int fp64, const SourceView& il, const SourceView& vw, const FramesView& fr,
+                          const VertexBatch& vb, const CbsCross& c, uint32_t npaths, double lambda, double* U,
+                          double2* part, uint32_t maxSplits, hipStream_t st);
+// Slices launchCbsCross uses for an L x V x nt field (its `part` holds that many).
+uint32_t cbsCrossSplits(uint32_t L, uint32_t V, uint32_t nt, uint32_t npaths);
 
 // Contraction U[t][v][l] += scale(l,v) * sum_p A[t][p][l] * B[t][p][v] for t < nt.
 // U is complex FP64 (interleaved). F (optional, real, [v][l]) multiplies the sum elementwise,

@@ -515,16 +515,22 @@ __device__ inline void nee_path_corr(const MediumView& m, const R* sig, const Sc
     const uint32_t K = poolOk ? vb.k[p] : 0u;
     const uint64_t off = vb.off[p];
     const C zero = Cx<R>::make(0, 0);
+    // coherent backscattering (TemporalCorrelationSampler): Ar / Br and the separate first-vertex
+    // terms (NextEventEstimation.cuh:1237-1378)
+    C* Ar = (C*)out.Ar;
+    C* Br = (C*)out.Br;
     if (K == 0) {
         for (uint32_t i = sub; i < L * NT; i += lpp) {
             const size_t o = ((size_t)(i / L) * P + p) * L + i % L;
             A[o] = zero;
             a[o] = zero;
+            if (Br) Br[o] = zero;
         }
         for (uint32_t i = sub; i < V * NT; i += lpp) {
             const size_t o = ((size_t)(i / V) * P + p) * V + i % V;
             B[o] = zero;
             b[o] = zero;
+            if (Ar) Ar[o] = zero;
         }
         return;
     }
@@ -565,8 +571,9 @@ __device__ inline void nee_path_corr(const MediumView& m, const R* sig, const Sc
     const R w1 = p1 > 0 ? (R)1 / p1 : (R)0;
     const R w2 = p2 > 0 ? (R)1 / p2 : (R)0;
     const bool isoPointIl = il.type == 1 && sc.allIsotropic;
-    // illumination terms carry the path weight: walked exactly; view terms take the cut
-    const R tauIl = sizeof(R) == 4 ? (R)prm.exactCutF : tauInf, tauVw = tauCutFF;
+    // illumination terms carry the path weight: walked exactly; view terms take the cut, except
+    // with coherent backscattering, where the view terms at x1 carry it too
+    const R tauIl = sizeof(R) == 4 ? (R)prm.exactCutF : tauInf, tauVw = Ar ? tauIl : tauCutFF;
 
     for (uint32_t i = sub; i < L * NT; i += lpp) {
         const uint32_t l = i % L, t = i / L;
@@ -593,12 +600,20 @@ __device__ inline void nee_path_corr(const MediumView& m, const R* sig, const Sc
             const d3 q = mkd(0.5 * (i1.x + i2.x) - seg12.x, 0.5 * (i1.y + i2.y) - seg12.y, 0.5 * (i1.z + i2.z) - seg12.z);
             const double e = kw2 * fabs(fr.Dd[m1] * tau) * ddot<DIMS>(q, q);
             A[o] = cis<R>(g1 * f1 * g2 * f2 * w2 * expR<R>((R)-e), gp1 + fp1 - gp2 - fp2);
+            if (out.F1l) {
+                ((C*)out.F1l)[o] = cis<R>(g1 * f1 * w2, gp1 + fp1);
+                ((C*)out.F2l)[o] = cis<R>(g2 * f2, gp2 + fp2);
+            }
         } else {
             A[o] = zero;
         }
     }
-    if (!multi)
+    if (!multi) {
         for (uint32_t i = sub; i < V * NT; i += lpp) B[((size_t)(i / V) * P + p) * V + i % V] = zero;
+        if (Br)
+            for (uint32_t i = sub; i < L * NT; i += lpp) Br[((size_t)(i / L) * P + p) * L + i % L] = zero;
+    }
+    const bool isoPointVw = vw.type == 1 && sc.allIsotropic;
     for (uint32_t i = sub; i < V * NT; i += lpp) {
         const uint32_t v = i % V, t = i / V;
         const double s = 0.5 * fr.ticks[t];
@@ -607,7 +622,33 @@ __device__ inline void nee_path_corr(const MediumView& m, const R* sig, const Sc
         double gp1, gp2;
         gTerm<DIMS, R>(m, sig, vw, v, x1a, invLambda, tauVw, &steps, &g1, &gp1);
         gTerm<DIMS, R>(m, sig, vw, V + v, x1b, invLambda, tauVw, &steps, &g2, &gp2);
-        b[((size_t)t * P + p) * V + v] = cis<R>(g1 * g2, gp1 - gp2);
+        const size_t o = ((size_t)t * P + p) * V + v;
+        b[o] = cis<R>(g1 * g2, gp1 - gp2);
+        if (Ar) {
+            // the reversed path's first connection: the views at x1 with f toward x2, u1 with the
+            // path weight; both amplitudes at the u1 positions (quirk Q3); C4_v1v2 at x1
+            if (multi) {
+                const Geo x2a = shiftGeo(x2m, fr.Ud[m2], -s);
+                const d3 dir = dnormalize<DIMS>(dsub(x2a.pd, x1a.pd));
+                R f1, f2;
+                double fp1 = 0, fp2 = 0;
+                if (isoPointVw) {
+                    f1 = f2 = isoAmp<DIMS, R>();
+                } else {
+                    f1 = phaseAmp<DIMS, R, TAB>(sc, m1, ddot<DIMS>(incoming<DIMS>(vw, v, x1a.pd), dir), &fp1);
+                    f2 = phaseAmp<DIMS, R, TAB>(sc, m1, ddot<DIMS>(incoming<DIMS>(vw, V + v, x1a.pd), dir), &fp2);
+                }
+                const d3 e1 = incoming<DIMS>(vw, v, x1c.pd), e2 = incoming<DIMS>(vw, V + v, x1c.pd);
+                const d3 q = mkd(0.5 * (e1.x + e2.x) - seg12.x, 0.5 * (e1.y + e2.y) - seg12.y,
+                                 0.5 * (e1.z + e2.z) - seg12.z);
+                const double ex = kw2 * fabs(fr.Dd[m1] * fr.ticks[t]) * ddot<DIMS>(q, q);
+                Ar[o] = cis<R>(g1 * f1 * w2 * g2 * f2 * expR<R>((R)-ex), gp1 + fp1 - gp2 - fp2);
+                ((C*)out.F1v)[o] = cis<R>(g1 * f1 * w2, gp1 + fp1);
+                ((C*)out.F2v)[o] = cis<R>(g2 * f2, gp2 + fp2);
+            } else {
+                Ar[o] = zero;
+            }
+        }
     }
     *stepsOut += steps;
 }
@@ -937,12 +978,14 @@ __device__ inline void corrViewG(const MediumView& m, const R* sig, const Scatte
     R ga;
     double gp;
     if constexpr (SRC == kSrcPoint) gPoint<DIMS, R>(m, sig, vd, vw.area != 0, xg, invLambda, tauCut, steps, &ga, &gp);
-    else gFar<DIMS, R>(m, sig, vd, vf, false, xg, invLambda, tauCut, steps, &ga, &gp);
+    else gFar<DIMS, R>(m, sig, vd, vf, SRC == kSrcFarIllum, xg, invLambda, tauCut, steps, &ga, &gp);
     R fv;
     if (isoPoint) {
         fv = isoAmp<DIMS, R>();
     } else {
-        const d3 av = SRC == kSrcFarView ? mkd(-vd.x, -vd.y, -vd.z) : dnormalize<DIMS>(dsub(pf, vd));
+        const d3 av = SRC == kSrcFarIllum  ? vd
+                      : SRC == kSrcFarView ? mkd(-vd.x, -vd.y, -vd.z)
+                                           : dnormalize<DIMS>(dsub(pf, vd));
         fv = phaseAmp<DIMS, R, TAB>(sc, mat, ddot<DIMS>(av, back), &gp);
     }
     *amp = ga * fv;
@@ -952,13 +995,17 @@ __device__ inline void corrViewG(const MediumView& m, const R* sig, const Scatte
 // Multiple-scattering view terms of the TemporalCorrelationSampler: per (path, chunk of up to
 // kCorrLags lags, block of views), one view per lane,
 //   B[t][p][v] = sum_{k>=2} G_k,v1(-tau/2) conj G_k,v2(+tau/2) exp(-k^2 |tau| (D_k |q_k|^2 + mt_k))
-// with q_k = -(o_v1 + o_v2) / 2 - n(x_{k-1} - x_k) (o: direction toward the view; C1_v1v2) and
+// with q_k = (e_v1 + e_v2) / 2 - n(x_{k-1} - x_k) (e: incoming direction, minus the direction
+// toward the view; C1_v1v2) and
 // mt_k = sum_{j=2}^{k-1} D_j |n(x_j - x_{j-1}) - n(x_{j+1} - x_j)|^2 (momentTransfer, applied on u1
 // only: pathContriobutionKernel, TemporalSamplerCorrelation.cuh:384-450), all from unshifted
 // positions. u1's g sees x_k at -tau/2, u2's at +tau/2; both amplitudes see x_{k-1} and x_k at
 // -tau/2 (quirk Q3). Vertices whose own and previous materials are at rest are lag-independent
 // except for the exponential: their G products are formed once for all the chunk's lags (once,
-// not twice, when view2 holds the same sources as view).
+// not twice, when view2 holds the same sources as view). With coherent backscattering the same
+// kernel runs over the illuminations (SRC kSrcFarIllum or kSrcPoint) into Br (C4_l1l2), and each
+// vertex's u1 and u2 terms G_k,1 (without the inner transfer) and G_k,2 go to out.Gv1 / Gv2 for
+// the mixed pairings.
 template <int DIMS, typename R, int SRC, bool WAVE, bool TAB>
 __global__ void __launch_bounds__(256) k_nee_view_corrsampler(MediumView m, ScatterView sc, SourceView vw,
                                                               FramesView fr, VertexBatch vb, NeeParams prm,
@@ -978,6 +1025,8 @@ __global__ void __launch_bounds__(256) k_nee_view_corrsampler(MediumView m, Scat
     const double kw2 = 39.478417604357434 * invLambda * invLambda;
     const R tauCut = viewCut<R>(prm);
     C* B = (C*)out.B;
+    C* G1out = (C*)out.Gv1;
+    C* G2out = (C*)out.Gv2;
     const bool isoPoint = SRC == kSrcPoint && sc.allIsotropic;
     const bool same = prm.sameViews != 0;
     uint32_t steps = 0;
@@ -1004,9 +1053,13 @@ __global__ void __launch_bounds__(256) k_nee_view_corrsampler(MediumView m, Scat
             }
             prevB = bk;
             const d3 xk = mkd(xkm.x, xkm.y, xkm.z);
-            const d3 o1 = SRC == kSrcFarView ? vd1 : dnormalize<DIMS>(dsub(vd1, xk));
-            const d3 o2 = SRC == kSrcFarView ? vd2 : dnormalize<DIMS>(dsub(vd2, xk));
-            const d3 q = mkd(-0.5 * (o1.x + o2.x) - bk.x, -0.5 * (o1.y + o2.y) - bk.y, -0.5 * (o1.z + o2.z) - bk.z);
+            const d3 e1 = SRC == kSrcFarIllum  ? vd1
+                          : SRC == kSrcFarView ? mkd(-vd1.x, -vd1.y, -vd1.z)
+                                               : dnormalize<DIMS>(dsub(xk, vd1));
+            const d3 e2 = SRC == kSrcFarIllum  ? vd2
+                          : SRC == kSrcFarView ? mkd(-vd2.x, -vd2.y, -vd2.z)
+                                               : dnormalize<DIMS>(dsub(xk, vd2));
+            const d3 q = mkd(0.5 * (e1.x + e2.x) - bk.x, 0.5 * (e1.y + e2.y) - bk.y, 0.5 * (e1.z + e2.z) - bk.z);
             const double rate = kw2 * (fr.Dd[mat] * ddot<DIMS>(q, q) + mt);   // times |tau|
             const double* Uk = fr.Ud[mat];
             const double* Up = fr.Ud[mp];
@@ -1026,6 +1079,14 @@ __global__ void __launch_bounds__(256) k_nee_view_corrsampler(MediumView m, Scat
                                                  &steps, &a2, &ph2);
                 }
                 const C g = cis<R>(a1 * a2, ph1 - ph2);
+                if (G1out) {
+                    const C g1 = cis<R>(a1, ph1), g2 = cis<R>(a2, ph2);
+                    for (uint32_t j = 0; j < nt; ++j) {
+                        const size_t o = ((size_t)(t0 + j) * out.gcap + ok) * V + v;
+                        G1out[o] = g1;
+                        G2out[o] = g2;
+                    }
+                }
                 for (uint32_t j = 0; j < nt; ++j) {
                     const R w = expR<R>((R)(-rate * fabs(fr.ticks[t0 + j])));
                     C c = acc[j][threadIdx.x];
@@ -1046,6 +1107,11 @@ __global__ void __launch_bounds__(256) k_nee_view_corrsampler(MediumView m, Scat
                     corrViewG<DIMS, R, SRC, TAB>(m, sig, sc, vw, vd2, vf2, xb, xa.pd, back, mat, invLambda, tauCut,
                                                  isoPoint, &steps, &a2, &ph2);
                     const C g = cis<R>(a1 * a2 * expR<R>((R)(-rate * fabs(tau))), ph1 - ph2);
+                    if (G1out) {
+                        const size_t o = ((size_t)(t0 + j) * out.gcap + ok) * V + v;
+                        G1out[o] = cis<R>(a1, ph1);
+                        G2out[o] = cis<R>(a2, ph2);
+                    }
                     C c = acc[j][threadIdx.x];
                     c.x += g.x;
                     c.y += g.y;
@@ -1058,6 +1124,154 @@ __global__ void __launch_bounds__(256) k_nee_view_corrsampler(MediumView m, Scat
     }
     uint32_t tot = waveSum(steps);
     if ((threadIdx.x & 63) == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
+}
+
+// Mixed pairings of coherent backscattering under the TemporalCorrelationSampler
+// (NextEventEstimation.cuh:1237-1330), which do not factor over (l, v): per path,
+//   CBS02 = conj(u2Gl0_l) Gv0_v C2_v1l2(x1) * sum_k Gl_k,l conj(u2Gv_k,v) C2_l1v2(x_k)
+//   CBS03 = Gl0_l conj(u2Gv0_v) C3_l1v2(x1) * sum_k conj(u2Gl_k,l) Gv_k,v C3_v1l2(x_k)
+// each u1 term at x_k times the inner transfer exp(-k^2 |tau| mt_k), the C factors
+// exp(-k^2 |D tau| |(e1 + e2) / 2 - n|^2) of the two sources' incoming directions e at the
+// connected vertex and n toward its neighbour on the path (x2 for x1, x_{k-1} for x_k),
+// temporalCorrelationSourceContributionMultiple (SourceInterface.cuh:280-352). One thread per
+// (l, v) of a tile of tl x (256 / tl), one lag per grid row, paths split over grid rows into
+// `part` slices; FP64 sums.
+template <int DIMS, typename R>
+__global__ void __launch_bounds__(256) k_cbs_cross(SourceView il, SourceView vw, FramesView fr, VertexBatch vb,
+                                                   CbsCross c, uint32_t npaths, uint32_t tl, uint32_t splits,
+                                                   uint32_t pathsPerSplit, double kw2, double2* __restrict__ part) {
+    typedef typename Cx<R>::T C;
+    const uint32_t L = il.n / 2, V = vw.n / 2;
+    const uint32_t l = blockIdx.x * tl + threadIdx.x % tl;
+    const uint32_t v = blockIdx.y * (256 / tl) + threadIdx.x / tl;
+    const uint32_t t = blockIdx.z / splits, split = blockIdx.z % splits;
+    if (l >= L || v >= V) return;
+    const bool poolOk = (uint64_t)vb.off[npaths] <= vb.cap;
+    const uint32_t pBegin = split * pathsPerSplit;
+    const uint32_t pEnd = poolOk ? min(npaths, pBegin + pathsPerSplit) : 0u;
+    const double tau = fr.ticks[t];
+    const C* F1l = (const C*)c.F1l;
+    const C* F2l = (const C*)c.F2l;
+    const C* F1v = (const C*)c.F1v;
+    const C* F2v = (const C*)c.F2v;
+    const C* Gl1 = (const C*)c.Gl1;
+    const C* Gl2 = (const C*)c.Gl2;
+    const C* Gv1 = (const C*)c.Gv1;
+    const C* Gv2 = (const C*)c.Gv2;
+    auto decay = [&](const d3& e1, const d3& e2, const d3& n, double dtD) {
+        const d3 q = mkd(0.5 * (e1.x + e2.x) - n.x, 0.5 * (e1.y + e2.y) - n.y, 0.5 * (e1.z + e2.z) - n.z);
+        return exp(-dtD * ddot<DIMS>(q, q));
+    };
+    double accR = 0, accI = 0;
+    for (uint32_t p = pBegin; p < pEnd; ++p) {
+        const uint32_t K = vb.k[p];
+        if (K < 2) continue;
+        const uint64_t off = vb.off[p];
+        const float4 x1m = vb.xm[off], x2m = vb.xm[off + 1];
+        const d3 x1 = mkd(x1m.x, x1m.y, x1m.z);
+        const d3 n12 = dnormalize<DIMS>(dsub(mkd(x2m.x, x2m.y, x2m.z), x1));
+        const double D1 = kw2 * fabs(fr.Dd[__float_as_uint(x1m.w)] * tau);
+        // first connections C2_v1l2 (view u1, illumination u2) and C3_l1v2 at x1 toward x2
+        const size_t ol = ((size_t)t * npaths + p) * L + l, ov = ((size_t)t * npaths + p) * V + v;
+        const C a1 = F1l[ol], a2 = F2l[ol], c1 = F1v[ov], c2 = F2v[ov];
+        const double t2 = decay(incoming<DIMS>(vw, v, x1), incoming<DIMS>(il, L + l, x1), n12, D1);
+        const double t3 = decay(incoming<DIMS>(il, l, x1), incoming<DIMS>(vw, V + v, x1), n12, D1);
+        // conj(a2) c1 t2 and a1 conj(c2) t3
+        const double f2r = ((double)a2.x * c1.x + (double)a2.y * c1.y) * t2;
+        const double f2i = ((double)a2.x * c1.y - (double)a2.y * c1.x) * t2;
+        const double f3r = ((double)a1.x * c2.x + (double)a1.y * c2.y) * t3;
+        const double f3i = ((double)a1.y * c2.x - (double)a1.x * c2.y) * t3;
+        double s2r = 0, s2i = 0, s3r = 0, s3i = 0, mt = 0;
+        d3 prevB = mkd(0.0, 0.0, 0.0);
+        uint32_t mp = __float_as_uint(x1m.w);
+        for (uint32_t k = 1; k < K; ++k) {
+            const uint64_t ok = off + k;
+            const float4 xkm = vb.xm[ok];
+            const uint32_t mk = __float_as_uint(xkm.w);
+            const VertexRec rec = vb.rec[ok];
+            const d3 bk = mkd(rec.bx, rec.by, rec.bz);   // n(x_{k-1} - x_k)
+            if (k >= 2) {
+                const d3 dq = dsub(bk, prevB);
+                mt += fr.Dd[mp] * ddot<DIMS>(dq, dq);
+            }
+            prevB = bk;
+            mp = mk;
+            const d3 xk = mkd(xkm.x, xkm.y, xkm.z);
+            const double inner = exp(-kw2 * fabs(tau) * mt);
+            const double Dk = kw2 * fabs(fr.Dd[mk] * tau);
+            const double u2 = decay(incoming<DIMS>(il, l, xk), incoming<DIMS>(vw, V + v, xk), bk, Dk) * inner;
+            const double u3 = decay(incoming<DIMS>(vw, v, xk), incoming<DIMS>(il, L + l, xk), bk, Dk) * inner;
+            const size_t gl = ((size_t)t * c.gcap + ok) * L + l, gv = ((size_t)t * c.gcap + ok) * V + v;
+            const C l1 = Gl1[gl], l2 = Gl2[gl], v1 = Gv1[gv], v2 = Gv2[gv];
+            // Gl_k conj(u2Gv_k) and conj(u2Gl_k) Gv_k
+            s2r += ((double)l1.x * v2.x + (double)l1.y * v2.y) * u2;
+            s2i += ((double)l1.y * v2.x - (double)l1.x * v2.y) * u2;
+            s3r += ((double)l2.x * v1.x + (double)l2.y * v1.y) * u3;
+            s3i += ((double)l2.x * v1.y - (double)l2.y * v1.x) * u3;
+        }
+        accR += f2r * s2r - f2i * s2i + f3r * s3r - f3i * s3i;
+        accI += f2r * s2i + f2i * s2r + f3r * s3i + f3i * s3r;
+    }
+    part[((size_t)blockIdx.z * V + v) * L + l] = make_double2(accR, accI);
+}
+
+// U[t][v][l] += 1/2 sum over slices (in order) of part[t][slice][v][l]
+__global__ void __launch_bounds__(256) k_cbs_cross_reduce(const double2* __restrict__ part, uint32_t splits,
+                                                          uint32_t LV, uint32_t nt, double* __restrict__ U) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (uint64_t)LV * nt) return;
+    const uint32_t t = (uint32_t)(i / LV), e = (uint32_t)(i % LV);
+    double re = 0, im = 0;
+    for (uint32_t s = 0; s < splits; ++s) {
+        const double2 x = part[((size_t)t * splits + s) * LV + e];
+        re += x.x;
+        im += x.y;
+    }
+    U[2 * i] += 0.5 * re;
+    U[2 * i + 1] += 0.5 * im;
+}
+
+uint32_t cbsCrossSplits(uint32_t L, uint32_t V, uint32_t nt, uint32_t npaths) {
+    uint32_t tl = 1;
+    while (tl < 16 && tl < L) tl *= 2;
+    const uint64_t tiles = (uint64_t)((L + tl - 1) / tl) * ((V + 256 / tl - 1) / (256 / tl)) * nt;
+    uint64_t s = (2048 + tiles - 1) / tiles;
+    if (s > 64) s = 64;
+    if (s > npaths / 16) s = npaths / 16;
+    return s < 1 ? 1 : (uint32_t)s;
+}
+
+template <int DIMS, typename R>
+static void launchCbsCrossT(const SourceView& il, const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
+                            const CbsCross& c, uint32_t npaths, double lambda, double* U, double2* part,
+                            uint32_t maxSplits, hipStream_t st) {
+    const uint32_t L = il.n / 2, V = vw.n / 2, nt = fr.nt;
+    uint32_t tl = 1;
+    while (tl < 16 && tl < L) tl *= 2;
+    uint32_t splits = cbsCrossSplits(L, V, nt, npaths);
+    if (splits > maxSplits) splits = maxSplits;
+    const uint32_t pps = (npaths + splits - 1) / splits;
+    const double kw2 = 39.478417604357434 / (lambda * lambda);
+    const dim3 grid((L + tl - 1) / tl, (V + 256 / tl - 1) / (256 / tl), nt * splits);
+    hipLaunchKernelGGL((k_cbs_cross<DIMS, R>), grid, dim3(256), 0, st, il, vw, fr, vb, c, npaths, tl, splits, pps,
+                       kw2, part);
+    const uint64_t n = (uint64_t)L * V * nt;
+    hipLaunchKernelGGL(k_cbs_cross_reduce, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, part, splits, L * V,
+                       nt, U);
+}
+
+hipError_t launchCbsCross(int dims, int fp64, const SourceView& il, const SourceView& vw, const FramesView& fr,
+                          const VertexBatch& vb, const CbsCross& c, uint32_t npaths, double lambda, double* U,
+                          double2* part, uint32_t maxSplits, hipStream_t st) {
+    if (npaths == 0) return hipSuccess;
+    if (dims == 2) {
+        if (fp64) launchCbsCrossT<2, double>(il, vw, fr, vb, c, npaths, lambda, U, part, maxSplits, st);
+        else launchCbsCrossT<2, float>(il, vw, fr, vb, c, npaths, lambda, U, part, maxSplits, st);
+    } else {
+        if (fp64) launchCbsCrossT<3, double>(il, vw, fr, vb, c, npaths, lambda, U, part, maxSplits, st);
+        else launchCbsCrossT<3, float>(il, vw, fr, vb, c, npaths, lambda, U, part, maxSplits, st);
+    }
+    return hipGetLastError();
 }
 
 // The reference of the relative attenuation cut: the batch's least view optical depth. One
@@ -1205,6 +1419,20 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
         if (between) hipEventRecord(between, st);
         if (vw.type == 2) launchViewCorrSampler<DIMS, R, kSrcFarView>(m, sc, vw, fr, vb, p, out, st);
         else launchViewCorrSampler<DIMS, R, kSrcPoint>(m, sc, vw, fr, vb, p, out, st);
+        if (out.Br) {
+            // coherent backscattering: the same per-vertex sums over the illuminations connected at
+            // x_k (the reversed path's last connection) into Br, their u1 / u2 terms into Gl1 / Gl2
+            NeeParams q = p;
+            viewLanes(il.n / 2, &q);
+            q.tauRef = nullptr;
+            q.sameViews = p.sameIllums;
+            NeeOut o = out;
+            o.B = out.Br;
+            o.Gv1 = out.Gl1;
+            o.Gv2 = out.Gl2;
+            if (il.type == 2) launchViewCorrSampler<DIMS, R, kSrcFarIllum>(m, sc, il, fr, vb, q, o, st);
+            else launchViewCorrSampler<DIMS, R, kSrcPoint>(m, sc, il, fr, vb, q, o, st);
+        }
         return;
     }
     if (DIMS == 3 && il.type == 3 && sc.tab)

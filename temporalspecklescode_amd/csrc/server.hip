@@ -171,7 +171,8 @@ static uint64_t estimate(const sstmc_config& c) {
                            (c.dims == 3 ? (uint64_t)(c.medium.zAxisSize - 1) : 1ull);
     const uint32_t R = c.simulation.renderingsNum ? c.simulation.renderingsNum : 1;
     return NextEventEstimation::footprint(P, c.illumination.num, c.view.num, c.sampler.tNum, fp64, frames, cells,
-                                          c.tracer.isCBS != 0) +
+                                          c.tracer.isCBS != 0,
+                                          c.sampler.type == SSTMC_SAMPLER_TEMPORAL_CORRELATION) +
            16ull * c.illumination.num * c.view.num * c.sampler.tNum * R;   // the result buffer
 }
 

@@ -259,7 +259,7 @@ public:
     uint32_t batches() const { return opt_.fullIteration; }
     // device bytes a tracer with these sizes allocates (admission control)
     static uint64_t footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames, uint64_t cells,
-                              bool cbs);
+                              bool cbs, bool acorr = false);
     int debugBatch(uint64_t pathBegin, uint32_t npaths, uint64_t* nvtxOut, uint32_t* pathK, uint32_t* pathMisses,
                    uint64_t* pathOff, float* vtxPos, uint32_t* vtxMat, float* vtxRho, float* vtxXt, double* vtxLen,
                    double* termA, double* termB, double* termSa, double* termSb, double* uBatch);
@@ -293,6 +293,9 @@ private:
         DevBuf<float> tauRef;                 // the batch's least view optical depth (relative cut)
         DevBuf<unsigned char> A, B, a, b;
         DevBuf<unsigned char> Ar, Br;         // coherent backscattering: reversed-path terms
+        // coherent backscattering under the TemporalCorrelationSampler: first-vertex u1 / u2
+        // terms [t][p][l|v] and per-vertex u1 / u2 terms [t][vertex][l|v] (NeeOut)
+        DevBuf<unsigned char> F1l, F2l, F1v, F2v, Gl1, Gl2, Gv1, Gv2;
         hipEvent_t transported = nullptr;     // transport done (transport stream)
         hipEvent_t traced = nullptr;          // NEE done (NEE stream)
         hipEvent_t released = nullptr;        // contraction done: the set may be reused
@@ -323,6 +326,9 @@ private:
     int singleMode_;
     double fscalar_ = 1.0;
     bool sameViews_ = false;        // correlation sampler: view2 holds the same sources as view
+    bool sameIllums_ = false;       // correlation sampler: illumination2 holds the same sources
+    bool cbsCorr_ = false;          // coherent backscattering under the correlation sampler
+    DevBuf<double2> cbsPart_;       // its mixed pairings' split slices
     bool fp64_;
     int contractKind_;
     hipStream_t stream_ = nullptr;    // NEE (and every stage when not pipelined)

@@ -788,6 +788,10 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
         }
         const std::vector<double>& hv = vw_.hostPoints();
         sameViews_ = std::equal(hv.begin(), hv.begin() + 3 * (size_t)V_, hv.begin() + 3 * (size_t)V_);
+        const std::vector<double>& hl = il_.hostPoints();
+        sameIllums_ = std::equal(hl.begin(), hl.begin() + 3 * (size_t)L_, hl.begin() + 3 * (size_t)L_);
+        // coherent backscattering: CBS01-CBS04 (NextEventEstimation.cuh:1138-1391)
+        cbsCorr_ = opt.cbs != 0;
     } else if (opt_.output == SSTMC_OUTPUT_CORRELATION) {
         // S(0) conj S(t): a scalar or fixed F enters squared, a per-path f as f(0) conj f(t)
         if (singleMode_ != kSingleSeparable && singleMode_ != kSingleGeneric && singleMode_ != kSingleGauss) {
@@ -863,6 +867,16 @@ int NextEventEstimation::allocate() {
         if ((e = bs.A.alloc(na)) || (e = bs.a.alloc(na)) || (e = bs.B.alloc(nb)) || (e = bs.b.alloc(nb))) return e;
         if (opt_.cbs && ((e = bs.Ar.alloc(nb)) || (e = bs.Br.alloc(na)))) return e;
         bytes_ += (opt_.cbs ? 3 : 2) * (na + nb) + 20 * (size_t)P_;
+        if (cbsCorr_) {
+            if ((e = bs.F1l.alloc(na)) || (e = bs.F2l.alloc(na)) || (e = bs.F1v.alloc(nb)) || (e = bs.F2v.alloc(nb)))
+                return e;
+            bytes_ += 2 * (na + nb);
+        }
+    }
+    if (cbsCorr_) {
+        const size_t n = (size_t)cbsCrossSplits(L_, V_, T_, P_) * T_ * L_ * V_;
+        if ((e = cbsPart_.alloc(n))) return e;
+        bytes_ += 16 * n;
     }
     if ((e = U_.alloc(2 * (size_t)T_ * L_ * V_))) return e;
     bytes_ += 16 * (size_t)T_ * L_ * V_;
@@ -887,7 +901,16 @@ int NextEventEstimation::growPool(uint64_t cap) {
             bs.delta.release();
         }
     }
-    const uint64_t perVertex = 20 + 32 * recT + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_);
+    uint64_t perVertex = 20 + 32 * recT + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_);
+    if (cbsCorr_) {
+        // per-vertex u1 / u2 terms of every illumination and view, per lag
+        const size_t elem = fp64_ ? 16 : 8;
+        const size_t nl = (size_t)T_ * cap * L_ * elem, nv = (size_t)T_ * cap * V_ * elem;
+        for (BatchSet& bs : sets_)
+            if ((e = bs.Gl1.alloc(nl)) || (e = bs.Gl2.alloc(nl)) || (e = bs.Gv1.alloc(nv)) || (e = bs.Gv2.alloc(nv)))
+                return e;
+        perVertex += 2 * (uint64_t)T_ * (L_ + V_) * elem;
+    }
     bytes_ += 2 * (cap - cap_) * perVertex;
     cap_ = cap;
     return SSTMC_NO_ERROR;
@@ -965,7 +988,19 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     prm.tauCutRel = (float)opt_.tauCut;
     prm.npaths = npaths;
     prm.sameViews = sameViews_ ? 1u : 0u;
+    prm.sameIllums = sameIllums_ ? 1u : 0u;
     NeeOut out{bs.A.p, bs.B.p, bs.a.p, bs.b.p, bs.Ar.p, bs.Br.p, totals_.p + 2};
+    if (cbsCorr_) {
+        out.F1l = bs.F1l.p;
+        out.F2l = bs.F2l.p;
+        out.F1v = bs.F1v.p;
+        out.F2v = bs.F2v.p;
+        out.Gl1 = bs.Gl1.p;
+        out.Gl2 = bs.Gl2.p;
+        out.Gv1 = bs.Gv1.p;
+        out.Gv2 = bs.Gv2.p;
+        out.gcap = cap_;
+    }
     if (launchNee(dims_, fp64_ ? 1 : 0, m, sc, il, vw, fr, vb, prm, out, sN, e ? e[5] : nullptr) != hipSuccess)
         return SSTMC_KERNEL_ERROR;
     if (e) hipEventRecord(e[2], sN);
@@ -973,16 +1008,24 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
 
     // ---- contraction: multiple scattering U += sum_p A^T B; with coherent backscattering the
     // reversed paths add sum_p Br^T Ar and both are scaled by 1/sqrt(2)
-    // (NextEventEstimation.cuh:1385-1391,1448-1454); then single scattering
+    // (NextEventEstimation.cuh:1385-1391,1448-1454); under the correlation sampler CBS01 and CBS04
+    // are these two products, the mixed pairings CBS02 + CBS03 follow, and all four take 1/2;
+    // then single scattering
     if (pipelined && hipStreamWaitEvent(sC, bs.traced, 0) != hipSuccess) return SSTMC_DEVICE_ERROR;
     if (e) hipEventRecord(e[7], sC);
-    const double msScale = opt_.cbs ? 0.70710678118654752440 : 1.0;
+    const double msScale = opt_.cbs ? (cbsCorr_ ? 0.5 : 0.70710678118654752440) : 1.0;
     if (launchContract(contractKind_, bs.A.p, bs.B.p, npaths, L_, V_, T_, nullptr, nullptr, 0, msScale, U, &wsMulti_,
                        sC) != hipSuccess)
         return SSTMC_KERNEL_ERROR;
     if (opt_.cbs && launchContract(contractKind_, bs.Br.p, bs.Ar.p, npaths, L_, V_, T_, nullptr, nullptr, 0, msScale,
                                    U, &wsMulti_, sC) != hipSuccess)
         return SSTMC_KERNEL_ERROR;
+    if (cbsCorr_) {
+        const CbsCross cc{bs.F1l.p, bs.F2l.p, bs.F1v.p, bs.F2v.p, bs.Gv1.p, bs.Gv2.p, bs.Gl1.p, bs.Gl2.p, cap_};
+        if (launchCbsCross(dims_, fp64_ ? 1 : 0, il_.view(), vw_.view(), fr, vb, cc, npaths, opt_.lambda, U,
+                           cbsPart_.p, cbsCrossSplits(L_, V_, T_, P_), sC) != hipSuccess)
+            return SSTMC_KERNEL_ERROR;
+    }
     if (e) hipEventRecord(e[3], sC);
     int rc = contractSingle(npaths, U, set, sC);
     if (rc) return rc;
@@ -992,7 +1035,7 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
 }
 
 uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames,
-                                        uint64_t cells, bool cbs) {
+                                        uint64_t cells, bool cbs, bool acorr) {
     uint64_t elem = fp64 ? 16 : 8;
     // two batch sets (the pipeline's): terms, per-path arrays and vertex pools
     uint64_t terms = 2 * (cbs ? 3ull : 2ull) * T * P * (uint64_t)(L + V) * elem;
@@ -1001,6 +1044,12 @@ uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint
     uint64_t ws = 2ull * 8 * T * L * V * elem;   // split-K partials, two workspaces
     if (!fp64) ws += 2ull * 8 * ((uint64_t)P + 64) * (L + V + 256);   // and their bf16 hi/lo planes
     uint64_t grids = 2 * cells;                  // material grid + packed walk grid (<= 4 bits a cell)
+    if (cbs && acorr) {
+        // correlation CBS: first-vertex u1 / u2 terms, per-vertex u1 / u2 terms, mixed-pairing slices
+        terms += 2 * 2ull * T * P * (uint64_t)(L + V) * elem;
+        pool += 2 * (uint64_t)P * 32 * 2ull * T * (L + V) * elem;
+        field += 16ull * cbsCrossSplits(L, V, T, P) * T * L * V;
+    }
     return terms + field + pool + ws + grids + 48ull * P + (1ull << 20);
 }
 

@@ -89,8 +89,8 @@ def test_python_mirror_messages():
 
 def test_correlation_sampler_config_checks(native):
     """TemporalCorrelationSampler: illumination2 / view2 must be present and as large as
-    illumination / view (MatlabInterface.cu:1313-1331); u1 and u2 share one source type here; its
-    coherent-backscattering form is not built."""
+    illumination / view (MatlabInterface.cu:1313-1331); u1 and u2 share one source type here; with
+    isCBS it takes the correlation pairings CBS01-CBS04."""
     base = W.c1(paths=64, batch=64)
     cs = dict(base, sampler=dict(base["sampler"], type="TemporalCorrelationSampler"),
               illumination2=base["illumination"], view2=base["view"])
@@ -104,7 +104,7 @@ def test_correlation_sampler_config_checks(native):
     pt = dict(cs, view2={"type": "PointSource", "location": {"x": np.zeros(16), "y": np.full(16, 30.0)}})
     assert native.sstmc_validate(config.CConfig(pt, 2).ptr()) == 5   # u2 of another type
     cb = dict(cs, tracer={"type": "nee", "isCBS": True})
-    assert native.sstmc_validate(config.CConfig(cb, 2).ptr()) == 5
+    assert native.sstmc_validate(config.CConfig(cb, 2).ptr()) == 0
     with pytest.raises(ValueError, match="illuminationsHandle2"):
         config.CConfig({k: v for k, v in cs.items() if k != "view2"}, 2)
     with pytest.raises(ValueError, match="Available sampler types are TemporalSampler and TemporalCorrelationSampler"):
