@@ -127,7 +127,7 @@ typedef struct sstmc_source {
     const double* z;               /* ignored in 2D */
     int32_t isAreaSource;          /* PointSource only (PointSource.cuh:140-147) */
     /* GaussianBeam only (MatlabInterface.cu:1153-1190, GaussianBeam.cuh:1026-1196): the beam
-     * directions focalDirection.{x,y,z} (unit vectors) and the aperture */
+     * directions focalDirection.{x,y,z} (unit vectors; dirZ ignored in 2D) and the aperture */
     const double* dirX;
     const double* dirY;
     const double* dirZ;

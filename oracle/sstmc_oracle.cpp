@@ -126,6 +126,39 @@ inline cplx twoSinh(cplx z, cplx r) {
     return out;
 }
 
+// log I_0(z) (the modified Bessel function of the first kind, order 0; the reference's
+// logbesseli(0, z), SpecialFunctions.cuh:42-180, takes its own series and asymptotic forms):
+// I_0 is even, so Re z >= 0 is taken; |z| <= 12 by the series sum_k (z^2 / 4)^k / (k!)^2, beyond by
+// the large-argument expansion I_0(z) ~ e^z / sqrt(2 pi z) [sum c_k z^-k + s i e^(-2z) sum (-1)^k
+// c_k z^-k], c_k = ((2k - 1)!!)^2 / (k! 8^k), s = sign(Im z), stopped before the least term
+// (DLMF 10.40.5), in log form.
+inline cplx logI0(cplx z) {
+    if (z.real() < 0) z = -z;
+    if (std::abs(z) <= 12.0) {
+        const cplx q = 0.25 * z * z;
+        cplx term(1.0, 0.0), sum(1.0, 0.0);
+        for (int k = 1; k < 100; ++k) {
+            term *= q / ((double)k * (double)k);
+            sum += term;
+            if (std::abs(term) < 1e-17 * std::abs(sum)) break;
+        }
+        return std::log(sum);
+    }
+    const cplx iz = 1.0 / z;
+    cplx s1(1.0, 0.0), s2(1.0, 0.0), t(1.0, 0.0);
+    double prev = 1.0;
+    for (int k = 1; k < 200; ++k) {
+        t *= iz * ((2.0 * k - 1.0) * (2.0 * k - 1.0) / (8.0 * k));
+        if (std::abs(t) >= prev || std::abs(t) < 1e-18) break;
+        prev = std::abs(t);
+        s1 += t;
+        s2 += (k & 1) ? -t : t;
+    }
+    cplx sum = s1;
+    if (z.imag() != 0.0) sum += cplx(0.0, z.imag() > 0 ? 1.0 : -1.0) * std::exp(-2.0 * z) * s2;
+    return z - 0.5 * std::log(2.0 * PI * z) + std::log(sum);
+}
+
 struct Problem;
 
 // VectorType.cuh:95-165 device semantics: dot, norm, normalize (zero vector -> +z / +y).
@@ -820,7 +853,8 @@ struct Problem {
         for (size_t j = 0; j < v.kappa.size(); ++j) {
             const double lognu = tc + v.c[j];
             const cplx sm = csqrtRef(sumSquare(cfma(v.kappa[j], w0, T)));
-            if (std::fabs(sm.real()) < 0.0001 && std::fabs(sm.imag()) < 0.0001) f += 2.0 * std::exp(lognu);
+            if (dims == 2) f += std::exp(logI0(sm) + lognu);   // :876-878
+            else if (std::fabs(sm.real()) < 0.0001 && std::fabs(sm.imag()) < 0.0001) f += 2.0 * std::exp(lognu);
             else f += twoSinh(sm, cplx(lognu, 0)) / sm;
         }
         return f;
@@ -852,6 +886,10 @@ struct Problem {
                 gs *= ra;
                 C = csqrtRef(sumSquare(cfma(gs, w, aT)));
                 c = (-ra) * gob * dotR(aT, w);
+            }
+            if (dims == 2) {   // :989-991
+                fs += std::exp(logI0(C) + logI0(sm) + c + lognu);
+                continue;
             }
             const bool smZero = std::fabs(sm.real()) <= 0.0001 && std::fabs(sm.imag()) <= 0.0001;
             const bool cZero = std::fabs(C.real()) <= 0.0001 && std::fabs(C.imag()) <= 0.0001;
@@ -911,7 +949,9 @@ struct Problem {
     // gives every path slot on average.
     void sampleSource(Stream& s, V3* x0, V3* dir) const {
         uint32_t l = s.index(il.n());
-        if (il.type == SSTMC_SOURCE_POINT) {
+        // a 2D GaussianBeam starts from its focal point as a point source does (sampleFirstBeam,
+        // GaussianBeam.cuh:754-773, then the medium's sample)
+        if (il.type == SSTMC_SOURCE_POINT || il.type == SSTMC_SOURCE_GAUSSIAN_BEAM) {
             *x0 = il.pt[l];
             *dir = sc.randomDirection(s);
             return;
@@ -976,7 +1016,7 @@ struct Problem {
         for (;; ++event) {
             if (event >= maxAttempts) return SSTMC_NO_HIT;
             Stream s(seed, path, event, 0);
-            if (gauss()) {   // the first point drawn from the beam profile (sampleFirstBeam)
+            if (gauss() && dims == 3) {   // the first point drawn from the beam profile (sampleFirstBeam)
                 const V3 x = sampleGaussFirst(s, &beam);
                 const uint32_t m = matAt(x);
                 if (m > 0) { v1.x = x; v1.dirIn = il.dir[beam]; v1.m = m; break; }
@@ -1001,7 +1041,7 @@ struct Problem {
             Stream s(seed, path, ev, 0);
             // a GaussianBeam illumination proposes the second point's direction itself
             // (sampleSecondPoint, GaussianBeam.cuh:1673-1748)
-            V3 d = (gauss() && k == 2) ? sampleGaussSecond(beam, prev.x, s) : sc.sample(prev.m, prev.dirIn, s);
+            V3 d = (gauss() && dims == 3 && k == 2) ? sampleGaussSecond(beam, prev.x, s) : sc.sample(prev.m, prev.dirIn, s);
             V3 x;
             int m = med.woodcock(prev.x, d, s, &x);
             if (m <= 0) break;
@@ -1044,7 +1084,9 @@ struct Problem {
     }
     // firstPointProbability (FarField.cuh:469-499, PointSource.cuh:263-294, quirk Q7 in 2D).
     double pr(uint32_t l, const V3& x1, uint64_t* steps) const {
-        if (il.type == SSTMC_SOURCE_GAUSSIAN_BEAM) return prGauss(l, x1);
+        // a 2D GaussianBeam: the point-source density about its focal point (attenuationPoint and
+        // probabilityPointNormalizeKernel, GaussianBeam.cuh:775-792, 1810-1832)
+        if (il.type == SSTMC_SOURCE_GAUSSIAN_BEAM && dims == 3) return prGauss(l, x1);
         if (il.type == SSTMC_SOURCE_FARFIELD)
             return std::exp(-med.opticalDepth(x1, scale(-1.0, il.pt[l]), steps)) * il.prob[l];
         V3 q = sub(il.pt[l], x1);
@@ -1145,7 +1187,8 @@ struct Problem {
         const Vertex& v2 = vs[1];
         double p2 = 0;
         for (uint32_t l = 0; l < nl; ++l) {
-            if (gauss()) {   // secondPointProbability (GaussianBeam.cuh:1835-1940); 0 becomes 1e20
+            // 2D: the scattering density about the focal point's direction (multiplyPdf, :1933-1937)
+            if (gauss() && dims == 3) {   // secondPointProbability (GaussianBeam.cuh:1835-1940); 0 becomes 1e20
                 double q = prl[l] * pdirGauss(l, v1.x, v2.x);
                 p2 += (q == 0 && !sc.allIsotropic) ? 1e20 : q;
             } else {
@@ -1529,13 +1572,16 @@ int build(const sstmc_config* c, Problem& pb) {
                 d.pt.push_back(s.type == SSTMC_SOURCE_FARFIELD ? pb.geo.normalize(p) : p);
             }
             if (s.type == SSTMC_SOURCE_GAUSSIAN_BEAM) {
-                // GaussianBeam (MatlabInterface.cu:1153-1190; GaussianBeam.cuh:1026-1062): 3D, focal
+                // GaussianBeam (MatlabInterface.cu:1153-1190; GaussianBeam.cuh:1026-1062): focal
                 // points and beam directions as given, gamma_a = 1 / aperture^2, the aperture
-                // normalisation vMFnormalizationLog(gamma_a)
-                if (dims != 3 || !s.dirX || !s.dirY || !s.dirZ || !(s.aperture > 0)) return SSTMC_INVALID_CONFIG;
-                for (uint32_t k = 0; k < s.num; ++k) d.dir.push_back(V3(s.dirX[k], s.dirY[k], s.dirZ[k]));
+                // normalisation vMFnormalizationLog(gamma_a) (2D: -log 2 pi - log I_0(gamma_a),
+                // SpecialFunctions.cuh:202-204)
+                if (!s.dirX || !s.dirY || (dims == 3 && !s.dirZ) || !(s.aperture > 0)) return SSTMC_INVALID_CONFIG;
+                for (uint32_t k = 0; k < s.num; ++k)
+                    d.dir.push_back(V3(s.dirX[k], s.dirY[k], dims == 3 ? s.dirZ[k] : 0.0));
                 d.gammaA = 1.0 / (s.aperture * s.aperture);
-                d.apNorm = Scatter::vmfNormLog(d.gammaA);
+                d.apNorm = dims == 3 ? Scatter::vmfNormLog(d.gammaA)
+                                     : -std::log(2.0 * PI) - logI0(cplx(d.gammaA, 0.0)).real();
             }
         }
         if (d.type == SSTMC_SOURCE_FARFIELD && i == 0) pb.buildFarField(d);
@@ -1553,7 +1599,8 @@ int build(const sstmc_config* c, Problem& pb) {
         for (const V3& dv : pb.vw.dir) mv = add(mv, scale(1.0 / pb.vw.n(), dv));
         const double r = pb.geo.norm(mv);
         pb.meanView = r < 0.0001 ? V3(0, 0, 1) : scale(1.0 / r, mv);
-        if (int e = pb.buildGaussian(pb.il)) return e;
+        if (dims == 3)
+            if (int e = pb.buildGaussian(pb.il)) return e;
     }
     // sampler: time table (TemporalSampler.cuh:941-1132); the correlation sampler takes the lags
     // as they are (TemporalSamplerCorrelation.cuh:455-530)
@@ -1965,6 +2012,14 @@ int oracle_cbs_cross_from_vertices(const sstmc_config* cfg, uint32_t n, const ui
         x_sum[2 * j] = acc.real();
         x_sum[2 * j + 1] = acc.imag();
     }
+    return SSTMC_NO_ERROR;
+}
+
+// log I_0(z) of the 2D GaussianBeam forms, out = (re, im), for checks against a library routine.
+int oracle_log_i0(double re, double im, double* out) {
+    const cplx v = logI0(cplx(re, im));
+    out[0] = v.real();
+    out[1] = v.imag();
     return SSTMC_NO_ERROR;
 }
 

@@ -306,13 +306,11 @@ class CConfig:
             loc = src["direction"]
         elif ty == "GaussianBeam":
             # MatlabInterface.cu:1153-1190: focalPoint, focalDirection (one per focal point), aperture
-            if dims != 3:
-                raise ValueError("GaussianBeam sources are supported in 3D")
             dst.type = SOURCE_GAUSSIAN_BEAM
             loc = src["focalPoint"]
             fd = src["focalDirection"]
             n = np.atleast_1d(np.asarray(loc["x"])).size
-            for k, attr in (("x", "dirX"), ("y", "dirY"), ("z", "dirZ")):
+            for k, attr in (("x", "dirX"), ("y", "dirY"), ("z", "dirZ"))[:dims]:
                 v = np.atleast_1d(np.asarray(fd[k], np.float64)).ravel()
                 if v.size != n:
                     raise ValueError("%s focalDirection must hold one direction per focal point" % name)

@@ -113,9 +113,9 @@ static int validate(const sstmc_config* c, bool checkGrid = true) {
         if (s->type != SSTMC_SOURCE_FARFIELD && s->type != SSTMC_SOURCE_POINT && s->type != SSTMC_SOURCE_GAUSSIAN_BEAM)
             return SSTMC_INVALID_CONFIG;
         if (s->num == 0 || !s->x || !s->y || (dims == 3 && !s->z)) return SSTMC_INVALID_CONFIG;
-        // GaussianBeam (MatlabInterface.cu:1153-1190): 3D, one direction per focal point, aperture > 0
+        // GaussianBeam (MatlabInterface.cu:1153-1190): one direction per focal point, aperture > 0
         if (s->type == SSTMC_SOURCE_GAUSSIAN_BEAM &&
-            (dims != 3 || !s->dirX || !s->dirY || !s->dirZ || !(s->aperture > 0)))
+            (!s->dirX || !s->dirY || (dims == 3 && !s->dirZ) || !(s->aperture > 0)))
             return SSTMC_INVALID_CONFIG;
     }
     if (c->illumination.type == SSTMC_SOURCE_GAUSSIAN_BEAM || c->view.type == SSTMC_SOURCE_GAUSSIAN_BEAM) {

@@ -777,8 +777,8 @@ hipError_t launchSingleGeneric(int dims, int fp64, const ScatterView& sc, const 
 // S_p[l,v] = a_p[l] b_p[v] fs(l, v, x1) with fs the two beams' throughputs through the vMF lobes of
 // x1's material (fsGaussianBeam, GaussianBeam.cuh:894-999), FP64 whatever the term precision.
 // Block = 16 x 16 (l, v) outputs; per chunk of 16 paths the first vertices, their materials and
-// the terms are staged in LDS.
-template <typename R>
+// the terms are staged in LDS. DIMS = 2: the I_0 form of fs.
+template <int DIMS, typename R>
 __global__ void __launch_bounds__(256) k_single_gauss(MediumView m, ScatterView sc, SourceView il, SourceView vw,
                                                       FramesView fr, VertexBatch vb,
                                                       const typename SingleT<R>::C* __restrict__ a,
@@ -814,9 +814,9 @@ __global__ void __launch_bounds__(256) k_single_gauss(MediumView m, ScatterView 
                     x10 = mkd(x10.x + (double)d0[0], x10.y + (double)d0[1], x10.z + (double)d0[2]);
                 }
                 sx[tq][0] = x1.x; sx[tq][1] = x1.y; sx[tq][2] = x1.z;
-                sm[tq] = gbMaterial(m, x1);
+                sm[tq] = gbMaterial<DIMS>(m, x1);
                 sx0[tq][0] = x10.x; sx0[tq][1] = x10.y; sx0[tq][2] = x10.z;
-                sm0[tq] = gbMaterial(m, x10);
+                sm0[tq] = gbMaterial<DIMS>(m, x10);
             }
             if (ls < L) av = a[((size_t)t * P + p) * L + ls];
             if (vs < V) bv = b[((size_t)t * P + p) * V + vs];
@@ -830,11 +830,11 @@ __global__ void __launch_bounds__(256) k_single_gauss(MediumView m, ScatterView 
         if (l < L && v < V)
             for (int q = 0; q < 16; ++q) {
                 if (sm[q] == 0) continue;
-                Cd f = gbFs(il.g, vw.g, tab, l, v, sm[q], mkd(sx[q][0], sx[q][1], sx[q][2]), il.g.waveK);
+                Cd f = gbFs<DIMS>(il.g, vw.g, tab, l, v, sm[q], mkd(sx[q][0], sx[q][1], sx[q][2]), il.g.waveK);
                 if (corr) {   // S(0) conj S(t): fs at frame 0 times the conjugate at frame t
                     const bool same = sx0[q][0] == sx[q][0] && sx0[q][1] == sx[q][1] && sx0[q][2] == sx[q][2];
                     const Cd f0 = same ? f
-                                       : (sm0[q] ? gbFs(il.g, vw.g, tab, l, v, sm0[q], mkd(sx0[q][0], sx0[q][1], sx0[q][2]),
+                                       : (sm0[q] ? gbFs<DIMS>(il.g, vw.g, tab, l, v, sm0[q], mkd(sx0[q][0], sx0[q][1], sx0[q][2]),
                                                         il.g.waveK)
                                                  : cd(0, 0));
                     f = cd(f0.re * f.re + f0.im * f.im, f0.im * f.re - f0.re * f.im);
@@ -906,7 +906,7 @@ __global__ void __launch_bounds__(256) k_single_gauss_uv(MediumView m, ScatterVi
                 x1 = mkd(x1.x + (double)d[0], x1.y + (double)d[1], x1.z + (double)d[2]);
             }
             sx[0] = x1.x; sx[1] = x1.y; sx[2] = x1.z;
-            sm1 = gbMaterial(m, x1);
+            sm1 = gbMaterial<3>(m, x1);
         }
         __syncthreads();
         const uint32_t mat = sm1;
@@ -994,10 +994,12 @@ __global__ void __launch_bounds__(256) k_single_gauss_uv(MediumView m, ScatterVi
     }
 }
 
-hipError_t launchSingleGauss(int fp64, int uniformViewDir, int corr, const MediumView& m, const ScatterView& sc,
-                             const SourceView& il, const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
-                             const void* a, const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st) {
-    if (corr) uniformViewDir = 0;   // the shared-direction kernel forms the field fs only
+hipError_t launchSingleGauss(int dims, int fp64, int uniformViewDir, int corr, const MediumView& m,
+                             const ScatterView& sc, const SourceView& il, const SourceView& vw, const FramesView& fr,
+                             const VertexBatch& vb, const void* a, const void* b, uint32_t P, double* U, Workspace* ws,
+                             hipStream_t st) {
+    // the shared-direction kernel forms the 3D field fs only
+    if (corr || dims == 2) uniformViewDir = 0;
     const uint32_t L = il.n, V = vw.n, NT = fr.nt;
     uint32_t tiles = ((L + 15) / 16) * ((V + 15) / 16);
     uint32_t splits = chooseSplits(tiles, P, 64);
@@ -1030,8 +1032,11 @@ hipError_t launchSingleGauss(int fp64, int uniformViewDir, int corr, const Mediu
         else if (uniformViewDir)
             hipLaunchKernelGGL((k_single_gauss_uv<double, kFsFrames>), gridUv, dim3(256), 0, st, m, sc, il, vw, fr, vb,
                                (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr, fpb);
-        else hipLaunchKernelGGL(k_single_gauss<double>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb, (const double2*)a,
-                           (const double2*)b, P, pps, splits, (double2*)ws->ptr, corr);
+        else if (dims == 2)
+            hipLaunchKernelGGL((k_single_gauss<2, double>), grid, dim3(256), 0, st, m, sc, il, vw, fr, vb,
+                               (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr, corr);
+        else hipLaunchKernelGGL((k_single_gauss<3, double>), grid, dim3(256), 0, st, m, sc, il, vw, fr, vb,
+                                (const double2*)a, (const double2*)b, P, pps, splits, (double2*)ws->ptr, corr);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<double2>, dim3((LV + 255) / 256), dim3(256), 0, st,
                                (const double2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,
@@ -1043,8 +1048,11 @@ hipError_t launchSingleGauss(int fp64, int uniformViewDir, int corr, const Mediu
         else if (uniformViewDir)
             hipLaunchKernelGGL((k_single_gauss_uv<float, kFsFrames>), gridUv, dim3(256), 0, st, m, sc, il, vw, fr, vb,
                                (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr, fpb);
-        else hipLaunchKernelGGL(k_single_gauss<float>, grid, dim3(256), 0, st, m, sc, il, vw, fr, vb, (const float2*)a,
-                           (const float2*)b, P, pps, splits, (float2*)ws->ptr, corr);
+        else if (dims == 2)
+            hipLaunchKernelGGL((k_single_gauss<2, float>), grid, dim3(256), 0, st, m, sc, il, vw, fr, vb,
+                               (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr, corr);
+        else hipLaunchKernelGGL((k_single_gauss<3, float>), grid, dim3(256), 0, st, m, sc, il, vw, fr, vb,
+                                (const float2*)a, (const float2*)b, P, pps, splits, (float2*)ws->ptr, corr);
         for (uint32_t t = 0; t < NT; ++t)
             hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st,
                                (const float2*)ws->ptr + (size_t)t * splits * LV, splits, LV, (const float2*)nullptr,

@@ -1,7 +1,9 @@
-// gauss.h -- GaussianBeam sources (GaussianBeam.cuh, 3D) on the device: the beam's complex
-// throughput convolved with the vMF lobes of the scattering material (f, fs), the first-point
-// sampler and probability from the per-beam z tables, and the second-point proposal (a mixture of
-// vMF lobes over lobe pairs). FP64 throughout; complex arithmetic in a small local type.
+// gauss.h -- GaussianBeam sources (GaussianBeam.cuh) on the device: the beam's complex
+// throughput convolved with the vMF lobes of the scattering material (f, fs; 3D through
+// sinh(s) / s, 2D through the modified Bessel function I_0), and in 3D the first-point sampler and
+// probability from the per-beam z tables and the second-point proposal (a mixture of vMF lobes
+// over lobe pairs; in 2D both points are drawn as from a point source at the focal point). FP64
+// throughout; complex arithmetic in a small local type.
 #pragma once
 
 #include "common.h"
@@ -44,6 +46,45 @@ __host__ __device__ inline Cd twoSinhd(Cd z, Cd r) {
     if (n.re > -10.0) out = out - cexpd(n);
     return out;
 }
+__host__ __device__ inline Cd clogd(Cd a) { return cd(log(cabsd(a)), atan2(a.im, a.re)); }
+
+// log I_0(z) for complex z (I_0 is even: the argument is taken with Re z >= 0). For |z| <= 12 the
+// power series sum_k (z^2 / 4)^k / (k!)^2, summed until a term is below 1e-17 of the sum (near the
+// imaginary axis, where I_0 oscillates like J_0, its terms cancel to about e^|z| 2^-53 relative);
+// beyond, the large-argument expansion (DLMF 10.40.5)
+//   I_0(z) ~ e^z / sqrt(2 pi z) [ sum_k c_k z^-k + s i e^(-2z) sum_k (-1)^k c_k z^-k ],
+//   c_k = ((2k - 1)!!)^2 / (k! 8^k), s = sign(Im z) (0 on the real axis),
+// both sums stopped before their least term (about e^(-2 |z|) relative), and taken in log form so
+// that large Re z does not overflow.
+__host__ __device__ inline Cd logI0(Cd z) {
+    if (z.re < 0) z = cd(-z.re, -z.im);
+    const double r = cabsd(z);
+    if (r <= 12.0) {
+        const Cd q = 0.25 * (z * z);
+        Cd term = cd(1.0), sum = cd(1.0);
+        for (int k = 1; k < 100; ++k) {
+            term = (1.0 / ((double)k * (double)k)) * (term * q);
+            sum = sum + term;
+            if (cabsd(term) < 1e-17 * cabsd(sum)) break;
+        }
+        return clogd(sum);
+    }
+    const Cd iz = cdiv(cd(1.0), z);
+    Cd s1 = cd(1.0), s2 = cd(1.0), t = cd(1.0);
+    double prev = 1.0;
+    for (int k = 1; k < 200; ++k) {
+        t = ((2.0 * k - 1.0) * (2.0 * k - 1.0) / (8.0 * k)) * (t * iz);
+        const double mag = cabsd(t);
+        if (mag >= prev || mag < 1e-18) break;
+        prev = mag;
+        s1 = s1 + t;
+        s2 = (k & 1) ? s2 - t : s2 + t;
+    }
+    Cd sum = s1;
+    if (z.im != 0.0) sum = sum + (cd(0.0, z.im > 0 ? 1.0 : -1.0) * cexpd(-2.0 * z)) * s2;
+    return z - 0.5 * clogd(6.283185307179586477 * z) + clogd(sum);
+}
+
 // complex 3-vectors: real part a, imaginary part b
 struct CV {
     d3 a, b;
@@ -85,8 +126,10 @@ __host__ __device__ inline CV gbThroughput(const GaussBeams& g, uint32_t i, cons
 }
 
 // fGaussianBeam (GaussianBeam.cuh:818-892): sum over the lobes of material m of
-// exp(apNorm + log 2 pi + C_j) 2 sinh(s_j) / s_j with s_j = sqrt((gamma_j w0 + T).(gamma_j w0 + T)),
+// exp(apNorm + log 2 pi + C_j) 2 sinh(s_j) / s_j (3D; in 2D exp(log I_0(s_j) + apNorm + log 2 pi +
+// C_j), :876-878) with s_j = sqrt((gamma_j w0 + T).(gamma_j w0 + T)),
 // T = gamma_a d + i k (x0 - focal) the beam's complex throughput (k negative for a view).
+template <int DIMS = 3>
 __device__ inline Cd gbF(const GaussBeams& g, const PhaseTables& t, const d3& focal, const d3& dir, uint32_t m,
                          const d3& x0, const d3& w0, double k) {
     const CV T = CV{scale3(g.gammaA, dir), scale3(k, sub3(x0, focal))};
@@ -96,14 +139,17 @@ __device__ inline Cd gbF(const GaussBeams& g, const PhaseTables& t, const d3& fo
     for (uint32_t j = 0; j < M; ++j) {
         const double lognu = tc + t.vmfC[o + j];
         const Cd s = csqrtd(sumSq(cfmaR(t.vmfKappa[o + j], w0, T)));
-        if (fabs(s.re) < 0.0001 && fabs(s.im) < 0.0001) f = f + cd(2.0 * exp(lognu));
+        if constexpr (DIMS == 2) f = f + cexpd(logI0(s) + cd(lognu));
+        else if (fabs(s.re) < 0.0001 && fabs(s.im) < 0.0001) f = f + cd(2.0 * exp(lognu));
         else f = f + cdiv(twoSinhd(s, cd(lognu)), s);
     }
     return f;
 }
 
 // fsGaussianBeam (GaussianBeam.cuh:894-999): illumination beam l of `a` and view beam v of `b`
-// through the lobes of material m at x0.
+// through the lobes of material m at x0 (2D: exp(log I_0(C) + log I_0(sqrt mu) + c + log nu),
+// :989-991).
+template <int DIMS = 3>
 __device__ inline Cd gbFs(const GaussBeams& a, const GaussBeams& b, const PhaseTables& t, uint32_t l, uint32_t v,
                           uint32_t m, const d3& x0, double k) {
     const CV aT = CV{scale3(a.gammaA, gbDir(a, l)), scale3(k, sub3(x0, gbFocal(a, l)))};
@@ -127,6 +173,10 @@ __device__ inline Cd gbFs(const GaussBeams& a, const GaussBeams& b, const PhaseT
             C = csqrtd(sumSq(cfmaR(gs, w, aT)));
             c = (-ra) * (gob * dotCV(aT, w));
         }
+        if constexpr (DIMS == 2) {
+            fs = fs + cexpd(logI0(C) + logI0(sm) + c + cd(lognu));
+            continue;
+        }
         const bool smZero = fabs(sm.re) <= 0.0001 && fabs(sm.im) <= 0.0001;
         const bool cZero = fabs(C.re) <= 0.0001 && fabs(C.im) <= 0.0001;
         if (!smZero && !cZero) fs = fs + cdiv(twoSinhd(C + sm, c + cd(lognu)), C * sm);
@@ -144,6 +194,8 @@ __host__ __device__ inline double vmfNormLog3(double x) {
     if (x < 5.0) return log(x) - log(12.566370614359172954 * sinh(x));
     return log(x) - 1.8378770664093454836 - x;
 }
+// the 2D one, 1 / (2 pi I_0(kappa)) (SpecialFunctions.cuh:202-204)
+__host__ __device__ inline double vmfNormLog2(double x) { return -1.8378770664093454836 - logI0(cd(x)).re; }
 
 // One term of the second-point proposal (computeCdfBuffer, GaussianBeam.cuh:489-621): the lobe's
 // real axis contribution and log-weight about the mean view direction.
@@ -207,9 +259,11 @@ __device__ inline double gbPdir(const GaussBeams& g, const PhaseTables& t, bool 
 }
 
 // Material at an FP64 position, 0 outside the box (getMaterialInfo, GaussianBeam.cuh:1642).
+template <int DIMS = 3>
 __device__ inline uint32_t gbMaterial(const MediumView& m, const d3& p) {
-    if (!inside<3, double>(m, p)) return 0;
-    return m.grid[cellIndex(m, locate<double>(m, 0, p.x), locate<double>(m, 1, p.y), locate<double>(m, 2, p.z))];
+    if (!inside<DIMS, double>(m, p)) return 0;
+    return m.grid[cellIndex(m, locate<double>(m, 0, p.x), locate<double>(m, 1, p.y),
+                            DIMS == 3 ? locate<double>(m, 2, p.z) : 0)];
 }
 
 // firstPointProbabilityKernel (GaussianBeam.cuh:420-487) for beam l at x1 (material m1): the

@@ -126,9 +126,10 @@ hipError_t launchGaussSetup(const MediumView& m, const double* pts, const double
 // U[t][v][l] += sum_p a[t][p][l] b[t][p][v] fs(l, v, x1), or with corr fs(x1(0)) conj fs(x1(t)).
 // uniformViewDir: every view beam has the direction of view 0 (the per-illumination part of fs is
 // then shared by the views; field output).
-hipError_t launchSingleGauss(int fp64, int uniformViewDir, int corr, const MediumView& m, const ScatterView& sc,
-                             const SourceView& il, const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
-                             const void* a, const void* b, uint32_t P, double* U, Workspace* ws, hipStream_t st);
+hipError_t launchSingleGauss(int dims, int fp64, int uniformViewDir, int corr, const MediumView& m,
+                             const ScatterView& sc, const SourceView& il, const SourceView& vw, const FramesView& fr,
+                             const VertexBatch& vb, const void* a, const void* b, uint32_t P, double* U, Workspace* ws,
+                             hipStream_t st);
 
 // U += 0 helpers
 hipError_t launchZeroU(double* U, size_t n, hipStream_t st);

@@ -206,13 +206,13 @@ __device__ inline void gGauss(const MediumView& m, const R* sig, const d3& dir, 
                               uint32_t* steps, R* amp, double* ph) {
     const d3 w = illum ? mkd(-dir.x, -dir.y, -dir.z) : dir;
     const R tau = walk<DIMS, R>(m, sig, Pos<R>::pick(x.pf, x.pd), Pos<R>::fromD(w), tauCut, steps);
-    *amp = (gbMaterial(m, x.pd) && !(tau > tauCut)) ? expR<R>(-(R)0.5 * tau) : (R)0;
+    *amp = (gbMaterial<DIMS>(m, x.pd) && !(tau > tauCut)) ? expR<R>(-(R)0.5 * tau) : (R)0;
     *ph = 0;
 }
 template <int DIMS, typename R, bool GB = false>
 __device__ inline void gTerm(const MediumView& m, const R* sig, const SourceView& s, uint32_t i, const Geo& x,
                              double invLambda, R tauCut, uint32_t* steps, R* amp, double* ph) {
-    if constexpr (GB && DIMS == 3) {
+    if constexpr (GB) {
         gGauss<DIMS, R>(m, sig, gbDir(s.g, i), s.illum != 0, x, tauCut, steps, amp, ph);
         return;
     }
@@ -329,11 +329,14 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
 
     // ---- path probabilities from the central first vertex (pathProbabilitySingle/Multiple)
     R s1 = 0, s2 = 0;
-    constexpr bool gauss = GB && TAB && DIMS == 3;   // GaussianBeam illumination and view
-    const uint32_t m1g = gauss ? gbMaterial(m, x1c.pd) : 0u;
+    constexpr bool gauss = GB && TAB;   // GaussianBeam illumination and view
+    const uint32_t m1g = gauss ? gbMaterial<DIMS>(m, x1c.pd) : 0u;
     for (uint32_t l = sub; l < L; l += lpp) {
         R pr;
-        if (gauss) {
+        // 3D beams have their own first- and second-point densities; a 2D beam's first point is
+        // drawn as from a point source at its focal point, and so are its densities
+        // (GaussianBeam.cuh:1667-1683, 1810-1832, 1933-1937)
+        if constexpr (gauss && DIMS == 3) {
             // firstPointProbability and secondPointProbability (GaussianBeam.cuh:420-487,
             // 1835-1940): a zero second-point density becomes 1e20 unless every material scatters
             // isotropically
@@ -391,9 +394,9 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
                 const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
                 const d3 pb = fr.temporalAmp ? x2t.pd : x2cd;
                 if (gauss) {   // fGaussianBeam (GaussianBeam.cuh:818-892) of x1's material toward x2
-                    const uint32_t mg = gbMaterial(m, pa);
+                    const uint32_t mg = gbMaterial<DIMS>(m, pa);
                     if (mg) {
-                        const Cd fz = gbF(il.g, *sc.tab, gbFocal(il.g, l), gbDir(il.g, l), mg, pa,
+                        const Cd fz = gbF<DIMS>(il.g, *sc.tab, gbFocal(il.g, l), gbDir(il.g, l), mg, pa,
                                           dnormalize<DIMS>(dsub(pb, pa)), il.g.waveK);
                         fa[f] = (R)cabsd(fz);
                         fp[f] = atan2(fz.im, fz.re) * 0.15915494309189533577;
@@ -735,9 +738,9 @@ __device__ inline typename Cx<R>::T viewTerm(const MediumView& m, const R* sig, 
         double gp;
         gGauss<DIMS, R>(m, sig, vdir, false, xk, tauCut, steps, &ga, &gp);
         const d3 pk = fr.temporalAmp ? xk.pd : mkd(xkm.x, xkm.y, xkm.z);
-        const uint32_t mg = gbMaterial(m, pk);
+        const uint32_t mg = gbMaterial<DIMS>(m, pk);
         if (mg == 0 || ga == (R)0) return Cx<R>::make(0, 0);
-        const Cd fz = gbF(vw.g, *sc.tab, vd, vdir, mg, pk, mkd(-rec.bx, -rec.by, -rec.bz), -vw.g.waveK);
+        const Cd fz = gbF<DIMS>(vw.g, *sc.tab, vd, vdir, mg, pk, mkd(-rec.bx, -rec.by, -rec.bz), -vw.g.waveK);
         return cis<R>(ga * (R)cabsd(fz), atan2(fz.im, fz.re) * 0.15915494309189533577 + rec.phase);
     }
     R ga;
@@ -1342,7 +1345,7 @@ static void launchTauRef(const MediumView& m, const SourceView& vw, const Frames
     const dim3 grid((units + 3) / 4), block(256);
     if (vw.type == 2) hipLaunchKernelGGL((k_tau_ref<DIMS, kSrcFarView>), grid, block, 0, st, m, vw, fr, vb, p, stride);
     else if (vw.type == 1) hipLaunchKernelGGL((k_tau_ref<DIMS, kSrcPoint>), grid, block, 0, st, m, vw, fr, vb, p, stride);
-    else if constexpr (DIMS == 3) hipLaunchKernelGGL((k_tau_ref<DIMS, kSrcGauss>), grid, block, 0, st, m, vw, fr, vb, p, stride);
+    else hipLaunchKernelGGL((k_tau_ref<DIMS, kSrcGauss>), grid, block, 0, st, m, vw, fr, vb, p, stride);
 }
 
 // Lanes per view-kernel work unit for n sources: a wavefront, halved (down to 8) while half still
@@ -1435,8 +1438,8 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
         }
         return;
     }
-    if (DIMS == 3 && il.type == 3 && sc.tab)
-        hipLaunchKernelGGL((k_nee<DIMS, R, true, DIMS == 3>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    if (il.type == 3 && sc.tab)
+        hipLaunchKernelGGL((k_nee<DIMS, R, true, true>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
     else if (sc.tab) hipLaunchKernelGGL((k_nee<DIMS, R, true, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
     else hipLaunchKernelGGL((k_nee<DIMS, R, false, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
     hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, vb, fr, p.npaths,
@@ -1444,7 +1447,7 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
     if (between) hipEventRecord(between, st);
     const bool corr = p.output == 1;
     if (vw.type == 3) {
-        if constexpr (DIMS == 3) launchView<DIMS, R, kSrcGauss>(corr, m, sc, vw, fr, vb, p, out, st);
+        launchView<DIMS, R, kSrcGauss>(corr, m, sc, vw, fr, vb, p, out, st);
     } else if (vw.type == 2) launchView<DIMS, R, kSrcFarView>(corr, m, sc, vw, fr, vb, p, out, st);
     else launchView<DIMS, R, kSrcPoint>(corr, m, sc, vw, fr, vb, p, out, st);
     if (out.Br) {

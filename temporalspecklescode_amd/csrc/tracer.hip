@@ -457,7 +457,8 @@ GaussianBeam::GaussianBeam(int* err, const Medium& medium, const Scattering& sca
     v_.type = SSTMC_SOURCE_GAUSSIAN_BEAM;
     v_.illum = illumination ? 1 : 0;
     v_.n = n;
-    if (medium.dims() != 3 || n == 0 || !s.dirX || !s.dirY || !s.dirZ || !(s.aperture > 0) || !(lambda > 0)) {
+    const bool d3d = medium.dims() == 3;
+    if (n == 0 || !s.dirX || !s.dirY || (d3d && (!s.dirZ || !s.z)) || !(s.aperture > 0) || !(lambda > 0)) {
         *err = SSTMC_INVALID_CONFIG;
         return;
     }
@@ -465,8 +466,8 @@ GaussianBeam::GaussianBeam(int* err, const Medium& medium, const Scattering& sca
     std::vector<double> dir(3 * (size_t)n);
     std::vector<float> pf(3 * (size_t)n);
     for (uint32_t i = 0; i < n; ++i) {
-        pts_[3 * i] = s.x[i]; pts_[3 * i + 1] = s.y[i]; pts_[3 * i + 2] = s.z[i];
-        dir[3 * i] = s.dirX[i]; dir[3 * i + 1] = s.dirY[i]; dir[3 * i + 2] = s.dirZ[i];
+        pts_[3 * i] = s.x[i]; pts_[3 * i + 1] = s.y[i]; pts_[3 * i + 2] = d3d ? s.z[i] : 0.0;
+        dir[3 * i] = s.dirX[i]; dir[3 * i + 1] = s.dirY[i]; dir[3 * i + 2] = d3d ? s.dirZ[i] : 0.0;
         for (int a = 0; a < 3; ++a) pf[3 * i + a] = (float)pts_[3 * i + a];
     }
     dirs_ = dir;
@@ -476,11 +477,14 @@ GaussianBeam::GaussianBeam(int* err, const Medium& medium, const Scattering& sca
     v_.pd = pd_.p;
     GaussBeams& g = v_.g;
     g.gammaA = 1.0 / (s.aperture * s.aperture);
-    g.apNorm = vmfNormLog3(g.gammaA);
+    // the aperture's vMF normalisation (GaussianBeam.cuh:1058-1064; 2D through I_0)
+    g.apNorm = d3d ? vmfNormLog3(g.gammaA) : vmfNormLog2(g.gammaA);
     g.waveK = 2.0 * PI / lambda;
     g.focal = pd_.p;
     g.meanView = mkd(0.0, 0.0, 1.0);
-    if (!illumination) {
+    // views, and 2D illuminations (whose points are drawn as from point sources at the focal
+    // points, GaussianBeam.cuh:1667-1683), need no sampling tables
+    if (!illumination || !d3d) {
         if ((*err = gd_.upload(dir.data(), dir.size()))) return;
         g.dir = gd_.p;
         return;
@@ -759,7 +763,7 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
     if (il_.type() == SSTMC_SOURCE_GAUSSIAN_BEAM || vw_.type() == SSTMC_SOURCE_GAUSSIAN_BEAM) {
         // both roles GaussianBeam (fsGaussianBeam pairs two beams), the TemporalSampler without
         // coherent backscattering
-        if (il_.type() != vw_.type() || acorr || opt.cbs || dims_ != 3) {
+        if (il_.type() != vw_.type() || acorr || opt.cbs) {
             *err = SSTMC_NOT_SUPPORTED;
             return;
         }
@@ -928,7 +932,7 @@ int NextEventEstimation::contractSingle(uint32_t npaths, double* U, int set, hip
         const std::vector<double>& vd = static_cast<const GaussianBeam&>(vw_).directions();
         bool uniformDir = true;
         for (size_t i = 3; i < vd.size() && uniformDir; ++i) uniformDir = vd[i] == vd[i % 3];
-        he = launchSingleGauss(fp64_ ? 1 : 0, uniformDir ? 1 : 0, opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0,
+        he = launchSingleGauss(dims_, fp64_ ? 1 : 0, uniformDir ? 1 : 0, opt_.output == SSTMC_OUTPUT_CORRELATION ? 1 : 0,
                                mview_, sc_.view(), il_.view(), vw_.view(), fr, vb, bs.a.p, bs.b.p, npaths, U,
                                &wsSingle_, st);
     } else if (singleMode_ == kSingleCorrelation) {

@@ -15,11 +15,13 @@
 namespace sstmc {
 
 // sampleFarFieldKernel (FarField.cuh:129-205) / PointSourceNS::sampleFirstPointKernel
-// (PointSource.cuh:72-90): source point and direction of the first flight.
+// (PointSource.cuh:72-90): source point and direction of the first flight. A 2D GaussianBeam draws
+// its first flight as a point source at its focal point does (sampleFirstBeam,
+// GaussianBeam.cuh:754-773, then the medium's sample; the focal points are its locations).
 template <int DIMS>
 __device__ inline void sampleSource(const SourceView& il, const MediumView& m, Rng& rng, f3* x0, f3* dir) {
     uint32_t l = rng.index(il.n);
-    if (il.type == 1) {
+    if (il.type == 1 || il.type == 3) {
         *x0 = mk(il.pf[3 * l], il.pf[3 * l + 1], il.pf[3 * l + 2]);
         *dir = randomDirection<DIMS>(rng);
         return;
@@ -247,8 +249,8 @@ hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc,
     const bool tab = sc.tab != nullptr;
 #define SSTMC_TRANSPORT(D, T, G) \
     hipLaunchKernelGGL((k_transport<D, T, G>), grid, block, 0, st, m, sc, il, fr, vb, seed, pathBegin, npaths, write)
-    if (il.type == 3) {
-        if (dims != 3 || !tab) return hipErrorInvalidValue;
+    if (il.type == 3 && dims == 3) {
+        if (!tab) return hipErrorInvalidValue;
         SSTMC_TRANSPORT(3, true, true);
     } else if (dims == 2) { if (tab) SSTMC_TRANSPORT(2, true, false); else SSTMC_TRANSPORT(2, false, false); }
     else { if (tab) SSTMC_TRANSPORT(3, true, false); else SSTMC_TRANSPORT(3, false, false); }

@@ -141,6 +141,13 @@ def cbs_cross_from_vertices(cfg: dict, dims: int, k, off, pos, mat, rho, xt, ln)
     return (x[..., 0] + 1j * x[..., 1]).transpose(2, 1, 0)
 
 
+def log_i0(z):
+    """The oracle's log I_0(z) for complex z."""
+    out = (C.c_double * 2)()
+    lib().oracle_log_i0(C.c_double(z.real), C.c_double(z.imag), out)
+    return complex(out[0], out[1])
+
+
 def philox(ctr, key):
     c = (C.c_uint32 * 4)(*ctr)
     k = (C.c_uint32 * 2)(*key)
@@ -230,8 +237,8 @@ def frame_table(cfg, dims):
     return dt, neg.value
 
 
-def gauss_f(cfg, view, i, x0, p2):
-    cc = CConfig(cfg, 3)   # keeps the arrays behind the struct's pointers alive
+def gauss_f(cfg, view, i, x0, p2, dims=3):
+    cc = CConfig(cfg, dims)   # keeps the arrays behind the struct's pointers alive
     out = np.zeros(2)
     rc = lib().oracle_gauss_f(cc.ptr(), C.c_int(view), C.c_uint32(i), _p(np.asarray(x0, np.float64)),
                               _p(np.asarray(p2, np.float64)), _p(out))
@@ -240,8 +247,8 @@ def gauss_f(cfg, view, i, x0, p2):
     return out[0] + 1j * out[1]
 
 
-def gauss_fs(cfg, l, v, x0):
-    cc = CConfig(cfg, 3)   # keeps the arrays behind the struct's pointers alive
+def gauss_fs(cfg, l, v, x0, dims=3):
+    cc = CConfig(cfg, dims)   # keeps the arrays behind the struct's pointers alive
     out = np.zeros(2)
     rc = lib().oracle_gauss_fs(cc.ptr(), C.c_uint32(l), C.c_uint32(v), _p(np.asarray(x0, np.float64)),
                                _p(out))

@@ -112,7 +112,7 @@ def test_correlation_sampler_config_checks(native):
 
 
 def test_gaussian_beam_config_checks(native):
-    """GaussianBeam (MatlabInterface.cu:1153-1190): 3D, a direction per focal point, aperture > 0;
+    """GaussianBeam (MatlabInterface.cu:1153-1190): 2D or 3D, a direction per focal point, aperture > 0;
     built for GaussianBeam illumination and view together, vMF mixtures of at most 16 lobes, the
     TemporalSampler without coherent backscattering."""
     cfg = W.confocal(focal_x=[0.0, 2.0], focal_z=[8.0], tau_x=[0.0], batch=64, full_iteration=1)
@@ -129,8 +129,7 @@ def test_gaussian_beam_config_checks(native):
     hg = dict(cfg, scattering={"type": ["HG"] * 5, "amplitudeFunction": [0.8] * 5})
     assert native.sstmc_validate(config.CConfig(hg, 3).ptr()) == 5
     flat = dict(cfg, medium=dict(cfg["medium"], materialGrid=np.array([[1, 2]], np.uint32)))
-    with pytest.raises(ValueError, match="GaussianBeam sources are supported in 3D"):
-        config.CConfig(flat, 2)
+    assert native.sstmc_validate(config.CConfig(flat, 2).ptr()) == 0   # 2D beams (x, y only)
 
 
 def test_config_pointer_keeps_its_arrays(native):
