@@ -142,3 +142,32 @@ def test_transmission_matrix_example(backscattering):
     A, B, S, U = _oracle_u(cfg, 3, d, d["a"], d["b"])
     assert np.linalg.norm(S) > 0 and np.linalg.norm(B) > 0
     assert _rel(d["u"], U) < 1e-9
+
+
+@pytest.mark.parametrize("case", ["moving_amplitude_fixed", "static_ten_frames"])
+def test_shared_direction_fs_frames(case):
+    """One view direction for every beam and field output: the shared-direction fs kernel with
+    frames sharing one fs evaluation (the first vertex does not move with the frame), over three
+    moving frames with isTemporalAmplitude false, and over ten static frames (an 8-frame chunk and a
+    partial one of 2)."""
+    cfg = _small(128)
+    if case == "moving_amplitude_fixed":
+        cfg["sampler"] = {"type": "TemporalSampler", "t": np.array([0.0, 1.0, 2.5]),
+                          "D": np.array([0.0, 1e-3, 0.0, 4e-3, 2e-3]),
+                          "U": {"x": np.array([0.0, 0.02, -0.01, 0.0, 0.01]), "y": np.zeros(5),
+                                "z": np.array([0.0, 0.0, 0.03, 0.01, 0.0])},
+                          "isTemporalAmplitude": False}
+    else:
+        cfg["sampler"] = {"type": "TemporalSampler", "t": np.arange(10.0), "D": np.zeros(5),
+                          "U": {"x": np.zeros(5), "y": np.zeros(5), "z": np.zeros(5)}}
+    cfg["precision"] = {"terms": "fp64", "contraction": "fp64"}
+    with Tracer(cfg, 3) as tr:
+        d = tr.debug_batch(0, 128)
+    A, B, S, U = _oracle_u(cfg, 3, d, d["a"], d["b"])
+    nt = U.shape[2]
+    print("case", case, "frames", nt, "single-scattering share %.3f" % (np.linalg.norm(S.sum(0)) / np.linalg.norm(U)),
+          flush=True)
+    assert np.linalg.norm(S) > 0
+    assert _rel(d["u"], U) < 1e-9
+    for t in range(nt):   # every frame, the partial chunk's included
+        assert _rel(d["u"][:, :, t], U[:, :, t]) < 1e-9, t
