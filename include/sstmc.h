@@ -82,6 +82,8 @@ typedef struct sstmc_simulation {
     uint32_t deviceNumber;     /* HIP device ordinal */
     uint64_t seed;             /* config.simulation.seed (MatlabInterface.cu:1649) */
     double lambda;             /* default 1.0; one wavelength (Nw = 1) */
+    uint32_t graphs;           /* nonzero: each batch's contraction stage is captured once as a HIP
+                                  graph and replayed for the later batches (same results) */
 } sstmc_simulation;
 
 /* config.medium, heterogeneous rectilinear grid (HeterogeneousMedium.cuh:107-233). */
@@ -162,12 +164,14 @@ typedef struct sstmc_tracer {
 typedef struct sstmc_precision {
     int32_t terms;                 /* SSTMC_TERMS_FP32 (default) or SSTMC_TERMS_FP64 */
     int32_t contraction;           /* SSTMC_CONTRACT_BF16X3 (default), _FP32 or _FP64 */
-    /* Error-budgeted attenuation cut: an NEE term (g of one illumination or view at one vertex)
-     * whose optical depth to the source exceeds tauCut is dropped, and its voxel walk stops
-     * there; the dropped amplitude is below exp(-tauCut / 2) times the term's unattenuated one.
-     * 0 (default) is exact: walks run to the box (FP32 walks stop at 210, past which exp(-tau/2)
-     * is exactly 0 in FP32). The reference walks every term to the box
-     * (HeterogeneousMedium.cuh:386-573). */
+    /* Error-budgeted, batch-relative attenuation cut: per batch, tau_ref is the least optical
+     * depth from a sample of the batch's vertices to the views; a view-side term that carries no
+     * path weight (b, B: g of one view at one vertex) whose optical depth exceeds tau_ref + tauCut
+     * is dropped and its voxel walk stops there, so every dropped term is below exp(-tauCut / 2)
+     * times the batch's least attenuated view term. Terms that carry the path weight (a, A, Ar),
+     * the reversed paths' Br and the path probabilities stay exact. 0 (default) is exact: walks
+     * run to the box (FP32 walks stop at 210, past which exp(-tau/2) is exactly 0 in FP32). The
+     * reference walks every term to the box (HeterogeneousMedium.cuh:386-573). */
     double tauCut;
 } sstmc_precision;
 
@@ -239,6 +243,8 @@ typedef struct sstmc_timing {
     double nee_view_ms;            /* part of nee_ms: multiple-scattering view terms (the voxel-walk
                                       kernel that dominates nee_ms) */
     uint64_t dda_steps_view;       /* part of dda_steps walked by that kernel */
+    uint32_t graph_replays;        /* batches whose contraction stage replayed captured graphs
+                                      (simulation.graphs) */
 } sstmc_timing;
 int sstmc_last_timing(const sstmc_handle* h, sstmc_timing* out);
 

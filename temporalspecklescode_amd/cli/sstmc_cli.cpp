@@ -11,7 +11,7 @@
 // Exit status: 0, or the sstmc error code (config errors found while reading the JSON: 7).
 //
 // JSON layout (MATLAB struct field names):
-//   simulation: batchSize, fullIteration, renderingsNum, deviceNumber, seed, lambda
+//   simulation: batchSize, fullIteration, renderingsNum, deviceNumber, seed, lambda, graphs
 //   medium: type "Heterogeneous", xAxis, yAxis[, zAxis], materialGrid (flat, x fastest, or nested
 //           [x][y][z]) or materialGridFile (raw bytes, x fastest; materialGridType "uint8"|"uint32"),
 //           sigs, siga
@@ -168,6 +168,7 @@ void fill(HostConfig& h, const Value& cfg, uint32_t dims) {
     if (sim.has("lambda") && nums(sim.at("lambda")).size() != 1)
         throw ConfigError("only one wavelength is supported (Nw = 1)");
     s.simulation.lambda = numOr(sim, "lambda", 1.0);
+    s.simulation.graphs = numOr(sim, "graphs", 0) != 0 ? 1u : 0u;
 
     // scattering (MatlabInterface.cu:712-888)
     const Value& sc = cfg.at("scattering");

@@ -42,7 +42,7 @@ _I32P = C.POINTER(C.c_int32)
 class Simulation(C.Structure):
     _fields_ = [("batchSize", C.c_uint32), ("fullIteration", C.c_uint32),
                 ("renderingsNum", C.c_uint32), ("deviceNumber", C.c_uint32),
-                ("seed", C.c_uint64), ("lambda_", C.c_double)]
+                ("seed", C.c_uint64), ("lambda_", C.c_double), ("graphs", C.c_uint32)]
 
 
 class Medium(C.Structure):
@@ -130,6 +130,7 @@ class CConfig:
         if lam.size != 1:
             raise ValueError("only one wavelength is supported (Nw = 1)")
         s.simulation.lambda_ = float(lam[0])
+        s.simulation.graphs = 1 if sim.get("graphs", False) else 0
 
         # scattering (MatlabInterface.cu:712-888)
         sc = cfg["scattering"]

@@ -259,6 +259,7 @@ static int create(const sstmc_config* c, sstmc_handle** out, std::shared_ptr<Med
     opt.terms = c->precision.terms;
     opt.contraction = c->precision.contraction;
     opt.tauCut = c->precision.tauCut;
+    opt.graphs = c->simulation.graphs ? 1 : 0;
     h->tracer.reset(new NextEventEstimation(&err, *h->medium, *h->scattering, *h->illumination, *h->view,
                                             *h->sampler, opt));
     if (err) return err;

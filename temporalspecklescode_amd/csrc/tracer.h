@@ -222,6 +222,7 @@ struct TracerOptions {
     int terms;         // SSTMC_TERMS_*
     int contraction;   // SSTMC_CONTRACT_*
     double tauCut;     // precision.tauCut (0: exact)
+    int graphs = 0;    // replay the contraction stage from captured HIP graphs
 };
 
 class Tracer {
@@ -328,6 +329,21 @@ private:
     bool sameViews_ = false;        // correlation sampler: view2 holds the same sources as view
     bool sameIllums_ = false;       // correlation sampler: illumination2 holds the same sources
     bool cbsCorr_ = false;          // coherent backscattering under the correlation sampler
+    // the contraction stage captured as HIP graphs, per (set, batch size, destination): the
+    // multiple-scattering part and the single-scattering part (timed apart); dropped whenever a
+    // buffer they name is reallocated
+    struct StageGraph {
+        int set;
+        uint32_t npaths;
+        double* U;
+        hipGraphExec_t multi = nullptr, single = nullptr;
+        void *wsMulti = nullptr, *wsSingle = nullptr;
+    };
+    std::vector<StageGraph> graphs_;
+    uint32_t graphReplays_ = 0;   // of the current rendering
+    void dropGraphs();
+    int contractStage(uint32_t npaths, double* U, int set, hipStream_t sC, hipEvent_t mid);
+    int contractMulti(uint32_t npaths, double* U, int set, hipStream_t sC);
     DevBuf<double2> cbsPart_;       // its mixed pairings' split slices
     bool fp64_;
     int contractKind_;

@@ -840,6 +840,7 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
 
 NextEventEstimation::~NextEventEstimation() {
     syncAll();
+    dropGraphs();
     for (hipEvent_t e : ev_) hipEventDestroy(e);
     for (BatchSet& bs : sets_)
         for (hipEvent_t e : {bs.transported, bs.traced, bs.released})
@@ -895,6 +896,7 @@ VertexBatch NextEventEstimation::vertexBatch(int set) const {
 // Both sets' vertex pools hold `cap` vertices; the streams must be idle.
 int NextEventEstimation::growPool(uint64_t cap) {
     int e;
+    dropGraphs();   // they name the pools being replaced
     // vertex records per frame; the correlation sampler's vertices do not move: one record each
     const uint64_t recT = sampler_.correlation() ? 1 : T_;
     for (BatchSet& bs : sets_) {
@@ -1017,6 +1019,17 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     // then single scattering
     if (pipelined && hipStreamWaitEvent(sC, bs.traced, 0) != hipSuccess) return SSTMC_DEVICE_ERROR;
     if (e) hipEventRecord(e[7], sC);
+    int rc = contractStage(npaths, U, set, sC, e ? e[3] : nullptr);
+    if (rc) return rc;
+    if (e) hipEventRecord(e[4], sC);
+    if (hipEventRecord(bs.released, sC) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    return SSTMC_NO_ERROR;
+}
+
+// The multiple-scattering contraction of a batch: U += msScale (sum_p A^T B [+ sum_p Br^T Ar]),
+// and under the correlation sampler with coherent backscattering the mixed pairings.
+int NextEventEstimation::contractMulti(uint32_t npaths, double* U, int set, hipStream_t sC) {
+    const BatchSet& bs = sets_[set];
     const double msScale = opt_.cbs ? (cbsCorr_ ? 0.5 : 0.70710678118654752440) : 1.0;
     if (launchContract(contractKind_, bs.A.p, bs.B.p, npaths, L_, V_, T_, nullptr, nullptr, 0, msScale, U, &wsMulti_,
                        sC) != hipSuccess)
@@ -1026,15 +1039,82 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
         return SSTMC_KERNEL_ERROR;
     if (cbsCorr_) {
         const CbsCross cc{bs.F1l.p, bs.F2l.p, bs.F1v.p, bs.F2v.p, bs.Gv1.p, bs.Gv2.p, bs.Gl1.p, bs.Gl2.p, cap_};
-        if (launchCbsCross(dims_, fp64_ ? 1 : 0, il_.view(), vw_.view(), fr, vb, cc, npaths, opt_.lambda, U,
-                           cbsPart_.p, cbsCrossSplits(L_, V_, T_, P_), sC) != hipSuccess)
+        if (launchCbsCross(dims_, fp64_ ? 1 : 0, il_.view(), vw_.view(), sampler_.view(), vertexBatch(set), cc, npaths,
+                           opt_.lambda, U, cbsPart_.p, cbsCrossSplits(L_, V_, T_, P_), sC) != hipSuccess)
             return SSTMC_KERNEL_ERROR;
     }
-    if (e) hipEventRecord(e[3], sC);
-    int rc = contractSingle(npaths, U, set, sC);
+    return SSTMC_NO_ERROR;
+}
+
+void NextEventEstimation::dropGraphs() {
+    for (StageGraph& g : graphs_) {
+        if (g.multi) hipGraphExecDestroy(g.multi);
+        if (g.single) hipGraphExecDestroy(g.single);
+    }
+    graphs_.clear();
+}
+
+// Captures the launches `run` makes on st (no allocation, copy or synchronisation among them) as
+// an executable graph; null when capture or instantiation fails.
+template <typename F>
+static hipGraphExec_t captureStage(hipStream_t st, F run) {
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    const int rc = run();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc || e != hipSuccess || !graph) {
+        if (graph) hipGraphDestroy(graph);
+        hipGetLastError();
+        return nullptr;
+    }
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) exec = nullptr;
+    hipGraphDestroy(graph);
+    hipGetLastError();
+    return exec;
+}
+
+// The contraction stage of one batch on sC: multiple scattering, `mid` (if set) recorded, then
+// single scattering. With opt.graphs the first batch of each (set, batch size, destination) runs
+// eagerly, which sizes the workspaces, and is then captured; later batches of that key replay
+// the two graphs: the same kernels with the same arguments in the same order, hence the same
+// bits. A reallocated workspace or vertex pool drops the graphs.
+int NextEventEstimation::contractStage(uint32_t npaths, double* U, int set, hipStream_t sC, hipEvent_t mid) {
+    if (opt_.graphs) {
+        for (size_t i = 0; i < graphs_.size(); ++i) {
+            StageGraph& g = graphs_[i];
+            if (g.set != set || g.npaths != npaths || g.U != U) continue;
+            if (g.wsMulti != wsMulti_.ptr || g.wsSingle != wsSingle_.ptr) {
+                dropGraphs();
+                break;
+            }
+            if (hipGraphLaunch(g.multi, sC) != hipSuccess) return SSTMC_KERNEL_ERROR;
+            if (mid) hipEventRecord(mid, sC);
+            if (hipGraphLaunch(g.single, sC) != hipSuccess) return SSTMC_KERNEL_ERROR;
+            ++graphReplays_;
+            return SSTMC_NO_ERROR;
+        }
+    }
+    int rc = contractMulti(npaths, U, set, sC);
     if (rc) return rc;
-    if (e) hipEventRecord(e[4], sC);
-    if (hipEventRecord(bs.released, sC) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    if (mid) hipEventRecord(mid, sC);
+    if ((rc = contractSingle(npaths, U, set, sC))) return rc;
+    if (opt_.graphs) {
+        StageGraph g;
+        g.set = set;
+        g.npaths = npaths;
+        g.U = U;
+        g.multi = captureStage(sC, [&] { return contractMulti(npaths, U, set, sC); });
+        g.single = g.multi ? captureStage(sC, [&] { return contractSingle(npaths, U, set, sC); }) : nullptr;
+        if (g.multi && g.single) {
+            g.wsMulti = wsMulti_.ptr;
+            g.wsSingle = wsSingle_.ptr;
+            graphs_.push_back(g);
+        } else {
+            if (g.multi) hipGraphExecDestroy(g.multi);
+            opt_.graphs = 0;   // capture unavailable here: stay eager
+        }
+    }
     return SSTMC_NO_ERROR;
 }
 
@@ -1195,6 +1275,7 @@ int NextEventEstimation::sizePool() {
 int NextEventEstimation::begin() {
     int rc = sizePool();
     if (rc) return rc;
+    graphReplays_ = 0;
     uint32_t nb = opt_.fullIteration;
     if (evBatches_ < nb) {
         for (hipEvent_t ev : ev_) hipEventDestroy(ev);
@@ -1290,6 +1371,7 @@ int NextEventEstimation::finish(double* u, bool deviceDst) {
     t.dda_steps = tot[2] + tot[4];
     t.dda_steps_view = tot[4];
     t.batches = opt_.fullIteration;
+    t.graph_replays = graphReplays_;
     last_ = t;
     return SSTMC_NO_ERROR;
 }
