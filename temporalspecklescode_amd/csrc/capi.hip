@@ -68,8 +68,9 @@ static int validate(const sstmc_config* c, bool checkGrid = true) {
     int dims = (int)c->dims;
     if (dims != 2 && dims != 3) return SSTMC_INVALID_CONFIG;
     if (c->tracer.type != SSTMC_TRACER_NEE && c->tracer.type != 0) return SSTMC_INVALID_CONFIG;
-    // coherent backscattering is implemented for the field output (NextEventEstimation.cuh:
-    // 1212-1235, 1385-1391, 1432-1454); its correlation form (CBS02-CBS04, :1237-1383) is not
+    // coherent backscattering: the field output (NextEventEstimation.cuh:1212-1235, 1385-1391,
+    // 1432-1454) and the TemporalCorrelationSampler's correlation (CBS01-CBS04, :1138-1391) are
+    // built; the TemporalSampler's per-sub-path correlation output has no reversed-path form
     if (c->tracer.isCBS && c->tracer.output == SSTMC_OUTPUT_CORRELATION) return SSTMC_NOT_SUPPORTED;
     if (c->tracer.output != SSTMC_OUTPUT_FIELD && c->tracer.output != SSTMC_OUTPUT_CORRELATION)
         return SSTMC_INVALID_CONFIG;
