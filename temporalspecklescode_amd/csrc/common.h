@@ -269,6 +269,27 @@ struct __align__(32) VertexRec {
     double phase;
 };
 
+// Per (vertex k >= 2, frame t), for the far-field view kernel on equally spaced grids: everything
+// of a term that depends on the vertex alone, so the kernel's per-(vertex, source) work is what
+// depends on both. The walk's start: per axis the cell index b (as an exact float) and the offsets
+// of the cell's low and high faces from the position (FP32 face minus FP32 position, as the walk
+// forms them), the walk-grid bit offset of the cell (kFarOutside for a position outside the box,
+// whose optical depth is 0); the material; the back direction rounded to FP32; the FP64 position
+// and phase base of VertexRec.
+constexpr uint32_t kFarOutside = 0xffffffffu;
+struct __align__(32) FarRec {
+    float dlo[3];
+    uint32_t cell;
+    float dhi[3];
+    uint32_t mat;
+    float b[3];
+    float back[3];
+    float pad[2];
+    double px, py, pz;
+    double phase;
+};
+static_assert(sizeof(FarRec) == 96, "FarRec is three 32-byte lines");
+
 struct VertexBatch {
     uint32_t* k;                   // [P] scattering vertices per path
     uint32_t* misses;              // [P]
@@ -279,6 +300,7 @@ struct VertexBatch {
     uint64_t cap;
     uint32_t* status;              // [4]: 0 overflow flag, 1 no-hit flag
     VertexRec* rec;                // [cap * nt], record of vertex o, frame t at o * nt + t
+    FarRec* far = nullptr;         // [cap * nt] like rec, when the far-field view kernel runs
 };
 
 }  // namespace sstmc

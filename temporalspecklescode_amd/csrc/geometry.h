@@ -376,31 +376,76 @@ __host__ __device__ inline float opticalDepthFast(const MediumView& m, const flo
     return opticalDepthFastDir<DIMS>(m, sig, p, walkDir<DIMS>(m, d), tauCut, steps);
 }
 
+// The walk's start at p: per axis the cell index b, the offsets of the cell's low and high faces
+// from p (FP32), and the walk-grid bit offset of the cell. Per vertex, independent of the
+// direction; the far-field view kernel reads it from a per-vertex record (FarRec).
+struct WalkStart {
+    float dlo[3], dhi[3], b[3];
+    uint32_t cell;
+};
+template <int DIMS>
+__host__ __device__ inline WalkStart walkStart(const MediumView& m, const f3& p) {
+    const float pp[3] = {p.x, p.y, DIMS == 3 ? p.z : 0.f};
+    WalkStart s;
+    s.cell = m.walkOrigin;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (a >= DIMS) {
+            s.dlo[a] = s.dhi[a] = s.b[a] = 0.f;
+            continue;
+        }
+        const int b = locateUniform<float>(m, a, pp[a]);   // the fast walk runs on equally spaced grids
+        // both faces of the cell (edges hold n + 1 entries): when p is shared by a wavefront's
+        // walks these are two scalar loads rather than a gather
+        s.dlo[a] = m.ef[a][b] - pp[a];
+        s.dhi[a] = m.ef[a][b + 1] - pp[a];
+        s.b[a] = (float)b;
+        s.cell += (uint32_t)b * m.walkStride[a];
+    }
+    return s;
+}
+
+template <int DIMS>
+__host__ __device__ inline float walkFastCore(const MediumView& m, const float* sig, const float* tf,
+                                              const float* td, const uint32_t* stride, uint32_t cell, float tExit,
+                                              float tauCut, uint32_t* steps);
+
+// The walk from a start (walkStart) in the direction part wd (walkDir). Per axis the first face
+// crossing ahead is (face ahead - p) / d, and the box is left at the crossing of the face
+// n - 1 - b (up) or b (down) faces further on.
+template <int DIMS>
+__host__ __device__ inline float walkFromStart(const MediumView& m, const float* sig, const float* dlo,
+                                               const float* dhi, const float* bf, uint32_t cell, const WalkDir& wd,
+                                               float tauCut, uint32_t* steps) {
+    float tf[3] = {3.0e38f, 3.0e38f, 3.0e38f}, td[3] = {0.f, 0.f, 0.f};
+    uint32_t stride[3] = {0u, 0u, 0u};   // walk-grid bit strides in two's complement
+    float tExit = 3.0e38f;
+#pragma unroll
+    for (int a = 0; a < DIMS; ++a) {
+        const int up = wd.up[a];
+        tf[a] = (up ? dhi[a] : dlo[a]) * wd.inv[a];
+        td[a] = wd.td[a];
+        // faces crossed before the one that leaves the box along this axis (exact integers)
+        const float ahead = up ? (float)(m.n[a] - 1) - bf[a] : bf[a];
+        tExit = fminf(tExit, fmaf(ahead, td[a], tf[a]));
+        stride[a] = wd.stride[a];
+    }
+    return walkFastCore<DIMS>(m, sig, tf, td, stride, cell, tExit, tauCut, steps);
+}
+
 // The walk from p with the direction part wd already formed (walkDir).
 template <int DIMS>
 __host__ __device__ inline float opticalDepthFastDir(const MediumView& m, const float* sig, const f3& p,
                                                      const WalkDir& wd, float tauCut, uint32_t* steps) {
     if (!inside<DIMS, float>(m, p)) return 0.f;
-    const float pp[3] = {p.x, p.y, DIMS == 3 ? p.z : 0.f};
-    float tf[3] = {3.0e38f, 3.0e38f, 3.0e38f}, td[3] = {0.f, 0.f, 0.f};
-    uint32_t stride[3] = {0u, 0u, 0u};   // walk-grid bit strides in two's complement
-    uint32_t cell = m.walkOrigin;        // bit offset of the current cell
-    float tExit = 3.0e38f;
-#pragma unroll
-    for (int a = 0; a < DIMS; ++a) {
-        const int n = (int)m.n[a];
-        const int b = locateUniform<float>(m, a, pp[a]);   // the fast walk runs on equally spaced grids
-        const int up = wd.up[a];
-        // both faces of the cell (edges hold n + 1 entries), then the one ahead: when p is shared
-        // by a wavefront's walks these are two scalar loads rather than a gather
-        const float elo = m.ef[a][b], ehi = m.ef[a][b + 1];
-        tf[a] = ((up ? ehi : elo) - pp[a]) * wd.inv[a];
-        td[a] = wd.td[a];
-        // faces crossed before the one that leaves the box along this axis
-        tExit = fminf(tExit, fmaf((float)(up ? n - 1 - b : b), td[a], tf[a]));
-        stride[a] = wd.stride[a];
-        cell += (uint32_t)b * m.walkStride[a];
-    }
+    const WalkStart s = walkStart<DIMS>(m, p);
+    return walkFromStart<DIMS>(m, sig, s.dlo, s.dhi, s.b, s.cell, wd, tauCut, steps);
+}
+
+template <int DIMS>
+__host__ __device__ inline float walkFastCore(const MediumView& m, const float* sig, const float* tf,
+                                              const float* td, const uint32_t* stride, uint32_t cell, float tExit,
+                                              float tauCut, uint32_t* steps) {
     const WalkWords words(m);
     const uint32_t bits = m.walkBits;
     const char* sigb = (const char*)sig;

@@ -674,9 +674,12 @@ __global__ void __launch_bounds__(256) k_nee_corr(MediumView m, ScatterView sc, 
 // view-side HG amplitude is evaluated against, FarField.cuh:546-599 / PointSource.cuh:347-446)
 // and the phase base rho_k + L_k^t / lambda with L_k^t the FP64 path length from x_1^t
 // (pathContriobutionKernel, TemporalSampler.cuh:668-719). Done once per (vertex, frame) here
-// instead of once per (vertex, view, frame) in k_nee_view.
+// instead of once per (vertex, view, frame) in k_nee_view. With vb.far (3D) also the far-field
+// view kernel's record: the walk's start from x_k^t (FP32), the material, the back direction in
+// FP32 and the FP64 position.
 template <int DIMS>
-__global__ void __launch_bounds__(256) k_vertex_prep(VertexBatch vb, FramesView fr, uint32_t npaths, double invLambda) {
+__global__ void __launch_bounds__(256) k_vertex_prep(MediumView m, VertexBatch vb, FramesView fr, uint32_t npaths,
+                                                     double invLambda) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npaths) return;
     if ((uint64_t)vb.off[npaths] > vb.cap) return;   // pool overflow: the batch is re-run
@@ -700,6 +703,31 @@ __global__ void __launch_bounds__(256) k_vertex_prep(VertexBatch vb, FramesView 
             r.bx = b.x; r.by = b.y; r.bz = b.z;
             r.phase = (double)vb.rho[ok] + len * invLambda;
             vb.rec[ok * NT + t] = r;
+            if (DIMS == 3 && vb.far) {
+                FarRec f;
+                if (inside<DIMS, float>(m, xk.pf)) {
+                    const WalkStart w = walkStart<DIMS>(m, xk.pf);
+                    for (int a = 0; a < 3; ++a) {
+                        f.dlo[a] = w.dlo[a];
+                        f.dhi[a] = w.dhi[a];
+                        f.b[a] = w.b[a];
+                    }
+                    f.cell = w.cell;
+                } else {
+                    for (int a = 0; a < 3; ++a) f.dlo[a] = f.dhi[a] = f.b[a] = 0.f;
+                    f.cell = kFarOutside;
+                }
+                f.mat = __float_as_uint(xkm.w);
+                f.back[0] = (float)b.x;
+                f.back[1] = (float)b.y;
+                f.back[2] = (float)b.z;
+                f.pad[0] = f.pad[1] = 0.f;
+                f.px = xk.pd.x;
+                f.py = xk.pd.y;
+                f.pz = xk.pd.z;
+                f.phase = r.phase;
+                vb.far[ok * NT + t] = f;
+            }
             xpm = xkm;
             xp = xk;
         }
@@ -853,7 +881,9 @@ struct HgConst {
 // with the per-(vertex, source) work cut to what depends on both:
 //   * the walk's direction part (inverse direction, face step, bit stride), the FP64 phase
 //     direction d / lambda and the HG incoming direction are formed once per lane, before the
-//     vertex loop; the vertex (wave-uniform) part of the walk's set-up is one locate per axis;
+//     vertex loop; the vertex part (the walk's start cell and face offsets, the material, the back
+//     direction, the FP64 position and phase base) is one wave-uniform record (FarRec, written by
+//     k_vertex_prep), so a vertex costs the lanes two selects and two multiply-adds per axis;
 //   * 1 - cos of the HG amplitude is 0.5 |a - b|^2 in FP32, which keeps its relative accuracy in
 //     the forward peak where 1 + g^2 - 2 g c cancels; |f| = C rsqrt(den) sqrt(rsqrt(den));
 //   * exp(i 2 pi phase) is the hardware sine and cosine (argument in turns) of the FP64-reduced
@@ -895,19 +925,19 @@ k_nee_view_far(MediumView m, ScatterView sc, SourceView vw, FramesView fr, Verte
         const f3 av = mk(as * (float)vd.x, as * (float)vd.y, as * (float)vd.z);
         float accRe = 0.f, accIm = 0.f;
         for (uint32_t k = 1; k < K; ++k) {
-            const uint64_t ok = off + k;
-            const float4 xkm = vb.xm[ok];
-            const Geo xk = vertexFrame(vb, fr, ok, t, xkm);
-            const float tau = opticalDepthFastDir<3>(m, sig + kMaxMaterials, xk.pf, wd, tauCut, &steps);
+            const FarRec rec = vb.far[(off + k) * fr.nt + t];
+            const float tau = rec.cell == kFarOutside
+                                  ? 0.f
+                                  : walkFromStart<3>(m, sig + kMaxMaterials, rec.dlo, rec.dhi, rec.b, rec.cell, wd,
+                                                     tauCut, &steps);
             const float amp = tau > tauCut ? 0.f : __expf(-0.5f * tau);
             if (__ballot(amp != 0.f) == 0) continue;   // every term of this vertex is zero
-            const VertexRec rec = vb.rec[ok * fr.nt + t];
-            const HgConst h = hg[__float_as_uint(xkm.w)];
-            const float dx = av.x - (float)rec.bx, dy = av.y - (float)rec.by, dz = av.z - (float)rec.bz;
+            const HgConst h = hg[rec.mat];
+            const float dx = av.x - rec.back[0], dy = av.y - rec.back[1], dz = av.z - rec.back[2];
             const float den = fmaf(h.b, 0.5f * fmaf(dx, dx, fmaf(dy, dy, dz * dz)), h.a);
             const float r = __builtin_amdgcn_rsqf(den);
             const float fv = h.c * r * __builtin_amdgcn_sqrtf(r);
-            const double ph = kd.x * xk.pd.x + kd.y * xk.pd.y + kd.z * xk.pd.z + rec.phase;
+            const double ph = kd.x * rec.px + kd.y * rec.py + kd.z * rec.pz + rec.phase;
             const float turns = (float)(ph - floor(ph));
             const float a = amp * fv;
             accRe = fmaf(a, __builtin_amdgcn_cosf(turns), accRe);
@@ -1377,7 +1407,8 @@ static void launchView(bool corr, const MediumView& m, const ScatterView& sc, co
         else SSTMC_VIEW(k_nee_view, false, true);
     } else if constexpr (DIMS == 3 && sizeof(R) == 4 && (SRC == kSrcFarView || SRC == kSrcFarIllum)) {
         // far-field sources, FP32 terms: the specialised kernel when its preconditions hold
-        if (wave && !tab && m.walkFast) hipLaunchKernelGGL((k_nee_view_far<SRC>), grid, block, 0, st, m, sc, src, fr, vb, q, out);
+        if (wave && !tab && m.walkFast && vb.far)
+            hipLaunchKernelGGL((k_nee_view_far<SRC>), grid, block, 0, st, m, sc, src, fr, vb, q, out);
         else if (wave) { if (tab) SSTMC_VIEW(k_nee_view, true, true); else SSTMC_VIEW(k_nee_view, true, false); }
         else { if (tab) SSTMC_VIEW(k_nee_view, false, true); else SSTMC_VIEW(k_nee_view, false, false); }
     } else if (wave) {
@@ -1417,7 +1448,7 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
         else hipLaunchKernelGGL((k_nee_corr<DIMS, R, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
         FramesView one = fr;
         one.nt = 1;
-        hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, vb, one, p.npaths,
+        hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, m, vb, one, p.npaths,
                            1.0 / p.lambda);
         if (between) hipEventRecord(between, st);
         if (vw.type == 2) launchViewCorrSampler<DIMS, R, kSrcFarView>(m, sc, vw, fr, vb, p, out, st);
@@ -1442,7 +1473,7 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
         hipLaunchKernelGGL((k_nee<DIMS, R, true, true>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
     else if (sc.tab) hipLaunchKernelGGL((k_nee<DIMS, R, true, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
     else hipLaunchKernelGGL((k_nee<DIMS, R, false, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
-    hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, vb, fr, p.npaths,
+    hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, m, vb, fr, p.npaths,
                        1.0 / p.lambda);
     if (between) hipEventRecord(between, st);
     const bool corr = p.output == 1;

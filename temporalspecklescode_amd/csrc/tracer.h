@@ -291,6 +291,7 @@ private:
         DevBuf<float4> xm;
         DevBuf<float> rho, delta;
         DevBuf<VertexRec> rec;
+        DevBuf<FarRec> far;                   // the far-field view kernel's records (farRecs_)
         DevBuf<float> tauRef;                 // the batch's least view optical depth (relative cut)
         DevBuf<unsigned char> A, B, a, b;
         DevBuf<unsigned char> Ar, Br;         // coherent backscattering: reversed-path terms
@@ -329,6 +330,7 @@ private:
     bool sameViews_ = false;        // correlation sampler: view2 holds the same sources as view
     bool sameIllums_ = false;       // correlation sampler: illumination2 holds the same sources
     bool cbsCorr_ = false;          // coherent backscattering under the correlation sampler
+    bool farRecs_ = false;          // the far-field view kernel runs: FarRec per (vertex, frame)
     // the contraction stage captured as HIP graphs, per (set, batch size, destination): the
     // multiple-scattering part and the single-scattering part (timed apart); dropped whenever a
     // buffer they name is reallocated

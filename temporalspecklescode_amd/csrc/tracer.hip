@@ -48,6 +48,7 @@ template struct DevBuf<double2>;
 template struct DevBuf<PhaseTables>;
 template struct DevBuf<unsigned long long>;
 template struct DevBuf<VertexRec>;
+template struct DevBuf<FarRec>;
 
 // ====================================================================== HeterogeneousMedium
 HeterogeneousMedium::HeterogeneousMedium(int* err, int dims, const uint32_t* grid, const double* xAxis,
@@ -818,6 +819,13 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
             return;
         }
     if ((*err = buildWalkGrid())) return;
+    // the far-field view kernel (k_nee_view_far): 3D, FP32 terms, HG or isotropic materials,
+    // equally spaced grid, field output of the TemporalSampler, far-field views (or far-field
+    // illuminations for the reversed paths) of more than 32 sources, one wavefront each
+    farRecs_ = dims_ == 3 && !fp64_ && !acorr && opt_.output == SSTMC_OUTPUT_FIELD && !sc_.view().tab &&
+               mview_.walkFast && singleMode_ != kSingleGauss &&
+               ((vw_.type() == SSTMC_SOURCE_FARFIELD && V_ > 32) ||
+                (opt.cbs && il_.type() == SSTMC_SOURCE_FARFIELD && L_ > 32));
     if ((*err = allocate())) return;
     if (needF) {
         std::vector<float2> ff((size_t)L_ * V_);
@@ -890,7 +898,8 @@ int NextEventEstimation::allocate() {
 
 VertexBatch NextEventEstimation::vertexBatch(int set) const {
     const BatchSet& bs = sets_[set];
-    return VertexBatch{bs.k.p, bs.misses.p, bs.off.p, bs.xm.p, bs.rho.p, bs.delta.p, cap_, status_.p, bs.rec.p};
+    return VertexBatch{bs.k.p, bs.misses.p, bs.off.p, bs.xm.p, bs.rho.p, bs.delta.p, cap_, status_.p, bs.rec.p,
+                       farRecs_ ? bs.far.p : nullptr};
 }
 
 // Both sets' vertex pools hold `cap` vertices; the streams must be idle.
@@ -901,13 +910,14 @@ int NextEventEstimation::growPool(uint64_t cap) {
     const uint64_t recT = sampler_.correlation() ? 1 : T_;
     for (BatchSet& bs : sets_) {
         if ((e = bs.xm.alloc(cap)) || (e = bs.rho.alloc(cap)) || (e = bs.rec.alloc(cap * recT))) return e;
+        if (farRecs_ && (e = bs.far.alloc(cap * recT))) return e;
         if (!sampler_.view().trivial) {
             if ((e = bs.delta.alloc(cap * T_ * 3))) return e;
         } else {
             bs.delta.release();
         }
     }
-    uint64_t perVertex = 20 + 32 * recT + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_);
+    uint64_t perVertex = 20 + (farRecs_ ? 128 : 32) * recT + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_);
     if (cbsCorr_) {
         // per-vertex u1 / u2 terms of every illumination and view, per lag
         const size_t elem = fp64_ ? 16 : 8;
@@ -1124,7 +1134,8 @@ uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint
     // two batch sets (the pipeline's): terms, per-path arrays and vertex pools
     uint64_t terms = 2 * (cbs ? 3ull : 2ull) * T * P * (uint64_t)(L + V) * elem;
     uint64_t field = 16ull * T * L * V;
-    uint64_t pool = 2 * (uint64_t)P * 32 * (20 + 32ull * T + (frames ? 12ull * T : 0));
+    // vertex records: 32 bytes, and 96 more for the far-field view kernel's (FP32 terms)
+    uint64_t pool = 2 * (uint64_t)P * 32 * (20 + (fp64 ? 32ull : 128ull) * T + (frames ? 12ull * T : 0));
     uint64_t ws = 2ull * 8 * T * L * V * elem;   // split-K partials, two workspaces
     if (!fp64) ws += 2ull * 8 * ((uint64_t)P + 64) * (L + V + 256);   // and their bf16 hi/lo planes
     uint64_t grids = 2 * cells;                  // material grid + packed walk grid (<= 4 bits a cell)
