@@ -276,6 +276,8 @@ struct __align__(32) VertexRec {
 // forms them), the walk-grid bit offset of the cell (kFarOutside for a position outside the box,
 // whose optical depth is 0); the material; the back direction rounded to FP32; the FP64 position
 // and phase base of VertexRec.
+// lb: lower bounds on the optical depth of the walks from the vertex along every direction of the
+// views (lb[0]) and of the reversed paths' illuminations (lb[1]), 0 when none applies (k_far_bound).
 constexpr uint32_t kFarOutside = 0xffffffffu;
 struct __align__(32) FarRec {
     float dlo[3];
@@ -284,11 +286,38 @@ struct __align__(32) FarRec {
     uint32_t mat;
     float b[3];
     float back[3];
-    float pad[2];
+    float lb[2];
     double px, py, pz;
     double phase;
 };
 static_assert(sizeof(FarRec) == 96, "FarRec is three 32-byte lines");
+
+// Per-layer least sigt over aligned square blocks of cells, for lower bounds on optical depths
+// (k_far_bound). Layers run along `axis`; each layer is a 2D grid over the two other axes (b, c in
+// increasing order) reduced 2 x 2 per level, up to one block at the top level. Level L of layer l
+// holds block (ib >> L, ic >> L) at p[off[L] + (l * nc[L] + (ic >> L)) * nb[L] + (ib >> L)]; FP32
+// sigt as the walks use it.
+constexpr int kPyrLevels = 16;
+struct MinPyramid {
+    const float* p = nullptr;
+    int axis = -1;                 // -1: no pyramid (no bound)
+    uint32_t nl = 0, levels = 0;
+    uint64_t off[kPyrLevels] = {};
+    uint32_t nb[kPyrLevels] = {}, nc[kPyrLevels] = {};
+};
+// A set of walk directions seen from the axis they all advance along: sign of that component and
+// the range of each lateral component over |axis component| (the two other axes in order).
+struct BoundDirs {
+    int valid = 0;
+    int sgn = 1;
+    float rmin[2] = {0.f, 0.f}, rmax[2] = {0.f, 0.f};
+};
+// The bounds of the far-field view kernel's two launches: views (0) and, for coherent
+// backscattering, the reversed paths' illuminations (1).
+struct FarBound {
+    MinPyramid pyr[2];
+    BoundDirs dirs[2];
+};
 
 struct VertexBatch {
     uint32_t* k;                   // [P] scattering vertices per path

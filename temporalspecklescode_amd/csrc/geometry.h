@@ -2,6 +2,7 @@
 // Device functions templated on the real type R (float for transport and the FP32 NEE
 // variant, double for the FP64 NEE variant) and on DIMS (2 or 3).
 #pragma once
+#include <vector>
 
 #include "common.h"
 
@@ -433,6 +434,46 @@ __host__ __device__ inline float walkFromStart(const MediumView& m, const float*
     return walkFastCore<DIMS>(m, sig, tf, td, stride, cell, tExit, tauCut, steps);
 }
 
+// walkFromStart for a lane that walks one direction from many starts: the direction's selects
+// folded into per-axis factors once, so a start costs two multiply-adds and one multiply per axis.
+// Bit-identical to walkFromStart: with up, fma(dhi, inv, dlo * 0) rounds dhi * inv once and adds a
+// zero; without, fma(dhi, 0, dlo * inv) adds a zero to the rounded dlo * inv (dhi >= 0 >= dlo and
+// the sign of inv is the sense, so no product is a negative zero that could flip a +0 result); and
+// the faces ahead, -b + (n - 1) or b + 0, are exact small integers either way.
+struct WalkLane {
+    WalkDir d;
+    float invUp[3], invDn[3], aheadSgn[3], aheadBase[3];
+};
+template <int DIMS>
+__host__ __device__ inline WalkLane walkLane(const MediumView& m, const f3& dir) {
+    WalkLane w;
+    w.d = walkDir<DIMS>(m, dir);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        w.invUp[a] = w.d.up[a] ? w.d.inv[a] : 0.f;
+        w.invDn[a] = w.d.up[a] ? 0.f : w.d.inv[a];
+        w.aheadSgn[a] = w.d.up[a] ? -1.f : 1.f;
+        w.aheadBase[a] = w.d.up[a] ? (float)(m.n[a] - 1) : 0.f;
+    }
+    return w;
+}
+template <int DIMS>
+__host__ __device__ inline float walkFromStartLane(const MediumView& m, const float* sig, const float* dlo,
+                                                   const float* dhi, const float* bf, uint32_t cell,
+                                                   const WalkLane& wl, float tauCut, uint32_t* steps) {
+    float tf[3] = {3.0e38f, 3.0e38f, 3.0e38f}, td[3] = {0.f, 0.f, 0.f};
+    uint32_t stride[3] = {0u, 0u, 0u};
+    float tExit = 3.0e38f;
+#pragma unroll
+    for (int a = 0; a < DIMS; ++a) {
+        tf[a] = fmaf(dhi[a], wl.invUp[a], dlo[a] * wl.invDn[a]);
+        td[a] = wl.d.td[a];
+        tExit = fminf(tExit, fmaf(fmaf(wl.aheadSgn[a], bf[a], wl.aheadBase[a]), td[a], tf[a]));
+        stride[a] = wl.d.stride[a];
+    }
+    return walkFastCore<DIMS>(m, sig, tf, td, stride, cell, tExit, tauCut, steps);
+}
+
 // The walk from p with the direction part wd already formed (walkDir).
 template <int DIMS>
 __host__ __device__ inline float opticalDepthFastDir(const MediumView& m, const float* sig, const f3& p,
@@ -497,6 +538,156 @@ __host__ __device__ inline float walkFastCore(const MediumView& m, const float* 
     if (!(tau > tauCut)) tau = fmaf(lastSeg, lastSig, tau);   // left the box: last cell
     if (steps) *steps += (uint32_t)(k0 + k1 + k2);
     return tau;
+}
+
+// The skip threshold of the far-field view kernel for a walk cut `cut`: a vertex whose lower bound
+// (FarRec::lb) exceeds it has every term of the launch past the cut, whatever the FP32 walk's
+// rounding (a relative 1e-3 margin over the walk's and the bound's own rounding), so skipping it
+// leaves the terms as the walks would: zero.
+__host__ __device__ inline float farSkip(float cut) { return cut + 1e-3f * (1.f + cut); }
+
+// Least sigt over cells [b0, b1] x [c0, c1] of one layer: the smallest pyramid level at which the
+// range spans at most 2 x 2 blocks, min over those blocks (a superset of the range: a lower bound).
+__host__ __device__ inline float pyrMin(const MinPyramid& mp, uint32_t layer, uint32_t b0, uint32_t b1, uint32_t c0,
+                               uint32_t c1) {
+    uint32_t L = 0;
+    while (L + 1 < mp.levels && ((b1 >> L) - (b0 >> L) > 1u || (c1 >> L) - (c0 >> L) > 1u)) ++L;
+    const uint32_t nb = mp.nb[L];
+    const float* q = mp.p + mp.off[L] + (uint64_t)layer * nb * mp.nc[L];
+    const uint32_t bl = b0 >> L, bh = b1 >> L, cl = c0 >> L, ch = c1 >> L;
+    return fminf(fminf(q[cl * nb + bl], q[cl * nb + bh]), fminf(q[ch * nb + bl], q[ch * nb + bh]));
+}
+
+// A lower bound on the optical depth from p to the box boundary along every direction of the set
+// bd, stopping once past `stop`. Layer by layer along the set's axis (cells [i, i + 1] along it): a
+// ray advancing along the axis crosses a layer's thickness over a path at least as long, at lateral
+// positions p_q + s r_q with s in [near, far] (distances along the axis) and r_q within [rmin_q,
+// rmax_q], i.e. inside a rectangle of cells whose least sigt bounds the layer's from below. A
+// rectangle reaching the box's side ends the bound (rays may leave there); so does the box's face
+// along the axis. Indices carry a 1e-3-cell margin against rounding, and the result is scaled by
+// (1 - 1e-5). 0 for a point outside the box or no set.
+__host__ __device__ inline float farLowerBound(const MediumView& m, const MinPyramid& mp, const BoundDirs& bd, const f3& pf,
+                                      float stop) {
+    if (!bd.valid || mp.axis < 0 || !inside<3, float>(m, pf)) return 0.f;
+    const float p[3] = {pf.x, pf.y, pf.z};
+    const int a = mp.axis, qs[2] = {a == 0 ? 1 : 0, a == 2 ? 1 : 2};
+    int layer = locateUniform<float>(m, a, p[a]);
+    float near = 0.f, far = bd.sgn > 0 ? m.ef[a][layer + 1] - p[a] : p[a] - m.ef[a][layer];
+    float lb = 0.f;
+    for (;;) {
+        uint32_t lo[2], hi[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int q = qs[j];
+            const float xlo = p[q] + fminf(near * bd.rmin[j], far * bd.rmin[j]);
+            const float xhi = p[q] + fmaxf(near * bd.rmax[j], far * bd.rmax[j]);
+            const float mg = 1e-3f * m.h[q];
+            if (xlo <= m.bminf[q] + mg || xhi >= m.bmaxf[q] - mg) return lb * (1.f - 1e-5f);
+            const float flo = (xlo - m.e0[q]) * m.invH[q] - 1e-3f, fhi = (xhi - m.e0[q]) * m.invH[q] + 1e-3f;
+            const int n1 = (int)m.n[q] - 1;
+            const int il = (int)floorf(flo), ih = (int)floorf(fhi);
+            lo[j] = (uint32_t)(il < 0 ? 0 : (il > n1 ? n1 : il));
+            hi[j] = (uint32_t)(ih < 0 ? 0 : (ih > n1 ? n1 : ih));
+        }
+        lb = fmaf(far - near, pyrMin(mp, (uint32_t)layer, lo[0], hi[0], lo[1], hi[1]), lb);
+        if (lb * (1.f - 1e-5f) > stop) break;
+        layer += bd.sgn;
+        if (layer < 0 || layer >= (int)m.n[a]) break;
+        near = far;
+        far += m.h[a];
+    }
+    return lb * (1.f - 1e-5f);
+}
+
+
+// Host builders of the bound's inputs. minPyramidHost: the pyramid of a grid of material ids
+// (x fastest) with layers along `axis`, FP32 sigt per id; mp's offsets and sizes filled, mp->p left
+// for the caller (the returned values' device copy); an empty vector when levels exceed kPyrLevels.
+inline std::vector<float> minPyramidHost(const uint8_t* g, const uint32_t n[3], const float* sigt, int axis,
+                                         MinPyramid* mp) {
+    *mp = MinPyramid{};
+    const int qb = axis == 0 ? 1 : 0, qc = axis == 2 ? 1 : 2;
+    uint32_t levels = 1;
+    while ((1u << (levels - 1)) < (n[qb] > n[qc] ? n[qb] : n[qc])) ++levels;
+    if (levels > (uint32_t)kPyrLevels) return {};
+    mp->axis = axis;
+    mp->nl = n[axis];
+    mp->levels = levels;
+    uint64_t total = 0;
+    for (uint32_t L = 0; L < levels; ++L) {
+        mp->nb[L] = (n[qb] + (1u << L) - 1) >> L;
+        mp->nc[L] = (n[qc] + (1u << L) - 1) >> L;
+        mp->off[L] = total;
+        total += (uint64_t)mp->nl * mp->nb[L] * mp->nc[L];
+    }
+    std::vector<float> h(total);
+    uint32_t idx[3];
+    for (uint32_t l = 0; l < mp->nl; ++l)
+        for (uint32_t ic = 0; ic < n[qc]; ++ic)
+            for (uint32_t ib = 0; ib < n[qb]; ++ib) {
+                idx[axis] = l;
+                idx[qb] = ib;
+                idx[qc] = ic;
+                const size_t cell = idx[0] + (size_t)n[0] * (idx[1] + (size_t)n[1] * idx[2]);
+                h[((uint64_t)l * n[qc] + ic) * n[qb] + ib] = sigt[g[cell]];
+            }
+    for (uint32_t L = 1; L < levels; ++L) {
+        const uint32_t pb = mp->nb[L - 1], pc = mp->nc[L - 1], nb = mp->nb[L], nc = mp->nc[L];
+        for (uint32_t l = 0; l < mp->nl; ++l)
+            for (uint32_t ic = 0; ic < nc; ++ic)
+                for (uint32_t ib = 0; ib < nb; ++ib) {
+                    float v = 3.0e38f;
+                    for (uint32_t dc = 0; dc < 2; ++dc)
+                        for (uint32_t db = 0; db < 2; ++db) {
+                            const uint32_t cb = 2 * ib + db, cc = 2 * ic + dc;
+                            if (cb < pb && cc < pc) {
+                                const float w = h[mp->off[L - 1] + ((uint64_t)l * pc + cc) * pb + cb];
+                                v = w < v ? w : v;
+                            }
+                        }
+                    h[mp->off[L] + ((uint64_t)l * nc + ic) * nb + ib] = v;
+                }
+    }
+    return h;
+}
+
+// The bound's direction set for walks along sg * d_i (d: n FP32 directions, 3 each): the axis
+// and sense along which every walk advances most (the largest least component), the lateral slopes'
+// range widened by a relative 1e-6. valid = 0 when some walk advances less than 0.1 along it.
+inline BoundDirs boundDirsHost(const float* d, size_t n, double sg, int* axisOut) {
+    BoundDirs bd;
+    double best = -1.0;
+    int axis = -1, sgn = 1;
+    for (int a = 0; a < 3; ++a)
+        for (int s = -1; s <= 1; s += 2) {
+            double mn = 1e300;
+            for (size_t i = 0; i < n; ++i) {
+                const double c = s * sg * (double)d[3 * i + a];
+                mn = c < mn ? c : mn;
+            }
+            if (mn > best) {
+                best = mn;
+                axis = a;
+                sgn = s;
+            }
+        }
+    *axisOut = axis;
+    if (!(best >= 0.1)) return bd;
+    bd.valid = 1;
+    bd.sgn = sgn;
+    const int qs[2] = {axis == 0 ? 1 : 0, axis == 2 ? 1 : 2};
+    for (int j = 0; j < 2; ++j) {
+        double lo = 1e300, hi = -1e300;
+        for (size_t i = 0; i < n; ++i) {
+            const double da = std::fabs((double)d[3 * i + axis]);
+            const double r = sg * (double)d[3 * i + qs[j]] / da;
+            lo = r < lo ? r : lo;
+            hi = r > hi ? r : hi;
+        }
+        bd.rmin[j] = (float)(lo - 1e-6 * (1.0 + std::fabs(lo)));
+        bd.rmax[j] = (float)(hi + 1e-6 * (1.0 + std::fabs(hi)));
+    }
+    return bd;
 }
 
 }  // namespace sstmc

@@ -72,10 +72,11 @@ hipError_t launchBatchTotals(const uint32_t* misses, const uint32_t* off, uint32
                              uint64_t cap, uint32_t* overflow, uint32_t batch, float* tauRef, hipStream_t st);
 
 // [k_tau_ref when p.tauRef is set,] k_nee (first vertex) then k_nee_view (vertices k >= 2);
-// `between`, when not null, is recorded on `st` between the two launches.
+// `between`, when not null, is recorded on `st` between the two launches. With vb.far and fb,
+// k_far_bound bounds every vertex's far-field walks from below before the view kernels run.
 hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
                      const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
-                     const NeeOut& out, hipStream_t st, hipEvent_t between = nullptr);
+                     const NeeOut& out, hipStream_t st, hipEvent_t between = nullptr, const FarBound* fb = nullptr);
 
 // TemporalCorrelationSampler with coherent backscattering: the mixed pairings CBS02 + CBS03 of
 // every path with at least two vertices, from the NeeOut first-vertex (F*) and per-vertex (G*)

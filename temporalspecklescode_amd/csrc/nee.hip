@@ -721,7 +721,7 @@ __global__ void __launch_bounds__(256) k_vertex_prep(MediumView m, VertexBatch v
                 f.back[0] = (float)b.x;
                 f.back[1] = (float)b.y;
                 f.back[2] = (float)b.z;
-                f.pad[0] = f.pad[1] = 0.f;
+                f.lb[0] = f.lb[1] = 0.f;
                 f.px = xk.pd.x;
                 f.py = xk.pd.y;
                 f.pz = xk.pd.z;
@@ -868,6 +868,21 @@ k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBat
     if ((threadIdx.x & 63) == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
 }
 
+// Lower bounds of the far-field view kernel's walks (FarRec::lb) for every record of the batch:
+// lb[0] for the views under the launch's cut (viewCut), lb[1] for the reversed paths'
+// illuminations under the exact cut. One thread per record; records of first vertices, which no
+// kernel reads, are bounded too (from whatever they hold; every index is clamped).
+__global__ void __launch_bounds__(256) k_far_bound(MediumView m, VertexBatch vb, FramesView fr, NeeParams prm,
+                                                   FarBound fb, uint32_t doIllum) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t P = prm.npaths;
+    if ((uint64_t)vb.off[P] > vb.cap || i >= (uint64_t)vb.off[P] * fr.nt) return;
+    FarRec& r = vb.far[i];
+    const f3 pf = mk((float)r.px, (float)r.py, (float)r.pz);
+    r.lb[0] = farLowerBound(m, fb.pyr[0], fb.dirs[0], pf, farSkip(viewCut<float>(prm)));
+    r.lb[1] = doIllum ? farLowerBound(m, fb.pyr[1], fb.dirs[1], pf, farSkip(prm.exactCutF)) : 0.f;
+}
+
 // HG amplitude constants per material for the far-field view kernel: sqrt((1 - g^2) / 4 pi),
 // (1 - g)^2 and 2 g, so that |f| = C den^(-3/4) with den = (1 - g)^2 + 2 g (1 - c). An isotropic
 // material is g = 0.
@@ -918,18 +933,21 @@ k_nee_view_far(MediumView m, ScatterView sc, SourceView vw, FramesView fr, Verte
         // -d.x / lambda (view) or +d.x / lambda (illumination); the HG incoming direction is -d
         // (view) or d (illumination) (FarField.cuh:223-263, 546-599)
         const float sgn = SRC == kSrcFarIllum ? -1.f : 1.f;
-        const WalkDir wd = walkDir<3>(m, mk(sgn * vf.x, sgn * vf.y, sgn * vf.z));
+        const WalkLane wl = walkLane<3>(m, mk(sgn * vf.x, sgn * vf.y, sgn * vf.z));
         const double ks = (SRC == kSrcFarIllum ? 1.0 : -1.0) / prm.lambda;
         const d3 kd = mkd(vd.x * ks, vd.y * ks, vd.z * ks);
         const float as = SRC == kSrcFarIllum ? 1.f : -1.f;
         const f3 av = mk(as * (float)vd.x, as * (float)vd.y, as * (float)vd.z);
         float accRe = 0.f, accIm = 0.f;
+        const float skip = farSkip(tauCut);
         for (uint32_t k = 1; k < K; ++k) {
-            const FarRec rec = vb.far[(off + k) * fr.nt + t];
+            const FarRec* rp = vb.far + (off + k) * fr.nt + t;
+            if (rp->lb[SRC == kSrcFarIllum ? 1 : 0] > skip) continue;   // every term past the cut
+            const FarRec rec = *rp;
             const float tau = rec.cell == kFarOutside
                                   ? 0.f
-                                  : walkFromStart<3>(m, sig + kMaxMaterials, rec.dlo, rec.dhi, rec.b, rec.cell, wd,
-                                                     tauCut, &steps);
+                                  : walkFromStartLane<3>(m, sig + kMaxMaterials, rec.dlo, rec.dhi, rec.b, rec.cell,
+                                                         wl, tauCut, &steps);
             const float amp = tau > tauCut ? 0.f : __expf(-0.5f * tau);
             if (__ballot(amp != 0.f) == 0) continue;   // every term of this vertex is zero
             const HgConst h = hg[rec.mat];
@@ -1438,7 +1456,7 @@ static void launchViewCorrSampler(const MediumView& m, const ScatterView& sc, co
 template <int DIMS, typename R>
 static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceView& il, const SourceView& vw,
                        const FramesView& fr, const VertexBatch& vb, const NeeParams& p, const NeeOut& out,
-                       hipStream_t st, hipEvent_t between) {
+                       hipStream_t st, hipEvent_t between, const FarBound* fb) {
     const uint32_t ppb = 4 * (64 / p.lpp);   // paths per 256-thread block
     dim3 grid((p.npaths + ppb - 1) / ppb), block(256);
     if (p.tauRef) launchTauRef<DIMS>(m, vw, fr, vb, p, st);   // the relative cut's reference
@@ -1475,6 +1493,11 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
     else hipLaunchKernelGGL((k_nee<DIMS, R, false, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
     hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, m, vb, fr, p.npaths,
                        1.0 / p.lambda);
+    if (DIMS == 3 && vb.far && fb) {
+        const uint64_t n = vb.cap * fr.nt;
+        hipLaunchKernelGGL(k_far_bound, dim3((uint32_t)((n + 255) / 256)), block, 0, st, m, vb, fr, p, *fb,
+                           out.Br ? 1u : 0u);
+    }
     if (between) hipEventRecord(between, st);
     const bool corr = p.output == 1;
     if (vw.type == 3) {
@@ -1497,7 +1520,7 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
 
 hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
                      const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
-                     const NeeOut& out, hipStream_t st, hipEvent_t between) {
+                     const NeeOut& out, hipStream_t st, hipEvent_t between, const FarBound* fb) {
     // lane groups: k_nee gives each path lpp lanes (its L and V items), the view kernels give each
     // work unit lppView lanes (one source each); both shrink below a wavefront when there are few
     // items so that several paths share one. A view unit is (path, frame, block of views) for
@@ -1516,11 +1539,11 @@ hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView&
     viewLanes(fr.analytic ? vw.n / 2 : vw.n, &q);
     q.frameChunks = corr ? (fr.nt + kCorrLags - 1) / kCorrLags : fr.nt;
     if (dims == 2) {
-        if (fp64) launchNeeT<2, double>(m, sc, il, vw, fr, vb, q, out, st, between);
-        else launchNeeT<2, float>(m, sc, il, vw, fr, vb, q, out, st, between);
+        if (fp64) launchNeeT<2, double>(m, sc, il, vw, fr, vb, q, out, st, between, fb);
+        else launchNeeT<2, float>(m, sc, il, vw, fr, vb, q, out, st, between, fb);
     } else {
-        if (fp64) launchNeeT<3, double>(m, sc, il, vw, fr, vb, q, out, st, between);
-        else launchNeeT<3, float>(m, sc, il, vw, fr, vb, q, out, st, between);
+        if (fp64) launchNeeT<3, double>(m, sc, il, vw, fr, vb, q, out, st, between, fb);
+        else launchNeeT<3, float>(m, sc, il, vw, fr, vb, q, out, st, between, fb);
     }
     return hipGetLastError();
 }

@@ -54,6 +54,10 @@ public:
     // walk does not apply. Thread-safe.
     int walkGrid(int fast, MediumView* mv) const;
     uint64_t walkBytes() const;
+    // The per-layer least-sigt pyramid with layers along `axis` (3D, equally spaced grids), built on
+    // first use and kept like the walk grids; axis = -1 in *out when it does not apply (or would
+    // exceed 1 GiB). Thread-safe.
+    int minPyramid(int axis, MinPyramid* out) const;
 
 protected:
     MediumView v_{};
@@ -71,6 +75,12 @@ private:
     int buildWalk(int fast, WalkCache* out) const;   // caller holds walkMu_
     mutable std::mutex walkMu_;
     mutable WalkCache walk_[3];
+    struct PyrCache {
+        bool built = false;
+        DevBuf<float> buf;
+        MinPyramid view;
+    };
+    mutable PyrCache pyr_[3];                        // guarded by walkMu_
 };
 
 // HeterogeneousMedium (HeterogeneousMedium.cuh:52-233): rectilinear grid of material ids.
@@ -331,6 +341,9 @@ private:
     bool sameIllums_ = false;       // correlation sampler: illumination2 holds the same sources
     bool cbsCorr_ = false;          // coherent backscattering under the correlation sampler
     bool farRecs_ = false;          // the far-field view kernel runs: FarRec per (vertex, frame)
+    FarBound farBound_{};           // its per-vertex lower bounds (k_far_bound)
+    bool farBoundOn_ = false;
+    int setupFarBound();
     // the contraction stage captured as HIP graphs, per (set, batch size, destination): the
     // multiple-scattering part and the single-scattering part (timed apart); dropped whenever a
     // buffer they name is reallocated

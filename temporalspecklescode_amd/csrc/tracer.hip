@@ -826,6 +826,7 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
                mview_.walkFast && singleMode_ != kSingleGauss &&
                ((vw_.type() == SSTMC_SOURCE_FARFIELD && V_ > 32) ||
                 (opt.cbs && il_.type() == SSTMC_SOURCE_FARFIELD && L_ > 32));
+    if ((*err = setupFarBound())) return;
     if ((*err = allocate())) return;
     if (needF) {
         std::vector<float2> ff((size_t)L_ * V_);
@@ -1017,7 +1018,8 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
         out.Gv2 = bs.Gv2.p;
         out.gcap = cap_;
     }
-    if (launchNee(dims_, fp64_ ? 1 : 0, m, sc, il, vw, fr, vb, prm, out, sN, e ? e[5] : nullptr) != hipSuccess)
+    if (launchNee(dims_, fp64_ ? 1 : 0, m, sc, il, vw, fr, vb, prm, out, sN, e ? e[5] : nullptr,
+                  farBoundOn_ ? &farBound_ : nullptr) != hipSuccess)
         return SSTMC_KERNEL_ERROR;
     if (e) hipEventRecord(e[2], sN);
     if (hipEventRecord(bs.traced, sN) != hipSuccess) return SSTMC_DEVICE_ERROR;
@@ -1233,7 +1235,57 @@ uint64_t Medium::walkBytes() const {
     std::lock_guard<std::mutex> lk(walkMu_);
     uint64_t b = 0;
     for (const WalkCache& wc : walk_) b += wc.buf.n * sizeof(uint32_t);
+    for (const PyrCache& pc : pyr_) b += pc.buf.n * sizeof(float);
     return b;
+}
+
+int Medium::minPyramid(int axis, MinPyramid* out) const {
+    *out = MinPyramid{};
+    if (v_.dims != 3 || !v_.allUniform || axis < 0 || axis > 2) return SSTMC_NO_ERROR;
+    std::lock_guard<std::mutex> lk(walkMu_);
+    PyrCache& pc = pyr_[axis];
+    if (!pc.built) {
+        MinPyramid mp;
+        const std::vector<float> h = minPyramidHost(hostGrid().data(), v_.n, v_.sigtf, axis, &mp);
+        if (!h.empty() && h.size() * sizeof(float) <= (1ull << 30)) {
+            const int e = pc.buf.upload(h.data(), h.size());
+            if (e) return e;
+            mp.p = pc.buf.p;
+            pc.view = mp;
+        }
+        pc.built = true;
+    }
+    *out = pc.view;
+    return SSTMC_NO_ERROR;
+}
+
+// The far-field view kernel's lower bounds: per launch (views; reversed-path illuminations) the
+// axis the walks all advance along, its sign and the lateral slopes over the walks' FP32
+// directions (widened by a relative 1e-6), and the medium's pyramid along that axis. A set whose
+// walks do not all advance along one axis by at least 0.1 of their length gets no bound.
+int NextEventEstimation::setupFarBound() {
+    farBound_ = FarBound{};
+    farBoundOn_ = false;
+    if (!farRecs_) return SSTMC_NO_ERROR;
+    for (int role = 0; role < 2; ++role) {
+        const Source& src = role == 0 ? vw_ : il_;
+        if (src.type() != SSTMC_SOURCE_FARFIELD || (role == 1 && !opt_.cbs)) continue;
+        // the kernel's FP32 directions; walks along the view direction, against the illumination's
+        const std::vector<double>& hp = src.hostPoints();
+        std::vector<float> df(hp.size());
+        for (size_t i = 0; i < hp.size(); ++i) df[i] = (float)hp[i];
+        int axis = -1;
+        const BoundDirs bd = boundDirsHost(df.data(), df.size() / 3, role == 0 ? 1.0 : -1.0, &axis);
+        if (!bd.valid) continue;
+        MinPyramid mp;
+        const int e = medium_.minPyramid(axis, &mp);
+        if (e) return e;
+        if (mp.axis < 0) continue;
+        farBound_.pyr[role] = mp;
+        farBound_.dirs[role] = bd;
+        farBoundOn_ = true;
+    }
+    return SSTMC_NO_ERROR;
 }
 
 int NextEventEstimation::buildWalkGrid() {
