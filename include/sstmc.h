@@ -232,7 +232,9 @@ int sstmc_render(const sstmc_config* cfg, double* u, double* iter);
 /* Device time of each stage of the last trace, from HIP events on the tracer's stream, in ms.
  * Mirrors the TIME_REC TimeTracer report (NextEventEstimation.cuh:107-156). */
 typedef struct sstmc_timing {
-    double transport_ms;           /* free flight + direction sampling + frames (a) */
+    double transport_ms;           /* free flight + direction sampling + frames (a), then the
+                                      per-vertex NEE set-up (attenuation cut reference, vertex
+                                      records, walk bounds) on the same stream */
     double nee_ms;                 /* NEE attenuation, g, f, probabilities, phase (b) */
     double contract_ms;            /* complex contraction (c) */
     double single_ms;              /* single-scattering contraction / fused kernel */
@@ -245,6 +247,8 @@ typedef struct sstmc_timing {
     uint64_t dda_steps_view;       /* part of dda_steps walked by that kernel */
     uint32_t graph_replays;        /* batches whose contraction stage replayed captured graphs
                                       (simulation.graphs) */
+    uint32_t pipeline_sets;        /* buffer sets the batch pipeline cycles through (batch b uses
+                                      set b mod pipeline_sets) */
 } sstmc_timing;
 int sstmc_last_timing(const sstmc_handle* h, sstmc_timing* out);
 

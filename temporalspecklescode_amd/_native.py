@@ -25,7 +25,8 @@ class Timing(C.Structure):
     _fields_ = [("transport_ms", C.c_double), ("nee_ms", C.c_double), ("contract_ms", C.c_double),
                 ("single_ms", C.c_double), ("total_ms", C.c_double), ("vertices", C.c_uint64),
                 ("dda_steps", C.c_uint64), ("batches", C.c_uint32), ("nee_view_ms", C.c_double),
-                ("dda_steps_view", C.c_uint64), ("graph_replays", C.c_uint32)]
+                ("dda_steps_view", C.c_uint64), ("graph_replays", C.c_uint32),
+                ("pipeline_sets", C.c_uint32)]
 
 
 class ServerStats(C.Structure):

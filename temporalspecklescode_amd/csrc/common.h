@@ -332,4 +332,21 @@ struct VertexBatch {
     FarRec* far = nullptr;         // [cap * nt] like rec, when the far-field view kernel runs
 };
 
+// Staging of the single-pass transport. Lanes draw path slots from a queue and store each path's
+// vertices in chunks of kStageChunk taken from a second counter as the path grows (chunk c holds
+// vertices chunkJ0[c] .. chunkJ0[c] + kStageChunk - 1 of path slot chunkSlot[c]); once the
+// exclusive scan of the per-path counts is known, k_transport_pack copies every staged vertex to
+// off[slot] + j. With nchunks * kStageChunk >= cap + P * (kStageChunk - 1) the staging cannot run
+// out while the batch fits the pool (each path leaves at most kStageChunk - 1 slots unused).
+constexpr uint32_t kStageChunk = 16;
+struct TransportStage {
+    float4* xm;                    // [nchunks * kStageChunk] like VertexBatch::xm
+    float* rho;                    // [nchunks * kStageChunk]
+    float* delta;                  // [nchunks * kStageChunk * nt * 3] (if non-trivial frames)
+    uint32_t* chunkSlot;           // [nchunks] path slot of the chunk
+    uint32_t* chunkJ0;             // [nchunks] index within its path of the chunk's first vertex
+    uint32_t nchunks;
+    uint32_t* q;                   // [2]: next path slot, chunks taken (zeroed before each launch)
+};
+
 }  // namespace sstmc

@@ -62,16 +62,32 @@ struct NeeParams {
     uint32_t sameIllums;   // TemporalCorrelationSampler: illumination2 holds the same sources
 };
 
+// The single-pass transport of npaths path slots (zeroes ts.q, then k_transport): per-path
+// vertex counts and misses into vb, the vertices into the staging; launchTransportPack copies them
+// into vb's pool once vb.off holds the exclusive scan of the counts (launchScan).
 hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc, const SourceView& il,
-                           const FramesView& fr, const VertexBatch& vb, uint64_t seed, uint64_t pathBegin,
-                           uint32_t npaths, int write, hipStream_t st);
+                           const FramesView& fr, const VertexBatch& vb, const TransportStage& ts, uint64_t seed,
+                           uint64_t pathBegin, uint32_t npaths, hipStream_t st);
+hipError_t launchTransportPack(const VertexBatch& vb, const TransportStage& ts, uint32_t npaths, uint32_t nt,
+                               int frames, hipStream_t st);
 hipError_t launchScan(const uint32_t* k, uint32_t* off, uint32_t* blockSums, uint32_t n, hipStream_t st);
 // Adds a batch's misses and vertices to acc[0], acc[1] and its vertex count to the maximum in
-// acc[3]; a batch whose vertices exceed `cap` is flagged in the `overflow` bitmap instead.
-hipError_t launchBatchTotals(const uint32_t* misses, const uint32_t* off, uint32_t n, unsigned long long* acc,
-                             uint64_t cap, uint32_t* overflow, uint32_t batch, float* tauRef, hipStream_t st);
+// acc[3]; a batch whose vertices exceed `cap` (or the staging) is flagged in the `overflow`
+// bitmap and status[0] instead.
+hipError_t launchBatchTotals(const uint32_t* misses, uint32_t* off, uint32_t n, unsigned long long* acc,
+                             uint64_t cap, uint32_t* overflow, uint32_t batch, float* tauRef, const TransportStage& ts,
+                             uint32_t* status, hipStream_t st);
 
-// [k_tau_ref when p.tauRef is set,] k_nee (first vertex) then k_nee_view (vertices k >= 2);
+// The NEE stage in two parts. kNeeVertexPart reads the batch's vertices only: k_tau_ref (when
+// p.tauRef is set), k_vertex_prep (per-vertex records) and, with vb.far and fb, k_far_bound
+// (lower bounds of every vertex's far-field walks). kNeeTermPart forms the terms: k_nee (or
+// k_nee_corr) for the first vertex, then the view kernels for vertices k >= 2; `between`, when not
+// null, is recorded on `st` between the two.
+enum : int { kNeeVertexPart = 1, kNeeTermPart = 2, kNeeAll = 3 };
+hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
+                     const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
+                     const NeeOut& out, hipStream_t st, hipEvent_t between = nullptr, const FarBound* fb = nullptr,
+                     int parts = kNeeAll);
 // `between`, when not null, is recorded on `st` between the two launches. With vb.far and fb,
 // k_far_bound bounds every vertex's far-field walks from below before the view kernels run.
 hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,

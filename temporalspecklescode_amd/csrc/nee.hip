@@ -1456,18 +1456,26 @@ static void launchViewCorrSampler(const MediumView& m, const ScatterView& sc, co
 template <int DIMS, typename R>
 static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceView& il, const SourceView& vw,
                        const FramesView& fr, const VertexBatch& vb, const NeeParams& p, const NeeOut& out,
-                       hipStream_t st, hipEvent_t between, const FarBound* fb) {
+                       hipStream_t st, hipEvent_t between, const FarBound* fb, int parts) {
     const uint32_t ppb = 4 * (64 / p.lpp);   // paths per 256-thread block
     dim3 grid((p.npaths + ppb - 1) / ppb), block(256);
-    if (p.tauRef) launchTauRef<DIMS>(m, vw, fr, vb, p, st);   // the relative cut's reference
+    if (parts & kNeeVertexPart) {
+        // the cut's reference and the per-vertex records read the batch's vertices only
+        if (p.tauRef) launchTauRef<DIMS>(m, vw, fr, vb, p, st);
+        FramesView rf = fr;
+        if (fr.analytic) rf.nt = 1;   // TemporalCorrelationSampler: vertices never move, one record each
+        hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, m, vb, rf, p.npaths,
+                           1.0 / p.lambda);
+        if (DIMS == 3 && !fr.analytic && vb.far && fb) {
+            const uint64_t n = vb.cap * fr.nt;
+            hipLaunchKernelGGL(k_far_bound, dim3((uint32_t)((n + 255) / 256)), block, 0, st, m, vb, fr, p, *fb,
+                               out.Br ? 1u : 0u);
+        }
+    }
+    if (!(parts & kNeeTermPart)) return;
     if (fr.analytic) {
-        // TemporalCorrelationSampler: the vertices never move, one record per vertex
         if (sc.tab) hipLaunchKernelGGL((k_nee_corr<DIMS, R, true>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
         else hipLaunchKernelGGL((k_nee_corr<DIMS, R, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
-        FramesView one = fr;
-        one.nt = 1;
-        hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, m, vb, one, p.npaths,
-                           1.0 / p.lambda);
         if (between) hipEventRecord(between, st);
         if (vw.type == 2) launchViewCorrSampler<DIMS, R, kSrcFarView>(m, sc, vw, fr, vb, p, out, st);
         else launchViewCorrSampler<DIMS, R, kSrcPoint>(m, sc, vw, fr, vb, p, out, st);
@@ -1491,13 +1499,6 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
         hipLaunchKernelGGL((k_nee<DIMS, R, true, true>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
     else if (sc.tab) hipLaunchKernelGGL((k_nee<DIMS, R, true, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
     else hipLaunchKernelGGL((k_nee<DIMS, R, false, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
-    hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, m, vb, fr, p.npaths,
-                       1.0 / p.lambda);
-    if (DIMS == 3 && vb.far && fb) {
-        const uint64_t n = vb.cap * fr.nt;
-        hipLaunchKernelGGL(k_far_bound, dim3((uint32_t)((n + 255) / 256)), block, 0, st, m, vb, fr, p, *fb,
-                           out.Br ? 1u : 0u);
-    }
     if (between) hipEventRecord(between, st);
     const bool corr = p.output == 1;
     if (vw.type == 3) {
@@ -1520,7 +1521,7 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
 
 hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
                      const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
-                     const NeeOut& out, hipStream_t st, hipEvent_t between, const FarBound* fb) {
+                     const NeeOut& out, hipStream_t st, hipEvent_t between, const FarBound* fb, int parts) {
     // lane groups: k_nee gives each path lpp lanes (its L and V items), the view kernels give each
     // work unit lppView lanes (one source each); both shrink below a wavefront when there are few
     // items so that several paths share one. A view unit is (path, frame, block of views) for
@@ -1539,11 +1540,11 @@ hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView&
     viewLanes(fr.analytic ? vw.n / 2 : vw.n, &q);
     q.frameChunks = corr ? (fr.nt + kCorrLags - 1) / kCorrLags : fr.nt;
     if (dims == 2) {
-        if (fp64) launchNeeT<2, double>(m, sc, il, vw, fr, vb, q, out, st, between, fb);
-        else launchNeeT<2, float>(m, sc, il, vw, fr, vb, q, out, st, between, fb);
+        if (fp64) launchNeeT<2, double>(m, sc, il, vw, fr, vb, q, out, st, between, fb, parts);
+        else launchNeeT<2, float>(m, sc, il, vw, fr, vb, q, out, st, between, fb, parts);
     } else {
-        if (fp64) launchNeeT<3, double>(m, sc, il, vw, fr, vb, q, out, st, between, fb);
-        else launchNeeT<3, float>(m, sc, il, vw, fr, vb, q, out, st, between, fb);
+        if (fp64) launchNeeT<3, double>(m, sc, il, vw, fr, vb, q, out, st, between, fb, parts);
+        else launchNeeT<3, float>(m, sc, il, vw, fr, vb, q, out, st, between, fb, parts);
     }
     return hipGetLastError();
 }

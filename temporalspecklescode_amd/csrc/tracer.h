@@ -271,6 +271,11 @@ public:
     // device bytes a tracer with these sizes allocates (admission control)
     static uint64_t footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames, uint64_t cells,
                               bool cbs, bool acorr = false);
+    // device bytes of one batch set (terms, per-path arrays, a vertex pool of 32 vertices a path)
+    static uint64_t setBytes(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames, bool cbs,
+                             bool acorr);
+    // batch sets of the pipeline: three while a set is small next to the GPU's memory, else two
+    static int pipelineSets(uint64_t setBytes) { return setBytes <= (8ull << 30) ? 3 : 2; }
     int debugBatch(uint64_t pathBegin, uint32_t npaths, uint64_t* nvtxOut, uint32_t* pathK, uint32_t* pathMisses,
                    uint64_t* pathOff, float* vtxPos, uint32_t* vtxMat, float* vtxRho, float* vtxXt, double* vtxLen,
                    double* termA, double* termB, double* termSa, double* termSb, double* uBatch);
@@ -294,8 +299,8 @@ private:
     int allocate();
     int growPool(uint64_t cap);
     int sizePool();
-    // Per-batch device state. Batch b uses set b & 1: its per-path arrays, vertex pool, term
-    // slabs and cut reference, so that two batches can be in different stages at once.
+    // Per-batch device state. Batch b uses set b mod nsets_: its per-path arrays, vertex pool,
+    // term slabs and cut reference, so that batches can be in different stages at once.
     struct BatchSet {
         DevBuf<uint32_t> k, misses, off, blockSums;
         DevBuf<float4> xm;
@@ -367,7 +372,19 @@ private:
     hipStream_t streamC_ = nullptr;   // contraction
     uint64_t bytes_ = 0;
 
-    BatchSet sets_[2];
+    static constexpr int kMaxSets = 3;
+    BatchSet sets_[kMaxSets];
+    // staging of the single-pass transport, shared by the sets (every transport of a tracer runs
+    // on one stream, or with the streams idle)
+    struct Staging {
+        DevBuf<float4> xm;
+        DevBuf<float> rho, delta;
+        DevBuf<uint32_t> chunkSlot, chunkJ0, q;
+        uint32_t nchunks = 0;
+    } stage_;
+    TransportStage transportStage() const;
+    int nsets_ = 2;                  // sets in use (pipelineSets)
+    int setOf(uint32_t b) const { return (int)(b % (uint32_t)nsets_); }
     DevBuf<uint32_t> status_;
     DevBuf<uint32_t> overflow_;           // bit b: batch b's vertices did not fit the pool
     DevBuf<unsigned long long> totals_;   // [0] misses, [1] vertices, [2] NEE voxel steps

@@ -827,6 +827,7 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
                ((vw_.type() == SSTMC_SOURCE_FARFIELD && V_ > 32) ||
                 (opt.cbs && il_.type() == SSTMC_SOURCE_FARFIELD && L_ > 32));
     if ((*err = setupFarBound())) return;
+    nsets_ = pipelineSets(setBytes(P_, L_, V_, T_, fp64_, !sampler_.view().trivial, opt.cbs != 0, acorr));
     if ((*err = allocate())) return;
     if (needF) {
         std::vector<float2> ff((size_t)L_ * V_);
@@ -870,11 +871,13 @@ int NextEventEstimation::syncAll() const {
 
 int NextEventEstimation::allocate() {
     int e;
-    if ((e = status_.alloc(4)) || (e = totals_.alloc(kTotals)) || (e = overflow_.alloc((opt_.fullIteration + 31) / 32)))
+    if ((e = status_.alloc(4)) || (e = totals_.alloc(kTotals)) || (e = overflow_.alloc((opt_.fullIteration + 31) / 32)) ||
+        (e = stage_.q.alloc(2)))
         return e;
     size_t elem = fp64_ ? 16 : 8;
     size_t na = (size_t)T_ * P_ * L_ * elem, nb = (size_t)T_ * P_ * V_ * elem;
-    for (BatchSet& bs : sets_) {
+    for (int s = 0; s < nsets_; ++s) {
+        BatchSet& bs = sets_[s];
         if ((e = bs.k.alloc(P_)) || (e = bs.misses.alloc(P_)) || (e = bs.off.alloc(P_ + 1)) ||
             (e = bs.blockSums.alloc((P_ + 1023) / 1024 + 1)) || (e = bs.tauRef.alloc(1)))
             return e;
@@ -903,13 +906,14 @@ VertexBatch NextEventEstimation::vertexBatch(int set) const {
                        farRecs_ ? bs.far.p : nullptr};
 }
 
-// Both sets' vertex pools hold `cap` vertices; the streams must be idle.
+// Every set's vertex pool holds `cap` vertices; the streams must be idle.
 int NextEventEstimation::growPool(uint64_t cap) {
     int e;
     dropGraphs();   // they name the pools being replaced
     // vertex records per frame; the correlation sampler's vertices do not move: one record each
     const uint64_t recT = sampler_.correlation() ? 1 : T_;
-    for (BatchSet& bs : sets_) {
+    for (int s = 0; s < nsets_; ++s) {
+        BatchSet& bs = sets_[s];
         if ((e = bs.xm.alloc(cap)) || (e = bs.rho.alloc(cap)) || (e = bs.rec.alloc(cap * recT))) return e;
         if (farRecs_ && (e = bs.far.alloc(cap * recT))) return e;
         if (!sampler_.view().trivial) {
@@ -923,14 +927,35 @@ int NextEventEstimation::growPool(uint64_t cap) {
         // per-vertex u1 / u2 terms of every illumination and view, per lag
         const size_t elem = fp64_ ? 16 : 8;
         const size_t nl = (size_t)T_ * cap * L_ * elem, nv = (size_t)T_ * cap * V_ * elem;
-        for (BatchSet& bs : sets_)
-            if ((e = bs.Gl1.alloc(nl)) || (e = bs.Gl2.alloc(nl)) || (e = bs.Gv1.alloc(nv)) || (e = bs.Gv2.alloc(nv)))
+        for (int s = 0; s < nsets_; ++s)
+            if (BatchSet& bs = sets_[s]; (e = bs.Gl1.alloc(nl)) || (e = bs.Gl2.alloc(nl)) || (e = bs.Gv1.alloc(nv)) || (e = bs.Gv2.alloc(nv)))
                 return e;
         perVertex += 2 * (uint64_t)T_ * (L_ + V_) * elem;
     }
-    bytes_ += 2 * (cap - cap_) * perVertex;
+    bytes_ += (uint64_t)nsets_ * (cap - cap_) * perVertex;
+    // the transport's staging: the pool plus the chunk tails of every path (TransportStage)
+    const uint64_t stageNeed = cap + (uint64_t)P_ * (kStageChunk - 1);
+    const uint64_t nchunks = (stageNeed + kStageChunk - 1) / kStageChunk + 1;
+    if (nchunks > 0xffffffffull / kStageChunk) return SSTMC_ALLOCATION_ERROR;
+    if (nchunks > stage_.nchunks) {
+        const uint64_t nv = nchunks * kStageChunk;
+        if ((e = stage_.xm.alloc(nv)) || (e = stage_.rho.alloc(nv)) || (e = stage_.chunkSlot.alloc(nchunks)) ||
+            (e = stage_.chunkJ0.alloc(nchunks)))
+            return e;
+        if (!sampler_.view().trivial) {
+            if ((e = stage_.delta.alloc(nv * T_ * 3))) return e;
+        }
+        const uint64_t perStaged = 20 + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_);
+        bytes_ += (nchunks - stage_.nchunks) * (kStageChunk * perStaged + 8);
+        stage_.nchunks = (uint32_t)nchunks;
+    }
     cap_ = cap;
     return SSTMC_NO_ERROR;
+}
+
+TransportStage NextEventEstimation::transportStage() const {
+    return TransportStage{stage_.xm.p, stage_.rho.p, stage_.delta.p, stage_.chunkSlot.p, stage_.chunkJ0.p,
+                          stage_.nchunks, stage_.q.p};
 }
 
 int NextEventEstimation::contractSingle(uint32_t npaths, double* U, int set, hipStream_t st) {
@@ -960,11 +985,11 @@ int NextEventEstimation::contractSingle(uint32_t npaths, double* U, int set, hip
     return he == hipSuccess ? SSTMC_NO_ERROR : SSTMC_KERNEL_ERROR;
 }
 
-// One batch in three stages: transport (free flight, vertex offsets, totals), NEE (cut reference,
-// first-vertex terms, vertex records, view terms) and contraction (multiple then single
-// scattering into U). Pipelined, each stage runs on its own stream: the transport of a set waits
+// One batch in three stages: transport (free flight, vertex offsets, totals, then the cut
+// reference, vertex records and walk bounds), NEE (first-vertex terms, view terms) and
+// contraction (multiple then single scattering into U). Pipelined, each stage runs on its own stream: the transport of a set waits
 // until the set's previous batch has been contracted, the NEE waits for its transport, the
-// contraction for its NEE; batches alternate between the two sets, and the contractions keep
+// contraction for its NEE; batches take the sets in turn, and the contractions keep
 // batch order on their one stream, so U is summed in the same order as without the pipeline.
 int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U, bool timed, uint32_t b, int set,
                                   bool pipelined) {
@@ -979,21 +1004,6 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     hipEvent_t* e = timed ? &ev_[(size_t)b * kEventsPerBatch] : nullptr;
     lastSet_ = set;
 
-    // ---- transport
-    if (bs.used && hipStreamWaitEvent(sT, bs.released, 0) != hipSuccess) return SSTMC_DEVICE_ERROR;
-    bs.used = true;
-    if (e) hipEventRecord(e[0], sT);
-    if (launchTransport(dims_, m, sc, il, fr, vb, seed_, pathBegin, npaths, 0, sT) != hipSuccess) return SSTMC_KERNEL_ERROR;
-    if (launchScan(bs.k.p, bs.off.p, bs.blockSums.p, npaths, sT) != hipSuccess) return SSTMC_KERNEL_ERROR;
-    if (launchTransport(dims_, m, sc, il, fr, vb, seed_, pathBegin, npaths, 1, sT) != hipSuccess) return SSTMC_KERNEL_ERROR;
-    if (launchBatchTotals(bs.misses.p, bs.off.p, npaths, totals_.p, cap_, overflow_.p, b, bs.tauRef.p, sT) != hipSuccess)
-        return SSTMC_KERNEL_ERROR;
-    if (e) hipEventRecord(e[1], sT);
-    if (hipEventRecord(bs.transported, sT) != hipSuccess) return SSTMC_DEVICE_ERROR;
-
-    // ---- NEE
-    if (pipelined && hipStreamWaitEvent(sN, bs.transported, 0) != hipSuccess) return SSTMC_DEVICE_ERROR;
-    if (e) hipEventRecord(e[6], sN);
     NeeParams prm;
     prm.lambda = opt_.lambda;
     prm.output = opt_.output;
@@ -1018,8 +1028,29 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
         out.Gv2 = bs.Gv2.p;
         out.gcap = cap_;
     }
-    if (launchNee(dims_, fp64_ ? 1 : 0, m, sc, il, vw, fr, vb, prm, out, sN, e ? e[5] : nullptr,
-                  farBoundOn_ ? &farBound_ : nullptr) != hipSuccess)
+    const FarBound* fb = farBoundOn_ ? &farBound_ : nullptr;
+
+    // ---- transport, then what NEE needs of the vertices alone (the cut's reference, the
+    // per-vertex records and walk bounds), so that the NEE stream forms terms only
+    if (bs.used && hipStreamWaitEvent(sT, bs.released, 0) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    bs.used = true;
+    if (e) hipEventRecord(e[0], sT);
+    const TransportStage ts = transportStage();
+    if (launchTransport(dims_, m, sc, il, fr, vb, ts, seed_, pathBegin, npaths, sT) != hipSuccess ||
+        launchScan(bs.k.p, bs.off.p, bs.blockSums.p, npaths, sT) != hipSuccess ||
+        launchTransportPack(vb, ts, npaths, fr.nt, fr.trivial ? 0 : 1, sT) != hipSuccess ||
+        launchBatchTotals(bs.misses.p, bs.off.p, npaths, totals_.p, cap_, overflow_.p, b, bs.tauRef.p, ts, status_.p,
+                          sT) != hipSuccess ||
+        launchNee(dims_, fp64_ ? 1 : 0, m, sc, il, vw, fr, vb, prm, out, sT, nullptr, fb, kNeeVertexPart) != hipSuccess)
+        return SSTMC_KERNEL_ERROR;
+    if (e) hipEventRecord(e[1], sT);
+    if (hipEventRecord(bs.transported, sT) != hipSuccess) return SSTMC_DEVICE_ERROR;
+
+    // ---- NEE terms
+    if (pipelined && hipStreamWaitEvent(sN, bs.transported, 0) != hipSuccess) return SSTMC_DEVICE_ERROR;
+    if (e) hipEventRecord(e[6], sN);
+    if (launchNee(dims_, fp64_ ? 1 : 0, m, sc, il, vw, fr, vb, prm, out, sN, e ? e[5] : nullptr, fb, kNeeTermPart) !=
+        hipSuccess)
         return SSTMC_KERNEL_ERROR;
     if (e) hipEventRecord(e[2], sN);
     if (hipEventRecord(bs.traced, sN) != hipSuccess) return SSTMC_DEVICE_ERROR;
@@ -1130,24 +1161,33 @@ int NextEventEstimation::contractStage(uint32_t npaths, double* U, int set, hipS
     return SSTMC_NO_ERROR;
 }
 
+uint64_t NextEventEstimation::setBytes(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames,
+                                       bool cbs, bool acorr) {
+    const uint64_t elem = fp64 ? 16 : 8;
+    uint64_t terms = (cbs ? 3ull : 2ull) * T * P * (uint64_t)(L + V) * elem;
+    // vertex records: 32 bytes, and 96 more for the far-field view kernel's (FP32 terms)
+    uint64_t pool = (uint64_t)P * 32 * (20 + (fp64 ? 32ull : 128ull) * T + (frames ? 12ull * T : 0));
+    if (cbs && acorr) {
+        // correlation CBS: first-vertex u1 / u2 terms and per-vertex u1 / u2 terms
+        terms += 2ull * T * P * (uint64_t)(L + V) * elem;
+        pool += (uint64_t)P * 32 * 2ull * T * (L + V) * elem;
+    }
+    return terms + pool + 24ull * P;
+}
+
 uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames,
                                         uint64_t cells, bool cbs, bool acorr) {
-    uint64_t elem = fp64 ? 16 : 8;
-    // two batch sets (the pipeline's): terms, per-path arrays and vertex pools
-    uint64_t terms = 2 * (cbs ? 3ull : 2ull) * T * P * (uint64_t)(L + V) * elem;
+    const uint64_t elem = fp64 ? 16 : 8;
+    // the pipeline's batch sets: terms, per-path arrays and vertex pools
+    const uint64_t set = setBytes(P, L, V, T, fp64, frames, cbs, acorr);
     uint64_t field = 16ull * T * L * V;
-    // vertex records: 32 bytes, and 96 more for the far-field view kernel's (FP32 terms)
-    uint64_t pool = 2 * (uint64_t)P * 32 * (20 + (fp64 ? 32ull : 128ull) * T + (frames ? 12ull * T : 0));
     uint64_t ws = 2ull * 8 * T * L * V * elem;   // split-K partials, two workspaces
     if (!fp64) ws += 2ull * 8 * ((uint64_t)P + 64) * (L + V + 256);   // and their bf16 hi/lo planes
     uint64_t grids = 2 * cells;                  // material grid + packed walk grid (<= 4 bits a cell)
-    if (cbs && acorr) {
-        // correlation CBS: first-vertex u1 / u2 terms, per-vertex u1 / u2 terms, mixed-pairing slices
-        terms += 2 * 2ull * T * P * (uint64_t)(L + V) * elem;
-        pool += 2 * (uint64_t)P * 32 * 2ull * T * (L + V) * elem;
-        field += 16ull * cbsCrossSplits(L, V, T, P) * T * L * V;
-    }
-    return terms + field + pool + ws + grids + 48ull * P + (1ull << 20);
+    if (cbs && acorr) field += 16ull * cbsCrossSplits(L, V, T, P) * T * L * V;   // mixed-pairing slices
+    // the transport's staging: a pool of 32 vertices a path plus the chunk tails
+    const uint64_t stage = (uint64_t)P * (32 + kStageChunk) * (20 + (frames ? 12ull * T : 0) + 1);
+    return (uint64_t)pipelineSets(set) * set + field + ws + grids + stage + (1ull << 20);
 }
 
 // The NEE walk grid: per cell 4 x the index of its material in the palette of the materials
@@ -1309,8 +1349,8 @@ int NextEventEstimation::buildWalkGrid() {
     return medium_.walkGrid(fast, &mview_);
 }
 
-// Sizes the vertex pool from the exact vertex count of the first batch (its count pass and scan,
-// read back once), so that the batches queued after it do not overflow a pool sized by guess and
+// Sizes the vertex pool from the exact vertex count of the first batch (its transport and scan,
+// read back once; the counts are exact even where the staging is too small to hold the vertices), so that the batches queued after it do not overflow a pool sized by guess and
 // have to run again. The margin covers batch-to-batch variation; a batch that still overflows is
 // re-run by finish().
 int NextEventEstimation::sizePool() {
@@ -1319,7 +1359,8 @@ int NextEventEstimation::sizePool() {
     const FramesView& fr = sampler_.view();
     VertexBatch vb = vertexBatch(0);
     BatchSet& bs = sets_[0];
-    if (launchTransport(dims_, mview_, sc_.view(), il_.view(), fr, vb, seed_, 0, P_, 0, stream_) != hipSuccess ||
+    if (launchTransport(dims_, mview_, sc_.view(), il_.view(), fr, vb, transportStage(), seed_, 0, P_, stream_) !=
+            hipSuccess ||
         launchScan(bs.k.p, bs.off.p, bs.blockSums.p, P_, stream_) != hipSuccess)
         return SSTMC_KERNEL_ERROR;
     uint32_t need = 0;
@@ -1368,7 +1409,7 @@ int NextEventEstimation::orderAfterNee() {
 
 int NextEventEstimation::enqueueBatch(uint32_t b) {
     if (b >= opt_.fullIteration) return SSTMC_INVALID_CONFIG;
-    return runBatch((uint64_t)b * P_, P_, U_.p, true, b, (int)(b & 1u), true);
+    return runBatch((uint64_t)b * P_, P_, U_.p, true, b, setOf(b), true);
 }
 
 int NextEventEstimation::enqueue() {
@@ -1392,10 +1433,9 @@ int NextEventEstimation::finish(double* u, bool deviceDst) {
         if (hipMemcpy(st, status_.p, sizeof(st), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
         if (hipMemcpy(tot, totals_.p, sizeof(tot), hipMemcpyDeviceToHost) != hipSuccess) return SSTMC_DEVICE_ERROR;
         if (st[1]) return SSTMC_NO_HIT;
-        if (st[0] & 2u) return SSTMC_KERNEL_ERROR;   // count and write passes disagreed
         if (!(st[0] & 1u)) break;
         // the pool now holds the largest batch with margin, so a re-run batch fits; a second
-        // overflow would mean the count pass changed, which it cannot
+        // overflow would mean a batch's vertex count changed, which it cannot
         if (round == 2) return SSTMC_ALLOCATION_ERROR;
         std::vector<uint32_t> bits(overflow_.n);
         if (hipMemcpy(bits.data(), overflow_.p, bits.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
@@ -1409,7 +1449,7 @@ int NextEventEstimation::finish(double* u, bool deviceDst) {
         if ((rc = orderAfterNee())) return rc;
         for (uint32_t b = 0; b < opt_.fullIteration; ++b)
             if (bits[b >> 5] & (1u << (b & 31)))
-                if ((rc = runBatch((uint64_t)b * P_, P_, U_.p, true, b, (int)(b & 1u), true))) return rc;
+                if ((rc = runBatch((uint64_t)b * P_, P_, U_.p, true, b, setOf(b), true))) return rc;
     }
     zeroContributionPaths_ = tot[0];
     photonsBudget_ = tot[1];
@@ -1435,6 +1475,7 @@ int NextEventEstimation::finish(double* u, bool deviceDst) {
     t.dda_steps_view = tot[4];
     t.batches = opt_.fullIteration;
     t.graph_replays = graphReplays_;
+    t.pipeline_sets = (uint32_t)nsets_;
     last_ = t;
     return SSTMC_NO_ERROR;
 }
@@ -1495,7 +1536,6 @@ int NextEventEstimation::debugBatch(uint64_t pathBegin, uint32_t n, uint64_t* nv
         hipMemcpy(st, status_.p, sizeof(st), hipMemcpyDeviceToHost);
         hipMemcpy(tot, totals_.p, sizeof(tot), hipMemcpyDeviceToHost);
         if (st[1]) return SSTMC_NO_HIT;
-        if (st[0] & 2u) return SSTMC_KERNEL_ERROR;
         if (st[0] & 1u) {
             if ((rc = growPool(std::max<uint64_t>(cap_ * 2, tot[3] + tot[3] / 4 + 1024)))) return rc;
             continue;

@@ -1,12 +1,11 @@
 // transport.hip -- (a) free-flight sampling of every path of a batch, FP32.
 //
 // Replaces the reference's per-bounce chain of launches (sapmleFirstPathPoints /
-// sapmleNextPathPoints, NextEventEstimation.cuh:643-759; TemporalSampler::sampleFirst /
-// sampleSecond / sampleNext, TemporalSampler.cuh:1324-1462; dtshift_host, :778-937) by one
-// thread per path slot that walks the whole path. The same kernel runs twice per batch: a count
-// pass (vertices per path) and, after an exclusive scan, a write pass that stores the vertices at
-// their offsets. The counter-based stream makes both passes draw identical numbers; the write pass
-// never stores beyond the counted vertices and raises an overflow flag if the pool is too small.
+// sapmleNextPathPoints, NextEventEstimation.cuh:643-759, with their per-bounce compaction of live
+// paths; TemporalSampler::sampleFirst / sampleSecond / sampleNext, TemporalSampler.cuh:1324-1462;
+// dtshift_host, :778-937) by one pass of a persistent grid whose lanes take path slots from a
+// queue and keep their wavefront busy (k_transport), an exclusive scan of the vertex counts, and a
+// copy of the staged vertices to their offsets (k_transport_pack).
 #include "common.h"
 #include "gauss.h"
 #include "geometry.h"
@@ -88,84 +87,209 @@ __device__ inline void frameDeltas(const FramesView& fr, uint64_t seed, uint64_t
     }
 }
 
+// Hands every lane in `need` (a ballot taken by the calling lanes) its own consecutive index from
+// *ctr with one atomic per wavefront; lanes outside `need` get an unspecified value.
+__device__ inline uint32_t waveTake(uint32_t* ctr, uint64_t needMask, bool need) {
+    const int leader = __builtin_ctzll(needMask);
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(needMask));
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(needMask >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)needMask, 0u));
+    return need ? base + below : 0u;
+}
+
+constexpr uint32_t kNoSlot = 0xffffffffu;
+// A wavefront keeps stepping flights while at least this many lanes fly; below it, the lanes that
+// wait on an event (a collision to store, a miss, a finished path) are served together.
+constexpr uint32_t kFlyQuorum = 16;
+
+// Single-pass transport over a queue of path slots. Every lane is a small state machine that
+// advances one tentative Woodcock collision per flight step (woodcock's loop body, geometry.h)
+// and, at an event step, resolves the flight (absorption test), stores the vertex in its staging
+// chunk, samples the next direction, or finishes its path and takes the next slot from the queue.
+// The random numbers of a path depend on (seed, path id, event) only, so the vertices are those
+// of a thread that traces the path whole, in the same order; the lanes of a wavefront stay busy
+// however unequal their paths' lengths (heavy-tailed: a few paths outlast the rest many times).
 // GB: GaussianBeam illumination (3D), whose first point is drawn from the beam profile
 // (sampleFirstBeam, GaussianBeam.cuh:357-418; a point outside the medium is a miss, no absorption
 // test) and whose second direction comes from the beam's lobe-pair proposal
 // (sampleSecondPoint, :1673-1748).
 template <int DIMS, bool TAB, bool GB>
-__global__ void __launch_bounds__(256) k_transport(MediumView m, ScatterView sc, SourceView il, FramesView fr,
-                                                   VertexBatch vb, uint64_t seed, uint64_t pathBegin,
-                                                   uint32_t npaths, int write) {
-    uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= npaths) return;
-    uint64_t path = pathBegin + slot;
-    uint32_t misses = 0;
-    uint32_t event = 0;
-    f3 x, dir;
-    uint32_t mat = 0, beam = 0;
+__global__ void __launch_bounds__(64) k_transport(MediumView m, ScatterView sc, SourceView il, FramesView fr,
+                                                   VertexBatch vb, TransportStage ts, uint64_t seed,
+                                                   uint64_t pathBegin, uint32_t npaths) {
+    uint32_t slot = kNoSlot;       // the lane's path slot; kNoSlot: take one at the next event step
+    bool dead = false;             // the queue ran dry for this lane
+    bool flying = false;           // a flight is under way
+    bool outside = false;          // the finished flight left the medium
+    bool firstPhase = true;        // drawing the first vertex (attempts)
+    bool chunkOk = false;
+    uint64_t path = 0;
+    uint32_t event = 0, misses = 0, kcount = 0, ev = 0, chunk = 0, beam = 0;
+    uint32_t mat = 0, fmat = 0;
+    f3 x = mk(0.f, 0.f, 0.f), dir = x, o = x, d = x, xx = x;
+    float dist = 0.f;
+    Rng rng(seed, 0, 0, kPurposeTransport);
+
     for (;;) {
-        if (event >= kMaxAttempts) {
-            atomicOr(vb.status + 1, 1u);
-            if (!write) { vb.k[slot] = 0; vb.misses[slot] = misses; }
-            return;
+        const uint64_t live = __ballot(!dead);
+        if (!live) break;
+        const uint32_t nfly = (uint32_t)__popcll(__ballot(!dead && flying));
+        if (nfly == (uint32_t)__popcll(live) || nfly >= kFlyQuorum) {
+            // flight step: one tentative collision of every flying lane
+            if (flying) {
+                dist += -__logf(rng.uniform()) * m.invMaj;
+                xx = mk(fmaf(dist, d.x, o.x), fmaf(dist, d.y, o.y), DIMS == 3 ? fmaf(dist, d.z, o.z) : 0.f);
+                if (!inside<DIMS, float>(m, xx)) {
+                    if (!pointsToward<DIMS>(m, xx, d)) { flying = false; outside = true; }
+                } else {
+                    fmat = materialAt<DIMS>(m, xx);
+                    if (rng.uniform() <= m.accept[fmat]) { flying = false; outside = false; }
+                }
+            }
+            continue;
         }
-        Rng rng(seed, path, event, kPurposeTransport);
-        if constexpr (GB) {
-            const d3 xd = gbSampleFirst(il.g, *sc.tab, il.n, rng, &beam);
-            x = mk((float)xd.x, (float)xd.y, (float)xd.z);
-            dir = mk((float)il.g.dir[3 * beam], (float)il.g.dir[3 * beam + 1], (float)il.g.dir[3 * beam + 2]);
-            mat = gbMaterial(m, xd);
-        } else {
-            f3 x0;
-            sampleSource<DIMS>(il, m, rng, &x0, &dir);
-            mat = woodcock<DIMS>(m, x0, dir, rng, &x);
+        // event step of every lane that is not flying
+        if (dead || flying) continue;
+        bool store = false, startSegment = false, startAttempt = false, needPath = slot == kNoSlot;
+        if (!needPath) {
+            // the flight's end: out of the medium, absorbed, or a scattering vertex
+            const uint32_t nm = outside ? 0u : (rng.uniform() > m.albedo[fmat] ? 0u : fmat);
+            if (nm > 0) {
+                x = xx;
+                dir = d;
+                mat = nm;
+                store = true;
+                startSegment = true;
+                firstPhase = false;
+            } else if (firstPhase) {
+                ++misses;
+                ++event;
+                startAttempt = true;
+            } else {
+                vb.k[slot] = kcount;
+                vb.misses[slot] = misses;
+                slot = kNoSlot;
+                needPath = true;
+            }
         }
-        if (mat > 0) break;
-        ++misses;
-        ++event;
-    }
-    uint32_t base = 0, limit = 0;
-    bool store = false;
-    if (write) {
-        base = vb.off[slot];
-        limit = vb.off[slot + 1] - base;
-        store = (uint64_t)vb.off[npaths] <= vb.cap;
-        if (!store && slot == 0) atomicOr(vb.status, 1u);
-    }
-    uint32_t kcount = 0;
-    uint32_t ev = event;
-    for (uint32_t k = 1;; ++k) {
-        if (k > 1) {
-            ev = kEventVertex | k;
-            Rng rng(seed, path, ev, kPurposeTransport);
+        // the next path slot for lanes without one (one queue atomic per wavefront)
+        const uint64_t want = __ballot(needPath);
+        if (want) {
+            const uint32_t s = waveTake(ts.q, want, needPath);
+            if (needPath) {
+                if (s < npaths) {
+                    slot = s;
+                    path = pathBegin + s;
+                    event = misses = kcount = 0;
+                    firstPhase = true;
+                    startAttempt = true;
+                } else {
+                    dead = true;
+                }
+            }
+        }
+        // first-vertex attempt (sapmleFirstPathPoints, NextEventEstimation.cuh:643-679)
+        if (startAttempt) {
+            bool giveUp = false;
+            if constexpr (GB) {
+                for (;;) {
+                    if (event >= kMaxAttempts) { giveUp = true; break; }
+                    rng = Rng(seed, path, event, kPurposeTransport);
+                    const d3 xd = gbSampleFirst(il.g, *sc.tab, il.n, rng, &beam);
+                    x = mk((float)xd.x, (float)xd.y, (float)xd.z);
+                    dir = mk((float)il.g.dir[3 * beam], (float)il.g.dir[3 * beam + 1], (float)il.g.dir[3 * beam + 2]);
+                    mat = gbMaterial(m, xd);
+                    if (mat > 0) break;
+                    ++misses;
+                    ++event;
+                }
+                if (!giveUp) {
+                    ev = event;
+                    store = true;
+                    startSegment = true;
+                    firstPhase = false;
+                }
+            } else {
+                if (event >= kMaxAttempts) {
+                    giveUp = true;
+                } else {
+                    rng = Rng(seed, path, event, kPurposeTransport);
+                    sampleSource<DIMS>(il, m, rng, &o, &d);
+                    ev = event;
+                    dist = 0.f;
+                    flying = true;
+                }
+            }
+            if (giveUp) {
+                atomicOr(vb.status + 1, 1u);
+                vb.k[slot] = 0;
+                vb.misses[slot] = misses;
+                slot = kNoSlot;
+            }
+        }
+        // store vertex kcount + 1 in the path's staging chunk (a new chunk every kStageChunk)
+        const bool needChunk = store && kcount % kStageChunk == 0;
+        const uint64_t cm = __ballot(needChunk);
+        if (cm) {
+            const uint32_t c = waveTake(ts.q + 1, cm, needChunk);
+            if (needChunk) {
+                chunk = c;
+                chunkOk = c < ts.nchunks;
+                if (chunkOk) {
+                    ts.chunkSlot[c] = slot;
+                    ts.chunkJ0[c] = kcount;
+                }
+            }
+        }
+        if (store) {
+            if (chunkOk) {
+                const uint64_t so = (uint64_t)chunk * kStageChunk + kcount % kStageChunk;
+                ts.xm[so] = make_float4(x.x, x.y, x.z, __uint_as_float(mat));
+                ts.rho[so] = Rng(seed, path, ev, kPurposePhase).uniform();
+                if (!fr.trivial) frameDeltas<DIMS>(fr, seed, path, ev, mat, ts.delta + so * fr.nt * 3);
+            }
+            ++kcount;
+        }
+        // the flight toward vertex kcount + 1 (sapmleNextPathPoints, NextEventEstimation.cuh:681-759)
+        if (startSegment) {
+            ev = kEventVertex | (kcount + 1);
+            rng = Rng(seed, path, ev, kPurposeTransport);
             f3 nd;
             if constexpr (GB) {
-                if (k == 2) nd = gbSampleSecond(il.g, *sc.tab, sc.allIsotropic != 0, beam, mkd(x.x, x.y, x.z), rng);
+                if (kcount == 1) nd = gbSampleSecond(il.g, *sc.tab, sc.allIsotropic != 0, beam, mkd(x.x, x.y, x.z), rng);
                 else nd = sampleDirection<DIMS, TAB>(sc, mat, dir, rng);
             } else {
                 nd = sampleDirection<DIMS, TAB>(sc, mat, dir, rng);
             }
-            f3 nx;
-            uint32_t nm = woodcock<DIMS>(m, x, nd, rng, &nx);
-            if (nm == 0) break;
-            x = nx;
-            dir = nd;
-            mat = nm;
+            o = x;
+            d = nd;
+            dist = 0.f;
+            flying = true;
         }
-        if (store && kcount < limit) {
-            uint64_t o = (uint64_t)base + kcount;
-            vb.xm[o] = make_float4(x.x, x.y, x.z, __uint_as_float(mat));
-            vb.rho[o] = Rng(seed, path, ev, kPurposePhase).uniform();
-            if (!fr.trivial) frameDeltas<DIMS>(fr, seed, path, ev, mat, vb.delta + o * fr.nt * 3);
-        }
-        ++kcount;
     }
-    if (!write) {
-        vb.k[slot] = kcount;
-        vb.misses[slot] = misses;
-    } else if (kcount != limit) {
-        atomicOr(vb.status, 2u);
-    }
+}
+
+// Copies the staged vertices to their places in the vertex pool (k_transport's staging).
+__global__ void __launch_bounds__(256) k_transport_pack(VertexBatch vb, TransportStage ts, uint32_t npaths,
+                                                        uint32_t nt, int frames) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t used = min(ts.q[1], ts.nchunks);
+    if (i >= (uint64_t)used * kStageChunk || (uint64_t)vb.off[npaths] > vb.cap) return;
+    const uint32_t c = (uint32_t)(i / kStageChunk);
+    const uint32_t s = ts.chunkSlot[c];
+    const uint32_t j = ts.chunkJ0[c] + (uint32_t)(i % kStageChunk);
+    if (j >= vb.k[s]) return;
+    const uint64_t o = (uint64_t)vb.off[s] + j;
+    vb.xm[o] = ts.xm[i];
+    vb.rho[o] = ts.rho[i];
+    if (frames)
+        for (uint32_t t = 0; t < 3 * nt; ++t) vb.delta[o * 3 * nt + t] = ts.delta[i * 3 * nt + t];
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------- exclusive scan of k (3 phases)
@@ -216,11 +340,14 @@ __global__ void __launch_bounds__(1024) k_scan_add(uint32_t* out, const uint32_t
 // Per-batch totals: misses, vertex count and the largest batch, in 64-bit counters (one block,
 // launched in stream order after each batch, so the updates never race). A batch whose vertices
 // did not fit the pool contributes nothing (every later kernel sees the overflow and writes zero
-// terms): it is flagged in `overflow` (bit `batch`) for the host to re-run after growing the pool,
-// and its misses and vertices are counted by that re-run instead.
-__global__ void __launch_bounds__(256) k_batch_totals(const uint32_t* misses, const uint32_t* off, uint32_t n,
+// terms): it is flagged in `overflow` (bit `batch`) and status[0] for the host to re-run after
+// growing the pool, and its misses and vertices are counted by that re-run instead. Should the
+// staging have run out (it cannot while the pool holds the batch, see TransportStage), off[n] is
+// set past the pool so that the batch counts as overflowed everywhere.
+__global__ void __launch_bounds__(256) k_batch_totals(const uint32_t* misses, uint32_t* off, uint32_t n,
                                                       unsigned long long* acc, uint64_t cap, uint32_t* overflow,
-                                                      uint32_t batch, float* tauRef) {
+                                                      uint32_t batch, float* tauRef, TransportStage ts,
+                                                      uint32_t* status) {
     if (tauRef && threadIdx.x == 0) *tauRef = 3.0e38f;   // the batch's k_tau_ref lowers it
     __shared__ unsigned long long s[256];
     unsigned long long v = 0;
@@ -232,29 +359,47 @@ __global__ void __launch_bounds__(256) k_batch_totals(const uint32_t* misses, co
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        if ((uint64_t)off[n] <= cap) {
+        const uint32_t total = off[n];
+        const bool staged = ts.q[1] <= ts.nchunks;
+        if ((uint64_t)total <= cap && staged) {
             acc[0] += s[0];
-            acc[1] += off[n];
+            acc[1] += total;
         } else {
             overflow[batch >> 5] |= 1u << (batch & 31);
+            atomicOr(status, 1u);
+            if (!staged) off[n] = 0xffffffffu;
         }
-        if (off[n] > acc[3]) acc[3] = off[n];   // largest batch, sizes the vertex pool
+        if (total > acc[3]) acc[3] = total;   // largest batch, sizes the vertex pool
     }
 }
 
+// Lanes of the transport's persistent grid: enough wavefronts to keep every CU's share of the
+// batch's paths flowing, few enough that each lane traces several paths from the queue.
+constexpr uint32_t kTransportWaves = 256;
+
 hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc, const SourceView& il,
-                           const FramesView& fr, const VertexBatch& vb, uint64_t seed, uint64_t pathBegin,
-                           uint32_t npaths, int write, hipStream_t st) {
-    dim3 grid((npaths + 255) / 256), block(256);
+                           const FramesView& fr, const VertexBatch& vb, const TransportStage& ts, uint64_t seed,
+                           uint64_t pathBegin, uint32_t npaths, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ts.q, 2u);
+    const uint32_t need = (npaths + 63) / 64, waves = need < kTransportWaves ? need : kTransportWaves;
+    dim3 grid(waves), block(64);
     const bool tab = sc.tab != nullptr;
 #define SSTMC_TRANSPORT(D, T, G) \
-    hipLaunchKernelGGL((k_transport<D, T, G>), grid, block, 0, st, m, sc, il, fr, vb, seed, pathBegin, npaths, write)
+    hipLaunchKernelGGL((k_transport<D, T, G>), grid, block, 0, st, m, sc, il, fr, vb, ts, seed, pathBegin, npaths)
     if (il.type == 3 && dims == 3) {
         if (!tab) return hipErrorInvalidValue;
         SSTMC_TRANSPORT(3, true, true);
     } else if (dims == 2) { if (tab) SSTMC_TRANSPORT(2, true, false); else SSTMC_TRANSPORT(2, false, false); }
     else { if (tab) SSTMC_TRANSPORT(3, true, false); else SSTMC_TRANSPORT(3, false, false); }
 #undef SSTMC_TRANSPORT
+    return hipGetLastError();
+}
+
+hipError_t launchTransportPack(const VertexBatch& vb, const TransportStage& ts, uint32_t npaths, uint32_t nt,
+                               int frames, hipStream_t st) {
+    const uint64_t n = (uint64_t)ts.nchunks * kStageChunk;
+    hipLaunchKernelGGL(k_transport_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vb, ts, npaths, nt,
+                       frames);
     return hipGetLastError();
 }
 
@@ -284,9 +429,11 @@ hipError_t launchScan(const uint32_t* k, uint32_t* off, uint32_t* blockSums, uin
     return hipGetLastError();
 }
 
-hipError_t launchBatchTotals(const uint32_t* misses, const uint32_t* off, uint32_t n, unsigned long long* acc,
-                             uint64_t cap, uint32_t* overflow, uint32_t batch, float* tauRef, hipStream_t st) {
-    hipLaunchKernelGGL(k_batch_totals, dim3(1), dim3(256), 0, st, misses, off, n, acc, cap, overflow, batch, tauRef);
+hipError_t launchBatchTotals(const uint32_t* misses, uint32_t* off, uint32_t n, unsigned long long* acc,
+                             uint64_t cap, uint32_t* overflow, uint32_t batch, float* tauRef, const TransportStage& ts,
+                             uint32_t* status, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_totals, dim3(1), dim3(256), 0, st, misses, off, n, acc, cap, overflow, batch, tauRef,
+                       ts, status);
     return hipGetLastError();
 }
 

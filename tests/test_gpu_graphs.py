@@ -35,7 +35,8 @@ def _render(cfg, dims, graphs, seeds=(3, 4)):
     with Tracer(c, dims) as tr:
         for s in seeds:
             u, it = tr.trace(seed=s)
-            out.append((u.copy(), it, tr.timing()["graph_replays"]))
+            t = tr.timing()
+            out.append((u.copy(), it, t["graph_replays"], t["pipeline_sets"]))
     return out
 
 
@@ -49,11 +50,13 @@ def test_graph_replays_equal_eager(case):
     cfg["simulation"] = dict(cfg["simulation"], fullIteration=6, batchSize=min(batch, 256))
     eager = _render(cfg, dims, False)
     graph = _render(cfg, dims, True)
-    for i, ((ue, ie, re), (ug, ig, rg)) in enumerate(zip(eager, graph)):
+    for i, ((ue, ie, re, _), (ug, ig, rg, _)) in enumerate(zip(eager, graph)):
         print(case, "norm", np.linalg.norm(ue), "max diff", np.abs(ue - ug).max(), "replays", rg, flush=True)
         assert ie == ig
         assert np.array_equal(ue, ug)
         assert re == 0
-        # 6 batches over two sets: the first of each set captured in the first rendering
-        assert rg == (4 if i == 0 else 6)
+        # 6 batches over the pipeline's sets: the first of each set captured in the first rendering
+        sets = graph[i][3]
+        assert sets in (2, 3)
+        assert rg == (6 - sets if i == 0 else 6)
     assert not np.array_equal(eager[0][0], eager[1][0])   # the two seeds differ
