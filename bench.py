@@ -39,7 +39,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0
-VALU_PEAK_LANE_INSTR = 157.3e12 / 2       # published FP32 vector peak in lane-FMAs (2 flops each) per s
+# VALU issue peak for the unpacked instructions this library's kernels issue (packed FP32 is
+# disabled): a wave64 instruction occupies a 16-lane SIMD for 4 cycles, so 256 CUs x 4 SIMDs x 16
+# lanes x 2.4 GHz lane-instructions per second. MI355X's published 157.3 TFLOP/s FP32 vector
+# figure counts packed FMAs (2 lanes' worth per lane, 2 flops each): 4x this in flops, 2x in FMAs.
+# A microbenchmark of independent FP32 FMA chains issues 0.23 wave64 instructions per SIMD-cycle.
+VALU_PEAK_LANE_INSTR = 256 * 4 * 16 * 2.4e9
 METRIC = "path-contributions/sec (L×V complex) at err≤1e-3 vs FP64; p50/p99 render ms"
 
 
@@ -200,7 +205,8 @@ def serving(n_requests: int, max_paths_log2: int, prec: dict):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=None, help="batches timed (default: the workload's full path "
+                    "count for tm1024, 256 x 65536 = 2^24 paths; 8 for the others)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=1 << 16)
     ap.add_argument("--workload", default="tm1024")
@@ -215,6 +221,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2, help="batches timed for each plain (no split "
                     "precision, no cut) variant of the same workload (0: skip)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = (1 << 24) // args.batch if args.workload == "tm1024" else 8
     if args.gpus != 1:
         print("bench.py: this build runs and measures one GPU; paths would shard over GPUs by path-id range "
               "with one all-reduce of the field, which is not implemented, so --gpus %d is not measured"
@@ -268,8 +276,7 @@ def main():
             "algorithmic_bytes_per_launch": alg_bytes, "voxel_steps_per_s": steps_s,
             "limiter": "valu issue (the walk's grid reads hit L2; see valu)"}
     # the walk is bound by VALU issue, not HBM: VALU lane-instructions per voxel step (SQ_INSTS_VALU
-    # over the same kernel, inactive lanes included) against MI355X's published FP32 vector peak
-    # counted in lane-instructions (157.3 TFLOP/s / 2 flops per FMA)
+    # over the same kernel, inactive lanes included) against the unpacked VALU issue peak
     if same_form and "view_valu_lane_instr_per_step" in prof:
         ips = prof["view_valu_lane_instr_per_step"]
         ach = ips * steps_s

@@ -278,7 +278,7 @@ template <int DIMS, typename R, bool TAB, bool GB = false>
 __device__ inline void nee_path(const MediumView& m, const R* sig, const ScatterView& sc, const SourceView& il,
                                 const SourceView& vw,
                                 const FramesView& fr, const VertexBatch& vb, const NeeParams& prm, const NeeOut& out,
-                                uint32_t p, uint32_t sub, uint32_t lpp, uint32_t* stepsOut) {
+                                uint32_t p, uint32_t sub, uint32_t lpp, uint32_t* stepsOut, float* tauCache) {
     typedef typename Cx<R>::T C;
     const uint32_t P = prm.npaths, L = il.n, V = vw.n, NT = fr.nt;
     const bool corr = prm.output == 1;
@@ -354,6 +354,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
             if (sizeof(R) == 4) w = Pos<R>::fromD(mkd(-(double)il.pf[3 * l], -(double)il.pf[3 * l + 1], -(double)il.pf[3 * l + 2]));
             else w = Pos<R>::fromD(mkd(-il.pd[3 * l], -il.pd[3 * l + 1], -il.pd[3 * l + 2]));
             R tau = walk<DIMS, R>(m, sig, p1, w, sizeof(R) == 4 ? (R)prm.exactCutF : tauInf, &steps);
+            if (tauCache) tauCache[l] = (float)tau;
             pr = expR<R>(-tau) * (sizeof(R) == 4 ? (R)il.probf[l] : (R)il.probd[l]);
         } else {
             d3 sp = mkd(il.pd[3 * l], il.pd[3 * l + 1], DIMS == 3 ? il.pd[3 * l + 2] : 0.0);
@@ -373,6 +374,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
     }
     s1 = groupSum(s1, lpp);
     s2 = groupSum(s2, lpp);
+    if (tauCache) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // the group's cached walks
     const R p1 = s1 / (R)L, p2 = s2 / (R)L;
     const R w1 = p1 > 0 ? (R)1 / sqrtR<R>(p1) : (R)0;
     const R w2 = p2 > 0 ? (R)1 / sqrtR<R>(p2) : (R)0;
@@ -387,8 +389,16 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         for (int f = 0; f < nf; ++f) {
             uint32_t tt = (corr && f == 0) ? 0 : t;
             Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
-            gTerm<DIMS, R, gauss>(m, sig, il, l, x1t, invLambda, tauExact, &steps, &ga[f],
-                                  &gp[f]);
+            if (tauCache) {
+                // the probability loop's walk from x1 toward illumination l (gFar, the same FP32
+                // start, direction and cut: frames are static, so x1t is x1)
+                const R tau = (R)tauCache[l];
+                const double phase = ddot<DIMS>(sourceD<DIMS>(il, l), x1t.pd) * invLambda;
+                gp[f] = il.illum != 0 ? phase : -phase;
+                ga[f] = tau > tauExact ? (R)0 : expR<R>(-(R)0.5 * tau);
+            } else {
+                gTerm<DIMS, R, gauss>(m, sig, il, l, x1t, invLambda, tauExact, &steps, &ga[f], &gp[f]);
+            }
             if (multi) {
                 Geo x2t = vertexFrame(vb, fr, off + 1, tt, x2m);
                 const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
@@ -458,6 +468,9 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
     *stepsOut += steps;
 }
 
+// Illuminations whose optical depth from x1 k_nee keeps per 64-lane path (16 KiB of LDS a block).
+constexpr uint32_t kTauCacheL = 1024;
+
 // GB: GaussianBeam sources (their FP64 beam functions kept out of the other instantiations)
 template <int DIMS, typename R, bool TAB, bool GB>
 __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, SourceView il, SourceView vw, FramesView fr,
@@ -468,9 +481,18 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
     const uint32_t lane = threadIdx.x & 63, lpp = prm.lpp, sub = lane & (lpp - 1);
     const uint32_t p = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / lpp) + lane / lpp;
     __shared__ R sigLds[kSigLds];
+    // far-field illuminations, FP32 terms, field output, static frames: each illumination's walk
+    // from x1 is kept from the path probabilities for the illumination terms (kTauCacheL per
+    // 64-lane path; smaller groups hold at most lpp illuminations)
+    __shared__ float tauLds[4 * kTauCacheL];
+    const uint32_t cap = lpp == 64 ? kTauCacheL : lpp;
+    float* tauCache = (sizeof(R) == 4 && !GB && il.type == 2 && prm.output == 0 && fr.trivial && il.n <= cap)
+                          ? tauLds + (threadIdx.x / lpp) * cap
+                          : nullptr;
     const R* sig = loadSigTable<R>(m, sigLds);
     uint32_t steps = 0;
-    if (p < prm.npaths) nee_path<DIMS, R, TAB, GB>(m, sig, sc, il, vw, fr, vb, prm, out, p, sub, lpp, &steps);
+    if (p < prm.npaths)
+        nee_path<DIMS, R, TAB, GB>(m, sig, sc, il, vw, fr, vb, prm, out, p, sub, lpp, &steps, tauCache);
     uint32_t tot = waveSum(steps);
     if (lane == 0 && out.steps) atomicAdd(out.steps, (unsigned long long)tot);
 }
