@@ -74,6 +74,27 @@ struct Rng {
     }
 };
 
+// Hands every lane in `need` (a ballot taken by the calling lanes) its own consecutive index from
+// *ctr with one atomic per wavefront; lanes outside `need` get 0.
+__device__ inline uint32_t waveTake(uint32_t* ctr, uint64_t needMask, bool need) {
+    const int leader = __builtin_ctzll(needMask);
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(needMask));
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(needMask >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)needMask, 0u));
+    return need ? base + below : 0u;
+}
+
+// Diagnostic counters spread over kCountSlots lines of 128 bytes: a wavefront adds to the slot of
+// its workgroup, so that a launch of a million wavefronts does not queue on one address
+// (device-scope atomics to one address complete one at a time, ~10 ns each); the host sums the
+// slots. A counter set is kCountSlots * kCountStride values.
+constexpr uint32_t kCountSlots = 256, kCountStride = 16;
+__device__ inline void slotAdd(unsigned long long* counter, unsigned long long v) {
+    atomicAdd(counter + (size_t)(blockIdx.x & (kCountSlots - 1)) * kCountStride, v);
+}
+
 // ---------------------------------------------------------------- small vectors
 struct f3 { float x, y, z; };
 struct d3 { double x, y, z; };
@@ -276,8 +297,9 @@ struct __align__(32) VertexRec {
 // forms them), the walk-grid bit offset of the cell (kFarOutside for a position outside the box,
 // whose optical depth is 0); the material; the back direction rounded to FP32; the FP64 position
 // and phase base of VertexRec.
-// lb: lower bounds on the optical depth of the walks from the vertex along every direction of the
-// views (lb[0]) and of the reversed paths' illuminations (lb[1]), 0 when none applies (k_far_bound).
+// live: per role (views 0, the reversed paths' illuminations 1), bit b set when block b of the
+// far-field view kernel's sources (FarBound::perm) may hold a term inside the launch's cut, bit 31
+// standing for every block from 31 on; all set when no bound applies (k_far_bound).
 constexpr uint32_t kFarOutside = 0xffffffffu;
 struct __align__(32) FarRec {
     float dlo[3];
@@ -286,7 +308,7 @@ struct __align__(32) FarRec {
     uint32_t mat;
     float b[3];
     float back[3];
-    float lb[2];
+    uint32_t live[2];
     double px, py, pz;
     double phase;
 };
@@ -313,10 +335,20 @@ struct BoundDirs {
     float rmin[2] = {0.f, 0.f}, rmax[2] = {0.f, 0.f};
 };
 // The bounds of the far-field view kernel's two launches: views (0) and, for coherent
-// backscattering, the reversed paths' illuminations (1).
+// backscattering, the reversed paths' illuminations (1). Per role the kernel's sources are taken
+// in the order perm (a Z-order over their lateral slopes, so a wavefront's 64 sources form a compact
+// cone), and blockDirs[b] is the direction set of sources perm[64 b .. 64 b + 63] about the
+// pyramid's axis, for a lower bound per (vertex, block) tighter than the whole set's.
 struct FarBound {
     MinPyramid pyr[2];
     BoundDirs dirs[2];
+    const uint32_t* perm[2] = {nullptr, nullptr};
+    const BoundDirs* blockDirs[2] = {nullptr, nullptr};
+    uint32_t blocks[2] = {0u, 0u};
+    // records the whole set's bound leaves alive, as record * 2 + role, for the per-block pass
+    uint32_t* cand = nullptr;
+    uint32_t* candCount = nullptr;
+    uint64_t candCap = 0;
 };
 
 struct VertexBatch {

@@ -2,6 +2,9 @@
 // Device functions templated on the real type R (float for transport and the FP32 NEE
 // variant, double for the FP64 NEE variant) and on DIMS (2 or 3).
 #pragma once
+#include <algorithm>
+#include <cmath>
+#include <utility>
 #include <vector>
 
 #include "common.h"
@@ -541,7 +544,7 @@ __host__ __device__ inline float walkFastCore(const MediumView& m, const float* 
 }
 
 // The skip threshold of the far-field view kernel for a walk cut `cut`: a vertex whose lower bound
-// (FarRec::lb) exceeds it has every term of the launch past the cut, whatever the FP32 walk's
+// over a set of sources exceeds it has every term of those sources past the cut, whatever the FP32 walk's
 // rounding (a relative 1e-3 margin over the walk's and the bound's own rounding), so skipping it
 // leaves the terms as the walks would: zero.
 __host__ __device__ inline float farSkip(float cut) { return cut + 1e-3f * (1.f + cut); }
@@ -599,6 +602,71 @@ __host__ __device__ inline float farLowerBound(const MediumView& m, const MinPyr
     return lb * (1.f - 1e-5f);
 }
 
+
+// The order in which the far-field view kernel takes n sources (FP32 directions, 3 each) walked
+// along sg * d: by Morton code of the ranks of their lateral slopes about `axis`, so that each run
+// of 64 is a compact cone (a regular 32 x 32 grid gives 8 x 8 squares).
+inline std::vector<uint32_t> farOrderHost(const float* d, size_t n, double sg, int axis) {
+    const int qs[2] = {axis == 0 ? 1 : 0, axis == 2 ? 1 : 2};
+    std::vector<double> r[2];
+    std::vector<uint32_t> rank[2];
+    for (int j = 0; j < 2; ++j) {
+        r[j].resize(n);
+        for (size_t i = 0; i < n; ++i)
+            r[j][i] = sg * (double)d[3 * i + qs[j]] / std::fabs((double)d[3 * i + axis] + 1e-300);
+        std::vector<double> u(r[j]);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end(), [](double x, double y) { return std::fabs(x - y) <= 1e-9 * (1.0 + std::fabs(x)); }), u.end());
+        rank[j].resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            const size_t k = std::lower_bound(u.begin(), u.end(), r[j][i] - 1e-9 * (1.0 + std::fabs(r[j][i]))) - u.begin();
+            rank[j][i] = (uint32_t)(k < u.size() ? k : u.size() - 1);
+        }
+    }
+    auto spread = [](uint64_t x) {
+        x &= 0xffffffffull;
+        x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+        x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+        x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+        x = (x | (x << 2)) & 0x3333333333333333ull;
+        x = (x | (x << 1)) & 0x5555555555555555ull;
+        return x;
+    };
+    std::vector<std::pair<uint64_t, uint32_t>> key(n);
+    for (size_t i = 0; i < n; ++i) key[i] = {spread(rank[0][i]) | (spread(rank[1][i]) << 1), (uint32_t)i};
+    std::stable_sort(key.begin(), key.end(), [](const std::pair<uint64_t, uint32_t>& x, const std::pair<uint64_t, uint32_t>& y) { return x.first < y.first; });
+    std::vector<uint32_t> perm(n);
+    for (size_t i = 0; i < n; ++i) perm[i] = key[i].second;
+    return perm;
+}
+
+// boundDirsHost for a subset of directions about a given axis and sense (a block of the kernel's
+// sources about the whole set's pyramid axis): valid = 0 when some walk advances less than 0.1
+// along it.
+inline BoundDirs boundDirsAxisHost(const float* d, const uint32_t* idx, size_t n, double sg, int axis, int sgn) {
+    BoundDirs bd;
+    double mn = 1e300;
+    for (size_t i = 0; i < n; ++i) {
+        const double c = sgn * sg * (double)d[3 * idx[i] + axis];
+        mn = c < mn ? c : mn;
+    }
+    if (!(mn >= 0.1)) return bd;
+    bd.valid = 1;
+    bd.sgn = sgn;
+    const int qs[2] = {axis == 0 ? 1 : 0, axis == 2 ? 1 : 2};
+    for (int j = 0; j < 2; ++j) {
+        double lo = 1e300, hi = -1e300;
+        for (size_t i = 0; i < n; ++i) {
+            const double da = std::fabs((double)d[3 * idx[i] + axis]);
+            const double r = sg * (double)d[3 * idx[i] + qs[j]] / da;
+            lo = r < lo ? r : lo;
+            hi = r > hi ? r : hi;
+        }
+        bd.rmin[j] = (float)(lo - 1e-6 * (1.0 + std::fabs(lo)));
+        bd.rmax[j] = (float)(hi + 1e-6 * (1.0 + std::fabs(hi)));
+    }
+    return bd;
+}
 
 // Host builders of the bound's inputs. minPyramidHost: the pyramid of a grid of material ids
 // (x fastest) with layers along `axis`, FP32 sigt per id; mp's offsets and sizes filled, mp->p left

@@ -19,8 +19,8 @@ struct NeeOut {
     // coherent backscattering (null when off): the reversed path v -> x1 -> ... -> xK -> l
     void* Ar;         // [t][p][v] view connected at x1: g_v(x1) f_v(x1 -> x2) w2
     void* Br;         // [t][p][l] illumination connected at x_k, k >= 2, summed with the path phase
-    unsigned long long* steps;   // voxel steps of the NEE walks (accumulated): k_nee adds to
-                                 // steps[0], k_nee_view to steps[2]
+    unsigned long long* steps;   // voxel steps of the NEE walks (accumulated, slotAdd): k_nee adds to
+                                 // the counter set at steps, the view kernels to the one after it
     // TemporalCorrelationSampler with coherent backscattering (null otherwise; Ar then holds
     // Gv0 conj(u2Gv0) C4_v1v2 and Br sum_k Gl conj(u2Gl) C4_l1l2): the first vertex's u1 and u2
     // terms of either source kept apart, F1l / F2l [t][p][l] (u1 with the path weight), F1v / F2v
@@ -79,8 +79,9 @@ hipError_t launchBatchTotals(const uint32_t* misses, uint32_t* off, uint32_t n, 
                              uint32_t* status, hipStream_t st);
 
 // The NEE stage in two parts. kNeeVertexPart reads the batch's vertices only: k_tau_ref (when
-// p.tauRef is set), k_vertex_prep (per-vertex records) and, with vb.far and fb, k_far_bound
-// (lower bounds of every vertex's far-field walks). kNeeTermPart forms the terms: k_nee (or
+// p.tauRef is set), k_vertex_prep (per-vertex records) and, with vb.far and fb, k_far_bound and
+// k_far_blocks (which blocks of the far-field view kernel's sources each vertex's walks can reach
+// inside the cut, from lower bounds on their optical depth). kNeeTermPart forms the terms: k_nee (or
 // k_nee_corr) for the first vertex, then the view kernels for vertices k >= 2; `between`, when not
 // null, is recorded on `st` between the two.
 enum : int { kNeeVertexPart = 1, kNeeTermPart = 2, kNeeAll = 3 };
@@ -88,11 +89,6 @@ hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView&
                      const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
                      const NeeOut& out, hipStream_t st, hipEvent_t between = nullptr, const FarBound* fb = nullptr,
                      int parts = kNeeAll);
-// `between`, when not null, is recorded on `st` between the two launches. With vb.far and fb,
-// k_far_bound bounds every vertex's far-field walks from below before the view kernels run.
-hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
-                     const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
-                     const NeeOut& out, hipStream_t st, hipEvent_t between = nullptr, const FarBound* fb = nullptr);
 
 // TemporalCorrelationSampler with coherent backscattering: the mixed pairings CBS02 + CBS03 of
 // every path with at least two vertices, from the NeeOut first-vertex (F*) and per-vertex (G*)

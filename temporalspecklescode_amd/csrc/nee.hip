@@ -494,7 +494,7 @@ __global__ void __launch_bounds__(256) k_nee(MediumView m, ScatterView sc, Sourc
     if (p < prm.npaths)
         nee_path<DIMS, R, TAB, GB>(m, sig, sc, il, vw, fr, vb, prm, out, p, sub, lpp, &steps, tauCache);
     uint32_t tot = waveSum(steps);
-    if (lane == 0 && out.steps) atomicAdd(out.steps, (unsigned long long)tot);
+    if (lane == 0 && out.steps) slotAdd(out.steps, tot);
 }
 
 
@@ -688,7 +688,7 @@ __global__ void __launch_bounds__(256) k_nee_corr(MediumView m, ScatterView sc, 
     uint32_t steps = 0;
     if (p < prm.npaths) nee_path_corr<DIMS, R, TAB>(m, sig, sc, il, vw, fr, vb, prm, out, p, sub, lpp, &steps);
     uint32_t tot = waveSum(steps);
-    if (lane == 0 && out.steps) atomicAdd(out.steps, (unsigned long long)tot);
+    if (lane == 0 && out.steps) slotAdd(out.steps, tot);
 }
 
 // Vertex records for the view terms: one thread per path walks its vertices once per frame and
@@ -743,7 +743,7 @@ __global__ void __launch_bounds__(256) k_vertex_prep(MediumView m, VertexBatch v
                 f.back[0] = (float)b.x;
                 f.back[1] = (float)b.y;
                 f.back[2] = (float)b.z;
-                f.lb[0] = f.lb[1] = 0.f;
+                f.live[0] = f.live[1] = 0xffffffffu;
                 f.px = xk.pd.x;
                 f.py = xk.pd.y;
                 f.pz = xk.pd.z;
@@ -887,13 +887,23 @@ k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBat
         B[((size_t)t * P + p) * V + v] = Cx<R>::make(accRe, accIm);
     }
     uint32_t tot = waveSum(steps);
-    if ((threadIdx.x & 63) == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
+    if ((threadIdx.x & 63) == 0 && out.steps) slotAdd(out.steps + kCountSlots * kCountStride, tot);
 }
 
-// Lower bounds of the far-field view kernel's walks (FarRec::lb) for every record of the batch:
-// lb[0] for the views under the launch's cut (viewCut), lb[1] for the reversed paths'
-// illuminations under the exact cut. One thread per record; records of first vertices, which no
-// kernel reads, are bounded too (from whatever they hold; every index is clamped).
+// The live-block masks of the far-field view kernel (FarRec::live) for every record of the batch:
+// live[0] for the views under the launch's cut (viewCut), live[1] for the reversed paths'
+// illuminations under the exact cut; a block is dead when the lower bound of its sources' walks
+// (farLowerBound over the block's direction set) passes the cut's skip threshold. k_far_bound,
+// one thread per record, tries the whole set's bound, which kills every block at once; a record it
+// leaves alive goes to a candidate list, and k_far_blocks bounds each (candidate, block) pair in a
+// thread of its own and sets the pair's bit. Records of first vertices, which no kernel reads, are
+// bounded too (from whatever they hold; every index is clamped).
+__device__ inline float farRoleSkip(const NeeParams& prm, int role) {
+    return role == 0 ? farSkip(viewCut<float>(prm)) : farSkip(prm.exactCutF);
+}
+__global__ void k_far_reset(uint32_t* count) {
+    if (threadIdx.x == 0) *count = 0u;
+}
 __global__ void __launch_bounds__(256) k_far_bound(MediumView m, VertexBatch vb, FramesView fr, NeeParams prm,
                                                    FarBound fb, uint32_t doIllum) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -901,8 +911,40 @@ __global__ void __launch_bounds__(256) k_far_bound(MediumView m, VertexBatch vb,
     if ((uint64_t)vb.off[P] > vb.cap || i >= (uint64_t)vb.off[P] * fr.nt) return;
     FarRec& r = vb.far[i];
     const f3 pf = mk((float)r.px, (float)r.py, (float)r.pz);
-    r.lb[0] = farLowerBound(m, fb.pyr[0], fb.dirs[0], pf, farSkip(viewCut<float>(prm)));
-    r.lb[1] = doIllum ? farLowerBound(m, fb.pyr[1], fb.dirs[1], pf, farSkip(prm.exactCutF)) : 0.f;
+#pragma unroll
+    for (int role = 0; role < 2; ++role) {
+        bool cand = false;
+        uint32_t live = 0xffffffffu;
+        if (role == 0 || doIllum) {
+            const float skip = farRoleSkip(prm, role);
+            if (farLowerBound(m, fb.pyr[role], fb.dirs[role], pf, skip) > skip) live = 0u;
+            else if (fb.blockDirs[role]) {
+                live = 0u;
+                cand = true;
+            }
+        }
+        r.live[role] = live;
+        const uint64_t cm = __ballot(cand);
+        if (cm) {
+            const uint32_t slot = waveTake(fb.candCount, cm, cand);
+            if (cand && slot < fb.candCap) fb.cand[slot] = (uint32_t)i * 2u + (uint32_t)role;
+        }
+    }
+}
+__global__ void __launch_bounds__(256) k_far_blocks(MediumView m, VertexBatch vb, NeeParams prm, FarBound fb) {
+    const uint32_t P = prm.npaths;
+    if ((uint64_t)vb.off[P] > vb.cap) return;
+    const uint64_t nc = min((uint64_t)*fb.candCount, fb.candCap);
+    const uint32_t nb = fb.blocks[0] > fb.blocks[1] ? fb.blocks[0] : fb.blocks[1];
+    for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < nc * nb; j += (uint64_t)gridDim.x * 256) {
+        const uint32_t e = fb.cand[j / nb], b = (uint32_t)(j % nb), role = e & 1u;
+        if (b >= fb.blocks[role]) continue;
+        FarRec& r = vb.far[e >> 1];
+        const f3 pf = mk((float)r.px, (float)r.py, (float)r.pz);
+        const float skip = farRoleSkip(prm, (int)role);
+        if (!(farLowerBound(m, fb.pyr[role], fb.blockDirs[role][b], pf, skip) > skip))
+            atomicOr(&r.live[role], 1u << (b < 31u ? b : 31u));
+    }
 }
 
 // HG amplitude constants per material for the far-field view kernel: sqrt((1 - g^2) / 4 pi),
@@ -928,9 +970,11 @@ struct HgConst {
 //   * a vertex whose every lane's term is zero (past the attenuation cut, or outside) skips the
 //     amplitude and phase.
 template <int SRC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
-k_nee_view_far(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBatch vb, NeeParams prm, NeeOut out) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+k_nee_view_far(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBatch vb, NeeParams prm, NeeOut out,
+               FarBound fb) {
     static_assert(SRC == kSrcFarView || SRC == kSrcFarIllum, "far-field sources only");
+    constexpr int role = SRC == kSrcFarIllum ? 1 : 0;
     __shared__ float sigLds[kSigLds];
     __shared__ HgConst hg[kMaxMaterials];
     if (threadIdx.x < kMaxMaterials) {
@@ -947,8 +991,11 @@ k_nee_view_far(MediumView m, ScatterView sc, SourceView vw, FramesView fr, Verte
     const float tauCut = viewCut<float>(prm);
     float2* B = (float2*)out.B;
     uint32_t steps = 0;
-    const uint32_t v = u.blk * 64u + u.sub;
-    if (v < V && K >= 2) {
+    // this lane's source: position blk * 64 + lane of the kernel's order (a compact cone per wavefront)
+    const uint32_t pos = u.blk * 64u + u.sub;
+    const bool act = pos < V;
+    const uint32_t v = act ? (fb.perm[role] ? fb.perm[role][pos] : pos) : 0u;
+    if (K >= 2 && u.blk * 64u < V) {
         const d3 vd = sourceD<3>(vw, v);
         const f3 vf = sourceF(vw, v);
         // walk along the view direction (view) or against it (illumination); the phase is
@@ -961,32 +1008,41 @@ k_nee_view_far(MediumView m, ScatterView sc, SourceView vw, FramesView fr, Verte
         const float as = SRC == kSrcFarIllum ? 1.f : -1.f;
         const f3 av = mk(as * (float)vd.x, as * (float)vd.y, as * (float)vd.z);
         float accRe = 0.f, accIm = 0.f;
-        const float skip = farSkip(tauCut);
-        for (uint32_t k = 1; k < K; ++k) {
-            const FarRec* rp = vb.far + (off + k) * fr.nt + t;
-            if (rp->lb[SRC == kSrcFarIllum ? 1 : 0] > skip) continue;   // every term past the cut
-            const FarRec rec = *rp;
-            const float tau = rec.cell == kFarOutside
-                                  ? 0.f
-                                  : walkFromStartLane<3>(m, sig + kMaxMaterials, rec.dlo, rec.dhi, rec.b, rec.cell,
-                                                         wl, tauCut, &steps);
-            const float amp = tau > tauCut ? 0.f : __expf(-0.5f * tau);
-            if (__ballot(amp != 0.f) == 0) continue;   // every term of this vertex is zero
-            const HgConst h = hg[rec.mat];
-            const float dx = av.x - rec.back[0], dy = av.y - rec.back[1], dz = av.z - rec.back[2];
-            const float den = fmaf(h.b, 0.5f * fmaf(dx, dx, fmaf(dy, dy, dz * dz)), h.a);
-            const float r = __builtin_amdgcn_rsqf(den);
-            const float fv = h.c * r * __builtin_amdgcn_sqrtf(r);
-            const double ph = kd.x * rec.px + kd.y * rec.py + kd.z * rec.pz + rec.phase;
-            const float turns = (float)(ph - floor(ph));
-            const float a = amp * fv;
-            accRe = fmaf(a, __builtin_amdgcn_cosf(turns), accRe);
-            accIm = fmaf(a, __builtin_amdgcn_sinf(turns), accIm);
+        // vertices in runs of 64: every lane reads one vertex's live-block mask, and the run's
+        // vertices whose block is alive are visited in increasing k (the same order and sums as a
+        // visit of every vertex)
+        const uint32_t bsel = u.blk < 31u ? u.blk : 31u;
+        for (uint32_t c0 = 1; c0 < K; c0 += 64) {
+            const uint32_t kl = c0 + u.sub;
+            const uint32_t bits = kl < K ? vb.far[(off + kl) * fr.nt + t].live[role] : 0u;
+            uint64_t alive = __ballot((bits >> bsel) & 1u);
+            while (alive) {
+                const uint32_t k = c0 + (uint32_t)__builtin_ctzll(alive);
+                alive &= alive - 1;
+                const FarRec rec = vb.far[(off + k) * fr.nt + t];
+                float tau = 3.0e38f;
+                if (act)
+                    tau = rec.cell == kFarOutside ? 0.f
+                                                  : walkFromStartLane<3>(m, sig + kMaxMaterials, rec.dlo, rec.dhi, rec.b,
+                                                                         rec.cell, wl, tauCut, &steps);
+                const float amp = tau > tauCut ? 0.f : __expf(-0.5f * tau);
+                if (__ballot(amp != 0.f) == 0) continue;   // every term of this vertex is zero
+                const HgConst h = hg[rec.mat];
+                const float dx = av.x - rec.back[0], dy = av.y - rec.back[1], dz = av.z - rec.back[2];
+                const float den = fmaf(h.b, 0.5f * fmaf(dx, dx, fmaf(dy, dy, dz * dz)), h.a);
+                const float r = __builtin_amdgcn_rsqf(den);
+                const float fv = h.c * r * __builtin_amdgcn_sqrtf(r);
+                const double ph = kd.x * rec.px + kd.y * rec.py + kd.z * rec.pz + rec.phase;
+                const float turns = (float)(ph - floor(ph));
+                const float a = amp * fv;
+                accRe = fmaf(a, __builtin_amdgcn_cosf(turns), accRe);
+                accIm = fmaf(a, __builtin_amdgcn_sinf(turns), accIm);
+            }
         }
-        B[((size_t)t * P + p) * V + v] = make_float2(accRe, accIm);
+        if (act) B[((size_t)t * P + p) * V + v] = make_float2(accRe, accIm);
     }
     uint32_t tot = waveSum(steps);
-    if ((threadIdx.x & 63) == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
+    if ((threadIdx.x & 63) == 0 && out.steps) slotAdd(out.steps + kCountSlots * kCountStride, tot);
 }
 
 // Correlation view terms sum_k G_k(0) conj G_k(t) (the per-sub-path estimator). Work unit =
@@ -1038,7 +1094,7 @@ __global__ void __launch_bounds__(256) k_nee_view_corr(MediumView m, ScatterView
         for (uint32_t j = 0; j < nt; ++j) B[((size_t)(t0 + j) * P + p) * V + v] = acc[j][threadIdx.x];
     }
     uint32_t tot = waveSum(steps);
-    if ((threadIdx.x & 63) == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
+    if ((threadIdx.x & 63) == 0 && out.steps) slotAdd(out.steps + kCountSlots * kCountStride, tot);
 }
 
 // One view term of the correlation sampler: g of the source at xg times the amplitude at pf
@@ -1196,7 +1252,7 @@ __global__ void __launch_bounds__(256) k_nee_view_corrsampler(MediumView m, Scat
         for (uint32_t j = 0; j < nt; ++j) B[((size_t)(t0 + j) * P + p) * V + v] = acc[j][threadIdx.x];
     }
     uint32_t tot = waveSum(steps);
-    if ((threadIdx.x & 63) == 0 && out.steps) atomicAdd(out.steps + 2, (unsigned long long)tot);
+    if ((threadIdx.x & 63) == 0 && out.steps) slotAdd(out.steps + kCountSlots * kCountStride, tot);
 }
 
 // Mixed pairings of coherent backscattering under the TemporalCorrelationSampler
@@ -1429,7 +1485,7 @@ static inline void viewLanes(uint32_t n, NeeParams* q) {
 template <int DIMS, typename R, int SRC>
 static void launchView(bool corr, const MediumView& m, const ScatterView& sc, const SourceView& src,
                        const FramesView& fr, const VertexBatch& vb, const NeeParams& q, const NeeOut& out,
-                       hipStream_t st) {
+                       hipStream_t st, const FarBound* fb = nullptr) {
     const uint64_t units = (uint64_t)q.npaths * q.viewBlocks * q.frameChunks, upb = 4 * (64 / q.lppView);
     const dim3 grid((uint32_t)((units + upb - 1) / upb)), block(256);
     const bool wave = q.lppView == 64, tab = sc.tab != nullptr;
@@ -1448,7 +1504,8 @@ static void launchView(bool corr, const MediumView& m, const ScatterView& sc, co
     } else if constexpr (DIMS == 3 && sizeof(R) == 4 && (SRC == kSrcFarView || SRC == kSrcFarIllum)) {
         // far-field sources, FP32 terms: the specialised kernel when its preconditions hold
         if (wave && !tab && m.walkFast && vb.far)
-            hipLaunchKernelGGL((k_nee_view_far<SRC>), grid, block, 0, st, m, sc, src, fr, vb, q, out);
+            hipLaunchKernelGGL((k_nee_view_far<SRC>), grid, block, 0, st, m, sc, src, fr, vb, q, out,
+                               fb ? *fb : FarBound{});
         else if (wave) { if (tab) SSTMC_VIEW(k_nee_view, true, true); else SSTMC_VIEW(k_nee_view, true, false); }
         else { if (tab) SSTMC_VIEW(k_nee_view, false, true); else SSTMC_VIEW(k_nee_view, false, false); }
     } else if (wave) {
@@ -1490,8 +1547,11 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
                            1.0 / p.lambda);
         if (DIMS == 3 && !fr.analytic && vb.far && fb) {
             const uint64_t n = vb.cap * fr.nt;
+            hipLaunchKernelGGL(k_far_reset, dim3(1), dim3(64), 0, st, fb->candCount);
             hipLaunchKernelGGL(k_far_bound, dim3((uint32_t)((n + 255) / 256)), block, 0, st, m, vb, fr, p, *fb,
                                out.Br ? 1u : 0u);
+            if (fb->blockDirs[0] || fb->blockDirs[1])
+                hipLaunchKernelGGL(k_far_blocks, dim3(2048), block, 0, st, m, vb, p, *fb);
         }
     }
     if (!(parts & kNeeTermPart)) return;
@@ -1525,7 +1585,7 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
     const bool corr = p.output == 1;
     if (vw.type == 3) {
         launchView<DIMS, R, kSrcGauss>(corr, m, sc, vw, fr, vb, p, out, st);
-    } else if (vw.type == 2) launchView<DIMS, R, kSrcFarView>(corr, m, sc, vw, fr, vb, p, out, st);
+    } else if (vw.type == 2) launchView<DIMS, R, kSrcFarView>(corr, m, sc, vw, fr, vb, p, out, st, fb);
     else launchView<DIMS, R, kSrcPoint>(corr, m, sc, vw, fr, vb, p, out, st);
     if (out.Br) {
         // coherent backscattering: the same per-vertex sums with the illumination connected at
@@ -1536,7 +1596,7 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
         q.tauRef = nullptr;   // the reversed path's Br stays exact
         NeeOut o = out;
         o.B = out.Br;
-        if (il.type == 2) launchView<DIMS, R, kSrcFarIllum>(false, m, sc, il, fr, vb, q, o, st);
+        if (il.type == 2) launchView<DIMS, R, kSrcFarIllum>(false, m, sc, il, fr, vb, q, o, st, fb);
         else launchView<DIMS, R, kSrcPoint>(false, m, sc, il, fr, vb, q, o, st);
     }
 }

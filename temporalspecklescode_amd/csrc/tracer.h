@@ -347,6 +347,10 @@ private:
     bool cbsCorr_ = false;          // coherent backscattering under the correlation sampler
     bool farRecs_ = false;          // the far-field view kernel runs: FarRec per (vertex, frame)
     FarBound farBound_{};           // its per-vertex lower bounds (k_far_bound)
+    DevBuf<uint32_t> farPerm_[2];   // the kernel's source order per role (FarBound::perm)
+    DevBuf<BoundDirs> farBlockDirs_[2];
+    DevBuf<uint32_t> farCand_;      // FarBound::cand (two entries a record) and its count
+    DevBuf<uint32_t> farCandCount_;
     bool farBoundOn_ = false;
     int setupFarBound();
     // the contraction stage captured as HIP graphs, per (set, batch size, destination): the
@@ -387,7 +391,8 @@ private:
     int setOf(uint32_t b) const { return (int)(b % (uint32_t)nsets_); }
     DevBuf<uint32_t> status_;
     DevBuf<uint32_t> overflow_;           // bit b: batch b's vertices did not fit the pool
-    DevBuf<unsigned long long> totals_;   // [0] misses, [1] vertices, [2] NEE voxel steps
+    DevBuf<unsigned long long> stepSlots_;   // voxel steps: k_nee's counter set, the view kernels' (slotAdd)
+    DevBuf<unsigned long long> totals_;   // [0] misses, [1] vertices, [3] largest batch
     uint64_t cap_ = 0;
     bool poolSized_ = false;   // sizePool ran: the pool holds a batch of this request with margin
     DevBuf<double> U_;
@@ -398,9 +403,11 @@ private:
     // k_nee and k_nee_view, 2 NEE end (NEE stream); 7 contraction start, 3 after the
     // multiple-scattering contraction, 4 after the single-scattering one (contraction stream)
     static constexpr size_t kEventsPerBatch = 8;
-    // device totals: 0 missed attempts, 1 vertices, 2 k_nee voxel steps, 3 largest batch (vertices),
-    // 4 k_nee_view voxel steps
+    // device totals: 0 missed attempts, 1 vertices, 3 largest batch (vertices); the voxel steps are
+    // in stepSlots_
     static constexpr size_t kTotals = 5;
+    int zeroCounters(hipStream_t st);
+    void stepCounts(uint64_t* nee, uint64_t* view) const;
     std::vector<hipEvent_t> ev_;
     uint32_t evBatches_ = 0;
     sstmc_timing last_{};

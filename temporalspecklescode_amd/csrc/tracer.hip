@@ -49,6 +49,7 @@ template struct DevBuf<PhaseTables>;
 template struct DevBuf<unsigned long long>;
 template struct DevBuf<VertexRec>;
 template struct DevBuf<FarRec>;
+template struct DevBuf<BoundDirs>;
 
 // ====================================================================== HeterogeneousMedium
 HeterogeneousMedium::HeterogeneousMedium(int* err, int dims, const uint32_t* grid, const double* xAxis,
@@ -872,7 +873,7 @@ int NextEventEstimation::syncAll() const {
 int NextEventEstimation::allocate() {
     int e;
     if ((e = status_.alloc(4)) || (e = totals_.alloc(kTotals)) || (e = overflow_.alloc((opt_.fullIteration + 31) / 32)) ||
-        (e = stage_.q.alloc(2)))
+        (e = stage_.q.alloc(2)) || (e = stepSlots_.alloc(2 * (size_t)kCountSlots * kCountStride)))
         return e;
     size_t elem = fp64_ ? 16 : 8;
     size_t na = (size_t)T_ * P_ * L_ * elem, nb = (size_t)T_ * P_ * V_ * elem;
@@ -933,6 +934,14 @@ int NextEventEstimation::growPool(uint64_t cap) {
         perVertex += 2 * (uint64_t)T_ * (L_ + V_) * elem;
     }
     bytes_ += (uint64_t)nsets_ * (cap - cap_) * perVertex;
+    // the far bound's candidate list (one list: the bound kernels of every set run on one stream)
+    if (farBoundOn_) {
+        if ((e = farCand_.alloc(2 * cap * recT)) || (!farCandCount_.p && (e = farCandCount_.alloc(1)))) return e;
+        bytes_ += 8 * (cap - cap_) * recT;
+        farBound_.cand = farCand_.p;
+        farBound_.candCount = farCandCount_.p;
+        farBound_.candCap = 2 * cap * recT;
+    }
     // the transport's staging: the pool plus the chunk tails of every path (TransportStage)
     const uint64_t stageNeed = cap + (uint64_t)P_ * (kStageChunk - 1);
     const uint64_t nchunks = (stageNeed + kStageChunk - 1) / kStageChunk + 1;
@@ -1016,7 +1025,7 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     prm.npaths = npaths;
     prm.sameViews = sameViews_ ? 1u : 0u;
     prm.sameIllums = sameIllums_ ? 1u : 0u;
-    NeeOut out{bs.A.p, bs.B.p, bs.a.p, bs.b.p, bs.Ar.p, bs.Br.p, totals_.p + 2};
+    NeeOut out{bs.A.p, bs.B.p, bs.a.p, bs.b.p, bs.Ar.p, bs.Br.p, stepSlots_.p};
     if (cbsCorr_) {
         out.F1l = bs.F1l.p;
         out.F2l = bs.F2l.p;
@@ -1324,6 +1333,21 @@ int NextEventEstimation::setupFarBound() {
         farBound_.pyr[role] = mp;
         farBound_.dirs[role] = bd;
         farBoundOn_ = true;
+        // the kernel's source order (compact cones of 64) and each block's direction set
+        const size_t n = df.size() / 3;
+        const std::vector<uint32_t> perm = farOrderHost(df.data(), n, role == 0 ? 1.0 : -1.0, axis);
+        std::vector<BoundDirs> blocks((n + 63) / 64);
+        for (size_t b = 0; b < blocks.size(); ++b) {
+            const size_t m = std::min<size_t>(64, n - 64 * b);
+            blocks[b] = boundDirsAxisHost(df.data(), perm.data() + 64 * b, m, role == 0 ? 1.0 : -1.0, axis, bd.sgn);
+        }
+        int e2;
+        if ((e2 = farPerm_[role].upload(perm.data(), perm.size())) ||
+            (e2 = farBlockDirs_[role].upload(blocks.data(), blocks.size())))
+            return e2;
+        farBound_.perm[role] = farPerm_[role].p;
+        farBound_.blockDirs[role] = farBlockDirs_[role].p;
+        farBound_.blocks[role] = (uint32_t)blocks.size();
     }
     return SSTMC_NO_ERROR;
 }
@@ -1376,6 +1400,28 @@ int NextEventEstimation::sizePool() {
     return SSTMC_NO_ERROR;
 }
 
+// The batch totals and the slotted voxel-step counters, cleared on st.
+int NextEventEstimation::zeroCounters(hipStream_t st) {
+    if (hipMemsetAsync(totals_.p, 0, kTotals * sizeof(unsigned long long), st) != hipSuccess ||
+        hipMemsetAsync(stepSlots_.p, 0, stepSlots_.n * sizeof(unsigned long long), st) != hipSuccess)
+        return SSTMC_DEVICE_ERROR;
+    return SSTMC_NO_ERROR;
+}
+
+// The voxel steps of k_nee and of the view kernels: the sums of their counter slots (the streams
+// must be idle).
+void NextEventEstimation::stepCounts(uint64_t* nee, uint64_t* view) const {
+    std::vector<unsigned long long> h(stepSlots_.n, 0ull);
+    *nee = *view = 0;
+    if (hipMemcpy(h.data(), stepSlots_.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess)
+        return;
+    const size_t set = (size_t)kCountSlots * kCountStride;
+    for (size_t i = 0; i < set; i += kCountStride) {
+        *nee += h[i];
+        *view += h[set + i];
+    }
+}
+
 int NextEventEstimation::begin() {
     int rc = sizePool();
     if (rc) return rc;
@@ -1389,8 +1435,7 @@ int NextEventEstimation::begin() {
         evBatches_ = nb;
     }
     if (hipMemsetAsync(status_.p, 0, 4 * sizeof(uint32_t), stream_) != hipSuccess) return SSTMC_DEVICE_ERROR;
-    if (hipMemsetAsync(totals_.p, 0, kTotals * sizeof(unsigned long long), stream_) != hipSuccess)
-        return SSTMC_DEVICE_ERROR;
+    if (zeroCounters(stream_)) return SSTMC_DEVICE_ERROR;
     if (hipMemsetAsync(overflow_.p, 0, overflow_.n * sizeof(uint32_t), stream_) != hipSuccess)
         return SSTMC_DEVICE_ERROR;
     if (launchZeroU(U_.p, U_.n, stream_) != hipSuccess) return SSTMC_KERNEL_ERROR;
@@ -1471,8 +1516,10 @@ int NextEventEstimation::finish(double* u, bool deviceDst) {
     if (opt_.fullIteration && hipEventElapsedTime(&ms, ev_[0], ev_[(size_t)(opt_.fullIteration - 1) * kEventsPerBatch + 4]) == hipSuccess)
         t.total_ms = ms;
     t.vertices = tot[1];
-    t.dda_steps = tot[2] + tot[4];
-    t.dda_steps_view = tot[4];
+    uint64_t neeSteps = 0, viewSteps = 0;
+    stepCounts(&neeSteps, &viewSteps);
+    t.dda_steps = neeSteps + viewSteps;
+    t.dda_steps_view = viewSteps;
     t.batches = opt_.fullIteration;
     t.graph_replays = graphReplays_;
     t.pipeline_sets = (uint32_t)nsets_;
@@ -1526,7 +1573,7 @@ int NextEventEstimation::debugBatch(uint64_t pathBegin, uint32_t n, uint64_t* nv
     const BatchSet& bs = sets_[0];
     for (int attempt = 0; attempt < 8; ++attempt) {
         hipMemsetAsync(status_.p, 0, 4 * sizeof(uint32_t), stream_);
-        hipMemsetAsync(totals_.p, 0, kTotals * sizeof(unsigned long long), stream_);
+        zeroCounters(stream_);
         hipMemsetAsync(overflow_.p, 0, overflow_.n * sizeof(uint32_t), stream_);
         launchZeroU(Ub.p, Ub.n, stream_);
         if ((rc = runBatch(pathBegin, n, Ub.p, false, 0, 0, false))) return rc;

@@ -87,18 +87,6 @@ __device__ inline void frameDeltas(const FramesView& fr, uint64_t seed, uint64_t
     }
 }
 
-// Hands every lane in `need` (a ballot taken by the calling lanes) its own consecutive index from
-// *ctr with one atomic per wavefront; lanes outside `need` get an unspecified value.
-__device__ inline uint32_t waveTake(uint32_t* ctr, uint64_t needMask, bool need) {
-    const int leader = __builtin_ctzll(needMask);
-    uint32_t base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(needMask));
-    base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(needMask >> 32),
-                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)needMask, 0u));
-    return need ? base + below : 0u;
-}
-
 constexpr uint32_t kNoSlot = 0xffffffffu;
 // A wavefront keeps stepping flights while at least this many lanes fly; below it, the lanes that
 // wait on an event (a collision to store, a miss, a finished path) are served together.

@@ -137,9 +137,9 @@ static int run(int n, double h, int trials, unsigned seed, int fast, int nmat) {
     return bad != 0 || laneBad != 0 || g_bad_index != 0;
 }
 
-// The far-field view kernel's lower bound (farLowerBound over a MinPyramid, BoundDirs of a cone of
-// walk directions): for every point and every direction of the set, the FP32 fast walk's tau is at
-// least the bound, and wherever the bound passes farSkip(cut) the walk under that cut ends past
+// The far-field view kernel's lower bounds (farLowerBound over a MinPyramid, BoundDirs of a cone of
+// walk directions, and the same bound per block of its source order, blocks of farOrderHost): for every point and every direction of the set (block), the FP32 fast walk's
+// tau is at least the bound, and wherever the bound passes farSkip(cut) the walk under that cut ends past
 // it, so skipping the point's walks changes no term. Grids of smooth blobs (box-blurred noise cut
 // into nmat levels) so that the bound is often large; cones of up to 30 degrees about random
 // axes, some containing the axis-aligned direction, points inside, on faces and on edges.
@@ -199,7 +199,7 @@ static int runBound(int n, double h, int trials, unsigned seed, int fast, int nm
         pyr[a].p = pyrData[a].data();
     }
     std::uniform_real_distribution<double> U(ed[0], ed[n]), N(-1, 1);
-    long bad = 0, tested = 0, skipped = 0, valid = 0;
+    long bad = 0, tested = 0, skipped = 0, valid = 0, blockTested = 0, blockSkipped = 0;
     double tight = 0;
     const float cuts[4] = {2.f, 8.f, 24.f, 210.f};
     for (int t = 0; t < trials; ++t) {
@@ -228,6 +228,7 @@ static int runBound(int n, double h, int trials, unsigned seed, int fast, int nm
         const BoundDirs bd = boundDirsHost(dirs.data(), 48, 1.0, &axis);
         if (!bd.valid) continue;
         ++valid;
+        const std::vector<uint32_t> order = farOrderHost(dirs.data(), 48, 1.0, axis);
         for (int q = 0; q < 8; ++q) {
             double p[3] = {U(rng), U(rng), U(rng)};
             if (q == 1) p[t % 3] = ed[rng() % (n + 1)];
@@ -257,11 +258,35 @@ static int runBound(int n, double h, int trials, unsigned seed, int fast, int nm
             ++tested;
             if (lb > farSkip(24.f)) ++skipped;
             if (minTau > 0) tight += lb / minTau;
+            // the per-block bound of the kernel's source order: blocks of 16 directions about the
+            // whole set's axis
+            for (size_t b = 0; b < 48; b += 16) {
+                const BoundDirs bb = boundDirsAxisHost(dirs.data(), order.data() + b, 16, 1.0, axis, bd.sgn);
+                const float lbb = farLowerBound(m, pyr[axis], bb, pf, 3.0e38f);
+                ++blockTested;
+                if (lbb > farSkip(24.f)) ++blockSkipped;
+                for (size_t k = b; k < b + 16; ++k) {
+                    const uint32_t i = order[k];
+                    const f3 df = mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
+                    uint32_t s0 = 0;
+                    const float tau = opticalDepthFast<3>(m, m.walkSig, pf, df, 3.0e38f, &s0);
+                    if (!(tau >= lbb)) {
+                        if (bad < 5) printf("  set %d point %d block %zu dir %u: bound %.9g above the walk %.9g\n", t, q, b, i, lbb, tau);
+                        ++bad;
+                    }
+                    for (float cut : cuts)
+                        if (lbb > farSkip(cut)) {
+                            uint32_t s1 = 0;
+                            if (!(opticalDepthFast<3>(m, m.walkSig, pf, df, cut, &s1) > cut)) ++bad;
+                        }
+                }
+            }
         }
     }
     printf("bound: n %d h %g fastest axis %d, %d materials: %ld direction sets, %ld points, %ld violations, "
-           "skippable at cut 24: %.3f, mean bound / least walk %.3f\n", n, h, fast, nmat, valid, tested, bad,
-           tested ? (double)skipped / tested : 0.0, tested ? tight / tested : 0.0);
+           "skippable at cut 24: %.3f (blocks of 16: %.3f), mean bound / least walk %.3f\n", n, h, fast, nmat, valid,
+           tested, bad, tested ? (double)skipped / tested : 0.0, blockTested ? (double)blockSkipped / blockTested : 0.0,
+           tested ? tight / tested : 0.0);
     return bad != 0 || g_bad_index != 0 || tested == 0;
 }
 
