@@ -79,11 +79,11 @@ hipError_t launchBatchTotals(const uint32_t* misses, uint32_t* off, uint32_t n, 
                              uint32_t* status, hipStream_t st);
 
 // The NEE stage in two parts. kNeeVertexPart reads the batch's vertices only: k_tau_ref (when
-// p.tauRef is set), k_vertex_prep (per-vertex records) and, with vb.far and fb, k_far_bound and
-// k_far_blocks (which blocks of the far-field view kernel's sources each vertex's walks can reach
-// inside the cut, from lower bounds on their optical depth). kNeeTermPart forms the terms: k_nee (or
-// k_nee_corr) for the first vertex, then the view kernels for vertices k >= 2; `between`, when not
-// null, is recorded on `st` between the two.
+// p.tauRef is set) and k_vertex_prep (per-vertex records). kNeeTermPart forms the terms: with
+// vb.far and fb first k_far_bound and k_far_blocks (which blocks of the far-field view kernel's
+// sources each vertex's walks can reach inside the cut, from lower bounds on their optical
+// depth), k_nee (or k_nee_corr) for the first vertex, then the view kernels for vertices k >= 2;
+// `between`, when not null, is recorded on `st` between the two.
 enum : int { kNeeVertexPart = 1, kNeeTermPart = 2, kNeeAll = 3 };
 hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
                      const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,

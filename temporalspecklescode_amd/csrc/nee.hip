@@ -1545,16 +1545,18 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
         if (fr.analytic) rf.nt = 1;   // TemporalCorrelationSampler: vertices never move, one record each
         hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, m, vb, rf, p.npaths,
                            1.0 / p.lambda);
-        if (DIMS == 3 && !fr.analytic && vb.far && fb) {
-            const uint64_t n = vb.cap * fr.nt;
-            hipLaunchKernelGGL(k_far_reset, dim3(1), dim3(64), 0, st, fb->candCount);
-            hipLaunchKernelGGL(k_far_bound, dim3((uint32_t)((n + 255) / 256)), block, 0, st, m, vb, fr, p, *fb,
-                               out.Br ? 1u : 0u);
-            if (fb->blockDirs[0] || fb->blockDirs[1])
-                hipLaunchKernelGGL(k_far_blocks, dim3(2048), block, 0, st, m, vb, p, *fb);
-        }
     }
     if (!(parts & kNeeTermPart)) return;
+    if (DIMS == 3 && !fr.analytic && vb.far && fb) {
+        // which blocks of the far-field view kernel's sources each record's walks can reach inside
+        // the cut (the bounds need the cut reference)
+        const uint64_t n = vb.cap * fr.nt;
+        hipLaunchKernelGGL(k_far_reset, dim3(1), dim3(64), 0, st, fb->candCount);
+        hipLaunchKernelGGL(k_far_bound, dim3((uint32_t)((n + 255) / 256)), block, 0, st, m, vb, fr, p, *fb,
+                           out.Br ? 1u : 0u);
+        if (fb->blockDirs[0] || fb->blockDirs[1])
+            hipLaunchKernelGGL(k_far_blocks, dim3(2048), block, 0, st, m, vb, p, *fb);
+    }
     if (fr.analytic) {
         if (sc.tab) hipLaunchKernelGGL((k_nee_corr<DIMS, R, true>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
         else hipLaunchKernelGGL((k_nee_corr<DIMS, R, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);

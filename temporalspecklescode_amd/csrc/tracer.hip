@@ -1039,8 +1039,8 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     }
     const FarBound* fb = farBoundOn_ ? &farBound_ : nullptr;
 
-    // ---- transport, then what NEE needs of the vertices alone (the cut's reference, the
-    // per-vertex records and walk bounds), so that the NEE stream forms terms only
+    // ---- transport, then what NEE needs of the vertices alone (the cut's reference and the
+    // per-vertex records), so that the NEE stream bounds the walks and forms the terms
     if (bs.used && hipStreamWaitEvent(sT, bs.released, 0) != hipSuccess) return SSTMC_DEVICE_ERROR;
     bs.used = true;
     if (e) hipEventRecord(e[0], sT);

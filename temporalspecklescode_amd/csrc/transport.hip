@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(256) k_batch_totals(const uint32_t* misses, ui
 
 // Lanes of the transport's persistent grid: enough wavefronts to keep every CU's share of the
 // batch's paths flowing, few enough that each lane traces several paths from the queue.
-constexpr uint32_t kTransportWaves = 256;
+constexpr uint32_t kTransportWaves = 512;
 
 hipError_t launchTransport(int dims, const MediumView& m, const ScatterView& sc, const SourceView& il,
                            const FramesView& fr, const VertexBatch& vb, const TransportStage& ts, uint64_t seed,
