@@ -297,9 +297,7 @@ struct __align__(32) VertexRec {
 // forms them), the walk-grid bit offset of the cell (kFarOutside for a position outside the box,
 // whose optical depth is 0); the material; the back direction rounded to FP32; the FP64 position
 // and phase base of VertexRec.
-// live: per role (views 0, the reversed paths' illuminations 1), bit b set when block b of the
-// far-field view kernel's sources (FarBound::perm) may hold a term inside the launch's cut, bit 31
-// standing for every block from 31 on; all set when no bound applies (k_far_bound).
+// (The live-block masks of the far-field view kernel are kept apart, VertexBatch::live.)
 constexpr uint32_t kFarOutside = 0xffffffffu;
 struct __align__(32) FarRec {
     float dlo[3];
@@ -308,7 +306,7 @@ struct __align__(32) FarRec {
     uint32_t mat;
     float b[3];
     float back[3];
-    uint32_t live[2];
+    uint32_t pad[2];
     double px, py, pz;
     double phase;
 };
@@ -362,6 +360,11 @@ struct VertexBatch {
     uint32_t* status;              // [4]: 0 overflow flag, 1 no-hit flag
     VertexRec* rec;                // [cap * nt], record of vertex o, frame t at o * nt + t
     FarRec* far = nullptr;         // [cap * nt] like rec, when the far-field view kernel runs
+    // [cap * nt * 2] with far: the live-block masks of record i at live[2 i + role], role 0 the
+    // views and 1 the reversed paths' illuminations: bit b set when block b of the far-field view
+    // kernel's sources (FarBound::perm) may hold a term inside the launch's cut, bit 31 standing for
+    // every block from 31 on; all set when no bound applies (k_vertex_prep, k_far_bound)
+    uint32_t* live = nullptr;
 };
 
 // Staging of the single-pass transport. Lanes draw path slots from a queue and store each path's

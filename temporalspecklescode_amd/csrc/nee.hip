@@ -743,7 +743,8 @@ __global__ void __launch_bounds__(256) k_vertex_prep(MediumView m, VertexBatch v
                 f.back[0] = (float)b.x;
                 f.back[1] = (float)b.y;
                 f.back[2] = (float)b.z;
-                f.live[0] = f.live[1] = 0xffffffffu;
+                f.pad[0] = f.pad[1] = 0u;
+                if (vb.live) vb.live[2 * (ok * NT + t)] = vb.live[2 * (ok * NT + t) + 1] = 0xffffffffu;
                 f.px = xk.pd.x;
                 f.py = xk.pd.y;
                 f.pz = xk.pd.z;
@@ -890,7 +891,7 @@ k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBat
     if ((threadIdx.x & 63) == 0 && out.steps) slotAdd(out.steps + kCountSlots * kCountStride, tot);
 }
 
-// The live-block masks of the far-field view kernel (FarRec::live) for every record of the batch:
+// The live-block masks of the far-field view kernel (VertexBatch::live) for every record of the batch:
 // live[0] for the views under the launch's cut (viewCut), live[1] for the reversed paths'
 // illuminations under the exact cut; a block is dead when the lower bound of its sources' walks
 // (farLowerBound over the block's direction set) passes the cut's skip threshold. k_far_bound,
@@ -923,7 +924,7 @@ __global__ void __launch_bounds__(256) k_far_bound(MediumView m, VertexBatch vb,
                 cand = true;
             }
         }
-        r.live[role] = live;
+        vb.live[2 * i + role] = live;
         const uint64_t cm = __ballot(cand);
         if (cm) {
             const uint32_t slot = waveTake(fb.candCount, cm, cand);
@@ -939,11 +940,11 @@ __global__ void __launch_bounds__(256) k_far_blocks(MediumView m, VertexBatch vb
     for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < nc * nb; j += (uint64_t)gridDim.x * 256) {
         const uint32_t e = fb.cand[j / nb], b = (uint32_t)(j % nb), role = e & 1u;
         if (b >= fb.blocks[role]) continue;
-        FarRec& r = vb.far[e >> 1];
+        const FarRec& r = vb.far[e >> 1];
         const f3 pf = mk((float)r.px, (float)r.py, (float)r.pz);
         const float skip = farRoleSkip(prm, (int)role);
         if (!(farLowerBound(m, fb.pyr[role], fb.blockDirs[role][b], pf, skip) > skip))
-            atomicOr(&r.live[role], 1u << (b < 31u ? b : 31u));
+            atomicOr(vb.live + 2 * (uint64_t)(e >> 1) + role, 1u << (b < 31u ? b : 31u));
     }
 }
 
@@ -1014,7 +1015,7 @@ k_nee_view_far(MediumView m, ScatterView sc, SourceView vw, FramesView fr, Verte
         const uint32_t bsel = u.blk < 31u ? u.blk : 31u;
         for (uint32_t c0 = 1; c0 < K; c0 += 64) {
             const uint32_t kl = c0 + u.sub;
-            const uint32_t bits = kl < K ? vb.far[(off + kl) * fr.nt + t].live[role] : 0u;
+            const uint32_t bits = kl < K ? vb.live[2 * ((off + kl) * fr.nt + t) + role] : 0u;
             uint64_t alive = __ballot((bits >> bsel) & 1u);
             while (alive) {
                 const uint32_t k = c0 + (uint32_t)__builtin_ctzll(alive);

@@ -307,6 +307,7 @@ private:
         DevBuf<float> rho, delta;
         DevBuf<VertexRec> rec;
         DevBuf<FarRec> far;                   // the far-field view kernel's records (farRecs_)
+        DevBuf<uint32_t> live;                // and their live-block masks
         DevBuf<float> tauRef;                 // the batch's least view optical depth (relative cut)
         DevBuf<unsigned char> A, B, a, b;
         DevBuf<unsigned char> Ar, Br;         // coherent backscattering: reversed-path terms

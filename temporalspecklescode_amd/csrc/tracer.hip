@@ -904,7 +904,7 @@ int NextEventEstimation::allocate() {
 VertexBatch NextEventEstimation::vertexBatch(int set) const {
     const BatchSet& bs = sets_[set];
     return VertexBatch{bs.k.p, bs.misses.p, bs.off.p, bs.xm.p, bs.rho.p, bs.delta.p, cap_, status_.p, bs.rec.p,
-                       farRecs_ ? bs.far.p : nullptr};
+                       farRecs_ ? bs.far.p : nullptr, farRecs_ ? bs.live.p : nullptr};
 }
 
 // Every set's vertex pool holds `cap` vertices; the streams must be idle.
@@ -916,14 +916,14 @@ int NextEventEstimation::growPool(uint64_t cap) {
     for (int s = 0; s < nsets_; ++s) {
         BatchSet& bs = sets_[s];
         if ((e = bs.xm.alloc(cap)) || (e = bs.rho.alloc(cap)) || (e = bs.rec.alloc(cap * recT))) return e;
-        if (farRecs_ && (e = bs.far.alloc(cap * recT))) return e;
+        if (farRecs_ && ((e = bs.far.alloc(cap * recT)) || (e = bs.live.alloc(2 * cap * recT)))) return e;
         if (!sampler_.view().trivial) {
             if ((e = bs.delta.alloc(cap * T_ * 3))) return e;
         } else {
             bs.delta.release();
         }
     }
-    uint64_t perVertex = 20 + (farRecs_ ? 128 : 32) * recT + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_);
+    uint64_t perVertex = 20 + (farRecs_ ? 136 : 32) * recT + (sampler_.view().trivial ? 0 : 12 * (uint64_t)T_);
     if (cbsCorr_) {
         // per-vertex u1 / u2 terms of every illumination and view, per lag
         const size_t elem = fp64_ ? 16 : 8;
@@ -1174,8 +1174,8 @@ uint64_t NextEventEstimation::setBytes(uint32_t P, uint32_t L, uint32_t V, uint3
                                        bool cbs, bool acorr) {
     const uint64_t elem = fp64 ? 16 : 8;
     uint64_t terms = (cbs ? 3ull : 2ull) * T * P * (uint64_t)(L + V) * elem;
-    // vertex records: 32 bytes, and 96 more for the far-field view kernel's (FP32 terms)
-    uint64_t pool = (uint64_t)P * 32 * (20 + (fp64 ? 32ull : 128ull) * T + (frames ? 12ull * T : 0));
+    // vertex records: 32 bytes, and 104 more for the far-field view kernel's (FP32 terms: record and masks)
+    uint64_t pool = (uint64_t)P * 32 * (20 + (fp64 ? 32ull : 136ull) * T + (frames ? 12ull * T : 0));
     if (cbs && acorr) {
         // correlation CBS: first-vertex u1 / u2 terms and per-vertex u1 / u2 terms
         terms += 2ull * T * P * (uint64_t)(L + V) * elem;
