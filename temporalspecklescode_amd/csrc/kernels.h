@@ -60,6 +60,9 @@ struct NeeParams {
                            // correlation: up to 16 frames each)
     uint32_t sameViews;    // TemporalCorrelationSampler: view2 holds the same sources as view
     uint32_t sameIllums;   // TemporalCorrelationSampler: illumination2 holds the same sources
+    // the far-field view kernel's view order (FarBound::perm[0]) when its records and live-block
+    // masks exist (k_nee takes its single-scattering view walks in blocks of it), else null
+    const uint32_t* farPerm = nullptr;
 };
 
 // The single-pass transport of npaths path slots (zeroes ts.q, then k_transport): per-path

@@ -437,16 +437,28 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
     const bool isoPointVw = vw.type == 1 && sc.allIsotropic;
 
     // ---- single-scattering view terms b = g_v(x1^t)
+    // Far-field views under the view cut with the far-field view kernel's records: the views in
+    // its order, and a view in a block whose every walk from x1 lies past the cut (x1's live-block
+    // mask) takes g = 0 without walking, as its walk would give.
+    const bool blockSkip = sizeof(R) == 4 && !gauss && !corr && !Ar && vw.type == 2 && prm.farPerm && vb.live;
     for (uint32_t i = sub; i < V * NT; i += lpp) {
-        const uint32_t v = i % V, t = i / V;
+        const uint32_t pos = i % V, t = i / V;
+        const uint32_t v = blockSkip ? prm.farPerm[pos] : pos;
         R ga[2];
         double gp[2];
         const int nf = corr ? 2 : 1;
         for (int f = 0; f < nf; ++f) {
             uint32_t tt = (corr && f == 0) ? 0 : t;
             Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
-            gTerm<DIMS, R, gauss>(m, sig, vw, v, x1t, invLambda, Ar ? tauExact : tauCutFF, &steps, &ga[f],
-                                  &gp[f]);
+            const uint32_t blk = pos / 64u;
+            if (blockSkip && !((vb.live[2 * (off * NT + t)] >> (blk < 31u ? blk : 31u)) & 1u)) {
+                const double phase = ddot<DIMS>(sourceD<DIMS>(vw, v), x1t.pd) * invLambda;   // gFar's phase
+                ga[f] = (R)0;
+                gp[f] = vw.illum != 0 ? phase : -phase;
+            } else {
+                gTerm<DIMS, R, gauss>(m, sig, vw, v, x1t, invLambda, Ar ? tauExact : tauCutFF, &steps, &ga[f],
+                                      &gp[f]);
+            }
         }
         const size_t o = ((size_t)t * P + p) * V + v;
         b[o] = corr ? cis<R>(ga[0] * ga[1], gp[0] - gp[1]) : cis<R>(ga[0], gp[0]);
@@ -691,14 +703,46 @@ __global__ void __launch_bounds__(256) k_nee_corr(MediumView m, ScatterView sc, 
     if (lane == 0 && out.steps) slotAdd(out.steps, tot);
 }
 
+// The far-field view kernel's record of a vertex at one frame (FarRec): the walk's start (cell and
+// face offsets, or kFarOutside), the material, the FP32 back direction, the FP64 position and
+// phase base; its live-block masks start all set (k_far_bound narrows them).
+template <int DIMS>
+__device__ inline void writeFarRec(const MediumView& m, const VertexBatch& vb, uint64_t idx, const Geo& xk,
+                                   uint32_t mat, const d3& back, double phase) {
+    FarRec f;
+    if (inside<DIMS, float>(m, xk.pf)) {
+        const WalkStart w = walkStart<DIMS>(m, xk.pf);
+        for (int a = 0; a < 3; ++a) {
+            f.dlo[a] = w.dlo[a];
+            f.dhi[a] = w.dhi[a];
+            f.b[a] = w.b[a];
+        }
+        f.cell = w.cell;
+    } else {
+        for (int a = 0; a < 3; ++a) f.dlo[a] = f.dhi[a] = f.b[a] = 0.f;
+        f.cell = kFarOutside;
+    }
+    f.mat = mat;
+    f.back[0] = (float)back.x;
+    f.back[1] = (float)back.y;
+    f.back[2] = (float)back.z;
+    f.pad[0] = f.pad[1] = 0u;
+    f.px = xk.pd.x;
+    f.py = xk.pd.y;
+    f.pz = xk.pd.z;
+    f.phase = phase;
+    vb.far[idx] = f;
+    if (vb.live) vb.live[2 * idx] = vb.live[2 * idx + 1] = 0xffffffffu;
+}
+
 // Vertex records for the view terms: one thread per path walks its vertices once per frame and
 // writes, for every vertex k >= 2, the unit vector back to the previous vertex (the direction the
 // view-side HG amplitude is evaluated against, FarField.cuh:546-599 / PointSource.cuh:347-446)
 // and the phase base rho_k + L_k^t / lambda with L_k^t the FP64 path length from x_1^t
 // (pathContriobutionKernel, TemporalSampler.cuh:668-719). Done once per (vertex, frame) here
 // instead of once per (vertex, view, frame) in k_nee_view. With vb.far (3D) also the far-field
-// view kernel's record: the walk's start from x_k^t (FP32), the material, the back direction in
-// FP32 and the FP64 position.
+// view kernel's record of every vertex, the first included (k_nee's single-scattering view walks
+// read its live-block masks).
 template <int DIMS>
 __global__ void __launch_bounds__(256) k_vertex_prep(MediumView m, VertexBatch vb, FramesView fr, uint32_t npaths,
                                                      double invLambda) {
@@ -706,11 +750,13 @@ __global__ void __launch_bounds__(256) k_vertex_prep(MediumView m, VertexBatch v
     if (p >= npaths) return;
     if ((uint64_t)vb.off[npaths] > vb.cap) return;   // pool overflow: the batch is re-run
     const uint32_t K = vb.k[p], NT = fr.nt;
-    if (K < 2) return;                                // records exist for vertices k >= 2 only
+    if (K == 0) return;
     const uint64_t off = vb.off[p];
     for (uint32_t t = 0; t < NT; ++t) {
         float4 xpm = vb.xm[off];
         Geo xp = vertexFrame(vb, fr, off, t, xpm);
+        if (DIMS == 3 && vb.far)
+            writeFarRec<DIMS>(m, vb, off * NT + t, xp, __float_as_uint(xpm.w), mkd(0.0, 0.0, 0.0), (double)vb.rho[off]);
         double len = 0;
         for (uint32_t k = 1; k < K; ++k) {
             const uint64_t ok = off + k;
@@ -725,32 +771,7 @@ __global__ void __launch_bounds__(256) k_vertex_prep(MediumView m, VertexBatch v
             r.bx = b.x; r.by = b.y; r.bz = b.z;
             r.phase = (double)vb.rho[ok] + len * invLambda;
             vb.rec[ok * NT + t] = r;
-            if (DIMS == 3 && vb.far) {
-                FarRec f;
-                if (inside<DIMS, float>(m, xk.pf)) {
-                    const WalkStart w = walkStart<DIMS>(m, xk.pf);
-                    for (int a = 0; a < 3; ++a) {
-                        f.dlo[a] = w.dlo[a];
-                        f.dhi[a] = w.dhi[a];
-                        f.b[a] = w.b[a];
-                    }
-                    f.cell = w.cell;
-                } else {
-                    for (int a = 0; a < 3; ++a) f.dlo[a] = f.dhi[a] = f.b[a] = 0.f;
-                    f.cell = kFarOutside;
-                }
-                f.mat = __float_as_uint(xkm.w);
-                f.back[0] = (float)b.x;
-                f.back[1] = (float)b.y;
-                f.back[2] = (float)b.z;
-                f.pad[0] = f.pad[1] = 0u;
-                if (vb.live) vb.live[2 * (ok * NT + t)] = vb.live[2 * (ok * NT + t) + 1] = 0xffffffffu;
-                f.px = xk.pd.x;
-                f.py = xk.pd.y;
-                f.pz = xk.pd.z;
-                f.phase = r.phase;
-                vb.far[ok * NT + t] = f;
-            }
+            if (DIMS == 3 && vb.far) writeFarRec<DIMS>(m, vb, ok * NT + t, xk, __float_as_uint(xkm.w), b, r.phase);
             xpm = xkm;
             xp = xk;
         }
@@ -1624,6 +1645,7 @@ hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView&
     const bool corr = p.output == 1 || fr.analytic;
     viewLanes(fr.analytic ? vw.n / 2 : vw.n, &q);
     q.frameChunks = corr ? (fr.nt + kCorrLags - 1) / kCorrLags : fr.nt;
+    q.farPerm = (fb && vb.far && vb.live) ? fb->perm[0] : nullptr;
     if (dims == 2) {
         if (fp64) launchNeeT<2, double>(m, sc, il, vw, fr, vb, q, out, st, between, fb, parts);
         else launchNeeT<2, float>(m, sc, il, vw, fr, vb, q, out, st, between, fb, parts);
