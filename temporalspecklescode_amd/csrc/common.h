@@ -313,7 +313,7 @@ struct __align__(32) FarRec {
 static_assert(sizeof(FarRec) == 96, "FarRec is three 32-byte lines");
 
 // Per-layer least sigt over aligned square blocks of cells, for lower bounds on optical depths
-// (k_far_bound). Layers run along `axis`; each layer is a 2D grid over the two other axes (b, c in
+// (farCellBound). Layers run along `axis`; each layer is a 2D grid over the two other axes (b, c in
 // increasing order) reduced 2 x 2 per level, up to one block at the top level. Level L of layer l
 // holds block (ib >> L, ic >> L) at p[off[L] + (l * nc[L] + (ic >> L)) * nb[L] + (ib >> L)]; FP32
 // sigt as the walks use it.
@@ -335,18 +335,16 @@ struct BoundDirs {
 // The bounds of the far-field view kernel's two launches: views (0) and, for coherent
 // backscattering, the reversed paths' illuminations (1). Per role the kernel's sources are taken
 // in the order perm (a Z-order over their lateral slopes, so a wavefront's 64 sources form a compact
-// cone), and blockDirs[b] is the direction set of sources perm[64 b .. 64 b + 63] about the
-// pyramid's axis, for a lower bound per (vertex, block) tighter than the whole set's.
+// cone of directions), and table[box * cols + j] is a lower bound on the optical depth from any
+// point of the box to the medium's boundary along every direction of block j, sources
+// perm[64 j .. 64 j + 63] (farBoxBound; column 31 the least over blocks 31 and up), for the
+// sub x sub x sub boxes of every cell (farSubBoxOf's index), built once per tracer.
 struct FarBound {
-    MinPyramid pyr[2];
-    BoundDirs dirs[2];
     const uint32_t* perm[2] = {nullptr, nullptr};
-    const BoundDirs* blockDirs[2] = {nullptr, nullptr};
+    const float* table[2] = {nullptr, nullptr};
     uint32_t blocks[2] = {0u, 0u};
-    // records the whole set's bound leaves alive, as record * 2 + role, for the per-block pass
-    uint32_t* cand = nullptr;
-    uint32_t* candCount = nullptr;
-    uint64_t candCap = 0;
+    uint32_t cols[2] = {0u, 0u};
+    uint32_t sub = 1;
 };
 
 struct VertexBatch {
@@ -363,7 +361,7 @@ struct VertexBatch {
     // [cap * nt * 2] with far: the live-block masks of record i at live[2 i + role], role 0 the
     // views and 1 the reversed paths' illuminations: bit b set when block b of the far-field view
     // kernel's sources (FarBound::perm) may hold a term inside the launch's cut, bit 31 standing for
-    // every block from 31 on; all set when no bound applies (k_vertex_prep, k_far_bound)
+    // every block from 31 on; all set when no bound applies (k_vertex_prep, k_far_live)
     uint32_t* live = nullptr;
 };
 

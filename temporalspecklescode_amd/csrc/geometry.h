@@ -602,6 +602,93 @@ __host__ __device__ inline float farLowerBound(const MediumView& m, const MinPyr
     return lb * (1.f - 1e-5f);
 }
 
+// Faces along axis a of sub-box k of cell c (sub boxes a cell along each axis): the cell's own
+// faces at its ends, equal divisions inside.
+__host__ __device__ inline void farSubBoxFaces(const MediumView& m, int a, uint32_t c, uint32_t k, uint32_t sub,
+                                               float* lo, float* hi) {
+    const float e0 = m.ef[a][c], e1 = m.ef[a][c + 1], d = (e1 - e0) / (float)sub;
+    *lo = k == 0 ? e0 : e0 + (float)k * d;
+    *hi = k + 1 == sub ? e1 : e0 + (float)(k + 1) * d;
+}
+
+// Least sigt of layer `layer` over cells [b0, b1] x [c0, c1] from the finest pyramid level at which
+// the range covers at most span + 1 blocks along each axis (span 1: at most four lookups).
+__host__ __device__ inline float pyrMinSpan(const MinPyramid& mp, uint32_t layer, uint32_t b0, uint32_t b1,
+                                            uint32_t c0, uint32_t c1, uint32_t span) {
+    uint32_t L = 0;
+    while (L + 1 < mp.levels && ((b1 >> L) - (b0 >> L) > span || (c1 >> L) - (c0 >> L) > span)) ++L;
+    const uint32_t nb = mp.nb[L];
+    const float* q = mp.p + mp.off[L] + (uint64_t)layer * nb * mp.nc[L];
+    float r = 3.0e38f;
+    for (uint32_t c = c0 >> L; c <= (c1 >> L); ++c)
+        for (uint32_t b = b0 >> L; b <= (b1 >> L); ++b) r = fminf(r, q[c * nb + b]);
+    return r;
+}
+
+// farLowerBound for every point of a box at once: a lower bound on the optical depth from any
+// point of the box [blo, bhi] (inside cell c, indices along the three axes) to the box boundary
+// along every direction of bd, with no stop, from pyrMinSpan(span). With the axis sense sgn, the
+// walks cross the rest of the cell's own layer over at least its face distance from the box's far
+// side, and layer c_a + s sgn (s >= 1) at distances along the axis between those of its faces from
+// the box's far and near sides; the lateral rectangles widen the box by those distances times the
+// lateral slopes. Same margins as farLowerBound.
+__host__ __device__ inline float farBoxBound(const MediumView& m, const MinPyramid& mp, const BoundDirs& bd,
+                                             const uint32_t c[3], const float blo[3], const float bhi[3],
+                                             uint32_t span) {
+    if (!bd.valid || mp.axis < 0) return 0.f;
+    const int a = mp.axis, qs[2] = {a == 0 ? 1 : 0, a == 2 ? 1 : 2};
+    float lb = 0.f;
+    for (int L = (int)c[a]; L >= 0 && L < (int)m.n[a]; L += bd.sgn) {
+        const bool own = L == (int)c[a];
+        // distances along the axis to the layer's entry and exit faces, least and greatest over the box
+        float tn, tf, w;
+        if (bd.sgn > 0) {
+            tn = own ? 0.f : m.ef[a][L] - bhi[a];
+            tf = m.ef[a][L + 1] - blo[a];
+            w = own ? m.ef[a][L + 1] - bhi[a] : m.h[a];
+        } else {
+            tn = own ? 0.f : blo[a] - m.ef[a][L + 1];
+            tf = bhi[a] - m.ef[a][L];
+            w = own ? blo[a] - m.ef[a][L] : m.h[a];
+        }
+        tn = fmaxf(tn, 0.f);
+        w = fmaxf(w, 0.f);
+        uint32_t lo[2], hi[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int q = qs[j];
+            const float xlo = blo[q] + fminf(tn * bd.rmin[j], tf * bd.rmin[j]);
+            const float xhi = bhi[q] + fmaxf(tn * bd.rmax[j], tf * bd.rmax[j]);
+            const float mg = 1e-3f * m.h[q];
+            if (xlo <= m.bminf[q] + mg || xhi >= m.bmaxf[q] - mg) return lb * (1.f - 1e-5f);
+            const float flo = (xlo - m.e0[q]) * m.invH[q] - 1e-3f, fhi = (xhi - m.e0[q]) * m.invH[q] + 1e-3f;
+            const int n1 = (int)m.n[q] - 1;
+            const int il = (int)floorf(flo), ih = (int)floorf(fhi);
+            lo[j] = (uint32_t)(il < 0 ? 0 : (il > n1 ? n1 : il));
+            hi[j] = (uint32_t)(ih < 0 ? 0 : (ih > n1 ? n1 : ih));
+        }
+        if (w > 0.f) lb = fmaf(w, pyrMinSpan(mp, (uint32_t)L, lo[0], hi[0], lo[1], hi[1], span), lb);
+    }
+    return lb * (1.f - 1e-5f);
+}
+
+// The sub-box of an FP32 point inside the medium's box for tables of `sub` x `sub` x `sub` boxes a
+// cell (cell c, box [blo, bhi]); returns the box's linear index, x fastest over the n * sub boxes
+// of each axis.
+__host__ __device__ inline uint64_t farSubBoxOf(const MediumView& m, const f3& pf, uint32_t sub, uint32_t c[3],
+                                                float blo[3], float bhi[3]) {
+    const float p[3] = {pf.x, pf.y, pf.z};
+    uint32_t k[3];
+    for (int a = 0; a < 3; ++a) {
+        c[a] = (uint32_t)locateUniform<float>(m, a, p[a]);
+        const float fs = (p[a] - m.ef[a][c[a]]) * m.invH[a] * (float)sub;
+        const int ks = (int)floorf(fs);
+        k[a] = (uint32_t)(ks < 0 ? 0 : (ks >= (int)sub ? (int)sub - 1 : ks));
+        farSubBoxFaces(m, a, c[a], k[a], sub, &blo[a], &bhi[a]);
+    }
+    return (((uint64_t)c[2] * sub + k[2]) * ((uint64_t)m.n[1] * sub) + c[1] * sub + k[1]) * ((uint64_t)m.n[0] * sub) +
+           c[0] * sub + k[0];
+}
 
 // The order in which the far-field view kernel takes n sources (FP32 directions, 3 each) walked
 // along sg * d: by Morton code of the ranks of their lateral slopes about `axis`, so that each run

@@ -82,16 +82,21 @@ hipError_t launchBatchTotals(const uint32_t* misses, uint32_t* off, uint32_t n, 
                              uint32_t* status, hipStream_t st);
 
 // The NEE stage in two parts. kNeeVertexPart reads the batch's vertices only: k_tau_ref (when
-// p.tauRef is set) and k_vertex_prep (per-vertex records). kNeeTermPart forms the terms: with
-// vb.far and fb first k_far_bound and k_far_blocks (which blocks of the far-field view kernel's
-// sources each vertex's walks can reach inside the cut, from lower bounds on their optical
-// depth), k_nee (or k_nee_corr) for the first vertex, then the view kernels for vertices k >= 2;
-// `between`, when not null, is recorded on `st` between the two.
-enum : int { kNeeVertexPart = 1, kNeeTermPart = 2, kNeeAll = 3 };
+// p.tauRef is set), k_vertex_prep (per-vertex records) and, with vb.far and a table in fb,
+// k_far_live (which blocks of the far-field view kernel's sources each record's walks can reach
+// inside the cut, from the cell bounds). kNeeTermPart forms the terms: k_nee (or k_nee_corr) for
+// the first vertex, then the view kernels for vertices k >= 2; `between`, when not null, is
+// recorded on `st` between the two.
+enum : int { kNeeRefPart = 1, kNeeRecordPart = 2, kNeeVertexPart = 3, kNeeTermPart = 4, kNeeAll = 7 };
 hipError_t launchNee(int dims, int fp64, const MediumView& m, const ScatterView& sc, const SourceView& il,
                      const SourceView& vw, const FramesView& fr, const VertexBatch& vb, const NeeParams& p,
                      const NeeOut& out, hipStream_t st, hipEvent_t between = nullptr, const FarBound* fb = nullptr,
                      int parts = kNeeAll);
+// FarBound::table of one role: for every sub-box of m (sub x sub x sub a cell) and every column
+// j < cols (at most 32, at most `blocks`), the least farBoxBound(span) over block j's directions
+// (column cols - 1: blocks cols - 1 and up), on st.
+hipError_t launchFarTable(const MediumView& m, const MinPyramid& mp, const BoundDirs* blockDirs, uint32_t blocks,
+                          uint32_t cols, uint32_t sub, uint32_t span, float* table, hipStream_t st);
 
 // TemporalCorrelationSampler with coherent backscattering: the mixed pairings CBS02 + CBS03 of
 // every path with at least two vertices, from the NeeOut first-vertex (F*) and per-vertex (G*)

@@ -735,45 +735,67 @@ __device__ inline void writeFarRec(const MediumView& m, const VertexBatch& vb, u
     if (vb.live) vb.live[2 * idx] = vb.live[2 * idx + 1] = 0xffffffffu;
 }
 
-// Vertex records for the view terms: one thread per path walks its vertices once per frame and
-// writes, for every vertex k >= 2, the unit vector back to the previous vertex (the direction the
-// view-side HG amplitude is evaluated against, FarField.cuh:546-599 / PointSource.cuh:347-446)
-// and the phase base rho_k + L_k^t / lambda with L_k^t the FP64 path length from x_1^t
-// (pathContriobutionKernel, TemporalSampler.cuh:668-719). Done once per (vertex, frame) here
-// instead of once per (vertex, view, frame) in k_nee_view. With vb.far (3D) also the far-field
-// view kernel's record of every vertex, the first included (k_nee's single-scattering view walks
-// read its live-block masks).
+// Vertex records for the view terms: for every vertex k >= 2 and frame, the unit vector back to
+// the previous vertex (the direction the view-side HG amplitude is evaluated against,
+// FarField.cuh:546-599 / PointSource.cuh:347-446) and the phase base rho_k + L_k^t / lambda with
+// L_k^t the FP64 path length from x_1^t (pathContriobutionKernel, TemporalSampler.cuh:668-719).
+// Done once per (vertex, frame) here instead of once per (vertex, view, frame) in k_nee_view.
+// With vb.far (3D) also the far-field view kernel's record of every vertex, the first included
+// (k_nee's single-scattering view walks read its live-block masks). kPrepLanes lanes take a path,
+// one vertex each in runs of kPrepLanes, and form L_k as an inclusive scan of the segment lengths
+// over the run plus the run before's total, so that a path of K vertices takes K / kPrepLanes
+// dependent steps (path lengths are heavy-tailed; the longest paths of a batch set its duration).
+constexpr uint32_t kPrepLanes = 16;
 template <int DIMS>
 __global__ void __launch_bounds__(256) k_vertex_prep(MediumView m, VertexBatch vb, FramesView fr, uint32_t npaths,
                                                      double invLambda) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x % kPrepLanes;
+    const uint32_t p = (blockIdx.x * blockDim.x + threadIdx.x) / kPrepLanes;
+    // every exit below is uniform over a path's lanes, which the scan's shuffles stay within
     if (p >= npaths) return;
     if ((uint64_t)vb.off[npaths] > vb.cap) return;   // pool overflow: the batch is re-run
     const uint32_t K = vb.k[p], NT = fr.nt;
     if (K == 0) return;
     const uint64_t off = vb.off[p];
     for (uint32_t t = 0; t < NT; ++t) {
-        float4 xpm = vb.xm[off];
-        Geo xp = vertexFrame(vb, fr, off, t, xpm);
-        if (DIMS == 3 && vb.far)
-            writeFarRec<DIMS>(m, vb, off * NT + t, xp, __float_as_uint(xpm.w), mkd(0.0, 0.0, 0.0), (double)vb.rho[off]);
-        double len = 0;
-        for (uint32_t k = 1; k < K; ++k) {
-            const uint64_t ok = off + k;
+        double carry = 0.0;
+        for (uint32_t k0 = 0; k0 < K; k0 += kPrepLanes) {
+            const uint32_t k = k0 + lane;
+            const bool act = k < K;
+            const uint64_t ok = off + (act ? k : 0u);
             const float4 xkm = vb.xm[ok];
             const Geo xk = vertexFrame(vb, fr, ok, t, xkm);
-            const d3 sg = dsub(xk.pd, xp.pd);
-            len += sqrt(ddot<DIMS>(sg, sg));
-            const d3 pk = fr.temporalAmp ? xk.pd : mkd(xkm.x, xkm.y, xkm.z);
-            const d3 pp = fr.temporalAmp ? xp.pd : mkd(xpm.x, xpm.y, xpm.z);
-            const d3 b = dnormalize<DIMS>(dsub(pp, pk));
+            double seg = 0.0;
+            d3 b = mkd(0.0, 0.0, 0.0);
+            if (act && k > 0) {
+                const float4 xpm = vb.xm[ok - 1];
+                const Geo xp = vertexFrame(vb, fr, ok - 1, t, xpm);
+                const d3 sg = dsub(xk.pd, xp.pd);
+                seg = sqrt(ddot<DIMS>(sg, sg));
+                const d3 pk = fr.temporalAmp ? xk.pd : mkd(xkm.x, xkm.y, xkm.z);
+                const d3 pp = fr.temporalAmp ? xp.pd : mkd(xpm.x, xpm.y, xpm.z);
+                b = dnormalize<DIMS>(dsub(pp, pk));
+            }
+            double len = seg;
+#pragma unroll
+            for (uint32_t d = 1; d < kPrepLanes; d <<= 1) {
+                const double u = __shfl_up(len, d, kPrepLanes);
+                if (lane >= d) len += u;
+            }
+            len += carry;
+            carry = __shfl(len, kPrepLanes - 1, kPrepLanes);
+            if (!act) continue;
+            if (k == 0) {
+                if (DIMS == 3 && vb.far)
+                    writeFarRec<DIMS>(m, vb, off * NT + t, xk, __float_as_uint(xkm.w), mkd(0.0, 0.0, 0.0),
+                                      (double)vb.rho[off]);
+                continue;
+            }
             VertexRec r;
             r.bx = b.x; r.by = b.y; r.bz = b.z;
             r.phase = (double)vb.rho[ok] + len * invLambda;
             vb.rec[ok * NT + t] = r;
             if (DIMS == 3 && vb.far) writeFarRec<DIMS>(m, vb, ok * NT + t, xk, __float_as_uint(xkm.w), b, r.phase);
-            xpm = xkm;
-            xp = xk;
         }
     }
 }
@@ -914,59 +936,65 @@ k_nee_view(MediumView m, ScatterView sc, SourceView vw, FramesView fr, VertexBat
 
 // The live-block masks of the far-field view kernel (VertexBatch::live) for every record of the batch:
 // live[0] for the views under the launch's cut (viewCut), live[1] for the reversed paths'
-// illuminations under the exact cut; a block is dead when the lower bound of its sources' walks
-// (farLowerBound over the block's direction set) passes the cut's skip threshold. k_far_bound,
-// one thread per record, tries the whole set's bound, which kills every block at once; a record it
-// leaves alive goes to a candidate list, and k_far_blocks bounds each (candidate, block) pair in a
-// thread of its own and sets the pair's bit. Records of first vertices, which no kernel reads, are
-// bounded too (from whatever they hold; every index is clamped).
+// illuminations under the exact cut; bit j is clear when the cell bound of block j (FarBound::table)
+// passes the cut's skip threshold. One thread per record, a gather of one table row; records of
+// first vertices, which only k_nee's single-scattering walks read, included.
 __device__ inline float farRoleSkip(const NeeParams& prm, int role) {
     return role == 0 ? farSkip(viewCut<float>(prm)) : farSkip(prm.exactCutF);
 }
-__global__ void k_far_reset(uint32_t* count) {
-    if (threadIdx.x == 0) *count = 0u;
-}
-__global__ void __launch_bounds__(256) k_far_bound(MediumView m, VertexBatch vb, FramesView fr, NeeParams prm,
-                                                   FarBound fb, uint32_t doIllum) {
+__global__ void __launch_bounds__(256) k_far_live(MediumView m, VertexBatch vb, FramesView fr, NeeParams prm,
+                                                  FarBound fb, uint32_t doIllum) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     const uint32_t P = prm.npaths;
     if ((uint64_t)vb.off[P] > vb.cap || i >= (uint64_t)vb.off[P] * fr.nt) return;
-    FarRec& r = vb.far[i];
+    const FarRec& r = vb.far[i];
     const f3 pf = mk((float)r.px, (float)r.py, (float)r.pz);
+    const bool in = inside<3, float>(m, pf);
+    uint32_t c[3];
+    float blo[3], bhi[3];
+    const uint64_t box = in ? farSubBoxOf(m, pf, fb.sub, c, blo, bhi) : 0;
 #pragma unroll
     for (int role = 0; role < 2; ++role) {
-        bool cand = false;
         uint32_t live = 0xffffffffu;
-        if (role == 0 || doIllum) {
+        if (in && fb.table[role] && (role == 0 || doIllum)) {
             const float skip = farRoleSkip(prm, role);
-            if (farLowerBound(m, fb.pyr[role], fb.dirs[role], pf, skip) > skip) live = 0u;
-            else if (fb.blockDirs[role]) {
-                live = 0u;
-                cand = true;
-            }
+            const uint32_t nc = fb.cols[role];
+            const float* t = fb.table[role] + box * nc;
+            live = 0u;
+            for (uint32_t j = 0; j < nc; ++j) live |= (t[j] > skip ? 0u : 1u) << j;
         }
         vb.live[2 * i + role] = live;
-        const uint64_t cm = __ballot(cand);
-        if (cm) {
-            const uint32_t slot = waveTake(fb.candCount, cm, cand);
-            if (cand && slot < fb.candCap) fb.cand[slot] = (uint32_t)i * 2u + (uint32_t)role;
-        }
     }
 }
-__global__ void __launch_bounds__(256) k_far_blocks(MediumView m, VertexBatch vb, NeeParams prm, FarBound fb) {
-    const uint32_t P = prm.npaths;
-    if ((uint64_t)vb.off[P] > vb.cap) return;
-    const uint64_t nc = min((uint64_t)*fb.candCount, fb.candCap);
-    const uint32_t nb = fb.blocks[0] > fb.blocks[1] ? fb.blocks[0] : fb.blocks[1];
-    for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < nc * nb; j += (uint64_t)gridDim.x * 256) {
-        const uint32_t e = fb.cand[j / nb], b = (uint32_t)(j % nb), role = e & 1u;
-        if (b >= fb.blocks[role]) continue;
-        const FarRec& r = vb.far[e >> 1];
-        const f3 pf = mk((float)r.px, (float)r.py, (float)r.pz);
-        const float skip = farRoleSkip(prm, (int)role);
-        if (!(farLowerBound(m, fb.pyr[role], fb.blockDirs[role][b], pf, skip) > skip))
-            atomicOr(vb.live + 2 * (uint64_t)(e >> 1) + role, 1u << (b < 31u ? b : 31u));
+
+// FarBound::table of one role: column j of every sub-box, one thread per (box, column).
+__global__ void __launch_bounds__(256) k_far_table(MediumView m, MinPyramid mp, const BoundDirs* blockDirs,
+                                                   uint32_t blocks, uint32_t cols, uint32_t sub, uint32_t span,
+                                                   float* table) {
+    const uint64_t nx = (uint64_t)m.n[0] * sub, ny = (uint64_t)m.n[1] * sub, nz = (uint64_t)m.n[2] * sub;
+    const uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= nx * ny * nz * cols) return;
+    const uint64_t box = j / cols;
+    const uint32_t col = (uint32_t)(j % cols), last = col + 1 < cols ? col + 1 : blocks;
+    const uint64_t bx[3] = {box % nx, box / nx % ny, box / (nx * ny)};
+    uint32_t c[3];
+    float blo[3], bhi[3];
+    for (int a = 0; a < 3; ++a) {
+        c[a] = (uint32_t)(bx[a] / sub);
+        farSubBoxFaces(m, a, c[a], (uint32_t)(bx[a] % sub), sub, &blo[a], &bhi[a]);
     }
+    float lb = 3.0e38f;
+    for (uint32_t b = col; b < last; ++b) lb = fminf(lb, farBoxBound(m, mp, blockDirs[b], c, blo, bhi, span));
+    table[j] = lb;
+}
+
+hipError_t launchFarTable(const MediumView& m, const MinPyramid& mp, const BoundDirs* blockDirs, uint32_t blocks,
+                          uint32_t cols, uint32_t sub, uint32_t span, float* table, hipStream_t st) {
+    const uint64_t n = (uint64_t)m.n[0] * m.n[1] * m.n[2] * sub * sub * sub * cols;
+    if (!n || cols > 32 || cols > blocks || !sub || !span) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_far_table, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, m, mp, blockDirs, blocks,
+                       cols, sub, span, table);
+    return hipGetLastError();
 }
 
 // HG amplitude constants per material for the far-field view kernel: sqrt((1 - g^2) / 4 pi),
@@ -1426,71 +1454,57 @@ hipError_t launchCbsCross(int dims, int fp64, const SourceView& il, const Source
 }
 
 // The reference of the relative attenuation cut: the batch's least view optical depth. One
-// wavefront per sampled path (every pathStride-th), one view per lane (every view when there are at
-// most 64, else 64 spread evenly), the path's vertices k >= 1 at frame 0, FP32 walks. A walk stops
-// once it passes the least depth found so far (by its lane, or by the batch as last read), which
-// cannot change the minimum: the least pair is walked to its end. The result is the exact minimum
-// over the pairs evaluated, hence at or above the least depth over all pairs, so the cut
-// tauRef + delta never drops a term that the same cut about the true least depth would keep.
+// wavefront per sampled vertex (every vertexStride-th of the batch's pool, frame 0), one view per
+// lane (every view when there are at most 64, else 64 spread evenly), FP32 walks. A walk stops once
+// it passes the least depth the batch has published when its wavefront starts, which cannot change
+// the minimum: the least pair is walked to its end. The result is the exact minimum over the pairs
+// evaluated (whatever order the wavefronts run in), hence at or above the least depth over all
+// pairs, so the cut tauRef + delta never drops a term that the same cut about the true least depth
+// would keep.
 template <int DIMS, int SRC>
 __global__ void __launch_bounds__(256) k_tau_ref(MediumView m, SourceView vw, FramesView fr, VertexBatch vb,
-                                                 NeeParams prm, uint32_t pathStride) {
+                                                 NeeParams prm, uint32_t vertexStride) {
     __shared__ float sigLds[kSigLds];
     const float* sig = loadSigTable<float>(m, sigLds);
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t unit = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
-    const uint64_t p64 = (uint64_t)unit * pathStride;
-    const uint32_t P = prm.npaths;
-    const bool poolOk = (uint64_t)vb.off[P] <= vb.cap;
-    const uint32_t K = (p64 < P && poolOk) ? vb.k[(uint32_t)p64] : 0u;
+    const uint64_t o = (uint64_t)unit * vertexStride;
+    const uint64_t nv = vb.off[prm.npaths];
+    if (nv > vb.cap || o >= nv) return;
     const uint32_t n = vw.n;
     unsigned int* ref = (unsigned int*)prm.tauRef;   // non-negative floats order as their bits
-    float best = 3.0e38f, published = 3.0e38f;
-    if (K > 0) {
-        const uint64_t off = vb.off[(uint32_t)p64];
-        const bool active = lane < n;
-        const uint32_t v = !active ? 0u : (n <= 64 ? lane : (uint32_t)(((uint64_t)lane * n) / 64u));
+    const float best = __uint_as_float(__hip_atomic_load(ref, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const bool active = lane < n;
+    const uint32_t v = !active ? 0u : (n <= 64 ? lane : (uint32_t)(((uint64_t)lane * n) / 64u));
+    const Geo x = vertexFrame(vb, fr, o, 0, vb.xm[o]);
+    float tau = 3.0e38f;
+    if (active) {
         const d3 vd = sourceD<DIMS>(vw, v);
-        for (uint32_t k = 0; k < K; ++k) {
-            if ((k & 7u) == 0)
-                best = fminf(best, __uint_as_float(__hip_atomic_load(ref, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
-            const uint64_t o = off + k;
-            const Geo x = vertexFrame(vb, fr, o, 0, vb.xm[o]);
-            float tau = 3.0e38f;
-            if (active) {
-                f3 w;
-                if constexpr (SRC == kSrcPoint) {
-                    const d3 q = dsub(vd, x.pd);
-                    const double rr = 1.0 / sqrt(ddot<DIMS>(q, q));
-                    w = mk((float)(q.x * rr), (float)(q.y * rr), (float)(q.z * rr));
-                } else if constexpr (SRC == kSrcGauss) {
-                    const d3 b = gbDir(vw.g, v);
-                    w = mk((float)b.x, (float)b.y, (float)b.z);
-                } else {
-                    w = sourceF(vw, v);
-                }
-                tau = walk<DIMS, float>(m, sig, x.pf, w, best, nullptr);
-            }
-            best = fminf(best, tau);
-            if ((k & 7u) == 7u || k + 1 == K) {
-                float wb = best;
-#pragma unroll
-                for (int s2 = 32; s2 > 0; s2 >>= 1) wb = fminf(wb, __shfl_xor(wb, s2, 64));
-                if (lane == 0 && wb < published) {
-                    atomicMin(ref, __float_as_uint(wb));
-                    published = wb;
-                }
-            }
+        f3 w;
+        if constexpr (SRC == kSrcPoint) {
+            const d3 q = dsub(vd, x.pd);
+            const double rr = 1.0 / sqrt(ddot<DIMS>(q, q));
+            w = mk((float)(q.x * rr), (float)(q.y * rr), (float)(q.z * rr));
+        } else if constexpr (SRC == kSrcGauss) {
+            const d3 b = gbDir(vw.g, v);
+            w = mk((float)b.x, (float)b.y, (float)b.z);
+        } else {
+            w = sourceF(vw, v);
         }
+        tau = walk<DIMS, float>(m, sig, x.pf, w, best, nullptr);
     }
+#pragma unroll
+    for (int s2 = 32; s2 > 0; s2 >>= 1) tau = fminf(tau, __shfl_xor(tau, s2, 64));
+    if (lane == 0 && tau < best) atomicMin(ref, __float_as_uint(tau));
 }
 
 template <int DIMS>
 static void launchTauRef(const MediumView& m, const SourceView& vw, const FramesView& fr, const VertexBatch& vb,
                          const NeeParams& p, hipStream_t st) {
+    // about as many vertices as 4096 whole paths hold
     const uint32_t stride = p.npaths > 4096 ? p.npaths / 4096 : 1u;
-    const uint32_t units = (p.npaths + stride - 1) / stride;
-    const dim3 grid((units + 3) / 4), block(256);
+    const uint64_t units = (vb.cap + stride - 1) / stride;
+    const dim3 grid((uint32_t)((units + 3) / 4)), block(256);
     if (vw.type == 2) hipLaunchKernelGGL((k_tau_ref<DIMS, kSrcFarView>), grid, block, 0, st, m, vw, fr, vb, p, stride);
     else if (vw.type == 1) hipLaunchKernelGGL((k_tau_ref<DIMS, kSrcPoint>), grid, block, 0, st, m, vw, fr, vb, p, stride);
     else hipLaunchKernelGGL((k_tau_ref<DIMS, kSrcGauss>), grid, block, 0, st, m, vw, fr, vb, p, stride);
@@ -1560,25 +1574,21 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
                        hipStream_t st, hipEvent_t between, const FarBound* fb, int parts) {
     const uint32_t ppb = 4 * (64 / p.lpp);   // paths per 256-thread block
     dim3 grid((p.npaths + ppb - 1) / ppb), block(256);
-    if (parts & kNeeVertexPart) {
-        // the cut's reference and the per-vertex records read the batch's vertices only
-        if (p.tauRef) launchTauRef<DIMS>(m, vw, fr, vb, p, st);
+    // the cut's reference, the per-vertex records and, for the far-field view kernel, which blocks
+    // of its sources each record's walks can reach inside the cut: all read the batch's vertices only
+    if ((parts & kNeeRefPart) && p.tauRef) launchTauRef<DIMS>(m, vw, fr, vb, p, st);
+    if (parts & kNeeRecordPart) {
         FramesView rf = fr;
         if (fr.analytic) rf.nt = 1;   // TemporalCorrelationSampler: vertices never move, one record each
-        hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 255) / 256), block, 0, st, m, vb, rf, p.npaths,
-                           1.0 / p.lambda);
+        hipLaunchKernelGGL(k_vertex_prep<DIMS>, dim3((p.npaths + 256 / kPrepLanes - 1) / (256 / kPrepLanes)), block,
+                           0, st, m, vb, rf, p.npaths, 1.0 / p.lambda);
+        if (DIMS == 3 && !fr.analytic && vb.far && fb && (fb->table[0] || fb->table[1])) {
+            const uint64_t n = vb.cap * fr.nt;
+            hipLaunchKernelGGL(k_far_live, dim3((uint32_t)((n + 255) / 256)), block, 0, st, m, vb, fr, p, *fb,
+                               out.Br ? 1u : 0u);
+        }
     }
     if (!(parts & kNeeTermPart)) return;
-    if (DIMS == 3 && !fr.analytic && vb.far && fb) {
-        // which blocks of the far-field view kernel's sources each record's walks can reach inside
-        // the cut (the bounds need the cut reference)
-        const uint64_t n = vb.cap * fr.nt;
-        hipLaunchKernelGGL(k_far_reset, dim3(1), dim3(64), 0, st, fb->candCount);
-        hipLaunchKernelGGL(k_far_bound, dim3((uint32_t)((n + 255) / 256)), block, 0, st, m, vb, fr, p, *fb,
-                           out.Br ? 1u : 0u);
-        if (fb->blockDirs[0] || fb->blockDirs[1])
-            hipLaunchKernelGGL(k_far_blocks, dim3(2048), block, 0, st, m, vb, p, *fb);
-    }
     if (fr.analytic) {
         if (sc.tab) hipLaunchKernelGGL((k_nee_corr<DIMS, R, true>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
         else hipLaunchKernelGGL((k_nee_corr<DIMS, R, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);

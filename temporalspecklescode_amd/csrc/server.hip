@@ -172,7 +172,8 @@ static uint64_t estimate(const sstmc_config& c) {
     const uint32_t R = c.simulation.renderingsNum ? c.simulation.renderingsNum : 1;
     return NextEventEstimation::footprint(P, c.illumination.num, c.view.num, c.sampler.tNum, fp64, frames, cells,
                                           c.tracer.isCBS != 0,
-                                          c.sampler.type == SSTMC_SAMPLER_TEMPORAL_CORRELATION) +
+                                          c.sampler.type == SSTMC_SAMPLER_TEMPORAL_CORRELATION,
+                                          (uint64_t)P * (c.simulation.fullIteration ? c.simulation.fullIteration : 1)) +
            16ull * c.illumination.num * c.view.num * c.sampler.tNum * R;   // the result buffer
 }
 

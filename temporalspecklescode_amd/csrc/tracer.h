@@ -268,9 +268,13 @@ public:
     int enqueue();
     int finish(double* u, bool deviceDst = false);
     uint32_t batches() const { return opt_.fullIteration; }
-    // device bytes a tracer with these sizes allocates (admission control)
+    // device bytes a tracer with these sizes allocates (admission control); `paths` per rendering
+    // (batch size times batches) sets the resolution of the far-field view kernel's bound tables
     static uint64_t footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames, uint64_t cells,
-                              bool cbs, bool acorr = false);
+                              bool cbs, bool acorr, uint64_t paths);
+    // the resolution of those tables: sub x sub x sub boxes a cell, pyramid queries of up to
+    // (span + 1)^2 blocks
+    static void farTableResolution(uint64_t paths, uint32_t* sub, uint32_t* span);
     // device bytes of one batch set (terms, per-path arrays, a vertex pool of 32 vertices a path)
     static uint64_t setBytes(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames, bool cbs,
                              bool acorr);
@@ -347,11 +351,9 @@ private:
     bool sameIllums_ = false;       // correlation sampler: illumination2 holds the same sources
     bool cbsCorr_ = false;          // coherent backscattering under the correlation sampler
     bool farRecs_ = false;          // the far-field view kernel runs: FarRec per (vertex, frame)
-    FarBound farBound_{};           // its per-vertex lower bounds (k_far_bound)
+    FarBound farBound_{};           // its source order and per-cell walk bounds (k_far_live)
     DevBuf<uint32_t> farPerm_[2];   // the kernel's source order per role (FarBound::perm)
-    DevBuf<BoundDirs> farBlockDirs_[2];
-    DevBuf<uint32_t> farCand_;      // FarBound::cand (two entries a record) and its count
-    DevBuf<uint32_t> farCandCount_;
+    DevBuf<float> farTable_[2];     // FarBound::table per role
     bool farBoundOn_ = false;
     int setupFarBound();
     // the contraction stage captured as HIP graphs, per (set, batch size, destination): the

@@ -934,14 +934,6 @@ int NextEventEstimation::growPool(uint64_t cap) {
         perVertex += 2 * (uint64_t)T_ * (L_ + V_) * elem;
     }
     bytes_ += (uint64_t)nsets_ * (cap - cap_) * perVertex;
-    // the far bound's candidate list (one list: the bound kernels of every set run on one stream)
-    if (farBoundOn_) {
-        if ((e = farCand_.alloc(2 * cap * recT)) || (!farCandCount_.p && (e = farCandCount_.alloc(1)))) return e;
-        bytes_ += 8 * (cap - cap_) * recT;
-        farBound_.cand = farCand_.p;
-        farBound_.candCount = farCandCount_.p;
-        farBound_.candCap = 2 * cap * recT;
-    }
     // the transport's staging: the pool plus the chunk tails of every path (TransportStage)
     const uint64_t stageNeed = cap + (uint64_t)P_ * (kStageChunk - 1);
     const uint64_t nchunks = (stageNeed + kStageChunk - 1) / kStageChunk + 1;
@@ -1184,8 +1176,18 @@ uint64_t NextEventEstimation::setBytes(uint32_t P, uint32_t L, uint32_t V, uint3
     return terms + pool + 24ull * P;
 }
 
+// Boxes of half a cell along each axis and pyramid queries of up to 5 x 5 blocks for renderings of
+// 2^21 paths and up (128^3 cells, 16 blocks of views: about 0.2 s to build, and as tight as the
+// bound at each point, farLowerBound), else whole cells and queries of up to 9 x 9 blocks (an
+// eighth of the boxes, about 60 ms).
+void NextEventEstimation::farTableResolution(uint64_t paths, uint32_t* sub, uint32_t* span) {
+    const bool fine = paths >= (1ull << 21);
+    *sub = fine ? 2u : 1u;
+    *span = fine ? 4u : 8u;
+}
+
 uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint32_t T, bool fp64, bool frames,
-                                        uint64_t cells, bool cbs, bool acorr) {
+                                        uint64_t cells, bool cbs, bool acorr, uint64_t paths) {
     const uint64_t elem = fp64 ? 16 : 8;
     // the pipeline's batch sets: terms, per-path arrays and vertex pools
     const uint64_t set = setBytes(P, L, V, T, fp64, frames, cbs, acorr);
@@ -1193,6 +1195,14 @@ uint64_t NextEventEstimation::footprint(uint32_t P, uint32_t L, uint32_t V, uint
     uint64_t ws = 2ull * 8 * T * L * V * elem;   // split-K partials, two workspaces
     if (!fp64) ws += 2ull * 8 * ((uint64_t)P + 64) * (L + V + 256);   // and their bf16 hi/lo planes
     uint64_t grids = 2 * cells;                  // material grid + packed walk grid (<= 4 bits a cell)
+    // the far-field view kernel's bound tables: 4 bytes a box per block of 64 sources, at most 32
+    if (!fp64) {
+        uint32_t sub, span;
+        farTableResolution(paths, &sub, &span);
+        const uint64_t boxes = cells * sub * sub * sub;
+        grids += 4 * boxes * std::min<uint64_t>(32, (V + 63) / 64);
+        if (cbs) grids += 4 * boxes * std::min<uint64_t>(32, (L + 63) / 64);
+    }
     if (cbs && acorr) field += 16ull * cbsCrossSplits(L, V, T, P) * T * L * V;   // mixed-pairing slices
     // the transport's staging: a pool of 32 vertices a path plus the chunk tails
     const uint64_t stage = (uint64_t)P * (32 + kStageChunk) * (20 + (frames ? 12ull * T : 0) + 1);
@@ -1311,11 +1321,16 @@ int Medium::minPyramid(int axis, MinPyramid* out) const {
 // The far-field view kernel's lower bounds: per launch (views; reversed-path illuminations) the
 // axis the walks all advance along, its sign and the lateral slopes over the walks' FP32
 // directions (widened by a relative 1e-6), and the medium's pyramid along that axis. A set whose
-// walks do not all advance along one axis by at least 0.1 of their length gets no bound.
+// walks do not all advance along one axis by at least 0.1 of their length gets no bound. The
+// kernel takes its sources in a Z-order of their lateral slopes (compact cones of 64); the bound of
+// each block of 64 about the set's axis is tabulated per cell of the medium once, here.
 int NextEventEstimation::setupFarBound() {
     farBound_ = FarBound{};
     farBoundOn_ = false;
     if (!farRecs_) return SSTMC_NO_ERROR;
+    uint32_t sub, span;
+    farTableResolution(paths(), &sub, &span);
+    const uint64_t boxes = (uint64_t)mview_.n[0] * mview_.n[1] * mview_.n[2] * sub * sub * sub;
     for (int role = 0; role < 2; ++role) {
         const Source& src = role == 0 ? vw_ : il_;
         if (src.type() != SSTMC_SOURCE_FARFIELD || (role == 1 && !opt_.cbs)) continue;
@@ -1324,30 +1339,37 @@ int NextEventEstimation::setupFarBound() {
         std::vector<float> df(hp.size());
         for (size_t i = 0; i < hp.size(); ++i) df[i] = (float)hp[i];
         int axis = -1;
-        const BoundDirs bd = boundDirsHost(df.data(), df.size() / 3, role == 0 ? 1.0 : -1.0, &axis);
+        const double sg = role == 0 ? 1.0 : -1.0;
+        const BoundDirs bd = boundDirsHost(df.data(), df.size() / 3, sg, &axis);
         if (!bd.valid) continue;
         MinPyramid mp;
         const int e = medium_.minPyramid(axis, &mp);
         if (e) return e;
         if (mp.axis < 0) continue;
-        farBound_.pyr[role] = mp;
-        farBound_.dirs[role] = bd;
-        farBoundOn_ = true;
-        // the kernel's source order (compact cones of 64) and each block's direction set
         const size_t n = df.size() / 3;
-        const std::vector<uint32_t> perm = farOrderHost(df.data(), n, role == 0 ? 1.0 : -1.0, axis);
+        const std::vector<uint32_t> perm = farOrderHost(df.data(), n, sg, axis);
         std::vector<BoundDirs> blocks((n + 63) / 64);
         for (size_t b = 0; b < blocks.size(); ++b) {
             const size_t m = std::min<size_t>(64, n - 64 * b);
-            blocks[b] = boundDirsAxisHost(df.data(), perm.data() + 64 * b, m, role == 0 ? 1.0 : -1.0, axis, bd.sgn);
+            blocks[b] = boundDirsAxisHost(df.data(), perm.data() + 64 * b, m, sg, axis, bd.sgn);
         }
+        const uint32_t cols = (uint32_t)std::min<size_t>(blocks.size(), 32);
+        DevBuf<BoundDirs> blockDirs;
         int e2;
         if ((e2 = farPerm_[role].upload(perm.data(), perm.size())) ||
-            (e2 = farBlockDirs_[role].upload(blocks.data(), blocks.size())))
+            (e2 = blockDirs.upload(blocks.data(), blocks.size())) || (e2 = farTable_[role].alloc(boxes * cols)))
             return e2;
+        if (launchFarTable(mview_, mp, blockDirs.p, (uint32_t)blocks.size(), cols, sub, span, farTable_[role].p,
+                           stream_) != hipSuccess ||
+            hipStreamSynchronize(stream_) != hipSuccess)
+            return SSTMC_KERNEL_ERROR;
+        bytes_ += 4 * boxes * cols + 4 * perm.size();
         farBound_.perm[role] = farPerm_[role].p;
-        farBound_.blockDirs[role] = farBlockDirs_[role].p;
+        farBound_.table[role] = farTable_[role].p;
         farBound_.blocks[role] = (uint32_t)blocks.size();
+        farBound_.cols[role] = cols;
+        farBound_.sub = sub;
+        farBoundOn_ = true;
     }
     return SSTMC_NO_ERROR;
 }

@@ -138,7 +138,8 @@ static int run(int n, double h, int trials, unsigned seed, int fast, int nmat) {
 }
 
 // The far-field view kernel's lower bounds (farLowerBound over a MinPyramid, BoundDirs of a cone of
-// walk directions, and the same bound per block of its source order, blocks of farOrderHost): for every point and every direction of the set (block), the FP32 fast walk's
+// walk directions, and the same bound per block of its source order, blocks of farOrderHost, at the
+// point and for boxes holding it, farBoxBound): for every point and every direction of the set (block), the FP32 fast walk's
 // tau is at least the bound, and wherever the bound passes farSkip(cut) the walk under that cut ends past
 // it, so skipping the point's walks changes no term. Grids of smooth blobs (box-blurred noise cut
 // into nmat levels) so that the bound is often large; cones of up to 30 degrees about random
@@ -199,7 +200,7 @@ static int runBound(int n, double h, int trials, unsigned seed, int fast, int nm
         pyr[a].p = pyrData[a].data();
     }
     std::uniform_real_distribution<double> U(ed[0], ed[n]), N(-1, 1);
-    long bad = 0, tested = 0, skipped = 0, valid = 0, blockTested = 0, blockSkipped = 0;
+    long bad = 0, tested = 0, skipped = 0, valid = 0, blockTested = 0, blockSkipped = 0, cellSkipped = 0;
     double tight = 0;
     const float cuts[4] = {2.f, 8.f, 24.f, 210.f};
     for (int t = 0; t < trials; ++t) {
@@ -258,24 +259,40 @@ static int runBound(int n, double h, int trials, unsigned seed, int fast, int nm
             ++tested;
             if (lb > farSkip(24.f)) ++skipped;
             if (minTau > 0) tight += lb / minTau;
-            // the per-block bound of the kernel's source order: blocks of 16 directions about the
-            // whole set's axis
+            // the per-block bound of the kernel's source order (blocks of 16 directions about the
+            // whole set's axis), at the point and for boxes holding it (the tabulated form: whole
+            // cells and halves, coarse and fine pyramid queries): each at most every walk of the block
+            uint32_t cc[3];
+            float blo[2][3], bhi[2][3];
+            farSubBoxOf(m, pf, 1, cc, blo[0], bhi[0]);
+            farSubBoxOf(m, pf, 2, cc, blo[1], bhi[1]);
+            for (int a = 0; a < 3; ++a)
+                if (!(blo[1][a] <= p[a] + 1e-5 * h && p[a] <= bhi[1][a] + 1e-5 * h)) {
+                    if (bad < 5) printf("  set %d point %d: half box [%g, %g] misses %g\n", t, q, blo[1][a], bhi[1][a], p[a]);
+                    ++bad;
+                }
             for (size_t b = 0; b < 48; b += 16) {
                 const BoundDirs bb = boundDirsAxisHost(dirs.data(), order.data() + b, 16, 1.0, axis, bd.sgn);
                 const float lbb = farLowerBound(m, pyr[axis], bb, pf, 3.0e38f);
+                const float lbc[3] = {farBoxBound(m, pyr[axis], bb, cc, blo[0], bhi[0], 1),
+                                      farBoxBound(m, pyr[axis], bb, cc, blo[1], bhi[1], 1),
+                                      farBoxBound(m, pyr[axis], bb, cc, blo[1], bhi[1], 4)};
                 ++blockTested;
                 if (lbb > farSkip(24.f)) ++blockSkipped;
+                if (lbc[2] > farSkip(24.f)) ++cellSkipped;
                 for (size_t k = b; k < b + 16; ++k) {
                     const uint32_t i = order[k];
                     const f3 df = mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
                     uint32_t s0 = 0;
                     const float tau = opticalDepthFast<3>(m, m.walkSig, pf, df, 3.0e38f, &s0);
-                    if (!(tau >= lbb)) {
-                        if (bad < 5) printf("  set %d point %d block %zu dir %u: bound %.9g above the walk %.9g\n", t, q, b, i, lbb, tau);
+                    float lmax = lbb;
+                    for (float x : lbc) lmax = x > lmax ? x : lmax;
+                    if (!(tau >= lmax)) {
+                        if (bad < 5) printf("  set %d point %d block %zu dir %u: bounds %.9g / %.9g %.9g %.9g above the walk %.9g\n", t, q, b, i, lbb, lbc[0], lbc[1], lbc[2], tau);
                         ++bad;
                     }
                     for (float cut : cuts)
-                        if (lbb > farSkip(cut)) {
+                        if (lmax > farSkip(cut)) {
                             uint32_t s1 = 0;
                             if (!(opticalDepthFast<3>(m, m.walkSig, pf, df, cut, &s1) > cut)) ++bad;
                         }
@@ -284,8 +301,9 @@ static int runBound(int n, double h, int trials, unsigned seed, int fast, int nm
         }
     }
     printf("bound: n %d h %g fastest axis %d, %d materials: %ld direction sets, %ld points, %ld violations, "
-           "skippable at cut 24: %.3f (blocks of 16: %.3f), mean bound / least walk %.3f\n", n, h, fast, nmat, valid,
-           tested, bad, tested ? (double)skipped / tested : 0.0, blockTested ? (double)blockSkipped / blockTested : 0.0,
+           "skippable at cut 24: %.3f (blocks of 16: %.3f, by half box %.3f), mean bound / least walk %.3f\n", n, h, fast,
+           nmat, valid, tested, bad, tested ? (double)skipped / tested : 0.0,
+           blockTested ? (double)blockSkipped / blockTested : 0.0, blockTested ? (double)cellSkipped / blockTested : 0.0,
            tested ? tight / tested : 0.0);
     return bad != 0 || g_bad_index != 0 || tested == 0;
 }
