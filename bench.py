@@ -274,6 +274,8 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 6),
             "traffic": prof.get("view_hbm_bytes") if same_form else None, "avg_launch_ms": round(view_ms, 3),
             "algorithmic_bytes_per_launch": alg_bytes, "voxel_steps_per_s": steps_s,
+            "voxel_steps_per_launch": tm["dda_steps_view"] / nb,
+            "first_vertex_voxel_steps_per_batch": (tm["dda_steps"] - tm["dda_steps_view"]) / nb,
             "limiter": "valu issue (the walk's grid reads hit L2; see valu)"}
     # the walk is bound by VALU issue, not HBM: VALU lane-instructions per voxel step (SQ_INSTS_VALU
     # over the same kernel, inactive lanes included) against the unpacked VALU issue peak

@@ -182,21 +182,88 @@ __device__ inline size_t planeIndex(uint32_t rows, uint32_t c, uint32_t p) {
     return ((size_t)(p / kChunk) * rows + c) * kChunk + (p % kChunk);
 }
 
+// Paths whose row of X (P rows of C complex values) holds a nonzero value: flag[p] = 1, else 0.
+// One wavefront a path.
+__global__ void __launch_bounds__(256) k_row_nonzero(const float2* __restrict__ X, uint32_t P, uint32_t C,
+                                                     uint32_t* __restrict__ flag) {
+    const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (p >= P) return;
+    const float2* row = X + (size_t)p * C;
+    bool any = false;
+    for (uint32_t c = lane; c < C; c += 64) {
+        const float2 x = row[c];
+        any = any || x.x != 0.f || x.y != 0.f;
+    }
+    const uint64_t b = __ballot(any);
+    if (lane == 0) flag[p] = b ? 1u : 0u;
+}
+
+// The flagged paths in increasing order (list[0 .. *count)), by one workgroup of 1024 threads:
+// each thread counts a contiguous range of flags, an exclusive scan of the counts gives its first
+// slot, and it writes its paths there. The list and the count depend on the flags only.
+__global__ void __launch_bounds__(1024) k_compact_paths(const uint32_t* __restrict__ flag, uint32_t P,
+                                                        uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+    __shared__ uint32_t waveTot[16];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t per = (P + 1023) / 1024, b = min(P, t * per), e = min(P, b + per);
+    uint32_t c = 0;
+    for (uint32_t p = b; p < e; ++p) c += flag[p] ? 1u : 0u;
+    // inclusive scan over the wavefront, then over the 16 wavefronts' totals
+    uint32_t x = c;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) waveTot[w] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t j = 0; j < w; ++j) before += waveTot[j];
+    uint32_t slot = before + x - c;
+    for (uint32_t p = b; p < e; ++p)
+        if (flag[p]) list[slot++] = p;
+    if (t == 1023) *count = slot;
+}
+
 // One workgroup splits 64 paths x 64 rows: reads along the rows (coalesced float2), transposes
-// through LDS, writes 2 chunks x 64 rows x 128 bytes per plane.
+// through LDS, writes 2 chunks x 64 rows x 128 bytes per plane (with a path list: the listed paths
+// in list order, zeros past the count). It also records, for each of its
+// two (chunk, 64-row unit) blocks, whether any value in it is nonzero (nz[chunk * rows / 64 +
+// unit]), and writes the planes of nonzero blocks only: the GEMM reads a block's planes only when
+// its flag is set, so a zero block's planes may hold anything.
 __global__ void __launch_bounds__(256) k_split_bf16(const float2* __restrict__ X, uint32_t P, uint32_t C,
                                                     uint32_t rows, uint32_t* __restrict__ hi,
-                                                    uint32_t* __restrict__ lo) {
+                                                    uint32_t* __restrict__ lo, uint32_t* __restrict__ nz,
+                                                    const uint32_t* __restrict__ list,
+                                                    const uint32_t* __restrict__ count) {
     __shared__ uint32_t th[64][65], tl[64][65];
     const uint32_t p0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
     const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // with a path list (k_compact_paths), plane path j holds path list[j] for j < *count
+    const uint32_t n = list ? *count : P;
     // all 16 loads of a thread are issued before the first conversion
     float2 v[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const uint32_t p = p0 + ty + 4 * k, c = c0 + tx;
-        v[k] = (p < P && c < C) ? X[(size_t)p * C + c] : make_float2(0.f, 0.f);
+        const uint32_t src = (list && p < n) ? list[p] : p;
+        v[k] = (p < n && c < C) ? X[(size_t)src * C + c] : make_float2(0.f, 0.f);
     }
+    // paths p0 + ty + 4 k: k < 8 in the first chunk, the rest in the second
+    bool n0 = false, n1 = false;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const bool n = v[k].x != 0.f || v[k].y != 0.f;
+        if (k < 8) n0 = n0 || n;
+        else n1 = n1 || n;
+    }
+    const bool any0 = __syncthreads_or(n0) != 0, any1 = __syncthreads_or(n1) != 0;
+    if (threadIdx.x == 0) {
+        const uint32_t units = rows / 64, chunk = p0 / kChunk;
+        nz[(size_t)chunk * units + c0 / 64] = any0 ? 1u : 0u;
+        nz[(size_t)(chunk + 1) * units + c0 / 64] = any1 ? 1u : 0u;
+    }
+    if (!any0 && !any1) return;   // the whole workgroup: nothing to write
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         const uint32_t hr = bf16bits(v[k].x), hm = bf16bits(v[k].y);
@@ -205,6 +272,7 @@ __global__ void __launch_bounds__(256) k_split_bf16(const float2* __restrict__ X
     }
     __syncthreads();
     // every (row, path) of the padded planes belongs to exactly one workgroup: no bounds checks
+    if (tx < 32 ? !any0 : !any1) return;
 #pragma unroll
     for (uint32_t j = ty; j < 64; j += 4) {
         const size_t o = planeIndex(rows, c0 + j, p0 + tx);
@@ -219,14 +287,24 @@ __global__ void __launch_bounds__(256, 2) k_contract_bf16x3_planes(const uint32_
                                                                 const uint32_t* __restrict__ Bl, uint32_t rowsA,
                                                                 uint32_t rowsB,
                                                                 uint32_t P, uint32_t L, uint32_t V,
-                                                                uint32_t pathsPerSplit, float2* __restrict__ part) {
+                                                                uint32_t pathsPerSplit, const uint32_t* __restrict__ nzA,
+                                                                const uint32_t* __restrict__ nzB,
+                                                                const uint32_t* __restrict__ count,
+                                                                float2* __restrict__ part) {
     __shared__ uint32_t sAh[kTileL * kRowWords], sAl[kTileL * kRowWords];
     __shared__ uint32_t sBh[kTileV * kRowWords], sBl[kTileV * kRowWords];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const TileIdx tile = xcdTile();
     const uint32_t l0 = tile.x * kTileL, v0 = tile.y * kTileV, split = tile.z;
-    const uint32_t pBegin = split * pathsPerSplit;
-    const uint32_t pEnd = min(P, pBegin + pathsPerSplit);
+    // with a device count (the planes hold the listed paths first), the count's paths split evenly
+    // over the launch's slices
+    uint32_t pps = pathsPerSplit;
+    if (count) {
+        P = *count;
+        pps = ((P + gridDim.z - 1) / gridDim.z + kChunk - 1) / kChunk * kChunk;
+    }
+    const uint32_t pBegin = min(P, split * pps);
+    const uint32_t pEnd = min(P, pBegin + pps);
     const uint32_t nChunks = (pEnd - pBegin + kChunk - 1) / kChunk;   // the planes hold zeros past P
     const uint32_t wl = (wave & 1) * 64, wv = (wave >> 1) * 64;
     f32x16 accRe[2][2], accIm[2][2];
@@ -305,7 +383,15 @@ __global__ void __launch_bounds__(256, 2) k_contract_bf16x3_planes(const uint32_
             }
         }
     };
+    // a chunk whose A rows or B rows of this tile are all zero (k_split_bf16's flags) adds exact
+    // zeros: skipped, with the same result
+    const uint32_t unitsA = rowsA / 64, unitsB = rowsB / 64, ua = l0 / 64, ub = v0 / 64;
+    static_assert(kTileL == 128 && kTileV == 128, "a tile spans two 64-row flag units");
     for (uint32_t c = 0; c < nChunks; ++c) {
+        const uint32_t cg = pBegin / kChunk + c;
+        const uint32_t* fa = nzA + (size_t)cg * unitsA + ua;
+        const uint32_t* fb = nzB + (size_t)cg * unitsB + ub;
+        if (!((fa[0] | fa[1]) && (fb[0] | fb[1]))) continue;
         stage(pBegin + c * kChunk);
         __syncthreads();
         // 4 k-steps (k' = 16) per chunk of 32 paths
@@ -452,7 +538,11 @@ static SplitPlan planSplits(int kind, uint32_t P, uint32_t L, uint32_t V, uint32
         plan.rowsA = (L + kTileL - 1) / kTileL * kTileL;
         plan.rowsB = (V + kTileV - 1) / kTileV * kTileV;
         plan.planeOffset = (plan.bytes + 255) / 256 * 256;
-        plan.planeBytes = 2ull * ((size_t)plan.rowsA + plan.rowsB) * plan.ldp * sizeof(uint32_t);
+        // hi and lo planes of both operands, the nonzero flags of their (chunk, 64-row) blocks, then
+        // the path flags, the path list and its count
+        plan.planeBytes = 2ull * ((size_t)plan.rowsA + plan.rowsB) * plan.ldp * sizeof(uint32_t) +
+                          (size_t)(plan.ldp / kChunk) * ((plan.rowsA + plan.rowsB) / 64) * sizeof(uint32_t) +
+                          (2ull * plan.ldp + 64) * sizeof(uint32_t);
     }
     return plan;
 }
@@ -495,12 +585,22 @@ static hipError_t launchContractInto(int kind, const void* A, const void* B, uin
             uint32_t* al = ah + (size_t)plan.rowsA * plan.ldp;
             uint32_t* bh = al + (size_t)plan.rowsA * plan.ldp;
             uint32_t* bl = bh + (size_t)plan.rowsB * plan.ldp;
+            uint32_t* nza = bl + (size_t)plan.rowsB * plan.ldp;
+            uint32_t* nzb = nza + (size_t)(plan.ldp / kChunk) * (plan.rowsA / 64);
+            uint32_t* flag = nzb + (size_t)(plan.ldp / kChunk) * (plan.rowsB / 64);
+            uint32_t* list = flag + plan.ldp;
+            uint32_t* count = list + plan.ldp;
+            // the paths whose B row is all zero add nothing: only the others are split and
+            // multiplied, in path order (the view-side terms of most paths are cut to zero in thick
+            // media)
+            hipLaunchKernelGGL(k_row_nonzero, dim3((P + 3) / 4), dim3(256), 0, st, b, P, V, flag);
+            hipLaunchKernelGGL(k_compact_paths, dim3(1), dim3(1024), 0, st, flag, P, list, count);
             hipLaunchKernelGGL(k_split_bf16, dim3(plan.ldp / 64, plan.rowsA / 64), dim3(256), 0, st, a, P, L,
-                               plan.rowsA, ah, al);
+                               plan.rowsA, ah, al, nza, list, count);
             hipLaunchKernelGGL(k_split_bf16, dim3(plan.ldp / 64, plan.rowsB / 64), dim3(256), 0, st, b, P, V,
-                               plan.rowsB, bh, bl);
+                               plan.rowsB, bh, bl, nzb, list, count);
             hipLaunchKernelGGL(k_contract_bf16x3_planes, grid, dim3(256), 0, st, ah, al, bh, bl, plan.rowsA,
-                               plan.rowsB, P, L, V, pps, (float2*)partials);
+                               plan.rowsB, P, L, V, pps, nza, nzb, count, (float2*)partials);
             hipLaunchKernelGGL(k_reduce<float2>, dim3((LV + 255) / 256), dim3(256), 0, st, (const float2*)partials,
                                splits, LV, F, Fd, fsq, fscalar, U + 2 * (size_t)t * LV);
         } else if (kind == kContractBf16x3) {
