@@ -64,7 +64,10 @@ struct Rng {
             ++c3;
             pos = 0;
         }
-        return buf[pos++];
+        // selects rather than buf[pos]: a dynamic index would put buf in memory (scratch)
+        const uint32_t w = pos == 0 ? buf[0] : (pos == 1 ? buf[1] : (pos == 2 ? buf[2] : buf[3]));
+        ++pos;
+        return w;
     }
     // ((w >> 8) + 0.5) * 2^-24: exactly representable in FP32, strictly inside (0, 1).
     __host__ __device__ float uniform() { return ((float)(word() >> 8) + 0.5f) * (1.0f / 16777216.0f); }

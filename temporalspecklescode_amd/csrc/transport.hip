@@ -88,6 +88,43 @@ __device__ inline void frameDeltas(const FramesView& fr, uint64_t seed, uint64_t
 }
 
 constexpr uint32_t kNoSlot = 0xffffffffu;
+
+// LDS copies of what a flight step reads besides the grid itself: the cell edges of each axis (when
+// every axis has at most kLdsEdges - 1 cells) and the per-material acceptance and albedo. The
+// lanes that trace a batch's longest paths run nearly alone at its end, so each flight step's
+// memory round trips set the transport's duration: the edges (locate's correction) are one.
+constexpr uint32_t kLdsEdges = 1025;
+struct FlightLds {
+    float e[3][kLdsEdges];
+    float accept[kMaxMaterials], albedo[kMaxMaterials];
+};
+// locate<float> (same cells) with the edges read from LDS.
+__device__ inline int locateLds(const MediumView& m, const FlightLds& f, int a, float x) {
+    const float* e = f.e[a];
+    const int n = (int)m.n[a];
+    if (m.uniformAxis[a]) {
+        const float g = (x - m.e0[a]) * m.invH[a];
+        int b = g <= 0.f ? 0 : (g >= m.nf[a] ? n - 1 : (int)g);
+        if (b > 0 && x < e[b]) --b;
+        if (b < n - 1 && x >= e[b + 1]) ++b;
+        return b;
+    }
+    if (x < e[0]) return 0;
+    if (x >= e[n]) return n - 1;
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (e[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+template <int DIMS>
+__device__ inline uint32_t materialAtLds(const MediumView& m, const FlightLds& f, bool lds, const f3& x) {
+    if (!lds) return materialAt<DIMS>(m, x);
+    const int bx = locateLds(m, f, 0, x.x), by = locateLds(m, f, 1, x.y);
+    const int bz = DIMS == 3 ? locateLds(m, f, 2, x.z) : 0;
+    return m.grid[cellIndex(m, bx, by, bz)];
+}
 // A wavefront keeps stepping flights while at least this many lanes fly; below it, the lanes that
 // wait on an event (a collision to store, a miss, a finished path) are served together.
 constexpr uint32_t kFlyQuorum = 16;
@@ -119,6 +156,15 @@ __global__ void __launch_bounds__(64) k_transport(MediumView m, ScatterView sc, 
     f3 x = mk(0.f, 0.f, 0.f), dir = x, o = x, d = x, xx = x;
     float dist = 0.f;
     Rng rng(seed, 0, 0, kPurposeTransport);
+    __shared__ FlightLds fl;
+    const bool edgesLds = m.n[0] < kLdsEdges && m.n[1] < kLdsEdges && (DIMS == 2 || m.n[2] < kLdsEdges);
+    for (uint32_t a = 0; a < (uint32_t)DIMS && edgesLds; ++a)
+        for (uint32_t i = threadIdx.x; i <= m.n[a]; i += blockDim.x) fl.e[a][i] = m.ef[a][i];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kMaxMaterials; i += blockDim.x) {
+        fl.accept[i] = m.accept[i];
+        fl.albedo[i] = m.albedo[i];
+    }
+    __syncthreads();
 
     for (;;) {
         const uint64_t live = __ballot(!dead);
@@ -132,8 +178,8 @@ __global__ void __launch_bounds__(64) k_transport(MediumView m, ScatterView sc, 
                 if (!inside<DIMS, float>(m, xx)) {
                     if (!pointsToward<DIMS>(m, xx, d)) { flying = false; outside = true; }
                 } else {
-                    fmat = materialAt<DIMS>(m, xx);
-                    if (rng.uniform() <= m.accept[fmat]) { flying = false; outside = false; }
+                    fmat = materialAtLds<DIMS>(m, fl, edgesLds, xx);
+                    if (rng.uniform() <= fl.accept[fmat]) { flying = false; outside = false; }
                 }
             }
             continue;
@@ -143,7 +189,7 @@ __global__ void __launch_bounds__(64) k_transport(MediumView m, ScatterView sc, 
         bool store = false, startSegment = false, startAttempt = false, needPath = slot == kNoSlot;
         if (!needPath) {
             // the flight's end: out of the medium, absorbed, or a scattering vertex
-            const uint32_t nm = outside ? 0u : (rng.uniform() > m.albedo[fmat] ? 0u : fmat);
+            const uint32_t nm = outside ? 0u : (rng.uniform() > fl.albedo[fmat] ? 0u : fmat);
             if (nm > 0) {
                 x = xx;
                 dir = d;
