@@ -366,6 +366,9 @@ struct VertexBatch {
     // kernel's sources (FarBound::perm) may hold a term inside the launch's cut, bit 31 standing for
     // every block from 31 on; all set when no bound applies (k_vertex_prep, k_far_live)
     uint32_t* live = nullptr;
+    // [P] with NeeParams::viewFirst: 1 when the path's B row holds a nonzero term (cleared by
+    // k_vertex_prep, set by k_nee_view_far)
+    uint32_t* rowNz = nullptr;
 };
 
 // Staging of the single-pass transport. Lanes draw path slots from a queue and store each path's

@@ -63,6 +63,10 @@ struct NeeParams {
     // the far-field view kernel's view order (FarBound::perm[0]) when its records and live-block
     // masks exist (k_nee takes its single-scattering view walks in blocks of it), else null
     const uint32_t* farPerm = nullptr;
+    // 1: the far-field view kernel runs before k_nee and flags the paths whose B row is nonzero
+    // (VertexBatch::rowNz), and k_nee forms b first and leaves a and A unwritten for the paths whose
+    // B and b rows are both zero (the compacted bf16x3 contraction reads no other rows)
+    uint32_t viewFirst = 0;
 };
 
 // The single-pass transport of npaths path slots (zeroes ts.q, then k_transport): per-path

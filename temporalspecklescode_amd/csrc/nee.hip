@@ -326,10 +326,78 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
     // and its terms are O(1). They are walked exactly. The cut of precision.tauCut applies to the
     // unweighted terms only: b (and the view kernels' B, Br).
     const R tauExact = sizeof(R) == 4 ? (R)prm.exactCutF : tauInf;
+    constexpr bool gauss = GB && TAB;   // GaussianBeam illumination and view
+
+    if (!multi) {
+        for (uint32_t i = sub; i < V * NT; i += lpp) {
+            const uint32_t v = i % V, t = i / V;
+            B[((size_t)t * P + p) * V + v] = zero;
+        }
+        if (Br)
+            for (uint32_t i = sub; i < L * NT; i += lpp) Br[((size_t)(i / L) * P + p) * L + i % L] = zero;
+    }
+    const bool isoPointVw = vw.type == 1 && sc.allIsotropic;
+    bool anyB = false;   // a nonzero b among this lane's views
+    // (w2v: the path weight of the reversed path's term Ar, formed with the probabilities)
+    auto viewTerms = [&](R w2v) {
+
+    // ---- single-scattering view terms b = g_v(x1^t)
+    // Far-field views under the view cut with the far-field view kernel's records: the views in
+    // its order, and a view in a block whose every walk from x1 lies past the cut (x1's live-block
+    // mask) takes g = 0 without walking, as its walk would give.
+    const bool blockSkip = sizeof(R) == 4 && !gauss && !corr && !Ar && vw.type == 2 && prm.farPerm && vb.live;
+    for (uint32_t i = sub; i < V * NT; i += lpp) {
+        const uint32_t pos = i % V, t = i / V;
+        const uint32_t v = blockSkip ? prm.farPerm[pos] : pos;
+        R ga[2];
+        double gp[2];
+        const int nf = corr ? 2 : 1;
+        for (int f = 0; f < nf; ++f) {
+            uint32_t tt = (corr && f == 0) ? 0 : t;
+            Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
+            const uint32_t blk = pos / 64u;
+            if (blockSkip && !((vb.live[2 * (off * NT + t)] >> (blk < 31u ? blk : 31u)) & 1u)) {
+                const double phase = ddot<DIMS>(sourceD<DIMS>(vw, v), x1t.pd) * invLambda;   // gFar's phase
+                ga[f] = (R)0;
+                gp[f] = vw.illum != 0 ? phase : -phase;
+            } else {
+                gTerm<DIMS, R, gauss>(m, sig, vw, v, x1t, invLambda, Ar ? tauExact : tauCutFF, &steps, &ga[f],
+                                      &gp[f]);
+            }
+        }
+        const size_t o = ((size_t)t * P + p) * V + v;
+        b[o] = corr ? cis<R>(ga[0] * ga[1], gp[0] - gp[1]) : cis<R>(ga[0], gp[0]);
+        anyB = anyB || ga[0] != (R)0;
+        if (Ar) {   // reversed path: the view connected at x1 with f toward x2 (field output only)
+            R fv = 0;
+            double fvp = 0;
+            if (multi) {
+                const Geo x1t = vertexFrame(vb, fr, off, t, x1m);
+                const Geo x2t = vertexFrame(vb, fr, off + 1, t, x2m);
+                const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
+                const d3 pb = fr.temporalAmp ? x2t.pd : x2cd;
+                fv = isoPointVw ? isoAmp<DIMS, R>()
+                                : phaseAmp<DIMS, R, TAB>(sc, m1, ddot<DIMS>(incoming<DIMS>(vw, v, pa), dnormalize<DIMS>(dsub(pb, pa))),
+                                                    &fvp);
+            }
+            Ar[o] = multi ? cis<R>(ga[0] * fv * w2v, gp[0] + fvp) : zero;
+        }
+    }
+    };
+    // with viewFirst the single-scattering view terms come first: a path whose b and B rows are
+    // zero adds nothing to either contraction, which reads only the rows of paths with a nonzero
+    // view-side row, so its probabilities and illumination terms are not formed
+    if (prm.viewFirst) {
+        viewTerms((R)0);   // no Ar with viewFirst
+        const bool keep = groupSum<R>(anyB ? (R)1 : (R)0, lpp) > (R)0 || (vb.rowNz && vb.rowNz[p]);
+        if (!keep) {
+            *stepsOut += steps;
+            return;
+        }
+    }
 
     // ---- path probabilities from the central first vertex (pathProbabilitySingle/Multiple)
     R s1 = 0, s2 = 0;
-    constexpr bool gauss = GB && TAB;   // GaussianBeam illumination and view
     const uint32_t m1g = gauss ? gbMaterial<DIMS>(m, x1c.pd) : 0u;
     for (uint32_t l = sub; l < L; l += lpp) {
         R pr;
@@ -426,57 +494,7 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         }
     }
 
-    if (!multi) {
-        for (uint32_t i = sub; i < V * NT; i += lpp) {
-            const uint32_t v = i % V, t = i / V;
-            B[((size_t)t * P + p) * V + v] = zero;
-        }
-        if (Br)
-            for (uint32_t i = sub; i < L * NT; i += lpp) Br[((size_t)(i / L) * P + p) * L + i % L] = zero;
-    }
-    const bool isoPointVw = vw.type == 1 && sc.allIsotropic;
-
-    // ---- single-scattering view terms b = g_v(x1^t)
-    // Far-field views under the view cut with the far-field view kernel's records: the views in
-    // its order, and a view in a block whose every walk from x1 lies past the cut (x1's live-block
-    // mask) takes g = 0 without walking, as its walk would give.
-    const bool blockSkip = sizeof(R) == 4 && !gauss && !corr && !Ar && vw.type == 2 && prm.farPerm && vb.live;
-    for (uint32_t i = sub; i < V * NT; i += lpp) {
-        const uint32_t pos = i % V, t = i / V;
-        const uint32_t v = blockSkip ? prm.farPerm[pos] : pos;
-        R ga[2];
-        double gp[2];
-        const int nf = corr ? 2 : 1;
-        for (int f = 0; f < nf; ++f) {
-            uint32_t tt = (corr && f == 0) ? 0 : t;
-            Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
-            const uint32_t blk = pos / 64u;
-            if (blockSkip && !((vb.live[2 * (off * NT + t)] >> (blk < 31u ? blk : 31u)) & 1u)) {
-                const double phase = ddot<DIMS>(sourceD<DIMS>(vw, v), x1t.pd) * invLambda;   // gFar's phase
-                ga[f] = (R)0;
-                gp[f] = vw.illum != 0 ? phase : -phase;
-            } else {
-                gTerm<DIMS, R, gauss>(m, sig, vw, v, x1t, invLambda, Ar ? tauExact : tauCutFF, &steps, &ga[f],
-                                      &gp[f]);
-            }
-        }
-        const size_t o = ((size_t)t * P + p) * V + v;
-        b[o] = corr ? cis<R>(ga[0] * ga[1], gp[0] - gp[1]) : cis<R>(ga[0], gp[0]);
-        if (Ar) {   // reversed path: the view connected at x1 with f toward x2 (field output only)
-            R fv = 0;
-            double fvp = 0;
-            if (multi) {
-                const Geo x1t = vertexFrame(vb, fr, off, t, x1m);
-                const Geo x2t = vertexFrame(vb, fr, off + 1, t, x2m);
-                const d3 pa = fr.temporalAmp ? x1t.pd : x1c.pd;
-                const d3 pb = fr.temporalAmp ? x2t.pd : x2cd;
-                fv = isoPointVw ? isoAmp<DIMS, R>()
-                                : phaseAmp<DIMS, R, TAB>(sc, m1, ddot<DIMS>(incoming<DIMS>(vw, v, pa), dnormalize<DIMS>(dsub(pb, pa))),
-                                                    &fvp);
-            }
-            Ar[o] = multi ? cis<R>(ga[0] * fv * w2, gp[0] + fvp) : zero;
-        }
-    }
+    if (!prm.viewFirst) viewTerms(w2);
     *stepsOut += steps;
 }
 
@@ -754,6 +772,7 @@ __global__ void __launch_bounds__(256) k_vertex_prep(MediumView m, VertexBatch v
     // every exit below is uniform over a path's lanes, which the scan's shuffles stay within
     if (p >= npaths) return;
     if ((uint64_t)vb.off[npaths] > vb.cap) return;   // pool overflow: the batch is re-run
+    if (lane == 0 && vb.rowNz) vb.rowNz[p] = 0u;
     const uint32_t K = vb.k[p], NT = fr.nt;
     if (K == 0) return;
     const uint64_t off = vb.off[p];
@@ -1090,6 +1109,7 @@ k_nee_view_far(MediumView m, ScatterView sc, SourceView vw, FramesView fr, Verte
             }
         }
         if (act) B[((size_t)t * P + p) * V + v] = make_float2(accRe, accIm);
+        if (role == 0 && vb.rowNz && __ballot(accRe != 0.f || accIm != 0.f) && u.sub == 0) vb.rowNz[p] = 1u;
     }
     uint32_t tot = waveSum(steps);
     if ((threadIdx.x & 63) == 0 && out.steps) slotAdd(out.steps + kCountSlots * kCountStride, tot);
@@ -1611,16 +1631,30 @@ static void launchNeeT(const MediumView& m, const ScatterView& sc, const SourceV
         }
         return;
     }
-    if (il.type == 3 && sc.tab)
-        hipLaunchKernelGGL((k_nee<DIMS, R, true, true>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
-    else if (sc.tab) hipLaunchKernelGGL((k_nee<DIMS, R, true, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
-    else hipLaunchKernelGGL((k_nee<DIMS, R, false, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
-    if (between) hipEventRecord(between, st);
+    auto first = [&]() {
+        if (il.type == 3 && sc.tab)
+            hipLaunchKernelGGL((k_nee<DIMS, R, true, true>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+        else if (sc.tab)
+            hipLaunchKernelGGL((k_nee<DIMS, R, true, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+        else hipLaunchKernelGGL((k_nee<DIMS, R, false, false>), grid, block, 0, st, m, sc, il, vw, fr, vb, p, out);
+    };
     const bool corr = p.output == 1;
-    if (vw.type == 3) {
-        launchView<DIMS, R, kSrcGauss>(corr, m, sc, vw, fr, vb, p, out, st);
-    } else if (vw.type == 2) launchView<DIMS, R, kSrcFarView>(corr, m, sc, vw, fr, vb, p, out, st, fb);
-    else launchView<DIMS, R, kSrcPoint>(corr, m, sc, vw, fr, vb, p, out, st);
+    auto views = [&]() {
+        if (vw.type == 3) launchView<DIMS, R, kSrcGauss>(corr, m, sc, vw, fr, vb, p, out, st);
+        else if (vw.type == 2) launchView<DIMS, R, kSrcFarView>(corr, m, sc, vw, fr, vb, p, out, st, fb);
+        else launchView<DIMS, R, kSrcPoint>(corr, m, sc, vw, fr, vb, p, out, st);
+    };
+    // with viewFirst the view terms (which flag the paths with a nonzero B row) precede k_nee;
+    // `between` follows the first of the two
+    if (p.viewFirst) {
+        views();
+        if (between) hipEventRecord(between, st);
+        first();
+    } else {
+        first();
+        if (between) hipEventRecord(between, st);
+        views();
+    }
     if (out.Br) {
         // coherent backscattering: the same per-vertex sums with the illumination connected at
         // x_k (the reversed path), into Br

@@ -307,6 +307,7 @@ private:
     // term slabs and cut reference, so that batches can be in different stages at once.
     struct BatchSet {
         DevBuf<uint32_t> k, misses, off, blockSums;
+        DevBuf<uint32_t> rowNz;               // VertexBatch::rowNz (viewFirst)
         DevBuf<float4> xm;
         DevBuf<float> rho, delta;
         DevBuf<VertexRec> rec;
@@ -355,6 +356,11 @@ private:
     DevBuf<uint32_t> farPerm_[2];   // the kernel's source order per role (FarBound::perm)
     DevBuf<float> farTable_[2];     // FarBound::table per role
     bool farBoundOn_ = false;
+    // the far-field view kernel runs before k_nee, which then skips the illumination terms of
+    // paths whose view-side rows are zero (NeeParams::viewFirst): FP32 far-field views without
+    // coherent backscattering, field output, bf16x3 contraction (the compacted one), separable
+    // single scattering
+    bool viewFirst_ = false;
     int setupFarBound();
     // the contraction stage captured as HIP graphs, per (set, batch size, destination): the
     // multiple-scattering part and the single-scattering part (timed apart); dropped whenever a

@@ -828,6 +828,8 @@ NextEventEstimation::NextEventEstimation(int* err, const Medium& medium, const S
                ((vw_.type() == SSTMC_SOURCE_FARFIELD && V_ > 32) ||
                 (opt.cbs && il_.type() == SSTMC_SOURCE_FARFIELD && L_ > 32));
     if ((*err = setupFarBound())) return;
+    viewFirst_ = farRecs_ && vw_.type() == SSTMC_SOURCE_FARFIELD && V_ > 32 && !opt.cbs &&
+                 contractKind_ == kContractBf16x3 && singleMode_ == kSingleSeparable;
     nsets_ = pipelineSets(setBytes(P_, L_, V_, T_, fp64_, !sampler_.view().trivial, opt.cbs != 0, acorr));
     if ((*err = allocate())) return;
     if (needF) {
@@ -879,7 +881,8 @@ int NextEventEstimation::allocate() {
     size_t na = (size_t)T_ * P_ * L_ * elem, nb = (size_t)T_ * P_ * V_ * elem;
     for (int s = 0; s < nsets_; ++s) {
         BatchSet& bs = sets_[s];
-        if ((e = bs.k.alloc(P_)) || (e = bs.misses.alloc(P_)) || (e = bs.off.alloc(P_ + 1)) ||
+        if ((viewFirst_ && (e = bs.rowNz.alloc(P_))) || (e = bs.k.alloc(P_)) || (e = bs.misses.alloc(P_)) ||
+            (e = bs.off.alloc(P_ + 1)) ||
             (e = bs.blockSums.alloc((P_ + 1023) / 1024 + 1)) || (e = bs.tauRef.alloc(1)))
             return e;
         if ((e = bs.A.alloc(na)) || (e = bs.a.alloc(na)) || (e = bs.B.alloc(nb)) || (e = bs.b.alloc(nb))) return e;
@@ -903,8 +906,10 @@ int NextEventEstimation::allocate() {
 
 VertexBatch NextEventEstimation::vertexBatch(int set) const {
     const BatchSet& bs = sets_[set];
-    return VertexBatch{bs.k.p, bs.misses.p, bs.off.p, bs.xm.p, bs.rho.p, bs.delta.p, cap_, status_.p, bs.rec.p,
-                       farRecs_ ? bs.far.p : nullptr, farRecs_ ? bs.live.p : nullptr};
+    VertexBatch vb{bs.k.p, bs.misses.p, bs.off.p, bs.xm.p, bs.rho.p, bs.delta.p, cap_, status_.p, bs.rec.p,
+                   farRecs_ ? bs.far.p : nullptr, farRecs_ ? bs.live.p : nullptr};
+    vb.rowNz = viewFirst_ ? bs.rowNz.p : nullptr;
+    return vb;
 }
 
 // Every set's vertex pool holds `cap` vertices; the streams must be idle.
@@ -1015,6 +1020,7 @@ int NextEventEstimation::runBatch(uint64_t pathBegin, uint32_t npaths, double* U
     prm.tauRef = opt_.tauCut > 0 ? bs.tauRef.p : nullptr;
     prm.tauCutRel = (float)opt_.tauCut;
     prm.npaths = npaths;
+    prm.viewFirst = viewFirst_ && pipelined ? 1u : 0u;   // debug batches form every term
     prm.sameViews = sameViews_ ? 1u : 0u;
     prm.sameIllums = sameIllums_ ? 1u : 0u;
     NeeOut out{bs.A.p, bs.B.p, bs.a.p, bs.b.p, bs.Ar.p, bs.Br.p, stepSlots_.p};
@@ -1530,7 +1536,9 @@ int NextEventEstimation::finish(double* u, bool deviceDst) {
         hipEvent_t* e = &ev_[(size_t)b * kEventsPerBatch];
         if (hipEventElapsedTime(&ms, e[0], e[1]) == hipSuccess) t.transport_ms += ms;
         if (hipEventElapsedTime(&ms, e[6], e[2]) == hipSuccess) t.nee_ms += ms;
-        if (hipEventElapsedTime(&ms, e[5], e[2]) == hipSuccess) t.nee_view_ms += ms;
+        // the multiple-scattering view kernel: after k_nee, or before it with viewFirst
+        if (hipEventElapsedTime(&ms, viewFirst_ ? e[6] : e[5], viewFirst_ ? e[5] : e[2]) == hipSuccess)
+            t.nee_view_ms += ms;
         if (hipEventElapsedTime(&ms, e[7], e[3]) == hipSuccess) t.contract_ms += ms;
         if (hipEventElapsedTime(&ms, e[3], e[4]) == hipSuccess) t.single_ms += ms;
     }
