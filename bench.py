@@ -289,7 +289,8 @@ def main():
     con_tflops = flops / (con_ms * 1e-3) / 1e12
     # the multiple-scattering contraction as it runs in the pipeline (split, GEMM and reduce, timed
     # by events on the contraction stream while the next batch's NEE runs on another stream)
-    roof_con = {"kernel": "k_split_bf16 + k_contract_bf16x3_planes + k_reduce" if args.contraction == "bf16x3"
+    roof_con = {"kernel": "k_row_nonzero + k_compact_paths + k_split_bf16 + k_contract_bf16x3_planes + k_reduce"
+                if args.contraction == "bf16x3"
                 else "k_contract_fma",
                 "bound": "mfma", "achieved": round(con_tflops, 3), "peak": MFMA_BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(con_tflops / MFMA_BF16_PEAK_TFLOPS, 6), "traffic": None,
