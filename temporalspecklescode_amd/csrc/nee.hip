@@ -352,7 +352,9 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         R ga[2];
         double gp[2];
         const int nf = corr ? 2 : 1;
-        for (int f = 0; f < nf; ++f) {
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {   // constant bound: the two-entry arrays stay in registers
+            if (f >= nf) break;
             uint32_t tt = (corr && f == 0) ? 0 : t;
             Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
             const uint32_t blk = pos / 64u;
@@ -454,7 +456,9 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
         R ga[2], fa[2] = {0, 0};
         double gp[2], fp[2] = {0, 0};   // fp: phase of a tabular amplitude, in cycles
         const int nf = corr ? 2 : 1;
-        for (int f = 0; f < nf; ++f) {
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {   // constant bound: the two-entry arrays stay in registers
+            if (f >= nf) break;
             uint32_t tt = (corr && f == 0) ? 0 : t;
             Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
             if (tauCache) {
