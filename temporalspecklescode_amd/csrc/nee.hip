@@ -359,16 +359,15 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
             Geo x1t = vertexFrame(vb, fr, off, tt, x1m);
             const uint32_t blk = pos / 64u;
             if (blockSkip && !((vb.live[2 * (off * NT + t)] >> (blk < 31u ? blk : 31u)) & 1u)) {
-                const double phase = ddot<DIMS>(sourceD<DIMS>(vw, v), x1t.pd) * invLambda;   // gFar's phase
-                ga[f] = (R)0;
-                gp[f] = vw.illum != 0 ? phase : -phase;
+                ga[f] = (R)0;   // the term is zero whatever its phase (b below)
+                gp[f] = 0.0;
             } else {
                 gTerm<DIMS, R, gauss>(m, sig, vw, v, x1t, invLambda, Ar ? tauExact : tauCutFF, &steps, &ga[f],
                                       &gp[f]);
             }
         }
         const size_t o = ((size_t)t * P + p) * V + v;
-        b[o] = corr ? cis<R>(ga[0] * ga[1], gp[0] - gp[1]) : cis<R>(ga[0], gp[0]);
+        b[o] = corr ? cis<R>(ga[0] * ga[1], gp[0] - gp[1]) : (ga[0] == (R)0 ? zero : cis<R>(ga[0], gp[0]));
         anyB = anyB || ga[0] != (R)0;
         if (Ar) {   // reversed path: the view connected at x1 with f toward x2 (field output only)
             R fv = 0;
