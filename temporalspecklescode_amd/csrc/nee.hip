@@ -399,6 +399,9 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
 
     // ---- path probabilities from the central first vertex (pathProbabilitySingle/Multiple)
     R s1 = 0, s2 = 0;
+    // x1's far-field record (static frames: the record of frame 0 is the central x1), when the
+    // fast walk applies
+    const FarRec* x1Rec = (sizeof(R) == 4 && DIMS == 3 && vb.far && fr.trivial && m.walkFast) ? &vb.far[off * NT] : nullptr;
     const uint32_t m1g = gauss ? gbMaterial<DIMS>(m, x1c.pd) : 0u;
     for (uint32_t l = sub; l < L; l += lpp) {
         R pr;
@@ -422,7 +425,22 @@ __device__ inline void nee_path(const MediumView& m, const R* sig, const Scatter
             typename Pos<R>::V w;
             if (sizeof(R) == 4) w = Pos<R>::fromD(mkd(-(double)il.pf[3 * l], -(double)il.pf[3 * l + 1], -(double)il.pf[3 * l + 2]));
             else w = Pos<R>::fromD(mkd(-il.pd[3 * l], -il.pd[3 * l + 1], -il.pd[3 * l + 2]));
-            R tau = walk<DIMS, R>(m, sig, p1, w, sizeof(R) == 4 ? (R)prm.exactCutF : tauInf, &steps);
+            R tau;
+            if constexpr (sizeof(R) == 4 && DIMS == 3) {
+                if (x1Rec) {
+                    // the walk's start from x1's far-field record (walkStart of the same FP32
+                    // point), the direction part per illumination: the same walk, the same bits
+                    const WalkLane wl = walkLane<3>(m, w);
+                    tau = x1Rec->cell == kFarOutside
+                              ? 0.f
+                              : walkFromStartLane<3>(m, sig + kMaxMaterials, x1Rec->dlo, x1Rec->dhi, x1Rec->b,
+                                                     x1Rec->cell, wl, (R)prm.exactCutF, &steps);
+                } else {
+                    tau = walk<DIMS, R>(m, sig, p1, w, (R)prm.exactCutF, &steps);
+                }
+            } else {
+                tau = walk<DIMS, R>(m, sig, p1, w, sizeof(R) == 4 ? (R)prm.exactCutF : tauInf, &steps);
+            }
             if (tauCache) tauCache[l] = (float)tau;
             pr = expR<R>(-tau) * (sizeof(R) == 4 ? (R)il.probf[l] : (R)il.probd[l]);
         } else {
@@ -602,6 +620,7 @@ __device__ inline void nee_path_corr(const MediumView& m, const R* sig, const Sc
     const d3 seg12 = dnormalize<DIMS>(dsub(mkd(x2m.x, x2m.y, x2m.z), x1c.pd));
 
     R s1 = 0, s2 = 0;
+    const FarRec* x1Rec = nullptr;   // (the plain walk: x1 moves with the lag)
     for (uint32_t l = sub; l < 2 * L; l += lpp) {
         R pr;
         if (il.type == 2) {
@@ -609,7 +628,22 @@ __device__ inline void nee_path_corr(const MediumView& m, const R* sig, const Sc
             typename Pos<R>::V w;
             if (sizeof(R) == 4) w = Pos<R>::fromD(mkd(-(double)il.pf[3 * l], -(double)il.pf[3 * l + 1], -(double)il.pf[3 * l + 2]));
             else w = Pos<R>::fromD(mkd(-il.pd[3 * l], -il.pd[3 * l + 1], -il.pd[3 * l + 2]));
-            R tau = walk<DIMS, R>(m, sig, p1, w, sizeof(R) == 4 ? (R)prm.exactCutF : tauInf, &steps);
+            R tau;
+            if constexpr (sizeof(R) == 4 && DIMS == 3) {
+                if (x1Rec) {
+                    // the walk's start from x1's far-field record (walkStart of the same FP32
+                    // point), the direction part per illumination: the same walk, the same bits
+                    const WalkLane wl = walkLane<3>(m, w);
+                    tau = x1Rec->cell == kFarOutside
+                              ? 0.f
+                              : walkFromStartLane<3>(m, sig + kMaxMaterials, x1Rec->dlo, x1Rec->dhi, x1Rec->b,
+                                                     x1Rec->cell, wl, (R)prm.exactCutF, &steps);
+                } else {
+                    tau = walk<DIMS, R>(m, sig, p1, w, (R)prm.exactCutF, &steps);
+                }
+            } else {
+                tau = walk<DIMS, R>(m, sig, p1, w, sizeof(R) == 4 ? (R)prm.exactCutF : tauInf, &steps);
+            }
             pr = expR<R>(-tau) * (sizeof(R) == 4 ? (R)il.probf[l] : (R)il.probd[l]);
         } else {
             d3 sp = mkd(il.pd[3 * l], il.pd[3 * l + 1], DIMS == 3 ? il.pd[3 * l + 2] : 0.0);
