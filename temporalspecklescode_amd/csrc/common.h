@@ -316,7 +316,7 @@ struct __align__(32) FarRec {
 static_assert(sizeof(FarRec) == 96, "FarRec is three 32-byte lines");
 
 // Per-layer least sigt over aligned square blocks of cells, for lower bounds on optical depths
-// (farCellBound). Layers run along `axis`; each layer is a 2D grid over the two other axes (b, c in
+// (farBoxBound). Layers run along `axis`; each layer is a 2D grid over the two other axes (b, c in
 // increasing order) reduced 2 x 2 per level, up to one block at the top level. Level L of layer l
 // holds block (ib >> L, ic >> L) at p[off[L] + (l * nc[L] + (ic >> L)) * nb[L] + (ib >> L)]; FP32
 // sigt as the walks use it.

@@ -760,7 +760,7 @@ __global__ void __launch_bounds__(256) k_nee_corr(MediumView m, ScatterView sc, 
 
 // The far-field view kernel's record of a vertex at one frame (FarRec): the walk's start (cell and
 // face offsets, or kFarOutside), the material, the FP32 back direction, the FP64 position and
-// phase base; its live-block masks start all set (k_far_bound narrows them).
+// phase base; its live-block masks start all set (k_far_live narrows them).
 template <int DIMS>
 __device__ inline void writeFarRec(const MediumView& m, const VertexBatch& vb, uint64_t idx, const Geo& xk,
                                    uint32_t mat, const d3& back, double phase) {
